@@ -1,0 +1,52 @@
+/* Host-side C ABI of the augmented-space chain engine (libasr_host.so): the node-set builder and the routines that stay on
+ * the host around the GPU chains (terminated continued fractions, tails, KPM DOS, contours).
+ *
+ * Conventions: 0-based indices, -1 = no link, row-major arrays, complex numbers as interleaved (re, im) doubles.  Every
+ * function returns 0 on success; a non-zero status is an error (the Fortran side turns it into `error stop`).
+ */
+#ifndef ASR_HOST_H
+#define ASR_HOST_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct asr_augcls_tables asr_augcls_tables;
+
+/* Node set and node tables of the site-class, point-group reduced augmented-space operator.
+ * Replaces build_augcls_operator (src/asr_augcls.f90:596-823) for the table part: the caller keeps the bond blocks,
+ * species parameters and orbital tables.  perm: nop x nsite (operation 0 = identity), flippable: nsite bytes or NULL,
+ * nsc: states per class, maxrank <= 0: unlimited.  err receives a message on failure. */
+int asr_host_build_augcls(int nsite, const int* nbr_ptr, const int* nbr, const int* cls, int nop, const int* perm, int start,
+                          int maxlevel, int maxrank, const unsigned char* flippable, const int* nsc, int ncls, int nthreads,
+                          asr_augcls_tables** out, char* err, int errlen);
+/* dims = {nn, nslot, nop, nsmax, nstab} */
+int asr_host_tables_dims(const asr_augcls_tables* t, int* dims);
+/* hopr/hopg: nn x nslot, flipr/flipg: nn x nsmax, lst/site/level/orb: nn, stab_ptr: nn + 1, stab: nstab */
+int asr_host_tables_get(const asr_augcls_tables* t, int* hopr, int* hopg, int* flipr, int* flipg, int* lst, int* site,
+                        int* level, int* orb, int* stab_ptr, int* stab);
+void asr_host_tables_free(asr_augcls_tables* t);
+
+/* Terminated continued fraction of a complex-symmetric chain: cf_species (src/asr_chain.f90:67-167).
+ * tail: 0 hull, 1 bp, 2 avg2, 3 none.  Returns the number of bp fallbacks. */
+int asr_host_cf_species(const double* a, const double* b, const double* c, int m, const double* z, int tail, double* g11,
+                        double* f, double* edges, double* pv);
+/* Beer-Pettifor asymptote: bp_asymptote (src/reference/asr_terminator.f90:39-67).  1 = not converged. */
+int asr_host_bp_asymptote(const double* a, const double* b, int n, double* ainf, double* binf, int* niter);
+/* G(z) of a real chain with a square-root tail: terminated_green (src/reference/asr_terminator.f90:117-126). */
+int asr_host_terminated_green(const double* a, const double* b, int n, double ainf, double binf, int ne, const double* z,
+                              double* g);
+/* Jackson-kernel DOS from Chebyshev moments: kpm_dos (src/reference/asr_kpm.f90:68-87). */
+int asr_host_kpm_dos(const double* mu, int nmom, double c, double s, int ne, const double* e, double* rho);
+/* Eigenvalues of a real symmetric tridiagonal matrix (ascending, in place in d): the role of dsterf. */
+int asr_host_tridiag_eigvals(int n, double* d, const double* e);
+/* Eigenvalues of the complex-symmetric tridiagonal chain matrix: the role of zgeev in cf_species. */
+int asr_host_chain_eigvals(int n, const double* a, const double* b, double* ev);
+/* Gauss-Legendre rule (src/asr_contour.f90:132-151) and the closed occupation contour (src/asr_contour.f90:39-58). */
+int asr_host_gauss_legendre(int n, double* x, double* w);
+int asr_host_contour_closed(double ebot, double ef, double eta, int n_arc, int n_vert, double* z, double* w);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
