@@ -1,0 +1,309 @@
+"""ORACLE — test infrastructure only (tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg).
+
+ctypes bindings of liborc.so (the C restatement in asr_oracle.c) plus small numpy restatements of the reference routines
+that call LAPACK on the chain matrices:
+    np_cf_species       src/asr_chain.f90:67-167          (zgeev -> numpy.linalg.eigvals, the same LAPACK routine)
+    np_bp_asymptote     src/reference/asr_terminator.f90:39-79  (dsterf -> scipy.linalg.eigvalsh_tridiagonal)
+    np_cf_green         src/reference/asr_greens.f90:207-217
+    np_sqrt_tail        src/reference/asr_greens.f90:139-149
+    np_kpm_dos          src/reference/asr_kpm.f90:68-87
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build() -> str:
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+    return os.path.join(_HERE, "liborc.so")
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liborc.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        P = C.c_void_p
+        L.orc_apply.argtypes = [P, P, P, P]
+        L.orc_rand_c.argtypes = [P, P, P, P, P]
+        L.orc_project_c.argtypes = [P, P, P]
+        L.orc_lanczos.argtypes = [P, P, C.c_int, C.c_int, C.c_double, P, P]
+        L.orc_edc_chain.argtypes = [P, P, P, C.c_int, C.c_int, C.c_int, P, P]
+        L.orc_lanczos_batch.argtypes = [P, P, P, P, C.c_int, C.c_int, P, P, P]
+        L.orc_edc_batch.argtypes = [P, P, P, P, P, C.c_int, C.c_int, C.c_int, P, P, P]
+        L.orc_real_apply.argtypes = [P, P, P, P]
+        L.orc_kpm_moments.argtypes = [P, C.c_double, C.c_double, C.c_int, C.c_int, C.c_uint64, P, P]
+        L.orc_real_lanczos.argtypes = [P, C.c_int, C.c_uint64, P, P]
+        L.orc_rademacher.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
+        L.orc_rademacher.restype = C.c_double
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class _Op(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("nn", "nslot", "nop", "nsmax", "ncls", "nsp", "ntype")] + \
+               [(n, C.c_void_p) for n in ("hopr", "hopg", "flipr", "flipg", "lst", "ncl", "ntyp", "stab_ptr", "stab", "pi",
+                                          "nsc", "spc", "fz", "sq", "sg", "block", "block0", "mt")]
+
+
+class _Member(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("chi", "c", "d", "o", "e", "mt")] + [("frozen", C.c_int)]
+
+
+class _Real(C.Structure):
+    _fields_ = [("nsite", C.c_int), ("nslot", C.c_int), ("nsp", C.c_int)] + \
+               [(n, C.c_void_p) for n in ("nbr", "species", "c", "d", "o", "e", "block", "block0")]
+
+
+class OracleOp:
+    """The operator tables of an asr_amd.augcls.AugclsOperator, pinned for the C oracle."""
+
+    def __init__(self, op):
+        self.keep = []
+        def arr(a, dt):
+            a = np.ascontiguousarray(a, dtype=dt)
+            self.keep.append(a)
+            return _p(a)
+        self.op = op
+        self.s = _Op(op.nn, op.nslot, op.nop, op.nsmax, op.ncls, op.nsp, op.ntype,
+                     arr(op.hopr, np.int32), arr(op.hopg, np.int32), arr(op.flipr, np.int32), arr(op.flipg, np.int32),
+                     arr(op.lst, np.int32), arr(op.ncl, np.int32), arr(op.ntyp, np.int32), arr(op.stab_ptr, np.int32),
+                     arr(op.stab if op.stab.size else np.zeros(1, np.int32), np.int32), arr(op.pi, np.int32),
+                     arr(op.nsc, np.int32), arr(op.spc, np.int32), arr(op.fz, np.uint8), arr(op.sq, np.float64),
+                     arr(op.sg, np.float64), arr(op.block, np.float64), arr(op.block0, np.float64), arr(op.mt, np.float64))
+
+    def member(self, m):
+        keep = []
+        def arr(a):
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            keep.append(a)
+            return _p(a)
+        s = _Member(arr(self.op.chi_for(m.l0)), arr(m.c), arr(m.d), arr(m.o), arr(m.e),
+                    arr(m.mt(self.op)) if m.conc is not None else None, int(m.frozen))
+        s._keep = keep
+        return s
+
+    def apply(self, m, x: np.ndarray) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        ms = self.member(m)
+        lib().orc_apply(C.byref(self.s), C.byref(ms), _p(x), _p(y))
+        return y
+
+    def rand_c(self, m, par: np.ndarray, x: np.ndarray) -> np.ndarray:
+        par = np.ascontiguousarray(par, dtype=np.complex128)
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        w = np.zeros_like(x)
+        ms = self.member(m)
+        lib().orc_rand_c(C.byref(self.s), C.byref(ms), _p(par), _p(x), _p(w))
+        return w
+
+    def project_c(self, m, x: np.ndarray) -> np.ndarray:
+        x = np.array(x, dtype=np.complex128)
+        ms = self.member(m)
+        lib().orc_project_c(C.byref(self.s), C.byref(ms), _p(x))
+        return x
+
+    def lanczos(self, m, iv: int, nmax: int, tiny: float = 1.0e-12):
+        a = np.zeros(nmax)
+        b = np.zeros(nmax)
+        ms = self.member(m)
+        n = lib().orc_lanczos(C.byref(self.s), C.byref(ms), int(iv), int(nmax), float(tiny), _p(a), _p(b))
+        return a, b, n
+
+    def edc_chain(self, m, ep: np.ndarray, iv: int, nlev: int, project: bool = True):
+        ep = np.ascontiguousarray(ep, dtype=np.complex128)
+        a = np.zeros(nlev, np.complex128)
+        b = np.zeros(nlev, np.complex128)
+        ms = self.member(m)
+        mm = lib().orc_edc_chain(C.byref(self.s), C.byref(ms), _p(ep), int(iv), int(nlev), int(project), _p(a), _p(b))
+        return a, b, mm
+
+    def lanczos_batch(self, members, iv, nlev):
+        nch = len(members)
+        ld = int(max(nlev))
+        a = np.zeros((nch, ld))
+        b = np.zeros((nch, ld))
+        mu = np.zeros(nch, np.int32)
+        mss = [self.member(m) for m in members]
+        arr = (_Member * nch)(*mss)
+        ivs = np.ascontiguousarray(iv, dtype=np.int32)
+        nl = np.ascontiguousarray(nlev, dtype=np.int32)
+        lib().orc_lanczos_batch(C.byref(self.s), arr, _p(ivs), _p(nl), nch, ld, _p(a), _p(b), _p(mu))
+        return a, b, mu
+
+    def edc_batch(self, members, eps, iv, nlev, project: bool = True):
+        nch = len(members)
+        ld = int(max(nlev))
+        a = np.zeros((nch, ld), np.complex128)
+        b = np.zeros((nch, ld), np.complex128)
+        mu = np.zeros(nch, np.int32)
+        mss = [self.member(m) for m in members]
+        arr = (_Member * nch)(*mss)
+        ep = np.ascontiguousarray(eps, dtype=np.complex128)
+        ivs = np.ascontiguousarray(iv, dtype=np.int32)
+        nl = np.ascontiguousarray(nlev, dtype=np.int32)
+        lib().orc_edc_batch(C.byref(self.s), arr, _p(ep), _p(ivs), _p(nl), nch, ld, int(project), _p(a), _p(b), _p(mu))
+        return a, b, mu
+
+
+class OracleReal:
+    """Supercell realisation operator (lmto_real_op with per-site species) for the oracle."""
+
+    def __init__(self, nbr, species, c, d, o, e, block, block0):
+        self.keep = [np.ascontiguousarray(nbr, dtype=np.int32), np.ascontiguousarray(species, dtype=np.uint8)]
+        f = [np.ascontiguousarray(v, dtype=np.float64) for v in (c, d, o, e, block, block0)]
+        self.keep += f
+        self.dim = 9 * self.keep[0].shape[0]
+        self.s = _Real(self.keep[0].shape[0], self.keep[0].shape[1], f[0].shape[0], *[_p(v) for v in self.keep])
+
+    def apply(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        wk = np.zeros(3 * x.size)
+        lib().orc_real_apply(C.byref(self.s), _p(x), _p(y), _p(wk))
+        return y
+
+    def kpm_moments(self, c: float, s: float, nmom: int, nvec: int, seed: int, per_vector: bool = False):
+        mu = np.zeros(nmom)
+        per = np.zeros((nvec, nmom)) if per_vector else None
+        lib().orc_kpm_moments(C.byref(self.s), float(c), float(s), int(nmom), int(nvec), int(seed), _p(mu),
+                              None if per is None else _p(per))
+        return (mu, per) if per_vector else mu
+
+    def lanczos(self, nstep: int, seed: int):
+        a = np.zeros(nstep)
+        b = np.zeros(nstep)
+        lib().orc_real_lanczos(C.byref(self.s), int(nstep), int(seed), _p(a), _p(b))
+        return a, b
+
+
+def rademacher(seed: int, v: int, row: int) -> float:
+    return lib().orc_rademacher(int(seed), int(v), int(row))
+
+
+# ---------------------------------------------------------------- numpy restatements of the host-side routines
+def np_sqrt_tail(z, ainf, binf):
+    w = z - ainf
+    s = np.sqrt(w * w - 4.0 * binf * binf + 0j)
+    g1 = (w - s) / (2.0 * binf * binf)
+    g2 = (w + s) / (2.0 * binf * binf)
+    g = np.where(np.abs(g1) <= np.abs(g2), g1, g2)
+    return binf * binf * g
+
+
+def np_cf_green(a, b, n, z, tail):
+    g = 1.0 / (z - a[n - 1] - tail)
+    for i in range(n - 2, -1, -1):
+        g = 1.0 / (z - a[i] - b[i] * b[i] * g)
+    return g
+
+
+def np_bp_asymptote(a, b, n):
+    from scipy.linalg import eigvalsh_tridiagonal
+    A = 0.5 * (a[n - 1] + a[n - 2])
+    B = np.sqrt(0.5 * (b[n - 1] ** 2 + b[n - 2] ** 2))
+    for _ in range(500):
+        d = np.array(a[:n], dtype=float)
+        d[n - 1] = a[n - 1] + B
+        ep = eigvalsh_tridiagonal(d, np.array(b[:n - 1], dtype=float), eigvals_only=True)[-1]
+        d[n - 1] = a[n - 1] - B
+        em = eigvalsh_tridiagonal(d, np.array(b[:n - 1], dtype=float), eigvals_only=True)[0]
+        an, bn = 0.5 * (ep + em), 0.25 * (ep - em)
+        if abs(an - A) + abs(bn - B) < 1.0e-13:
+            return an, bn
+        A, B = an, bn
+    return A, B
+
+
+def _chain_matrix(a, b, m, am=None):
+    t = np.zeros((m, m), dtype=np.complex128)
+    for j in range(m):
+        t[j, j] = a[j]
+        if j < m - 1:
+            t[j, j + 1] = b[j]
+            t[j + 1, j] = b[j]
+    if am is not None:
+        t[m - 1, m - 1] = am
+    return t
+
+
+def np_cf_species(a, b, c, m, z, tail="hull"):
+    a = np.asarray(a, dtype=np.complex128)
+    b = np.asarray(b, dtype=np.complex128)
+    c = np.asarray(c, dtype=np.complex128)
+    info = 1
+    if m >= 2 and tail != "none":
+        ev = np.linalg.eigvals(_chain_matrix(a, b, m))
+        info = 0
+    if info == 0:
+        ilo = int(np.argmin(ev.real))
+        ihi = int(np.argmax(ev.real))
+        aa = 0.5 * (ev[ilo] + ev[ihi])
+        bb = 0.25 * (ev[ihi] - ev[ilo])
+        if tail == "bp":
+            for it in range(200):
+                ep = np.linalg.eigvals(_chain_matrix(a, b, m, a[m - 1] + bb))
+                ep = ep[int(np.argmax(ep.real))]
+                em = np.linalg.eigvals(_chain_matrix(a, b, m, a[m - 1] - bb))
+                em = em[int(np.argmin(em.real))]
+                an, bn = 0.5 * (ep + em), 0.25 * (ep - em)
+                if abs(an - aa) + abs(bn - bb) < 1.0e-12 * (1.0 + abs(bn)):
+                    aa, bb = an, bn
+                    break
+                aa, bb = an, bn
+    else:
+        aa = a[m - 1]
+        bb = b[m - 2] if m > 1 else 0.0
+    if tail == "avg2" and m >= 3:
+        aa = 0.5 * (a[m - 1] + a[m - 2])
+        bb = np.sqrt(0.5 * (b[m - 2] ** 2 + b[m - 3] ** 2))
+    if tail == "none":
+        aa, bb = 0.0, 0.0
+    d = z - aa
+    if abs(bb) < 1.0e-14:
+        tl = 0.0
+    else:
+        s = np.sqrt(d * d - 4.0 * bb * bb)
+        g1 = (d - s) / (2.0 * bb * bb)
+        g2 = (d + s) / (2.0 * bb * bb)
+        tl = bb * bb * g1 if abs(bb * g1) <= abs(bb * g2) else bb * bb * g2
+    gt = np.zeros(m + 1, dtype=np.complex128)
+    gt[m - 1] = 1.0 / (z - a[m - 1] - tl)
+    for j in range(m - 2, -1, -1):
+        gt[j] = 1.0 / (z - a[j] - b[j] ** 2 * gt[j + 1])
+    f = 0.0
+    prod = gt[0]
+    for k in range(m):
+        f += c[k] * prod
+        if k < m - 1:
+            prod = prod * b[k] * gt[k + 1]
+    return gt[0], f
+
+
+def np_kpm_dos(mu, c, s, e):
+    nmom = len(mu)
+    n = np.arange(nmom)
+    g = ((nmom - n + 1) * np.cos(np.pi * n / (nmom + 1)) + np.sin(np.pi * n / (nmom + 1)) / np.tan(np.pi / (nmom + 1))) \
+        / (nmom + 1)
+    e = np.asarray(e, dtype=float)
+    x = (e - c) / s
+    rho = np.zeros_like(e)
+    ok = np.abs(x) < 1.0
+    th = np.arccos(x[ok])
+    acc = g[0] * mu[0] + 2.0 * np.sum((g[1:] * np.asarray(mu[1:]))[None, :] * np.cos(np.outer(th, n[1:])), axis=1)
+    rho[ok] = acc / (np.pi * np.sqrt(1.0 - x[ok] ** 2) * s)
+    return rho

@@ -1,0 +1,114 @@
+"""Independent truths for the oracle: exact configuration averages on tiny rings, dense matrices of the unreduced operator.
+
+The ring cases follow the brute-force checks of the reference (test/test_augn.f90:41-126, test_slow/test_augcls.f90:82-193):
+every configuration of the random sites is enumerated, the resolvent of its dense Hamiltonian is solved, and the
+configuration average is compared with the chain of the augmented-space operator (exhausted Krylov space).
+"""
+from __future__ import annotations
+
+import itertools
+
+import numpy as np
+
+from asr_amd import augcls as A
+from asr_amd import structure as S
+
+
+def ring_graph(n: int) -> S.Graph:
+    nbr = np.array([[(i + 1) % n, (i - 1) % n] for i in range(n)], np.int32)
+    return S.Graph(n=n, nbr=nbr, coord=np.zeros((n, 3), np.int64))
+
+
+def ring_model(seed: int = 20261111):
+    """Ring of 4 sites, three classes: 0 ordered {0}, 1 binary {1, 2} (0.4, 0.6), 2 ternary {0, 2, 3} (0.3, 0.5, 0.2),
+    site classes [0, 1, 2, 1]; random symmetric bond blocks (block of slot 1 = transpose of slot 0)."""
+    rng = np.random.default_rng(seed)
+    block = np.zeros((1, 2, 9, 9))
+    block[0, 0] = rng.uniform(-0.25, 0.25, (9, 9))
+    block[0, 1] = block[0, 0].T
+    b0 = rng.uniform(-0.3, 0.3, (9, 9))
+    b0 = np.triu(b0) + np.triu(b0, 1).T
+    model = dict(cls=np.array([0, 1, 2, 1], np.int32), nsc=[1, 2, 3], spc=[[0], [1, 2], [0, 2, 3]],
+                 conc=[[1.0], [0.4, 0.6], [0.3, 0.5, 0.2]], block=block, block0=b0[None], nsp=4)
+    par = dict(c=rng.uniform(-0.4, 0.4, (4, 9)), d=rng.uniform(0.3, 0.7, (4, 9)), o=rng.uniform(-0.2, 0.0, (4, 9)),
+               e=rng.uniform(-0.2, 0.2, (4, 9)), pa=rng.uniform(-0.15, 0.15, (4, 9)), pb=rng.uniform(-0.1, 0.1, (4, 9)))
+    return model, par
+
+
+def build_ring_operator(model, frozen_start: bool, maxlevel: int = 80):
+    n = model["cls"].size
+    g = ring_graph(n)
+    fl = np.ones(n, np.uint8)
+    if frozen_start:
+        fl[0] = 0
+    perm = np.arange(n, dtype=np.int32)[None, :]
+    return A.build_operator(g, model["cls"], model["nsc"], model["spc"], model["conc"], perm, np.eye(9)[None], 0, maxlevel,
+                            flippable=fl, block=model["block"], block0=model["block0"], nsp=model["nsp"])
+
+
+def dense_s(model) -> np.ndarray:
+    n = model["cls"].size
+    g = ring_graph(n)
+    s = np.zeros((9 * n, 9 * n))
+    for i in range(n):
+        s[9 * i:9 * i + 9, 9 * i:9 * i + 9] += model["block0"][0]
+        for k in range(2):
+            j = g.nbr[i, k]
+            s[9 * i:9 * i + 9, 9 * j:9 * j + 9] += model["block"][0, k]
+    return s
+
+
+def configurations(model, frozen_species: int | None):
+    """(species per site, probability) of every configuration; site 0 fixed to frozen_species if given."""
+    cls = model["cls"]
+    choices = []
+    for i, kap in enumerate(cls):
+        if i == 0 and frozen_species is not None:
+            choices.append([(frozen_species, 1.0)])
+        else:
+            choices.append(list(zip(model["spc"][kap], model["conc"][kap])))
+    for combo in itertools.product(*choices):
+        sp = np.array([c[0] for c in combo])
+        p = float(np.prod([c[1] for c in combo]))
+        yield sp, p
+
+
+def dense_h(model, par, species: np.ndarray, with_e: bool = True) -> np.ndarray:
+    """H = E_nu + h - h o h, h = c + d S d, for one configuration (species per site)."""
+    s = dense_s(model)
+    c = par["c"][species].reshape(-1)
+    d = par["d"][species].reshape(-1)
+    o = par["o"][species].reshape(-1)
+    e = par["e"][species].reshape(-1)
+    h = d[:, None] * s * d[None, :] + np.diag(c)
+    hm = h - h @ np.diag(o) @ h
+    if with_e:
+        hm = hm + np.diag(e)
+    return hm
+
+
+def exact_average_g(model, par, z: complex, iv: int, frozen_species: int | None, ep: bool = False) -> complex:
+    """<<[(z - K)^-1]_{iv, iv}>> over the configurations; K = H (with E_nu) or, ep=True, K = H(e = 0) + diag(z pa + pb)."""
+    tot = 0.0
+    for sp, p in configurations(model, frozen_species):
+        if ep:
+            k = dense_h(model, par, sp, with_e=False)
+            x = z * par["pa"][sp].reshape(-1) + par["pb"][sp].reshape(-1)
+            k = k + np.diag(x)
+        else:
+            k = dense_h(model, par, sp)
+        rhs = np.zeros(k.shape[0], np.complex128)
+        rhs[iv] = 1.0
+        sol = np.linalg.solve(z * np.eye(k.shape[0]) - k, rhs)
+        tot += p * sol[iv]
+    return tot
+
+
+def dense_operator(apply, dim: int) -> np.ndarray:
+    """Dense matrix of a linear map x -> apply(x) (columns = images of the unit vectors)."""
+    m = np.zeros((dim, dim))
+    for i in range(dim):
+        u = np.zeros(dim)
+        u[i] = 1.0
+        m[:, i] = apply(u)
+    return m

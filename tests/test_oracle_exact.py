@@ -1,0 +1,106 @@
+"""Pins the oracle (and the node-set builder that feeds it) to independent truths, on the CPU.
+
+* Ring brute force (test_slow/test_augcls.f90:82-148, test/test_augn.f90:41-100): recursion with an exhausted Krylov space
+  equals the exact configuration average of the resolvent, for three classes (ordered / binary / ternary), the start site
+  frozen to each allowed species, real (energy-independent) and complex-symmetric (energy-dependent) chains.
+* Symmetry of the matrix-free operator (test/test_augn.f90:46-49).
+* Second moment a_1^2 + b_1^2 = <u|H^2|u> (test/test_recursion_dense.f90:101-105).
+"""
+import numpy as np
+import pytest
+
+from asr_amd import augcls as A
+from asr_amd import host
+from oracle.oracle import OracleOp
+
+from exact import build_ring_operator, dense_operator, exact_average_g, ring_model
+
+
+def rotated(model, ic):
+    ib = int(np.nonzero(model["cls"] == ic)[0][0])
+    m = dict(model)
+    m["cls"] = np.roll(model["cls"], -ib)
+    return m
+
+
+CASES = [(ic, cen) for ic, spc in enumerate([[0], [1, 2], [0, 2, 3]]) for cen in spc]
+
+
+@pytest.mark.parametrize("ic,cen", CASES)
+def test_ring_real_chain_equals_exact_average(ic, cen):
+    model, par = ring_model()
+    model = rotated(model, ic)
+    op = build_ring_operator(model, frozen_start=True)
+    n_expect = 4 * int(np.prod([model["nsc"][k] for k in model["cls"][1:]]))
+    assert op.nn == n_expect
+    O = OracleOp(op)
+    l0 = 4
+    m = A.Member(l0=l0, c=par["c"], d=par["d"], o=par["o"], e=par["e"], frozen=cen)
+    a, b, n = O.lanczos(m, l0, op.dim, tiny=1.0e-9)
+    err = 0.0
+    for e in (-0.6, 0.0, 0.6):
+        z = complex(e, 0.3)
+        g = host.terminated_green(a, b, n, 0.0, 0.0, z)[0]
+        gx = exact_average_g(model, par, z, l0, cen)
+        err = max(err, abs(g - gx))
+    assert err < 1.0e-7
+
+
+@pytest.mark.parametrize("ic,cen", CASES[:4])
+def test_ring_edc_operator_and_chain(ic, cen):
+    """The energy-dependent operator K = H + X(z) (augcls_apply on Re/Im + augcls_rand_c) has the exact configuration
+    average as its resolvent (dense solve, 1e-10), and the complex-symmetric chain reproduces that solve (1e-6)."""
+    model, par = ring_model()
+    model = rotated(model, ic)
+    op = build_ring_operator(model, frozen_start=True)
+    O = OracleOp(op)
+    mem = A.Member(l0=0, c=par["c"], d=par["d"], o=par["o"], e=np.zeros((4, 9)), frozen=cen)
+    dim = op.dim
+    err_op = err_ch = 0.0
+    for e in (-0.3, 0.3):
+        z = complex(e, 0.5)
+        ep = z * par["pa"] + par["pb"]
+        k = np.zeros((dim, dim), np.complex128)
+        for i in range(dim):
+            u = np.zeros(dim)
+            u[i] = 1.0
+            k[:, i] = O.apply(mem, u) + O.rand_c(mem, ep, u.astype(np.complex128))
+        rhs = np.zeros(dim, np.complex128)
+        rhs[0] = 1.0
+        gaug = np.linalg.solve(z * np.eye(dim) - k, rhs)[0]
+        gx = exact_average_g(model, par, z, 0, cen, ep=True)
+        err_op = max(err_op, abs(gaug - gx))
+        a, b, mm = O.edc_chain(mem, ep, 0, 1500, project=True)  # no reorthogonalisation: about 3x dim steps here
+        g, _ = host.cf_species(a, b, np.zeros(mm), mm, z, tail="none")
+        err_ch = max(err_ch, abs(g - gaug))
+    assert err_op < 1.0e-10
+    assert err_ch < 1.0e-6
+
+
+def test_operator_symmetric_and_second_moment():
+    model, par = ring_model()
+    op = build_ring_operator(rotated(model, 2), frozen_start=False, maxlevel=80)
+    O = OracleOp(op)
+    m = A.Member(l0=0, c=par["c"], d=par["d"], o=par["o"], e=par["e"])
+    h = dense_operator(lambda x: O.apply(m, x), op.dim)
+    assert np.max(np.abs(h - h.T)) < 1.0e-12 * np.max(np.abs(h))
+    a, b, n = O.lanczos(m, 0, 6)
+    u = np.zeros(op.dim)
+    u[0] = 1.0
+    hu = h @ u
+    assert abs(a[0] ** 2 + b[0] ** 2 - hu @ hu) < 1.0e-12 * (hu @ hu)
+
+
+def test_oracle_mutation_is_detected():
+    """A planted mistake (one sign flipped in one bond block) must break the exact-average agreement."""
+    model, par = ring_model()
+    model = rotated(model, 1)
+    op = build_ring_operator(model, frozen_start=True)
+    op.block = op.block.copy()
+    op.block[0, 0, 2, 5] = -op.block[0, 0, 2, 5]
+    O = OracleOp(op)
+    m = A.Member(l0=4, c=par["c"], d=par["d"], o=par["o"], e=par["e"], frozen=1)
+    a, b, n = O.lanczos(m, 4, op.dim, tiny=1.0e-9)
+    z = complex(0.0, 0.3)
+    g = host.terminated_green(a, b, n, 0.0, 0.0, z)[0]
+    assert abs(g - exact_average_g(model, par, z, 4, 1)) > 1.0e-6
