@@ -1,0 +1,1260 @@
+// Augmented-space recursion and KPM moment engine for MI355X (gfx950): HIP kernels and the C ABI of include/asr_hip.h.
+//
+// Chains that share one node set (one sublattice operator) are stacked as the members of a block vector,
+//     V[(node * 9 + orbital) * nbp + member]        (member fastest, as asr_augblk's x(j, 9(i-1)+l), src/asr_augblk.f90:1-13)
+// so one pass over the node tables, the bond blocks and the gathered amplitudes serves every member.  A wavefront covers
+// npw = 64 / MT nodes x MT members (MT = member tile, a power of two <= 64): neighbour indices and bond blocks are
+// wave-uniform per node, and the gathers of one neighbour are contiguous across the lanes of its members.
+//
+// One chain level of the edc route (src/asr_edc.f90:95-155 / src/asr_augblk.f90:219-250) is
+//   K1  h1 = rand_c(u~) + D S D u~                         (apply_h, src/asr_augcls.f90:449-489, D uniform: diagonal)
+//   K2  r  = rand_o(h1)                                    (apply_rand, :380-403)
+//   K3  w  = sq (rand_e(u~) + h1 - rand_c(r) - D S D r + z rand_pa(u~) + rand_pb(u~)),  partial a = sum u w
+//   fin a;  K4  w -= a u + b up, project(w) (augcls_project_c, :329-348), partial sum w w;  fin b;  K5  u = w / b
+// with u~ = u / sq (normalised -> amplitude coordinates).  A species parameter that differs between species (d in the plain
+// route) takes the general path: t1 = rand_d(u~); t2 = S t1; h = rand_c(u~) + rand_d(t2).  All reductions are two-stage
+// (per-block partials in a fixed tree order, then a fixed-order sum), so results are bitwise reproducible run to run.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "asr_hip.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const std::string& msg) {
+  g_err = msg;
+  return 1;
+}
+
+#define HCHECK(call)                                                                                   \
+  do {                                                                                                 \
+    hipError_t e_ = (call);                                                                            \
+    if (e_ != hipSuccess) return fail(std::string(#call) + ": " + hipGetErrorString(e_));              \
+  } while (0)
+
+constexpr int NQ = 6;  // parameter slots: c, d, o, e, pa, pb
+enum { QC = 0, QD = 1, QO = 2, QE = 3, QPA = 4, QPB = 5 };
+constexpr int WAVE = 64;
+constexpr int BLOCK = 256;
+
+// ------------------------------------------------------------------------------------------------ scalar types
+struct cplx {
+  double re, im;
+};
+__host__ __device__ inline cplx operator+(cplx a, cplx b) { return {a.re + b.re, a.im + b.im}; }
+__host__ __device__ inline cplx operator-(cplx a, cplx b) { return {a.re - b.re, a.im - b.im}; }
+__host__ __device__ inline cplx operator*(double s, cplx a) { return {s * a.re, s * a.im}; }
+__host__ __device__ inline cplx operator*(cplx a, cplx b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__host__ __device__ inline cplx operator/(cplx a, double s) { return {a.re / s, a.im / s}; }
+
+template <class T> __device__ inline T zero();
+template <> __device__ inline double zero<double>() { return 0.0; }
+template <> __device__ inline cplx zero<cplx>() { return {0.0, 0.0}; }
+__device__ inline double fma_s(double s, double x, double acc) { return fma(s, x, acc); }
+__device__ inline cplx fma_s(double s, cplx x, cplx acc) { return {fma(s, x.re, acc.re), fma(s, x.im, acc.im)}; }
+__device__ inline double bilin(double a, double b) { return a * b; }
+__device__ inline cplx bilin(cplx a, cplx b) { return a * b; }
+__device__ inline double shfl_xor(double v, int m) { return __shfl_xor(v, m); }
+__device__ inline cplx shfl_xor(cplx v, int m) { return {__shfl_xor(v.re, m), __shfl_xor(v.im, m)}; }
+
+// ------------------------------------------------------------------------------------------------ device operator
+struct OpDev {
+  int nn, nslot, nop, nsmax, ncls, nsp, ntype, nstab;
+  const uint32_t* hop;   // nn x nslot: (node + 1) << 6 | op, 0 = no link
+  const uint32_t* flip;  // nn x nsmax, same packing
+  const uint32_t* info;  // nn: lst | ncl << 4 | ntyp << 12 | fz << 20
+  const double* sq;      // nn
+  const int* stab_ptr;   // nn + 1
+  const int* stab;
+  const int* pi;         // nop x 9
+  const double* sg;      // nop x 9
+  const double* blk;     // ntype x (nslot + 1) x 81: slot 0 = on-site block
+  const int* nsc;        // ncls
+};
+
+// per-member tables, member fastest (stride nbp)
+struct MemDev {
+  int nb, nbp, mt;            // members, padded stride, member tile
+  const uint64_t* chimask;    // nbp: bit g set <=> chi(g) = -1
+  const double* wts;          // NQ x ncls x nsmax x nsmax x 9 x nbp
+  const double* fpar;         // NQ x 9 x nbp  (parameters of the frozen species)
+  const int* frozen;          // nbp: 1 = the start site is frozen
+  const double* dvec;         // 9 x nbp: d per orbital (uniform-d members)
+  const cplx* z;              // nbp
+};
+
+__device__ inline void unpack_link(uint32_t e, int& r, int& g) {
+  r = (int)(e >> 6) - 1;
+  g = (int)(e & 63u);
+}
+
+template <class T>
+__device__ inline T ld(const T* x, int node, int l, int nbp, int m) {
+  return x[((size_t)node * 9 + l) * nbp + m];
+}
+
+// acc += coef * [dout o] S ([din o] x~)  with x~ = x / sq (xnorm) and the member's characters; B in LDS
+template <class T>
+__device__ void s_accum(const OpDev& op, const double* __restrict__ blk, const int* __restrict__ pi,
+                        const double* __restrict__ sg, uint64_t chim, int i, int t, int m, int nbp, const T* __restrict__ x,
+                        int xnorm, const double* dv, T acc[9]) {
+  T v[9];
+  const double* b0 = blk + (size_t)t * (op.nslot + 1) * 81;
+  {
+    double inv = xnorm ? 1.0 / op.sq[i] : 1.0;
+    for (int l = 0; l < 9; ++l) {
+      T xv = ld(x, i, l, nbp, m);
+      double s = dv ? dv[l] : 1.0;
+      v[l] = (xnorm ? s * (xv / op.sq[i]) : s * xv);
+    }
+    (void)inv;
+    for (int l = 0; l < 9; ++l) {
+      T a = acc[l];
+      for (int l2 = 0; l2 < 9; ++l2) a = fma_s(b0[l * 9 + l2], v[l2], a);
+      acc[l] = a;
+    }
+  }
+  const uint32_t* hp = op.hop + (size_t)i * op.nslot;
+  for (int k = 0; k < op.nslot; ++k) {
+    uint32_t e = hp[k];
+    if (e == 0u) continue;
+    int r, g;
+    unpack_link(e, r, g);
+    double chi = ((chim >> g) & 1ull) ? -1.0 : 1.0;
+    const int* pg = pi + g * 9;
+    const double* sgg = sg + g * 9;
+    double sqr = op.sq[r];
+    for (int l = 0; l < 9; ++l) {
+      int p = pg[l];
+      T xv = ld(x, r, p, nbp, m);
+      double s = chi * sgg[l] * (dv ? dv[p] : 1.0);
+      v[l] = xnorm ? s * (xv / sqr) : s * xv;
+    }
+    const double* bk = b0 + (size_t)(k + 1) * 81;
+    for (int l = 0; l < 9; ++l) {
+      T a = acc[l];
+      for (int l2 = 0; l2 < 9; ++l2) a = fma_s(bk[l * 9 + l2], v[l2], a);
+      acc[l] = a;
+    }
+  }
+}
+
+// acc += X(par q) x~ at node i (apply_rand: own state plus flip partners; frozen nodes carry the frozen species)
+template <class T>
+__device__ void rand_accum(const OpDev& op, const MemDev& md, const int* __restrict__ pi, const double* __restrict__ sg,
+                           uint64_t chim, int i, uint32_t inf, int m, int q, const T* __restrict__ x, int xnorm, double coef,
+                           T acc[9]) {
+  const int nbp = md.nbp, nsm = op.nsmax;
+  const bool fz = (inf >> 20) & 1u;
+  double sqi = op.sq[i];
+  if (fz && md.frozen[m]) {
+    const double* fp = md.fpar + (size_t)q * 9 * nbp;
+    for (int l = 0; l < 9; ++l) {
+      T xv = ld(x, i, l, nbp, m);
+      if (xnorm) xv = xv / sqi;
+      acc[l] = fma_s(coef * fp[(size_t)l * nbp + m], xv, acc[l]);
+    }
+    return;
+  }
+  const int l0 = inf & 15u, kap = (inf >> 4) & 255u;
+  const double* w = md.wts + (((size_t)q * op.ncls + kap) * nsm + l0) * nsm * 9 * nbp;
+  T r_acc[9];
+  for (int l = 0; l < 9; ++l) {
+    T xv = ld(x, i, l, nbp, m);
+    if (xnorm) xv = xv / sqi;
+    r_acc[l] = fma_s(w[((size_t)l0 * 9 + l) * nbp + m], xv, zero<T>());
+  }
+  const int nsc = op.nsc[kap];
+  const uint32_t* fl = op.flip + (size_t)i * nsm;
+  for (int k = 0; k < nsc; ++k) {
+    uint32_t e = fl[k];
+    if (k == l0 || e == 0u) continue;
+    int r, g;
+    unpack_link(e, r, g);
+    double chi = ((chim >> g) & 1ull) ? -1.0 : 1.0;
+    double sqr = op.sq[r];
+    for (int l = 0; l < 9; ++l) {
+      T xv = ld(x, r, pi[g * 9 + l], nbp, m);
+      if (xnorm) xv = xv / sqr;
+      T tf = (chi * sg[g * 9 + l]) * xv;
+      r_acc[l] = fma_s(w[((size_t)k * 9 + l) * nbp + m], tf, r_acc[l]);
+    }
+  }
+  for (int l = 0; l < 9; ++l) acc[l] = fma_s(coef, r_acc[l], acc[l]);
+}
+
+// X(z) x~ = z (X(pa) x~) + X(pb) x~ for complex members (blk_rand_lin / rand_lin_amp, src/asr_augblk.f90:127-165)
+__device__ void lin_accum(const OpDev& op, const MemDev& md, const int* pi, const double* sg, uint64_t chim, int i,
+                          uint32_t inf, int m, const cplx* x, int xnorm, double coef, cplx acc[9]) {
+  cplx ta[9], tb[9];
+  for (int l = 0; l < 9; ++l) { ta[l] = {0.0, 0.0}; tb[l] = {0.0, 0.0}; }
+  rand_accum<cplx>(op, md, pi, sg, chim, i, inf, m, QPA, x, xnorm, 1.0, ta);
+  rand_accum<cplx>(op, md, pi, sg, chim, i, inf, m, QPB, x, xnorm, 1.0, tb);
+  cplx z = md.z[m];
+  for (int l = 0; l < 9; ++l) acc[l] = acc[l] + coef * (z * ta[l] + tb[l]);
+}
+__device__ inline void lin_accum(const OpDev&, const MemDev&, const int*, const double*, uint64_t, int, uint32_t, int,
+                                 const double*, int, double, double*) {}
+
+struct FArgs {
+  const void* xs;  // S part input (null: no S part)
+  int xs_norm, s_din, s_dout;
+  double s_coef;
+  const void* xr[3];  // rand terms
+  int qr[3], r_norm[3];
+  double r_coef[3];
+  int nr;
+  const void* xl;  // complex X(z) term
+  int l_norm;
+  double l_coef;
+  const void* xo[2];  // own-node terms (amplitude vectors)
+  double o_coef[2];
+  int no;
+  void* out;
+  int out_sq;
+  const void* dotu;  // partial dot <dotu, out> per member (bilinear)
+  void* partial;
+};
+
+template <class T>
+__device__ void block_reduce_member(T v, int mt, T* lds, T* partial_out) {
+  // lanes with the same member (lane % mt) are summed over the node index within the wave, then over the waves
+  const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x >> 6;
+  for (int off = mt; off < WAVE; off <<= 1) v = v + shfl_xor(v, off);
+  if (lane < mt) lds[wave * WAVE + lane] = v;
+  __syncthreads();
+  if (threadIdx.x < (unsigned)mt) {
+    T s = lds[threadIdx.x];
+    for (int w = 1; w < BLOCK / WAVE; ++w) s = s + lds[w * WAVE + threadIdx.x];
+    *partial_out = s;
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(BLOCK) k_fused(OpDev op, MemDev md, FArgs fa) {
+  extern __shared__ double lds_raw[];
+  const int nblk = op.ntype * (op.nslot + 1) * 81;
+  double* blk = lds_raw;
+  int* pi = (int*)(blk + nblk);
+  double* sg = (double*)(pi + ((op.nop * 9 + 1) & ~1));
+  T* red = (T*)(sg + op.nop * 9);
+  for (int q = threadIdx.x; q < nblk; q += BLOCK) blk[q] = op.blk[q];
+  for (int q = threadIdx.x; q < op.nop * 9; q += BLOCK) { pi[q] = op.pi[q]; sg[q] = op.sg[q]; }
+  __syncthreads();
+
+  const int mt = md.mt, npw = WAVE / mt;
+  const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x >> 6;
+  const int m = blockIdx.y * mt + (lane & (mt - 1));
+  const int i = (blockIdx.x * (BLOCK / WAVE) + wave) * npw + lane / mt;
+  const bool live = (i < op.nn) && (m < md.nb);
+  T dotv = zero<T>();
+  if (live) {
+    const uint32_t inf = op.info[i];
+    const uint64_t chim = md.chimask[m];
+    T acc[9];
+    for (int l = 0; l < 9; ++l) acc[l] = zero<T>();
+    if (fa.xs) {
+      double dv[9];
+      for (int l = 0; l < 9; ++l) dv[l] = md.dvec[(size_t)l * md.nbp + m];
+      T sacc[9];
+      for (int l = 0; l < 9; ++l) sacc[l] = zero<T>();
+      s_accum<T>(op, blk, pi, sg, chim, i, (inf >> 12) & 255u, m, md.nbp, (const T*)fa.xs, fa.xs_norm, fa.s_din ? dv : nullptr,
+                 sacc);
+      for (int l = 0; l < 9; ++l) acc[l] = fma_s(fa.s_coef * (fa.s_dout ? dv[l] : 1.0), sacc[l], acc[l]);
+    }
+    for (int t = 0; t < fa.nr; ++t)
+      rand_accum<T>(op, md, pi, sg, chim, i, inf, m, fa.qr[t], (const T*)fa.xr[t], fa.r_norm[t], fa.r_coef[t], acc);
+    if (fa.xl) lin_accum(op, md, pi, sg, chim, i, inf, m, (const T*)fa.xl, fa.l_norm, fa.l_coef, acc);
+    for (int t = 0; t < fa.no; ++t) {
+      const T* xo = (const T*)fa.xo[t];
+      for (int l = 0; l < 9; ++l) acc[l] = fma_s(fa.o_coef[t], ld(xo, i, l, md.nbp, m), acc[l]);
+    }
+    const double so = fa.out_sq ? op.sq[i] : 1.0;
+    T* out = (T*)fa.out;
+    const T* du = (const T*)fa.dotu;
+    for (int l = 0; l < 9; ++l) {
+      T y = so * acc[l];
+      out[((size_t)i * 9 + l) * md.nbp + m] = y;
+      if (du) dotv = dotv + bilin(ld(du, i, l, md.nbp, m), y);
+    }
+  }
+  if (fa.dotu) {
+    T* part = (T*)fa.partial;
+    block_reduce_member<T>(dotv, mt, red, part + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * mt + threadIdx.x);
+  }
+}
+
+// chain state of a batch
+struct ChainDev {
+  int* level;        // current level (device scalar, so a captured level step can be replayed)
+  int* depth;        // nbp
+  int* active;       // nbp
+  int* mused;        // nbp
+  int ld;            // leading dimension of a, b
+  int real_chain;    // 1: real Lanczos conventions (b of the last level recorded)
+  double tiny;
+};
+
+template <class T> __device__ inline double mag(T v);
+template <> __device__ inline double mag<double>(double v) { return fabs(v); }
+template <> __device__ inline double mag<cplx>(cplx v) { return hypot(v.re, v.im); }
+__device__ inline double csqrt_re(double v) { return sqrt(v); }
+__device__ inline cplx csqrt_c(cplx v) {
+  // principal square root
+  double r = hypot(v.re, v.im);
+  double a = sqrt(0.5 * (r + fabs(v.re)));
+  if (a == 0.0) return {0.0, 0.0};
+  if (v.re >= 0.0) return {a, v.im / (2.0 * a)};
+  return {fabs(v.im) / (2.0 * a), (v.im >= 0.0 ? a : -a)};
+}
+__device__ inline double dsqrt(double v) { return csqrt_re(v); }
+__device__ inline cplx dsqrt(cplx v) { return csqrt_c(v); }
+__device__ inline double inv(double v) { return 1.0 / v; }
+__device__ inline cplx inv(cplx v) {
+  double d = v.re * v.re + v.im * v.im;
+  return {v.re / d, -v.im / d};
+}
+
+// fixed-order sum of the per-block partials of every member
+template <class T>
+__device__ T sum_partials(const T* part, int nbx, int mt, int m) {
+  const int tile = m / mt, ml = m % mt;
+  T s = zero<T>();
+  for (int b = 0; b < nbx; ++b) s = s + part[((size_t)tile * nbx + b) * mt + ml];
+  return s;
+}
+
+// a_k of every member; coefficient used by the update (0 for members that left)
+template <class T>
+__global__ void k_fin_a(ChainDev cd, int nb, const T* part, int nbx, int mt, T* a, T* a_eff) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nb) return;
+  int k = *cd.level;
+  T s = sum_partials(part, nbx, mt, m);
+  if (cd.active[m]) {
+    a[(size_t)m * cd.ld + k] = s;
+    cd.mused[m] = k + 1;
+    a_eff[m] = s;
+  } else {
+    a_eff[m] = zero<T>();
+  }
+}
+
+// b_k = sqrt(sum w w); members leave at their depth or when b_k < tiny; next u = w * inv_b
+template <class T>
+__global__ void k_fin_b(ChainDev cd, int nb, const T* part, int nbx, int mt, T* b, T* b_eff, T* inv_b) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nb) return;
+  int k = *cd.level;
+  T s = sum_partials(part, nbx, mt, m);
+  T bk = dsqrt(s);
+  int act = cd.active[m];
+  if (act) {
+    bool last = (k == cd.depth[m] - 1);
+    if (!last || cd.real_chain) b[(size_t)m * cd.ld + k] = bk;
+    if (last || mag(bk) < cd.tiny) act = 0;
+  }
+  cd.active[m] = act;
+  b_eff[m] = act ? bk : zero<T>();
+  inv_b[m] = act ? inv(bk) : zero<T>();
+}
+
+__global__ void k_level_inc(int* level) { *level += 1; }
+
+// w = w - a u - b_prev up; optional projection onto the symmetric subspace; partial sum w w
+template <class T>
+__global__ void __launch_bounds__(BLOCK) k_update(OpDev op, MemDev md, T* w, const T* u, const T* up, const T* a_eff,
+                                                   const T* b_prev, int project, T* partial) {
+  __shared__ T red[BLOCK];
+  const int mt = md.mt, npw = WAVE / mt;
+  const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x >> 6;
+  const int m = blockIdx.y * mt + (lane & (mt - 1));
+  const int i = (blockIdx.x * (BLOCK / WAVE) + wave) * npw + lane / mt;
+  T dv = zero<T>();
+  if (i < op.nn && m < md.nb) {
+    const T a = a_eff[m], b = b_prev[m];
+    T x[9];
+    for (int l = 0; l < 9; ++l) {
+      size_t q = ((size_t)i * 9 + l) * md.nbp + m;
+      x[l] = w[q] - bilin(a, u[q]) - bilin(b, up[q]);
+    }
+    if (project) {
+      int s0 = op.stab_ptr[i], s1 = op.stab_ptr[i + 1], ns = s1 - s0;
+      if (ns > 1) {
+        const uint64_t chim = md.chimask[m];
+        T acc[9];
+        for (int l = 0; l < 9; ++l) acc[l] = zero<T>();
+        for (int j = s0; j < s1; ++j) {
+          int g = op.stab[j];
+          double chi = ((chim >> g) & 1ull) ? -1.0 : 1.0;
+          for (int l = 0; l < 9; ++l) acc[l] = acc[l] + (chi * op.sg[g * 9 + l]) * x[op.pi[g * 9 + l]];
+        }
+        for (int l = 0; l < 9; ++l) x[l] = acc[l] / (double)ns;
+      }
+    }
+    for (int l = 0; l < 9; ++l) {
+      w[((size_t)i * 9 + l) * md.nbp + m] = x[l];
+      dv = dv + bilin(x[l], x[l]);
+    }
+  }
+  block_reduce_member<T>(dv, mt, red, partial + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * mt + threadIdx.x);
+}
+
+// u <- w * inv_b (in place), members fastest
+template <class T>
+__global__ void k_scale(T* w, const T* inv_b, size_t n, int nbp) {
+  size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  int m = (int)(q % nbp);
+  w[q] = bilin(inv_b[m], w[q]);
+}
+
+// ------------------------------------------------------------------------------------------------ KPM (supercell)
+struct RealDev {
+  int nsite, nslot, nsp;
+  const int* nbr;               // nsite x nslot
+  const unsigned char* spec;    // nsite
+  const double* par;            // 4 x nsp x 9: c, d, o, e
+  const double* blk;            // (nslot + 1) x 81, slot 0 = on-site
+};
+
+__device__ inline uint64_t splitmix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__device__ inline double rademacher(uint64_t seed, uint64_t v, uint64_t row) {
+  return (splitmix64((seed << 40) ^ (v << 28) ^ row) >> 63) ? -1.0 : 1.0;
+}
+
+// Value of the input vector: x stored, or the Rademacher vector itself (x == null).
+__device__ inline double kpm_in(const double* x, int nvp, uint64_t seed, int site, int l, int v) {
+  return x ? x[((size_t)site * 9 + l) * nvp + v] : rademacher(seed, (uint64_t)v, (uint64_t)site * 9 + l);
+}
+
+// One Chebyshev step for all vectors: y = alpha (h x - c x) / s - beta t0  (h = c + d S d, order 2 with o = 0),
+// partial mu: sum_l r_l y_l.  Lanes = vectors (one wave per site), sites wave-uniform: neighbour index and the bond block
+// are scalar; the 19 blocks sit in LDS.  x == null / t0 == null: the Rademacher vector (generated, never stored).
+__global__ void __launch_bounds__(BLOCK) k_kpm_step(RealDev R, int nvec, int nvp, const double* __restrict__ x,
+                                                    const double* __restrict__ t0, double* __restrict__ y, double cshift,
+                                                    double alpha_over_s, double beta, uint64_t seed, double* partial,
+                                                    int want_dot) {
+  __shared__ double blk[19 * 81 + 64];
+  __shared__ double red[BLOCK];
+  const int nb = (R.nslot + 1) * 81;
+  for (int q = threadIdx.x; q < nb; q += BLOCK) blk[q] = R.blk[q];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int v = blockIdx.y * 64 + lane;
+  const int site = blockIdx.x * (BLOCK / 64) + wave;
+  double dot = 0.0;
+  if (site < R.nsite && v < nvec) {
+    const int sp = R.spec[site];
+    const double* cpar = R.par + (size_t)sp * 9;
+    const double* dpar = R.par + (size_t)(R.nsp + sp) * 9;
+    double acc[9], xv[9];
+    for (int l = 0; l < 9; ++l) {
+      xv[l] = kpm_in(x, nvp, seed, site, l, v);
+      acc[l] = 0.0;
+    }
+    {
+      double t[9];
+      for (int l = 0; l < 9; ++l) t[l] = dpar[l] * xv[l];
+      for (int l = 0; l < 9; ++l) {
+        double a = acc[l];
+        for (int l2 = 0; l2 < 9; ++l2) a = fma(blk[l * 9 + l2], t[l2], a);
+        acc[l] = a;
+      }
+    }
+    const int* nb_ = R.nbr + (size_t)site * R.nslot;
+    for (int k = 0; k < R.nslot; ++k) {
+      const int j = nb_[k];
+      const double* dj = R.par + (size_t)(R.nsp + R.spec[j]) * 9;
+      double t[9];
+      for (int l = 0; l < 9; ++l) t[l] = dj[l] * kpm_in(x, nvp, seed, j, l, v);
+      const double* bk = blk + (k + 1) * 81;
+      for (int l = 0; l < 9; ++l) {
+        double a = acc[l];
+        for (int l2 = 0; l2 < 9; ++l2) a = fma(bk[l * 9 + l2], t[l2], a);
+        acc[l] = a;
+      }
+    }
+    for (int l = 0; l < 9; ++l) {
+      double hx = cpar[l] * xv[l] + dpar[l] * acc[l];
+      double yy = alpha_over_s * (hx - cshift * xv[l]);
+      if (beta != 0.0) yy -= beta * kpm_in(t0, nvp, seed, site, l, v);
+      y[((size_t)site * 9 + l) * nvp + v] = yy;
+      if (want_dot) dot = fma(rademacher(seed, (uint64_t)v, (uint64_t)site * 9 + l), yy, dot);
+    }
+  }
+  if (want_dot) {
+    // per (block, vector): sum over the 4 sites of the block in a fixed order
+    red[threadIdx.x] = dot;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+      double s = red[threadIdx.x];
+      for (int w = 1; w < BLOCK / 64; ++w) s += red[w * 64 + threadIdx.x];
+      partial[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * 64 + threadIdx.x] = s;
+    }
+  }
+}
+
+// mu[n] += (1/dim) sum_blocks partial (fixed order); one thread per vector, then per-vector values kept for the caller
+__global__ void k_kpm_fin(const double* partial, int nbx, int nvec, double inv_dim, double* mu_per, int nmom, int n) {
+  int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nvec) return;
+  int tile = v / 64, vl = v % 64;
+  double s = 0.0;
+  for (int b = 0; b < nbx; ++b) s += partial[((size_t)tile * nbx + b) * 64 + vl];
+  mu_per[(size_t)v * nmom + n] = s * inv_dim;
+}
+
+}  // namespace
+
+// ================================================================================================ host side
+struct asr_gpu_op {
+  int nn, nslot, nop, nsmax, ncls, nsp, ntype, nstab;
+  std::vector<int> pi_h, nsc_h, spc_h;
+  std::vector<double> sg_h, mt_h;  // mt: ncls x nsmax x nsmax x nsmax
+  uint32_t *hop = nullptr, *flip = nullptr, *info = nullptr;
+  double *sq = nullptr, *sg = nullptr, *blk = nullptr;
+  int *stab_ptr = nullptr, *stab = nullptr, *pi = nullptr, *nsc = nullptr;
+  hipStream_t stream = nullptr;
+  bool have_tables = false, have_ops = false, have_blocks = false, have_classes = false;
+};
+
+struct asr_gpu_real {
+  int nsite, nslot, nsp;
+  int* nbr = nullptr;
+  unsigned char* spec = nullptr;
+  double *par = nullptr, *blk = nullptr;
+  hipStream_t stream = nullptr;
+};
+
+namespace {
+
+template <class P>
+int dalloc(P** p, size_t n) {
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc((void**)p, n * sizeof(P));
+  if (e != hipSuccess) return fail(std::string("hipMalloc: ") + hipGetErrorString(e));
+  return 0;
+}
+
+OpDev dev_of(const asr_gpu_op* op) {
+  OpDev d;
+  d.nn = op->nn; d.nslot = op->nslot; d.nop = op->nop; d.nsmax = op->nsmax; d.ncls = op->ncls; d.nsp = op->nsp;
+  d.ntype = op->ntype; d.nstab = op->nstab;
+  d.hop = op->hop; d.flip = op->flip; d.info = op->info; d.sq = op->sq; d.stab_ptr = op->stab_ptr; d.stab = op->stab;
+  d.pi = op->pi; d.sg = op->sg; d.blk = op->blk; d.nsc = op->nsc;
+  return d;
+}
+
+int pow2_tile(int nb) {
+  int t = 1;
+  while (t < nb && t < 64) t <<= 1;
+  return t;
+}
+
+// Per-member host preparation: characters, local weights, frozen rows, uniform d.
+struct Members {
+  int nb = 0, nbp = 0, mt = 1;
+  bool uniform_d = true, zero_o = true, zero_e = true, complex_lin = false;
+  std::vector<uint64_t> chim;
+  std::vector<double> wts, fpar, dvec;
+  std::vector<int> frozen;
+  std::vector<cplx> z;
+};
+
+// chi(g) of the start orbital from the monomial tables: D(g)[l0, l0] = sg(l0, g) if pi(l0, g) = l0
+int chi_mask(const asr_gpu_op* op, int l0, uint64_t* mask) {
+  uint64_t mk = 0;
+  for (int g = 0; g < op->nop; ++g) {
+    if (op->pi_h[g * 9 + l0] != l0) return fail("augcls_set_start: start orbital is not one-dimensional under the group");
+    if (op->sg_h[g * 9 + l0] < 0) mk |= (1ull << g);
+  }
+  *mask = mk;
+  return 0;
+}
+
+// params: nchain x 4 x nsp x 9 (c, d, o, e); pab: nchain x 2 x nsp x 9 or null; conc: nchain x ncls x nsmax or null
+int prepare_members(const asr_gpu_op* op, int nchain, const int* start_orb, const double* params, const double* pab,
+                    const int* frozen_sp, const double* conc, const double* z, Members& M) {
+  const int nsm = op->nsmax, ncls = op->ncls, nsp = op->nsp;
+  M.nb = nchain;
+  M.mt = pow2_tile(nchain);
+  M.nbp = ((nchain + M.mt - 1) / M.mt) * M.mt;
+  const int nbp = M.nbp;
+  M.chim.assign(nbp, 0);
+  M.wts.assign((size_t)NQ * ncls * nsm * nsm * 9 * nbp, 0.0);
+  M.fpar.assign((size_t)NQ * 9 * nbp, 0.0);
+  M.dvec.assign((size_t)9 * nbp, 0.0);
+  M.frozen.assign(nbp, 0);
+  M.z.assign(nbp, cplx{0.0, 0.0});
+  M.complex_lin = (pab != nullptr);
+  std::vector<double> mt((size_t)ncls * nsm * nsm * nsm);
+  for (int m = 0; m < nchain; ++m) {
+    if (start_orb[m] < 0 || start_orb[m] > 8) return fail("start orbital out of range");
+    if (chi_mask(op, start_orb[m], &M.chim[m])) return 1;
+    // class tables: the operator's, or the member's concentrations (augcls_set_class_conc)
+    if (conc) {
+      std::fill(mt.begin(), mt.end(), 0.0);
+      for (int kap = 0; kap < ncls; ++kap) {
+        const double* cc = conc + ((size_t)m * ncls + kap) * nsm;
+        int n = op->nsc_h[kap];
+        double sum = 0.0;
+        for (int s = 0; s < n; ++s) {
+          if (cc[s] < 0.0) return fail("augcls_set_class_conc: concentrations");
+          sum += cc[s];
+        }
+        if (std::fabs(sum - 1.0) > 1e-8) return fail("augcls_set_class_conc: concentrations");
+        // augn_basis on the species of non-zero weight (class_mt, src/asr_augcls.f90:190-205)
+        std::vector<int> sup;
+        for (int s = 0; s < n; ++s)
+          if (cc[s] > 0.0) sup.push_back(s);
+        int ns = (int)sup.size();
+        std::vector<double> phi((size_t)ns * ns, 0.0);
+        for (int j = 0; j < ns; ++j) phi[j] = 1.0;
+        for (int l = 1; l < ns; ++l) {
+          for (int j = 0; j < ns; ++j) phi[l * ns + j] = 0.0;
+          phi[l * ns + l - 1] = 1.0;
+          for (int rep = 0; rep < 2; ++rep)
+            for (int q = 0; q < l; ++q) {
+              double ip = 0.0;
+              for (int j = 0; j < ns; ++j) ip += cc[sup[j]] * phi[q * ns + j] * phi[l * ns + j];
+              for (int j = 0; j < ns; ++j) phi[l * ns + j] -= ip * phi[q * ns + j];
+            }
+          double nr = 0.0;
+          for (int j = 0; j < ns; ++j) nr += cc[sup[j]] * phi[l * ns + j] * phi[l * ns + j];
+          nr = std::sqrt(nr);
+          for (int j = 0; j < ns; ++j) phi[l * ns + j] /= nr;
+        }
+        for (int l = 0; l < ns; ++l)
+          for (int k = 0; k < ns; ++k)
+            for (int j = 0; j < ns; ++j)
+              mt[(((size_t)kap * nsm + l) * nsm + k) * nsm + sup[j]] = cc[sup[j]] * phi[l * ns + j] * phi[k * ns + j];
+      }
+    } else {
+      mt = op->mt_h;
+    }
+    const double* pm = params + (size_t)m * 4 * nsp * 9;
+    for (int q = 0; q < NQ; ++q) {
+      const double* par = nullptr;
+      if (q < 4) par = pm + (size_t)q * nsp * 9;
+      else if (pab) par = pab + ((size_t)m * 2 + (q - 4)) * nsp * 9;
+      if (!par) continue;
+      // local weights wt(lam, k, kk, kap) = sum_s par(lam, spc(s, kap)) mt(k, kk, s, kap)  (local_weights)
+      for (int kap = 0; kap < ncls; ++kap)
+        for (int kk = 0; kk < op->nsc_h[kap]; ++kk)
+          for (int k = 0; k < op->nsc_h[kap]; ++k)
+            for (int lam = 0; lam < 9; ++lam) {
+              double s = 0.0;
+              for (int sp = 0; sp < op->nsc_h[kap]; ++sp)
+                s += par[op->spc_h[kap * nsm + sp] * 9 + lam] * mt[(((size_t)kap * nsm + k) * nsm + kk) * nsm + sp];
+              M.wts[(((((size_t)q * ncls + kap) * nsm + k) * nsm + kk) * 9 + lam) * nbp + m] = s;
+            }
+      if (frozen_sp && frozen_sp[m] >= 0) {
+        if (frozen_sp[m] >= nsp) return fail("augcls_freeze_site: species index out of range");
+        for (int l = 0; l < 9; ++l) M.fpar[((size_t)q * 9 + l) * nbp + m] = par[frozen_sp[m] * 9 + l];
+      }
+    }
+    M.frozen[m] = (frozen_sp && frozen_sp[m] >= 0) ? 1 : 0;
+    const double* dp = pm + (size_t)nsp * 9;
+    for (int s = 1; s < nsp; ++s)
+      for (int l = 0; l < 9; ++l)
+        if (dp[s * 9 + l] != dp[l]) M.uniform_d = false;
+    for (int l = 0; l < 9; ++l) M.dvec[(size_t)l * nbp + m] = dp[l];
+    for (int q = 0; q < nsp * 9; ++q) {
+      if (pm[2 * nsp * 9 + q] != 0.0) M.zero_o = false;
+      if (pm[3 * nsp * 9 + q] != 0.0) M.zero_e = false;
+    }
+    if (z) M.z[m] = cplx{z[2 * m], z[2 * m + 1]};
+  }
+  return 0;
+}
+
+struct MemBuffers {
+  uint64_t* chim = nullptr;
+  double *wts = nullptr, *fpar = nullptr, *dvec = nullptr;
+  int* frozen = nullptr;
+  cplx* z = nullptr;
+  ~MemBuffers() {
+    if (chim) (void)hipFree(chim);
+    if (wts) (void)hipFree(wts);
+    if (fpar) (void)hipFree(fpar);
+    if (dvec) (void)hipFree(dvec);
+    if (frozen) (void)hipFree(frozen);
+    if (z) (void)hipFree(z);
+  }
+};
+
+int upload_members(const asr_gpu_op* op, const Members& M, MemBuffers& B, MemDev& md) {
+  if (dalloc(&B.chim, M.chim.size()) || dalloc(&B.wts, M.wts.size()) || dalloc(&B.fpar, M.fpar.size()) ||
+      dalloc(&B.dvec, M.dvec.size()) || dalloc(&B.frozen, M.frozen.size()) || dalloc(&B.z, M.z.size()))
+    return 1;
+  HCHECK(hipMemcpyAsync(B.chim, M.chim.data(), M.chim.size() * 8, hipMemcpyHostToDevice, op->stream));
+  HCHECK(hipMemcpyAsync(B.wts, M.wts.data(), M.wts.size() * 8, hipMemcpyHostToDevice, op->stream));
+  HCHECK(hipMemcpyAsync(B.fpar, M.fpar.data(), M.fpar.size() * 8, hipMemcpyHostToDevice, op->stream));
+  HCHECK(hipMemcpyAsync(B.dvec, M.dvec.data(), M.dvec.size() * 8, hipMemcpyHostToDevice, op->stream));
+  HCHECK(hipMemcpyAsync(B.frozen, M.frozen.data(), M.frozen.size() * 4, hipMemcpyHostToDevice, op->stream));
+  HCHECK(hipMemcpyAsync(B.z, M.z.data(), M.z.size() * sizeof(cplx), hipMemcpyHostToDevice, op->stream));
+  md.nb = M.nb; md.nbp = M.nbp; md.mt = M.mt;
+  md.chimask = B.chim; md.wts = B.wts; md.fpar = B.fpar; md.frozen = B.frozen; md.dvec = B.dvec; md.z = B.z;
+  return 0;
+}
+
+size_t fused_lds_bytes(const asr_gpu_op* op, size_t tsize) {
+  size_t nblk = (size_t)op->ntype * (op->nslot + 1) * 81;
+  return nblk * 8 + (size_t)((op->nop * 9 + 1) & ~1) * 4 + (size_t)op->nop * 9 * 8 + BLOCK * tsize;
+}
+
+dim3 node_grid(const asr_gpu_op* op, const MemDev& md) {
+  int npw = WAVE / md.mt;
+  int nodes_per_block = npw * (BLOCK / WAVE);
+  return dim3((op->nn + nodes_per_block - 1) / nodes_per_block, md.nbp / md.mt, 1);
+}
+
+FArgs fargs0() {
+  FArgs f;
+  std::memset(&f, 0, sizeof(f));
+  return f;
+}
+
+template <class T>
+int launch_fused(const asr_gpu_op* op, const MemDev& md, const FArgs& fa) {
+  dim3 grid = node_grid(op, md);
+  size_t lds = fused_lds_bytes(op, sizeof(T));
+  hipLaunchKernelGGL(k_fused<T>, grid, dim3(BLOCK), lds, op->stream, dev_of(op), md, fa);
+  HCHECK(hipGetLastError());
+  return 0;
+}
+
+// y = H x (augcls_apply) for every member, normalised coordinates; w_dot: partial <dotu, y>.  Work vectors: t1, t2, h1.
+// lin: add X(z) x (complex members, energy-dependent diagonal of route edc).
+template <class T>
+int apply_chain_op(const asr_gpu_op* op, const Members& M, const MemDev& md, const T* x, T* y, T* t1, T* t2, T* h1, bool lin,
+                   const T* dotu, T* partial) {
+  // h1 = h x~ = rand_c(x~) + D S D x~
+  if (M.uniform_d) {
+    FArgs f = fargs0();
+    f.xs = x; f.xs_norm = 1; f.s_din = 1; f.s_dout = 1; f.s_coef = 1.0;
+    f.xr[0] = x; f.qr[0] = QC; f.r_norm[0] = 1; f.r_coef[0] = 1.0; f.nr = 1;
+    f.out = h1;
+    if (launch_fused<T>(op, md, f)) return 1;
+  } else {
+    FArgs f = fargs0();
+    f.xr[0] = x; f.qr[0] = QD; f.r_norm[0] = 1; f.r_coef[0] = 1.0; f.nr = 1; f.out = t1;
+    if (launch_fused<T>(op, md, f)) return 1;
+    f = fargs0();
+    f.xs = t1; f.s_coef = 1.0; f.out = t2;
+    if (launch_fused<T>(op, md, f)) return 1;
+    f = fargs0();
+    f.xr[0] = x; f.qr[0] = QC; f.r_norm[0] = 1; f.r_coef[0] = 1.0;
+    f.xr[1] = t2; f.qr[1] = QD; f.r_norm[1] = 0; f.r_coef[1] = 1.0; f.nr = 2;
+    f.out = h1;
+    if (launch_fused<T>(op, md, f)) return 1;
+  }
+  // final: y = sq (rand_e(x~) + h1 - h(rand_o(h1)) [+ X(z) x~])
+  FArgs f = fargs0();
+  int nr = 0;
+  if (!M.zero_o) {
+    FArgs g = fargs0();
+    g.xr[0] = h1; g.qr[0] = QO; g.r_norm[0] = 0; g.r_coef[0] = 1.0; g.nr = 1; g.out = t1;  // r = rand_o(h1)
+    if (launch_fused<T>(op, md, g)) return 1;
+    if (M.uniform_d) {
+      f.xs = t1; f.s_din = 1; f.s_dout = 1; f.s_coef = -1.0;
+    } else {
+      g = fargs0();
+      g.xr[0] = t1; g.qr[0] = QD; g.r_coef[0] = 1.0; g.nr = 1; g.out = t2;  // t2 = rand_d(r)
+      if (launch_fused<T>(op, md, g)) return 1;
+      g = fargs0();
+      g.xs = t2; g.s_coef = 1.0; g.out = y;  // y used as scratch: S rand_d(r)
+      if (launch_fused<T>(op, md, g)) return 1;
+      f.xr[nr] = y; f.qr[nr] = QD; f.r_norm[nr] = 0; f.r_coef[nr] = -1.0; ++nr;
+      // the final kernel reads y at flip partners and writes y: use t2 as its output and copy back below
+    }
+    f.xr[nr] = t1; f.qr[nr] = QC; f.r_norm[nr] = 0; f.r_coef[nr] = -1.0; ++nr;
+  }
+  if (!M.zero_e) { f.xr[nr] = x; f.qr[nr] = QE; f.r_norm[nr] = 1; f.r_coef[nr] = 1.0; ++nr; }
+  f.nr = nr;
+  if (lin) { f.xl = x; f.l_norm = 1; f.l_coef = 1.0; }
+  f.xo[0] = h1; f.o_coef[0] = 1.0; f.no = 1;
+  f.out_sq = 1;
+  f.dotu = dotu; f.partial = partial;
+  if (!M.zero_o && !M.uniform_d) {
+    f.out = t2;
+    if (launch_fused<T>(op, md, f)) return 1;
+    HCHECK(hipMemcpyAsync(y, t2, sizeof(T) * (size_t)op->nn * 9 * md.nbp, hipMemcpyDeviceToDevice, op->stream));
+  } else {
+    f.out = y;
+    if (launch_fused<T>(op, md, f)) return 1;
+  }
+  return 0;
+}
+
+template <class T>
+struct ChainBuffers {
+  T *u = nullptr, *up = nullptr, *w = nullptr, *t1 = nullptr, *t2 = nullptr, *h1 = nullptr, *part = nullptr;
+  T *a = nullptr, *b = nullptr, *a_eff = nullptr, *b_eff = nullptr, *inv_b = nullptr;
+  int *level = nullptr, *depth = nullptr, *active = nullptr, *mused = nullptr;
+  ~ChainBuffers() {
+    for (void* p : {(void*)u, (void*)up, (void*)w, (void*)t1, (void*)t2, (void*)h1, (void*)part, (void*)a, (void*)b,
+                    (void*)a_eff, (void*)b_eff, (void*)inv_b, (void*)level, (void*)depth, (void*)active, (void*)mused})
+      if (p) (void)hipFree(p);
+  }
+};
+
+template <class T>
+int run_chains(asr_gpu_op* op, int nchain, const int* start_orb, const double* params, const double* pab,
+               const int* frozen_sp, const double* conc, const double* z, const int* nlev, int project, int ldab, T* a_out,
+               T* b_out, int* mused_out, int mode, double tiny, bool real_chain, double* elapsed_ms) {
+  if (!op->have_tables || !op->have_ops || !op->have_blocks || !op->have_classes)
+    return fail("asr_gpu chains: operator tables incomplete");
+  if (nchain <= 0) return 0;
+  if (mode == 1 && nchain > 1) {
+    // plain form: one chain per launch sequence, host-driven levels
+    for (int c = 0; c < nchain; ++c) {
+      double ms = 0.0;
+      int st = run_chains<T>(op, 1, start_orb + c, params + (size_t)c * 4 * op->nsp * 9,
+                             pab ? pab + (size_t)c * 2 * op->nsp * 9 : nullptr, frozen_sp ? frozen_sp + c : nullptr,
+                             conc ? conc + (size_t)c * op->ncls * op->nsmax : nullptr, z ? z + 2 * c : nullptr, nlev + c,
+                             project, ldab, a_out + (size_t)c * ldab, b_out + (size_t)c * ldab, mused_out + c, 1, tiny,
+                             real_chain, &ms);
+      if (st) return st;
+      if (elapsed_ms) *elapsed_ms += ms;
+    }
+    return 0;
+  }
+  Members M;
+  if (prepare_members(op, nchain, start_orb, params, pab, frozen_sp, conc, z, M)) return 1;
+  MemBuffers MB;
+  MemDev md;
+  if (upload_members(op, M, MB, md)) return 1;
+  int maxdepth = 0;
+  for (int c = 0; c < nchain; ++c) maxdepth = std::max(maxdepth, nlev[c]);
+  if (maxdepth > ldab) return fail("chains: depth larger than the leading dimension of a, b");
+  const size_t nv = (size_t)op->nn * 9 * M.nbp;
+  ChainBuffers<T> B;
+  dim3 grid = node_grid(op, md);
+  const int nbx = grid.x;
+  const size_t npart = (size_t)grid.x * grid.y * md.mt;
+  if (dalloc(&B.u, nv) || dalloc(&B.up, nv) || dalloc(&B.w, nv) || dalloc(&B.t1, nv) || dalloc(&B.t2, nv) ||
+      dalloc(&B.h1, nv) || dalloc(&B.part, npart) || dalloc(&B.a, (size_t)M.nbp * ldab) ||
+      dalloc(&B.b, (size_t)M.nbp * ldab) || dalloc(&B.a_eff, M.nbp) || dalloc(&B.b_eff, M.nbp) ||
+      dalloc(&B.inv_b, M.nbp) || dalloc(&B.level, 1) || dalloc(&B.depth, M.nbp) || dalloc(&B.active, M.nbp) ||
+      dalloc(&B.mused, M.nbp))
+    return 1;
+  HCHECK(hipMemsetAsync(B.u, 0, nv * sizeof(T), op->stream));
+  HCHECK(hipMemsetAsync(B.up, 0, nv * sizeof(T), op->stream));
+  HCHECK(hipMemsetAsync(B.a, 0, (size_t)M.nbp * ldab * sizeof(T), op->stream));
+  HCHECK(hipMemsetAsync(B.b, 0, (size_t)M.nbp * ldab * sizeof(T), op->stream));
+  HCHECK(hipMemsetAsync(B.b_eff, 0, M.nbp * sizeof(T), op->stream));
+  HCHECK(hipMemsetAsync(B.a_eff, 0, M.nbp * sizeof(T), op->stream));
+  HCHECK(hipMemsetAsync(B.inv_b, 0, M.nbp * sizeof(T), op->stream));
+  HCHECK(hipMemsetAsync(B.w, 0, nv * sizeof(T), op->stream));
+  HCHECK(hipMemsetAsync(B.level, 0, sizeof(int), op->stream));
+  HCHECK(hipMemsetAsync(B.mused, 0, M.nbp * sizeof(int), op->stream));
+  std::vector<int> dep(M.nbp, 0), act(M.nbp, 0);
+  std::vector<T> one(1);
+  for (int c = 0; c < nchain; ++c) {
+    dep[c] = nlev[c];
+    act[c] = nlev[c] > 0 ? 1 : 0;
+    // start vector: unit at component iv = 9 * node 0 + l0
+    T v;
+    std::memset(&v, 0, sizeof(T));
+    *(double*)&v = 1.0;
+    HCHECK(hipMemcpyAsync(B.u + (size_t)start_orb[c] * M.nbp + c, &v, sizeof(T), hipMemcpyHostToDevice, op->stream));
+    HCHECK(hipStreamSynchronize(op->stream));
+  }
+  HCHECK(hipMemcpyAsync(B.depth, dep.data(), M.nbp * sizeof(int), hipMemcpyHostToDevice, op->stream));
+  HCHECK(hipMemcpyAsync(B.active, act.data(), M.nbp * sizeof(int), hipMemcpyHostToDevice, op->stream));
+  ChainDev cd;
+  cd.level = B.level; cd.depth = B.depth; cd.active = B.active; cd.mused = B.mused; cd.ld = ldab;
+  cd.real_chain = real_chain ? 1 : 0; cd.tiny = tiny;
+  const bool lin = (pab != nullptr);
+  hipEvent_t e0, e1;
+  HCHECK(hipEventCreate(&e0));
+  HCHECK(hipEventCreate(&e1));
+  HCHECK(hipStreamSynchronize(op->stream));
+  HCHECK(hipEventRecord(e0, op->stream));
+  const int fin_threads = 64, fin_blocks = (M.nb + fin_threads - 1) / fin_threads;
+  dim3 sgrid((unsigned)((nv + 255) / 256));
+  for (int k = 0; k < maxdepth; ++k) {
+    if (apply_chain_op<T>(op, M, md, B.u, B.w, B.t1, B.t2, B.h1, lin, B.u, B.part)) return 1;
+    hipLaunchKernelGGL(k_fin_a<T>, dim3(fin_blocks), dim3(fin_threads), 0, op->stream, cd, M.nb, B.part, nbx, md.mt, B.a,
+                       B.a_eff);
+    hipLaunchKernelGGL(k_update<T>, grid, dim3(BLOCK), 0, op->stream, dev_of(op), md, B.w, B.u, B.up, B.a_eff, B.b_eff,
+                       project, B.part);
+    hipLaunchKernelGGL(k_fin_b<T>, dim3(fin_blocks), dim3(fin_threads), 0, op->stream, cd, M.nb, B.part, nbx, md.mt, B.b,
+                       B.b_eff, B.inv_b);
+    hipLaunchKernelGGL(k_scale<T>, sgrid, dim3(256), 0, op->stream, B.w, B.inv_b, nv, M.nbp);
+    hipLaunchKernelGGL(k_level_inc, dim3(1), dim3(1), 0, op->stream, B.level);
+    HCHECK(hipGetLastError());
+    std::swap(B.up, B.u);  // up <- u
+    std::swap(B.u, B.w);   // u <- w / b ; w <- old up (overwritten next level)
+    if (mode == 1) {
+      // plain form: the host reads the level's coefficients back and stops when every chain has left
+      int actv = 0;
+      HCHECK(hipMemcpyAsync(&actv, B.active, sizeof(int), hipMemcpyDeviceToHost, op->stream));
+      HCHECK(hipStreamSynchronize(op->stream));
+      if (!actv) break;
+    }
+  }
+  HCHECK(hipEventRecord(e1, op->stream));
+  HCHECK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HCHECK(hipEventElapsedTime(&ms, e0, e1));
+  if (elapsed_ms) *elapsed_ms = ms;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  std::vector<T> ah((size_t)M.nbp * ldab), bh((size_t)M.nbp * ldab);
+  std::vector<int> mu(M.nbp);
+  HCHECK(hipMemcpy(ah.data(), B.a, ah.size() * sizeof(T), hipMemcpyDeviceToHost));
+  HCHECK(hipMemcpy(bh.data(), B.b, bh.size() * sizeof(T), hipMemcpyDeviceToHost));
+  HCHECK(hipMemcpy(mu.data(), B.mused, mu.size() * sizeof(int), hipMemcpyDeviceToHost));
+  for (int c = 0; c < nchain; ++c) {
+    std::memcpy(a_out + (size_t)c * ldab, ah.data() + (size_t)c * ldab, sizeof(T) * ldab);
+    std::memcpy(b_out + (size_t)c * ldab, bh.data() + (size_t)c * ldab, sizeof(T) * ldab);
+    mused_out[c] = mu[c];
+  }
+  return 0;
+}
+
+// host layout [member][dim] <-> device layout [(node*9 + l) * nbp + member]
+template <class T>
+void to_dev_layout(const T* host, std::vector<T>& dev, int nb, int nbp, size_t dim) {
+  dev.assign(dim * nbp, T());
+  std::memset(dev.data(), 0, dev.size() * sizeof(T));
+  for (int m = 0; m < nb; ++m)
+    for (size_t q = 0; q < dim; ++q) dev[q * nbp + m] = host[(size_t)m * dim + q];
+}
+template <class T>
+void from_dev_layout(const std::vector<T>& dev, T* host, int nb, int nbp, size_t dim) {
+  for (int m = 0; m < nb; ++m)
+    for (size_t q = 0; q < dim; ++q) host[(size_t)m * dim + q] = dev[q * nbp + m];
+}
+
+}  // namespace
+
+// ================================================================================================ C ABI
+extern "C" {
+
+const char* asr_gpu_last_error(void) { return g_err.c_str(); }
+
+int asr_gpu_device_count(int* n) {
+  HCHECK(hipGetDeviceCount(n));
+  return 0;
+}
+
+int asr_gpu_op_create(int nn, int nslot, int nop, int nsmax, int ncls, int nsp, int ntype, asr_gpu_op** out) {
+  *out = nullptr;
+  if (nn <= 0 || nslot < 0 || nop <= 0 || nop > 64 || nsmax <= 0 || nsmax > 16 || ncls <= 0 || ncls > 255 || nsp <= 0 ||
+      ntype <= 0 || ntype > 255)
+    return fail("asr_gpu_op_create: dimensions out of range (nop <= 64, nsmax <= 16, ncls, ntype <= 255)");
+  if ((size_t)nn >= (1u << 26)) return fail("asr_gpu_op_create: more than 2^26 nodes");
+  auto* op = new asr_gpu_op();
+  op->nn = nn; op->nslot = nslot; op->nop = nop; op->nsmax = nsmax; op->ncls = ncls; op->nsp = nsp; op->ntype = ntype;
+  op->nstab = 0;
+  hipError_t e = hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) { delete op; return fail(std::string("hipStreamCreate: ") + hipGetErrorString(e)); }
+  *out = op;
+  return 0;
+}
+
+int asr_gpu_op_set_tables(asr_gpu_op* op, const int* hopr, const int* hopg, const int* flipr, const int* flipg, const int* lst,
+                          const int* ncl, const int* ntyp, const unsigned char* fz, const double* sq, const int* stab_ptr,
+                          const int* stab, int nstab) {
+  const int nn = op->nn, ns = op->nslot, nsm = op->nsmax;
+  std::vector<uint32_t> hop((size_t)nn * ns), flip((size_t)nn * nsm), info(nn);
+  for (size_t q = 0; q < hop.size(); ++q) {
+    if (hopr[q] >= nn || (hopr[q] >= 0 && (hopg[q] < 0 || hopg[q] >= op->nop))) return fail("set_tables: hop link out of range");
+    hop[q] = hopr[q] < 0 ? 0u : ((uint32_t)(hopr[q] + 1) << 6) | (uint32_t)hopg[q];
+  }
+  for (size_t q = 0; q < flip.size(); ++q) {
+    if (flipr[q] >= nn || (flipr[q] >= 0 && (flipg[q] < 0 || flipg[q] >= op->nop)))
+      return fail("set_tables: flip link out of range");
+    flip[q] = flipr[q] < 0 ? 0u : ((uint32_t)(flipr[q] + 1) << 6) | (uint32_t)flipg[q];
+  }
+  for (int i = 0; i < nn; ++i) {
+    if (lst[i] < 0 || lst[i] >= nsm || ncl[i] < 0 || ncl[i] >= op->ncls || ntyp[i] < 0 || ntyp[i] >= op->ntype)
+      return fail("set_tables: node state, class or type out of range");
+    info[i] = (uint32_t)lst[i] | ((uint32_t)ncl[i] << 4) | ((uint32_t)ntyp[i] << 12) | ((uint32_t)(fz && fz[i] ? 1 : 0) << 20);
+  }
+  for (int i = 0; i <= nn; ++i)
+    if (stab_ptr[i] < 0 || stab_ptr[i] > nstab || (i > 0 && stab_ptr[i] < stab_ptr[i - 1]))
+      return fail("set_tables: stabiliser pointers");
+  for (int j = 0; j < nstab; ++j)
+    if (stab[j] < 0 || stab[j] >= op->nop) return fail("set_tables: stabiliser operation out of range");
+  if (!op->hop && (dalloc(&op->hop, hop.size()) || dalloc(&op->flip, flip.size()) || dalloc(&op->info, nn) ||
+                   dalloc(&op->sq, nn) || dalloc(&op->stab_ptr, nn + 1)))
+    return 1;
+  if (op->stab) (void)hipFree(op->stab);
+  op->stab = nullptr;
+  if (dalloc(&op->stab, std::max(nstab, 1))) return 1;
+  op->nstab = nstab;
+  HCHECK(hipMemcpy(op->hop, hop.data(), hop.size() * 4, hipMemcpyHostToDevice));
+  HCHECK(hipMemcpy(op->flip, flip.data(), flip.size() * 4, hipMemcpyHostToDevice));
+  HCHECK(hipMemcpy(op->info, info.data(), info.size() * 4, hipMemcpyHostToDevice));
+  HCHECK(hipMemcpy(op->sq, sq, (size_t)nn * 8, hipMemcpyHostToDevice));
+  HCHECK(hipMemcpy(op->stab_ptr, stab_ptr, (size_t)(nn + 1) * 4, hipMemcpyHostToDevice));
+  if (nstab > 0) HCHECK(hipMemcpy(op->stab, stab, (size_t)nstab * 4, hipMemcpyHostToDevice));
+  op->have_tables = true;
+  return 0;
+}
+
+int asr_gpu_op_set_orbital_ops(asr_gpu_op* op, const int* pi, const double* sg) {
+  for (int q = 0; q < op->nop * 9; ++q)
+    if (pi[q] < 0 || pi[q] > 8) return fail("set_orbital_ops: orbital index out of range");
+  op->pi_h.assign(pi, pi + op->nop * 9);
+  op->sg_h.assign(sg, sg + op->nop * 9);
+  if (!op->pi && (dalloc(&op->pi, op->nop * 9) || dalloc(&op->sg, op->nop * 9))) return 1;
+  HCHECK(hipMemcpy(op->pi, pi, (size_t)op->nop * 9 * 4, hipMemcpyHostToDevice));
+  HCHECK(hipMemcpy(op->sg, sg, (size_t)op->nop * 9 * 8, hipMemcpyHostToDevice));
+  op->have_ops = true;
+  return 0;
+}
+
+int asr_gpu_op_set_blocks(asr_gpu_op* op, const double* block, const double* block0) {
+  const int ns = op->nslot;
+  if ((size_t)op->ntype * (ns + 1) * 81 * 8 > 120 * 1024) return fail("set_blocks: bond tables exceed the LDS budget");
+  std::vector<double> blk((size_t)op->ntype * (ns + 1) * 81);
+  for (int t = 0; t < op->ntype; ++t) {
+    std::memcpy(&blk[(size_t)t * (ns + 1) * 81], block0 + (size_t)t * 81, 81 * 8);
+    std::memcpy(&blk[((size_t)t * (ns + 1) + 1) * 81], block + (size_t)t * ns * 81, (size_t)ns * 81 * 8);
+  }
+  if (!op->blk && dalloc(&op->blk, blk.size())) return 1;
+  HCHECK(hipMemcpy(op->blk, blk.data(), blk.size() * 8, hipMemcpyHostToDevice));
+  op->have_blocks = true;
+  return 0;
+}
+
+int asr_gpu_op_set_classes(asr_gpu_op* op, const int* nsc, const int* spc, const double* mt) {
+  for (int k = 0; k < op->ncls; ++k) {
+    if (nsc[k] < 1 || nsc[k] > op->nsmax) return fail("set_classes: species count of a class out of range");
+    for (int s = 0; s < nsc[k]; ++s)
+      if (spc[k * op->nsmax + s] < 0 || spc[k * op->nsmax + s] >= op->nsp) return fail("set_classes: species out of range");
+  }
+  op->nsc_h.assign(nsc, nsc + op->ncls);
+  op->spc_h.assign(spc, spc + (size_t)op->ncls * op->nsmax);
+  op->mt_h.assign(mt, mt + (size_t)op->ncls * op->nsmax * op->nsmax * op->nsmax);
+  if (!op->nsc && dalloc(&op->nsc, op->ncls)) return 1;
+  HCHECK(hipMemcpy(op->nsc, nsc, (size_t)op->ncls * 4, hipMemcpyHostToDevice));
+  op->have_classes = true;
+  return 0;
+}
+
+void asr_gpu_op_destroy(asr_gpu_op* op) {
+  if (!op) return;
+  for (void* p : {(void*)op->hop, (void*)op->flip, (void*)op->info, (void*)op->sq, (void*)op->sg, (void*)op->blk,
+                  (void*)op->stab_ptr, (void*)op->stab, (void*)op->pi, (void*)op->nsc})
+    if (p) (void)hipFree(p);
+  if (op->stream) (void)hipStreamDestroy(op->stream);
+  delete op;
+}
+
+// y = H x for nb members (real vectors, host layout [member][9 nn]); the members' parameters as the chain calls.
+int asr_gpu_apply_batched(asr_gpu_op* op, int nb, const int* start_orb, const double* params, const int* frozen_sp,
+                          const double* conc, const double* x, double* y) {
+  if (!op->have_tables || !op->have_ops || !op->have_blocks || !op->have_classes)
+    return fail("asr_gpu_apply: operator tables incomplete");
+  Members M;
+  if (prepare_members(op, nb, start_orb, params, nullptr, frozen_sp, conc, nullptr, M)) return 1;
+  MemBuffers MB;
+  MemDev md;
+  if (upload_members(op, M, MB, md)) return 1;
+  const size_t dim = (size_t)op->nn * 9, nv = dim * M.nbp;
+  std::vector<double> hx;
+  to_dev_layout(x, hx, nb, M.nbp, dim);
+  double *dx, *dy, *t1, *t2, *h1;
+  if (dalloc(&dx, nv) || dalloc(&dy, nv) || dalloc(&t1, nv) || dalloc(&t2, nv) || dalloc(&h1, nv)) return 1;
+  HCHECK(hipMemcpyAsync(dx, hx.data(), nv * 8, hipMemcpyHostToDevice, op->stream));
+  int st = apply_chain_op<double>(op, M, md, dx, dy, t1, t2, h1, false, nullptr, nullptr);
+  if (!st) {
+    HCHECK(hipMemcpyAsync(hx.data(), dy, nv * 8, hipMemcpyDeviceToHost, op->stream));
+    HCHECK(hipStreamSynchronize(op->stream));
+    from_dev_layout(hx, y, nb, M.nbp, dim);
+  }
+  for (double* p : {dx, dy, t1, t2, h1}) (void)hipFree(p);
+  return st;
+}
+
+// w = X(z) x (augcls_rand_c with X(z) = z pa + pb) plus H x: the edc operator K(z) x for complex members.
+int asr_gpu_apply_complex(asr_gpu_op* op, int nb, const int* start_orb, const double* params, const double* pab,
+                          const int* frozen_sp, const double* conc, const double* z, const double* x, double* y) {
+  if (!op->have_tables || !op->have_ops || !op->have_blocks || !op->have_classes)
+    return fail("asr_gpu_apply: operator tables incomplete");
+  Members M;
+  if (prepare_members(op, nb, start_orb, params, pab, frozen_sp, conc, z, M)) return 1;
+  MemBuffers MB;
+  MemDev md;
+  if (upload_members(op, M, MB, md)) return 1;
+  const size_t dim = (size_t)op->nn * 9, nv = dim * M.nbp;
+  std::vector<cplx> hx;
+  to_dev_layout((const cplx*)x, hx, nb, M.nbp, dim);
+  cplx *dx, *dy, *t1, *t2, *h1;
+  if (dalloc(&dx, nv) || dalloc(&dy, nv) || dalloc(&t1, nv) || dalloc(&t2, nv) || dalloc(&h1, nv)) return 1;
+  HCHECK(hipMemcpyAsync(dx, hx.data(), nv * sizeof(cplx), hipMemcpyHostToDevice, op->stream));
+  int st = apply_chain_op<cplx>(op, M, md, dx, dy, t1, t2, h1, pab != nullptr, nullptr, nullptr);
+  if (!st) {
+    HCHECK(hipMemcpyAsync(hx.data(), dy, nv * sizeof(cplx), hipMemcpyDeviceToHost, op->stream));
+    HCHECK(hipStreamSynchronize(op->stream));
+    from_dev_layout(hx, (cplx*)y, nb, M.nbp, dim);
+  }
+  for (cplx* p : {dx, dy, t1, t2, h1}) (void)hipFree(p);
+  return st;
+}
+
+int asr_gpu_lanczos_real(asr_gpu_op* op, int nchain, const int* start_orb, const double* params, const int* frozen_sp,
+                         const double* conc, const int* nlev, double tiny, int ldab, double* a, double* b, int* mused, int mode,
+                         double* elapsed_ms) {
+  if (elapsed_ms) *elapsed_ms = 0.0;
+  return run_chains<double>(op, nchain, start_orb, params, nullptr, frozen_sp, conc, nullptr, nlev, 0, ldab, a, b, mused, mode,
+                            tiny > 0 ? tiny : 1e-12, true, elapsed_ms);
+}
+
+int asr_gpu_chains_edc(asr_gpu_op* op, int nchain, const int* start_orb, const double* params, const double* pab,
+                       const int* frozen_sp, const double* conc, const double* z, const int* nlev, int project, int ldab,
+                       double* a, double* b, int* mused, int mode, double* elapsed_ms) {
+  if (elapsed_ms) *elapsed_ms = 0.0;
+  return run_chains<cplx>(op, nchain, start_orb, params, pab, frozen_sp, conc, z, nlev, project, ldab, (cplx*)a, (cplx*)b,
+                          mused, mode, 1e-12, false, elapsed_ms);
+}
+
+// ------------------------------------------------------------------------------------------------ KPM
+int asr_gpu_real_create(int nsite, int nslot, const int* nbr, int nsp, const unsigned char* species, const double* par,
+                        const double* block, const double* block0, asr_gpu_real** out) {
+  *out = nullptr;
+  if (nslot > 18) return fail("asr_gpu_real_create: at most 18 neighbour slots");
+  for (size_t q = 0; q < (size_t)nsite * nslot; ++q)
+    if (nbr[q] < 0 || nbr[q] >= nsite) return fail("asr_gpu_real_create: neighbour index out of range");
+  for (int i = 0; i < nsite; ++i)
+    if (species[i] >= nsp) return fail("asr_gpu_real_create: species out of range");
+  for (int q = 2 * nsp * 9; q < 4 * nsp * 9; ++q)
+    if (par[q] != 0.0) return fail("asr_gpu_real_create: the KPM kernel applies h = c + d S d (o = E_nu = 0)");
+  auto* R = new asr_gpu_real();
+  R->nsite = nsite; R->nslot = nslot; R->nsp = nsp;
+  std::vector<double> blk((size_t)(nslot + 1) * 81);
+  std::memcpy(blk.data(), block0, 81 * 8);
+  std::memcpy(blk.data() + 81, block, (size_t)nslot * 81 * 8);
+  if (dalloc(&R->nbr, (size_t)nsite * nslot) || dalloc(&R->spec, nsite) || dalloc(&R->par, (size_t)4 * nsp * 9) ||
+      dalloc(&R->blk, blk.size())) {
+    delete R;
+    return 1;
+  }
+  HCHECK(hipStreamCreateWithFlags(&R->stream, hipStreamNonBlocking));
+  HCHECK(hipMemcpy(R->nbr, nbr, (size_t)nsite * nslot * 4, hipMemcpyHostToDevice));
+  HCHECK(hipMemcpy(R->spec, species, nsite, hipMemcpyHostToDevice));
+  HCHECK(hipMemcpy(R->par, par, (size_t)4 * nsp * 9 * 8, hipMemcpyHostToDevice));
+  HCHECK(hipMemcpy(R->blk, blk.data(), blk.size() * 8, hipMemcpyHostToDevice));
+  *out = R;
+  return 0;
+}
+
+void asr_gpu_real_destroy(asr_gpu_real* R) {
+  if (!R) return;
+  for (void* p : {(void*)R->nbr, (void*)R->spec, (void*)R->par, (void*)R->blk})
+    if (p) (void)hipFree(p);
+  if (R->stream) (void)hipStreamDestroy(R->stream);
+  delete R;
+}
+
+static RealDev real_dev(const asr_gpu_real* R) {
+  RealDev d;
+  d.nsite = R->nsite; d.nslot = R->nslot; d.nsp = R->nsp; d.nbr = R->nbr; d.spec = R->spec; d.par = R->par; d.blk = R->blk;
+  return d;
+}
+
+// y = h x for nvec vectors (host layout [vector][9 nsite])
+int asr_gpu_real_apply(asr_gpu_real* R, int nvec, const double* x, double* y) {
+  const size_t dim = (size_t)R->nsite * 9;
+  const int nvp = ((nvec + 63) / 64) * 64;
+  std::vector<double> hx;
+  to_dev_layout(x, hx, nvec, nvp, dim);
+  double *dx, *dy;
+  if (dalloc(&dx, dim * nvp) || dalloc(&dy, dim * nvp)) return 1;
+  HCHECK(hipMemcpy(dx, hx.data(), dim * nvp * 8, hipMemcpyHostToDevice));
+  dim3 grid((R->nsite + 3) / 4, nvp / 64);
+  hipLaunchKernelGGL(k_kpm_step, grid, dim3(BLOCK), 0, R->stream, real_dev(R), nvec, nvp, dx, (const double*)nullptr, dy, 0.0,
+                     1.0, 0.0, 0ull, (double*)nullptr, 0);
+  HCHECK(hipGetLastError());
+  HCHECK(hipStreamSynchronize(R->stream));
+  HCHECK(hipMemcpy(hx.data(), dy, dim * nvp * 8, hipMemcpyDeviceToHost));
+  from_dev_layout(hx, y, nvec, nvp, dim);
+  (void)hipFree(dx);
+  (void)hipFree(dy);
+  return 0;
+}
+
+// mu_n = (1/R) sum_r <r|T_n((H - c)/s)|r> / dim (kpm_moments, src/reference/asr_kpm.f90:39-65) for nvec Rademacher vectors
+// in one batched SpMM per moment.  mu_per (optional): nvec x nmom per-vector terms.  step_ms: average duration of the
+// Chebyshev step kernel (HIP events on the engine's stream); total_ms: the whole recurrence.
+int asr_gpu_kpm_moments(asr_gpu_real* R, double c, double s, int nmom, int nvec, uint64_t seed, double* mu, double* mu_per,
+                        double* step_ms, double* total_ms) {
+  if (nmom < 1 || nvec < 1) return fail("kpm_moments: nmom and nvec must be positive");
+  if (seed >= (1ull << 24) || nvec > 4096) return fail("kpm_moments: seed < 2^24 and nvec <= 4096");
+  const size_t dim = (size_t)R->nsite * 9;
+  if (dim >= (1ull << 28)) return fail("kpm_moments: more than 2^28 rows");
+  const int nvp = ((nvec + 63) / 64) * 64;
+  dim3 grid((R->nsite + 3) / 4, nvp / 64);
+  double *t0, *t1, *t2, *part, *mper;
+  if (dalloc(&t0, dim * nvp) || dalloc(&t1, dim * nvp) || dalloc(&t2, dim * nvp) ||
+      dalloc(&part, (size_t)grid.x * grid.y * 64) || dalloc(&mper, (size_t)nvec * nmom))
+    return 1;
+  const double inv_dim = 1.0 / (double)dim;
+  hipEvent_t e0, e1, s0, s1;
+  HCHECK(hipEventCreate(&e0)); HCHECK(hipEventCreate(&e1)); HCHECK(hipEventCreate(&s0)); HCHECK(hipEventCreate(&s1));
+  HCHECK(hipEventRecord(e0, R->stream));
+  const RealDev rd = real_dev(R);
+  const int fb = (nvec + 63) / 64;
+  // mu_0 = <r|r>/dim: the step kernel with alpha = 0, beta = -1 gives y = r
+  hipLaunchKernelGGL(k_kpm_step, grid, dim3(BLOCK), 0, R->stream, rd, nvec, nvp, (const double*)nullptr,
+                     (const double*)nullptr, t0, 0.0, 0.0, -1.0, seed, part, 1);
+  hipLaunchKernelGGL(k_kpm_fin, dim3(fb), dim3(64), 0, R->stream, part, (int)grid.x, nvec, inv_dim, mper, nmom, 0);
+  if (nmom > 1) {
+    // t1 = (H r - c r) / s
+    hipLaunchKernelGGL(k_kpm_step, grid, dim3(BLOCK), 0, R->stream, rd, nvec, nvp, (const double*)nullptr,
+                       (const double*)nullptr, t1, c, 1.0 / s, 0.0, seed, part, 1);
+    hipLaunchKernelGGL(k_kpm_fin, dim3(fb), dim3(64), 0, R->stream, part, (int)grid.x, nvec, inv_dim, mper, nmom, 1);
+  }
+  HCHECK(hipEventRecord(s0, R->stream));
+  for (int n = 2; n < nmom; ++n) {
+    // t2 = 2 (H t1 - c t1) / s - t0
+    hipLaunchKernelGGL(k_kpm_step, grid, dim3(BLOCK), 0, R->stream, rd, nvec, nvp, (const double*)t1, (const double*)t0, t2,
+                       c, 2.0 / s, 1.0, seed, part, 1);
+    hipLaunchKernelGGL(k_kpm_fin, dim3(fb), dim3(64), 0, R->stream, part, (int)grid.x, nvec, inv_dim, mper, nmom, n);
+    double* tmp = t0; t0 = t1; t1 = t2; t2 = tmp;
+  }
+  HCHECK(hipEventRecord(s1, R->stream));
+  HCHECK(hipEventRecord(e1, R->stream));
+  HCHECK(hipGetLastError());
+  HCHECK(hipEventSynchronize(e1));
+  float ms_all = 0.f, ms_steps = 0.f;
+  HCHECK(hipEventElapsedTime(&ms_all, e0, e1));
+  HCHECK(hipEventElapsedTime(&ms_steps, s0, s1));
+  std::vector<double> per((size_t)nvec * nmom);
+  HCHECK(hipMemcpy(per.data(), mper, per.size() * 8, hipMemcpyDeviceToHost));
+  for (int n = 0; n < nmom; ++n) {
+    double acc = 0.0;
+    for (int v = 0; v < nvec; ++v) acc += per[(size_t)v * nmom + n];
+    mu[n] = acc / (double)nvec;
+  }
+  if (mu_per) std::memcpy(mu_per, per.data(), per.size() * 8);
+  if (step_ms) *step_ms = nmom > 2 ? ms_steps / (nmom - 2) : 0.0;
+  if (total_ms) *total_ms = ms_all;
+  for (double* p : {t0, t1, t2, part, mper}) (void)hipFree(p);
+  for (hipEvent_t e : {e0, e1, s0, s1}) (void)hipEventDestroy(e);
+  return 0;
+}
+
+}  // extern "C"

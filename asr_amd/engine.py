@@ -1,0 +1,220 @@
+"""Python side of the MI355X chain engine: ctypes bindings of libasr_hip.so (include/asr_hip.h).
+
+This mirrors the reference's operator interface for the hot path (augcls_op with augcls_set_params / set_start /
+freeze_site / set_class_conc, src/asr_augcls.f90:107-185; the chain routines lanczos, edc_green, edc_green_blk; kpm_moments):
+the operator tables live on the GPU, a chain is described by its start orbital, species parameters, frozen species and
+(for route edc) its energy.  There is no CPU fallback: without the HIP library every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .augcls import AugclsOperator, Member
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+def lib() -> C.CDLL:
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_PKG, "libasr_hip.so")
+        if not os.path.exists(path):
+            raise EngineError(f"{path} is missing: build it with `python -m asr_amd.build hip`")
+        L = C.CDLL(path)
+        P = C.c_void_p
+        L.asr_gpu_last_error.restype = C.c_char_p
+        L.asr_gpu_device_count.argtypes = [P]
+        L.asr_gpu_op_create.argtypes = [C.c_int] * 7 + [C.POINTER(C.c_void_p)]
+        L.asr_gpu_op_set_tables.argtypes = [P] * 11 + [P, C.c_int]
+        L.asr_gpu_op_set_orbital_ops.argtypes = [P, P, P]
+        L.asr_gpu_op_set_blocks.argtypes = [P, P, P]
+        L.asr_gpu_op_set_classes.argtypes = [P, P, P, P]
+        L.asr_gpu_op_destroy.argtypes = [P]
+        L.asr_gpu_op_destroy.restype = None
+        L.asr_gpu_apply_batched.argtypes = [P, C.c_int, P, P, P, P, P, P]
+        L.asr_gpu_apply_complex.argtypes = [P, C.c_int, P, P, P, P, P, P, P, P]
+        L.asr_gpu_lanczos_real.argtypes = [P, C.c_int, P, P, P, P, P, C.c_double, C.c_int, P, P, P, C.c_int, P]
+        L.asr_gpu_chains_edc.argtypes = [P, C.c_int, P, P, P, P, P, P, P, C.c_int, C.c_int, P, P, P, C.c_int, P]
+        L.asr_gpu_real_create.argtypes = [C.c_int, C.c_int, P, C.c_int, P, P, P, P, C.POINTER(C.c_void_p)]
+        L.asr_gpu_real_destroy.argtypes = [P]
+        L.asr_gpu_real_destroy.restype = None
+        L.asr_gpu_real_apply.argtypes = [P, C.c_int, P, P]
+        L.asr_gpu_kpm_moments.argtypes = [P, C.c_double, C.c_double, C.c_int, C.c_int, C.c_uint64, P, P, P, P]
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _check(status: int, what: str):
+    if status != 0:
+        raise EngineError(f"{what}: {lib().asr_gpu_last_error().decode()}")
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    _check(lib().asr_gpu_device_count(C.byref(n)), "asr_gpu_device_count")
+    return n.value
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def pack_members(op: AugclsOperator, members: list[Member], complex_route: bool = False):
+    nb = len(members)
+    start = _i32([m.l0 for m in members])
+    params = _f64(np.stack([np.stack([np.asarray(m.c, float).reshape(op.nsp, 9), np.asarray(m.d, float).reshape(op.nsp, 9),
+                                      np.asarray(m.o, float).reshape(op.nsp, 9), np.asarray(m.e, float).reshape(op.nsp, 9)])
+                            for m in members]))
+    frozen = _i32([m.frozen for m in members])
+    conc = None
+    if any(m.conc is not None for m in members):
+        conc = np.zeros((nb, op.ncls, op.nsmax))
+        for j, m in enumerate(members):
+            cc = op.conc if m.conc is None else m.conc
+            for kap in range(op.ncls):
+                conc[j, kap, :op.nsc[kap]] = np.asarray(cc[kap])[:op.nsc[kap]]
+        conc = _f64(conc)
+    pab = z = None
+    if complex_route:
+        pab = _f64(np.stack([np.stack([np.asarray(m.pa, float).reshape(op.nsp, 9), np.asarray(m.pb, float).reshape(op.nsp, 9)])
+                             for m in members]))
+        z = np.ascontiguousarray(np.array([complex(m.z) for m in members], dtype=np.complex128))
+    return start, params, frozen, conc, pab, z
+
+
+class GpuOperator:
+    """An augcls operator resident on the GPU (one node set, shared by every chain on it)."""
+
+    def __init__(self, op: AugclsOperator):
+        self.op = op
+        h = C.c_void_p()
+        _check(lib().asr_gpu_op_create(op.nn, op.nslot, op.nop, op.nsmax, op.ncls, op.nsp, op.ntype, C.byref(h)),
+               "asr_gpu_op_create")
+        self.h = h
+        stab = op.stab if op.stab.size else np.zeros(1, np.int32)
+        _check(lib().asr_gpu_op_set_tables(h, _p(_i32(op.hopr)), _p(_i32(op.hopg)), _p(_i32(op.flipr)), _p(_i32(op.flipg)),
+                                           _p(_i32(op.lst)), _p(_i32(op.ncl)), _p(_i32(op.ntyp)),
+                                           _p(np.ascontiguousarray(op.fz, np.uint8)), _p(_f64(op.sq)), _p(_i32(op.stab_ptr)),
+                                           _p(_i32(stab)), int(op.stab.size)), "asr_gpu_op_set_tables")
+        _check(lib().asr_gpu_op_set_orbital_ops(h, _p(_i32(op.pi)), _p(_f64(op.sg))), "asr_gpu_op_set_orbital_ops")
+        _check(lib().asr_gpu_op_set_blocks(h, _p(_f64(op.block)), _p(_f64(op.block0))), "asr_gpu_op_set_blocks")
+        _check(lib().asr_gpu_op_set_classes(h, _p(_i32(op.nsc)), _p(_i32(op.spc)), _p(_f64(op.mt))), "asr_gpu_op_set_classes")
+
+    def close(self):
+        if self.h:
+            lib().asr_gpu_op_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def apply(self, members: list[Member], x: np.ndarray) -> np.ndarray:
+        """y = H x for every member (x: nb x 9 nn, real)."""
+        x = _f64(np.atleast_2d(x))
+        if x.shape != (len(members), self.op.dim):
+            raise ValueError("apply: x must be (members, 9 nn)")
+        y = np.zeros_like(x)
+        s, p, f, c, _, _ = pack_members(self.op, members)
+        _check(lib().asr_gpu_apply_batched(self.h, len(members), _p(s), _p(p), _p(f), _p(c), _p(x), _p(y)),
+               "asr_gpu_apply_batched")
+        return y
+
+    def apply_complex(self, members: list[Member], x: np.ndarray) -> np.ndarray:
+        """y = H x + X(z) x for complex members (route edc operator)."""
+        x = np.ascontiguousarray(np.atleast_2d(x), dtype=np.complex128)
+        y = np.zeros_like(x)
+        s, p, f, c, pab, z = pack_members(self.op, members, complex_route=True)
+        _check(lib().asr_gpu_apply_complex(self.h, len(members), _p(s), _p(p), _p(pab), _p(f), _p(c), _p(z), _p(x), _p(y)),
+               "asr_gpu_apply_complex")
+        return y
+
+    def lanczos(self, members: list[Member], nlev, tiny: float = 1.0e-12, mode: int = 0):
+        """Real Lanczos chains (lanczos) of every member; mode 0 batched, 1 plain (one chain per launch)."""
+        nch = len(members)
+        nlev = _i32(np.broadcast_to(np.asarray(nlev), (nch,)))
+        ld = int(nlev.max())
+        a = np.zeros((nch, ld))
+        b = np.zeros((nch, ld))
+        mu = np.zeros(nch, np.int32)
+        ms = np.zeros(1)
+        s, p, f, c, _, _ = pack_members(self.op, members)
+        _check(lib().asr_gpu_lanczos_real(self.h, nch, _p(s), _p(p), _p(f), _p(c), _p(nlev), float(tiny), ld, _p(a), _p(b),
+                                          _p(mu), int(mode), _p(ms)), "asr_gpu_lanczos_real")
+        return a, b, mu, float(ms[0])
+
+    def chains_edc(self, members: list[Member], nlev, project: bool = True, mode: int = 0):
+        """Complex-symmetric chains of route edc (edc_green / edc_green_blk, depth_tol = 0)."""
+        nch = len(members)
+        nlev = _i32(np.broadcast_to(np.asarray(nlev), (nch,)))
+        ld = int(nlev.max())
+        a = np.zeros((nch, ld), np.complex128)
+        b = np.zeros((nch, ld), np.complex128)
+        mu = np.zeros(nch, np.int32)
+        ms = np.zeros(1)
+        s, p, f, c, pab, z = pack_members(self.op, members, complex_route=True)
+        _check(lib().asr_gpu_chains_edc(self.h, nch, _p(s), _p(p), _p(pab), _p(f), _p(c), _p(z), _p(nlev), int(project), ld,
+                                        _p(a), _p(b), _p(mu), int(mode), _p(ms)), "asr_gpu_chains_edc")
+        return a, b, mu, float(ms[0])
+
+
+class GpuRealization:
+    """Supercell realisation h = c + d S d on the GPU (KPM operator)."""
+
+    def __init__(self, nbr: np.ndarray, species: np.ndarray, par: np.ndarray, block: np.ndarray, block0: np.ndarray):
+        nbr = _i32(nbr)
+        self.nsite, self.nslot = nbr.shape
+        self.dim = 9 * self.nsite
+        par = _f64(par)
+        nsp = par.shape[1]
+        h = C.c_void_p()
+        _check(lib().asr_gpu_real_create(self.nsite, self.nslot, _p(nbr), nsp, _p(np.ascontiguousarray(species, np.uint8)),
+                                         _p(par), _p(_f64(block)), _p(_f64(block0)), C.byref(h)), "asr_gpu_real_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().asr_gpu_real_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def apply(self, x: np.ndarray) -> np.ndarray:
+        x = _f64(np.atleast_2d(x))
+        y = np.zeros_like(x)
+        _check(lib().asr_gpu_real_apply(self.h, x.shape[0], _p(x), _p(y)), "asr_gpu_real_apply")
+        return y
+
+    def kpm_moments(self, c: float, s: float, nmom: int, nvec: int, seed: int, per_vector: bool = False):
+        mu = np.zeros(nmom)
+        per = np.zeros((nvec, nmom)) if per_vector else None
+        step = np.zeros(1)
+        tot = np.zeros(1)
+        _check(lib().asr_gpu_kpm_moments(self.h, float(c), float(s), int(nmom), int(nvec), int(seed), _p(mu), _p(per),
+                                         _p(step), _p(tot)), "asr_gpu_kpm_moments")
+        out = dict(mu=mu, step_ms=float(step[0]), total_ms=float(tot[0]))
+        if per_vector:
+            out["mu_per"] = per
+        return out

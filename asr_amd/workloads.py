@@ -1,0 +1,138 @@
+"""Seeded synthetic workloads of the chain engine (the model systems the throughput is quoted on).
+
+    plain_binary      fcc A50B50, spd (9 orbitals), screened S^alpha to 2 shells (12 + 6 slots), c ~ U[-0.5, 0.5] Ry and
+                      d ~ U[0.05, 0.15] per species and l, o = E_nu = 0 (H = h = c + d S d), one class, real chains of the
+                      9 start orbitals (point group D2h: every orbital one-dimensional)
+    ternary_edc       fcc Fe0.66Ni0.14Cr0.20-like ternary, 2 spin parameter sets, start site frozen to each species,
+                      route-edc parameters (c, d, o, E_nu per species and spin mapped as edc_setup, src/asr_edc.f90:41-71),
+                      9 orbitals x 3 species x 2 spins complex-symmetric chains at one contour energy
+    l21_batch         4-sublattice L2_1-type bcc cell with partial disorder (1.0 / 0.5-0.5 / 0.4-0.6 / 0.3-0.3-0.4), one
+                      operator per start sublattice, Warren-Cowley alpha_1 short-range order of the start site's first shell,
+                      2 spins, chains of uneven depth
+    kpm_supercell     fcc supercell of L^3 conventional cells (4 L^3 sites, 9 orbitals), seeded random ternary,
+                      h = c + d S d with the same blocks
+Species parameters are drawn per (species, l) and spread over the orbitals of each l (s | p x3 | d x5).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import structure as S
+from .augcls import AugclsOperator, Member, build_operator
+
+ORB_L = S.ORB_L
+
+
+def per_orbital(v_l: np.ndarray) -> np.ndarray:
+    """(nsp, 3) per-l values -> (nsp, 9) per orbital."""
+    return np.ascontiguousarray(np.asarray(v_l)[:, ORB_L])
+
+
+_BLOCKS = {}
+
+
+def bond_blocks(kind: str, rcl: float = 1.0, alpha=S.STANDARD_ALPHA):
+    key = (kind, rcl)
+    if key not in _BLOCKS:
+        vec, blk, r2max, w = S.lmto_alloy_blocks(kind, rcl, alpha)
+        _BLOCKS[key] = (vec, blk, r2max)
+    return _BLOCKS[key]
+
+
+def graph_and_blocks(kind: str, L: int, rcl: float = 1.0):
+    vec, blk, r2max = bond_blocks(kind, rcl)
+    g = S.periodic_graph_shells(kind, L, r2max)
+    block, block0 = S.bond_blocks_for(g.bvec, vec, blk)
+    # symmetry zeros of the screened blocks are exact; rounding residue below 1e-14 is set to zero
+    block[np.abs(block) < 1.0e-14] = 0.0
+    block0 = block0.copy()
+    block0[np.abs(block0) < 1.0e-14] = 0.0
+    return g, block, block0
+
+
+def draw_cd(rng: np.random.Generator, nsp: int):
+    c = per_orbital(rng.uniform(-0.5, 0.5, (nsp, 3)))
+    d = per_orbital(rng.uniform(0.05, 0.15, (nsp, 3)))
+    return c, d
+
+
+@dataclass
+class Workload:
+    name: str
+    op: AugclsOperator
+    members: list
+    nlev: np.ndarray
+    complex_route: bool
+    info: dict
+
+
+def plain_binary(L: int = 8, levels: int = 4, depth: int = 20, seed: int = 1, group: str = "d2h") -> Workload:
+    rng = np.random.default_rng(seed)
+    g, block, block0 = graph_and_blocks("fcc", L)
+    cls = np.zeros(g.n, np.int32)
+    perm, mats = S.class_ops(g.coord, 2 * L, cls, group)
+    dmat = S.orbital_rep(mats)
+    op = build_operator(g, cls, [2], [[0, 1]], [[0.5, 0.5]], perm, dmat, 0, levels, maxrank=1,
+                        block=block[None], block0=block0[None], nsp=2)
+    c, d = draw_cd(rng, 2)
+    z = np.zeros((2, 9))
+    members = [Member(l0=l0, c=c, d=d, o=z, e=z) for l0 in range(9)]
+    return Workload("fcc A50B50 plain ASR", op, members, np.full(9, depth, np.int32), False,
+                    dict(L=L, levels=levels, group=group, seed=seed, nodes=op.nn))
+
+
+def edc_params(rng: np.random.Generator, nsp: int, nspin: int):
+    """Per spin: C - E_nu (c), sqrt(Delta) (d), E_nu and o per species and l (the gamma-to-alpha step is left out)."""
+    out = []
+    for _ in range(nspin):
+        c = rng.uniform(-0.5, 0.5, (nsp, 3))
+        sd = rng.uniform(0.05, 0.15, (nsp, 3))
+        enu = rng.uniform(-0.1, 0.1, (nsp, 3))
+        o = rng.uniform(-0.05, 0.05, (nsp, 3))
+        out.append(dict(c=c, sqrdel=sd, enu=enu, o=o))
+    return out
+
+
+def edc_member(pp: dict, central: int, l0: int, z: complex) -> Member:
+    """Species parameters for the central species as edc_setup (src/asr_edc.f90:41-71) and X(z) = z pa + pb (:253-258)."""
+    nsp = pp["c"].shape[0]
+    dc = per_orbital(pp["sqrdel"][central:central + 1] ** 2)[0]        # D_c (9)
+    ds = per_orbital(pp["sqrdel"] ** 2)                                 # D_s (nsp, 9)
+    c = dc[None, :] / ds * per_orbital(pp["c"])
+    d = np.broadcast_to(np.abs(per_orbital(pp["sqrdel"][central:central + 1]))[0], (nsp, 9)).copy()
+    o = per_orbital(pp["o"]) * ds / dc[None, :]
+    e = np.zeros((nsp, 9))
+    pa = 1.0 - dc[None, :] / ds
+    pb = dc[None, :] * per_orbital(pp["enu"]) / ds
+    return Member(l0=l0, c=c, d=d, o=o, e=e, frozen=central, pa=pa, pb=pb, z=z)
+
+
+def ternary_edc(L: int = 8, levels: int = 4, depth: int = 30, seed: int = 2, group: str = "d2h",
+                z: complex = complex(-0.2, 0.15), conc=(0.66, 0.14, 0.20)) -> Workload:
+    rng = np.random.default_rng(seed)
+    g, block, block0 = graph_and_blocks("fcc", L)
+    cls = np.zeros(g.n, np.int32)
+    perm, mats = S.class_ops(g.coord, 2 * L, cls, group)
+    dmat = S.orbital_rep(mats)
+    fl = np.ones(g.n, np.uint8)
+    fl[0] = 0                                    # the start site carries the central species (never flips)
+    op = build_operator(g, cls, [3], [[0, 1, 2]], [list(conc)], perm, dmat, 0, levels, maxrank=1, flippable=fl,
+                        block=block[None], block0=block0[None], nsp=3)
+    spins = edc_params(rng, 3, 2)
+    members = [edc_member(spins[sp], cen, l0, z) for sp in range(2) for cen in range(3) for l0 in range(9)]
+    return Workload("fcc ternary edc", op, members, np.full(len(members), depth, np.int32), True,
+                    dict(L=L, levels=levels, group=group, seed=seed, nodes=op.nn, z=z))
+
+
+def kpm_supercell(L: int = 40, seed: int = 4, conc=(0.66, 0.14, 0.20)):
+    """Random ternary fcc supercell: (graph, species per site, par (4, nsp, 9), block, block0)."""
+    rng = np.random.default_rng(seed)
+    g, block, block0 = graph_and_blocks("fcc", L)
+    species = rng.choice(len(conc), size=g.n, p=np.asarray(conc)).astype(np.uint8)
+    c, d = draw_cd(rng, len(conc))
+    par = np.zeros((4, len(conc), 9))
+    par[0] = c
+    par[1] = d
+    return g, species, par, block, block0

@@ -1,0 +1,87 @@
+/* C ABI of the MI355X chain engine (libasr_hip.so): the augmented-space operator, its real and complex-symmetric Lanczos
+ * chains, and the KPM moments of a supercell realisation.
+ *
+ * Conventions: 0-based indices, -1 = no link, row-major arrays, complex numbers as interleaved (re, im) doubles.  Host arrays
+ * are read only for the duration of a call.  Every call returns 0 on success; on failure it returns non-zero and
+ * asr_gpu_last_error() describes the error (the Fortran side turns it into `error stop`, as the reference does).
+ * Calls are made from one host thread; every handle owns its device memory and a HIP stream.
+ *
+ * The species parameter array of a chain, `params`, holds c, d, o, e (each nsp x 9, [species][orbital]), as augcls_op%c,
+ * %d, %o, %e of the reference (src/asr_augcls.f90:50, set by augcls_set_params :121-126).  A chain's start component is
+ * 9 * node 0 + start_orb (the start node is node 0, as iv of src/asr_edc.f90:73-74).
+ */
+#ifndef ASR_HIP_H
+#define ASR_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct asr_gpu_op asr_gpu_op;
+typedef struct asr_gpu_real asr_gpu_real;
+
+const char* asr_gpu_last_error(void);
+int asr_gpu_device_count(int* n);
+
+/* ---- the site-class augmented-space operator (augcls_op, src/asr_augcls.f90:45-68) */
+/* Dimensions: nn nodes, nslot bond slots, nop group operations (<= 64), nsmax states per site, ncls classes, nsp species,
+ * ntype site types.  Replaces the allocation part of build_augcls_operator (src/asr_augcls.f90:709-727). */
+int asr_gpu_op_create(int nn, int nslot, int nop, int nsmax, int ncls, int nsp, int ntype, asr_gpu_op** out);
+/* Node tables (src/asr_augcls.f90:55-63): hopr/hopg nn x nslot, flipr/flipg nn x nsmax, lst/ncl/ntyp/fz/sq nn,
+ * stab_ptr nn + 1, stab nstab.  fz[i] = 1: node i sits on the start site (augcls_freeze_site applies there, :129-141). */
+int asr_gpu_op_set_tables(asr_gpu_op* op, const int* hopr, const int* hopg, const int* flipr, const int* flipg, const int* lst,
+                          const int* ncl, const int* ntyp, const unsigned char* fz, const double* sq, const int* stab_ptr,
+                          const int* stab, int nstab);
+/* Orbital operation tables pi, sg (nop x 9): D(g)[pi(l), l] = sg(l) (src/asr_augcls.f90:727-736). */
+int asr_gpu_op_set_orbital_ops(asr_gpu_op* op, const int* pi, const double* sg);
+/* Bond blocks block (ntype x nslot x 9 x 9) and on-site blocks block0 (ntype x 9 x 9): mdl%block, mdl%block0
+ * (augcls_set_blocks, src/asr_augcls.f90:99-104). */
+int asr_gpu_op_set_blocks(asr_gpu_op* op, const double* block, const double* block0);
+/* Classes: nsc (ncls), spc (ncls x nsmax), mt (ncls x nsmax x nsmax x nsmax) = class_mt (src/asr_augcls.f90:190-205). */
+int asr_gpu_op_set_classes(asr_gpu_op* op, const int* nsc, const int* spc, const double* mt);
+void asr_gpu_op_destroy(asr_gpu_op* op);
+
+/* y = H x, H = E_nu + h - h o h in normalised coordinates (augcls_apply, src/asr_augcls.f90:275-297) for nb chains at once
+ * (blk_hamx without the energy term, src/asr_augblk.f90:219-250).  x, y: nb x 9 nn.  frozen_sp: species of the frozen
+ * start site per chain (-1 none) or NULL; conc: nb x ncls x nsmax class concentrations (augcls_set_class_conc) or NULL. */
+int asr_gpu_apply_batched(asr_gpu_op* op, int nb, const int* start_orb, const double* params, const int* frozen_sp,
+                          const double* conc, const double* x, double* y);
+/* y = H x + X(z) x for complex x (augcls_apply on Re and Im plus augcls_rand_c, src/asr_edc.f90:106-116); X(z) = z pa + pb,
+ * pab: nb x 2 x nsp x 9 (asr_edc.f90:253-258); z: nb complex.  x, y: nb x 9 nn complex. */
+int asr_gpu_apply_complex(asr_gpu_op* op, int nb, const int* start_orb, const double* params, const double* pab,
+                          const int* frozen_sp, const double* conc, const double* z, const double* x, double* y);
+
+/* Real Lanczos chains (lanczos, src/reference/asr_recursion.f90:17-58) from the unit vector of each chain's start
+ * component, nlev[c] levels each.  a, b: nchain x ldab; b of the last level is the norm of the last residual.  mused: levels
+ * done.  mode 0: the batched engine (all chains stacked as one block vector); mode 1: the plain form, one chain per launch
+ * sequence with a host synchronisation per level.  elapsed_ms: device time of the recursion (HIP events). */
+int asr_gpu_lanczos_real(asr_gpu_op* op, int nchain, const int* start_orb, const double* params, const int* frozen_sp,
+                         const double* conc, const int* nlev, double tiny, int ldab, double* a, double* b, int* mused, int mode,
+                         double* elapsed_ms);
+/* Complex-symmetric chains of route edc (edc_green, src/asr_edc.f90:75-164; block form edc_green_blk :237-316) without the
+ * adaptive depth: chain c at energy z[c], depth nlev[c], projection onto the symmetric subspace when project != 0.
+ * a, b: nchain x ldab complex.  Replaces the chain_worker task loop (src/asr_class_scf.f90:1093-1157) for one pass. */
+int asr_gpu_chains_edc(asr_gpu_op* op, int nchain, const int* start_orb, const double* params, const double* pab,
+                       const int* frozen_sp, const double* conc, const double* z, const int* nlev, int project, int ldab,
+                       double* a, double* b, int* mused, int mode, double* elapsed_ms);
+
+/* ---- KPM on a supercell realisation (lmto_real_op, src/reference/asr_lmto_realization.f90:14-23) */
+/* nbr: nsite x nslot neighbour sites (nslot <= 18), species: nsite, par: 4 x nsp x 9 (c, d, o, e; o = e = 0 required: the
+ * kernel applies h = c + d S d), block: nslot x 9 x 9, block0: 9 x 9. */
+int asr_gpu_real_create(int nsite, int nslot, const int* nbr, int nsp, const unsigned char* species, const double* par,
+                        const double* block, const double* block0, asr_gpu_real** out);
+void asr_gpu_real_destroy(asr_gpu_real* r);
+/* y = h x for nvec vectors (h_apply, src/reference/asr_lmto_realization.f90:120-147); x, y: nvec x 9 nsite. */
+int asr_gpu_real_apply(asr_gpu_real* r, int nvec, const double* x, double* y);
+/* Chebyshev moments with nvec Rademacher vectors as one batched SpMM per moment (kpm_moments,
+ * src/reference/asr_kpm.f90:39-65).  The vectors are r(v, row) = +-1 from splitmix64((seed << 40) ^ (v << 28) ^ row).
+ * mu: nmom; mu_per: nvec x nmom or NULL; step_ms: mean duration of one Chebyshev step; total_ms: whole recurrence. */
+int asr_gpu_kpm_moments(asr_gpu_real* r, double c, double s, int nmom, int nvec, uint64_t seed, double* mu, double* mu_per,
+                        double* step_ms, double* total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

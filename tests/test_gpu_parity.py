@@ -1,0 +1,150 @@
+"""GPU parity: the HIP engine (through the C ABI) against the oracle on the same seeded inputs.
+
+Bar: FP64 chain coefficients a_n, b_n to 1e-9 relative over the levels run, operator applications to 1e-12 relative,
+KPM moments to 1e-10 absolute (moments are O(1)).
+"""
+import numpy as np
+import pytest
+
+from asr_amd import augcls as A
+from asr_amd import structure as S
+from asr_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _engine():
+    from asr_amd import engine
+    assert engine.device_count() >= 1, "no GPU visible"
+    return engine
+
+
+def _oracle(op):
+    from oracle.oracle import OracleOp
+    return OracleOp(op)
+
+
+def rel(a, b):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / max(np.max(np.abs(np.asarray(b))), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def binary_small():
+    return W.plain_binary(L=4, levels=2, depth=12, seed=11)
+
+
+@pytest.fixture(scope="module")
+def ternary_small():
+    return W.ternary_edc(L=4, levels=2, depth=14, seed=12)
+
+
+def test_apply_real_members(binary_small):
+    eng = _engine()
+    wl = binary_small
+    op = wl.op
+    rng = np.random.default_rng(0)
+    # three members: non-uniform d, a frozen-free chain, o != 0 and e != 0 on one of them
+    m0 = wl.members[0]
+    m1 = wl.members[4]
+    m2 = A.Member(l0=6, c=m0.c, d=m0.d, o=rng.uniform(-0.05, 0.05, (2, 9)), e=rng.uniform(-0.1, 0.1, (2, 9)))
+    members = [m0, m1, m2]
+    x = rng.standard_normal((3, op.dim))
+    G = eng.GpuOperator(op)
+    y = G.apply(members, x)
+    O = _oracle(op)
+    for j, m in enumerate(members):
+        assert rel(y[j], O.apply(m, x[j])) < 1e-12, f"member {j}"
+
+
+def test_apply_complex_edc_operator(ternary_small):
+    eng = _engine()
+    wl = ternary_small
+    op = wl.op
+    rng = np.random.default_rng(1)
+    members = [wl.members[k] for k in (0, 13, 40)]
+    x = rng.standard_normal((3, op.dim)) + 1j * rng.standard_normal((3, op.dim))
+    G = eng.GpuOperator(op)
+    y = G.apply_complex(members, x)
+    O = _oracle(op)
+    for j, m in enumerate(members):
+        ep = m.z * m.pa + m.pb
+        ref = O.apply(m, x[j].real) + 1j * O.apply(m, x[j].imag) + O.rand_c(m, ep, x[j])
+        assert rel(y[j], ref) < 1e-12, f"member {j}"
+
+
+def test_lanczos_real_batched_and_plain(binary_small):
+    eng = _engine()
+    wl = binary_small
+    G = eng.GpuOperator(wl.op)
+    a, b, mu, _ = G.lanczos(wl.members, wl.nlev, mode=0)
+    a1, b1, mu1, _ = G.lanczos(wl.members[:3], wl.nlev[:3], mode=1)
+    O = _oracle(wl.op)
+    for j, m in enumerate(wl.members):
+        ar, br, n = O.lanczos(m, m.l0, int(wl.nlev[j]))
+        assert mu[j] == n
+        assert rel(a[j, :n], ar[:n]) < 1e-9
+        assert rel(b[j, :n], br[:n]) < 1e-9
+    assert np.array_equal(a[:3], a1) and np.array_equal(b[:3], b1), "plain and batched forms must agree bitwise"
+
+
+def test_lanczos_uneven_depths(binary_small):
+    eng = _engine()
+    wl = binary_small
+    G = eng.GpuOperator(wl.op)
+    nlev = np.array([3, 12, 7, 1, 9, 12, 5, 2, 11], np.int32)
+    a, b, mu, _ = G.lanczos(wl.members, nlev)
+    O = _oracle(wl.op)
+    for j, m in enumerate(wl.members):
+        ar, br, n = O.lanczos(m, m.l0, int(nlev[j]))
+        assert mu[j] == n == nlev[j]
+        assert rel(a[j, :n], ar[:n]) < 1e-9
+        assert rel(b[j, :n], br[:n]) < 1e-9
+        assert np.all(a[j, n:] == 0.0)
+
+
+@pytest.mark.parametrize("project", [True, False])
+def test_edc_chains_batched(ternary_small, project):
+    eng = _engine()
+    wl = ternary_small
+    G = eng.GpuOperator(wl.op)
+    idx = [0, 4, 9, 17, 22, 30, 44, 53]
+    members = [wl.members[k] for k in idx]
+    nlev = np.array([14, 14, 10, 14, 6, 14, 13, 14], np.int32)
+    a, b, mu, _ = G.chains_edc(members, nlev, project=project)
+    O = _oracle(wl.op)
+    for j, m in enumerate(members):
+        ep = m.z * m.pa + m.pb
+        ar, br, n = O.edc_chain(m, ep, m.l0, int(nlev[j]), project=project)
+        assert mu[j] == n
+        assert rel(a[j, :n], ar[:n]) < 1e-9, f"chain {j}"
+        assert rel(b[j, :n - 1], br[:n - 1]) < 1e-9, f"chain {j}"
+
+
+def test_edc_plain_equals_batched(ternary_small):
+    eng = _engine()
+    wl = ternary_small
+    G = eng.GpuOperator(wl.op)
+    members = wl.members[:4]
+    a, b, mu, _ = G.chains_edc(members, 10, mode=0)
+    a1, b1, mu1, _ = G.chains_edc(members, 10, mode=1)
+    assert np.array_equal(mu, mu1)
+    assert rel(a1, a) < 1e-13 and rel(b1, b) < 1e-13
+
+
+def test_kpm_apply_and_moments():
+    eng = _engine()
+    from oracle.oracle import OracleReal
+    g, species, par, block, block0 = W.kpm_supercell(L=3, seed=21)
+    R = eng.GpuRealization(g.nbr, species, par, block, block0)
+    O = OracleReal(g.nbr, species, par[0], par[1], par[2], par[3], block, block0)
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal((3, R.dim))
+    y = R.apply(x)
+    for j in range(3):
+        assert rel(y[j], O.apply(x[j])) < 1e-12
+    c, s = 0.05, 2.5
+    res = R.kpm_moments(c, s, 48, 5, seed=7, per_vector=True)
+    mu_o, per_o = O.kpm_moments(c, s, 48, 5, seed=7, per_vector=True)
+    assert np.max(np.abs(res["mu_per"] - per_o)) < 1e-10
+    assert np.max(np.abs(res["mu"] - mu_o)) < 1e-10
+    assert abs(res["mu"][0] - 1.0) < 1e-14
