@@ -725,6 +725,36 @@ dim3 node_grid(const asr_gpu_op* op, const MemDev& md) {
   return dim3((op->nn + nodes_per_block - 1) / nodes_per_block, md.nbp / md.mt, 1);
 }
 
+// Optional per-kernel timing (HIP events on the engine's stream), used by asr_gpu_profile_level.
+struct KTimer {
+  std::vector<hipEvent_t> ev;
+  std::vector<int> kind;
+  int ord = 0;
+};
+KTimer* g_timer = nullptr;
+std::vector<double> g_acc;  // summed duration per kernel ordinal of a level
+int g_levels = 0;
+
+enum { KIND_SPMM = 0, KIND_RAND = 1, KIND_FIN_A = 2, KIND_UPDATE = 3, KIND_FIN_B = 4, KIND_SCALE = 5 };
+
+int timer_mark(hipStream_t s, int kind, bool start) {
+  if (!g_timer) return 0;
+  if (start) {
+    size_t need = 2 * (size_t)(g_timer->ord + 1);
+    while (g_timer->ev.size() < need) {
+      hipEvent_t e;
+      HCHECK(hipEventCreate(&e));
+      g_timer->ev.push_back(e);
+    }
+    if (g_timer->kind.size() < (size_t)g_timer->ord + 1) g_timer->kind.push_back(kind);
+    HCHECK(hipEventRecord(g_timer->ev[2 * g_timer->ord], s));
+  } else {
+    HCHECK(hipEventRecord(g_timer->ev[2 * g_timer->ord + 1], s));
+    g_timer->ord++;
+  }
+  return 0;
+}
+
 FArgs fargs0() {
   FArgs f;
   std::memset(&f, 0, sizeof(f));
@@ -735,8 +765,10 @@ template <class T>
 int launch_fused(const asr_gpu_op* op, const MemDev& md, const FArgs& fa) {
   dim3 grid = node_grid(op, md);
   size_t lds = fused_lds_bytes(op, sizeof(T));
+  if (timer_mark(op->stream, fa.xs ? KIND_SPMM : KIND_RAND, true)) return 1;
   hipLaunchKernelGGL(k_fused<T>, grid, dim3(BLOCK), lds, op->stream, dev_of(op), md, fa);
   HCHECK(hipGetLastError());
+  if (timer_mark(op->stream, 0, false)) return 1;
   return 0;
 }
 
@@ -891,16 +923,33 @@ int run_chains(asr_gpu_op* op, int nchain, const int* start_orb, const double* p
   const int fin_threads = 64, fin_blocks = (M.nb + fin_threads - 1) / fin_threads;
   dim3 sgrid((unsigned)((nv + 255) / 256));
   for (int k = 0; k < maxdepth; ++k) {
+    if (g_timer) g_timer->ord = 0;
     if (apply_chain_op<T>(op, M, md, B.u, B.w, B.t1, B.t2, B.h1, lin, B.u, B.part)) return 1;
+    if (timer_mark(op->stream, KIND_FIN_A, true)) return 1;
     hipLaunchKernelGGL(k_fin_a<T>, dim3(fin_blocks), dim3(fin_threads), 0, op->stream, cd, M.nb, B.part, nbx, md.mt, B.a,
                        B.a_eff);
+    if (timer_mark(op->stream, 0, false) || timer_mark(op->stream, KIND_UPDATE, true)) return 1;
     hipLaunchKernelGGL(k_update<T>, grid, dim3(BLOCK), 0, op->stream, dev_of(op), md, B.w, B.u, B.up, B.a_eff, B.b_eff,
                        project, B.part);
+    if (timer_mark(op->stream, 0, false) || timer_mark(op->stream, KIND_FIN_B, true)) return 1;
     hipLaunchKernelGGL(k_fin_b<T>, dim3(fin_blocks), dim3(fin_threads), 0, op->stream, cd, M.nb, B.part, nbx, md.mt, B.b,
                        B.b_eff, B.inv_b);
+    if (timer_mark(op->stream, 0, false) || timer_mark(op->stream, KIND_SCALE, true)) return 1;
     hipLaunchKernelGGL(k_scale<T>, sgrid, dim3(256), 0, op->stream, B.w, B.inv_b, nv, M.nbp);
+    if (timer_mark(op->stream, 0, false)) return 1;
     hipLaunchKernelGGL(k_level_inc, dim3(1), dim3(1), 0, op->stream, B.level);
     HCHECK(hipGetLastError());
+    if (g_timer) {
+      // accumulate this level's kernel durations
+      HCHECK(hipStreamSynchronize(op->stream));
+      if (g_acc.size() < (size_t)g_timer->ord) g_acc.resize(g_timer->ord, 0.0);
+      for (int q = 0; q < g_timer->ord; ++q) {
+        float t = 0.f;
+        HCHECK(hipEventElapsedTime(&t, g_timer->ev[2 * q], g_timer->ev[2 * q + 1]));
+        g_acc[q] += t;
+      }
+      g_levels++;
+    }
     std::swap(B.up, B.u);  // up <- u
     std::swap(B.u, B.w);   // u <- w / b ; w <- old up (overwritten next level)
     if (mode == 1) {
@@ -1129,6 +1178,41 @@ int asr_gpu_chains_edc(asr_gpu_op* op, int nchain, const int* start_orb, const d
   if (elapsed_ms) *elapsed_ms = 0.0;
   return run_chains<cplx>(op, nchain, start_orb, params, pab, frozen_sp, conc, z, nlev, project, ldab, (cplx*)a, (cplx*)b,
                           mused, mode, 1e-12, false, elapsed_ms);
+}
+
+// Per-kernel durations of the chain level step: the same chains as asr_gpu_chains_edc (pab != NULL) or
+// asr_gpu_lanczos_real (pab == NULL), every kernel of every level bracketed by HIP events on the engine's stream.
+// kernel_ms[q], kind[q] (q < *nk <= 16): mean duration of the q-th launch of a level and its kind
+// (0 fused SpMM, 1 flip-gather term, 2 a reduction, 3 Lanczos update + projection, 4 b reduction, 5 scaling).
+int asr_gpu_profile_chains(asr_gpu_op* op, int nchain, const int* start_orb, const double* params, const double* pab,
+                           const int* frozen_sp, const double* conc, const double* z, const int* nlev, int project, int* nk,
+                           double* kernel_ms, int* kind) {
+  KTimer T;
+  g_timer = &T;
+  g_acc.clear();
+  g_levels = 0;
+  int maxd = 0;
+  for (int c = 0; c < nchain; ++c) maxd = std::max(maxd, nlev[c]);
+  std::vector<double> a((size_t)nchain * maxd * 2), b((size_t)nchain * maxd * 2);
+  std::vector<int> mu(nchain);
+  double ms = 0.0;
+  int st;
+  if (pab)
+    st = run_chains<cplx>(op, nchain, start_orb, params, pab, frozen_sp, conc, z, nlev, project, maxd, (cplx*)a.data(),
+                          (cplx*)b.data(), mu.data(), 0, 1e-12, false, &ms);
+  else
+    st = run_chains<double>(op, nchain, start_orb, params, nullptr, frozen_sp, conc, nullptr, nlev, 0, maxd, a.data(),
+                            b.data(), mu.data(), 0, 1e-12, true, &ms);
+  g_timer = nullptr;
+  for (hipEvent_t e : T.ev) (void)hipEventDestroy(e);
+  if (st) return st;
+  int n = std::min((int)g_acc.size(), 16);
+  *nk = n;
+  for (int q = 0; q < n; ++q) {
+    kernel_ms[q] = g_levels ? g_acc[q] / g_levels : 0.0;
+    kind[q] = T.kind[q];
+  }
+  return 0;
 }
 
 // ------------------------------------------------------------------------------------------------ KPM

@@ -43,6 +43,7 @@ def lib() -> C.CDLL:
         L.asr_gpu_apply_complex.argtypes = [P, C.c_int, P, P, P, P, P, P, P, P]
         L.asr_gpu_lanczos_real.argtypes = [P, C.c_int, P, P, P, P, P, C.c_double, C.c_int, P, P, P, C.c_int, P]
         L.asr_gpu_chains_edc.argtypes = [P, C.c_int, P, P, P, P, P, P, P, C.c_int, C.c_int, P, P, P, C.c_int, P]
+        L.asr_gpu_profile_chains.argtypes = [P, C.c_int, P, P, P, P, P, P, P, C.c_int, P, P, P]
         L.asr_gpu_real_create.argtypes = [C.c_int, C.c_int, P, C.c_int, P, P, P, P, C.POINTER(C.c_void_p)]
         L.asr_gpu_real_destroy.argtypes = [P]
         L.asr_gpu_real_destroy.restype = None
@@ -174,6 +175,19 @@ class GpuOperator:
         _check(lib().asr_gpu_chains_edc(self.h, nch, _p(s), _p(p), _p(pab), _p(f), _p(c), _p(z), _p(nlev), int(project), ld,
                                         _p(a), _p(b), _p(mu), int(mode), _p(ms)), "asr_gpu_chains_edc")
         return a, b, mu, float(ms[0])
+
+    def profile(self, members: list[Member], nlev, project: bool = True, complex_route: bool = True):
+        """Mean duration (ms) and kind of every kernel launch of one chain level."""
+        nch = len(members)
+        nlev = _i32(np.broadcast_to(np.asarray(nlev), (nch,)))
+        s, p, f, c, pab, z = pack_members(self.op, members, complex_route=complex_route)
+        nk = np.zeros(1, np.int32)
+        ms = np.zeros(16)
+        kind = np.zeros(16, np.int32)
+        _check(lib().asr_gpu_profile_chains(self.h, nch, _p(s), _p(p), _p(pab), _p(f), _p(c), _p(z), _p(nlev), int(project),
+                                            _p(nk), _p(ms), _p(kind)), "asr_gpu_profile_chains")
+        n = int(nk[0])
+        return ms[:n].copy(), kind[:n].copy()
 
 
 class GpuRealization:

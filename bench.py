@@ -1,0 +1,218 @@
+"""Throughput of the MI355X chain engine on its headline workload, one JSON line on stdout.
+
+Headline: chain-levels/s of the batched complex-symmetric (route edc) recursion on the fcc ternary workload
+(Fe0.66Ni0.14Cr0.20-like, spd, 2 shells = 18 slots, levels 4, rank 1, D2h-reduced node set, start site frozen to each
+species, 2 spins: 9 x 3 x 2 = 54 chains of depth 30, FP64), next to the plain form (one chain per launch sequence, host
+synchronisation per level) on the same chains, the KPM moments of the 40^3-cell fcc supercell (2,304,000 rows, 64
+Rademacher vectors, 2048 moments) and the oracle's CPU chains on the host cores.
+
+A step is one batched pass of all 54 chains to depth 30; inputs (operator tables, start vectors) are resident on the GPU
+when the timed region starts.  python bench.py [--steps K] [--warmup W] [--quick]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0       # MI355X HBM3E, GB/s
+FP64_PEAK_TFS = 78.6        # MI355X dense FP64 (vector and matrix), TFLOP/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def chain_kernel_cost(op, nb: int, kind: int, uniform_d: bool = True):
+    """Algorithmic bytes and flops of one launch of a chain-level kernel over nn nodes and nb complex members.
+
+    SpMM (kind 0, h = rand_c(x~) + D S D x~): read x once (9 nb complex per node), write the result once, read the node
+    tables once (hop 4 nslot + flip 4 nsmax + info 4 + sq 8 bytes per node); flops: (1 + nslot) 9x9 blocks on complex
+    members (4 flops per complex-by-real FMA pair) plus the flip terms and the D scalings."""
+    nn, ns, nsm = op.nn, op.nslot, op.nsmax
+    vec = nn * 9 * nb * 16
+    tables = nn * (4 * ns + 4 * nsm + 4 + 8)
+    if kind == 0:
+        bytes_ = 2 * vec + tables
+        flops = nn * nb * ((1 + ns) * 81 * 4 + nsm * 9 * 4 + (1 + ns) * 9 * 2 * 2 + 9 * 4)
+    elif kind == 1:
+        bytes_ = 2 * vec + nn * (4 * nsm + 4 + 8)
+        flops = nn * nb * (nsm * 9 * 4)
+    elif kind == 3:
+        bytes_ = 4 * vec
+        flops = nn * nb * 9 * 16
+    else:
+        bytes_ = 2 * vec
+        flops = nn * nb * 9 * 6
+    return float(bytes_), float(flops)
+
+
+def kpm_kernel_cost(nsite: int, nslot: int, nvec: int):
+    """One Chebyshev step over nvec vectors: read t_n, read t_{n-1}, write t_{n+1} (the Rademacher vector is regenerated,
+    never read); per site the species byte and nslot neighbour indices; flops: (1 + nslot) 9x9 blocks, the d scalings."""
+    rows = nsite * 9
+    bytes_ = rows * nvec * 8 * 3 + nsite * (1 + 4 * nslot)
+    flops = nsite * nvec * ((1 + nslot) * 81 * 2 + (1 + nslot) * 9 + 9 * 6)
+    return float(bytes_), float(flops)
+
+
+def roofline(bytes_, flops, ms):
+    t = ms * 1e-3
+    gbs = bytes_ / t / 1e9
+    tfs = flops / t / 1e12
+    intensity = flops / bytes_
+    if intensity > (FP64_PEAK_TFS * 1e12) / (HBM_PEAK_GBS * 1e9):
+        return dict(bound="mfma", achieved=round(tfs, 3), peak=FP64_PEAK_TFS, unit="TFLOP/s", frac=round(tfs / FP64_PEAK_TFS, 4),
+                    hbm_gbs=round(gbs, 1), hbm_frac=round(gbs / HBM_PEAK_GBS, 4), flop_per_byte=round(intensity, 2))
+    return dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(gbs / HBM_PEAK_GBS, 4),
+                fp64_tfs=round(tfs, 3), flop_per_byte=round(intensity, 2))
+
+
+def cpu_cores() -> int:
+    try:
+        return len(os.sched_getaffinity(0))
+    except Exception:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(wl, nlev_sample: int, cores: int):
+    """The oracle's CPU chains (one chain per thread, dynamic schedule) on a bounded sample of the same workload."""
+    from oracle.oracle import OracleOp
+    os.environ.setdefault("OMP_PROC_BIND", "spread")
+    nch = min(len(wl.members), cores)
+    members = wl.members[:nch]
+    eps = np.stack([m.z * m.pa + m.pb for m in members])
+    O = OracleOp(wl.op)
+    t0 = time.perf_counter()
+    a, b, mu = O.edc_batch(members, eps, [m.l0 for m in members], [nlev_sample] * nch, project=True)
+    dt = time.perf_counter() - t0
+    return dict(value=round(float(np.sum(mu)) / dt, 3), unit="chain-levels/s", cores=cores, kind="port",
+                sample=f"{nch} chains x {nlev_sample} levels of the same workload ({wl.op.nn} nodes), {dt:.1f} s"), (a, b, mu)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--quick", action="store_true", help="small sizes (smoke)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-kpm", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--kpm-moments", type=int, default=2048)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: the engine's multi-GPU form (chains dealt across GPUs, one all-gather of the "
+              "coefficients) is not part of this build; run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from asr_amd import engine, workloads as W
+    if engine.device_count() < 1:
+        raise SystemExit("no GPU visible")
+    L, levels, depth = (4, 2, 30) if args.quick else (8, 4, 30)
+    t = time.perf_counter()
+    wl = W.ternary_edc(L=L, levels=levels, depth=depth)
+    log(f"workload: {wl.name}, {wl.op.nn} nodes, {len(wl.members)} chains x {depth}, build {time.perf_counter() - t:.1f} s")
+    G = engine.GpuOperator(wl.op)
+    nb = len(wl.members)
+    chain_levels = int(np.sum(wl.nlev))
+
+    for _ in range(args.warmup):
+        G.chains_edc(wl.members, wl.nlev)
+    dev_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        a, b, mu, ms = G.chains_edc(wl.members, wl.nlev)
+        dev_ms.append(ms)
+    wall = time.perf_counter() - t0
+    ms_per_step = wall / args.steps * 1e3
+    value = chain_levels / (wall / args.steps)
+    log(f"batched: {ms_per_step:.1f} ms/step (device {np.mean(dev_ms):.1f} ms), {value:.0f} chain-levels/s")
+
+    # per-kernel durations of the level step (HIP events on the engine's stream)
+    kms, kinds = G.profile(wl.members, wl.nlev)
+    per_kernel = []
+    for q, (t_ms, kd) in enumerate(zip(kms, kinds)):
+        by, fl = chain_kernel_cost(wl.op, nb, int(kd))
+        per_kernel.append(dict(launch=q, kind=int(kd), ms=round(float(t_ms), 4), gbs=round(by / (t_ms * 1e-3) / 1e9, 1),
+                               tfs=round(fl / (t_ms * 1e-3) / 1e12, 3)))
+    spmm = [k for k in per_kernel if k["kind"] == 0]
+    dom = max(spmm, key=lambda k: k["ms"]) if spmm else max(per_kernel, key=lambda k: k["ms"])
+    by, fl = chain_kernel_cost(wl.op, nb, dom["kind"])
+    roof = roofline(by, fl, dom["ms"])
+    roof["traffic"] = None
+    roof["kernel"] = "k_fused<cplx> (S pass + flip terms), launch %d of the level" % dom["launch"]
+    log("level kernels:", per_kernel)
+
+    out = {
+        "metric": "chain-levels/s (batched complex-symmetric ASR chains, 1 GPU)",
+        "value": round(value, 1),
+        "unit": "chain-levels/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded species parameters, screened fcc structure constants)",
+        "config": {"workload": "fcc ternary 0.66/0.14/0.20, route edc, levels 4, rank 1, D2h, 54 chains x 30",
+                   "nodes": wl.op.nn, "chains": nb, "depth": depth, "L": L, "levels": levels,
+                   "device_ms_per_step": round(float(np.mean(dev_ms)), 2)},
+        "roofline": roof,
+        "level_kernels": per_kernel,
+    }
+
+    if not args.no_plain:
+        t0 = time.perf_counter()
+        a1, b1, mu1, ms1 = G.chains_edc(wl.members, wl.nlev, mode=1)
+        wall1 = time.perf_counter() - t0
+        plain = chain_levels / wall1
+        out["plain"] = {"value": round(plain, 1), "unit": "chain-levels/s", "ms": round(wall1 * 1e3, 1),
+                        "form": "one chain per launch sequence, host sync per level"}
+        out["speedup_batched_vs_plain"] = round(value / plain, 2)
+        out["plain_matches_batched"] = bool(np.max(np.abs(a1 - a)) <= 1e-12 * np.max(np.abs(a)))
+        log(f"plain: {wall1 * 1e3:.0f} ms, {plain:.0f} chain-levels/s, batched/plain {value / plain:.2f}x")
+
+    if not args.no_kpm:
+        Lk = 6 if args.quick else 40
+        nmom = 64 if args.quick else args.kpm_moments
+        g, species, par, block, block0 = W.kpm_supercell(L=Lk)
+        # spectral bound of h = c + d S d (Gershgorin over the 9 x 9 blocks), scaled into [-0.99, 0.99]
+        sabs = np.abs(block).sum(axis=0).sum(axis=1) + np.abs(block0).sum(axis=1)
+        hb = float(np.max(np.abs(par[0]).max(axis=0) + par[1].max() * par[1].max() * sabs.max()))
+        cshift, sscale = 0.0, hb / 0.99
+        R = engine.GpuRealization(g.nbr, species, par, block, block0)
+        R.kpm_moments(cshift, sscale, 8, 64, seed=5)  # warm-up
+        t0 = time.perf_counter()
+        res = R.kpm_moments(cshift, sscale, nmom, 64, seed=5)
+        wk = time.perf_counter() - t0
+        kby, kfl = kpm_kernel_cost(g.n, g.nbr.shape[1], 64)
+        kroof = roofline(kby, kfl, res["step_ms"])
+        kroof["kernel"] = "k_kpm_step"
+        out["kpm"] = {"value": round(64 * nmom / wk, 1), "unit": "vector-moments/s", "rows": g.n * 9, "nvec": 64,
+                      "nmom": nmom, "wall_s": round(wk, 3), "step_ms": round(res["step_ms"], 4), "roofline": kroof,
+                      "mu_head": [round(float(v), 8) for v in res["mu"][:4]]}
+        log(f"kpm: {nmom} moments x 64 vectors on {g.n * 9} rows in {wk:.2f} s, step {res['step_ms']:.3f} ms, {kroof}")
+        R.close()
+
+    if not args.no_cpu:
+        cores = cpu_cores()
+        cb, _ = cpu_baseline(wl, 2 if args.quick else 8, cores)
+        out["cpu_baseline"] = cb
+        log(f"cpu baseline: {cb}")
+    G.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

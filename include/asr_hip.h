@@ -67,6 +67,15 @@ int asr_gpu_chains_edc(asr_gpu_op* op, int nchain, const int* start_orb, const d
                        const int* frozen_sp, const double* conc, const double* z, const int* nlev, int project, int ldab,
                        double* a, double* b, int* mused, int mode, double* elapsed_ms);
 
+/* Per-kernel durations of the chain level step (HIP events around every launch, averaged over the levels): the chains of
+ * asr_gpu_chains_edc (pab != NULL) or asr_gpu_lanczos_real (pab == NULL).  On return *nk (<= 16) launches per level,
+ * kernel_ms[q] their mean duration and kind[q] their kind: 0 fused SpMM (S pass + flip terms), 1 flip-gather terms only,
+ * 2 a_n reduction, 3 Lanczos update + projection, 4 b_n reduction, 5 scaling.  Measurement support; no reference
+ * counterpart. */
+int asr_gpu_profile_chains(asr_gpu_op* op, int nchain, const int* start_orb, const double* params, const double* pab,
+                           const int* frozen_sp, const double* conc, const double* z, const int* nlev, int project, int* nk,
+                           double* kernel_ms, int* kind);
+
 /* ---- KPM on a supercell realisation (lmto_real_op, src/reference/asr_lmto_realization.f90:14-23) */
 /* nbr: nsite x nslot neighbour sites (nslot <= 18), species: nsite, par: 4 x nsp x 9 (c, d, o, e; o = e = 0 required: the
  * kernel applies h = c + d S d), block: nslot x 9 x 9, block0: 9 x 9. */

@@ -84,7 +84,10 @@ def test_lanczos_real_batched_and_plain(binary_small):
         assert mu[j] == n
         assert rel(a[j, :n], ar[:n]) < 1e-9
         assert rel(b[j, :n], br[:n]) < 1e-9
-    assert np.array_equal(a[:3], a1) and np.array_equal(b[:3], b1), "plain and batched forms must agree bitwise"
+    # the plain form reduces over a different tile shape (one member per lane): agreement to rounding
+    assert rel(a1, a[:3]) < 1e-13 and rel(b1, b[:3]) < 1e-13
+    a2, b2, mu2, _ = G.lanczos(wl.members, wl.nlev, mode=0)
+    assert np.array_equal(a, a2) and np.array_equal(b, b2), "a batched run must be bitwise reproducible"
 
 
 def test_lanczos_uneven_depths(binary_small):
