@@ -98,6 +98,13 @@ __device__ inline void unpack_link(uint32_t e, int& r, int& g) {
   g = (int)(e & 63u);
 }
 
+// Workgroups are dealt round-robin over the 8 XCDs (each with its own L2): map the hardware block index so that every XCD
+// processes one contiguous range of logical blocks, which keeps neighbouring sites / nodes in the same L2.
+__device__ inline int xcd_block(int bid, int nblk) {
+  const int nx = 8, per = nblk / nx, rem = nblk % nx, x = bid % nx, idx = bid / nx;
+  return x < rem ? x * (per + 1) + idx : rem * (per + 1) + (x - rem) * per + idx;
+}
+
 template <class T>
 __device__ inline T ld(const T* x, int node, int l, int nbp, int m) {
   return x[((size_t)node * 9 + l) * nbp + m];
@@ -255,7 +262,7 @@ __global__ void __launch_bounds__(BLOCK) k_fused(OpDev op, MemDev md, FArgs fa) 
   const int mt = md.mt, npw = WAVE / mt;
   const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x >> 6;
   const int m = blockIdx.y * mt + (lane & (mt - 1));
-  const int i = (blockIdx.x * (BLOCK / WAVE) + wave) * npw + lane / mt;
+  const int i = (xcd_block(blockIdx.x, gridDim.x) * (BLOCK / WAVE) + wave) * npw + lane / mt;
   const bool live = (i < op.nn) && (m < md.nb);
   T dotv = zero<T>();
   if (live) {
@@ -325,22 +332,30 @@ __device__ inline cplx inv(cplx v) {
   return {v.re / d, -v.im / d};
 }
 
-// fixed-order sum of the per-block partials of every member
+// fixed-order sum of the per-block partials of member m by one block of BLOCK threads: strided per-thread sums, then a
+// shuffle tree per wave and the wave sums in order (deterministic for a given launch shape)
 template <class T>
-__device__ T sum_partials(const T* part, int nbx, int mt, int m) {
+__device__ T block_sum_partials(const T* part, int nbx, int mt, int m, T* lds) {
   const int tile = m / mt, ml = m % mt;
   T s = zero<T>();
-  for (int b = 0; b < nbx; ++b) s = s + part[((size_t)tile * nbx + b) * mt + ml];
-  return s;
+  for (int b = threadIdx.x; b < nbx; b += BLOCK) s = s + part[((size_t)tile * nbx + b) * mt + ml];
+  for (int off = WAVE / 2; off > 0; off >>= 1) s = s + shfl_xor(s, off);
+  const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x >> 6;
+  if (lane == 0) lds[wave] = s;
+  __syncthreads();
+  T tot = lds[0];
+  for (int w = 1; w < BLOCK / WAVE; ++w) tot = tot + lds[w];
+  return tot;
 }
 
-// a_k of every member; coefficient used by the update (0 for members that left)
+// a_k of every member (one block per member); coefficient used by the update (0 for members that left)
 template <class T>
-__global__ void k_fin_a(ChainDev cd, int nb, const T* part, int nbx, int mt, T* a, T* a_eff) {
-  int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= nb) return;
+__global__ void __launch_bounds__(BLOCK) k_fin_a(ChainDev cd, int nb, const T* part, int nbx, int mt, T* a, T* a_eff) {
+  __shared__ T lds[BLOCK / WAVE];
+  const int m = blockIdx.x;
+  T s = block_sum_partials(part, nbx, mt, m, lds);
+  if (threadIdx.x != 0) return;
   int k = *cd.level;
-  T s = sum_partials(part, nbx, mt, m);
   if (cd.active[m]) {
     a[(size_t)m * cd.ld + k] = s;
     cd.mused[m] = k + 1;
@@ -352,11 +367,13 @@ __global__ void k_fin_a(ChainDev cd, int nb, const T* part, int nbx, int mt, T* 
 
 // b_k = sqrt(sum w w); members leave at their depth or when b_k < tiny; next u = w * inv_b
 template <class T>
-__global__ void k_fin_b(ChainDev cd, int nb, const T* part, int nbx, int mt, T* b, T* b_eff, T* inv_b) {
-  int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= nb) return;
+__global__ void __launch_bounds__(BLOCK) k_fin_b(ChainDev cd, int nb, const T* part, int nbx, int mt, T* b, T* b_eff,
+                                                  T* inv_b) {
+  __shared__ T lds[BLOCK / WAVE];
+  const int m = blockIdx.x;
+  T s = block_sum_partials(part, nbx, mt, m, lds);
+  if (threadIdx.x != 0) return;
   int k = *cd.level;
-  T s = sum_partials(part, nbx, mt, m);
   T bk = dsqrt(s);
   int act = cd.active[m];
   if (act) {
@@ -379,7 +396,7 @@ __global__ void __launch_bounds__(BLOCK) k_update(OpDev op, MemDev md, T* w, con
   const int mt = md.mt, npw = WAVE / mt;
   const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x >> 6;
   const int m = blockIdx.y * mt + (lane & (mt - 1));
-  const int i = (blockIdx.x * (BLOCK / WAVE) + wave) * npw + lane / mt;
+  const int i = (xcd_block(blockIdx.x, gridDim.x) * (BLOCK / WAVE) + wave) * npw + lane / mt;
   T dv = zero<T>();
   if (i < op.nn && m < md.nb) {
     const T a = a_eff[m], b = b_prev[m];
@@ -443,9 +460,11 @@ __device__ inline double kpm_in(const double* x, int nvp, uint64_t seed, int sit
   return x ? x[((size_t)site * 9 + l) * nvp + v] : rademacher(seed, (uint64_t)v, (uint64_t)site * 9 + l);
 }
 
-// One Chebyshev step for all vectors: y = alpha (h x - c x) / s - beta t0  (h = c + d S d, order 2 with o = 0),
-// partial mu: sum_l r_l y_l.  Lanes = vectors (one wave per site), sites wave-uniform: neighbour index and the bond block
-// are scalar; the 19 blocks sit in LDS.  x == null / t0 == null: the Rademacher vector (generated, never stored).
+// One Chebyshev step for a tile of VT vectors: y = alpha (h x - c x) / s - beta t0  (h = c + d S d, order 2 with o = 0),
+// partial mu: sum_l r_l y_l.  Lanes = (site, vector) with VT vectors per site; a wave covers 64 / VT sites, their
+// neighbour indices and bond blocks are wave-uniform for VT = 64.  The 19 blocks sit in LDS.  x == null / t0 == null: the
+// Rademacher vector (generated, never stored).
+template <int VT>
 __global__ void __launch_bounds__(BLOCK) k_kpm_step(RealDev R, int nvec, int nvp, const double* __restrict__ x,
                                                     const double* __restrict__ t0, double* __restrict__ y, double cshift,
                                                     double alpha_over_s, double beta, uint64_t seed, double* partial,
@@ -455,9 +474,11 @@ __global__ void __launch_bounds__(BLOCK) k_kpm_step(RealDev R, int nvec, int nvp
   const int nb = (R.nslot + 1) * 81;
   for (int q = threadIdx.x; q < nb; q += BLOCK) blk[q] = R.blk[q];
   __syncthreads();
+  constexpr int SPW = WAVE / VT;  // sites per wave
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int v = blockIdx.y * 64 + lane;
-  const int site = blockIdx.x * (BLOCK / 64) + wave;
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int v = blockIdx.y * VT + (lane % VT);
+  const int site = (lb * (BLOCK / WAVE) + wave) * SPW + lane / VT;
   double dot = 0.0;
   if (site < R.nsite && v < nvec) {
     const int sp = R.spec[site];
@@ -499,25 +520,26 @@ __global__ void __launch_bounds__(BLOCK) k_kpm_step(RealDev R, int nvec, int nvp
     }
   }
   if (want_dot) {
-    // per (block, vector): sum over the 4 sites of the block in a fixed order
-    red[threadIdx.x] = dot;
+    // per (block, vector): the sites of the wave by a shuffle tree, then the 4 waves in order
+    for (int off = VT; off < WAVE; off <<= 1) dot += __shfl_xor(dot, off);
+    if (lane < VT) red[wave * VT + lane] = dot;
     __syncthreads();
-    if (threadIdx.x < 64) {
+    if (threadIdx.x < VT) {
       double s = red[threadIdx.x];
-      for (int w = 1; w < BLOCK / 64; ++w) s += red[w * 64 + threadIdx.x];
-      partial[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * 64 + threadIdx.x] = s;
+      for (int w = 1; w < BLOCK / WAVE; ++w) s += red[w * VT + threadIdx.x];
+      partial[((size_t)blockIdx.y * gridDim.x + lb) * VT + threadIdx.x] = s;
     }
   }
 }
 
-// mu[n] += (1/dim) sum_blocks partial (fixed order); one thread per vector, then per-vector values kept for the caller
-__global__ void k_kpm_fin(const double* partial, int nbx, int nvec, double inv_dim, double* mu_per, int nmom, int n) {
-  int v = blockIdx.x * blockDim.x + threadIdx.x;
+// mu_per[v][n] = (1/dim) sum_blocks partial, one block per vector, fixed order
+__global__ void __launch_bounds__(BLOCK) k_kpm_fin(const double* partial, int nbx, int vt, int nvec, double inv_dim,
+                                                   double* mu_per, int nmom, int n) {
+  __shared__ double lds[BLOCK / WAVE];
+  const int v = blockIdx.x;
   if (v >= nvec) return;
-  int tile = v / 64, vl = v % 64;
-  double s = 0.0;
-  for (int b = 0; b < nbx; ++b) s += partial[((size_t)tile * nbx + b) * 64 + vl];
-  mu_per[(size_t)v * nmom + n] = s * inv_dim;
+  double s = block_sum_partials<double>(partial, nbx, vt, v, lds);
+  if (threadIdx.x == 0) mu_per[(size_t)v * nmom + n] = s * inv_dim;
 }
 
 }  // namespace
@@ -532,10 +554,12 @@ struct asr_gpu_op {
   int *stab_ptr = nullptr, *stab = nullptr, *pi = nullptr, *nsc = nullptr;
   hipStream_t stream = nullptr;
   bool have_tables = false, have_ops = false, have_blocks = false, have_classes = false;
+  int tile_pref = 0;                              // member tile of the batched chains, 0 = autotuned
+  std::vector<std::pair<long long, int>> tuned;   // (members, complex, uniform d, zero o) key -> member tile
 };
 
 struct asr_gpu_real {
-  int nsite, nslot, nsp;
+  int nsite, nslot, nsp, vt = 0;
   int* nbr = nullptr;
   unsigned char* spec = nullptr;
   double *par = nullptr, *blk = nullptr;
@@ -590,10 +614,10 @@ int chi_mask(const asr_gpu_op* op, int l0, uint64_t* mask) {
 
 // params: nchain x 4 x nsp x 9 (c, d, o, e); pab: nchain x 2 x nsp x 9 or null; conc: nchain x ncls x nsmax or null
 int prepare_members(const asr_gpu_op* op, int nchain, const int* start_orb, const double* params, const double* pab,
-                    const int* frozen_sp, const double* conc, const double* z, Members& M) {
+                    const int* frozen_sp, const double* conc, const double* z, Members& M, int tile = 0) {
   const int nsm = op->nsmax, ncls = op->ncls, nsp = op->nsp;
   M.nb = nchain;
-  M.mt = pow2_tile(nchain);
+  M.mt = tile > 0 ? std::min(tile, pow2_tile(nchain)) : pow2_tile(nchain);
   M.nbp = ((nchain + M.mt - 1) / M.mt) * M.mt;
   const int nbp = M.nbp;
   M.chim.assign(nbp, 0);
@@ -835,6 +859,55 @@ int apply_chain_op(const asr_gpu_op* op, const Members& M, const MemDev& md, con
   return 0;
 }
 
+// The autotuner of the batched chains (the role of chain_teams / blk_nbmax, src/asr_chain.f90:140-155,
+// src/asr_augblk.f90:32-36): the member tile MT (members per wave, 64 / MT nodes per wave) sets the cache footprint of a
+// node (9 MT values) against the lane occupancy of ragged member counts.  Every candidate runs one operator application
+// of the level (same kernels, same members) between HIP events; the fastest is kept per (operator, member count, kind).
+template <class T>
+int autotune_tile(asr_gpu_op* op, int nchain, const int* start_orb, const double* params, const double* pab,
+                  const int* frozen_sp, const double* conc, const double* z, int* best) {
+  Members M0;
+  if (prepare_members(op, nchain, start_orb, params, pab, frozen_sp, conc, z, M0)) return 1;
+  long long key = ((long long)nchain << 8) | (sizeof(T) == 16 ? 1 : 0) | (M0.uniform_d ? 2 : 0) | (M0.zero_o ? 4 : 0) |
+                  (pab ? 8 : 0);
+  for (auto& kv : op->tuned)
+    if (kv.first == key) { *best = kv.second; return 0; }
+  const int top = pow2_tile(nchain);
+  double best_ms = 1e300;
+  int best_mt = top;
+  for (int mt = std::min(8, top); mt <= top; mt <<= 1) {
+    Members M;
+    if (prepare_members(op, nchain, start_orb, params, pab, frozen_sp, conc, z, M, mt)) return 1;
+    MemBuffers MB;
+    MemDev md;
+    if (upload_members(op, M, MB, md)) return 1;
+    const size_t nv = (size_t)op->nn * 9 * M.nbp;
+    T *x = nullptr, *y = nullptr, *t1 = nullptr, *t2 = nullptr, *h1 = nullptr;
+    if (dalloc(&x, nv) || dalloc(&y, nv) || dalloc(&t1, nv) || dalloc(&t2, nv) || dalloc(&h1, nv)) return 1;
+    int st = 0;
+    hipEvent_t e0, e1;
+    st |= hipEventCreate(&e0) != hipSuccess;
+    st |= hipEventCreate(&e1) != hipSuccess;
+    st |= hipMemsetAsync(x, 0, nv * sizeof(T), op->stream) != hipSuccess;
+    if (!st) st = apply_chain_op<T>(op, M, md, x, y, t1, t2, h1, pab != nullptr, nullptr, nullptr);
+    if (!st) st |= hipEventRecord(e0, op->stream) != hipSuccess;
+    for (int r = 0; r < 2 && !st; ++r) st = apply_chain_op<T>(op, M, md, x, y, t1, t2, h1, pab != nullptr, nullptr, nullptr);
+    if (!st) st |= hipEventRecord(e1, op->stream) != hipSuccess;
+    if (!st) st |= hipEventSynchronize(e1) != hipSuccess;
+    float ms = 0.f;
+    if (!st) st |= hipEventElapsedTime(&ms, e0, e1) != hipSuccess;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    for (T* p : {x, y, t1, t2, h1}) (void)hipFree(p);
+    if (st) return fail("autotune: a timing launch failed");
+    // a tile that pads many idle members is charged for them
+    if (ms < best_ms) { best_ms = ms; best_mt = mt; }
+  }
+  op->tuned.push_back({key, best_mt});
+  *best = best_mt;
+  return 0;
+}
+
 template <class T>
 struct ChainBuffers {
   T *u = nullptr, *up = nullptr, *w = nullptr, *t1 = nullptr, *t2 = nullptr, *h1 = nullptr, *part = nullptr;
@@ -868,8 +941,12 @@ int run_chains(asr_gpu_op* op, int nchain, const int* start_orb, const double* p
     }
     return 0;
   }
+  int mt = op->tile_pref;
+  if (mt == 0 && mode == 0) {
+    if (autotune_tile<T>(op, nchain, start_orb, params, pab, frozen_sp, conc, z, &mt)) return 1;
+  }
   Members M;
-  if (prepare_members(op, nchain, start_orb, params, pab, frozen_sp, conc, z, M)) return 1;
+  if (prepare_members(op, nchain, start_orb, params, pab, frozen_sp, conc, z, M, mt)) return 1;
   MemBuffers MB;
   MemDev md;
   if (upload_members(op, M, MB, md)) return 1;
@@ -920,19 +997,18 @@ int run_chains(asr_gpu_op* op, int nchain, const int* start_orb, const double* p
   HCHECK(hipEventCreate(&e1));
   HCHECK(hipStreamSynchronize(op->stream));
   HCHECK(hipEventRecord(e0, op->stream));
-  const int fin_threads = 64, fin_blocks = (M.nb + fin_threads - 1) / fin_threads;
   dim3 sgrid((unsigned)((nv + 255) / 256));
   for (int k = 0; k < maxdepth; ++k) {
     if (g_timer) g_timer->ord = 0;
     if (apply_chain_op<T>(op, M, md, B.u, B.w, B.t1, B.t2, B.h1, lin, B.u, B.part)) return 1;
     if (timer_mark(op->stream, KIND_FIN_A, true)) return 1;
-    hipLaunchKernelGGL(k_fin_a<T>, dim3(fin_blocks), dim3(fin_threads), 0, op->stream, cd, M.nb, B.part, nbx, md.mt, B.a,
+    hipLaunchKernelGGL(k_fin_a<T>, dim3(M.nb), dim3(BLOCK), 0, op->stream, cd, M.nb, B.part, nbx, md.mt, B.a,
                        B.a_eff);
     if (timer_mark(op->stream, 0, false) || timer_mark(op->stream, KIND_UPDATE, true)) return 1;
     hipLaunchKernelGGL(k_update<T>, grid, dim3(BLOCK), 0, op->stream, dev_of(op), md, B.w, B.u, B.up, B.a_eff, B.b_eff,
                        project, B.part);
     if (timer_mark(op->stream, 0, false) || timer_mark(op->stream, KIND_FIN_B, true)) return 1;
-    hipLaunchKernelGGL(k_fin_b<T>, dim3(fin_blocks), dim3(fin_threads), 0, op->stream, cd, M.nb, B.part, nbx, md.mt, B.b,
+    hipLaunchKernelGGL(k_fin_b<T>, dim3(M.nb), dim3(BLOCK), 0, op->stream, cd, M.nb, B.part, nbx, md.mt, B.b,
                        B.b_eff, B.inv_b);
     if (timer_mark(op->stream, 0, false) || timer_mark(op->stream, KIND_SCALE, true)) return 1;
     hipLaunchKernelGGL(k_scale<T>, sgrid, dim3(256), 0, op->stream, B.w, B.inv_b, nv, M.nbp);
@@ -1103,6 +1179,21 @@ int asr_gpu_op_set_classes(asr_gpu_op* op, const int* nsc, const int* spc, const
   return 0;
 }
 
+// Member tile of the batched chains: 1, 2, ..., 64 members per wave (0 = autotuned per member count, the default).
+int asr_gpu_op_set_tile(asr_gpu_op* op, int mt) {
+  if (mt < 0 || mt > 64 || (mt & (mt - 1)) != 0) return fail("asr_gpu_op_set_tile: 0 or a power of two <= 64");
+  op->tile_pref = mt;
+  return 0;
+}
+
+// The member tile the autotuner picked for nchain members (0 if no batched run of that size has happened).
+int asr_gpu_op_get_tile(asr_gpu_op* op, int nchain, int is_complex, int* mt) {
+  *mt = 0;
+  for (auto& kv : op->tuned)
+    if ((kv.first >> 8) == nchain && ((kv.first & 1) != 0) == (is_complex != 0)) *mt = kv.second;
+  return 0;
+}
+
 void asr_gpu_op_destroy(asr_gpu_op* op) {
   if (!op) return;
   for (void* p : {(void*)op->hop, (void*)op->flip, (void*)op->info, (void*)op->sq, (void*)op->sg, (void*)op->blk,
@@ -1259,19 +1350,48 @@ static RealDev real_dev(const asr_gpu_real* R) {
   return d;
 }
 
+static int kpm_vt(const asr_gpu_real* R, int nvec) {
+  int vt = R->vt > 0 ? R->vt : 64;
+  while (vt > 8 && vt / 2 >= nvec) vt /= 2;
+  return vt;
+}
+
+static int launch_kpm_step(const asr_gpu_real* R, int vt, dim3 grid, int nvec, int nvp, const double* x, const double* t0,
+                           double* y, double c, double a, double b, uint64_t seed, double* part, int want_dot) {
+  const RealDev rd = real_dev(R);
+  switch (vt) {
+    case 8: hipLaunchKernelGGL(k_kpm_step<8>, grid, dim3(BLOCK), 0, R->stream, rd, nvec, nvp, x, t0, y, c, a, b, seed, part, want_dot); break;
+    case 16: hipLaunchKernelGGL(k_kpm_step<16>, grid, dim3(BLOCK), 0, R->stream, rd, nvec, nvp, x, t0, y, c, a, b, seed, part, want_dot); break;
+    case 32: hipLaunchKernelGGL(k_kpm_step<32>, grid, dim3(BLOCK), 0, R->stream, rd, nvec, nvp, x, t0, y, c, a, b, seed, part, want_dot); break;
+    default: hipLaunchKernelGGL(k_kpm_step<64>, grid, dim3(BLOCK), 0, R->stream, rd, nvec, nvp, x, t0, y, c, a, b, seed, part, want_dot); break;
+  }
+  HCHECK(hipGetLastError());
+  return 0;
+}
+
+static dim3 kpm_grid(const asr_gpu_real* R, int vt, int nvp) {
+  const int sites_per_block = (BLOCK / WAVE) * (WAVE / vt);
+  return dim3((R->nsite + sites_per_block - 1) / sites_per_block, nvp / vt);
+}
+
+// Vector tile of the KPM step (vectors per site in a wave: 8, 16, 32 or 64; 0 = automatic).
+int asr_gpu_real_set_tile(asr_gpu_real* R, int vt) {
+  if (vt != 0 && vt != 8 && vt != 16 && vt != 32 && vt != 64) return fail("asr_gpu_real_set_tile: 0, 8, 16, 32 or 64");
+  R->vt = vt;
+  return 0;
+}
+
 // y = h x for nvec vectors (host layout [vector][9 nsite])
 int asr_gpu_real_apply(asr_gpu_real* R, int nvec, const double* x, double* y) {
   const size_t dim = (size_t)R->nsite * 9;
-  const int nvp = ((nvec + 63) / 64) * 64;
+  const int vt = kpm_vt(R, nvec);
+  const int nvp = ((nvec + vt - 1) / vt) * vt;
   std::vector<double> hx;
   to_dev_layout(x, hx, nvec, nvp, dim);
   double *dx, *dy;
   if (dalloc(&dx, dim * nvp) || dalloc(&dy, dim * nvp)) return 1;
   HCHECK(hipMemcpy(dx, hx.data(), dim * nvp * 8, hipMemcpyHostToDevice));
-  dim3 grid((R->nsite + 3) / 4, nvp / 64);
-  hipLaunchKernelGGL(k_kpm_step, grid, dim3(BLOCK), 0, R->stream, real_dev(R), nvec, nvp, dx, (const double*)nullptr, dy, 0.0,
-                     1.0, 0.0, 0ull, (double*)nullptr, 0);
-  HCHECK(hipGetLastError());
+  if (launch_kpm_step(R, vt, kpm_grid(R, vt, nvp), nvec, nvp, dx, nullptr, dy, 0.0, 1.0, 0.0, 0ull, nullptr, 0)) return 1;
   HCHECK(hipStreamSynchronize(R->stream));
   HCHECK(hipMemcpy(hx.data(), dy, dim * nvp * 8, hipMemcpyDeviceToHost));
   from_dev_layout(hx, y, nvec, nvp, dim);
@@ -1281,51 +1401,53 @@ int asr_gpu_real_apply(asr_gpu_real* R, int nvec, const double* x, double* y) {
 }
 
 // mu_n = (1/R) sum_r <r|T_n((H - c)/s)|r> / dim (kpm_moments, src/reference/asr_kpm.f90:39-65) for nvec Rademacher vectors
-// in one batched SpMM per moment.  mu_per (optional): nvec x nmom per-vector terms.  step_ms: average duration of the
-// Chebyshev step kernel (HIP events on the engine's stream); total_ms: the whole recurrence.
+// in one batched SpMM per moment.  mu_per (optional): nvec x nmom per-vector terms.  step_ms: mean duration of the
+// Chebyshev step kernel alone (HIP events around each launch on the engine's stream); total_ms: the whole recurrence.
 int asr_gpu_kpm_moments(asr_gpu_real* R, double c, double s, int nmom, int nvec, uint64_t seed, double* mu, double* mu_per,
                         double* step_ms, double* total_ms) {
   if (nmom < 1 || nvec < 1) return fail("kpm_moments: nmom and nvec must be positive");
   if (seed >= (1ull << 24) || nvec > 4096) return fail("kpm_moments: seed < 2^24 and nvec <= 4096");
   const size_t dim = (size_t)R->nsite * 9;
   if (dim >= (1ull << 28)) return fail("kpm_moments: more than 2^28 rows");
-  const int nvp = ((nvec + 63) / 64) * 64;
-  dim3 grid((R->nsite + 3) / 4, nvp / 64);
+  const int vt = kpm_vt(R, nvec);
+  const int nvp = ((nvec + vt - 1) / vt) * vt;
+  const dim3 grid = kpm_grid(R, vt, nvp);
   double *t0, *t1, *t2, *part, *mper;
   if (dalloc(&t0, dim * nvp) || dalloc(&t1, dim * nvp) || dalloc(&t2, dim * nvp) ||
-      dalloc(&part, (size_t)grid.x * grid.y * 64) || dalloc(&mper, (size_t)nvec * nmom))
+      dalloc(&part, (size_t)grid.x * grid.y * vt) || dalloc(&mper, (size_t)nvec * nmom))
     return 1;
   const double inv_dim = 1.0 / (double)dim;
-  hipEvent_t e0, e1, s0, s1;
-  HCHECK(hipEventCreate(&e0)); HCHECK(hipEventCreate(&e1)); HCHECK(hipEventCreate(&s0)); HCHECK(hipEventCreate(&s1));
-  HCHECK(hipEventRecord(e0, R->stream));
-  const RealDev rd = real_dev(R);
-  const int fb = (nvec + 63) / 64;
-  // mu_0 = <r|r>/dim: the step kernel with alpha = 0, beta = -1 gives y = r
-  hipLaunchKernelGGL(k_kpm_step, grid, dim3(BLOCK), 0, R->stream, rd, nvec, nvp, (const double*)nullptr,
-                     (const double*)nullptr, t0, 0.0, 0.0, -1.0, seed, part, 1);
-  hipLaunchKernelGGL(k_kpm_fin, dim3(fb), dim3(64), 0, R->stream, part, (int)grid.x, nvec, inv_dim, mper, nmom, 0);
+  const int nstep_ev = nmom > 2 ? nmom - 2 : 0;
+  std::vector<hipEvent_t> ev(2 + 2 * nstep_ev);
+  for (auto& e : ev) HCHECK(hipEventCreate(&e));
+  HCHECK(hipEventRecord(ev[0], R->stream));
+  // mu_0 = <r|r>/dim: the step with alpha = 0, beta = -1 gives y = r
+  if (launch_kpm_step(R, vt, grid, nvec, nvp, nullptr, nullptr, t0, 0.0, 0.0, -1.0, seed, part, 1)) return 1;
+  hipLaunchKernelGGL(k_kpm_fin, dim3(nvec), dim3(BLOCK), 0, R->stream, part, (int)grid.x, vt, nvec, inv_dim, mper, nmom, 0);
   if (nmom > 1) {
     // t1 = (H r - c r) / s
-    hipLaunchKernelGGL(k_kpm_step, grid, dim3(BLOCK), 0, R->stream, rd, nvec, nvp, (const double*)nullptr,
-                       (const double*)nullptr, t1, c, 1.0 / s, 0.0, seed, part, 1);
-    hipLaunchKernelGGL(k_kpm_fin, dim3(fb), dim3(64), 0, R->stream, part, (int)grid.x, nvec, inv_dim, mper, nmom, 1);
+    if (launch_kpm_step(R, vt, grid, nvec, nvp, nullptr, nullptr, t1, c, 1.0 / s, 0.0, seed, part, 1)) return 1;
+    hipLaunchKernelGGL(k_kpm_fin, dim3(nvec), dim3(BLOCK), 0, R->stream, part, (int)grid.x, vt, nvec, inv_dim, mper, nmom, 1);
   }
-  HCHECK(hipEventRecord(s0, R->stream));
   for (int n = 2; n < nmom; ++n) {
     // t2 = 2 (H t1 - c t1) / s - t0
-    hipLaunchKernelGGL(k_kpm_step, grid, dim3(BLOCK), 0, R->stream, rd, nvec, nvp, (const double*)t1, (const double*)t0, t2,
-                       c, 2.0 / s, 1.0, seed, part, 1);
-    hipLaunchKernelGGL(k_kpm_fin, dim3(fb), dim3(64), 0, R->stream, part, (int)grid.x, nvec, inv_dim, mper, nmom, n);
+    HCHECK(hipEventRecord(ev[2 + 2 * (n - 2)], R->stream));
+    if (launch_kpm_step(R, vt, grid, nvec, nvp, t1, t0, t2, c, 2.0 / s, 1.0, seed, part, 1)) return 1;
+    HCHECK(hipEventRecord(ev[3 + 2 * (n - 2)], R->stream));
+    hipLaunchKernelGGL(k_kpm_fin, dim3(nvec), dim3(BLOCK), 0, R->stream, part, (int)grid.x, vt, nvec, inv_dim, mper, nmom, n);
     double* tmp = t0; t0 = t1; t1 = t2; t2 = tmp;
   }
-  HCHECK(hipEventRecord(s1, R->stream));
-  HCHECK(hipEventRecord(e1, R->stream));
+  HCHECK(hipEventRecord(ev[1], R->stream));
   HCHECK(hipGetLastError());
-  HCHECK(hipEventSynchronize(e1));
-  float ms_all = 0.f, ms_steps = 0.f;
-  HCHECK(hipEventElapsedTime(&ms_all, e0, e1));
-  HCHECK(hipEventElapsedTime(&ms_steps, s0, s1));
+  HCHECK(hipEventSynchronize(ev[1]));
+  float ms_all = 0.f;
+  HCHECK(hipEventElapsedTime(&ms_all, ev[0], ev[1]));
+  double ms_steps = 0.0;
+  for (int q = 0; q < nstep_ev; ++q) {
+    float t = 0.f;
+    HCHECK(hipEventElapsedTime(&t, ev[2 + 2 * q], ev[3 + 2 * q]));
+    ms_steps += t;
+  }
   std::vector<double> per((size_t)nvec * nmom);
   HCHECK(hipMemcpy(per.data(), mper, per.size() * 8, hipMemcpyDeviceToHost));
   for (int n = 0; n < nmom; ++n) {
@@ -1334,10 +1456,10 @@ int asr_gpu_kpm_moments(asr_gpu_real* R, double c, double s, int nmom, int nvec,
     mu[n] = acc / (double)nvec;
   }
   if (mu_per) std::memcpy(mu_per, per.data(), per.size() * 8);
-  if (step_ms) *step_ms = nmom > 2 ? ms_steps / (nmom - 2) : 0.0;
+  if (step_ms) *step_ms = nstep_ev ? ms_steps / nstep_ev : 0.0;
   if (total_ms) *total_ms = ms_all;
   for (double* p : {t0, t1, t2, part, mper}) (void)hipFree(p);
-  for (hipEvent_t e : {e0, e1, s0, s1}) (void)hipEventDestroy(e);
+  for (hipEvent_t e : ev) (void)hipEventDestroy(e);
   return 0;
 }
 

@@ -37,6 +37,9 @@ def lib() -> C.CDLL:
         L.asr_gpu_op_set_orbital_ops.argtypes = [P, P, P]
         L.asr_gpu_op_set_blocks.argtypes = [P, P, P]
         L.asr_gpu_op_set_classes.argtypes = [P, P, P, P]
+        L.asr_gpu_op_set_tile.argtypes = [P, C.c_int]
+        L.asr_gpu_op_get_tile.argtypes = [P, C.c_int, C.c_int, P]
+        L.asr_gpu_real_set_tile.argtypes = [P, C.c_int]
         L.asr_gpu_op_destroy.argtypes = [P]
         L.asr_gpu_op_destroy.restype = None
         L.asr_gpu_apply_batched.argtypes = [P, C.c_int, P, P, P, P, P, P]
@@ -128,6 +131,15 @@ class GpuOperator:
         except Exception:
             pass
 
+    def set_tile(self, mt: int):
+        """Member tile of the batched chains (power of two <= 64; 0 = autotuned)."""
+        _check(lib().asr_gpu_op_set_tile(self.h, int(mt)), "asr_gpu_op_set_tile")
+
+    def tuned_tile(self, nchain: int, is_complex: bool) -> int:
+        mt = np.zeros(1, np.int32)
+        _check(lib().asr_gpu_op_get_tile(self.h, int(nchain), int(is_complex), _p(mt)), "asr_gpu_op_get_tile")
+        return int(mt[0])
+
     def apply(self, members: list[Member], x: np.ndarray) -> np.ndarray:
         """y = H x for every member (x: nb x 9 nn, real)."""
         x = _f64(np.atleast_2d(x))
@@ -214,6 +226,9 @@ class GpuRealization:
             self.close()
         except Exception:
             pass
+
+    def set_tile(self, vt: int):
+        _check(lib().asr_gpu_real_set_tile(self.h, int(vt)), "asr_gpu_real_set_tile")
 
     def apply(self, x: np.ndarray) -> np.ndarray:
         x = _f64(np.atleast_2d(x))

@@ -20,7 +20,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import structure as S
-from .augcls import AugclsOperator, Member, build_operator
+from .augcls import AugclsOperator, Member, build_operator, reorder_nodes
 
 ORB_L = S.ORB_L
 
@@ -68,7 +68,8 @@ class Workload:
     info: dict
 
 
-def plain_binary(L: int = 8, levels: int = 4, depth: int = 20, seed: int = 1, group: str = "d2h") -> Workload:
+def plain_binary(L: int = 8, levels: int = 4, depth: int = 20, seed: int = 1, group: str = "d2h",
+                 reorder: bool = True) -> Workload:
     rng = np.random.default_rng(seed)
     g, block, block0 = graph_and_blocks("fcc", L)
     cls = np.zeros(g.n, np.int32)
@@ -76,6 +77,8 @@ def plain_binary(L: int = 8, levels: int = 4, depth: int = 20, seed: int = 1, gr
     dmat = S.orbital_rep(mats)
     op = build_operator(g, cls, [2], [[0, 1]], [[0.5, 0.5]], perm, dmat, 0, levels, maxrank=1,
                         block=block[None], block0=block0[None], nsp=2)
+    if reorder:
+        op = reorder_nodes(op)
     c, d = draw_cd(rng, 2)
     z = np.zeros((2, 9))
     members = [Member(l0=l0, c=c, d=d, o=z, e=z) for l0 in range(9)]
@@ -110,7 +113,7 @@ def edc_member(pp: dict, central: int, l0: int, z: complex) -> Member:
 
 
 def ternary_edc(L: int = 8, levels: int = 4, depth: int = 30, seed: int = 2, group: str = "d2h",
-                z: complex = complex(-0.2, 0.15), conc=(0.66, 0.14, 0.20)) -> Workload:
+                z: complex = complex(-0.2, 0.15), conc=(0.66, 0.14, 0.20), reorder: bool = True) -> Workload:
     rng = np.random.default_rng(seed)
     g, block, block0 = graph_and_blocks("fcc", L)
     cls = np.zeros(g.n, np.int32)
@@ -120,16 +123,38 @@ def ternary_edc(L: int = 8, levels: int = 4, depth: int = 30, seed: int = 2, gro
     fl[0] = 0                                    # the start site carries the central species (never flips)
     op = build_operator(g, cls, [3], [[0, 1, 2]], [list(conc)], perm, dmat, 0, levels, maxrank=1, flippable=fl,
                         block=block[None], block0=block0[None], nsp=3)
+    if reorder:
+        op = reorder_nodes(op)
     spins = edc_params(rng, 3, 2)
     members = [edc_member(spins[sp], cen, l0, z) for sp in range(2) for cen in range(3) for l0 in range(9)]
     return Workload("fcc ternary edc", op, members, np.full(len(members), depth, np.int32), True,
                     dict(L=L, levels=levels, group=group, seed=seed, nodes=op.nn, z=z))
 
 
-def kpm_supercell(L: int = 40, seed: int = 4, conc=(0.66, 0.14, 0.20)):
-    """Random ternary fcc supercell: (graph, species per site, par (4, nsp, 9), block, block0)."""
+def brick_order(g: S.Graph, brick: int = 4) -> S.Graph:
+    """Renumber the sites of a periodic cubic supercell brick by brick (brick^3 conventional cells, bricks and cells in
+    raster order): the neighbours of a brick then sit in a few nearby bricks, which keeps a sweep over the sites inside
+    the L2 of one XCD.  The operator is the same up to the relabelling."""
+    cell = g.coord // 2
+    L = int(cell.max()) + 1
+    bk = cell // brick
+    nb = (L + brick - 1) // brick
+    sub = (g.coord % 2) @ np.array([4, 2, 1])
+    key = (((bk[:, 2] * nb + bk[:, 1]) * nb + bk[:, 0]) * brick ** 3 + ((cell[:, 2] % brick) * brick + cell[:, 1] % brick)
+           * brick + cell[:, 0] % brick) * 8 + sub
+    perm = np.argsort(key, kind="stable")            # new index -> old index
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(perm.size)
+    nbr = inv[g.nbr[perm]].astype(np.int32)
+    return S.Graph(n=g.n, nbr=nbr, coord=g.coord[perm], bvec=g.bvec)
+
+
+def kpm_supercell(L: int = 40, seed: int = 4, conc=(0.66, 0.14, 0.20), brick: int = 4):
+    """Random ternary fcc supercell: (graph, species per site, par (4, nsp, 9), block, block0); sites in brick order."""
     rng = np.random.default_rng(seed)
     g, block, block0 = graph_and_blocks("fcc", L)
+    if brick and L % brick == 0:
+        g = brick_order(g, brick)
     species = rng.choice(len(conc), size=g.n, p=np.asarray(conc)).astype(np.uint8)
     c, d = draw_cd(rng, len(conc))
     par = np.zeros((4, len(conc), 9))

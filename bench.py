@@ -137,6 +137,8 @@ def main():
     value = chain_levels / (wall / args.steps)
     log(f"batched: {ms_per_step:.1f} ms/step (device {np.mean(dev_ms):.1f} ms), {value:.0f} chain-levels/s")
 
+    out_tile = G.tuned_tile(nb, True)
+    log(f"autotuned member tile: {out_tile}")
     # per-kernel durations of the level step (HIP events on the engine's stream)
     kms, kinds = G.profile(wl.members, wl.nlev)
     per_kernel = []
@@ -167,7 +169,8 @@ def main():
         "data": "synthetic (seeded species parameters, screened fcc structure constants)",
         "config": {"workload": "fcc ternary 0.66/0.14/0.20, route edc, levels 4, rank 1, D2h, 54 chains x 30",
                    "nodes": wl.op.nn, "chains": nb, "depth": depth, "L": L, "levels": levels,
-                   "device_ms_per_step": round(float(np.mean(dev_ms)), 2)},
+                   "device_ms_per_step": round(float(np.mean(dev_ms)), 2), "member_tile": out_tile,
+                   "node_order": wl.op.meta.get("node_order", "bfs")},
         "roofline": roof,
         "level_kernels": per_kernel,
     }
@@ -192,7 +195,13 @@ def main():
         hb = float(np.max(np.abs(par[0]).max(axis=0) + par[1].max() * par[1].max() * sabs.max()))
         cshift, sscale = 0.0, hb / 0.99
         R = engine.GpuRealization(g.nbr, species, par, block, block0)
-        R.kpm_moments(cshift, sscale, 8, 64, seed=5)  # warm-up
+        tiles = {}
+        for vt in (16, 32, 64):  # vector-tile autotuning on short recurrences
+            R.set_tile(vt)
+            tiles[vt] = R.kpm_moments(cshift, sscale, 12, 64, seed=5)["step_ms"]
+        vt_best = min(tiles, key=tiles.get)
+        R.set_tile(vt_best)
+        log(f"kpm vector tiles (ms/step): {tiles} -> {vt_best}")
         t0 = time.perf_counter()
         res = R.kpm_moments(cshift, sscale, nmom, 64, seed=5)
         wk = time.perf_counter() - t0
@@ -201,6 +210,7 @@ def main():
         kroof["kernel"] = "k_kpm_step"
         out["kpm"] = {"value": round(64 * nmom / wk, 1), "unit": "vector-moments/s", "rows": g.n * 9, "nvec": 64,
                       "nmom": nmom, "wall_s": round(wk, 3), "step_ms": round(res["step_ms"], 4), "roofline": kroof,
+                      "vector_tile": vt_best,
                       "mu_head": [round(float(v), 8) for v in res["mu"][:4]]}
         log(f"kpm: {nmom} moments x 64 vectors on {g.n * 9} rows in {wk:.2f} s, step {res['step_ms']:.3f} ms, {kroof}")
         R.close()

@@ -104,3 +104,20 @@ def test_oracle_mutation_is_detected():
     z = complex(0.0, 0.3)
     g = host.terminated_green(a, b, n, 0.0, 0.0, z)[0]
     assert abs(g - exact_average_g(model, par, z, 4, 1)) > 1.0e-6
+
+
+def test_node_reordering_keeps_the_chains():
+    """The locality relabelling of the node set changes no chain coefficient beyond rounding (oracle on both orders)."""
+    from asr_amd import workloads as W
+    w0 = W.ternary_edc(L=4, levels=2, depth=10, reorder=False)
+    w1 = W.ternary_edc(L=4, levels=2, depth=10, reorder=True)
+    assert w1.op.meta.get("node_order") == "rcm" and not np.array_equal(w0.op.hopr, w1.op.hopr)
+    O0, O1 = OracleOp(w0.op), OracleOp(w1.op)
+    for k in (0, 17, 40):
+        m = w0.members[k]
+        ep = m.z * m.pa + m.pb
+        a0, b0, n0 = O0.edc_chain(m, ep, m.l0, 10)
+        a1, b1, n1 = O1.edc_chain(m, ep, m.l0, 10)
+        assert n0 == n1
+        assert np.max(np.abs(a1 - a0)) < 1e-12 * np.max(np.abs(a0))
+        assert np.max(np.abs(b1 - b0)) < 1e-12 * np.max(np.abs(b0))
