@@ -91,6 +91,8 @@ struct MemDev {
   const int* frozen;          // nbp: 1 = the start site is frozen
   const double* dvec;         // 9 x nbp: d per orbital (uniform-d members)
   const cplx* z;              // nbp
+  const double* dperm;        // nop x 9 x nbp: d at the permuted orbital, d[pi(l, g)]
+  int pair;                   // complex members as lane pairs (member tile 64 only)
 };
 
 __device__ inline void unpack_link(uint32_t e, int& r, int& g) {
@@ -298,6 +300,239 @@ __global__ void __launch_bounds__(BLOCK) k_fused(OpDev op, MemDev md, FArgs fa) 
   if (fa.dotu) {
     T* part = (T*)fa.partial;
     block_reduce_member<T>(dotv, mt, red, part + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * mt + threadIdx.x);
+  }
+}
+
+// The same fused operator pass with one node per wave and 64 members per wave (member tile 64): the node's links, weights
+// table offsets, the orbital tables of each link's operation and the bond blocks are wave-uniform and come through the
+// scalar cache; only the member amplitudes are per lane, each gather a uniform row address plus the lane's offset.
+// NSLOT > 0: the slot loop is unrolled at compile time (a missing link points at the all-zero padding node nn, so the
+// loop has no branch and the loads of later slots can be issued early).  PIID: every operation of the group maps each
+// orbital onto itself (pi = identity, e.g. D2h), so the member's d enters at the orbital index itself.
+template <class T, bool PIID, int NSLOT>
+__global__ void __launch_bounds__(BLOCK, 4) k_node(OpDev op, MemDev md, FArgs fa, const double* __restrict__ blk,
+                                                   const double* __restrict__ dperm) {
+  __shared__ T red[BLOCK];
+  const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x >> 6;
+  const int i = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * (BLOCK / WAVE) + wave);
+  const int m = blockIdx.y * WAVE + lane;
+  const bool mlive = m < md.nb;
+  const int nbp = md.nbp;
+  T dotv = zero<T>();
+  if (i < op.nn) {
+    const uint32_t inf = op.info[i];
+    const uint64_t chim = md.chimask[m];
+    T acc[9];
+    for (int l = 0; l < 9; ++l) acc[l] = zero<T>();
+    if (fa.xs) {
+      const T* __restrict__ x = (const T*)fa.xs;
+      const int t = (inf >> 12) & 255u;
+      const double* __restrict__ b0 = blk + (size_t)t * (op.nslot + 1) * 81;
+      {
+        const double sc = fa.xs_norm ? 1.0 / op.sq[i] : 1.0;
+        const T* __restrict__ row = x + (size_t)i * 9 * nbp;
+        T xn[9];
+        for (int l = 0; l < 9; ++l) xn[l] = (sc * (fa.s_din ? md.dvec[(size_t)l * nbp + m] : 1.0)) * row[(size_t)l * nbp + m];
+        for (int l = 0; l < 9; ++l) {
+          T a2 = zero<T>();
+          for (int l2 = 0; l2 < 9; ++l2) a2 = fma_s(b0[l * 9 + l2], xn[l2], a2);
+          acc[l] = a2;
+        }
+      }
+      const uint32_t* __restrict__ hp = op.hop + (size_t)i * op.nslot;
+      const int nslot = NSLOT > 0 ? NSLOT : op.nslot;
+#pragma unroll 2
+      for (int kk = 0; kk < nslot; ++kk) {
+        {
+          const uint32_t e = hp[kk];
+          const int r = (int)(e >> 6) - 1, g = (int)(e & 63u);
+          const double chi = ((chim >> g) & 1ull) ? -1.0 : 1.0;
+          const double sc = fa.xs_norm ? 1.0 / op.sq[r] : 1.0;
+          const T* __restrict__ row = x + (size_t)r * 9 * nbp;
+          T v[9];
+          for (int l = 0; l < 9; ++l) {
+            const int pl = PIID ? l : op.pi[g * 9 + l];
+            const double dd = fa.s_din ? (PIID ? md.dvec[(size_t)l * nbp + m] : dperm[((size_t)g * 9 + l) * nbp + m]) : 1.0;
+            v[l] = (chi * op.sg[g * 9 + l] * sc * dd) * row[(size_t)pl * nbp + m];
+          }
+          const double* __restrict__ bk = b0 + (size_t)(kk + 1) * 81;
+          for (int l = 0; l < 9; ++l) {
+            T a2 = acc[l];
+            for (int l2 = 0; l2 < 9; ++l2) a2 = fma_s(bk[l * 9 + l2], v[l2], a2);
+            acc[l] = a2;
+          }
+        }
+      }
+      for (int l = 0; l < 9; ++l) acc[l] = (fa.s_coef * (fa.s_dout ? md.dvec[(size_t)l * nbp + m] : 1.0)) * acc[l];
+    }
+    for (int t = 0; t < fa.nr; ++t)
+      rand_accum<T>(op, md, op.pi, op.sg, chim, i, inf, m, fa.qr[t], (const T*)fa.xr[t], fa.r_norm[t], fa.r_coef[t], acc);
+    if (fa.xl) lin_accum(op, md, op.pi, op.sg, chim, i, inf, m, (const T*)fa.xl, fa.l_norm, fa.l_coef, acc);
+    for (int t = 0; t < fa.no; ++t) {
+      const T* xo = (const T*)fa.xo[t];
+      for (int l = 0; l < 9; ++l) acc[l] = fma_s(fa.o_coef[t], ld(xo, i, l, nbp, m), acc[l]);
+    }
+    const double so = fa.out_sq ? op.sq[i] : 1.0;
+    T* out = (T*)fa.out;
+    const T* du = (const T*)fa.dotu;
+    for (int l = 0; l < 9; ++l) {
+      T y = so * acc[l];
+      if (mlive) {
+        out[((size_t)i * 9 + l) * nbp + m] = y;
+        if (du) dotv = dotv + bilin(ld(du, i, l, nbp, m), y);
+      }
+    }
+  }
+  if (fa.dotu) {
+    T* part = (T*)fa.partial;
+    block_reduce_member<T>(dotv, WAVE, red, part + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * WAVE + threadIdx.x);
+  }
+}
+
+// Complex members as pairs of real lanes (lane 2j: Re of member j, lane 2j + 1: Im): everything the real operator does
+// acts on each part alone, so the pass runs in FP64 on half the registers of a complex lane; only X(z) = z pa + pb and the
+// bilinear dot sum(u w) exchange the partner's part (lane ^ 1).  32 complex members and one node per wave.
+__device__ void rand_accum_pair(const OpDev& op, const MemDev& md, uint64_t chim, int i, uint32_t inf, int m, int vi, int vs,
+                                int q, const double* __restrict__ x, int xnorm, double coef, double acc[9]) {
+  const int nbp = md.nbp, nsm = op.nsmax;
+  const bool fz = (inf >> 20) & 1u;
+  const double sqi = op.sq[i];
+  if (fz && md.frozen[m]) {
+    const double* fp = md.fpar + (size_t)q * 9 * nbp;
+    for (int l = 0; l < 9; ++l) {
+      double xv = x[((size_t)i * 9 + l) * vs + vi];
+      if (xnorm) xv = xv / sqi;
+      acc[l] = fma(coef * fp[(size_t)l * nbp + m], xv, acc[l]);
+    }
+    return;
+  }
+  const int l0 = inf & 15u, kap = (inf >> 4) & 255u;
+  const double* w = md.wts + (((size_t)q * op.ncls + kap) * nsm + l0) * nsm * 9 * nbp;
+  double r_acc[9];
+  for (int l = 0; l < 9; ++l) {
+    double xv = x[((size_t)i * 9 + l) * vs + vi];
+    if (xnorm) xv = xv / sqi;
+    r_acc[l] = w[((size_t)l0 * 9 + l) * nbp + m] * xv;
+  }
+  const int nsc = op.nsc[kap];
+  const uint32_t* fl = op.flip + (size_t)i * nsm;
+  for (int k = 0; k < nsc; ++k) {
+    const uint32_t e = fl[k];
+    if (k == l0 || e == 0u) continue;
+    int r, g;
+    unpack_link(e, r, g);
+    const double chi = ((chim >> g) & 1ull) ? -1.0 : 1.0;
+    const double sqr = op.sq[r];
+    for (int l = 0; l < 9; ++l) {
+      double xv = x[((size_t)r * 9 + op.pi[g * 9 + l]) * vs + vi];
+      if (xnorm) xv = xv / sqr;
+      r_acc[l] = fma(w[((size_t)k * 9 + l) * nbp + m], (chi * op.sg[g * 9 + l]) * xv, r_acc[l]);
+    }
+  }
+  for (int l = 0; l < 9; ++l) acc[l] = fma(coef, r_acc[l], acc[l]);
+}
+
+template <bool PIID, int NSLOT>
+__global__ void __launch_bounds__(BLOCK, 4) k_pair(OpDev op, MemDev md, FArgs fa, const double* __restrict__ blk,
+                                                   const double* __restrict__ dperm) {
+  __shared__ cplx red[BLOCK / WAVE][WAVE / 2];
+  const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x >> 6;
+  const int i = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * (BLOCK / WAVE) + wave);
+  const int part = lane & 1;
+  const int m = blockIdx.y * (WAVE / 2) + (lane >> 1);
+  const int vs = 2 * md.nbp, vi = 2 * m + part;
+  const bool mlive = m < md.nb;
+  const int nbp = md.nbp;
+  double cre = 0.0, cim = 0.0;
+  if (i < op.nn) {
+    const uint32_t inf = op.info[i];
+    const uint64_t chim = md.chimask[m];
+    double acc[9];
+    for (int l = 0; l < 9; ++l) acc[l] = 0.0;
+    if (fa.xs) {
+      const double* __restrict__ x = (const double*)fa.xs;
+      const int t = (inf >> 12) & 255u;
+      const double* __restrict__ b0 = blk + (size_t)t * (op.nslot + 1) * 81;
+      {
+        const double sc = fa.xs_norm ? 1.0 / op.sq[i] : 1.0;
+        const double* __restrict__ row = x + (size_t)i * 9 * vs;
+        double xn[9];
+        for (int l = 0; l < 9; ++l) xn[l] = (sc * (fa.s_din ? md.dvec[(size_t)l * nbp + m] : 1.0)) * row[(size_t)l * vs + vi];
+        for (int l = 0; l < 9; ++l) {
+          double a2 = 0.0;
+          for (int l2 = 0; l2 < 9; ++l2) a2 = fma(b0[l * 9 + l2], xn[l2], a2);
+          acc[l] = a2;
+        }
+      }
+      const uint32_t* __restrict__ hp = op.hop + (size_t)i * op.nslot;
+      const int nslot = NSLOT > 0 ? NSLOT : op.nslot;
+#pragma unroll 2
+      for (int kk = 0; kk < nslot; ++kk) {
+        {
+          const uint32_t e = hp[kk];
+          const int r = (int)(e >> 6) - 1, g = (int)(e & 63u);
+          const double chi = ((chim >> g) & 1ull) ? -1.0 : 1.0;
+          const double sc = fa.xs_norm ? 1.0 / op.sq[r] : 1.0;
+          const double* __restrict__ row = x + (size_t)r * 9 * vs;
+          double v[9];
+          for (int l = 0; l < 9; ++l) {
+            const int pl = PIID ? l : op.pi[g * 9 + l];
+            const double dd = fa.s_din ? (PIID ? md.dvec[(size_t)l * nbp + m] : dperm[((size_t)g * 9 + l) * nbp + m]) : 1.0;
+            v[l] = (chi * op.sg[g * 9 + l] * sc * dd) * row[(size_t)pl * vs + vi];
+          }
+          const double* __restrict__ bk = b0 + (size_t)(kk + 1) * 81;
+          for (int l = 0; l < 9; ++l) {
+            double a2 = acc[l];
+            for (int l2 = 0; l2 < 9; ++l2) a2 = fma(bk[l * 9 + l2], v[l2], a2);
+            acc[l] = a2;
+          }
+        }
+      }
+      for (int l = 0; l < 9; ++l) acc[l] *= fa.s_coef * (fa.s_dout ? md.dvec[(size_t)l * nbp + m] : 1.0);
+    }
+    for (int t = 0; t < fa.nr; ++t)
+      rand_accum_pair(op, md, chim, i, inf, m, vi, vs, fa.qr[t], (const double*)fa.xr[t], fa.r_norm[t], fa.r_coef[t], acc);
+    if (fa.xl) {
+      double ta[9], tb[9];
+      for (int l = 0; l < 9; ++l) { ta[l] = 0.0; tb[l] = 0.0; }
+      rand_accum_pair(op, md, chim, i, inf, m, vi, vs, QPA, (const double*)fa.xl, fa.l_norm, 1.0, ta);
+      rand_accum_pair(op, md, chim, i, inf, m, vi, vs, QPB, (const double*)fa.xl, fa.l_norm, 1.0, tb);
+      const cplx z = md.z[m];
+      const double sgn = part ? 1.0 : -1.0;
+      for (int l = 0; l < 9; ++l) {
+        const double oth = __shfl_xor(ta[l], 1);
+        acc[l] = fma(fa.l_coef, fma(z.re, ta[l], fma(sgn * z.im, oth, tb[l])), acc[l]);
+      }
+    }
+    for (int t = 0; t < fa.no; ++t) {
+      const double* xo = (const double*)fa.xo[t];
+      for (int l = 0; l < 9; ++l) acc[l] = fma(fa.o_coef[t], xo[((size_t)i * 9 + l) * vs + vi], acc[l]);
+    }
+    const double so = fa.out_sq ? op.sq[i] : 1.0;
+    double* out = (double*)fa.out;
+    const double* du = (const double*)fa.dotu;
+    for (int l = 0; l < 9; ++l) {
+      const double y = so * acc[l];
+      if (mlive) out[((size_t)i * 9 + l) * vs + vi] = y;
+      if (du) {
+        const double yo = __shfl_xor(y, 1);
+        const double u = mlive ? du[((size_t)i * 9 + l) * vs + vi] : 0.0;
+        const double p1 = u * y;
+        cre += part ? -p1 : p1;
+        cim = fma(u, yo, cim);
+      }
+    }
+  }
+  if (fa.dotu) {
+    cre += __shfl_xor(cre, 1);
+    cim += __shfl_xor(cim, 1);
+    if (!part) red[wave][lane >> 1] = cplx{cre, cim};
+    __syncthreads();
+    if (threadIdx.x < WAVE / 2) {
+      cplx s2 = red[0][threadIdx.x];
+      for (int w = 1; w < BLOCK / WAVE; ++w) s2 = s2 + red[w][threadIdx.x];
+      ((cplx*)fa.partial)[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * (WAVE / 2) + threadIdx.x] = s2;
+    }
   }
 }
 
@@ -532,6 +767,162 @@ __global__ void __launch_bounds__(BLOCK) k_kpm_step(RealDev R, int nvec, int nvp
   }
 }
 
+// Chebyshev step on d-scaled vectors, one site per wave (wave-uniform site: its neighbour list, species and the bond blocks
+// are read through the scalar cache), lanes = 64 vectors.  Stored vectors are u_n = d o t_n, so the gathered neighbour
+// values enter the bond blocks directly (h t = c t + d S (d t) needs d o t at the neighbours):
+//     t2 = alpha (c t1 + d (B0 u1_i + sum_k B_k u1_j) - cshift t1) - beta t0,   t1 = u1 / d,  u2 = d o t2,
+// partial mu: sum_l r_l t2_l.  GEN: t1 = r (the Rademacher vector, generated) for the first steps.  The neighbour loads
+// of slot k + 1 are issued before the block of slot k is applied.
+template <bool GEN, int NSLOT>
+__global__ void __launch_bounds__(BLOCK) k_kpm_u(RealDev R, const double* __restrict__ blk, int nvec, int nvp,
+                                                 const double* __restrict__ u1, const double* __restrict__ u0,
+                                                 double* __restrict__ u2, double cshift, double alpha, double beta,
+                                                 uint64_t seed, double* partial, int want_dot) {
+  __shared__ double red[BLOCK];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int site = __builtin_amdgcn_readfirstlane(lb * (BLOCK / WAVE) + wave);
+  const int v = blockIdx.y * 64 + lane;
+  double dot = 0.0;
+  if (site < R.nsite) {
+    const int sp = R.spec[site];
+    const double* cp = R.par + (size_t)sp * 9;
+    const double* dp = R.par + (size_t)(R.nsp + sp) * 9;
+    const int nslot = NSLOT > 0 ? NSLOT : R.nslot;
+    double acc[9], xo[9];
+    {
+      const double* __restrict__ row = u1 + (size_t)site * 9 * nvp;
+      for (int l = 0; l < 9; ++l)
+        xo[l] = GEN ? dp[l] * rademacher(seed, (uint64_t)v, (uint64_t)site * 9 + l) : row[(size_t)l * nvp + v];
+    }
+    for (int l = 0; l < 9; ++l) {
+      double a = 0.0;
+      for (int l2 = 0; l2 < 9; ++l2) a = fma(blk[l * 9 + l2], xo[l2], a);
+      acc[l] = a;
+    }
+    const int* __restrict__ nbl = R.nbr + (size_t)site * R.nslot;
+#pragma unroll
+    for (int k = 0; k < (NSLOT > 0 ? NSLOT : 1); ++k) {
+      for (int kk = (NSLOT > 0 ? k : 0); kk < (NSLOT > 0 ? k + 1 : nslot); ++kk) {
+        const int j = nbl[kk];
+        double xn[9];
+        if (GEN) {
+          const double* dj = R.par + (size_t)(R.nsp + R.spec[j]) * 9;
+          for (int l = 0; l < 9; ++l) xn[l] = dj[l] * rademacher(seed, (uint64_t)v, (uint64_t)j * 9 + l);
+        } else {
+          const double* __restrict__ row = u1 + (size_t)j * 9 * nvp;
+          for (int l = 0; l < 9; ++l) xn[l] = row[(size_t)l * nvp + v];
+        }
+        const double* __restrict__ bk = blk + (size_t)(kk + 1) * 81;
+        for (int l = 0; l < 9; ++l) {
+          double a = acc[l];
+          for (int l2 = 0; l2 < 9; ++l2) a = fma(bk[l * 9 + l2], xn[l2], a);
+          acc[l] = a;
+        }
+      }
+    }
+    if (v < nvec) {
+      for (int l = 0; l < 9; ++l) {
+        const uint64_t row = (uint64_t)site * 9 + l;
+        const double r = rademacher(seed, (uint64_t)v, row);
+        const double t1 = GEN ? r : xo[l] / dp[l];
+        const double hx = cp[l] * t1 + dp[l] * acc[l];
+        double t2 = alpha * (hx - cshift * t1);
+        if (beta != 0.0) t2 -= beta * (u0 ? u0[row * nvp + v] / dp[l] : r);
+        u2[row * nvp + v] = dp[l] * t2;
+        if (want_dot) dot = fma(r, t2, dot);
+      }
+    }
+  }
+  if (want_dot) {
+    red[threadIdx.x] = dot;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+      double s2 = red[threadIdx.x];
+      for (int w = 1; w < BLOCK / WAVE; ++w) s2 += red[w * 64 + threadIdx.x];
+      partial[((size_t)blockIdx.y * gridDim.x + lb) * 64 + threadIdx.x] = s2;
+    }
+  }
+}
+
+// The same step with two vectors per lane (16-byte loads: half the L1 requests of one vector per lane) and two sites per
+// wave (32 lanes x 2 vectors = the 64-vector tile of one site).
+struct d2 {
+  double x, y;
+};
+template <int NSLOT>
+__global__ void __launch_bounds__(BLOCK) k_kpm_v2(RealDev R, const double* __restrict__ blk, int nvec, int nvp,
+                                                  const double* __restrict__ u1, const double* __restrict__ u0,
+                                                  double* __restrict__ u2, double cshift, double alpha, double beta,
+                                                  uint64_t seed, double* partial) {
+  __shared__ d2 red[BLOCK / WAVE][32];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, half = lane >> 5, vl = lane & 31;
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int site = (lb * (BLOCK / WAVE) + wave) * 2 + half;
+  const int v = blockIdx.y * 64 + 2 * vl;
+  d2 dot = {0.0, 0.0};
+  if (site < R.nsite) {
+    const int sp = R.spec[site];
+    const double* cp = R.par + (size_t)sp * 9;
+    const double* dp = R.par + (size_t)(R.nsp + sp) * 9;
+    const int nslot = NSLOT > 0 ? NSLOT : R.nslot;
+    d2 acc[9], xo[9];
+    {
+      const d2* __restrict__ row = (const d2*)(u1 + (size_t)site * 9 * nvp + v);
+      for (int l = 0; l < 9; ++l) xo[l] = row[(size_t)l * (nvp / 2)];
+    }
+    for (int l = 0; l < 9; ++l) {
+      d2 a = {0.0, 0.0};
+      for (int l2 = 0; l2 < 9; ++l2) {
+        a.x = fma(blk[l * 9 + l2], xo[l2].x, a.x);
+        a.y = fma(blk[l * 9 + l2], xo[l2].y, a.y);
+      }
+      acc[l] = a;
+    }
+    const int* __restrict__ nbl = R.nbr + (size_t)site * R.nslot;
+#pragma unroll 2
+    for (int kk = 0; kk < nslot; ++kk) {
+      const int j = nbl[kk];
+      const d2* __restrict__ row = (const d2*)(u1 + (size_t)j * 9 * nvp + v);
+      d2 xn[9];
+      for (int l = 0; l < 9; ++l) xn[l] = row[(size_t)l * (nvp / 2)];
+      const double* __restrict__ bk = blk + (size_t)(kk + 1) * 81;
+      for (int l = 0; l < 9; ++l) {
+        d2 a = acc[l];
+        for (int l2 = 0; l2 < 9; ++l2) {
+          a.x = fma(bk[l * 9 + l2], xn[l2].x, a.x);
+          a.y = fma(bk[l * 9 + l2], xn[l2].y, a.y);
+        }
+        acc[l] = a;
+      }
+    }
+    for (int l = 0; l < 9; ++l) {
+      const uint64_t row = (uint64_t)site * 9 + l;
+      const double r0 = rademacher(seed, (uint64_t)v, row), r1 = rademacher(seed, (uint64_t)v + 1, row);
+      const d2 t0v = ((const d2*)(u0 + row * nvp + v))[0];
+      const double t1x = xo[l].x / dp[l], t1y = xo[l].y / dp[l];
+      double t2x = alpha * (cp[l] * t1x + dp[l] * acc[l].x - cshift * t1x) - beta * (t0v.x / dp[l]);
+      double t2y = alpha * (cp[l] * t1y + dp[l] * acc[l].y - cshift * t1y) - beta * (t0v.y / dp[l]);
+      if (v < nvec) u2[row * nvp + v] = dp[l] * t2x;
+      if (v + 1 < nvec) u2[row * nvp + v + 1] = dp[l] * t2y;
+      dot.x = fma(r0, t2x, dot.x);
+      dot.y = fma(r1, t2y, dot.y);
+    }
+  }
+  // per (block, vector): the two sites of the wave, then the 4 waves in order
+  dot.x += __shfl_xor(dot.x, 32);
+  dot.y += __shfl_xor(dot.y, 32);
+  if (half == 0) red[wave][vl] = dot;
+  __syncthreads();
+  if (threadIdx.x < 32) {
+    d2 s2 = red[0][threadIdx.x];
+    for (int w = 1; w < BLOCK / WAVE; ++w) { s2.x += red[w][threadIdx.x].x; s2.y += red[w][threadIdx.x].y; }
+    double* pp = partial + ((size_t)blockIdx.y * gridDim.x + lb) * 64 + 2 * threadIdx.x;
+    pp[0] = s2.x;
+    pp[1] = s2.y;
+  }
+}
+
 // mu_per[v][n] = (1/dim) sum_blocks partial, one block per vector, fixed order
 __global__ void __launch_bounds__(BLOCK) k_kpm_fin(const double* partial, int nbx, int vt, int nvec, double inv_dim,
                                                    double* mu_per, int nmom, int n) {
@@ -554,12 +945,14 @@ struct asr_gpu_op {
   int *stab_ptr = nullptr, *stab = nullptr, *pi = nullptr, *nsc = nullptr;
   hipStream_t stream = nullptr;
   bool have_tables = false, have_ops = false, have_blocks = false, have_classes = false;
+  bool piid = false;                              // every operation maps each orbital onto itself
   int tile_pref = 0;                              // member tile of the batched chains, 0 = autotuned
   std::vector<std::pair<long long, int>> tuned;   // (members, complex, uniform d, zero o) key -> member tile
 };
 
 struct asr_gpu_real {
   int nsite, nslot, nsp, vt = 0;
+  bool v2 = true;  // Chebyshev steps with two vectors per lane
   int* nbr = nullptr;
   unsigned char* spec = nullptr;
   double *par = nullptr, *blk = nullptr;
@@ -593,10 +986,10 @@ int pow2_tile(int nb) {
 
 // Per-member host preparation: characters, local weights, frozen rows, uniform d.
 struct Members {
-  int nb = 0, nbp = 0, mt = 1;
+  int nb = 0, nbp = 0, mt = 1, pair = 0;
   bool uniform_d = true, zero_o = true, zero_e = true, complex_lin = false;
   std::vector<uint64_t> chim;
-  std::vector<double> wts, fpar, dvec;
+  std::vector<double> wts, fpar, dvec, dperm;
   std::vector<int> frozen;
   std::vector<cplx> z;
 };
@@ -617,13 +1010,17 @@ int prepare_members(const asr_gpu_op* op, int nchain, const int* start_orb, cons
                     const int* frozen_sp, const double* conc, const double* z, Members& M, int tile = 0) {
   const int nsm = op->nsmax, ncls = op->ncls, nsp = op->nsp;
   M.nb = nchain;
+  M.pair = (tile == 65) ? 1 : 0;  // 65: member tile 64 with complex members as lane pairs
+  if (tile == 65) tile = 64;
   M.mt = tile > 0 ? std::min(tile, pow2_tile(nchain)) : pow2_tile(nchain);
+  if (M.mt != WAVE) M.pair = 0;
   M.nbp = ((nchain + M.mt - 1) / M.mt) * M.mt;
   const int nbp = M.nbp;
   M.chim.assign(nbp, 0);
   M.wts.assign((size_t)NQ * ncls * nsm * nsm * 9 * nbp, 0.0);
   M.fpar.assign((size_t)NQ * 9 * nbp, 0.0);
   M.dvec.assign((size_t)9 * nbp, 0.0);
+  M.dperm.assign((size_t)op->nop * 9 * nbp, 0.0);
   M.frozen.assign(nbp, 0);
   M.z.assign(nbp, cplx{0.0, 0.0});
   M.complex_lin = (pab != nullptr);
@@ -699,6 +1096,8 @@ int prepare_members(const asr_gpu_op* op, int nchain, const int* start_orb, cons
       for (int l = 0; l < 9; ++l)
         if (dp[s * 9 + l] != dp[l]) M.uniform_d = false;
     for (int l = 0; l < 9; ++l) M.dvec[(size_t)l * nbp + m] = dp[l];
+    for (int g = 0; g < op->nop; ++g)
+      for (int l = 0; l < 9; ++l) M.dperm[((size_t)g * 9 + l) * nbp + m] = dp[op->pi_h[g * 9 + l]];
     for (int q = 0; q < nsp * 9; ++q) {
       if (pm[2 * nsp * 9 + q] != 0.0) M.zero_o = false;
       if (pm[3 * nsp * 9 + q] != 0.0) M.zero_e = false;
@@ -710,10 +1109,11 @@ int prepare_members(const asr_gpu_op* op, int nchain, const int* start_orb, cons
 
 struct MemBuffers {
   uint64_t* chim = nullptr;
-  double *wts = nullptr, *fpar = nullptr, *dvec = nullptr;
+  double *wts = nullptr, *fpar = nullptr, *dvec = nullptr, *dperm = nullptr;
   int* frozen = nullptr;
   cplx* z = nullptr;
   ~MemBuffers() {
+    if (dperm) (void)hipFree(dperm);
     if (chim) (void)hipFree(chim);
     if (wts) (void)hipFree(wts);
     if (fpar) (void)hipFree(fpar);
@@ -725,16 +1125,19 @@ struct MemBuffers {
 
 int upload_members(const asr_gpu_op* op, const Members& M, MemBuffers& B, MemDev& md) {
   if (dalloc(&B.chim, M.chim.size()) || dalloc(&B.wts, M.wts.size()) || dalloc(&B.fpar, M.fpar.size()) ||
-      dalloc(&B.dvec, M.dvec.size()) || dalloc(&B.frozen, M.frozen.size()) || dalloc(&B.z, M.z.size()))
+      dalloc(&B.dvec, M.dvec.size()) || dalloc(&B.frozen, M.frozen.size()) || dalloc(&B.z, M.z.size()) ||
+      dalloc(&B.dperm, M.dperm.size()))
     return 1;
+  HCHECK(hipMemcpyAsync(B.dperm, M.dperm.data(), M.dperm.size() * 8, hipMemcpyHostToDevice, op->stream));
   HCHECK(hipMemcpyAsync(B.chim, M.chim.data(), M.chim.size() * 8, hipMemcpyHostToDevice, op->stream));
   HCHECK(hipMemcpyAsync(B.wts, M.wts.data(), M.wts.size() * 8, hipMemcpyHostToDevice, op->stream));
   HCHECK(hipMemcpyAsync(B.fpar, M.fpar.data(), M.fpar.size() * 8, hipMemcpyHostToDevice, op->stream));
   HCHECK(hipMemcpyAsync(B.dvec, M.dvec.data(), M.dvec.size() * 8, hipMemcpyHostToDevice, op->stream));
   HCHECK(hipMemcpyAsync(B.frozen, M.frozen.data(), M.frozen.size() * 4, hipMemcpyHostToDevice, op->stream));
   HCHECK(hipMemcpyAsync(B.z, M.z.data(), M.z.size() * sizeof(cplx), hipMemcpyHostToDevice, op->stream));
-  md.nb = M.nb; md.nbp = M.nbp; md.mt = M.mt;
+  md.nb = M.nb; md.nbp = M.nbp; md.mt = M.mt; md.pair = M.pair;
   md.chimask = B.chim; md.wts = B.wts; md.fpar = B.fpar; md.frozen = B.frozen; md.dvec = B.dvec; md.z = B.z;
+  md.dperm = B.dperm;
   return 0;
 }
 
@@ -790,7 +1193,28 @@ int launch_fused(const asr_gpu_op* op, const MemDev& md, const FArgs& fa) {
   dim3 grid = node_grid(op, md);
   size_t lds = fused_lds_bytes(op, sizeof(T));
   if (timer_mark(op->stream, fa.xs ? KIND_SPMM : KIND_RAND, true)) return 1;
-  hipLaunchKernelGGL(k_fused<T>, grid, dim3(BLOCK), lds, op->stream, dev_of(op), md, fa);
+  const OpDev od = dev_of(op);
+  const double* bl = op->blk;
+  if (md.mt == WAVE && md.pair && sizeof(T) == sizeof(cplx)) {
+    grid.y = md.nbp / (WAVE / 2);
+#define ASR_PAIR(PI, NS) hipLaunchKernelGGL((k_pair<PI, NS>), grid, dim3(BLOCK), 0, op->stream, od, md, fa, bl, md.dperm)
+    if (op->piid && op->nslot == 18) ASR_PAIR(true, 18);
+    else if (op->piid && op->nslot == 14) ASR_PAIR(true, 14);
+    else if (op->piid) ASR_PAIR(true, 0);
+    else if (op->nslot == 18) ASR_PAIR(false, 18);
+    else ASR_PAIR(false, 0);
+#undef ASR_PAIR
+  } else if (md.mt == WAVE) {
+#define ASR_NODE(PI, NS) hipLaunchKernelGGL((k_node<T, PI, NS>), grid, dim3(BLOCK), 0, op->stream, od, md, fa, bl, md.dperm)
+    if (op->piid && op->nslot == 18) ASR_NODE(true, 18);
+    else if (op->piid && op->nslot == 14) ASR_NODE(true, 14);
+    else if (op->piid) ASR_NODE(true, 0);
+    else if (op->nslot == 18) ASR_NODE(false, 18);
+    else ASR_NODE(false, 0);
+#undef ASR_NODE
+  } else {
+    hipLaunchKernelGGL(k_fused<T>, grid, dim3(BLOCK), lds, op->stream, dev_of(op), md, fa);
+  }
   HCHECK(hipGetLastError());
   if (timer_mark(op->stream, 0, false)) return 1;
   return 0;
@@ -875,20 +1299,23 @@ int autotune_tile(asr_gpu_op* op, int nchain, const int* start_orb, const double
   const int top = pow2_tile(nchain);
   double best_ms = 1e300;
   int best_mt = top;
-  for (int mt = std::min(8, top); mt <= top; mt <<= 1) {
+  std::vector<int> cands;
+  for (int mt = std::min(8, top); mt <= top; mt <<= 1) cands.push_back(mt);
+  if (top == WAVE && sizeof(T) == sizeof(cplx)) cands.push_back(65);
+  for (int mt : cands) {
     Members M;
     if (prepare_members(op, nchain, start_orb, params, pab, frozen_sp, conc, z, M, mt)) return 1;
     MemBuffers MB;
     MemDev md;
     if (upload_members(op, M, MB, md)) return 1;
-    const size_t nv = (size_t)op->nn * 9 * M.nbp;
+    const size_t nv = (size_t)(op->nn + 1) * 9 * M.nbp;  // + the zero padding node
     T *x = nullptr, *y = nullptr, *t1 = nullptr, *t2 = nullptr, *h1 = nullptr;
     if (dalloc(&x, nv) || dalloc(&y, nv) || dalloc(&t1, nv) || dalloc(&t2, nv) || dalloc(&h1, nv)) return 1;
     int st = 0;
     hipEvent_t e0, e1;
     st |= hipEventCreate(&e0) != hipSuccess;
     st |= hipEventCreate(&e1) != hipSuccess;
-    st |= hipMemsetAsync(x, 0, nv * sizeof(T), op->stream) != hipSuccess;
+    for (T* p : {x, y, t1, t2, h1}) st |= hipMemsetAsync(p, 0, nv * sizeof(T), op->stream) != hipSuccess;
     if (!st) st = apply_chain_op<T>(op, M, md, x, y, t1, t2, h1, pab != nullptr, nullptr, nullptr);
     if (!st) st |= hipEventRecord(e0, op->stream) != hipSuccess;
     for (int r = 0; r < 2 && !st; ++r) st = apply_chain_op<T>(op, M, md, x, y, t1, t2, h1, pab != nullptr, nullptr, nullptr);
@@ -953,11 +1380,13 @@ int run_chains(asr_gpu_op* op, int nchain, const int* start_orb, const double* p
   int maxdepth = 0;
   for (int c = 0; c < nchain; ++c) maxdepth = std::max(maxdepth, nlev[c]);
   if (maxdepth > ldab) return fail("chains: depth larger than the leading dimension of a, b");
-  const size_t nv = (size_t)op->nn * 9 * M.nbp;
+  const size_t nv = (size_t)(op->nn + 1) * 9 * M.nbp;  // + the zero padding node (target of missing links)
   ChainBuffers<T> B;
   dim3 grid = node_grid(op, md);
   const int nbx = grid.x;
   const size_t npart = (size_t)grid.x * grid.y * md.mt;
+  // the dot partials of the apply pass: complex members with tile 64 run as lane pairs, 32 members per wave
+  const int apply_mt = (md.mt == WAVE && md.pair && sizeof(T) == sizeof(cplx)) ? WAVE / 2 : md.mt;
   if (dalloc(&B.u, nv) || dalloc(&B.up, nv) || dalloc(&B.w, nv) || dalloc(&B.t1, nv) || dalloc(&B.t2, nv) ||
       dalloc(&B.h1, nv) || dalloc(&B.part, npart) || dalloc(&B.a, (size_t)M.nbp * ldab) ||
       dalloc(&B.b, (size_t)M.nbp * ldab) || dalloc(&B.a_eff, M.nbp) || dalloc(&B.b_eff, M.nbp) ||
@@ -972,6 +1401,9 @@ int run_chains(asr_gpu_op* op, int nchain, const int* start_orb, const double* p
   HCHECK(hipMemsetAsync(B.a_eff, 0, M.nbp * sizeof(T), op->stream));
   HCHECK(hipMemsetAsync(B.inv_b, 0, M.nbp * sizeof(T), op->stream));
   HCHECK(hipMemsetAsync(B.w, 0, nv * sizeof(T), op->stream));
+  HCHECK(hipMemsetAsync(B.t1, 0, nv * sizeof(T), op->stream));
+  HCHECK(hipMemsetAsync(B.t2, 0, nv * sizeof(T), op->stream));
+  HCHECK(hipMemsetAsync(B.h1, 0, nv * sizeof(T), op->stream));
   HCHECK(hipMemsetAsync(B.level, 0, sizeof(int), op->stream));
   HCHECK(hipMemsetAsync(B.mused, 0, M.nbp * sizeof(int), op->stream));
   std::vector<int> dep(M.nbp, 0), act(M.nbp, 0);
@@ -1002,7 +1434,7 @@ int run_chains(asr_gpu_op* op, int nchain, const int* start_orb, const double* p
     if (g_timer) g_timer->ord = 0;
     if (apply_chain_op<T>(op, M, md, B.u, B.w, B.t1, B.t2, B.h1, lin, B.u, B.part)) return 1;
     if (timer_mark(op->stream, KIND_FIN_A, true)) return 1;
-    hipLaunchKernelGGL(k_fin_a<T>, dim3(M.nb), dim3(BLOCK), 0, op->stream, cd, M.nb, B.part, nbx, md.mt, B.a,
+    hipLaunchKernelGGL(k_fin_a<T>, dim3(M.nb), dim3(BLOCK), 0, op->stream, cd, M.nb, B.part, nbx, apply_mt, B.a,
                        B.a_eff);
     if (timer_mark(op->stream, 0, false) || timer_mark(op->stream, KIND_UPDATE, true)) return 1;
     hipLaunchKernelGGL(k_update<T>, grid, dim3(BLOCK), 0, op->stream, dev_of(op), md, B.w, B.u, B.up, B.a_eff, B.b_eff,
@@ -1087,7 +1519,7 @@ int asr_gpu_op_create(int nn, int nslot, int nop, int nsmax, int ncls, int nsp, 
   if (nn <= 0 || nslot < 0 || nop <= 0 || nop > 64 || nsmax <= 0 || nsmax > 16 || ncls <= 0 || ncls > 255 || nsp <= 0 ||
       ntype <= 0 || ntype > 255)
     return fail("asr_gpu_op_create: dimensions out of range (nop <= 64, nsmax <= 16, ncls, ntype <= 255)");
-  if ((size_t)nn >= (1u << 26)) return fail("asr_gpu_op_create: more than 2^26 nodes");
+  if ((size_t)nn + 1 >= (1u << 26)) return fail("asr_gpu_op_create: 2^26 nodes or more");
   auto* op = new asr_gpu_op();
   op->nn = nn; op->nslot = nslot; op->nop = nop; op->nsmax = nsmax; op->ncls = ncls; op->nsp = nsp; op->ntype = ntype;
   op->nstab = 0;
@@ -1104,7 +1536,8 @@ int asr_gpu_op_set_tables(asr_gpu_op* op, const int* hopr, const int* hopg, cons
   std::vector<uint32_t> hop((size_t)nn * ns), flip((size_t)nn * nsm), info(nn);
   for (size_t q = 0; q < hop.size(); ++q) {
     if (hopr[q] >= nn || (hopr[q] >= 0 && (hopg[q] < 0 || hopg[q] >= op->nop))) return fail("set_tables: hop link out of range");
-    hop[q] = hopr[q] < 0 ? 0u : ((uint32_t)(hopr[q] + 1) << 6) | (uint32_t)hopg[q];
+    // a missing link points at the padding node nn, whose amplitudes are zero in every vector
+    hop[q] = hopr[q] < 0 ? ((uint32_t)(nn + 1) << 6) : ((uint32_t)(hopr[q] + 1) << 6) | (uint32_t)hopg[q];
   }
   for (size_t q = 0; q < flip.size(); ++q) {
     if (flipr[q] >= nn || (flipr[q] >= 0 && (flipg[q] < 0 || flipg[q] >= op->nop)))
@@ -1122,7 +1555,7 @@ int asr_gpu_op_set_tables(asr_gpu_op* op, const int* hopr, const int* hopg, cons
   for (int j = 0; j < nstab; ++j)
     if (stab[j] < 0 || stab[j] >= op->nop) return fail("set_tables: stabiliser operation out of range");
   if (!op->hop && (dalloc(&op->hop, hop.size()) || dalloc(&op->flip, flip.size()) || dalloc(&op->info, nn) ||
-                   dalloc(&op->sq, nn) || dalloc(&op->stab_ptr, nn + 1)))
+                   dalloc(&op->sq, nn + 1) || dalloc(&op->stab_ptr, nn + 1)))
     return 1;
   if (op->stab) (void)hipFree(op->stab);
   op->stab = nullptr;
@@ -1132,6 +1565,10 @@ int asr_gpu_op_set_tables(asr_gpu_op* op, const int* hopr, const int* hopg, cons
   HCHECK(hipMemcpy(op->flip, flip.data(), flip.size() * 4, hipMemcpyHostToDevice));
   HCHECK(hipMemcpy(op->info, info.data(), info.size() * 4, hipMemcpyHostToDevice));
   HCHECK(hipMemcpy(op->sq, sq, (size_t)nn * 8, hipMemcpyHostToDevice));
+  {
+    const double one = 1.0;
+    HCHECK(hipMemcpy(op->sq + nn, &one, 8, hipMemcpyHostToDevice));
+  }
   HCHECK(hipMemcpy(op->stab_ptr, stab_ptr, (size_t)(nn + 1) * 4, hipMemcpyHostToDevice));
   if (nstab > 0) HCHECK(hipMemcpy(op->stab, stab, (size_t)nstab * 4, hipMemcpyHostToDevice));
   op->have_tables = true;
@@ -1143,6 +1580,9 @@ int asr_gpu_op_set_orbital_ops(asr_gpu_op* op, const int* pi, const double* sg) 
     if (pi[q] < 0 || pi[q] > 8) return fail("set_orbital_ops: orbital index out of range");
   op->pi_h.assign(pi, pi + op->nop * 9);
   op->sg_h.assign(sg, sg + op->nop * 9);
+  op->piid = true;
+  for (int q = 0; q < op->nop * 9; ++q)
+    if (pi[q] != q % 9) op->piid = false;
   if (!op->pi && (dalloc(&op->pi, op->nop * 9) || dalloc(&op->sg, op->nop * 9))) return 1;
   HCHECK(hipMemcpy(op->pi, pi, (size_t)op->nop * 9 * 4, hipMemcpyHostToDevice));
   HCHECK(hipMemcpy(op->sg, sg, (size_t)op->nop * 9 * 8, hipMemcpyHostToDevice));
@@ -1181,7 +1621,8 @@ int asr_gpu_op_set_classes(asr_gpu_op* op, const int* nsc, const int* spc, const
 
 // Member tile of the batched chains: 1, 2, ..., 64 members per wave (0 = autotuned per member count, the default).
 int asr_gpu_op_set_tile(asr_gpu_op* op, int mt) {
-  if (mt < 0 || mt > 64 || (mt & (mt - 1)) != 0) return fail("asr_gpu_op_set_tile: 0 or a power of two <= 64");
+  if (mt != 65 && (mt < 0 || mt > 64 || (mt & (mt - 1)) != 0))
+    return fail("asr_gpu_op_set_tile: 0, a power of two <= 64, or 65 (64 with complex members as lane pairs)");
   op->tile_pref = mt;
   return 0;
 }
@@ -1213,11 +1654,12 @@ int asr_gpu_apply_batched(asr_gpu_op* op, int nb, const int* start_orb, const do
   MemBuffers MB;
   MemDev md;
   if (upload_members(op, M, MB, md)) return 1;
-  const size_t dim = (size_t)op->nn * 9, nv = dim * M.nbp;
+  const size_t dim = (size_t)op->nn * 9, nv = dim * M.nbp, nvp = nv + (size_t)9 * M.nbp;
   std::vector<double> hx;
   to_dev_layout(x, hx, nb, M.nbp, dim);
   double *dx, *dy, *t1, *t2, *h1;
-  if (dalloc(&dx, nv) || dalloc(&dy, nv) || dalloc(&t1, nv) || dalloc(&t2, nv) || dalloc(&h1, nv)) return 1;
+  if (dalloc(&dx, nvp) || dalloc(&dy, nvp) || dalloc(&t1, nvp) || dalloc(&t2, nvp) || dalloc(&h1, nvp)) return 1;
+  for (double* q : {dx, dy, t1, t2, h1}) HCHECK(hipMemsetAsync(q, 0, nvp * 8, op->stream));
   HCHECK(hipMemcpyAsync(dx, hx.data(), nv * 8, hipMemcpyHostToDevice, op->stream));
   int st = apply_chain_op<double>(op, M, md, dx, dy, t1, t2, h1, false, nullptr, nullptr);
   if (!st) {
@@ -1239,11 +1681,12 @@ int asr_gpu_apply_complex(asr_gpu_op* op, int nb, const int* start_orb, const do
   MemBuffers MB;
   MemDev md;
   if (upload_members(op, M, MB, md)) return 1;
-  const size_t dim = (size_t)op->nn * 9, nv = dim * M.nbp;
+  const size_t dim = (size_t)op->nn * 9, nv = dim * M.nbp, nvp = nv + (size_t)9 * M.nbp;
   std::vector<cplx> hx;
   to_dev_layout((const cplx*)x, hx, nb, M.nbp, dim);
   cplx *dx, *dy, *t1, *t2, *h1;
-  if (dalloc(&dx, nv) || dalloc(&dy, nv) || dalloc(&t1, nv) || dalloc(&t2, nv) || dalloc(&h1, nv)) return 1;
+  if (dalloc(&dx, nvp) || dalloc(&dy, nvp) || dalloc(&t1, nvp) || dalloc(&t2, nvp) || dalloc(&h1, nvp)) return 1;
+  for (cplx* q : {dx, dy, t1, t2, h1}) HCHECK(hipMemsetAsync(q, 0, nvp * sizeof(cplx), op->stream));
   HCHECK(hipMemcpyAsync(dx, hx.data(), nv * sizeof(cplx), hipMemcpyHostToDevice, op->stream));
   int st = apply_chain_op<cplx>(op, M, md, dx, dy, t1, t2, h1, pab != nullptr, nullptr, nullptr);
   if (!st) {
@@ -1376,8 +1819,8 @@ static dim3 kpm_grid(const asr_gpu_real* R, int vt, int nvp) {
 
 // Vector tile of the KPM step (vectors per site in a wave: 8, 16, 32 or 64; 0 = automatic).
 int asr_gpu_real_set_tile(asr_gpu_real* R, int vt) {
-  if (vt != 0 && vt != 8 && vt != 16 && vt != 32 && vt != 64) return fail("asr_gpu_real_set_tile: 0, 8, 16, 32 or 64");
-  R->vt = vt;
+  if (vt != 0 && vt != 1 && vt != 2) return fail("asr_gpu_real_set_tile: 0 (automatic), 1 or 2 vectors per lane");
+  R->v2 = (vt != 1);
   return 0;
 }
 
@@ -1409,7 +1852,7 @@ int asr_gpu_kpm_moments(asr_gpu_real* R, double c, double s, int nmom, int nvec,
   if (seed >= (1ull << 24) || nvec > 4096) return fail("kpm_moments: seed < 2^24 and nvec <= 4096");
   const size_t dim = (size_t)R->nsite * 9;
   if (dim >= (1ull << 28)) return fail("kpm_moments: more than 2^28 rows");
-  const int vt = kpm_vt(R, nvec);
+  const int vt = 64;
   const int nvp = ((nvec + vt - 1) / vt) * vt;
   const dim3 grid = kpm_grid(R, vt, nvp);
   double *t0, *t1, *t2, *part, *mper;
@@ -1420,21 +1863,38 @@ int asr_gpu_kpm_moments(asr_gpu_real* R, double c, double s, int nmom, int nvec,
   const int nstep_ev = nmom > 2 ? nmom - 2 : 0;
   std::vector<hipEvent_t> ev(2 + 2 * nstep_ev);
   for (auto& e : ev) HCHECK(hipEventCreate(&e));
+  const RealDev rd = real_dev(R);
   HCHECK(hipEventRecord(ev[0], R->stream));
-  // mu_0 = <r|r>/dim: the step with alpha = 0, beta = -1 gives y = r
-  if (launch_kpm_step(R, vt, grid, nvec, nvp, nullptr, nullptr, t0, 0.0, 0.0, -1.0, seed, part, 1)) return 1;
+  // t0 = r (stored as d o r), mu_0 = <r|r>/dim: the step with alpha = 0, beta = -1
+  hipLaunchKernelGGL((k_kpm_u<true, 0>), grid, dim3(BLOCK), 0, R->stream, rd, (const double*)R->blk, nvec, nvp,
+                     (const double*)nullptr, (const double*)nullptr, t0, 0.0, 0.0, -1.0, seed, part, 1);
   hipLaunchKernelGGL(k_kpm_fin, dim3(nvec), dim3(BLOCK), 0, R->stream, part, (int)grid.x, vt, nvec, inv_dim, mper, nmom, 0);
   if (nmom > 1) {
     // t1 = (H r - c r) / s
-    if (launch_kpm_step(R, vt, grid, nvec, nvp, nullptr, nullptr, t1, c, 1.0 / s, 0.0, seed, part, 1)) return 1;
+    hipLaunchKernelGGL((k_kpm_u<true, 0>), grid, dim3(BLOCK), 0, R->stream, rd, (const double*)R->blk, nvec, nvp,
+                       (const double*)nullptr, (const double*)nullptr, t1, c, 1.0 / s, 0.0, seed, part, 1);
     hipLaunchKernelGGL(k_kpm_fin, dim3(nvec), dim3(BLOCK), 0, R->stream, part, (int)grid.x, vt, nvec, inv_dim, mper, nmom, 1);
   }
   for (int n = 2; n < nmom; ++n) {
     // t2 = 2 (H t1 - c t1) / s - t0
     HCHECK(hipEventRecord(ev[2 + 2 * (n - 2)], R->stream));
-    if (launch_kpm_step(R, vt, grid, nvec, nvp, t1, t0, t2, c, 2.0 / s, 1.0, seed, part, 1)) return 1;
+    if (R->v2 && R->nslot == 18)
+      hipLaunchKernelGGL((k_kpm_v2<18>), dim3((R->nsite + 7) / 8, grid.y), dim3(BLOCK), 0, R->stream, rd,
+                         (const double*)R->blk, nvec, nvp, (const double*)t1, (const double*)t0, t2, c, 2.0 / s, 1.0, seed,
+                         part);
+    else if (R->v2)
+      hipLaunchKernelGGL((k_kpm_v2<0>), dim3((R->nsite + 7) / 8, grid.y), dim3(BLOCK), 0, R->stream, rd,
+                         (const double*)R->blk, nvec, nvp, (const double*)t1, (const double*)t0, t2, c, 2.0 / s, 1.0, seed,
+                         part);
+    else if (R->nslot == 18)
+      hipLaunchKernelGGL((k_kpm_u<false, 18>), grid, dim3(BLOCK), 0, R->stream, rd, (const double*)R->blk, nvec, nvp,
+                         (const double*)t1, (const double*)t0, t2, c, 2.0 / s, 1.0, seed, part, 1);
+    else
+      hipLaunchKernelGGL((k_kpm_u<false, 0>), grid, dim3(BLOCK), 0, R->stream, rd, (const double*)R->blk, nvec, nvp,
+                         (const double*)t1, (const double*)t0, t2, c, 2.0 / s, 1.0, seed, part, 1);
     HCHECK(hipEventRecord(ev[3 + 2 * (n - 2)], R->stream));
-    hipLaunchKernelGGL(k_kpm_fin, dim3(nvec), dim3(BLOCK), 0, R->stream, part, (int)grid.x, vt, nvec, inv_dim, mper, nmom, n);
+    hipLaunchKernelGGL(k_kpm_fin, dim3(nvec), dim3(BLOCK), 0, R->stream, part, R->v2 ? (int)((R->nsite + 7) / 8) : (int)grid.x,
+                       vt, nvec, inv_dim, mper, nmom, n);
     double* tmp = t0; t0 = t1; t1 = t2; t2 = tmp;
   }
   HCHECK(hipEventRecord(ev[1], R->stream));

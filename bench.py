@@ -196,7 +196,7 @@ def main():
         cshift, sscale = 0.0, hb / 0.99
         R = engine.GpuRealization(g.nbr, species, par, block, block0)
         tiles = {}
-        for vt in (16, 32, 64):  # vector-tile autotuning on short recurrences
+        for vt in (1, 2):  # vectors per lane, tuned on short recurrences
             R.set_tile(vt)
             tiles[vt] = R.kpm_moments(cshift, sscale, 12, 64, seed=5)["step_ms"]
         vt_best = min(tiles, key=tiles.get)

@@ -41,8 +41,8 @@ int asr_gpu_op_set_orbital_ops(asr_gpu_op* op, const int* pi, const double* sg);
 int asr_gpu_op_set_blocks(asr_gpu_op* op, const double* block, const double* block0);
 /* Classes: nsc (ncls), spc (ncls x nsmax), mt (ncls x nsmax x nsmax x nsmax) = class_mt (src/asr_augcls.f90:190-205). */
 int asr_gpu_op_set_classes(asr_gpu_op* op, const int* nsc, const int* spc, const double* mt);
-/* Member tile of the batched chains (members per wave; 64 / mt nodes per wave): a power of two <= 64, or 0 = chosen by the
- * engine's autotuner per member count (default).  Replaces the reference's chain_teams / blk_nbmax rules
+/* Member tile of the batched chains (members per wave; 64 / mt nodes per wave): a power of two <= 64, 65 = tile 64 with
+ * complex members run as pairs of real lanes, or 0 = chosen by the engine's autotuner per member count (default).  Replaces the reference's chain_teams / blk_nbmax rules
  * (src/asr_chain.f90:140-155, src/asr_augblk.f90:32-36). */
 int asr_gpu_op_set_tile(asr_gpu_op* op, int mt);
 /* The tile the autotuner picked for nchain members of the given kind (0: none tuned yet). */
@@ -88,7 +88,7 @@ int asr_gpu_profile_chains(asr_gpu_op* op, int nchain, const int* start_orb, con
 int asr_gpu_real_create(int nsite, int nslot, const int* nbr, int nsp, const unsigned char* species, const double* par,
                         const double* block, const double* block0, asr_gpu_real** out);
 void asr_gpu_real_destroy(asr_gpu_real* r);
-/* Vectors per site in a wave of the KPM step: 8, 16, 32 or 64, or 0 = automatic (64). */
+/* Vectors per lane in the Chebyshev steps: 1, 2, or 0 = automatic (2: 16-byte loads, two sites per wave). */
 int asr_gpu_real_set_tile(asr_gpu_real* r, int vt);
 /* y = h x for nvec vectors (h_apply, src/reference/asr_lmto_realization.f90:120-147); x, y: nvec x 9 nsite. */
 int asr_gpu_real_apply(asr_gpu_real* r, int nvec, const double* x, double* y);

@@ -151,3 +151,35 @@ def test_kpm_apply_and_moments():
     assert np.max(np.abs(res["mu_per"] - per_o)) < 1e-10
     assert np.max(np.abs(res["mu"] - mu_o)) < 1e-10
     assert abs(res["mu"][0] - 1.0) < 1e-14
+
+
+@pytest.mark.parametrize("tile", [8, 16, 64, 65])
+def test_edc_member_tiles(ternary_small, tile):
+    """Every member tile (several nodes per wave, or one node per wave with 64 members) gives the oracle's chains."""
+    eng = _engine()
+    wl = ternary_small
+    G = eng.GpuOperator(wl.op)
+    G.set_tile(tile)
+    members = wl.members  # 54 chains: one ragged 64-member tile
+    nlev = np.array([8 + (k % 5) for k in range(len(members))], np.int32)
+    a, b, mu, _ = G.chains_edc(members, nlev)
+    O = _oracle(wl.op)
+    for j in (0, 7, 26, 53):
+        m = members[j]
+        ar, br, n = O.edc_chain(m, m.z * m.pa + m.pb, m.l0, int(nlev[j]))
+        assert mu[j] == n
+        assert rel(a[j, :n], ar[:n]) < 1e-9, f"chain {j}"
+        assert rel(b[j, :n - 1], br[:n - 1]) < 1e-9, f"chain {j}"
+
+
+def test_real_member_tile_64(binary_small):
+    eng = _engine()
+    wl = binary_small
+    G = eng.GpuOperator(wl.op)
+    G.set_tile(64)
+    members = wl.members * 5  # 45 real chains
+    a, b, mu, _ = G.lanczos(members, 10)
+    O = _oracle(wl.op)
+    for j in (0, 13, 44):
+        ar, br, n = O.lanczos(members[j], members[j].l0, 10)
+        assert rel(a[j, :n], ar[:n]) < 1e-9 and rel(b[j, :n], br[:n]) < 1e-9
