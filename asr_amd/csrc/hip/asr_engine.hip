@@ -93,6 +93,7 @@ struct MemDev {
   const cplx* z;              // nbp
   const double* dperm;        // nop x 9 x nbp: d at the permuted orbital, d[pi(l, g)]
   int pair;                   // complex members as lane pairs (member tile 64 only)
+  const int* tact;            // per member tile: 0 = every member has left (the tile's blocks return at once), or null
 };
 
 __device__ inline void unpack_link(uint32_t e, int& r, int& g) {
@@ -257,6 +258,7 @@ __global__ void __launch_bounds__(BLOCK) k_fused(OpDev op, MemDev md, FArgs fa) 
   int* pi = (int*)(blk + nblk);
   double* sg = (double*)(pi + ((op.nop * 9 + 1) & ~1));
   T* red = (T*)(sg + op.nop * 9);
+  if (md.tact && !md.tact[blockIdx.y]) return;  // every member of this tile has left
   for (int q = threadIdx.x; q < nblk; q += BLOCK) blk[q] = op.blk[q];
   for (int q = threadIdx.x; q < op.nop * 9; q += BLOCK) { pi[q] = op.pi[q]; sg[q] = op.sg[q]; }
   __syncthreads();
@@ -313,6 +315,7 @@ template <class T, bool PIID, int NSLOT>
 __global__ void __launch_bounds__(BLOCK, 4) k_node(OpDev op, MemDev md, FArgs fa, const double* __restrict__ blk,
                                                    const double* __restrict__ dperm) {
   __shared__ T red[BLOCK];
+  if (md.tact && !md.tact[blockIdx.y]) return;  // every member of this tile has left
   const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x >> 6;
   const int i = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * (BLOCK / WAVE) + wave);
   const int m = blockIdx.y * WAVE + lane;
@@ -436,6 +439,7 @@ template <bool PIID, int NSLOT>
 __global__ void __launch_bounds__(BLOCK, 4) k_pair(OpDev op, MemDev md, FArgs fa, const double* __restrict__ blk,
                                                    const double* __restrict__ dperm) {
   __shared__ cplx red[BLOCK / WAVE][WAVE / 2];
+  if (md.tact && !md.tact[blockIdx.y / 2]) return;  // every member of this 64-member tile has left
   const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x >> 6;
   const int i = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * (BLOCK / WAVE) + wave);
   const int part = lane & 1;
@@ -621,13 +625,22 @@ __global__ void __launch_bounds__(BLOCK) k_fin_b(ChainDev cd, int nb, const T* p
   inv_b[m] = act ? inv(bk) : zero<T>();
 }
 
-__global__ void k_level_inc(int* level) { *level += 1; }
+// end of a level: level += 1 and the activity of every member tile (a tile whose members have all left is skipped)
+__global__ void k_level_end(int* level, const int* active, int nb, int mt, int* tact, int ntile) {
+  for (int t = threadIdx.x; t < ntile; t += blockDim.x) {
+    int any = 0;
+    for (int q = t * mt; q < std::min(nb, (t + 1) * mt); ++q) any |= active[q];
+    tact[t] = any;
+  }
+  if (threadIdx.x == 0) *level += 1;
+}
 
 // w = w - a u - b_prev up; optional projection onto the symmetric subspace; partial sum w w
 template <class T>
 __global__ void __launch_bounds__(BLOCK) k_update(OpDev op, MemDev md, T* w, const T* u, const T* up, const T* a_eff,
                                                    const T* b_prev, int project, T* partial) {
   __shared__ T red[BLOCK];
+  if (md.tact && !md.tact[blockIdx.y]) return;  // every member of this tile has left
   const int mt = md.mt, npw = WAVE / mt;
   const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x >> 6;
   const int m = blockIdx.y * mt + (lane & (mt - 1));
@@ -1135,7 +1148,7 @@ int upload_members(const asr_gpu_op* op, const Members& M, MemBuffers& B, MemDev
   HCHECK(hipMemcpyAsync(B.dvec, M.dvec.data(), M.dvec.size() * 8, hipMemcpyHostToDevice, op->stream));
   HCHECK(hipMemcpyAsync(B.frozen, M.frozen.data(), M.frozen.size() * 4, hipMemcpyHostToDevice, op->stream));
   HCHECK(hipMemcpyAsync(B.z, M.z.data(), M.z.size() * sizeof(cplx), hipMemcpyHostToDevice, op->stream));
-  md.nb = M.nb; md.nbp = M.nbp; md.mt = M.mt; md.pair = M.pair;
+  md.nb = M.nb; md.nbp = M.nbp; md.mt = M.mt; md.pair = M.pair; md.tact = nullptr;
   md.chimask = B.chim; md.wts = B.wts; md.fpar = B.fpar; md.frozen = B.frozen; md.dvec = B.dvec; md.z = B.z;
   md.dperm = B.dperm;
   return 0;
@@ -1347,109 +1360,122 @@ struct ChainBuffers {
   }
 };
 
+// One batch of chains on one operator: members sorted by depth (deepest first, so member tiles retire in order), device
+// state, and the launches of one level on the group's stream.  Several groups (one per operator) run concurrently on
+// their own streams; the whole run can be captured into one HIP graph (all initialisation happens before the capture).
 template <class T>
-int run_chains(asr_gpu_op* op, int nchain, const int* start_orb, const double* params, const double* pab,
-               const int* frozen_sp, const double* conc, const double* z, const int* nlev, int project, int ldab, T* a_out,
-               T* b_out, int* mused_out, int mode, double tiny, bool real_chain, double* elapsed_ms) {
-  if (!op->have_tables || !op->have_ops || !op->have_blocks || !op->have_classes)
-    return fail("asr_gpu chains: operator tables incomplete");
-  if (nchain <= 0) return 0;
-  if (mode == 1 && nchain > 1) {
-    // plain form: one chain per launch sequence, host-driven levels
-    for (int c = 0; c < nchain; ++c) {
-      double ms = 0.0;
-      int st = run_chains<T>(op, 1, start_orb + c, params + (size_t)c * 4 * op->nsp * 9,
-                             pab ? pab + (size_t)c * 2 * op->nsp * 9 : nullptr, frozen_sp ? frozen_sp + c : nullptr,
-                             conc ? conc + (size_t)c * op->ncls * op->nsmax : nullptr, z ? z + 2 * c : nullptr, nlev + c,
-                             project, ldab, a_out + (size_t)c * ldab, b_out + (size_t)c * ldab, mused_out + c, 1, tiny,
-                             real_chain, &ms);
-      if (st) return st;
-      if (elapsed_ms) *elapsed_ms += ms;
-    }
-    return 0;
-  }
-  int mt = op->tile_pref;
-  if (mt == 0 && mode == 0) {
-    if (autotune_tile<T>(op, nchain, start_orb, params, pab, frozen_sp, conc, z, &mt)) return 1;
-  }
+struct ChainRun {
+  asr_gpu_op* op = nullptr;
+  hipStream_t stream = nullptr;
   Members M;
-  if (prepare_members(op, nchain, start_orb, params, pab, frozen_sp, conc, z, M, mt)) return 1;
   MemBuffers MB;
   MemDev md;
-  if (upload_members(op, M, MB, md)) return 1;
-  int maxdepth = 0;
-  for (int c = 0; c < nchain; ++c) maxdepth = std::max(maxdepth, nlev[c]);
-  if (maxdepth > ldab) return fail("chains: depth larger than the leading dimension of a, b");
-  const size_t nv = (size_t)(op->nn + 1) * 9 * M.nbp;  // + the zero padding node (target of missing links)
   ChainBuffers<T> B;
-  dim3 grid = node_grid(op, md);
-  const int nbx = grid.x;
-  const size_t npart = (size_t)grid.x * grid.y * md.mt;
-  // the dot partials of the apply pass: complex members with tile 64 run as lane pairs, 32 members per wave
-  const int apply_mt = (md.mt == WAVE && md.pair && sizeof(T) == sizeof(cplx)) ? WAVE / 2 : md.mt;
-  if (dalloc(&B.u, nv) || dalloc(&B.up, nv) || dalloc(&B.w, nv) || dalloc(&B.t1, nv) || dalloc(&B.t2, nv) ||
-      dalloc(&B.h1, nv) || dalloc(&B.part, npart) || dalloc(&B.a, (size_t)M.nbp * ldab) ||
-      dalloc(&B.b, (size_t)M.nbp * ldab) || dalloc(&B.a_eff, M.nbp) || dalloc(&B.b_eff, M.nbp) ||
-      dalloc(&B.inv_b, M.nbp) || dalloc(&B.level, 1) || dalloc(&B.depth, M.nbp) || dalloc(&B.active, M.nbp) ||
-      dalloc(&B.mused, M.nbp))
-    return 1;
-  HCHECK(hipMemsetAsync(B.u, 0, nv * sizeof(T), op->stream));
-  HCHECK(hipMemsetAsync(B.up, 0, nv * sizeof(T), op->stream));
-  HCHECK(hipMemsetAsync(B.a, 0, (size_t)M.nbp * ldab * sizeof(T), op->stream));
-  HCHECK(hipMemsetAsync(B.b, 0, (size_t)M.nbp * ldab * sizeof(T), op->stream));
-  HCHECK(hipMemsetAsync(B.b_eff, 0, M.nbp * sizeof(T), op->stream));
-  HCHECK(hipMemsetAsync(B.a_eff, 0, M.nbp * sizeof(T), op->stream));
-  HCHECK(hipMemsetAsync(B.inv_b, 0, M.nbp * sizeof(T), op->stream));
-  HCHECK(hipMemsetAsync(B.w, 0, nv * sizeof(T), op->stream));
-  HCHECK(hipMemsetAsync(B.t1, 0, nv * sizeof(T), op->stream));
-  HCHECK(hipMemsetAsync(B.t2, 0, nv * sizeof(T), op->stream));
-  HCHECK(hipMemsetAsync(B.h1, 0, nv * sizeof(T), op->stream));
-  HCHECK(hipMemsetAsync(B.level, 0, sizeof(int), op->stream));
-  HCHECK(hipMemsetAsync(B.mused, 0, M.nbp * sizeof(int), op->stream));
-  std::vector<int> dep(M.nbp, 0), act(M.nbp, 0);
-  std::vector<T> one(1);
-  for (int c = 0; c < nchain; ++c) {
-    dep[c] = nlev[c];
-    act[c] = nlev[c] > 0 ? 1 : 0;
-    // start vector: unit at component iv = 9 * node 0 + l0
-    T v;
-    std::memset(&v, 0, sizeof(T));
-    *(double*)&v = 1.0;
-    HCHECK(hipMemcpyAsync(B.u + (size_t)start_orb[c] * M.nbp + c, &v, sizeof(T), hipMemcpyHostToDevice, op->stream));
-    HCHECK(hipStreamSynchronize(op->stream));
-  }
-  HCHECK(hipMemcpyAsync(B.depth, dep.data(), M.nbp * sizeof(int), hipMemcpyHostToDevice, op->stream));
-  HCHECK(hipMemcpyAsync(B.active, act.data(), M.nbp * sizeof(int), hipMemcpyHostToDevice, op->stream));
   ChainDev cd;
-  cd.level = B.level; cd.depth = B.depth; cd.active = B.active; cd.mused = B.mused; cd.ld = ldab;
-  cd.real_chain = real_chain ? 1 : 0; cd.tiny = tiny;
-  const bool lin = (pab != nullptr);
-  hipEvent_t e0, e1;
-  HCHECK(hipEventCreate(&e0));
-  HCHECK(hipEventCreate(&e1));
-  HCHECK(hipStreamSynchronize(op->stream));
-  HCHECK(hipEventRecord(e0, op->stream));
-  dim3 sgrid((unsigned)((nv + 255) / 256));
-  for (int k = 0; k < maxdepth; ++k) {
+  dim3 grid, sgrid;
+  int nbx = 0, apply_mt = 1, maxdepth = 0, ld = 0, project = 0, nchain = 0, ntile = 1;
+  size_t nv = 0;
+  bool lin = false;
+  std::vector<int> order;  // sorted member -> input chain
+  int* tact = nullptr;     // per member tile: 1 while any member of the tile is active
+
+  ~ChainRun() {
+    if (tact) (void)hipFree(tact);
+  }
+
+  int setup(asr_gpu_op* op_, hipStream_t s, int nch, const int* start_orb, const double* params, const double* pab,
+            const int* frozen_sp, const double* conc, const double* z, const int* nlev, int project_, int ldab, int tile,
+            double tiny, bool real_chain) {
+    op = op_; stream = s; nchain = nch; project = project_; ld = ldab; lin = (pab != nullptr);
+    if (!op->have_tables || !op->have_ops || !op->have_blocks || !op->have_classes)
+      return fail("asr_gpu chains: operator tables incomplete");
+    // deepest chains first
+    order.resize(nch);
+    for (int c = 0; c < nch; ++c) order[c] = c;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return nlev[x] > nlev[y]; });
+    const int nsp9 = op->nsp * 9, ncn = op->ncls * op->nsmax;
+    std::vector<int> so(nch), fz(nch), nl(nch);
+    std::vector<double> pp((size_t)nch * 4 * nsp9), pa(pab ? (size_t)nch * 2 * nsp9 : 0), cc(conc ? (size_t)nch * ncn : 0),
+        zz(z ? 2 * nch : 0);
+    for (int q = 0; q < nch; ++q) {
+      const int c = order[q];
+      so[q] = start_orb[c];
+      fz[q] = frozen_sp ? frozen_sp[c] : -1;
+      nl[q] = nlev[c];
+      std::memcpy(&pp[(size_t)q * 4 * nsp9], params + (size_t)c * 4 * nsp9, sizeof(double) * 4 * nsp9);
+      if (pab) std::memcpy(&pa[(size_t)q * 2 * nsp9], pab + (size_t)c * 2 * nsp9, sizeof(double) * 2 * nsp9);
+      if (conc) std::memcpy(&cc[(size_t)q * ncn], conc + (size_t)c * ncn, sizeof(double) * ncn);
+      if (z) { zz[2 * q] = z[2 * c]; zz[2 * q + 1] = z[2 * c + 1]; }
+    }
+    const double* pab_s = pab ? pa.data() : nullptr;
+    const double* conc_s = conc ? cc.data() : nullptr;
+    const double* z_s = z ? zz.data() : nullptr;
+    if (tile == 0 && autotune_tile<T>(op, nch, so.data(), pp.data(), pab_s, fz.data(), conc_s, z_s, &tile)) return 1;
+    if (prepare_members(op, nch, so.data(), pp.data(), pab_s, fz.data(), conc_s, z_s, M, tile)) return 1;
+    if (upload_members(op, M, MB, md)) return 1;
+    maxdepth = 0;
+    for (int q = 0; q < nch; ++q) maxdepth = std::max(maxdepth, nl[q]);
+    if (maxdepth > ldab) return fail("chains: depth larger than the leading dimension of a, b");
+    nv = (size_t)(op->nn + 1) * 9 * M.nbp;  // + the zero padding node (target of missing links)
+    grid = node_grid(op, md);
+    nbx = grid.x;
+    ntile = M.nbp / M.mt;
+    const size_t npart = (size_t)grid.x * grid.y * md.mt;
+    // the dot partials of the apply pass: complex members with tile 64 run as lane pairs, 32 members per wave
+    apply_mt = (md.mt == WAVE && md.pair && sizeof(T) == sizeof(cplx)) ? WAVE / 2 : md.mt;
+    if (dalloc(&B.u, nv) || dalloc(&B.up, nv) || dalloc(&B.w, nv) || dalloc(&B.t1, nv) || dalloc(&B.t2, nv) ||
+        dalloc(&B.h1, nv) || dalloc(&B.part, npart) || dalloc(&B.a, (size_t)M.nbp * ldab) ||
+        dalloc(&B.b, (size_t)M.nbp * ldab) || dalloc(&B.a_eff, M.nbp) || dalloc(&B.b_eff, M.nbp) ||
+        dalloc(&B.inv_b, M.nbp) || dalloc(&B.level, 1) || dalloc(&B.depth, M.nbp) || dalloc(&B.active, M.nbp) ||
+        dalloc(&B.mused, M.nbp) || dalloc(&tact, std::max(ntile, 2)))
+      return 1;
+    for (T* q : {B.u, B.up, B.w, B.t1, B.t2, B.h1}) HCHECK(hipMemsetAsync(q, 0, nv * sizeof(T), stream));
+    HCHECK(hipMemsetAsync(B.a, 0, (size_t)M.nbp * ldab * sizeof(T), stream));
+    HCHECK(hipMemsetAsync(B.b, 0, (size_t)M.nbp * ldab * sizeof(T), stream));
+    for (T* q : {B.a_eff, B.b_eff, B.inv_b}) HCHECK(hipMemsetAsync(q, 0, M.nbp * sizeof(T), stream));
+    HCHECK(hipMemsetAsync(B.level, 0, sizeof(int), stream));
+    HCHECK(hipMemsetAsync(B.mused, 0, M.nbp * sizeof(int), stream));
+    std::vector<int> dep(M.nbp, 0), act(M.nbp, 0), ta(std::max(ntile, 2), 0);
+    std::vector<T> ustart((size_t)9 * M.nbp);
+    std::memset(ustart.data(), 0, ustart.size() * sizeof(T));
+    for (int q = 0; q < nch; ++q) {
+      dep[q] = nl[q];
+      act[q] = nl[q] > 0 ? 1 : 0;
+      if (act[q]) ta[q / M.mt] = 1;
+      // start vector: unit at component iv = 9 * node 0 + l0
+      *(double*)&ustart[(size_t)so[q] * M.nbp + q] = 1.0;
+    }
+    HCHECK(hipMemcpyAsync(B.u, ustart.data(), ustart.size() * sizeof(T), hipMemcpyHostToDevice, stream));
+    HCHECK(hipMemcpyAsync(B.depth, dep.data(), M.nbp * sizeof(int), hipMemcpyHostToDevice, stream));
+    HCHECK(hipMemcpyAsync(B.active, act.data(), M.nbp * sizeof(int), hipMemcpyHostToDevice, stream));
+    HCHECK(hipMemcpyAsync(tact, ta.data(), ta.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+    HCHECK(hipStreamSynchronize(stream));
+    cd.level = B.level; cd.depth = B.depth; cd.active = B.active; cd.mused = B.mused; cd.ld = ldab;
+    cd.real_chain = real_chain ? 1 : 0; cd.tiny = tiny;
+    sgrid = dim3((unsigned)((nv + 255) / 256));
+    return 0;
+  }
+
+  // the launches of one level on the group's stream (no host synchronisation)
+  int step() {
+    md.tact = tact;
     if (g_timer) g_timer->ord = 0;
     if (apply_chain_op<T>(op, M, md, B.u, B.w, B.t1, B.t2, B.h1, lin, B.u, B.part)) return 1;
-    if (timer_mark(op->stream, KIND_FIN_A, true)) return 1;
-    hipLaunchKernelGGL(k_fin_a<T>, dim3(M.nb), dim3(BLOCK), 0, op->stream, cd, M.nb, B.part, nbx, apply_mt, B.a,
-                       B.a_eff);
-    if (timer_mark(op->stream, 0, false) || timer_mark(op->stream, KIND_UPDATE, true)) return 1;
-    hipLaunchKernelGGL(k_update<T>, grid, dim3(BLOCK), 0, op->stream, dev_of(op), md, B.w, B.u, B.up, B.a_eff, B.b_eff,
-                       project, B.part);
-    if (timer_mark(op->stream, 0, false) || timer_mark(op->stream, KIND_FIN_B, true)) return 1;
-    hipLaunchKernelGGL(k_fin_b<T>, dim3(M.nb), dim3(BLOCK), 0, op->stream, cd, M.nb, B.part, nbx, md.mt, B.b,
-                       B.b_eff, B.inv_b);
-    if (timer_mark(op->stream, 0, false) || timer_mark(op->stream, KIND_SCALE, true)) return 1;
-    hipLaunchKernelGGL(k_scale<T>, sgrid, dim3(256), 0, op->stream, B.w, B.inv_b, nv, M.nbp);
-    if (timer_mark(op->stream, 0, false)) return 1;
-    hipLaunchKernelGGL(k_level_inc, dim3(1), dim3(1), 0, op->stream, B.level);
+    if (timer_mark(stream, KIND_FIN_A, true)) return 1;
+    hipLaunchKernelGGL(k_fin_a<T>, dim3(M.nb), dim3(BLOCK), 0, stream, cd, M.nb, B.part, nbx, apply_mt, B.a, B.a_eff);
+    if (timer_mark(stream, 0, false) || timer_mark(stream, KIND_UPDATE, true)) return 1;
+    hipLaunchKernelGGL(k_update<T>, grid, dim3(BLOCK), 0, stream, dev_of(op), md, B.w, B.u, B.up, B.a_eff, B.b_eff, project,
+                       B.part);
+    if (timer_mark(stream, 0, false) || timer_mark(stream, KIND_FIN_B, true)) return 1;
+    hipLaunchKernelGGL(k_fin_b<T>, dim3(M.nb), dim3(BLOCK), 0, stream, cd, M.nb, B.part, nbx, md.mt, B.b, B.b_eff,
+                       B.inv_b);
+    if (timer_mark(stream, 0, false) || timer_mark(stream, KIND_SCALE, true)) return 1;
+    hipLaunchKernelGGL(k_scale<T>, sgrid, dim3(256), 0, stream, B.w, B.inv_b, nv, M.nbp);
+    if (timer_mark(stream, 0, false)) return 1;
+    hipLaunchKernelGGL(k_level_end, dim3(1), dim3(64), 0, stream, B.level, B.active, M.nb, M.mt, tact, ntile);
     HCHECK(hipGetLastError());
     if (g_timer) {
-      // accumulate this level's kernel durations
-      HCHECK(hipStreamSynchronize(op->stream));
+      HCHECK(hipStreamSynchronize(stream));
       if (g_acc.size() < (size_t)g_timer->ord) g_acc.resize(g_timer->ord, 0.0);
       for (int q = 0; q < g_timer->ord; ++q) {
         float t = 0.f;
@@ -1460,11 +1486,65 @@ int run_chains(asr_gpu_op* op, int nchain, const int* start_orb, const double* p
     }
     std::swap(B.up, B.u);  // up <- u
     std::swap(B.u, B.w);   // u <- w / b ; w <- old up (overwritten next level)
+    return 0;
+  }
+
+  int any_active(int* actv) {
+    HCHECK(hipMemcpyAsync(actv, B.active, sizeof(int), hipMemcpyDeviceToHost, stream));
+    HCHECK(hipStreamSynchronize(stream));
+    return 0;
+  }
+
+  int finish(T* a_out, T* b_out, int* mused_out) {
+    std::vector<T> ah((size_t)M.nbp * ld), bh((size_t)M.nbp * ld);
+    std::vector<int> mu(M.nbp);
+    HCHECK(hipMemcpyAsync(ah.data(), B.a, ah.size() * sizeof(T), hipMemcpyDeviceToHost, stream));
+    HCHECK(hipMemcpyAsync(bh.data(), B.b, bh.size() * sizeof(T), hipMemcpyDeviceToHost, stream));
+    HCHECK(hipMemcpyAsync(mu.data(), B.mused, mu.size() * sizeof(int), hipMemcpyDeviceToHost, stream));
+    HCHECK(hipStreamSynchronize(stream));
+    for (int q = 0; q < nchain; ++q) {
+      const int c = order[q];
+      std::memcpy(a_out + (size_t)c * ld, ah.data() + (size_t)q * ld, sizeof(T) * ld);
+      std::memcpy(b_out + (size_t)c * ld, bh.data() + (size_t)q * ld, sizeof(T) * ld);
+      mused_out[c] = mu[q];
+    }
+    return 0;
+  }
+};
+
+template <class T>
+int run_chains(asr_gpu_op* op, int nchain, const int* start_orb, const double* params, const double* pab,
+               const int* frozen_sp, const double* conc, const double* z, const int* nlev, int project, int ldab, T* a_out,
+               T* b_out, int* mused_out, int mode, double tiny, bool real_chain, double* elapsed_ms) {
+  if (nchain <= 0) return 0;
+  const int nsp9 = op->nsp * 9, ncn = op->ncls * op->nsmax;
+  if (mode == 1 && nchain > 1) {
+    // plain form: one chain per launch sequence, host-driven levels
+    for (int c = 0; c < nchain; ++c) {
+      double ms = 0.0;
+      int st = run_chains<T>(op, 1, start_orb + c, params + (size_t)c * 4 * nsp9, pab ? pab + (size_t)c * 2 * nsp9 : nullptr,
+                             frozen_sp ? frozen_sp + c : nullptr, conc ? conc + (size_t)c * ncn : nullptr,
+                             z ? z + 2 * c : nullptr, nlev + c, project, ldab, a_out + (size_t)c * ldab,
+                             b_out + (size_t)c * ldab, mused_out + c, 1, tiny, real_chain, &ms);
+      if (st) return st;
+      if (elapsed_ms) *elapsed_ms += ms;
+    }
+    return 0;
+  }
+  ChainRun<T> R;
+  if (R.setup(op, op->stream, nchain, start_orb, params, pab, frozen_sp, conc, z, nlev, project, ldab,
+              mode == 1 ? 1 : op->tile_pref, tiny, real_chain))
+    return 1;
+  hipEvent_t e0, e1;
+  HCHECK(hipEventCreate(&e0));
+  HCHECK(hipEventCreate(&e1));
+  HCHECK(hipEventRecord(e0, op->stream));
+  for (int k = 0; k < R.maxdepth; ++k) {
+    if (R.step()) return 1;
     if (mode == 1) {
-      // plain form: the host reads the level's coefficients back and stops when every chain has left
+      // plain form: the host reads the chain's state back after every level and stops when it has left
       int actv = 0;
-      HCHECK(hipMemcpyAsync(&actv, B.active, sizeof(int), hipMemcpyDeviceToHost, op->stream));
-      HCHECK(hipStreamSynchronize(op->stream));
+      if (R.any_active(&actv)) return 1;
       if (!actv) break;
     }
   }
@@ -1475,17 +1555,113 @@ int run_chains(asr_gpu_op* op, int nchain, const int* start_orb, const double* p
   if (elapsed_ms) *elapsed_ms = ms;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  std::vector<T> ah((size_t)M.nbp * ldab), bh((size_t)M.nbp * ldab);
-  std::vector<int> mu(M.nbp);
-  HCHECK(hipMemcpy(ah.data(), B.a, ah.size() * sizeof(T), hipMemcpyDeviceToHost));
-  HCHECK(hipMemcpy(bh.data(), B.b, bh.size() * sizeof(T), hipMemcpyDeviceToHost));
-  HCHECK(hipMemcpy(mu.data(), B.mused, mu.size() * sizeof(int), hipMemcpyDeviceToHost));
-  for (int c = 0; c < nchain; ++c) {
-    std::memcpy(a_out + (size_t)c * ldab, ah.data() + (size_t)c * ldab, sizeof(T) * ldab);
-    std::memcpy(b_out + (size_t)c * ldab, bh.data() + (size_t)c * ldab, sizeof(T) * ldab);
-    mused_out[c] = mu[c];
+  return R.finish(a_out, b_out, mused_out);
+}
+
+// Chains on several operators at once (one group per operator: the sublattices of a multi-sublattice alloy).  Every group
+// runs on its own stream so that small operators fill the GPU together; members of a group are sorted by depth and tiles
+// whose members have all left are skipped.  Per-chain arrays are concatenated over the groups in order; conc (optional)
+// holds per chain the ncls x nsmax concentrations of its own operator.  mode 0: host-issued levels on the group streams;
+// mode 2: the whole run captured into one HIP graph and launched once; mode 1: the plain form, one chain per launch sequence.
+template <class T>
+int run_multi(int ngroup, asr_gpu_op* const* ops, const int* ngch, const int* start_orb, const double* params,
+              const double* pab, const int* frozen_sp, const double* conc, const double* z, const int* nlev, int project,
+              int ldab, T* a, T* b, int* mused, int mode, double tiny, bool real_chain, double* elapsed_ms) {
+  if (elapsed_ms) *elapsed_ms = 0.0;
+  std::vector<size_t> off_c(ngroup + 1, 0), off_p(ngroup + 1, 0), off_pa(ngroup + 1, 0), off_cc(ngroup + 1, 0);
+  for (int g = 0; g < ngroup; ++g) {
+    off_c[g + 1] = off_c[g] + ngch[g];
+    off_p[g + 1] = off_p[g] + (size_t)ngch[g] * 4 * ops[g]->nsp * 9;
+    off_pa[g + 1] = off_pa[g] + (size_t)ngch[g] * 2 * ops[g]->nsp * 9;
+    off_cc[g + 1] = off_cc[g] + (size_t)ngch[g] * ops[g]->ncls * ops[g]->nsmax;
   }
-  return 0;
+  if (mode == 1) {
+    for (int g = 0; g < ngroup; ++g) {
+      double ms = 0.0;
+      const size_t c0 = off_c[g];
+      if (run_chains<T>(ops[g], ngch[g], start_orb + c0, params + off_p[g], pab ? pab + off_pa[g] : nullptr,
+                        frozen_sp ? frozen_sp + c0 : nullptr, conc ? conc + off_cc[g] : nullptr, z ? z + 2 * c0 : nullptr,
+                        nlev + c0, project, ldab, a + c0 * ldab, b + c0 * ldab, mused + c0, 1, tiny, real_chain, &ms))
+        return 1;
+      if (elapsed_ms) *elapsed_ms += ms;
+    }
+    return 0;
+  }
+  std::vector<hipStream_t> st(ngroup, nullptr);
+  std::vector<ChainRun<T>> runs(ngroup);
+  auto cleanup = [&]() {
+    for (auto s2 : st)
+      if (s2) (void)hipStreamDestroy(s2);
+  };
+  for (int g = 0; g < ngroup; ++g) {
+    if (hipStreamCreateWithFlags(&st[g], hipStreamNonBlocking) != hipSuccess) { cleanup(); return fail("stream create"); }
+    const size_t c0 = off_c[g];
+    if (ngch[g] <= 0) continue;
+    if (runs[g].setup(ops[g], st[g], ngch[g], start_orb + c0, params + off_p[g], pab ? pab + off_pa[g] : nullptr,
+                      frozen_sp ? frozen_sp + c0 : nullptr, conc ? conc + off_cc[g] : nullptr, z ? z + 2 * c0 : nullptr,
+                      nlev + c0, project, ldab, ops[g]->tile_pref, tiny, real_chain)) {
+      cleanup();
+      return 1;
+    }
+  }
+  hipStream_t s0 = st[0];
+  hipEvent_t e0, e1, fork;
+  std::vector<hipEvent_t> join(ngroup);
+  HCHECK(hipEventCreate(&e0));
+  HCHECK(hipEventCreate(&e1));
+  HCHECK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+  for (auto& e : join) HCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  int maxd = 0;
+  for (int g = 0; g < ngroup; ++g) maxd = std::max(maxd, ngch[g] > 0 ? runs[g].maxdepth : 0);
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int status = 0;
+  auto issue = [&]() -> int {
+    // fork every group stream off s0, levels interleaved over the groups, join back into s0
+    if (hipEventRecord(fork, s0) != hipSuccess) return fail("event record");
+    for (int g = 1; g < ngroup; ++g)
+      if (hipStreamWaitEvent(st[g], fork, 0) != hipSuccess) return fail("stream wait");
+    for (int k = 0; k < maxd; ++k)
+      for (int g = 0; g < ngroup; ++g)
+        if (ngch[g] > 0 && k < runs[g].maxdepth && runs[g].step()) return 1;
+    for (int g = 1; g < ngroup; ++g) {
+      if (hipEventRecord(join[g], st[g]) != hipSuccess) return fail("event record");
+      if (hipStreamWaitEvent(s0, join[g], 0) != hipSuccess) return fail("stream wait");
+    }
+    return 0;
+  };
+  if (mode == 2) {
+    HCHECK(hipStreamBeginCapture(s0, hipStreamCaptureModeThreadLocal));
+    status = issue();
+    hipError_t ce = hipStreamEndCapture(s0, &graph);
+    if (!status && ce != hipSuccess) status = fail(std::string("graph capture: ") + hipGetErrorString(ce));
+    if (!status && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) status = fail("graph instantiate");
+    if (!status) {
+      HCHECK(hipEventRecord(e0, s0));
+      if (hipGraphLaunch(exec, s0) != hipSuccess) status = fail("graph launch");
+      HCHECK(hipEventRecord(e1, s0));
+    }
+  } else {
+    HCHECK(hipEventRecord(e0, s0));
+    status = issue();
+    HCHECK(hipEventRecord(e1, s0));
+  }
+  if (!status) {
+    HCHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HCHECK(hipEventElapsedTime(&ms, e0, e1));
+    if (elapsed_ms) *elapsed_ms = ms;
+    for (int g = 0; g < ngroup && !status; ++g) {
+      const size_t c0 = off_c[g];
+      if (ngch[g] > 0) status = runs[g].finish(a + c0 * ldab, b + c0 * ldab, mused + c0);
+    }
+  }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  for (hipEvent_t e : {e0, e1, fork}) (void)hipEventDestroy(e);
+  for (auto& e : join) (void)hipEventDestroy(e);
+  cleanup();
+  return status;
 }
 
 // host layout [member][dim] <-> device layout [(node*9 + l) * nbp + member]
@@ -1712,6 +1888,23 @@ int asr_gpu_chains_edc(asr_gpu_op* op, int nchain, const int* start_orb, const d
   if (elapsed_ms) *elapsed_ms = 0.0;
   return run_chains<cplx>(op, nchain, start_orb, params, pab, frozen_sp, conc, z, nlev, project, ldab, (cplx*)a, (cplx*)b,
                           mused, mode, 1e-12, false, elapsed_ms);
+}
+
+int asr_gpu_chains_edc_multi(int ngroup, asr_gpu_op* const* ops, const int* nchain, const int* start_orb,
+                             const double* params, const double* pab, const int* frozen_sp, const double* conc, const double* z,
+                             const int* nlev, int project, int ldab, double* a, double* b, int* mused, int mode,
+                             double* elapsed_ms) {
+  if (ngroup <= 0) return fail("chains_multi: no groups");
+  return run_multi<cplx>(ngroup, ops, nchain, start_orb, params, pab, frozen_sp, conc, z, nlev, project, ldab, (cplx*)a,
+                         (cplx*)b, mused, mode, 1e-12, false, elapsed_ms);
+}
+
+int asr_gpu_lanczos_real_multi(int ngroup, asr_gpu_op* const* ops, const int* nchain, const int* start_orb,
+                               const double* params, const int* frozen_sp, const double* conc, const int* nlev, double tiny,
+                               int ldab, double* a, double* b, int* mused, int mode, double* elapsed_ms) {
+  if (ngroup <= 0) return fail("lanczos_multi: no groups");
+  return run_multi<double>(ngroup, ops, nchain, start_orb, params, nullptr, frozen_sp, conc, nullptr, nlev, 0, ldab, a, b,
+                           mused, mode, tiny > 0 ? tiny : 1e-12, true, elapsed_ms);
 }
 
 // Per-kernel durations of the chain level step: the same chains as asr_gpu_chains_edc (pab != NULL) or

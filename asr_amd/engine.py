@@ -46,6 +46,8 @@ def lib() -> C.CDLL:
         L.asr_gpu_apply_complex.argtypes = [P, C.c_int, P, P, P, P, P, P, P, P]
         L.asr_gpu_lanczos_real.argtypes = [P, C.c_int, P, P, P, P, P, C.c_double, C.c_int, P, P, P, C.c_int, P]
         L.asr_gpu_chains_edc.argtypes = [P, C.c_int, P, P, P, P, P, P, P, C.c_int, C.c_int, P, P, P, C.c_int, P]
+        L.asr_gpu_chains_edc_multi.argtypes = [C.c_int, P, P, P, P, P, P, P, P, P, C.c_int, C.c_int, P, P, P, C.c_int, P]
+        L.asr_gpu_lanczos_real_multi.argtypes = [C.c_int, P, P, P, P, P, P, P, C.c_double, C.c_int, P, P, P, C.c_int, P]
         L.asr_gpu_profile_chains.argtypes = [P, C.c_int, P, P, P, P, P, P, P, C.c_int, P, P, P]
         L.asr_gpu_real_create.argtypes = [C.c_int, C.c_int, P, C.c_int, P, P, P, P, C.POINTER(C.c_void_p)]
         L.asr_gpu_real_destroy.argtypes = [P]
@@ -200,6 +202,56 @@ class GpuOperator:
                                             _p(nk), _p(ms), _p(kind)), "asr_gpu_profile_chains")
         n = int(nk[0])
         return ms[:n].copy(), kind[:n].copy()
+
+
+def chains_multi(groups, project: bool = True, mode: int = 2, complex_route: bool = True, tiny: float = 1.0e-12):
+    """Chains on several operators in one batched run.  groups: list of (GpuOperator, members, nlev).  mode 2: one HIP
+    graph for the whole run, 0: host-issued levels on one stream per group, 1: plain (one chain per launch sequence).
+    Returns per group (a, b, mused) and the device time in ms."""
+    ng = len(groups)
+    ops = (C.c_void_p * ng)(*[g[0].h.value for g in groups])
+    nch = _i32([len(g[1]) for g in groups])
+    packs = [pack_members(g[0].op, g[1], complex_route=complex_route) for g in groups]
+    start = _i32(np.concatenate([p[0] for p in packs]))
+    params = _f64(np.concatenate([p[1].reshape(-1) for p in packs]))
+    frozen = _i32(np.concatenate([p[2] for p in packs]))
+    conc = None
+    if any(p[3] is not None for p in packs):
+        parts = []
+        for (G, mem, _), p in zip(groups, packs):
+            if p[3] is not None:
+                parts.append(p[3].reshape(-1))
+            else:
+                c = np.zeros((len(mem), G.op.ncls, G.op.nsmax))
+                for kap in range(G.op.ncls):
+                    c[:, kap, :G.op.nsc[kap]] = G.op.conc[kap, :G.op.nsc[kap]]
+                parts.append(c.reshape(-1))
+        conc = _f64(np.concatenate(parts))
+    nlev = _i32(np.concatenate([np.broadcast_to(np.asarray(g[2]), (len(g[1]),)) for g in groups]))
+    ld = int(nlev.max())
+    tot = int(nch.sum())
+    mu = np.zeros(tot, np.int32)
+    ms = np.zeros(1)
+    if complex_route:
+        pab = _f64(np.concatenate([p[4].reshape(-1) for p in packs]))
+        z = np.ascontiguousarray(np.concatenate([p[5] for p in packs]))
+        a = np.zeros((tot, ld), np.complex128)
+        b = np.zeros((tot, ld), np.complex128)
+        _check(lib().asr_gpu_chains_edc_multi(ng, ops, _p(nch), _p(start), _p(params), _p(pab), _p(frozen), _p(conc), _p(z),
+                                              _p(nlev), int(project), ld, _p(a), _p(b), _p(mu), int(mode), _p(ms)),
+               "asr_gpu_chains_edc_multi")
+    else:
+        a = np.zeros((tot, ld))
+        b = np.zeros((tot, ld))
+        _check(lib().asr_gpu_lanczos_real_multi(ng, ops, _p(nch), _p(start), _p(params), _p(frozen), _p(conc), _p(nlev),
+                                                float(tiny), ld, _p(a), _p(b), _p(mu), int(mode), _p(ms)),
+               "asr_gpu_lanczos_real_multi")
+    out = []
+    off = 0
+    for n in nch:
+        out.append((a[off:off + n], b[off:off + n], mu[off:off + n]))
+        off += int(n)
+    return out, float(ms[0])
 
 
 class GpuRealization:

@@ -16,6 +16,7 @@ engine).  Each function restates one reference routine; indices are 0-based.
     orbital_rep             src/asr_symmetry.f90:100-129
     lmto_alloy_blocks       src/asr_lmto_alloy.f90:315-354 (bond blocks only)
     augn_basis, class_mt    src/asr_augcls.f90:73-91, 190-205
+    sro_conditional         src/asr_sro.f90:28-80
 """
 from __future__ import annotations
 
@@ -407,3 +408,34 @@ def class_mt(conc, nsmax: int) -> np.ndarray:
             for j in range(n):
                 mt[l, k, sup[j]] = conc[sup[j]] * phi[l, j] * phi[k, j]
     return mt
+
+
+def sro_conditional(x, y, alpha, same: bool = False):
+    """Conditional concentrations cond[c, s] = P(species s on a first-shell neighbour | species c at the centre)
+    = y_s (1 - alpha_cs) for the Warren-Cowley table alpha (n x m); one sublattice (same): alpha symmetric with the diagonal
+    derived, alpha_cc = -sum_s x_s alpha_cs / x_c; two sublattices: both marginal conditions must hold."""
+    x = np.asarray(x, dtype=float)
+    y = np.asarray(y, dtype=float)
+    alpha = np.array(alpha, dtype=float)
+    n, m = x.size, y.size
+    if alpha.shape != (n, m):
+        raise ValueError("sro: alpha must be n(centre) x n(neighbour)")
+    if np.any(x <= 0) or np.any(y <= 0) or abs(x.sum() - 1) > 1e-8 or abs(y.sum() - 1) > 1e-8:
+        raise ValueError("sro: concentrations must be > 0 and sum to 1")
+    if same:
+        if n != m or np.max(np.abs(x - y)) > 1e-10:
+            raise ValueError("sro: one sublattice needs x = y")
+        if np.any(np.abs(np.diag(alpha)) > 1e-10) or np.max(np.abs(alpha - alpha.T)) > 1e-10:
+            raise ValueError("sro: alpha must be symmetric with a zero diagonal on one sublattice")
+        for i in range(n):
+            alpha[i, i] = -np.sum(x * alpha[i, :]) / x[i]
+    else:
+        if np.max(np.abs(alpha @ y)) > 1e-8 or np.max(np.abs(x @ alpha)) > 1e-8:
+            raise ValueError("sro: the marginal conditions sum_s y_s alpha_cs = 0 = sum_c x_c alpha_cs do not hold")
+    cond = y[None, :] * (1.0 - alpha)
+    if np.any(cond < -1e-10):
+        raise ValueError("sro: a conditional probability is negative")
+    cond[np.abs(cond) <= 1e-10] = 0.0
+    if np.max(np.abs(cond.sum(axis=1) - 1.0)) > 1e-8:
+        raise ValueError("sro: internal error, conditional sum /= 1")
+    return cond, alpha

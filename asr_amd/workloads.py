@@ -161,3 +161,79 @@ def kpm_supercell(L: int = 40, seed: int = 4, conc=(0.66, 0.14, 0.20), brick: in
     par[0] = c
     par[1] = d
     return g, species, par, block, block0
+
+
+def warren_cowley(rng: np.random.Generator, x, y, amp: float = 0.1) -> np.ndarray:
+    """A seeded Warren-Cowley table alpha (n x m) with entries of order amp that satisfies both marginal conditions of two
+    sublattices (sum_s y_s alpha_cs = 0 = sum_c x_c alpha_cs): a uniform draw projected onto them (the two projections
+    commute), scaled down until every conditional concentration is non-negative."""
+    x = np.asarray(x, dtype=float)
+    y = np.asarray(y, dtype=float)
+    a = rng.uniform(-amp, amp, (x.size, y.size))
+    a = a - (a @ y)[:, None]
+    a = a - (x @ a)[None, :]
+    while np.any(y[None, :] * (1.0 - a) < 0.0):
+        a *= 0.5
+    return a
+
+
+L21_SPC = [[0], [1, 2], [1, 3], [2, 3, 4]]
+L21_CONC = [[1.0], [0.5, 0.5], [0.4, 0.6], [0.3, 0.3, 0.4]]
+
+
+def l21_batch(L: int = 8, levels=(3, 3, 3, 2), nchains: int = 256, depth=(20, 60), seed: int = 3, sro: bool = True,
+              group: str = "d2h", energies=(complex(-0.25, 0.12), complex(0.05, 0.12)), reorder: bool = True):
+    """4-sublattice L2_1-type bcc cell (sublattice of a site: (x + y + z) mod 4 in half-lattice units) with occupations
+    1.0 / 0.5-0.5 / 0.4-0.6 / 0.3-0.3-0.4 over 5 species, screened S^alpha to 2 bcc shells (8 + 6 slots).  One operator per
+    start sublattice, the start site frozen to each of its species, 2 spins, 9 orbitals, 2 contour energies; with sro the 8
+    first-shell sites of the start get shell classes whose concentrations are the Warren-Cowley conditionals of the central
+    species (augcls_set_class_conc per chain, src/asr_class_scf.f90:1122-1126).  nchains of the 288 (sublattice, species,
+    spin, orbital, energy) chains are drawn, depths uniform in [depth].  Returns (list of Workload per operator, info)."""
+    rng = np.random.default_rng(seed)
+    g, block, block0 = graph_and_blocks("bcc", L)
+    base = (g.coord.sum(axis=1) % 4).astype(np.int32)
+    nsp = 5
+    spins = edc_params(rng, nsp, 2)
+    shell = g.nbr[0, :8]
+    # Warren-Cowley tables centre sublattice -> neighbour sublattice (alpha_1 in [-0.1, 0.1])
+    wc = {(k1, k2): warren_cowley(rng, L21_CONC[k1], L21_CONC[k2]) for k1 in range(4) for k2 in range(4)}
+    cands = []
+    ops = []
+    for kap in range(4):
+        cls = ((base + kap) % 4).astype(np.int32)
+        if sro:
+            cls[shell] = 4 + cls[shell]
+        nsc = [len(c) for c in L21_SPC] * (2 if sro else 1)
+        spc = L21_SPC * (2 if sro else 1)
+        conc = L21_CONC * (2 if sro else 1)
+        perm, mats = S.class_ops(g.coord, 2 * L, cls, group)
+        dmat = S.orbital_rep(mats)
+        fl = np.ones(g.n, np.uint8)
+        fl[0] = 0
+        op = build_operator(g, cls, nsc, spc, conc, perm, dmat, 0, levels[kap], maxrank=1, flippable=fl,
+                            block=block[None], block0=block0[None], nsp=nsp)
+        if reorder:
+            op = reorder_nodes(op)
+        ops.append(op)
+        for ic, cen in enumerate(L21_SPC[kap]):
+            mconc = None
+            if sro:
+                mconc = [list(c) for c in conc]
+                for lam in range(4):
+                    cond, _ = S.sro_conditional(L21_CONC[kap], L21_CONC[lam], wc[(kap, lam)])
+                    mconc[4 + lam] = list(cond[ic])
+            for sp in range(2):
+                for l0 in range(9):
+                    for z in energies:
+                        m = edc_member(spins[sp], cen, l0, z)
+                        m.conc = mconc
+                        cands.append((kap, m))
+    pick = np.sort(rng.choice(len(cands), size=min(nchains, len(cands)), replace=False))
+    depths = rng.integers(depth[0], depth[1] + 1, size=pick.size)
+    wls = []
+    for kap in range(4):
+        idx = [q for q in range(pick.size) if cands[pick[q]][0] == kap]
+        members = [cands[pick[q]][1] for q in idx]
+        wls.append(Workload(f"L2_1 sublattice {kap}", ops[kap], members, depths[idx].astype(np.int32), True,
+                            dict(L=L, levels=levels[kap], group=group, nodes=ops[kap].nn, sro=sro)))
+    return wls

@@ -108,6 +108,7 @@ def main():
     ap.add_argument("--no-kpm", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--kpm-moments", type=int, default=2048)
+    ap.add_argument("--no-l21", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the engine's multi-GPU form (chains dealt across GPUs, one all-gather of the "
@@ -185,6 +186,30 @@ def main():
         out["speedup_batched_vs_plain"] = round(value / plain, 2)
         out["plain_matches_batched"] = bool(np.max(np.abs(a1 - a)) <= 1e-12 * np.max(np.abs(a)))
         log(f"plain: {wall1 * 1e3:.0f} ms, {plain:.0f} chain-levels/s, batched/plain {value / plain:.2f}x")
+
+    if not args.no_l21:
+        # uneven chains over four sublattice operators (L2_1 partial disorder + SRO) in one batched run
+        t = time.perf_counter()
+        wls = W.l21_batch(L=4, levels=(2, 2, 2, 1), nchains=48, depth=(5, 12)) if args.quick else W.l21_batch()
+        groups = [(engine.GpuOperator(w.op), w.members, w.nlev) for w in wls]
+        tot_lev = int(sum(int(np.sum(w.nlev)) for w in wls))
+        log(f"l21: {[(w.op.nn, len(w.members)) for w in wls]} nodes/chains, {tot_lev} chain-levels, build "
+            f"{time.perf_counter() - t:.1f} s")
+        engine.chains_multi(groups, mode=2)  # warm-up (autotuning)
+        res = {}
+        for mode, name in ((2, "graph"), (0, "streams"), (1, "plain")):
+            t0 = time.perf_counter()
+            _, dms = engine.chains_multi(groups, mode=mode)
+            wall = time.perf_counter() - t0
+            res[name] = {"value": round(tot_lev / wall, 1), "wall_ms": round(wall * 1e3, 1), "device_ms": round(dms, 1)}
+        out["batched_uneven"] = {"workload": "L2_1 4 sublattices (1.0 / 0.5-0.5 / 0.4-0.6 / 0.3-0.3-0.4), SRO alpha_1, "
+                                             "2 spins, 256 chains of depth 20-60, levels (3, 3, 3, 2)",
+                                 "nodes": [w.op.nn for w in wls], "chains": [len(w.members) for w in wls],
+                                 "chain_levels": tot_lev, "unit": "chain-levels/s", **res,
+                                 "speedup_graph_vs_plain": round(res["graph"]["value"] / res["plain"]["value"], 2)}
+        log(f"l21: {res}")
+        for G, _, _ in groups:
+            G.close()
 
     if not args.no_kpm:
         Lk = 6 if args.quick else 40

@@ -73,6 +73,20 @@ int asr_gpu_chains_edc(asr_gpu_op* op, int nchain, const int* start_orb, const d
                        const int* frozen_sp, const double* conc, const double* z, const int* nlev, int project, int ldab,
                        double* a, double* b, int* mused, int mode, double* elapsed_ms);
 
+/* Chains on several operators at once (one group per operator, e.g. the start sublattices of an ordered or partially
+ * disordered compound: the per-class operator copies of chain_worker, src/asr_class_scf.f90:1093-1130).  Group g holds
+ * nchain[g] chains; every per-chain array is concatenated over the groups in order (params and pab in each operator's
+ * nsp, conc in its ncls x nsmax).  Members are sorted by depth inside a group and member tiles whose chains have all left
+ * are skipped.  mode 0: levels issued from the host onto one stream per group (groups overlap on the GPU); mode 2: the whole
+ * run captured into one HIP graph and launched once; mode 1: the plain form (one chain per launch sequence). */
+int asr_gpu_chains_edc_multi(int ngroup, asr_gpu_op* const* ops, const int* nchain, const int* start_orb,
+                             const double* params, const double* pab, const int* frozen_sp, const double* conc, const double* z,
+                             const int* nlev, int project, int ldab, double* a, double* b, int* mused, int mode,
+                             double* elapsed_ms);
+int asr_gpu_lanczos_real_multi(int ngroup, asr_gpu_op* const* ops, const int* nchain, const int* start_orb,
+                               const double* params, const int* frozen_sp, const double* conc, const int* nlev, double tiny,
+                               int ldab, double* a, double* b, int* mused, int mode, double* elapsed_ms);
+
 /* Per-kernel durations of the chain level step (HIP events around every launch, averaged over the levels): the chains of
  * asr_gpu_chains_edc (pab != NULL) or asr_gpu_lanczos_real (pab == NULL).  On return *nk (<= 16) launches per level,
  * kernel_ms[q] their mean duration and kind[q] their kind: 0 fused SpMM (S pass + flip terms), 1 flip-gather terms only,
