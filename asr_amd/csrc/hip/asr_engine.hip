@@ -1151,6 +1151,7 @@ int upload_members(const asr_gpu_op* op, const Members& M, MemBuffers& B, MemDev
   md.nb = M.nb; md.nbp = M.nbp; md.mt = M.mt; md.pair = M.pair; md.tact = nullptr;
   md.chimask = B.chim; md.wts = B.wts; md.fpar = B.fpar; md.frozen = B.frozen; md.dvec = B.dvec; md.z = B.z;
   md.dperm = B.dperm;
+  HCHECK(hipStreamSynchronize(op->stream));  // the tables are read from other streams too
   return 0;
 }
 
@@ -1202,15 +1203,15 @@ FArgs fargs0() {
 }
 
 template <class T>
-int launch_fused(const asr_gpu_op* op, const MemDev& md, const FArgs& fa) {
+int launch_fused(const asr_gpu_op* op, const MemDev& md, const FArgs& fa, hipStream_t strm) {
   dim3 grid = node_grid(op, md);
   size_t lds = fused_lds_bytes(op, sizeof(T));
-  if (timer_mark(op->stream, fa.xs ? KIND_SPMM : KIND_RAND, true)) return 1;
+  if (timer_mark(strm, fa.xs ? KIND_SPMM : KIND_RAND, true)) return 1;
   const OpDev od = dev_of(op);
   const double* bl = op->blk;
   if (md.mt == WAVE && md.pair && sizeof(T) == sizeof(cplx)) {
     grid.y = md.nbp / (WAVE / 2);
-#define ASR_PAIR(PI, NS) hipLaunchKernelGGL((k_pair<PI, NS>), grid, dim3(BLOCK), 0, op->stream, od, md, fa, bl, md.dperm)
+#define ASR_PAIR(PI, NS) hipLaunchKernelGGL((k_pair<PI, NS>), grid, dim3(BLOCK), 0, strm, od, md, fa, bl, md.dperm)
     if (op->piid && op->nslot == 18) ASR_PAIR(true, 18);
     else if (op->piid && op->nslot == 14) ASR_PAIR(true, 14);
     else if (op->piid) ASR_PAIR(true, 0);
@@ -1218,7 +1219,7 @@ int launch_fused(const asr_gpu_op* op, const MemDev& md, const FArgs& fa) {
     else ASR_PAIR(false, 0);
 #undef ASR_PAIR
   } else if (md.mt == WAVE) {
-#define ASR_NODE(PI, NS) hipLaunchKernelGGL((k_node<T, PI, NS>), grid, dim3(BLOCK), 0, op->stream, od, md, fa, bl, md.dperm)
+#define ASR_NODE(PI, NS) hipLaunchKernelGGL((k_node<T, PI, NS>), grid, dim3(BLOCK), 0, strm, od, md, fa, bl, md.dperm)
     if (op->piid && op->nslot == 18) ASR_NODE(true, 18);
     else if (op->piid && op->nslot == 14) ASR_NODE(true, 14);
     else if (op->piid) ASR_NODE(true, 0);
@@ -1226,10 +1227,10 @@ int launch_fused(const asr_gpu_op* op, const MemDev& md, const FArgs& fa) {
     else ASR_NODE(false, 0);
 #undef ASR_NODE
   } else {
-    hipLaunchKernelGGL(k_fused<T>, grid, dim3(BLOCK), lds, op->stream, dev_of(op), md, fa);
+    hipLaunchKernelGGL(k_fused<T>, grid, dim3(BLOCK), lds, strm, dev_of(op), md, fa);
   }
   HCHECK(hipGetLastError());
-  if (timer_mark(op->stream, 0, false)) return 1;
+  if (timer_mark(strm, 0, false)) return 1;
   return 0;
 }
 
@@ -1237,26 +1238,26 @@ int launch_fused(const asr_gpu_op* op, const MemDev& md, const FArgs& fa) {
 // lin: add X(z) x (complex members, energy-dependent diagonal of route edc).
 template <class T>
 int apply_chain_op(const asr_gpu_op* op, const Members& M, const MemDev& md, const T* x, T* y, T* t1, T* t2, T* h1, bool lin,
-                   const T* dotu, T* partial) {
+                   const T* dotu, T* partial, hipStream_t strm) {
   // h1 = h x~ = rand_c(x~) + D S D x~
   if (M.uniform_d) {
     FArgs f = fargs0();
     f.xs = x; f.xs_norm = 1; f.s_din = 1; f.s_dout = 1; f.s_coef = 1.0;
     f.xr[0] = x; f.qr[0] = QC; f.r_norm[0] = 1; f.r_coef[0] = 1.0; f.nr = 1;
     f.out = h1;
-    if (launch_fused<T>(op, md, f)) return 1;
+    if (launch_fused<T>(op, md, f, strm)) return 1;
   } else {
     FArgs f = fargs0();
     f.xr[0] = x; f.qr[0] = QD; f.r_norm[0] = 1; f.r_coef[0] = 1.0; f.nr = 1; f.out = t1;
-    if (launch_fused<T>(op, md, f)) return 1;
+    if (launch_fused<T>(op, md, f, strm)) return 1;
     f = fargs0();
     f.xs = t1; f.s_coef = 1.0; f.out = t2;
-    if (launch_fused<T>(op, md, f)) return 1;
+    if (launch_fused<T>(op, md, f, strm)) return 1;
     f = fargs0();
     f.xr[0] = x; f.qr[0] = QC; f.r_norm[0] = 1; f.r_coef[0] = 1.0;
     f.xr[1] = t2; f.qr[1] = QD; f.r_norm[1] = 0; f.r_coef[1] = 1.0; f.nr = 2;
     f.out = h1;
-    if (launch_fused<T>(op, md, f)) return 1;
+    if (launch_fused<T>(op, md, f, strm)) return 1;
   }
   // final: y = sq (rand_e(x~) + h1 - h(rand_o(h1)) [+ X(z) x~])
   FArgs f = fargs0();
@@ -1264,16 +1265,16 @@ int apply_chain_op(const asr_gpu_op* op, const Members& M, const MemDev& md, con
   if (!M.zero_o) {
     FArgs g = fargs0();
     g.xr[0] = h1; g.qr[0] = QO; g.r_norm[0] = 0; g.r_coef[0] = 1.0; g.nr = 1; g.out = t1;  // r = rand_o(h1)
-    if (launch_fused<T>(op, md, g)) return 1;
+    if (launch_fused<T>(op, md, g, strm)) return 1;
     if (M.uniform_d) {
       f.xs = t1; f.s_din = 1; f.s_dout = 1; f.s_coef = -1.0;
     } else {
       g = fargs0();
       g.xr[0] = t1; g.qr[0] = QD; g.r_coef[0] = 1.0; g.nr = 1; g.out = t2;  // t2 = rand_d(r)
-      if (launch_fused<T>(op, md, g)) return 1;
+      if (launch_fused<T>(op, md, g, strm)) return 1;
       g = fargs0();
       g.xs = t2; g.s_coef = 1.0; g.out = y;  // y used as scratch: S rand_d(r)
-      if (launch_fused<T>(op, md, g)) return 1;
+      if (launch_fused<T>(op, md, g, strm)) return 1;
       f.xr[nr] = y; f.qr[nr] = QD; f.r_norm[nr] = 0; f.r_coef[nr] = -1.0; ++nr;
       // the final kernel reads y at flip partners and writes y: use t2 as its output and copy back below
     }
@@ -1287,11 +1288,11 @@ int apply_chain_op(const asr_gpu_op* op, const Members& M, const MemDev& md, con
   f.dotu = dotu; f.partial = partial;
   if (!M.zero_o && !M.uniform_d) {
     f.out = t2;
-    if (launch_fused<T>(op, md, f)) return 1;
-    HCHECK(hipMemcpyAsync(y, t2, sizeof(T) * (size_t)op->nn * 9 * md.nbp, hipMemcpyDeviceToDevice, op->stream));
+    if (launch_fused<T>(op, md, f, strm)) return 1;
+    HCHECK(hipMemcpyAsync(y, t2, sizeof(T) * (size_t)op->nn * 9 * md.nbp, hipMemcpyDeviceToDevice, strm));
   } else {
     f.out = y;
-    if (launch_fused<T>(op, md, f)) return 1;
+    if (launch_fused<T>(op, md, f, strm)) return 1;
   }
   return 0;
 }
@@ -1329,9 +1330,9 @@ int autotune_tile(asr_gpu_op* op, int nchain, const int* start_orb, const double
     st |= hipEventCreate(&e0) != hipSuccess;
     st |= hipEventCreate(&e1) != hipSuccess;
     for (T* p : {x, y, t1, t2, h1}) st |= hipMemsetAsync(p, 0, nv * sizeof(T), op->stream) != hipSuccess;
-    if (!st) st = apply_chain_op<T>(op, M, md, x, y, t1, t2, h1, pab != nullptr, nullptr, nullptr);
+    if (!st) st = apply_chain_op<T>(op, M, md, x, y, t1, t2, h1, pab != nullptr, nullptr, nullptr, op->stream);
     if (!st) st |= hipEventRecord(e0, op->stream) != hipSuccess;
-    for (int r = 0; r < 2 && !st; ++r) st = apply_chain_op<T>(op, M, md, x, y, t1, t2, h1, pab != nullptr, nullptr, nullptr);
+    for (int r = 0; r < 2 && !st; ++r) st = apply_chain_op<T>(op, M, md, x, y, t1, t2, h1, pab != nullptr, nullptr, nullptr, op->stream);
     if (!st) st |= hipEventRecord(e1, op->stream) != hipSuccess;
     if (!st) st |= hipEventSynchronize(e1) != hipSuccess;
     float ms = 0.f;
@@ -1460,7 +1461,7 @@ struct ChainRun {
   int step() {
     md.tact = tact;
     if (g_timer) g_timer->ord = 0;
-    if (apply_chain_op<T>(op, M, md, B.u, B.w, B.t1, B.t2, B.h1, lin, B.u, B.part)) return 1;
+    if (apply_chain_op<T>(op, M, md, B.u, B.w, B.t1, B.t2, B.h1, lin, B.u, B.part, stream)) return 1;
     if (timer_mark(stream, KIND_FIN_A, true)) return 1;
     hipLaunchKernelGGL(k_fin_a<T>, dim3(M.nb), dim3(BLOCK), 0, stream, cd, M.nb, B.part, nbx, apply_mt, B.a, B.a_eff);
     if (timer_mark(stream, 0, false) || timer_mark(stream, KIND_UPDATE, true)) return 1;
@@ -1837,7 +1838,7 @@ int asr_gpu_apply_batched(asr_gpu_op* op, int nb, const int* start_orb, const do
   if (dalloc(&dx, nvp) || dalloc(&dy, nvp) || dalloc(&t1, nvp) || dalloc(&t2, nvp) || dalloc(&h1, nvp)) return 1;
   for (double* q : {dx, dy, t1, t2, h1}) HCHECK(hipMemsetAsync(q, 0, nvp * 8, op->stream));
   HCHECK(hipMemcpyAsync(dx, hx.data(), nv * 8, hipMemcpyHostToDevice, op->stream));
-  int st = apply_chain_op<double>(op, M, md, dx, dy, t1, t2, h1, false, nullptr, nullptr);
+  int st = apply_chain_op<double>(op, M, md, dx, dy, t1, t2, h1, false, nullptr, nullptr, op->stream);
   if (!st) {
     HCHECK(hipMemcpyAsync(hx.data(), dy, nv * 8, hipMemcpyDeviceToHost, op->stream));
     HCHECK(hipStreamSynchronize(op->stream));
@@ -1864,7 +1865,7 @@ int asr_gpu_apply_complex(asr_gpu_op* op, int nb, const int* start_orb, const do
   if (dalloc(&dx, nvp) || dalloc(&dy, nvp) || dalloc(&t1, nvp) || dalloc(&t2, nvp) || dalloc(&h1, nvp)) return 1;
   for (cplx* q : {dx, dy, t1, t2, h1}) HCHECK(hipMemsetAsync(q, 0, nvp * sizeof(cplx), op->stream));
   HCHECK(hipMemcpyAsync(dx, hx.data(), nv * sizeof(cplx), hipMemcpyHostToDevice, op->stream));
-  int st = apply_chain_op<cplx>(op, M, md, dx, dy, t1, t2, h1, pab != nullptr, nullptr, nullptr);
+  int st = apply_chain_op<cplx>(op, M, md, dx, dy, t1, t2, h1, pab != nullptr, nullptr, nullptr, op->stream);
   if (!st) {
     HCHECK(hipMemcpyAsync(hx.data(), dy, nv * sizeof(cplx), hipMemcpyDeviceToHost, op->stream));
     HCHECK(hipStreamSynchronize(op->stream));
