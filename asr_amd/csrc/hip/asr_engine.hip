@@ -94,6 +94,8 @@ struct MemDev {
   const double* dperm;        // nop x 9 x nbp: d at the permuted orbital, d[pi(l, g)]
   int pair;                   // complex members as lane pairs (member tile 64 only)
   const int* tact;            // per member tile: 0 = every member has left (the tile's blocks return at once), or null
+  const double* mtm;          // ncls x nsmax x nsmax x nsmax x nbp: class tables per member
+  const int* fsp;             // nbp: species of the frozen start site (-1: none)
 };
 
 __device__ inline void unpack_link(uint32_t e, int& r, int& g) {
@@ -540,6 +542,126 @@ __global__ void __launch_bounds__(BLOCK, 4) k_pair(OpDev op, MemDev md, FArgs fa
   }
 }
 
+// ------------------------------------------------------------------------------------------------ adjoint (real chains)
+// Node/member mapping of the elementwise adjoint kernels: as k_update (64 / mt nodes per wave, mt members).
+#define ASR_NODE_MEMBER                                                                              \
+  const int mt = md.mt, npw = WAVE / mt;                                                             \
+  const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x >> 6;                                \
+  const int m = blockIdx.y * mt + (lane & (mt - 1));                                                 \
+  const int i = (xcd_block(blockIdx.x, gridDim.x) * (BLOCK / WAVE) + wave) * npw + lane / mt;        \
+  const bool live = (i < op.nn) && (m < md.nb);
+
+// out = sum_t c_t[m] v_t (t < nt, c_t per member, or 1 when null), optional partial dots <out, p_q> (q < np)
+__global__ void __launch_bounds__(BLOCK) k_lincomb(OpDev op, MemDev md, double* out, int nt, const double* v0,
+                                                   const double* c0, const double* v1, const double* c1, const double* v2,
+                                                   const double* c2, const double* v3, const double* c3, int np,
+                                                   const double* p0, double* part0, const double* p1, double* part1) {
+  __shared__ double red[BLOCK];
+  ASR_NODE_MEMBER
+  double d0 = 0.0, d1 = 0.0;
+  if (live) {
+    const double* vv[4] = {v0, v1, v2, v3};
+    const double* cc[4] = {c0, c1, c2, c3};
+    double co[4];
+    for (int t = 0; t < 4; ++t) co[t] = (t < nt) ? (cc[t] ? cc[t][m] : 1.0) : 0.0;
+    for (int l = 0; l < 9; ++l) {
+      const size_t q = ((size_t)i * 9 + l) * md.nbp + m;
+      double y = 0.0;
+      for (int t = 0; t < nt; ++t) y = fma(co[t], vv[t][q], y);
+      out[q] = y;
+      if (np > 0) d0 = fma(p0[q], y, d0);
+      if (np > 1) d1 = fma(p1[q], y, d1);
+    }
+  }
+  if (np > 0) block_reduce_member<double>(d0, mt, red, part0 + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * mt + threadIdx.x);
+  if (np > 1) {
+    __syncthreads();
+    block_reduce_member<double>(d1, mt, red, part1 + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * mt + threadIdx.x);
+  }
+}
+
+// gradient contributions of one level, per member: with x = u_k, y = w_bar (normalised), x~ = x / sq, y~ = y / sq,
+// t2x = S D x~, t2y = S D y~ (amplitude), and R_sl the species-s, orbital-l projection of the random operators,
+//   dL/dc_sL += sum_i sq_i y_i . (R_sL x~)_i,   dL/dd_sL += sum_i sq_i^2 [(R_sL y~)_i . t2x_i + t2y_i . (R_sL x~)_i]
+// (the normalised operators are symmetric on the symmetric subspace).  L = l shell (s, p, d).  NSP species.
+template <int NSP>
+__global__ void __launch_bounds__(BLOCK) k_adj_grad(OpDev op, MemDev md, const double* x, const double* y, const double* t2x,
+                                                    const double* t2y, const int* spc, double* part) {
+  __shared__ double red[BLOCK];
+  ASR_NODE_MEMBER
+  double gc[NSP][3], gd[NSP][3];
+  for (int s = 0; s < NSP; ++s)
+    for (int L = 0; L < 3; ++L) { gc[s][L] = 0.0; gd[s][L] = 0.0; }
+  if (live) {
+    const int nbp = md.nbp, nsm = op.nsmax;
+    const uint32_t inf = op.info[i];
+    const double sqi = op.sq[i];
+    const uint64_t chim = md.chimask[m];
+    double xt[9], yt[9];
+    for (int l = 0; l < 9; ++l) {
+      xt[l] = ld(x, i, l, nbp, m) / sqi;
+      yt[l] = ld(y, i, l, nbp, m) / sqi;
+    }
+    // the frozen start site carries its species: there R_sl is the identity on orbital l for that species alone
+    const bool fz = ((inf >> 20) & 1u) && md.frozen[m];
+    const int l0 = inf & 15u, kap = (inf >> 4) & 255u;
+    const int nsc = op.nsc[kap];
+    for (int sl = 0; sl < (fz ? 1 : nsc); ++sl) {
+      const int s = fz ? md.fsp[m] : spc[kap * nsm + sl];
+      double rx[9], ry[9];
+      if (fz) {
+        for (int l = 0; l < 9; ++l) { rx[l] = xt[l]; ry[l] = yt[l]; }
+      } else {
+        const double w0 = md.mtm[((((size_t)kap * nsm + l0) * nsm + l0) * nsm + sl) * nbp + m];
+        for (int l = 0; l < 9; ++l) { rx[l] = w0 * xt[l]; ry[l] = w0 * yt[l]; }
+        const uint32_t* fl = op.flip + (size_t)i * nsm;
+        for (int k = 0; k < nsc; ++k) {
+          const uint32_t e = fl[k];
+          if (k == l0 || e == 0u) continue;
+          int r, g;
+          unpack_link(e, r, g);
+          const double wk = md.mtm[((((size_t)kap * nsm + l0) * nsm + k) * nsm + sl) * nbp + m];
+          const double chi = ((chim >> g) & 1ull) ? -1.0 : 1.0;
+          const double sqr = op.sq[r];
+          for (int l = 0; l < 9; ++l) {
+            const double cs = chi * op.sg[g * 9 + l] * wk / sqr;
+            const int pl = op.pi[g * 9 + l];
+            rx[l] = fma(cs, ld(x, r, pl, nbp, m), rx[l]);
+            ry[l] = fma(cs, ld(y, r, pl, nbp, m), ry[l]);
+          }
+        }
+      }
+      for (int l = 0; l < 9; ++l) {
+        const int L = l == 0 ? 0 : (l < 4 ? 1 : 2);
+        const double gcv = sqi * ld(y, i, l, nbp, m) * rx[l];
+        const double gdv = sqi * sqi * (ry[l] * ld(t2x, i, l, nbp, m) + ld(t2y, i, l, nbp, m) * rx[l]);
+        for (int q = 0; q < NSP; ++q)
+          if (q == s) { gc[q][L] += gcv; gd[q][L] += gdv; }
+      }
+    }
+  }
+  const size_t base = ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * mt;
+  const size_t plane = (size_t)gridDim.x * gridDim.y * mt;
+  for (int q = 0; q < NSP * 3; ++q) {
+    block_reduce_member<double>(gc[q / 3][q % 3], mt, red, part + (size_t)q * plane + base + threadIdx.x);
+    __syncthreads();
+    block_reduce_member<double>(gd[q / 3][q % 3], mt, red, part + (size_t)(NSP * 3 + q) * plane + base + threadIdx.x);
+    __syncthreads();
+  }
+}
+
+// per member: the forward coefficients of level k for a recomputed level (a_k, b_{k-1}, 1 / b_k)
+__global__ void k_adj_coef(int nb, int ld, int k, const double* a, const double* b, double* a_eff, double* b_prev,
+                           double* inv_b, double* nega, double* negb_prev) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nb) return;
+  a_eff[m] = a[(size_t)m * ld + k];
+  b_prev[m] = k > 0 ? b[(size_t)m * ld + k - 1] : 0.0;
+  inv_b[m] = 1.0 / b[(size_t)m * ld + k];
+  nega[m] = -a[(size_t)m * ld + k];
+  negb_prev[m] = -b_prev[m];
+}
+
 // chain state of a batch
 struct ChainDev {
   int* level;        // current level (device scalar, so a captured level step can be replayed)
@@ -588,6 +710,44 @@ __device__ T block_sum_partials(const T* part, int nbx, int mt, int m, T* lds) {
 }
 
 // a_k of every member (one block per member); coefficient used by the update (0 for members that left)
+// b_bar_k = dL/db_k + carry - <u_bar_{k+1}, u_{k+1}> / b_k;  r_bar = u_bar_{k+1} / b_k + b_bar_k u_{k+1}
+__global__ void __launch_bounds__(BLOCK) k_adj_fin1(int nb, int ld, int k, const double* part, int nbx, int mt,
+                                                    const double* b, const double* Lb, const double* cbb, double* alpha,
+                                                    double* beta) {
+  __shared__ double lds[BLOCK / WAVE];
+  const int m = blockIdx.x;
+  double d1 = block_sum_partials<double>(part, nbx, mt, m, lds);
+  if (threadIdx.x != 0) return;
+  const double bk = b[(size_t)m * ld + k];
+  alpha[m] = 1.0 / bk;
+  beta[m] = Lb[(size_t)m * ld + k] + cbb[m] - d1 / bk;
+}
+
+// a_bar_k = dL/da_k - <r_bar, u_k>;  the carry to b_bar_{k-1}: -<r_bar, u_{k-1}>
+__global__ void __launch_bounds__(BLOCK) k_adj_fin2(int nb, int ld, int k, const double* part2, const double* part3, int nbx,
+                                                    int mt, const double* La, double* abar, double* cbb) {
+  __shared__ double lds[BLOCK / WAVE];
+  const int m = blockIdx.x;
+  double d2 = block_sum_partials<double>(part2, nbx, mt, m, lds);
+  __syncthreads();
+  double d3 = block_sum_partials<double>(part3, nbx, mt, m, lds);
+  if (threadIdx.x != 0) return;
+  abar[m] = La[(size_t)m * ld + k] - d2;
+  cbb[m] = -d3;
+}
+
+// g[m][q] += sum of the level's partials of gradient component q (fixed order)
+__global__ void __launch_bounds__(BLOCK) k_adj_fin_grad(int nb, int nq, const double* part, int nbx, int mt, size_t plane,
+                                                        double* g) {
+  __shared__ double lds[BLOCK / WAVE];
+  const int m = blockIdx.x;
+  for (int q = 0; q < nq; ++q) {
+    double v = block_sum_partials<double>(part + (size_t)q * plane, nbx, mt, m, lds);
+    if (threadIdx.x == 0) g[(size_t)m * nq + q] += v;
+    __syncthreads();
+  }
+}
+
 template <class T>
 __global__ void __launch_bounds__(BLOCK) k_fin_a(ChainDev cd, int nb, const T* part, int nbx, int mt, T* a, T* a_eff) {
   __shared__ T lds[BLOCK / WAVE];
@@ -1002,8 +1162,8 @@ struct Members {
   int nb = 0, nbp = 0, mt = 1, pair = 0;
   bool uniform_d = true, zero_o = true, zero_e = true, complex_lin = false;
   std::vector<uint64_t> chim;
-  std::vector<double> wts, fpar, dvec, dperm;
-  std::vector<int> frozen;
+  std::vector<double> wts, fpar, dvec, dperm, mtm;  // mtm: ncls x nsmax x nsmax x nsmax x nbp (class tables per member)
+  std::vector<int> frozen, fsp;
   std::vector<cplx> z;
 };
 
@@ -1034,7 +1194,9 @@ int prepare_members(const asr_gpu_op* op, int nchain, const int* start_orb, cons
   M.fpar.assign((size_t)NQ * 9 * nbp, 0.0);
   M.dvec.assign((size_t)9 * nbp, 0.0);
   M.dperm.assign((size_t)op->nop * 9 * nbp, 0.0);
+  M.mtm.assign((size_t)ncls * nsm * nsm * nsm * nbp, 0.0);
   M.frozen.assign(nbp, 0);
+  M.fsp.assign(nbp, -1);
   M.z.assign(nbp, cplx{0.0, 0.0});
   M.complex_lin = (pab != nullptr);
   std::vector<double> mt((size_t)ncls * nsm * nsm * nsm);
@@ -1104,6 +1266,8 @@ int prepare_members(const asr_gpu_op* op, int nchain, const int* start_orb, cons
       }
     }
     M.frozen[m] = (frozen_sp && frozen_sp[m] >= 0) ? 1 : 0;
+    M.fsp[m] = M.frozen[m] ? frozen_sp[m] : -1;
+    for (size_t q = 0; q < mt.size(); ++q) M.mtm[q * nbp + m] = mt[q];
     const double* dp = pm + (size_t)nsp * 9;
     for (int s = 1; s < nsp; ++s)
       for (int l = 0; l < 9; ++l)
@@ -1122,10 +1286,12 @@ int prepare_members(const asr_gpu_op* op, int nchain, const int* start_orb, cons
 
 struct MemBuffers {
   uint64_t* chim = nullptr;
-  double *wts = nullptr, *fpar = nullptr, *dvec = nullptr, *dperm = nullptr;
-  int* frozen = nullptr;
+  double *wts = nullptr, *fpar = nullptr, *dvec = nullptr, *dperm = nullptr, *mtm = nullptr;
+  int *frozen = nullptr, *fsp = nullptr;
   cplx* z = nullptr;
   ~MemBuffers() {
+    if (mtm) (void)hipFree(mtm);
+    if (fsp) (void)hipFree(fsp);
     if (dperm) (void)hipFree(dperm);
     if (chim) (void)hipFree(chim);
     if (wts) (void)hipFree(wts);
@@ -1139,8 +1305,10 @@ struct MemBuffers {
 int upload_members(const asr_gpu_op* op, const Members& M, MemBuffers& B, MemDev& md) {
   if (dalloc(&B.chim, M.chim.size()) || dalloc(&B.wts, M.wts.size()) || dalloc(&B.fpar, M.fpar.size()) ||
       dalloc(&B.dvec, M.dvec.size()) || dalloc(&B.frozen, M.frozen.size()) || dalloc(&B.z, M.z.size()) ||
-      dalloc(&B.dperm, M.dperm.size()))
+      dalloc(&B.dperm, M.dperm.size()) || dalloc(&B.mtm, M.mtm.size()) || dalloc(&B.fsp, M.fsp.size()))
     return 1;
+  HCHECK(hipMemcpyAsync(B.fsp, M.fsp.data(), M.fsp.size() * 4, hipMemcpyHostToDevice, op->stream));
+  HCHECK(hipMemcpyAsync(B.mtm, M.mtm.data(), M.mtm.size() * 8, hipMemcpyHostToDevice, op->stream));
   HCHECK(hipMemcpyAsync(B.dperm, M.dperm.data(), M.dperm.size() * 8, hipMemcpyHostToDevice, op->stream));
   HCHECK(hipMemcpyAsync(B.chim, M.chim.data(), M.chim.size() * 8, hipMemcpyHostToDevice, op->stream));
   HCHECK(hipMemcpyAsync(B.wts, M.wts.data(), M.wts.size() * 8, hipMemcpyHostToDevice, op->stream));
@@ -1151,6 +1319,8 @@ int upload_members(const asr_gpu_op* op, const Members& M, MemBuffers& B, MemDev
   md.nb = M.nb; md.nbp = M.nbp; md.mt = M.mt; md.pair = M.pair; md.tact = nullptr;
   md.chimask = B.chim; md.wts = B.wts; md.fpar = B.fpar; md.frozen = B.frozen; md.dvec = B.dvec; md.z = B.z;
   md.dperm = B.dperm;
+  md.mtm = B.mtm;
+  md.fsp = B.fsp;
   HCHECK(hipStreamSynchronize(op->stream));  // the tables are read from other streams too
   return 0;
 }
@@ -1665,6 +1835,182 @@ int run_multi(int ngroup, asr_gpu_op* const* ops, const int* ngch, const int* st
   return status;
 }
 
+// Adjoint of the real Lanczos recursion (the reference has none; the forward is lanczos, src/reference/asr_recursion.f90:17-58)
+// for chains of equal depth nlev on H = c + d S d (o = E_nu = 0).  Forward: the batched chains, (u_k, u_{k-1}) checkpointed
+// every ckpt levels.  Reverse, segment by segment: the levels of the segment are recomputed from its checkpoint (keeping
+// u_k, w_k = H u_k and S D u~_k), then for k from the segment's end down to its start
+//   b_bar_k = dL/db_k - <r_bar_{k+1}, u_k> - <u_bar_{k+1}, u_{k+1}> / b_k,   r_bar_k = u_bar_{k+1} / b_k + b_bar_k u_{k+1}
+//   a_bar_k = dL/da_k - <r_bar_k, u_k>,   w_bar_k = r_bar_k + a_bar_k u_k
+//   u_bar_k = -b_k r_bar_{k+1} - a_k r_bar_k + a_bar_k w_k + H w_bar_k,   grad += <w_bar_k, (dH/dlambda) u_k>
+// (r_k = w_k - a_k u_k - b_{k-1} u_{k-1}, u_{k+1} = r_k / b_k; u_bar_nlev = 0).  dL/da, dL/db: nchain x nlev.
+// grad: nchain x 2 x nsp x 3 (dL/dc, dL/dd per species and l shell).  Cost: one forward, one recomputed forward and one
+// operator application per level on the adjoint vector.
+int run_adjoint(asr_gpu_op* op, int nchain, const int* start_orb, const double* params, const int* frozen_sp,
+                const double* conc, int nlev, int ckpt, const double* La, const double* Lb, double* a_out, double* b_out,
+                double* grad, double* fwd_ms, double* adj_ms) {
+  const int nsp = op->nsp;
+  if (nsp > 5) return fail("lanczos_adjoint: at most 5 species");
+  if (ckpt < 1) ckpt = 8;
+  for (int c = 0; c < nchain; ++c)
+    for (int q = 2 * nsp * 9; q < 4 * nsp * 9; ++q)
+      if (params[(size_t)c * 4 * nsp * 9 + q] != 0.0)
+        return fail("lanczos_adjoint: the adjoint is written for H = c + d S d (o = E_nu = 0)");
+  std::vector<int> nl(nchain, nlev);
+  ChainRun<double> R;
+  if (R.setup(op, op->stream, nchain, start_orb, params, nullptr, frozen_sp, conc, nullptr, nl.data(), 0, nlev,
+              op->tile_pref, 1e-300, true))
+    return 1;
+  R.M.uniform_d = false;  // every application materialises S D x~ (the general path)
+  hipStream_t st = R.stream;
+  const size_t nv = R.nv;
+  const int nseg = (nlev + ckpt - 1) / ckpt;
+  const MemDev md0 = R.md;
+  std::vector<double*> bufs;
+  auto vec = [&](double** p) -> int {
+    if (dalloc(p, nv)) return 1;
+    bufs.push_back(*p);
+    return hipMemsetAsync(*p, 0, nv * 8, st) != hipSuccess;
+  };
+  struct Free {
+    std::vector<double*>* b;
+    ~Free() {
+      for (double* p : *b) (void)hipFree(p);
+    }
+  } freer{&bufs};
+  std::vector<double*> CK(2 * nseg), U(ckpt + 2), W(ckpt), T2(ckpt);
+  for (auto& p : CK) if (vec(&p)) return 1;
+  for (auto& p : U) if (vec(&p)) return 1;
+  for (auto& p : W) if (vec(&p)) return 1;
+  for (auto& p : T2) if (vec(&p)) return 1;
+  double *rbar, *wbar, *hw, *t2y, *ubar, *unext, *carry, *carry2, *wtmp;
+  for (double** p : {&rbar, &wbar, &hw, &t2y, &ubar, &unext, &carry, &carry2, &wtmp}) if (vec(p)) return 1;
+  const dim3 grid = R.grid;
+  const int nbx = grid.x, mt = R.M.mt, nbp = R.M.nbp;
+  const size_t plane = (size_t)grid.x * grid.y * mt;
+  const int nq = 2 * nsp * 3;
+  double *P1, *P2, *P3, *PG, *coef, *gacc, *dLa, *dLb;
+  for (double** p : {&P1, &P2, &P3}) {
+    if (dalloc(p, plane)) return 1;
+    bufs.push_back(*p);
+  }
+  if (dalloc(&PG, plane * nq) || dalloc(&coef, (size_t)12 * nbp) || dalloc(&gacc, (size_t)nbp * nq) ||
+      dalloc(&dLa, (size_t)nbp * nlev) || dalloc(&dLb, (size_t)nbp * nlev))
+    return 1;
+  for (double* p : {PG, coef, gacc, dLa, dLb}) bufs.push_back(p);
+  double *a_eff = coef, *b_prev = coef + nbp, *inv_b = coef + 2 * nbp, *nega = coef + 3 * nbp, *negb_prev = coef + 4 * nbp,
+         *alpha = coef + 5 * nbp, *beta = coef + 6 * nbp, *abar = coef + 7 * nbp, *cbb = coef + 8 * nbp;
+  HCHECK(hipMemsetAsync(coef, 0, (size_t)12 * nbp * 8, st));
+  HCHECK(hipMemsetAsync(gacc, 0, (size_t)nbp * nq * 8, st));
+  {
+    // loss derivatives in the sorted member order
+    std::vector<double> la((size_t)nbp * nlev, 0.0), lb((size_t)nbp * nlev, 0.0);
+    for (int q = 0; q < nchain; ++q) {
+      std::memcpy(&la[(size_t)q * nlev], La + (size_t)R.order[q] * nlev, 8 * nlev);
+      std::memcpy(&lb[(size_t)q * nlev], Lb + (size_t)R.order[q] * nlev, 8 * nlev);
+    }
+    HCHECK(hipMemcpyAsync(dLa, la.data(), la.size() * 8, hipMemcpyHostToDevice, st));
+    HCHECK(hipMemcpyAsync(dLb, lb.data(), lb.size() * 8, hipMemcpyHostToDevice, st));
+  }
+  int* spc_d = nullptr;
+  if (dalloc(&spc_d, (size_t)op->ncls * op->nsmax)) return 1;
+  HCHECK(hipMemcpy(spc_d, op->spc_h.data(), op->spc_h.size() * 4, hipMemcpyHostToDevice));
+  struct FreeI {
+    int* p;
+    ~FreeI() { (void)hipFree(p); }
+  } freei{spc_d};
+  const OpDev od = dev_of(op);
+  auto lincomb = [&](double* out, int nt, const double* v0, const double* c0, const double* v1, const double* c1,
+                     const double* v2, const double* c2, const double* v3, const double* c3, int np, const double* p0,
+                     double* q0, const double* p1, double* q1) -> int {
+    hipLaunchKernelGGL(k_lincomb, grid, dim3(BLOCK), 0, st, od, md0, out, nt, v0, c0, v1, c1, v2, c2, v3, c3, np, p0, q0, p1,
+                       q1);
+    HCHECK(hipGetLastError());
+    return 0;
+  };
+  hipEvent_t e0, e1, e2;
+  HCHECK(hipEventCreate(&e0));
+  HCHECK(hipEventCreate(&e1));
+  HCHECK(hipEventCreate(&e2));
+  // ---- forward with checkpoints
+  HCHECK(hipEventRecord(e0, st));
+  for (int k = 0; k < nlev; ++k) {
+    if (k % ckpt == 0) {
+      HCHECK(hipMemcpyAsync(CK[2 * (k / ckpt)], R.B.u, nv * 8, hipMemcpyDeviceToDevice, st));
+      HCHECK(hipMemcpyAsync(CK[2 * (k / ckpt) + 1], R.B.up, nv * 8, hipMemcpyDeviceToDevice, st));
+    }
+    if (R.step()) return 1;
+  }
+  HCHECK(hipEventRecord(e1, st));
+  // ---- reverse
+  HCHECK(hipMemsetAsync(unext, 0, nv * 8, st));
+  HCHECK(hipMemsetAsync(carry, 0, nv * 8, st));
+  for (int seg = nseg - 1; seg >= 0; --seg) {
+    const int k0 = seg * ckpt, k1 = std::min(nlev, k0 + ckpt) - 1;
+    // U[0] = u_{k0 - 1}, U[1 + j] = u_{k0 + j}
+    HCHECK(hipMemcpyAsync(U[1], CK[2 * seg], nv * 8, hipMemcpyDeviceToDevice, st));
+    HCHECK(hipMemcpyAsync(U[0], CK[2 * seg + 1], nv * 8, hipMemcpyDeviceToDevice, st));
+    for (int k = k0; k <= k1; ++k) {
+      const int j = k - k0;
+      hipLaunchKernelGGL(k_adj_coef, dim3((nbp + 63) / 64), dim3(64), 0, st, R.M.nb, nlev, k, R.B.a, R.B.b, a_eff, b_prev,
+                         inv_b, nega, negb_prev);
+      if (apply_chain_op<double>(op, R.M, md0, U[1 + j], W[j], R.B.t1, T2[j], R.B.h1, false, nullptr, nullptr, st)) return 1;
+      // u_{k+1} = (w - a u_k - b_{k-1} u_{k-1}) / b_k with the forward's own kernels (bitwise the forward's vectors)
+      HCHECK(hipMemcpyAsync(U[2 + j], W[j], nv * 8, hipMemcpyDeviceToDevice, st));
+      hipLaunchKernelGGL(k_update<double>, grid, dim3(BLOCK), 0, st, od, md0, U[2 + j], U[1 + j], U[j], a_eff, b_prev, 0, P1);
+      hipLaunchKernelGGL(k_scale<double>, R.sgrid, dim3(256), 0, st, U[2 + j], inv_b, nv, nbp);
+      HCHECK(hipGetLastError());
+    }
+    for (int k = k1; k >= k0; --k) {
+      const int j = k - k0;
+      hipLaunchKernelGGL(k_adj_coef, dim3((nbp + 63) / 64), dim3(64), 0, st, R.M.nb, nlev, k, R.B.a, R.B.b, a_eff, b_prev,
+                         inv_b, nega, negb_prev);
+      // d1 = <u_bar_{k+1}, u_{k+1}>  ->  b_bar_k, then r_bar with <r_bar, u_k>, <r_bar, u_{k-1}>  (wtmp: a copy, unused)
+      if (lincomb(wtmp, 1, unext, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 1, U[2 + j], P1, nullptr,
+                  nullptr))
+        return 1;
+      hipLaunchKernelGGL(k_adj_fin1, dim3(R.M.nb), dim3(BLOCK), 0, st, R.M.nb, nlev, k, P1, nbx, mt, R.B.b, dLb, cbb, alpha,
+                         beta);
+      if (lincomb(rbar, 2, unext, alpha, U[2 + j], beta, nullptr, nullptr, nullptr, nullptr, 2, U[1 + j], P2, U[j], P3))
+        return 1;
+      hipLaunchKernelGGL(k_adj_fin2, dim3(R.M.nb), dim3(BLOCK), 0, st, R.M.nb, nlev, k, P2, P3, nbx, mt, dLa, abar, cbb);
+      // w_bar = r_bar + a_bar u_k;  H w_bar (keeping S D y~)
+      if (lincomb(wbar, 2, rbar, nullptr, U[1 + j], abar, nullptr, nullptr, nullptr, nullptr, 0, nullptr, nullptr, nullptr,
+                  nullptr))
+        return 1;
+      if (apply_chain_op<double>(op, R.M, md0, wbar, hw, R.B.t1, t2y, R.B.h1, false, nullptr, nullptr, st)) return 1;
+      switch (nsp) {
+        case 1: hipLaunchKernelGGL(k_adj_grad<1>, grid, dim3(BLOCK), 0, st, od, md0, U[1 + j], wbar, T2[j], t2y, spc_d, PG); break;
+        case 2: hipLaunchKernelGGL(k_adj_grad<2>, grid, dim3(BLOCK), 0, st, od, md0, U[1 + j], wbar, T2[j], t2y, spc_d, PG); break;
+        case 3: hipLaunchKernelGGL(k_adj_grad<3>, grid, dim3(BLOCK), 0, st, od, md0, U[1 + j], wbar, T2[j], t2y, spc_d, PG); break;
+        case 4: hipLaunchKernelGGL(k_adj_grad<4>, grid, dim3(BLOCK), 0, st, od, md0, U[1 + j], wbar, T2[j], t2y, spc_d, PG); break;
+        default: hipLaunchKernelGGL(k_adj_grad<5>, grid, dim3(BLOCK), 0, st, od, md0, U[1 + j], wbar, T2[j], t2y, spc_d, PG); break;
+      }
+      hipLaunchKernelGGL(k_adj_fin_grad, dim3(R.M.nb), dim3(BLOCK), 0, st, R.M.nb, nq, PG, nbx, mt, plane, gacc);
+      // u_bar_k = carry - a_k r_bar + a_bar w_k + H w_bar;  carry for u_{k-1}: -b_{k-1} r_bar
+      if (lincomb(ubar, 4, carry, nullptr, rbar, nega, W[j], abar, hw, nullptr, 0, nullptr, nullptr, nullptr, nullptr) ||
+          lincomb(carry2, 1, rbar, negb_prev, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0, nullptr, nullptr, nullptr,
+                  nullptr))
+        return 1;
+      std::swap(unext, ubar);
+      std::swap(carry, carry2);
+    }
+  }
+  HCHECK(hipEventRecord(e2, st));
+  HCHECK(hipGetLastError());
+  HCHECK(hipEventSynchronize(e2));
+  float f1 = 0.f, f2 = 0.f;
+  HCHECK(hipEventElapsedTime(&f1, e0, e1));
+  HCHECK(hipEventElapsedTime(&f2, e1, e2));
+  if (fwd_ms) *fwd_ms = f1;
+  if (adj_ms) *adj_ms = f2;
+  for (hipEvent_t e : {e0, e1, e2}) (void)hipEventDestroy(e);
+  std::vector<double> g((size_t)nbp * nq);
+  HCHECK(hipMemcpy(g.data(), gacc, g.size() * 8, hipMemcpyDeviceToHost));
+  for (int q = 0; q < nchain; ++q) std::memcpy(grad + (size_t)R.order[q] * nq, &g[(size_t)q * nq], 8 * nq);
+  std::vector<int> mu(nchain);
+  return R.finish(a_out, b_out, mu.data());
+}
+
 // host layout [member][dim] <-> device layout [(node*9 + l) * nbp + member]
 template <class T>
 void to_dev_layout(const T* host, std::vector<T>& dev, int nb, int nbp, size_t dim) {
@@ -1906,6 +2252,13 @@ int asr_gpu_lanczos_real_multi(int ngroup, asr_gpu_op* const* ops, const int* nc
   if (ngroup <= 0) return fail("lanczos_multi: no groups");
   return run_multi<double>(ngroup, ops, nchain, start_orb, params, nullptr, frozen_sp, conc, nullptr, nlev, 0, ldab, a, b,
                            mused, mode, tiny > 0 ? tiny : 1e-12, true, elapsed_ms);
+}
+
+int asr_gpu_lanczos_adjoint(asr_gpu_op* op, int nchain, const int* start_orb, const double* params, const int* frozen_sp,
+                            const double* conc, int nlev, int ckpt, const double* dl_da, const double* dl_db, double* a,
+                            double* b, double* grad, double* fwd_ms, double* adj_ms) {
+  if (nchain <= 0 || nlev <= 0) return fail("lanczos_adjoint: no chains");
+  return run_adjoint(op, nchain, start_orb, params, frozen_sp, conc, nlev, ckpt, dl_da, dl_db, a, b, grad, fwd_ms, adj_ms);
 }
 
 // Per-kernel durations of the chain level step: the same chains as asr_gpu_chains_edc (pab != NULL) or

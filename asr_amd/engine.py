@@ -48,6 +48,7 @@ def lib() -> C.CDLL:
         L.asr_gpu_chains_edc.argtypes = [P, C.c_int, P, P, P, P, P, P, P, C.c_int, C.c_int, P, P, P, C.c_int, P]
         L.asr_gpu_chains_edc_multi.argtypes = [C.c_int, P, P, P, P, P, P, P, P, P, C.c_int, C.c_int, P, P, P, C.c_int, P]
         L.asr_gpu_lanczos_real_multi.argtypes = [C.c_int, P, P, P, P, P, P, P, C.c_double, C.c_int, P, P, P, C.c_int, P]
+        L.asr_gpu_lanczos_adjoint.argtypes = [P, C.c_int, P, P, P, P, C.c_int, C.c_int, P, P, P, P, P, P, P]
         L.asr_gpu_profile_chains.argtypes = [P, C.c_int, P, P, P, P, P, P, P, C.c_int, P, P, P]
         L.asr_gpu_real_create.argtypes = [C.c_int, C.c_int, P, C.c_int, P, P, P, P, C.POINTER(C.c_void_p)]
         L.asr_gpu_real_destroy.argtypes = [P]
@@ -175,6 +176,25 @@ class GpuOperator:
         _check(lib().asr_gpu_lanczos_real(self.h, nch, _p(s), _p(p), _p(f), _p(c), _p(nlev), float(tiny), ld, _p(a), _p(b),
                                           _p(mu), int(mode), _p(ms)), "asr_gpu_lanczos_real")
         return a, b, mu, float(ms[0])
+
+    def lanczos_adjoint(self, members: list[Member], nlev: int, dl_da: np.ndarray, dl_db: np.ndarray, ckpt: int = 8):
+        """Forward real chains of depth nlev and the gradient of L(a, b) with respect to (c, d) of every species and l shell.
+
+        dl_da, dl_db: nchain x nlev.  Returns a, b (nchain x nlev), grad (nchain x 2 x nsp x 3: dL/dc, dL/dd) and the device
+        times (forward ms, reverse ms)."""
+        nch = len(members)
+        nlev = int(nlev)
+        la = _f64(np.broadcast_to(np.asarray(dl_da, np.float64), (nch, nlev)))
+        lb = _f64(np.broadcast_to(np.asarray(dl_db, np.float64), (nch, nlev)))
+        a = np.zeros((nch, nlev))
+        b = np.zeros((nch, nlev))
+        g = np.zeros((nch, 2, self.op.nsp, 3))
+        fm = np.zeros(1)
+        am = np.zeros(1)
+        s, p, f, c, _, _ = pack_members(self.op, members)
+        _check(lib().asr_gpu_lanczos_adjoint(self.h, nch, _p(s), _p(p), _p(f), _p(c), nlev, int(ckpt), _p(la), _p(lb), _p(a),
+                                             _p(b), _p(g), _p(fm), _p(am)), "asr_gpu_lanczos_adjoint")
+        return a, b, g, (float(fm[0]), float(am[0]))
 
     def chains_edc(self, members: list[Member], nlev, project: bool = True, mode: int = 0):
         """Complex-symmetric chains of route edc (edc_green / edc_green_blk, depth_tol = 0)."""

@@ -9,6 +9,9 @@
     l21_batch         4-sublattice L2_1-type bcc cell with partial disorder (1.0 / 0.5-0.5 / 0.4-0.6 / 0.3-0.3-0.4), one
                       operator per start sublattice, Warren-Cowley alpha_1 short-range order of the start site's first shell,
                       2 spins, chains of uneven depth
+    ternary_adjoint   the ternary operator with real chains on H = c + d S d: 18 shared parameters (c, d per species and l),
+                      each chain with its own fixed offset of c (its potential set), start site frozen to each species,
+                      the 9 start orbitals; gradients of a contour objective through the chains
     kpm_supercell     fcc supercell of L^3 conventional cells (4 L^3 sites, 9 orbitals), seeded random ternary,
                       h = c + d S d with the same blocks
 Species parameters are drawn per (species, l) and spread over the orbitals of each l (s | p x3 | d x5).
@@ -129,6 +132,34 @@ def ternary_edc(L: int = 8, levels: int = 4, depth: int = 30, seed: int = 2, gro
     members = [edc_member(spins[sp], cen, l0, z) for sp in range(2) for cen in range(3) for l0 in range(9)]
     return Workload("fcc ternary edc", op, members, np.full(len(members), depth, np.int32), True,
                     dict(L=L, levels=levels, group=group, seed=seed, nodes=op.nn, z=z))
+
+
+def ternary_adjoint(L: int = 8, levels: int = 4, nchains: int = 256, depth: int = 40, seed: int = 5, group: str = "d2h",
+                    conc=(0.66, 0.14, 0.20), reorder: bool = True, offset: float = 0.02) -> Workload:
+    """Real chains of the ternary operator for gradients: theta = (c, d) of shape (2, 3 species, 3 l shells).  Chain j
+    starts at orbital j % 9 with the start site frozen to species (j // 9) % 3 and adds its own fixed offset to c."""
+    rng = np.random.default_rng(seed)
+    g, block, block0 = graph_and_blocks("fcc", L)
+    cls = np.zeros(g.n, np.int32)
+    perm, mats = S.class_ops(g.coord, 2 * L, cls, group)
+    dmat = S.orbital_rep(mats)
+    fl = np.ones(g.n, np.uint8)
+    fl[0] = 0
+    op = build_operator(g, cls, [3], [[0, 1, 2]], [list(conc)], perm, dmat, 0, levels, maxrank=1, flippable=fl,
+                        block=block[None], block0=block0[None], nsp=3)
+    if reorder:
+        op = reorder_nodes(op)
+    theta = np.stack([rng.uniform(-0.5, 0.5, (3, 3)), rng.uniform(0.05, 0.15, (3, 3))])
+    dc = rng.uniform(-offset, offset, (nchains, 3, 3))
+    members = []
+    for j in range(nchains):
+        c = per_orbital(theta[0] + dc[j])
+        d = per_orbital(theta[1])
+        z = np.zeros((3, 9))
+        members.append(Member(l0=j % 9, c=c, d=d, o=z, e=z, frozen=(j // 9) % 3))
+    return Workload("fcc ternary adjoint", op, members, np.full(nchains, depth, np.int32), False,
+                    dict(L=L, levels=levels, group=group, seed=seed, nodes=op.nn, theta=theta, c_offset=dc,
+                         conc=np.asarray(conc)))
 
 
 def brick_order(g: S.Graph, brick: int = 4) -> S.Graph:

@@ -87,6 +87,20 @@ int asr_gpu_lanczos_real_multi(int ngroup, asr_gpu_op* const* ops, const int* nc
                                const double* params, const int* frozen_sp, const double* conc, const int* nlev, double tiny,
                                int ldab, double* a, double* b, int* mused, int mode, double* elapsed_ms);
 
+/* Gradient of a loss L(a, b) of real Lanczos chains with respect to the chain parameters (asr_gpu_adjoint of the
+ * replacement interface).  The reference has no adjoint: its nearest analogue is the Hellmann-Feynman trace of EPI,
+ * dOmega/dlambda = -(1/pi) Im sum_j w_j Tr[G(z_j) dH/dlambda] (src/asr_epi.f90:14-28), for site-0 parameters only.  Here the
+ * forward is asr_gpu_lanczos_real (lanczos, src/reference/asr_recursion.f90:17-58) for nchain chains of depth nlev on
+ * H = c + d S d (the o and e parameter rows must be zero), with the Lanczos vectors checkpointed every ckpt levels
+ * (0 = 8); the reverse sweep recomputes each segment from its checkpoint and runs the adjoint three-term recursion with
+ * one operator application per level.  dl_da, dl_db: nchain x nlev (dL/da_n, dL/db_n of every chain).  a, b: nchain x
+ * nlev, the forward's coefficients.  grad: nchain x 2 x nsp x 3, dL/dc and dL/dd of each species summed over the l shell
+ * (s, p, d) of the nine orbitals, per chain (sum over chains for parameters shared between chains).  fwd_ms, adj_ms:
+ * device time of the forward and of the reverse sweep. */
+int asr_gpu_lanczos_adjoint(asr_gpu_op* op, int nchain, const int* start_orb, const double* params, const int* frozen_sp,
+                            const double* conc, int nlev, int ckpt, const double* dl_da, const double* dl_db, double* a,
+                            double* b, double* grad, double* fwd_ms, double* adj_ms);
+
 /* Per-kernel durations of the chain level step (HIP events around every launch, averaged over the levels): the chains of
  * asr_gpu_chains_edc (pab != NULL) or asr_gpu_lanczos_real (pab == NULL).  On return *nk (<= 16) launches per level,
  * kernel_ms[q] their mean duration and kind[q] their kind: 0 fused SpMM (S pass + flip terms), 1 flip-gather terms only,
