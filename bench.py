@@ -109,6 +109,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--kpm-moments", type=int, default=2048)
     ap.add_argument("--no-l21", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the engine's multi-GPU form (chains dealt across GPUs, one all-gather of the "
@@ -232,13 +233,52 @@ def main():
         wk = time.perf_counter() - t0
         kby, kfl = kpm_kernel_cost(g.n, g.nbr.shape[1], 64)
         kroof = roofline(kby, kfl, res["step_ms"])
-        kroof["kernel"] = "k_kpm_step"
+        kroof["kernel"] = "k_kpm_v2 (2 vectors per lane)" if vt_best == 2 else "k_kpm_u (1 vector per lane)"
         out["kpm"] = {"value": round(64 * nmom / wk, 1), "unit": "vector-moments/s", "rows": g.n * 9, "nvec": 64,
                       "nmom": nmom, "wall_s": round(wk, 3), "step_ms": round(res["step_ms"], 4), "roofline": kroof,
                       "vector_tile": vt_best,
                       "mu_head": [round(float(v), 8) for v in res["mu"][:4]]}
         log(f"kpm: {nmom} moments x 64 vectors on {g.n * 9} rows in {wk:.2f} s, step {res['step_ms']:.3f} ms, {kroof}")
         R.close()
+
+    if not args.no_adjoint:
+        # gradients of a contour objective through 256 real chains of depth 40 (18 parameters, checkpoints every 8)
+        from asr_amd import adjoint as AD
+        import torch
+        t = time.perf_counter()
+        wa = W.ternary_adjoint(L=4, levels=2, nchains=32, depth=16) if args.quick else W.ternary_adjoint()
+        nla, nca = int(wa.nlev[0]), len(wa.members)
+        log(f"adjoint: {wa.op.nn} nodes, {nca} chains x {nla}, build {time.perf_counter() - t:.1f} s")
+        GA = engine.GpuOperator(wa.op)
+        mem = AD.members_at(wa.members, wa.info["theta"], wa.info["c_offset"])
+        rng = np.random.default_rng(9)
+        la, lb = rng.standard_normal((nca, nla)), rng.standard_normal((nca, nla))
+        GA.lanczos(mem, nla, tiny=0.0)  # warm-up (autotuning)
+        _, _, _, fwd_plain = GA.lanczos(mem, nla, tiny=0.0)
+        GA.lanczos_adjoint(mem, nla, la, lb)
+        t0 = time.perf_counter()
+        _, _, gr, (fms, ams) = GA.lanczos_adjoint(mem, nla, la, lb)
+        wadj = time.perf_counter() - t0
+        # the torch harness end to end: band energy + charge penalty on a 64-node closed contour
+        from asr_amd import host as H
+        z, w = H.contour_closed(-1.6, -0.05, 0.6, 48, 16)
+        weight = torch.full((nca,), 1.0 / nca, dtype=torch.float64)
+        q0 = torch.full((nca,), 0.5, dtype=torch.float64)
+        theta = torch.tensor(wa.info["theta"], requires_grad=True)
+        t0 = time.perf_counter()
+        a_t, b_t = AD.chain_coefficients(theta, GA, wa.members, nla, wa.info["c_offset"])
+        lossv = AD.contour_objective(a_t, b_t, torch.from_numpy(z), torch.from_numpy(w), weight, q0)
+        t1 = time.perf_counter()
+        lossv.backward()
+        t2 = time.perf_counter()
+        out["adjoint"] = {"workload": "fcc ternary, real chains on h = c + d S d, 18 parameters (c, d x 3 l x 3 species), "
+                                      "checkpoints every 8 levels", "nodes": wa.op.nn, "chains": nca, "depth": nla,
+                          "forward_ms": round(fwd_plain, 2), "forward_ckpt_ms": round(fms, 2), "reverse_ms": round(ams, 2),
+                          "reverse_over_forward": round(ams / fwd_plain, 2), "wall_ms": round(wadj * 1e3, 1),
+                          "torch_forward_s": round(t1 - t0, 3), "torch_backward_s": round(t2 - t1, 3),
+                          "grad_norm": float(np.linalg.norm(gr.sum(axis=0)))}
+        log(f"adjoint: {out['adjoint']}")
+        GA.close()
 
     if not args.no_cpu:
         cores = cpu_cores()
