@@ -252,6 +252,20 @@ __device__ void block_reduce_member(T v, int mt, T* lds, T* partial_out) {
   }
 }
 
+// as block_reduce_member, adding the block's sum to the slot (the slot is owned by this block and lane)
+template <class T>
+__device__ void block_reduce_member_add(T v, int mt, T* lds, T* partial_out) {
+  const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x >> 6;
+  for (int off = mt; off < WAVE; off <<= 1) v = v + shfl_xor(v, off);
+  if (lane < mt) lds[wave * WAVE + lane] = v;
+  __syncthreads();
+  if (threadIdx.x < (unsigned)mt) {
+    T s = lds[threadIdx.x];
+    for (int w = 1; w < BLOCK / WAVE; ++w) s = s + lds[w * WAVE + threadIdx.x];
+    *partial_out = *partial_out + s;
+  }
+}
+
 template <class T>
 __global__ void __launch_bounds__(BLOCK) k_fused(OpDev op, MemDev md, FArgs fa) {
   extern __shared__ double lds_raw[];
@@ -580,86 +594,133 @@ __global__ void __launch_bounds__(BLOCK) k_lincomb(OpDev op, MemDev md, double* 
   }
 }
 
-// gradient contributions of one level, per member: with x = u_k, y = w_bar (normalised), x~ = x / sq, y~ = y / sq,
-// t2x = S D x~, t2y = S D y~ (amplitude), and R_sl the species-s, orbital-l projection of the random operators,
+// gradient contributions of one level, per member: with x = u_k, y = w_bar = r + abar x (normalised), x~ = x / sq,
+// y~ = y / sq, t2x = S D x~, t2y = S D y~ = t2r + abar t2x (amplitude), and R_sl the species-s, orbital-l projection of
+// the random operators,
 //   dL/dc_sL += sum_i sq_i y_i . (R_sL x~)_i,   dL/dd_sL += sum_i sq_i^2 [(R_sL y~)_i . t2x_i + t2y_i . (R_sL x~)_i]
-// (the normalised operators are symmetric on the symmetric subspace).  L = l shell (s, p, d).  NSP species.
+// (the normalised operators are symmetric on the symmetric subspace).  L = l shell (s, p, d).  R_s gathers the node's own
+// state and its flip partners with the class-table weights mt(l0, k, s): every partner is read once and its three
+// per-shell products are spread over the species.  The block's partials are added to part (one slot per block and
+// member, planes q = 0..3 NSP-1 for c, 3 NSP..6 NSP-1 for d): the sum over levels stays in place until one final reduction.
 template <int NSP>
-__global__ void __launch_bounds__(BLOCK) k_adj_grad(OpDev op, MemDev md, const double* x, const double* y, const double* t2x,
-                                                    const double* t2y, const int* spc, double* part) {
+__global__ void __launch_bounds__(BLOCK) k_adj_grad(OpDev op, MemDev md, const double* x, const double* r,
+                                                    const double* abar, const double* t2x, const double* t2r, const int* spc,
+                                                    double* part) {
   __shared__ double red[BLOCK];
   ASR_NODE_MEMBER
   double gc[NSP][3], gd[NSP][3];
+#pragma unroll
   for (int s = 0; s < NSP; ++s)
+#pragma unroll
     for (int L = 0; L < 3; ++L) { gc[s][L] = 0.0; gd[s][L] = 0.0; }
   if (live) {
     const int nbp = md.nbp, nsm = op.nsmax;
     const uint32_t inf = op.info[i];
-    const double sqi = op.sq[i];
+    const double sqi = op.sq[i], isq = 1.0 / sqi, ab = abar[m];
     const uint64_t chim = md.chimask[m];
-    double xt[9], yt[9];
+    double yo[9], tx[9], ty[9];
+#pragma unroll
     for (int l = 0; l < 9; ++l) {
-      xt[l] = ld(x, i, l, nbp, m) / sqi;
-      yt[l] = ld(y, i, l, nbp, m) / sqi;
+      const double xv = ld(x, i, l, nbp, m);
+      yo[l] = ld(r, i, l, nbp, m) + ab * xv;
+      tx[l] = ld(t2x, i, l, nbp, m);
+      ty[l] = ld(t2r, i, l, nbp, m) + ab * tx[l];
     }
-    // the frozen start site carries its species: there R_sl is the identity on orbital l for that species alone
+    // one pass per gathered term (the node's own state, then every flip partner): px = R-term of x~, py = of y~, and the
+    // per-shell products A = y . px, B = t2x . py + t2y . px spread over the species slots with the weights mt(l0, k, s)
     const bool fz = ((inf >> 20) & 1u) && md.frozen[m];
     const int l0 = inf & 15u, kap = (inf >> 4) & 255u;
-    const int nsc = op.nsc[kap];
-    for (int sl = 0; sl < (fz ? 1 : nsc); ++sl) {
-      const int s = fz ? md.fsp[m] : spc[kap * nsm + sl];
-      double rx[9], ry[9];
-      if (fz) {
-        for (int l = 0; l < 9; ++l) { rx[l] = xt[l]; ry[l] = yt[l]; }
-      } else {
-        const double w0 = md.mtm[((((size_t)kap * nsm + l0) * nsm + l0) * nsm + sl) * nbp + m];
-        for (int l = 0; l < 9; ++l) { rx[l] = w0 * xt[l]; ry[l] = w0 * yt[l]; }
-        const uint32_t* fl = op.flip + (size_t)i * nsm;
-        for (int k = 0; k < nsc; ++k) {
-          const uint32_t e = fl[k];
-          if (k == l0 || e == 0u) continue;
-          int r, g;
-          unpack_link(e, r, g);
-          const double wk = md.mtm[((((size_t)kap * nsm + l0) * nsm + k) * nsm + sl) * nbp + m];
-          const double chi = ((chim >> g) & 1ull) ? -1.0 : 1.0;
-          const double sqr = op.sq[r];
-          for (int l = 0; l < 9; ++l) {
-            const double cs = chi * op.sg[g * 9 + l] * wk / sqr;
-            const int pl = op.pi[g * 9 + l];
-            rx[l] = fma(cs, ld(x, r, pl, nbp, m), rx[l]);
-            ry[l] = fma(cs, ld(y, r, pl, nbp, m), ry[l]);
-          }
-        }
+    const int nsc = fz ? 1 : op.nsc[kap];
+    const double* mtk = md.mtm + ((size_t)kap * nsm + l0) * nsm * nsm * nbp;  // [k][sl][nbp]
+    const uint32_t* fl = op.flip + (size_t)i * nsm;
+    for (int kk = -1; kk < nsc; ++kk) {
+      int rn = i, k = l0;
+      double cs0 = isq;
+      const int* pg = nullptr;
+      const double* sgg = nullptr;
+      if (kk >= 0) {
+        if (fz || kk == l0) continue;
+        const uint32_t e = fl[kk];
+        if (e == 0u) continue;
+        int g;
+        unpack_link(e, rn, g);
+        k = kk;
+        cs0 = (((chim >> g) & 1ull) ? -1.0 : 1.0) / op.sq[rn];
+        pg = op.pi + g * 9;
+        sgg = op.sg + g * 9;
       }
+      double A[3] = {0.0, 0.0, 0.0}, B[3] = {0.0, 0.0, 0.0};
+#pragma unroll
       for (int l = 0; l < 9; ++l) {
         const int L = l == 0 ? 0 : (l < 4 ? 1 : 2);
-        const double gcv = sqi * ld(y, i, l, nbp, m) * rx[l];
-        const double gdv = sqi * sqi * (ry[l] * ld(t2x, i, l, nbp, m) + ld(t2y, i, l, nbp, m) * rx[l]);
+        const int pl = pg ? pg[l] : l;
+        const double cs = pg ? cs0 * sgg[l] : cs0;
+        const double xv = ld(x, rn, pl, nbp, m);
+        const double pxl = cs * xv;
+        const double pyl = cs * (ld(r, rn, pl, nbp, m) + ab * xv);
+        A[L] = fma(yo[l], pxl, A[L]);
+        B[L] = fma(tx[l], pyl, fma(ty[l], pxl, B[L]));
+      }
+      if (fz) {
+        // the frozen start site carries its species: R_s is the identity there for that species alone
+        const int s = md.fsp[m];
+#pragma unroll
         for (int q = 0; q < NSP; ++q)
-          if (q == s) { gc[q][L] += gcv; gd[q][L] += gdv; }
+          if (q == s)
+#pragma unroll
+            for (int L = 0; L < 3; ++L) { gc[q][L] += sqi * A[L]; gd[q][L] += sqi * sqi * B[L]; }
+      } else {
+#pragma unroll
+        for (int sl = 0; sl < NSP; ++sl) {
+          if (sl < op.nsc[kap]) {
+            const double w = mtk[((size_t)k * nsm + sl) * nbp + m];
+            const int s = spc[kap * nsm + sl];
+#pragma unroll
+            for (int q = 0; q < NSP; ++q)
+              if (q == s)
+#pragma unroll
+                for (int L = 0; L < 3; ++L) { gc[q][L] += sqi * w * A[L]; gd[q][L] += sqi * sqi * w * B[L]; }
+          }
+        }
       }
     }
   }
   const size_t base = ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * mt;
   const size_t plane = (size_t)gridDim.x * gridDim.y * mt;
+#pragma unroll
   for (int q = 0; q < NSP * 3; ++q) {
-    block_reduce_member<double>(gc[q / 3][q % 3], mt, red, part + (size_t)q * plane + base + threadIdx.x);
+    block_reduce_member_add<double>(gc[q / 3][q % 3], mt, red, part + (size_t)q * plane + base + threadIdx.x);
     __syncthreads();
-    block_reduce_member<double>(gd[q / 3][q % 3], mt, red, part + (size_t)(NSP * 3 + q) * plane + base + threadIdx.x);
+    block_reduce_member_add<double>(gd[q / 3][q % 3], mt, red, part + (size_t)(NSP * 3 + q) * plane + base + threadIdx.x);
     __syncthreads();
   }
 }
 
-// per member: the forward coefficients of level k for a recomputed level (a_k, b_{k-1}, 1 / b_k)
+// u_{k+1} = (w - a_k u_k - b_{k-1} u_{k-1}) / b_k of a recomputed level, the arithmetic of k_update (projection off) and
+// k_scale element for element, so that a recomputed vector equals the forward's bit for bit
+__global__ void __launch_bounds__(BLOCK) k_recompute(OpDev op, MemDev md, double* out, const double* w, const double* u,
+                                                     const double* up, const double* a_eff, const double* b_prev,
+                                                     const double* inv_b) {
+  ASR_NODE_MEMBER
+  if (!live) return;
+  const double a = a_eff[m], b = b_prev[m], ib = inv_b[m];
+  for (int l = 0; l < 9; ++l) {
+    const size_t q = ((size_t)i * 9 + l) * md.nbp + m;
+    const double xv = w[q] - bilin(a, u[q]) - bilin(b, up[q]);
+    out[q] = bilin(ib, xv);
+  }
+}
+
+// per member: the forward coefficients of level k (a_k, b_{k-1}, 1 / b_k, -a_k, -b_k)
 __global__ void k_adj_coef(int nb, int ld, int k, const double* a, const double* b, double* a_eff, double* b_prev,
-                           double* inv_b, double* nega, double* negb_prev) {
+                           double* inv_b, double* nega, double* negb) {
   int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= nb) return;
   a_eff[m] = a[(size_t)m * ld + k];
   b_prev[m] = k > 0 ? b[(size_t)m * ld + k - 1] : 0.0;
   inv_b[m] = 1.0 / b[(size_t)m * ld + k];
   nega[m] = -a[(size_t)m * ld + k];
-  negb_prev[m] = -b_prev[m];
+  negb[m] = -b[(size_t)m * ld + k];
 }
 
 // chain state of a batch
@@ -723,9 +784,9 @@ __global__ void __launch_bounds__(BLOCK) k_adj_fin1(int nb, int ld, int k, const
   beta[m] = Lb[(size_t)m * ld + k] + cbb[m] - d1 / bk;
 }
 
-// a_bar_k = dL/da_k - <r_bar, u_k>;  the carry to b_bar_{k-1}: -<r_bar, u_{k-1}>
+// a_bar_k = dL/da_k - <r_bar, u_k> (and 2 a_bar_k);  the carry to b_bar_{k-1}: -<r_bar, u_{k-1}>
 __global__ void __launch_bounds__(BLOCK) k_adj_fin2(int nb, int ld, int k, const double* part2, const double* part3, int nbx,
-                                                    int mt, const double* La, double* abar, double* cbb) {
+                                                    int mt, const double* La, double* abar, double* abar2, double* cbb) {
   __shared__ double lds[BLOCK / WAVE];
   const int m = blockIdx.x;
   double d2 = block_sum_partials<double>(part2, nbx, mt, m, lds);
@@ -733,6 +794,7 @@ __global__ void __launch_bounds__(BLOCK) k_adj_fin2(int nb, int ld, int k, const
   double d3 = block_sum_partials<double>(part3, nbx, mt, m, lds);
   if (threadIdx.x != 0) return;
   abar[m] = La[(size_t)m * ld + k] - d2;
+  abar2[m] = 2.0 * abar[m];
   cbb[m] = -d3;
 }
 
@@ -1882,8 +1944,8 @@ int run_adjoint(asr_gpu_op* op, int nchain, const int* start_orb, const double* 
   for (auto& p : U) if (vec(&p)) return 1;
   for (auto& p : W) if (vec(&p)) return 1;
   for (auto& p : T2) if (vec(&p)) return 1;
-  double *rbar, *wbar, *hw, *t2y, *ubar, *unext, *carry, *carry2, *wtmp;
-  for (double** p : {&rbar, &wbar, &hw, &t2y, &ubar, &unext, &carry, &carry2, &wtmp}) if (vec(p)) return 1;
+  double *rbar, *rprev, *hr, *t2r, *ubar, *unext;
+  for (double** p : {&rbar, &rprev, &hr, &t2r, &ubar, &unext}) if (vec(p)) return 1;
   const dim3 grid = R.grid;
   const int nbx = grid.x, mt = R.M.mt, nbp = R.M.nbp;
   const size_t plane = (size_t)grid.x * grid.y * mt;
@@ -1897,10 +1959,13 @@ int run_adjoint(asr_gpu_op* op, int nchain, const int* start_orb, const double* 
       dalloc(&dLa, (size_t)nbp * nlev) || dalloc(&dLb, (size_t)nbp * nlev))
     return 1;
   for (double* p : {PG, coef, gacc, dLa, dLb}) bufs.push_back(p);
-  double *a_eff = coef, *b_prev = coef + nbp, *inv_b = coef + 2 * nbp, *nega = coef + 3 * nbp, *negb_prev = coef + 4 * nbp,
-         *alpha = coef + 5 * nbp, *beta = coef + 6 * nbp, *abar = coef + 7 * nbp, *cbb = coef + 8 * nbp;
+  double *a_eff = coef, *b_prev = coef + nbp, *inv_b = coef + 2 * nbp, *nega = coef + 3 * nbp, *negb = coef + 4 * nbp,
+         *alpha = coef + 5 * nbp, *beta = coef + 6 * nbp, *abar = coef + 7 * nbp, *cbb = coef + 8 * nbp,
+         *abar2 = coef + 9 * nbp;
   HCHECK(hipMemsetAsync(coef, 0, (size_t)12 * nbp * 8, st));
   HCHECK(hipMemsetAsync(gacc, 0, (size_t)nbp * nq * 8, st));
+  HCHECK(hipMemsetAsync(PG, 0, plane * nq * 8, st));
+  HCHECK(hipMemsetAsync(P1, 0, plane * 8, st));
   {
     // loss derivatives in the sorted member order
     std::vector<double> la((size_t)nbp * nlev, 0.0), lb((size_t)nbp * nlev, 0.0);
@@ -1941,9 +2006,7 @@ int run_adjoint(asr_gpu_op* op, int nchain, const int* start_orb, const double* 
     if (R.step()) return 1;
   }
   HCHECK(hipEventRecord(e1, st));
-  // ---- reverse
-  HCHECK(hipMemsetAsync(unext, 0, nv * 8, st));
-  HCHECK(hipMemsetAsync(carry, 0, nv * 8, st));
+  // ---- reverse: u_bar_nlev = 0, r_bar_nlev = 0, <u_bar_nlev, u_nlev> = 0 (P1 cleared)
   for (int seg = nseg - 1; seg >= 0; --seg) {
     const int k0 = seg * ckpt, k1 = std::min(nlev, k0 + ckpt) - 1;
     // U[0] = u_{k0 - 1}, U[1 + j] = u_{k0 + j}
@@ -1952,49 +2015,39 @@ int run_adjoint(asr_gpu_op* op, int nchain, const int* start_orb, const double* 
     for (int k = k0; k <= k1; ++k) {
       const int j = k - k0;
       hipLaunchKernelGGL(k_adj_coef, dim3((nbp + 63) / 64), dim3(64), 0, st, R.M.nb, nlev, k, R.B.a, R.B.b, a_eff, b_prev,
-                         inv_b, nega, negb_prev);
+                         inv_b, nega, negb);
       if (apply_chain_op<double>(op, R.M, md0, U[1 + j], W[j], R.B.t1, T2[j], R.B.h1, false, nullptr, nullptr, st)) return 1;
-      // u_{k+1} = (w - a u_k - b_{k-1} u_{k-1}) / b_k with the forward's own kernels (bitwise the forward's vectors)
-      HCHECK(hipMemcpyAsync(U[2 + j], W[j], nv * 8, hipMemcpyDeviceToDevice, st));
-      hipLaunchKernelGGL(k_update<double>, grid, dim3(BLOCK), 0, st, od, md0, U[2 + j], U[1 + j], U[j], a_eff, b_prev, 0, P1);
-      hipLaunchKernelGGL(k_scale<double>, R.sgrid, dim3(256), 0, st, U[2 + j], inv_b, nv, nbp);
+      hipLaunchKernelGGL(k_recompute, grid, dim3(BLOCK), 0, st, od, md0, U[2 + j], W[j], U[1 + j], U[j], a_eff, b_prev, inv_b);
       HCHECK(hipGetLastError());
     }
     for (int k = k1; k >= k0; --k) {
       const int j = k - k0;
       hipLaunchKernelGGL(k_adj_coef, dim3((nbp + 63) / 64), dim3(64), 0, st, R.M.nb, nlev, k, R.B.a, R.B.b, a_eff, b_prev,
-                         inv_b, nega, negb_prev);
-      // d1 = <u_bar_{k+1}, u_{k+1}>  ->  b_bar_k, then r_bar with <r_bar, u_k>, <r_bar, u_{k-1}>  (wtmp: a copy, unused)
-      if (lincomb(wtmp, 1, unext, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 1, U[2 + j], P1, nullptr,
-                  nullptr))
-        return 1;
+                         inv_b, nega, negb);
+      // b_bar_k from d1 = <u_bar_{k+1}, u_{k+1}> (partials of the previous step), then r_bar_k with <r_bar, u_k>,
+      // <r_bar, u_{k-1}>, then a_bar_k
       hipLaunchKernelGGL(k_adj_fin1, dim3(R.M.nb), dim3(BLOCK), 0, st, R.M.nb, nlev, k, P1, nbx, mt, R.B.b, dLb, cbb, alpha,
                          beta);
       if (lincomb(rbar, 2, unext, alpha, U[2 + j], beta, nullptr, nullptr, nullptr, nullptr, 2, U[1 + j], P2, U[j], P3))
         return 1;
-      hipLaunchKernelGGL(k_adj_fin2, dim3(R.M.nb), dim3(BLOCK), 0, st, R.M.nb, nlev, k, P2, P3, nbx, mt, dLa, abar, cbb);
-      // w_bar = r_bar + a_bar u_k;  H w_bar (keeping S D y~)
-      if (lincomb(wbar, 2, rbar, nullptr, U[1 + j], abar, nullptr, nullptr, nullptr, nullptr, 0, nullptr, nullptr, nullptr,
-                  nullptr))
-        return 1;
-      if (apply_chain_op<double>(op, R.M, md0, wbar, hw, R.B.t1, t2y, R.B.h1, false, nullptr, nullptr, st)) return 1;
+      hipLaunchKernelGGL(k_adj_fin2, dim3(R.M.nb), dim3(BLOCK), 0, st, R.M.nb, nlev, k, P2, P3, nbx, mt, dLa, abar, abar2,
+                         cbb);
+      // w_bar = r_bar + a_bar u_k enters linearly: H w_bar = H r_bar + a_bar w_k, S D w~_bar = S D r~_bar + a_bar S D u~_k
+      if (apply_chain_op<double>(op, R.M, md0, rbar, hr, R.B.t1, t2r, R.B.h1, false, nullptr, nullptr, st)) return 1;
       switch (nsp) {
-        case 1: hipLaunchKernelGGL(k_adj_grad<1>, grid, dim3(BLOCK), 0, st, od, md0, U[1 + j], wbar, T2[j], t2y, spc_d, PG); break;
-        case 2: hipLaunchKernelGGL(k_adj_grad<2>, grid, dim3(BLOCK), 0, st, od, md0, U[1 + j], wbar, T2[j], t2y, spc_d, PG); break;
-        case 3: hipLaunchKernelGGL(k_adj_grad<3>, grid, dim3(BLOCK), 0, st, od, md0, U[1 + j], wbar, T2[j], t2y, spc_d, PG); break;
-        case 4: hipLaunchKernelGGL(k_adj_grad<4>, grid, dim3(BLOCK), 0, st, od, md0, U[1 + j], wbar, T2[j], t2y, spc_d, PG); break;
-        default: hipLaunchKernelGGL(k_adj_grad<5>, grid, dim3(BLOCK), 0, st, od, md0, U[1 + j], wbar, T2[j], t2y, spc_d, PG); break;
+        case 1: hipLaunchKernelGGL(k_adj_grad<1>, grid, dim3(BLOCK), 0, st, od, md0, U[1 + j], rbar, abar, T2[j], t2r, spc_d, PG); break;
+        case 2: hipLaunchKernelGGL(k_adj_grad<2>, grid, dim3(BLOCK), 0, st, od, md0, U[1 + j], rbar, abar, T2[j], t2r, spc_d, PG); break;
+        case 3: hipLaunchKernelGGL(k_adj_grad<3>, grid, dim3(BLOCK), 0, st, od, md0, U[1 + j], rbar, abar, T2[j], t2r, spc_d, PG); break;
+        case 4: hipLaunchKernelGGL(k_adj_grad<4>, grid, dim3(BLOCK), 0, st, od, md0, U[1 + j], rbar, abar, T2[j], t2r, spc_d, PG); break;
+        default: hipLaunchKernelGGL(k_adj_grad<5>, grid, dim3(BLOCK), 0, st, od, md0, U[1 + j], rbar, abar, T2[j], t2r, spc_d, PG); break;
       }
-      hipLaunchKernelGGL(k_adj_fin_grad, dim3(R.M.nb), dim3(BLOCK), 0, st, R.M.nb, nq, PG, nbx, mt, plane, gacc);
-      // u_bar_k = carry - a_k r_bar + a_bar w_k + H w_bar;  carry for u_{k-1}: -b_{k-1} r_bar
-      if (lincomb(ubar, 4, carry, nullptr, rbar, nega, W[j], abar, hw, nullptr, 0, nullptr, nullptr, nullptr, nullptr) ||
-          lincomb(carry2, 1, rbar, negb_prev, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0, nullptr, nullptr, nullptr,
-                  nullptr))
-        return 1;
+      // u_bar_k = -b_k r_bar_{k+1} - a_k r_bar_k + 2 a_bar_k w_k + H r_bar_k, with <u_bar_k, u_k> for the next level's b_bar
+      if (lincomb(ubar, 4, rprev, negb, rbar, nega, W[j], abar2, hr, nullptr, 1, U[1 + j], P1, nullptr, nullptr)) return 1;
       std::swap(unext, ubar);
-      std::swap(carry, carry2);
+      std::swap(rprev, rbar);
     }
   }
+  hipLaunchKernelGGL(k_adj_fin_grad, dim3(R.M.nb), dim3(BLOCK), 0, st, R.M.nb, nq, PG, nbx, mt, plane, gacc);
   HCHECK(hipEventRecord(e2, st));
   HCHECK(hipGetLastError());
   HCHECK(hipEventSynchronize(e2));
