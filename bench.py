@@ -31,20 +31,27 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def chain_kernel_cost(op, nb: int, kind: int, uniform_d: bool = True):
+# vectors read by the operator launches of one level of the route-edc chains (uniform d, o != 0, X(z) diagonal):
+# launch 0: h1 = rand_c(x~) + D S D x~ (reads x); launch 1: t1 = rand_o(h1) (reads h1); launch 2: w = sq (h1 - h t1
+# + X(z) x~) with the dot <u, w> (reads t1, h1 and x = u).  Each writes one vector.
+EDC_LAUNCH_READS = {0: 1, 1: 1, 2: 3}
+
+
+def chain_kernel_cost(op, nb: int, kind: int, nread: int = 1):
     """Algorithmic bytes and flops of one launch of a chain-level kernel over nn nodes and nb complex members.
 
-    SpMM (kind 0, h = rand_c(x~) + D S D x~): read x once (9 nb complex per node), write the result once, read the node
-    tables once (hop 4 nslot + flip 4 nsmax + info 4 + sq 8 bytes per node); flops: (1 + nslot) 9x9 blocks on complex
-    members (4 flops per complex-by-real FMA pair) plus the flip terms and the D scalings."""
+    Operator pass (kind 0, S pass + flip terms; kind 1, flip terms only): read nread vectors once (9 nb complex per node
+    each), write the result once, read the node tables once (hop 4 nslot + flip 4 nsmax + info 4 + sq 8 bytes per node);
+    flops: (1 + nslot) 9x9 blocks on complex members (4 flops per complex-by-real FMA pair) plus the flip terms and the
+    D scalings."""
     nn, ns, nsm = op.nn, op.nslot, op.nsmax
     vec = nn * 9 * nb * 16
     tables = nn * (4 * ns + 4 * nsm + 4 + 8)
     if kind == 0:
-        bytes_ = 2 * vec + tables
+        bytes_ = (nread + 1) * vec + tables
         flops = nn * nb * ((1 + ns) * 81 * 4 + nsm * 9 * 4 + (1 + ns) * 9 * 2 * 2 + 9 * 4)
     elif kind == 1:
-        bytes_ = 2 * vec + nn * (4 * nsm + 4 + 8)
+        bytes_ = (nread + 1) * vec + nn * (4 * nsm + 4 + 8)
         flops = nn * nb * (nsm * 9 * 4)
     elif kind == 3:
         bytes_ = 4 * vec
@@ -74,6 +81,24 @@ def roofline(bytes_, flops, ms):
                     hbm_gbs=round(gbs, 1), hbm_frac=round(gbs / HBM_PEAK_GBS, 4), flop_per_byte=round(intensity, 2))
     return dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(gbs / HBM_PEAK_GBS, 4),
                 fp64_tfs=round(tfs, 3), flop_per_byte=round(intensity, 2))
+
+
+def measured_traffic(workload: str, kernel: str, nodes: int, members: int):
+    """HBM bytes per launch of a kernel measured with rocprofv3 --pmc (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled
+    by the same counter's reading for a streaming kernel of known size: k_scale, which reads and writes one chain vector),
+    kept as text under profiles/ for the workload it was measured on; None when there is no measurement for this kernel
+    and size."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    try:
+        with open(path) as f:
+            tab = json.load(f)
+    except (OSError, ValueError):
+        return None
+    for rec in tab.get("kernels", []):
+        if rec.get("workload") == workload and rec.get("kernel") == kernel and rec.get("nodes") == nodes and \
+                rec.get("members") == members:
+            return rec.get("hbm_bytes_per_launch")
+    return None
 
 
 def cpu_cores() -> int:
@@ -145,15 +170,24 @@ def main():
     kms, kinds = G.profile(wl.members, wl.nlev)
     per_kernel = []
     for q, (t_ms, kd) in enumerate(zip(kms, kinds)):
-        by, fl = chain_kernel_cost(wl.op, nb, int(kd))
+        by, fl = chain_kernel_cost(wl.op, nb, int(kd), EDC_LAUNCH_READS.get(q, 1))
         per_kernel.append(dict(launch=q, kind=int(kd), ms=round(float(t_ms), 4), gbs=round(by / (t_ms * 1e-3) / 1e9, 1),
                                tfs=round(fl / (t_ms * 1e-3) / 1e12, 3)))
-    spmm = [k for k in per_kernel if k["kind"] == 0]
-    dom = max(spmm, key=lambda k: k["ms"]) if spmm else max(per_kernel, key=lambda k: k["ms"])
-    by, fl = chain_kernel_cost(wl.op, nb, dom["kind"])
-    roof = roofline(by, fl, dom["ms"])
-    roof["traffic"] = None
-    roof["kernel"] = "k_fused<cplx> (S pass + flip terms), launch %d of the level" % dom["launch"]
+    # the dominant kernel is the operator pass (every launch of kinds 0 and 1 is the same kernel: the S pass with the flip
+    # terms, or the flip terms alone); its roofline uses the summed algorithmic cost over the level's launches of it
+    # divided by their summed duration, so that the mean duration per launch is comparable with a kernel-trace average
+    ops = [k for k in per_kernel if k["kind"] in (0, 1)]
+    tb = tf = tm = 0.0
+    for k in ops:
+        by, fl = chain_kernel_cost(wl.op, nb, k["kind"], EDC_LAUNCH_READS.get(k["launch"], 1))
+        tb, tf, tm = tb + by, tf + fl, tm + k["ms"]
+    roof = roofline(tb / len(ops), tf / len(ops), tm / len(ops))
+    kname = {65: "k_pair<true, %d>" % wl.op.nslot, 64: "k_node<cplx, true, %d>" % wl.op.nslot}.get(out_tile, "k_fused<cplx>")
+    roof["kernel"] = kname
+    roof["launches_per_level"] = len(ops)
+    roof["avg_launch_ms"] = round(tm / len(ops), 4)
+    roof["algorithmic_bytes_per_launch"] = round(tb / len(ops))
+    roof["traffic"] = measured_traffic("chain_edc", kname, wl.op.nn, nb)
     log("level kernels:", per_kernel)
 
     out = {
@@ -233,7 +267,9 @@ def main():
         wk = time.perf_counter() - t0
         kby, kfl = kpm_kernel_cost(g.n, g.nbr.shape[1], 64)
         kroof = roofline(kby, kfl, res["step_ms"])
-        kroof["kernel"] = "k_kpm_v2 (2 vectors per lane)" if vt_best == 2 else "k_kpm_u (1 vector per lane)"
+        kroof["kernel"] = "k_kpm_v2<%d>" % g.nbr.shape[1] if vt_best == 2 else "k_kpm_u<false, %d>" % g.nbr.shape[1]
+        kroof["algorithmic_bytes_per_launch"] = round(kby)
+        kroof["traffic"] = measured_traffic("kpm", kroof["kernel"], g.n, 64)
         out["kpm"] = {"value": round(64 * nmom / wk, 1), "unit": "vector-moments/s", "rows": g.n * 9, "nvec": 64,
                       "nmom": nmom, "wall_s": round(wk, 3), "step_ms": round(res["step_ms"], 4), "roofline": kroof,
                       "vector_tile": vt_best,
