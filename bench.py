@@ -108,11 +108,12 @@ def cpu_cores() -> int:
         return os.cpu_count() or 1
 
 
-def cpu_baseline(wl, nlev_sample: int, cores: int):
-    """The oracle's CPU chains (one chain per thread, dynamic schedule) on a bounded sample of the same workload."""
+def cpu_baseline(wl, nlev_sample: int, cores: int, nchain: int | None = None):
+    """The oracle's CPU chains (one chain per thread, dynamic schedule, as chain_worker) on the host cores: nchain chains
+    (default: all of the workload) to nlev_sample levels each."""
     from oracle.oracle import OracleOp
     os.environ.setdefault("OMP_PROC_BIND", "spread")
-    nch = min(len(wl.members), cores)
+    nch = len(wl.members) if nchain is None else min(len(wl.members), nchain)
     members = wl.members[:nch]
     eps = np.stack([m.z * m.pa + m.pb for m in members])
     O = OracleOp(wl.op)
@@ -318,7 +319,7 @@ def main():
 
     if not args.no_cpu:
         cores = cpu_cores()
-        cb, _ = cpu_baseline(wl, 2 if args.quick else 8, cores)
+        cb, _ = cpu_baseline(wl, 2 if args.quick else depth, cores)
         out["cpu_baseline"] = cb
         log(f"cpu baseline: {cb}")
     G.close()
