@@ -23,6 +23,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "asr_hip.h"
@@ -73,7 +74,10 @@ struct OpDev {
   const uint32_t* hop;   // nn x nslot: (node + 1) << 6 | op, 0 = no link
   const uint32_t* flip;  // nn x nsmax, same packing
   const uint32_t* info;  // nn: lst | ncl << 4 | ntyp << 12 | fz << 20
-  const double* sq;      // nn
+  const double* sq;      // nn + 1 (sq[nn] = 1: the padding node)
+  const double* isq;     // nn + 1: 1 / sq
+  const uint32_t* sgm;   // nop: bit l set when sg(l, g) < 0
+  const double* blkp;    // ntype x (nslot + 1) x 9 x 10: the bond blocks with rows padded to 10 (16-byte aligned rows)
   const int* stab_ptr;   // nn + 1
   const int* stab;
   const int* pi;         // nop x 9
@@ -115,6 +119,13 @@ __device__ inline T ld(const T* x, int node, int l, int nbp, int m) {
   return x[((size_t)node * 9 + l) * nbp + m];
 }
 
+// Copies the padded bond blocks of every site type into LDS (all threads of the block), then synchronises.
+__device__ __forceinline__ void stage_blocks(const OpDev& op, double* lds) {
+  const int n = op.ntype * (op.nslot + 1) * 90;
+  for (int q = threadIdx.x; q < n; q += blockDim.x) lds[q] = op.blkp[q];
+  __syncthreads();
+}
+
 // acc += coef * [dout o] S ([din o] x~)  with x~ = x / sq (xnorm) and the member's characters; B in LDS
 template <class T>
 __device__ void s_accum(const OpDev& op, const double* __restrict__ blk, const int* __restrict__ pi,
@@ -123,13 +134,12 @@ __device__ void s_accum(const OpDev& op, const double* __restrict__ blk, const i
   T v[9];
   const double* b0 = blk + (size_t)t * (op.nslot + 1) * 81;
   {
-    double inv = xnorm ? 1.0 / op.sq[i] : 1.0;
+    const double inv = xnorm ? op.isq[i] : 1.0;
     for (int l = 0; l < 9; ++l) {
       T xv = ld(x, i, l, nbp, m);
       double s = dv ? dv[l] : 1.0;
-      v[l] = (xnorm ? s * (xv / op.sq[i]) : s * xv);
+      v[l] = (s * inv) * xv;
     }
-    (void)inv;
     for (int l = 0; l < 9; ++l) {
       T a = acc[l];
       for (int l2 = 0; l2 < 9; ++l2) a = fma_s(b0[l * 9 + l2], v[l2], a);
@@ -145,12 +155,12 @@ __device__ void s_accum(const OpDev& op, const double* __restrict__ blk, const i
     double chi = ((chim >> g) & 1ull) ? -1.0 : 1.0;
     const int* pg = pi + g * 9;
     const double* sgg = sg + g * 9;
-    double sqr = op.sq[r];
+    const double isr = xnorm ? op.isq[r] : 1.0;
     for (int l = 0; l < 9; ++l) {
       int p = pg[l];
       T xv = ld(x, r, p, nbp, m);
       double s = chi * sgg[l] * (dv ? dv[p] : 1.0);
-      v[l] = xnorm ? s * (xv / sqr) : s * xv;
+      v[l] = (s * isr) * xv;
     }
     const double* bk = b0 + (size_t)(k + 1) * 81;
     for (int l = 0; l < 9; ++l) {
@@ -168,12 +178,12 @@ __device__ void rand_accum(const OpDev& op, const MemDev& md, const int* __restr
                            T acc[9]) {
   const int nbp = md.nbp, nsm = op.nsmax;
   const bool fz = (inf >> 20) & 1u;
-  double sqi = op.sq[i];
+  const double isi = op.isq[i];
   if (fz && md.frozen[m]) {
     const double* fp = md.fpar + (size_t)q * 9 * nbp;
     for (int l = 0; l < 9; ++l) {
       T xv = ld(x, i, l, nbp, m);
-      if (xnorm) xv = xv / sqi;
+      if (xnorm) xv = isi * xv;
       acc[l] = fma_s(coef * fp[(size_t)l * nbp + m], xv, acc[l]);
     }
     return;
@@ -183,7 +193,7 @@ __device__ void rand_accum(const OpDev& op, const MemDev& md, const int* __restr
   T r_acc[9];
   for (int l = 0; l < 9; ++l) {
     T xv = ld(x, i, l, nbp, m);
-    if (xnorm) xv = xv / sqi;
+    if (xnorm) xv = isi * xv;
     r_acc[l] = fma_s(w[((size_t)l0 * 9 + l) * nbp + m], xv, zero<T>());
   }
   const int nsc = op.nsc[kap];
@@ -194,10 +204,10 @@ __device__ void rand_accum(const OpDev& op, const MemDev& md, const int* __restr
     int r, g;
     unpack_link(e, r, g);
     double chi = ((chim >> g) & 1ull) ? -1.0 : 1.0;
-    double sqr = op.sq[r];
+    const double isr = op.isq[r];
     for (int l = 0; l < 9; ++l) {
       T xv = ld(x, r, pi[g * 9 + l], nbp, m);
-      if (xnorm) xv = xv / sqr;
+      if (xnorm) xv = isr * xv;
       T tf = (chi * sg[g * 9 + l]) * xv;
       r_acc[l] = fma_s(w[((size_t)k * 9 + l) * nbp + m], tf, r_acc[l]);
     }
@@ -321,17 +331,286 @@ __global__ void __launch_bounds__(BLOCK) k_fused(OpDev op, MemDev md, FArgs fa) 
   }
 }
 
+// The fused operator pass for complex members with a small member tile MT (8 or 16; 64 / MT nodes per wave, one complex
+// member per lane), every operation mapping each orbital onto itself (pi = identity) and NSLOT links per node.  The S pass
+// reads each neighbour row (9 complex values of the lane's member) with raw buffer loads: the lane's offset (row of the
+// neighbour, member) is one integer operation per link, the orbital offset a scalar register; the padded bond blocks come
+// from LDS one row at a time.  The scale of a gathered value, dd(l) chi(g) sg(g, l) / sq(r), is one multiplication and a
+// sign flip per orbital and link, shared by the real and imaginary parts.  The rest of the pass (species-diagonal terms,
+// X(z), output scaling, the dot partials) is that of k_fused.  Member tiles of 8-16 keep one member slice of a 1-2 GB
+// vector small enough to stay in the last-level caches while all nodes are swept.
+// acc += coef X_q(x~) for complex members (apply_rand with pi = identity): the node's own state and its flip partners,
+// the member's local weights wt(q, class, l0, k, l), the partner's sign chi(g) sg(g, l) from the LDS table cs (member x
+// operation x orbital), raw buffer loads for the rows.  With q2 >= 0 the same gathered values also feed a second
+// parameter set into acc2 (X(pa) and X(pb) of the energy-dependent diagonal share one pass).  Frozen nodes: the frozen
+// species' parameters on the node's own row.
+__device__ __forceinline__ void rand_ct(const OpDev& op, const MemDev& md, const double* __restrict__ cs, int i, uint32_t inf,
+                                        int m, uint32_t node16, uint32_t vs16, uint32_t coff, __amdgpu_buffer_rsrc_t xr,
+                                        int xnorm, int q, int q2, double coef, cplx acc[9], cplx acc2[9]) {
+  const int nbp = md.nbp, nsm = op.nsmax;
+  const bool fz = ((inf >> 20) & 1u) && md.frozen[m];
+  const double isi = xnorm ? op.isq[i] : 1.0;
+  cplx xo[9];
+#pragma unroll
+  for (int l = 0; l < 9; ++l) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(xr, (int)((uint32_t)i * node16 + coff), (int)(l * vs16), 0);
+    xo[l] = cplx{__builtin_bit_cast(double, (unsigned long long)v[0] | ((unsigned long long)v[1] << 32)) * isi,
+                 __builtin_bit_cast(double, (unsigned long long)v[2] | ((unsigned long long)v[3] << 32)) * isi};
+  }
+  if (fz) {
+    const double* fp = md.fpar + (size_t)q * 9 * nbp;
+    const double* fp2 = md.fpar + (size_t)(q2 < 0 ? q : q2) * 9 * nbp;
+#pragma unroll
+    for (int l = 0; l < 9; ++l) {
+      acc[l] = fma_s(coef * fp[(size_t)l * nbp + m], xo[l], acc[l]);
+      if (q2 >= 0) acc2[l] = fma_s(coef * fp2[(size_t)l * nbp + m], xo[l], acc2[l]);
+    }
+    return;
+  }
+  const int l0 = inf & 15u, kap = (inf >> 4) & 255u;
+  const double* w = md.wts + (((size_t)q * op.ncls + kap) * nsm + l0) * nsm * 9 * nbp + m;
+  const double* w2 = md.wts + (((size_t)(q2 < 0 ? q : q2) * op.ncls + kap) * nsm + l0) * nsm * 9 * nbp + m;
+  cplx ra[9], rb[9];
+#pragma unroll
+  for (int l = 0; l < 9; ++l) {
+    const double a = w[((size_t)l0 * 9 + l) * nbp];
+    ra[l] = a * xo[l];
+    if (q2 >= 0) rb[l] = w2[((size_t)l0 * 9 + l) * nbp] * xo[l];
+  }
+  const int nsc = op.nsc[kap];
+  const uint32_t* fl = op.flip + (size_t)i * nsm;
+  for (int k = 0; k < nsc; ++k) {
+    const uint32_t e = fl[k];
+    if (k == l0 || e == 0u) continue;
+    const int r = (int)(e >> 6) - 1, g = (int)(e & 63u);
+    const double isr = xnorm ? op.isq[r] : 1.0;
+    const double* __restrict__ cg = cs + g * 9;
+    const uint32_t voff = (uint32_t)r * node16 + coff;
+#pragma unroll
+    for (int l = 0; l < 9; ++l) {
+      const auto v = __builtin_amdgcn_raw_buffer_load_b128(xr, (int)voff, (int)(l * vs16), 0);
+      const double c = isr * cg[l];
+      const cplx xv = cplx{c * __builtin_bit_cast(double, (unsigned long long)v[0] | ((unsigned long long)v[1] << 32)),
+                           c * __builtin_bit_cast(double, (unsigned long long)v[2] | ((unsigned long long)v[3] << 32))};
+      ra[l] = fma_s(w[((size_t)k * 9 + l) * nbp], xv, ra[l]);
+      if (q2 >= 0) rb[l] = fma_s(w2[((size_t)k * 9 + l) * nbp], xv, rb[l]);
+    }
+  }
+#pragma unroll
+  for (int l = 0; l < 9; ++l) {
+    acc[l] = fma_s(coef, ra[l], acc[l]);
+    if (q2 >= 0) acc2[l] = fma_s(coef, rb[l], acc2[l]);
+  }
+}
+
+template <int MT, int NSLOT>
+__global__ void __launch_bounds__(BLOCK, 4) k_ctile(OpDev op, MemDev md, FArgs fa) {
+  extern __shared__ double bls[];
+  __shared__ cplx red[BLOCK];
+  if (md.tact && !md.tact[blockIdx.y]) return;  // every member of this tile has left
+  // LDS: the padded bond blocks, then the gather scales e[member][g][l] = dd(l) chi(g) sg(g, l) of the tile's members
+  const int nblk = op.ntype * (op.nslot + 1) * 90;
+  double* etab = bls + nblk;
+  double* cst = etab + MT * (op.nop + 1) * 9;  // chi(g) sg(g, l) per member of the tile
+  for (int q = threadIdx.x; q < MT * op.nop * 9; q += BLOCK) {
+    const int ml = q / (op.nop * 9), g = (q / 9) % op.nop, l = q % 9;
+    const int mm = min(blockIdx.y * MT + ml, md.nb - 1);
+    cst[q] = (((md.chimask[mm] >> g) & 1ull) ? -1.0 : 1.0) * op.sg[g * 9 + l];
+  }
+  if (!fa.xs) __syncthreads();
+  if (fa.xs) {
+    for (int q = threadIdx.x; q < nblk; q += BLOCK) bls[q] = op.blkp[q];
+    // row g = nop: the on-site term (no operation)
+    const int ng = op.nop + 1, ne = MT * ng * 9;
+    for (int q = threadIdx.x; q < ne; q += BLOCK) {
+      const int ml = q / (ng * 9), g = (q / 9) % ng, l = q % 9;
+      const int mm = min(blockIdx.y * MT + ml, md.nb - 1);
+      const double d = fa.s_din ? md.dvec[(size_t)l * md.nbp + mm] : 1.0;
+      const double chi = (g < op.nop && ((md.chimask[mm] >> g) & 1ull)) ? -1.0 : 1.0;
+      etab[q] = d * chi * (g < op.nop ? op.sg[g * 9 + l] : 1.0);
+    }
+    __syncthreads();
+  }
+  constexpr int npw = WAVE / MT;
+  const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x >> 6;
+  const int ml = lane & (MT - 1);
+  const int m = blockIdx.y * MT + ml;
+  const int i = (xcd_block(blockIdx.x, gridDim.x) * (BLOCK / WAVE) + wave) * npw + lane / MT;
+  const bool live = (i < op.nn) && (m < md.nb);
+  const int nbp = md.nbp;
+  cplx dotv = zero<cplx>();
+  if (live) {
+    const uint32_t inf = op.info[i];
+    const uint64_t chim = md.chimask[m];
+    cplx acc[9];
+    for (int l = 0; l < 9; ++l) acc[l] = zero<cplx>();
+    if (fa.xs) {
+      const uint32_t vs16 = (uint32_t)nbp * 16u, node16 = 9u * vs16, coff = (uint32_t)m * 16u;
+      const __amdgpu_buffer_rsrc_t xr = __builtin_amdgcn_make_buffer_rsrc((void*)fa.xs, (short)0,
+                                                                          (int)((uint32_t)(op.nn + 1) * node16), 0x00020000);
+      const double* __restrict__ bl = bls + (size_t)((inf >> 12) & 255u) * (NSLOT + 1) * 90;
+      const double* __restrict__ em = etab + (size_t)ml * (op.nop + 1) * 9;
+      const uint32_t* __restrict__ hp = op.hop + (size_t)i * NSLOT;
+#pragma unroll 1
+      for (int k = 0; k <= NSLOT; ++k) {
+        // k = 0: the node itself with the on-site block; k > 0: link k - 1
+        const uint32_t e = hp[k > 0 ? k - 1 : 0];
+        const int r = k > 0 ? (int)(e >> 6) - 1 : i;
+        const int g = k > 0 ? (int)(e & 63u) : op.nop;
+        const uint32_t voff = (uint32_t)r * node16 + coff;
+        double xre[9], xim[9];
+#pragma unroll
+        for (int l = 0; l < 9; ++l) {
+          const auto q = __builtin_amdgcn_raw_buffer_load_b128(xr, (int)voff, (int)(l * vs16), 0);
+          xre[l] = __builtin_bit_cast(double, (unsigned long long)q[0] | ((unsigned long long)q[1] << 32));
+          xim[l] = __builtin_bit_cast(double, (unsigned long long)q[2] | ((unsigned long long)q[3] << 32));
+        }
+        const double sc = fa.xs_norm ? op.isq[r] : 1.0;
+        const double* __restrict__ eg = em + g * 9;
+        cplx v[9];
+#pragma unroll
+        for (int l = 0; l < 9; ++l) {
+          const double c = sc * eg[l];
+          v[l] = cplx{c * xre[l], c * xim[l]};
+        }
+        const double* __restrict__ bk = bl + k * 90;
+#pragma unroll
+        for (int l = 0; l < 9; ++l) {
+          cplx a = acc[l];
+#pragma unroll
+          for (int l2 = 0; l2 < 9; ++l2) a = fma_s(bk[l * 10 + l2], v[l2], a);
+          acc[l] = a;
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      }
+      for (int l = 0; l < 9; ++l) acc[l] = (fa.s_coef * (fa.s_dout ? md.dvec[(size_t)l * nbp + m] : 1.0)) * acc[l];
+    }
+    {
+      const uint32_t vs16 = (uint32_t)nbp * 16u, node16 = 9u * vs16, coff = (uint32_t)m * 16u;
+      const int nrec = (int)((uint32_t)(op.nn + 1) * node16);
+      const double* __restrict__ cs = cst + (size_t)ml * op.nop * 9;
+      for (int t = 0; t < fa.nr; ++t) {
+        const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc((void*)fa.xr[t], (short)0, nrec, 0x00020000);
+        rand_ct(op, md, cs, i, inf, m, node16, vs16, coff, rr, fa.r_norm[t], fa.qr[t], -1, fa.r_coef[t], acc, acc);
+      }
+      if (fa.xl) {
+        // X(z) x~ = z X(pa) x~ + X(pb) x~, both from one pass over x
+        const __amdgpu_buffer_rsrc_t rl = __builtin_amdgcn_make_buffer_rsrc((void*)fa.xl, (short)0, nrec, 0x00020000);
+        cplx ta[9], tb[9];
+        for (int l = 0; l < 9; ++l) { ta[l] = zero<cplx>(); tb[l] = zero<cplx>(); }
+        rand_ct(op, md, cs, i, inf, m, node16, vs16, coff, rl, fa.l_norm, QPA, QPB, 1.0, ta, tb);
+        const cplx z = md.z[m];
+        for (int l = 0; l < 9; ++l) acc[l] = acc[l] + fa.l_coef * (z * ta[l] + tb[l]);
+      }
+    }
+    for (int t = 0; t < fa.no; ++t) {
+      const cplx* xo = (const cplx*)fa.xo[t];
+      for (int l = 0; l < 9; ++l) acc[l] = fma_s(fa.o_coef[t], ld(xo, i, l, nbp, m), acc[l]);
+    }
+    const double so = fa.out_sq ? op.sq[i] : 1.0;
+    cplx* out = (cplx*)fa.out;
+    const cplx* du = (const cplx*)fa.dotu;
+    for (int l = 0; l < 9; ++l) {
+      const cplx y = so * acc[l];
+      out[((size_t)i * 9 + l) * nbp + m] = y;
+      if (du) dotv = dotv + bilin(ld(du, i, l, nbp, m), y);
+    }
+  }
+  if (fa.dotu) {
+    cplx* part = (cplx*)fa.partial;
+    block_reduce_member<cplx>(dotv, MT, red, part + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * MT + threadIdx.x);
+  }
+}
+
 // The same fused operator pass with one node per wave and 64 members per wave (member tile 64): the node's links, weights
 // table offsets, the orbital tables of each link's operation and the bond blocks are wave-uniform and come through the
 // scalar cache; only the member amplitudes are per lane, each gather a uniform row address plus the lane's offset.
 // NSLOT > 0: the slot loop is unrolled at compile time (a missing link points at the all-zero padding node nn, so the
 // loop has no branch and the loads of later slots can be issued early).  PIID: every operation of the group maps each
 // orbital onto itself (pi = identity, e.g. D2h), so the member's d enters at the orbital index itself.
+// The S pass of one node for one real column per lane when every operation maps each orbital onto itself (pi = identity):
+//   acc += sum_k f_k B_k (dd . sg_k . x_k),   f_k = chi(g_k) / sq(r_k) (xnorm) or chi(g_k),
+// over the on-site block (k = 0) and the NSLOT links.  The neighbour rows are raw buffer loads: a uniform byte offset
+// (row of the neighbour, orbital) on the scalar unit plus the lane's column offset, so no vector instruction is spent on
+// addresses; the loads of link k + 1 are issued before the block product of link k.  The bond blocks of the node's type
+// come from LDS (rows padded to 10 doubles), one row at a time next to its nine FMAs (a scheduling barrier per row keeps
+// the block out of the registers).  The orbital signs sg_k are sign-bit flips with a uniform mask; chi(g_k) / sq(r_k) enters
+// once per orbital.  Per link and column: 9 loads, 81 FMA, 18 multiplications, 9 sign flips.
+// The caller guarantees that the vector is below 4 GiB (32-bit buffer offsets).
+template <int NSLOT>
+__device__ __forceinline__ void s_link_load(__amdgpu_buffer_rsrc_t xr, uint32_t base, uint32_t vs8, uint32_t coff, double x[9]) {
+#pragma unroll
+  for (int l = 0; l < 9; ++l)
+    x[l] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(xr, (int)coff, (int)(base + l * vs8), 0));
+}
+
+// acc += B (s dd . sign . x): the block B (LDS) one row at a time, each row's nine FMAs next to its reads
+__device__ __forceinline__ void s_link_apply(const double* bk, const double x[9], const double dd[9], double s, uint32_t sgm,
+                                             uint32_t chb, double acc[9]) {
+  double v[9];
+#pragma unroll
+  for (int l = 0; l < 9; ++l) {
+    const double w = x[l] * (dd[l] * s);
+    const int flip = (int)((((sgm >> l) & 1u) ^ chb) << 31);
+    v[l] = __hiloint2double(__double2hiint(w) ^ flip, __double2loint(w));
+  }
+#pragma unroll
+  for (int l = 0; l < 9; ++l) {
+    double a = acc[l];
+#pragma unroll
+    for (int l2 = 0; l2 < 9; ++l2) a = fma(bk[l * 10 + l2], v[l2], a);
+    acc[l] = a;
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+template <int NSLOT>
+__device__ __forceinline__ void s_links_piid(const OpDev& op, const double* bl, const double* x, uint32_t vs8, uint32_t coff,
+                                             int i, uint64_t chim, const double dd[9], bool xnorm, double acc[9]) {
+  const uint32_t node8 = 9u * vs8;
+  const __amdgpu_buffer_rsrc_t xr =
+      __builtin_amdgcn_make_buffer_rsrc((void*)x, (short)0, (int)((uint32_t)(op.nn + 1) * node8), 0x00020000);
+  const uint32_t* __restrict__ hp = op.hop + (size_t)i * NSLOT;
+  // link k = 0 is the node itself (on-site block), k = 1..NSLOT the bonds; two register sets alternate so that the
+  // loads of link k + 1 are in flight during the block product of link k
+  double xa[9], xb[9];
+  s_link_load<NSLOT>(xr, (uint32_t)i * node8, vs8, coff, xa);
+  double sa = xnorm ? op.isq[i] : 1.0, sb = 1.0;
+  uint32_t ga = 0u, gb = 0u, ca = 0u, cb = 0u;
+#pragma unroll 1
+  for (int k = 0; k <= NSLOT; k += 2) {
+    if (k + 1 <= NSLOT) {
+      const uint32_t e = hp[k];
+      const int r = (int)(e >> 6) - 1, g = (int)(e & 63u);
+      s_link_load<NSLOT>(xr, (uint32_t)r * node8, vs8, coff, xb);
+      sb = xnorm ? op.isq[r] : 1.0;
+      gb = op.sgm[g];
+      cb = (uint32_t)(chim >> g) & 1u;
+    }
+    s_link_apply(bl + k * 90, xa, dd, sa, ga, ca, acc);
+    if (k + 1 > NSLOT) break;
+    if (k + 2 <= NSLOT) {
+      const uint32_t e = hp[k + 1];
+      const int r = (int)(e >> 6) - 1, g = (int)(e & 63u);
+      s_link_load<NSLOT>(xr, (uint32_t)r * node8, vs8, coff, xa);
+      sa = xnorm ? op.isq[r] : 1.0;
+      ga = op.sgm[g];
+      ca = (uint32_t)(chim >> g) & 1u;
+    }
+    s_link_apply(bl + (k + 1) * 90, xb, dd, sb, gb, cb, acc);
+  }
+}
+
+
 template <class T, bool PIID, int NSLOT>
 __global__ void __launch_bounds__(BLOCK, 4) k_node(OpDev op, MemDev md, FArgs fa, const double* __restrict__ blk,
                                                    const double* __restrict__ dperm) {
   __shared__ T red[BLOCK];
+  extern __shared__ double blds[];
   if (md.tact && !md.tact[blockIdx.y]) return;  // every member of this tile has left
+  if constexpr (std::is_same<T, double>::value && PIID && NSLOT > 0) {
+    if (fa.xs) stage_blocks(op, blds);
+  }
   const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x >> 6;
   const int i = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * (BLOCK / WAVE) + wave);
   const int m = blockIdx.y * WAVE + lane;
@@ -346,9 +625,15 @@ __global__ void __launch_bounds__(BLOCK, 4) k_node(OpDev op, MemDev md, FArgs fa
     if (fa.xs) {
       const T* __restrict__ x = (const T*)fa.xs;
       const int t = (inf >> 12) & 255u;
+      if constexpr (std::is_same<T, double>::value && PIID && NSLOT > 0) {
+        double dd[9];
+        for (int l = 0; l < 9; ++l) dd[l] = fa.s_din ? md.dvec[(size_t)l * nbp + m] : 1.0;
+        s_links_piid<NSLOT>(op, blds + (size_t)t * (NSLOT + 1) * 90, (const double*)x, (uint32_t)nbp * 8u, (uint32_t)m * 8u, i,
+                            chim, dd, fa.xs_norm != 0, acc);
+      } else {
       const double* __restrict__ b0 = blk + (size_t)t * (op.nslot + 1) * 81;
       {
-        const double sc = fa.xs_norm ? 1.0 / op.sq[i] : 1.0;
+        const double sc = fa.xs_norm ? op.isq[i] : 1.0;
         const T* __restrict__ row = x + (size_t)i * 9 * nbp;
         T xn[9];
         for (int l = 0; l < 9; ++l) xn[l] = (sc * (fa.s_din ? md.dvec[(size_t)l * nbp + m] : 1.0)) * row[(size_t)l * nbp + m];
@@ -366,7 +651,7 @@ __global__ void __launch_bounds__(BLOCK, 4) k_node(OpDev op, MemDev md, FArgs fa
           const uint32_t e = hp[kk];
           const int r = (int)(e >> 6) - 1, g = (int)(e & 63u);
           const double chi = ((chim >> g) & 1ull) ? -1.0 : 1.0;
-          const double sc = fa.xs_norm ? 1.0 / op.sq[r] : 1.0;
+          const double sc = fa.xs_norm ? op.isq[r] : 1.0;
           const T* __restrict__ row = x + (size_t)r * 9 * nbp;
           T v[9];
           for (int l = 0; l < 9; ++l) {
@@ -381,6 +666,7 @@ __global__ void __launch_bounds__(BLOCK, 4) k_node(OpDev op, MemDev md, FArgs fa
             acc[l] = a2;
           }
         }
+      }
       }
       for (int l = 0; l < 9; ++l) acc[l] = (fa.s_coef * (fa.s_dout ? md.dvec[(size_t)l * nbp + m] : 1.0)) * acc[l];
     }
@@ -415,12 +701,12 @@ __device__ void rand_accum_pair(const OpDev& op, const MemDev& md, uint64_t chim
                                 int q, const double* __restrict__ x, int xnorm, double coef, double acc[9]) {
   const int nbp = md.nbp, nsm = op.nsmax;
   const bool fz = (inf >> 20) & 1u;
-  const double sqi = op.sq[i];
+  const double isi = op.isq[i];
   if (fz && md.frozen[m]) {
     const double* fp = md.fpar + (size_t)q * 9 * nbp;
     for (int l = 0; l < 9; ++l) {
       double xv = x[((size_t)i * 9 + l) * vs + vi];
-      if (xnorm) xv = xv / sqi;
+      if (xnorm) xv = isi * xv;
       acc[l] = fma(coef * fp[(size_t)l * nbp + m], xv, acc[l]);
     }
     return;
@@ -430,7 +716,7 @@ __device__ void rand_accum_pair(const OpDev& op, const MemDev& md, uint64_t chim
   double r_acc[9];
   for (int l = 0; l < 9; ++l) {
     double xv = x[((size_t)i * 9 + l) * vs + vi];
-    if (xnorm) xv = xv / sqi;
+    if (xnorm) xv = isi * xv;
     r_acc[l] = w[((size_t)l0 * 9 + l) * nbp + m] * xv;
   }
   const int nsc = op.nsc[kap];
@@ -441,10 +727,10 @@ __device__ void rand_accum_pair(const OpDev& op, const MemDev& md, uint64_t chim
     int r, g;
     unpack_link(e, r, g);
     const double chi = ((chim >> g) & 1ull) ? -1.0 : 1.0;
-    const double sqr = op.sq[r];
+    const double isr = op.isq[r];
     for (int l = 0; l < 9; ++l) {
       double xv = x[((size_t)r * 9 + op.pi[g * 9 + l]) * vs + vi];
-      if (xnorm) xv = xv / sqr;
+      if (xnorm) xv = isr * xv;
       r_acc[l] = fma(w[((size_t)k * 9 + l) * nbp + m], (chi * op.sg[g * 9 + l]) * xv, r_acc[l]);
     }
   }
@@ -455,7 +741,11 @@ template <bool PIID, int NSLOT>
 __global__ void __launch_bounds__(BLOCK, 4) k_pair(OpDev op, MemDev md, FArgs fa, const double* __restrict__ blk,
                                                    const double* __restrict__ dperm) {
   __shared__ cplx red[BLOCK / WAVE][WAVE / 2];
+  extern __shared__ double blds[];
   if (md.tact && !md.tact[blockIdx.y / 2]) return;  // every member of this 64-member tile has left
+  if constexpr (PIID && NSLOT > 0) {
+    if (fa.xs) stage_blocks(op, blds);
+  }
   const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x >> 6;
   const int i = __builtin_amdgcn_readfirstlane(xcd_block(blockIdx.x, gridDim.x) * (BLOCK / WAVE) + wave);
   const int part = lane & 1;
@@ -472,9 +762,15 @@ __global__ void __launch_bounds__(BLOCK, 4) k_pair(OpDev op, MemDev md, FArgs fa
     if (fa.xs) {
       const double* __restrict__ x = (const double*)fa.xs;
       const int t = (inf >> 12) & 255u;
+      if constexpr (PIID && NSLOT > 0) {
+        double dd[9];
+        for (int l = 0; l < 9; ++l) dd[l] = fa.s_din ? md.dvec[(size_t)l * nbp + m] : 1.0;
+        s_links_piid<NSLOT>(op, blds + (size_t)t * (NSLOT + 1) * 90, x, (uint32_t)vs * 8u, (uint32_t)vi * 8u, i, chim, dd,
+                            fa.xs_norm != 0, acc);
+      } else {
       const double* __restrict__ b0 = blk + (size_t)t * (op.nslot + 1) * 81;
       {
-        const double sc = fa.xs_norm ? 1.0 / op.sq[i] : 1.0;
+        const double sc = fa.xs_norm ? op.isq[i] : 1.0;
         const double* __restrict__ row = x + (size_t)i * 9 * vs;
         double xn[9];
         for (int l = 0; l < 9; ++l) xn[l] = (sc * (fa.s_din ? md.dvec[(size_t)l * nbp + m] : 1.0)) * row[(size_t)l * vs + vi];
@@ -492,7 +788,7 @@ __global__ void __launch_bounds__(BLOCK, 4) k_pair(OpDev op, MemDev md, FArgs fa
           const uint32_t e = hp[kk];
           const int r = (int)(e >> 6) - 1, g = (int)(e & 63u);
           const double chi = ((chim >> g) & 1ull) ? -1.0 : 1.0;
-          const double sc = fa.xs_norm ? 1.0 / op.sq[r] : 1.0;
+          const double sc = fa.xs_norm ? op.isq[r] : 1.0;
           const double* __restrict__ row = x + (size_t)r * 9 * vs;
           double v[9];
           for (int l = 0; l < 9; ++l) {
@@ -507,6 +803,7 @@ __global__ void __launch_bounds__(BLOCK, 4) k_pair(OpDev op, MemDev md, FArgs fa
             acc[l] = a2;
           }
         }
+      }
       }
       for (int l = 0; l < 9; ++l) acc[l] *= fa.s_coef * (fa.s_dout ? md.dvec[(size_t)l * nbp + m] : 1.0);
     }
@@ -616,7 +913,7 @@ __global__ void __launch_bounds__(BLOCK) k_adj_grad(OpDev op, MemDev md, const d
   if (live) {
     const int nbp = md.nbp, nsm = op.nsmax;
     const uint32_t inf = op.info[i];
-    const double sqi = op.sq[i], isq = 1.0 / sqi, ab = abar[m];
+    const double sqi = op.sq[i], isq = op.isq[i], ab = abar[m];
     const uint64_t chim = md.chimask[m];
     double yo[9], tx[9], ty[9];
 #pragma unroll
@@ -645,7 +942,7 @@ __global__ void __launch_bounds__(BLOCK) k_adj_grad(OpDev op, MemDev md, const d
         int g;
         unpack_link(e, rn, g);
         k = kk;
-        cs0 = (((chim >> g) & 1ull) ? -1.0 : 1.0) / op.sq[rn];
+        cs0 = (((chim >> g) & 1ull) ? -1.0 : 1.0) * op.isq[rn];
         pg = op.pi + g * 9;
         sgg = op.sg + g * 9;
       }
@@ -1176,7 +1473,8 @@ struct asr_gpu_op {
   std::vector<int> pi_h, nsc_h, spc_h;
   std::vector<double> sg_h, mt_h;  // mt: ncls x nsmax x nsmax x nsmax
   uint32_t *hop = nullptr, *flip = nullptr, *info = nullptr;
-  double *sq = nullptr, *sg = nullptr, *blk = nullptr;
+  double *sq = nullptr, *isq = nullptr, *sg = nullptr, *blk = nullptr, *blkp = nullptr;
+  uint32_t* sgm = nullptr;
   int *stab_ptr = nullptr, *stab = nullptr, *pi = nullptr, *nsc = nullptr;
   hipStream_t stream = nullptr;
   bool have_tables = false, have_ops = false, have_blocks = false, have_classes = false;
@@ -1209,6 +1507,7 @@ OpDev dev_of(const asr_gpu_op* op) {
   d.nn = op->nn; d.nslot = op->nslot; d.nop = op->nop; d.nsmax = op->nsmax; d.ncls = op->ncls; d.nsp = op->nsp;
   d.ntype = op->ntype; d.nstab = op->nstab;
   d.hop = op->hop; d.flip = op->flip; d.info = op->info; d.sq = op->sq; d.stab_ptr = op->stab_ptr; d.stab = op->stab;
+  d.isq = op->isq; d.sgm = op->sgm; d.blkp = op->blkp;
   d.pi = op->pi; d.sg = op->sg; d.blk = op->blk; d.nsc = op->nsc;
   return d;
 }
@@ -1441,9 +1740,11 @@ int launch_fused(const asr_gpu_op* op, const MemDev& md, const FArgs& fa, hipStr
   if (timer_mark(strm, fa.xs ? KIND_SPMM : KIND_RAND, true)) return 1;
   const OpDev od = dev_of(op);
   const double* bl = op->blk;
+  const size_t blds_bytes = (size_t)op->ntype * (op->nslot + 1) * 90 * 8;
   if (md.mt == WAVE && md.pair && sizeof(T) == sizeof(cplx)) {
     grid.y = md.nbp / (WAVE / 2);
-#define ASR_PAIR(PI, NS) hipLaunchKernelGGL((k_pair<PI, NS>), grid, dim3(BLOCK), 0, strm, od, md, fa, bl, md.dperm)
+#define ASR_PAIR(PI, NS) hipLaunchKernelGGL((k_pair<PI, NS>), grid, dim3(BLOCK), (PI && NS > 0) ? blds_bytes : 0, strm, od, md, \
+                                         fa, bl, md.dperm)
     if (op->piid && op->nslot == 18) ASR_PAIR(true, 18);
     else if (op->piid && op->nslot == 14) ASR_PAIR(true, 14);
     else if (op->piid) ASR_PAIR(true, 0);
@@ -1451,13 +1752,23 @@ int launch_fused(const asr_gpu_op* op, const MemDev& md, const FArgs& fa, hipStr
     else ASR_PAIR(false, 0);
 #undef ASR_PAIR
   } else if (md.mt == WAVE) {
-#define ASR_NODE(PI, NS) hipLaunchKernelGGL((k_node<T, PI, NS>), grid, dim3(BLOCK), 0, strm, od, md, fa, bl, md.dperm)
+#define ASR_NODE(PI, NS) hipLaunchKernelGGL((k_node<T, PI, NS>), grid, dim3(BLOCK), \
+                                         (PI && NS > 0 && sizeof(T) == 8) ? blds_bytes : 0, strm, od, md, fa, bl, md.dperm)
     if (op->piid && op->nslot == 18) ASR_NODE(true, 18);
     else if (op->piid && op->nslot == 14) ASR_NODE(true, 14);
     else if (op->piid) ASR_NODE(true, 0);
     else if (op->nslot == 18) ASR_NODE(false, 18);
     else ASR_NODE(false, 0);
 #undef ASR_NODE
+  } else if (sizeof(T) == sizeof(cplx) && (md.mt == 8 || md.mt == 16) && op->piid && (op->nslot == 18 || op->nslot == 14) &&
+             (size_t)(op->nn + 1) * 9 * md.nbp * 16 < ((size_t)1 << 31)) {
+#define ASR_CT(MT, NS) hipLaunchKernelGGL((k_ctile<MT, NS>), grid, dim3(BLOCK), blds_bytes + (size_t)MT * (2 * op->nop + 1) * 9 * 8, \
+                                           strm, od, md, fa)
+    if (md.mt == 8 && op->nslot == 18) ASR_CT(8, 18);
+    else if (md.mt == 8) ASR_CT(8, 14);
+    else if (op->nslot == 18) ASR_CT(16, 18);
+    else ASR_CT(16, 14);
+#undef ASR_CT
   } else {
     hipLaunchKernelGGL(k_fused<T>, grid, dim3(BLOCK), lds, strm, dev_of(op), md, fa);
   }
@@ -2131,7 +2442,7 @@ int asr_gpu_op_set_tables(asr_gpu_op* op, const int* hopr, const int* hopg, cons
   for (int j = 0; j < nstab; ++j)
     if (stab[j] < 0 || stab[j] >= op->nop) return fail("set_tables: stabiliser operation out of range");
   if (!op->hop && (dalloc(&op->hop, hop.size()) || dalloc(&op->flip, flip.size()) || dalloc(&op->info, nn) ||
-                   dalloc(&op->sq, nn + 1) || dalloc(&op->stab_ptr, nn + 1)))
+                   dalloc(&op->sq, nn + 1) || dalloc(&op->isq, nn + 1) || dalloc(&op->stab_ptr, nn + 1)))
     return 1;
   if (op->stab) (void)hipFree(op->stab);
   op->stab = nullptr;
@@ -2144,6 +2455,12 @@ int asr_gpu_op_set_tables(asr_gpu_op* op, const int* hopr, const int* hopg, cons
   {
     const double one = 1.0;
     HCHECK(hipMemcpy(op->sq + nn, &one, 8, hipMemcpyHostToDevice));
+    std::vector<double> isq(nn + 1, 1.0);
+    for (int i = 0; i < nn; ++i) {
+      if (!(sq[i] > 0.0)) return fail("set_tables: sq must be positive");
+      isq[i] = 1.0 / sq[i];
+    }
+    HCHECK(hipMemcpy(op->isq, isq.data(), isq.size() * 8, hipMemcpyHostToDevice));
   }
   HCHECK(hipMemcpy(op->stab_ptr, stab_ptr, (size_t)(nn + 1) * 4, hipMemcpyHostToDevice));
   if (nstab > 0) HCHECK(hipMemcpy(op->stab, stab, (size_t)nstab * 4, hipMemcpyHostToDevice));
@@ -2159,9 +2476,16 @@ int asr_gpu_op_set_orbital_ops(asr_gpu_op* op, const int* pi, const double* sg) 
   op->piid = true;
   for (int q = 0; q < op->nop * 9; ++q)
     if (pi[q] != q % 9) op->piid = false;
-  if (!op->pi && (dalloc(&op->pi, op->nop * 9) || dalloc(&op->sg, op->nop * 9))) return 1;
+  std::vector<uint32_t> sgm(op->nop, 0u);
+  for (int g = 0; g < op->nop; ++g)
+    for (int l = 0; l < 9; ++l) {
+      if (sg[g * 9 + l] != 1.0 && sg[g * 9 + l] != -1.0) return fail("set_orbital_ops: signs must be +-1 (monomial tables)");
+      if (sg[g * 9 + l] < 0.0) sgm[g] |= 1u << l;
+    }
+  if (!op->pi && (dalloc(&op->pi, op->nop * 9) || dalloc(&op->sg, op->nop * 9) || dalloc(&op->sgm, op->nop))) return 1;
   HCHECK(hipMemcpy(op->pi, pi, (size_t)op->nop * 9 * 4, hipMemcpyHostToDevice));
   HCHECK(hipMemcpy(op->sg, sg, (size_t)op->nop * 9 * 8, hipMemcpyHostToDevice));
+  HCHECK(hipMemcpy(op->sgm, sgm.data(), sgm.size() * 4, hipMemcpyHostToDevice));
   op->have_ops = true;
   return 0;
 }
@@ -2174,8 +2498,13 @@ int asr_gpu_op_set_blocks(asr_gpu_op* op, const double* block, const double* blo
     std::memcpy(&blk[(size_t)t * (ns + 1) * 81], block0 + (size_t)t * 81, 81 * 8);
     std::memcpy(&blk[((size_t)t * (ns + 1) + 1) * 81], block + (size_t)t * ns * 81, (size_t)ns * 81 * 8);
   }
-  if (!op->blk && dalloc(&op->blk, blk.size())) return 1;
+  std::vector<double> blkp((size_t)op->ntype * (ns + 1) * 90, 0.0);
+  for (size_t b = 0; b < (size_t)op->ntype * (ns + 1); ++b)
+    for (int l = 0; l < 9; ++l)
+      for (int l2 = 0; l2 < 9; ++l2) blkp[b * 90 + l * 10 + l2] = blk[b * 81 + l * 9 + l2];
+  if (!op->blk && (dalloc(&op->blk, blk.size()) || dalloc(&op->blkp, blkp.size()))) return 1;
   HCHECK(hipMemcpy(op->blk, blk.data(), blk.size() * 8, hipMemcpyHostToDevice));
+  HCHECK(hipMemcpy(op->blkp, blkp.data(), blkp.size() * 8, hipMemcpyHostToDevice));
   op->have_blocks = true;
   return 0;
 }
@@ -2213,6 +2542,7 @@ int asr_gpu_op_get_tile(asr_gpu_op* op, int nchain, int is_complex, int* mt) {
 
 void asr_gpu_op_destroy(asr_gpu_op* op) {
   if (!op) return;
+  for (void* p : {(void*)op->isq, (void*)op->sgm, (void*)op->blkp}) if (p) (void)hipFree(p);
   for (void* p : {(void*)op->hop, (void*)op->flip, (void*)op->info, (void*)op->sq, (void*)op->sg, (void*)op->blk,
                   (void*)op->stab_ptr, (void*)op->stab, (void*)op->pi, (void*)op->nsc})
     if (p) (void)hipFree(p);
