@@ -339,44 +339,48 @@ __global__ void __launch_bounds__(BLOCK) k_fused(OpDev op, MemDev md, FArgs fa) 
 // sign flip per orbital and link, shared by the real and imaginary parts.  The rest of the pass (species-diagonal terms,
 // X(z), output scaling, the dot partials) is that of k_fused.  Member tiles of 8-16 keep one member slice of a 1-2 GB
 // vector small enough to stay in the last-level caches while all nodes are swept.
-// acc += coef X_q(x~) for complex members (apply_rand with pi = identity): the node's own state and its flip partners,
-// the member's local weights wt(q, class, l0, k, l), the partner's sign chi(g) sg(g, l) from the LDS table cs (member x
-// operation x orbital), raw buffer loads for the rows.  With q2 >= 0 the same gathered values also feed a second
-// parameter set into acc2 (X(pa) and X(pb) of the energy-dependent diagonal share one pass).  Frozen nodes: the frozen
-// species' parameters on the node's own row.
+// acc += coef X(x~) for complex members (apply_rand with pi = identity): the node's own state and its flip partners, the
+// member's local weights, the partner's sign chi(g) sg(g, l) from the LDS table cs (member x operation x orbital), raw
+// buffer loads for the rows.  LIN = false: X = X_q with the real weights wt(q, class, l0, k, l).  LIN = true: the
+// energy-dependent diagonal X(z) = z X(pa) + X(pb) with the complex weight z wt(pa) + wt(pb) (one pass, one accumulator).
+// Frozen nodes: the frozen species' parameters on the node's own row.
+template <bool LIN>
 __device__ __forceinline__ void rand_ct(const OpDev& op, const MemDev& md, const double* __restrict__ cs, int i, uint32_t inf,
                                         int m, uint32_t node16, uint32_t vs16, uint32_t coff, __amdgpu_buffer_rsrc_t xr,
-                                        int xnorm, int q, int q2, double coef, cplx acc[9], cplx acc2[9]) {
+                                        int xnorm, int q, double coef, cplx acc[9]) {
   const int nbp = md.nbp, nsm = op.nsmax;
   const bool fz = ((inf >> 20) & 1u) && md.frozen[m];
   const double isi = xnorm ? op.isq[i] : 1.0;
-  cplx xo[9];
-#pragma unroll
-  for (int l = 0; l < 9; ++l) {
-    const auto v = __builtin_amdgcn_raw_buffer_load_b128(xr, (int)((uint32_t)i * node16 + coff), (int)(l * vs16), 0);
-    xo[l] = cplx{__builtin_bit_cast(double, (unsigned long long)v[0] | ((unsigned long long)v[1] << 32)) * isi,
-                 __builtin_bit_cast(double, (unsigned long long)v[2] | ((unsigned long long)v[3] << 32)) * isi};
-  }
-  if (fz) {
-    const double* fp = md.fpar + (size_t)q * 9 * nbp;
-    const double* fp2 = md.fpar + (size_t)(q2 < 0 ? q : q2) * 9 * nbp;
-#pragma unroll
-    for (int l = 0; l < 9; ++l) {
-      acc[l] = fma_s(coef * fp[(size_t)l * nbp + m], xo[l], acc[l]);
-      if (q2 >= 0) acc2[l] = fma_s(coef * fp2[(size_t)l * nbp + m], xo[l], acc2[l]);
+  const cplx z = LIN ? md.z[m] : cplx{0.0, 0.0};
+  const int qa = LIN ? QPA : q, qb = LIN ? QPB : q;
+  auto row = [&](uint32_t voff, int l) -> cplx {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(xr, (int)voff, (int)(l * vs16), 0);
+    return cplx{__builtin_bit_cast(double, (unsigned long long)v[0] | ((unsigned long long)v[1] << 32)),
+                __builtin_bit_cast(double, (unsigned long long)v[2] | ((unsigned long long)v[3] << 32))};
+  };
+  // acc += (coef (z wa + wb) or coef wa) xv
+  auto add = [&](double wa, double wb, cplx xv, int l) {
+    if (LIN) {
+      const cplx cw = cplx{coef * fma(z.re, wa, wb), coef * z.im * wa};
+      acc[l] = acc[l] + cw * xv;
+    } else {
+      acc[l] = fma_s(coef * wa, xv, acc[l]);
     }
+  };
+  const uint32_t own = (uint32_t)i * node16 + coff;
+  if (fz) {
+    const double* fa_ = md.fpar + (size_t)qa * 9 * nbp + m;
+    const double* fb_ = md.fpar + (size_t)qb * 9 * nbp + m;
+#pragma unroll
+    for (int l = 0; l < 9; ++l) add(fa_[(size_t)l * nbp], LIN ? fb_[(size_t)l * nbp] : 0.0, isi * row(own, l), l);
     return;
   }
   const int l0 = inf & 15u, kap = (inf >> 4) & 255u;
-  const double* w = md.wts + (((size_t)q * op.ncls + kap) * nsm + l0) * nsm * 9 * nbp + m;
-  const double* w2 = md.wts + (((size_t)(q2 < 0 ? q : q2) * op.ncls + kap) * nsm + l0) * nsm * 9 * nbp + m;
-  cplx ra[9], rb[9];
+  const double* wa_ = md.wts + (((size_t)qa * op.ncls + kap) * nsm + l0) * nsm * 9 * nbp + m;
+  const double* wb_ = md.wts + (((size_t)qb * op.ncls + kap) * nsm + l0) * nsm * 9 * nbp + m;
 #pragma unroll
-  for (int l = 0; l < 9; ++l) {
-    const double a = w[((size_t)l0 * 9 + l) * nbp];
-    ra[l] = a * xo[l];
-    if (q2 >= 0) rb[l] = w2[((size_t)l0 * 9 + l) * nbp] * xo[l];
-  }
+  for (int l = 0; l < 9; ++l)
+    add(wa_[((size_t)l0 * 9 + l) * nbp], LIN ? wb_[((size_t)l0 * 9 + l) * nbp] : 0.0, isi * row(own, l), l);
   const int nsc = op.nsc[kap];
   const uint32_t* fl = op.flip + (size_t)i * nsm;
   for (int k = 0; k < nsc; ++k) {
@@ -387,19 +391,8 @@ __device__ __forceinline__ void rand_ct(const OpDev& op, const MemDev& md, const
     const double* __restrict__ cg = cs + g * 9;
     const uint32_t voff = (uint32_t)r * node16 + coff;
 #pragma unroll
-    for (int l = 0; l < 9; ++l) {
-      const auto v = __builtin_amdgcn_raw_buffer_load_b128(xr, (int)voff, (int)(l * vs16), 0);
-      const double c = isr * cg[l];
-      const cplx xv = cplx{c * __builtin_bit_cast(double, (unsigned long long)v[0] | ((unsigned long long)v[1] << 32)),
-                           c * __builtin_bit_cast(double, (unsigned long long)v[2] | ((unsigned long long)v[3] << 32))};
-      ra[l] = fma_s(w[((size_t)k * 9 + l) * nbp], xv, ra[l]);
-      if (q2 >= 0) rb[l] = fma_s(w2[((size_t)k * 9 + l) * nbp], xv, rb[l]);
-    }
-  }
-#pragma unroll
-  for (int l = 0; l < 9; ++l) {
-    acc[l] = fma_s(coef, ra[l], acc[l]);
-    if (q2 >= 0) acc2[l] = fma_s(coef, rb[l], acc2[l]);
+    for (int l = 0; l < 9; ++l)
+      add(wa_[((size_t)k * 9 + l) * nbp], LIN ? wb_[((size_t)k * 9 + l) * nbp] : 0.0, (isr * cg[l]) * row(voff, l), l);
   }
 }
 
@@ -491,16 +484,12 @@ __global__ void __launch_bounds__(BLOCK, 4) k_ctile(OpDev op, MemDev md, FArgs f
       const double* __restrict__ cs = cst + (size_t)ml * op.nop * 9;
       for (int t = 0; t < fa.nr; ++t) {
         const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc((void*)fa.xr[t], (short)0, nrec, 0x00020000);
-        rand_ct(op, md, cs, i, inf, m, node16, vs16, coff, rr, fa.r_norm[t], fa.qr[t], -1, fa.r_coef[t], acc, acc);
+        rand_ct<false>(op, md, cs, i, inf, m, node16, vs16, coff, rr, fa.r_norm[t], fa.qr[t], fa.r_coef[t], acc);
       }
       if (fa.xl) {
-        // X(z) x~ = z X(pa) x~ + X(pb) x~, both from one pass over x
+        // X(z) x~ = z X(pa) x~ + X(pb) x~ in one pass over x
         const __amdgpu_buffer_rsrc_t rl = __builtin_amdgcn_make_buffer_rsrc((void*)fa.xl, (short)0, nrec, 0x00020000);
-        cplx ta[9], tb[9];
-        for (int l = 0; l < 9; ++l) { ta[l] = zero<cplx>(); tb[l] = zero<cplx>(); }
-        rand_ct(op, md, cs, i, inf, m, node16, vs16, coff, rl, fa.l_norm, QPA, QPB, 1.0, ta, tb);
-        const cplx z = md.z[m];
-        for (int l = 0; l < 9; ++l) acc[l] = acc[l] + fa.l_coef * (z * ta[l] + tb[l]);
+        rand_ct<true>(op, md, cs, i, inf, m, node16, vs16, coff, rl, fa.l_norm, 0, fa.l_coef, acc);
       }
     }
     for (int t = 0; t < fa.no; ++t) {
