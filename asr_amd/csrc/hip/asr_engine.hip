@@ -1444,6 +1444,82 @@ __global__ void __launch_bounds__(BLOCK) k_kpm_v2(RealDev R, const double* __res
   }
 }
 
+// The Chebyshev step on vector slices: the 64-vector rows are processed as slices of 16 vectors (one 128-byte line per
+// site and orbital), 8 lanes per site with two vectors each, 8 sites per wave; blockIdx.y is the slice, so one slice of
+// all sites is swept before the next and the slice of the input (a quarter of the vector) stays in the last-level caches
+// while its rows are gathered 19 times.  Bond blocks from LDS, one row at a time.  Same arithmetic and partials layout as
+// k_kpm_v2 (partials indexed by vector within its 64-vector tile).
+template <int NSLOT>
+__global__ void __launch_bounds__(BLOCK) k_kpm_s16(RealDev R, int nvec, int nvp, const double* __restrict__ u1,
+                                                   const double* __restrict__ u0, double* __restrict__ u2, double cshift,
+                                                   double alpha, double beta, uint64_t seed, double* partial) {
+  extern __shared__ double bls[];
+  __shared__ d2 red[BLOCK / WAVE][8][8];
+  const int nb = (NSLOT + 1) * 81;
+  for (int q = threadIdx.x; q < (NSLOT + 1) * 90; q += BLOCK) bls[q] = (q % 10 < 9) ? R.blk[(q / 90) * 81 + ((q % 90) / 10) * 9 + q % 10] : 0.0;
+  (void)nb;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, sl = lane >> 3, vp = lane & 7;
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int site = (lb * (BLOCK / WAVE) + wave) * 8 + sl;
+  const int v = blockIdx.y * 16 + 2 * vp;
+  d2 dot = {0.0, 0.0};
+  if (site < R.nsite) {
+    const int sp = R.spec[site];
+    const double* cp = R.par + (size_t)sp * 9;
+    const double* dp = R.par + (size_t)(R.nsp + sp) * 9;
+    d2 acc[9], xo[9];
+    for (int l = 0; l < 9; ++l) acc[l] = d2{0.0, 0.0};
+    const int* __restrict__ nbl = R.nbr + (size_t)site * NSLOT;
+#pragma unroll 1
+    for (int k = 0; k <= NSLOT; ++k) {
+      const int j = k == 0 ? site : nbl[k - 1];
+      const d2* __restrict__ row = (const d2*)(u1 + (size_t)j * 9 * nvp + v);
+      d2 xn[9];
+#pragma unroll
+      for (int l = 0; l < 9; ++l) xn[l] = row[(size_t)l * (nvp / 2)];
+      if (k == 0)
+        for (int l = 0; l < 9; ++l) xo[l] = xn[l];
+      const double* __restrict__ bk = bls + k * 90;
+#pragma unroll
+      for (int l = 0; l < 9; ++l) {
+        d2 a = acc[l];
+#pragma unroll
+        for (int l2 = 0; l2 < 9; ++l2) {
+          a.x = fma(bk[l * 10 + l2], xn[l2].x, a.x);
+          a.y = fma(bk[l * 10 + l2], xn[l2].y, a.y);
+        }
+        acc[l] = a;
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+    for (int l = 0; l < 9; ++l) {
+      const uint64_t row = (uint64_t)site * 9 + l;
+      const double r0 = rademacher(seed, (uint64_t)v, row), r1 = rademacher(seed, (uint64_t)v + 1, row);
+      const d2 t0v = ((const d2*)(u0 + row * nvp + v))[0];
+      const double t1x = xo[l].x / dp[l], t1y = xo[l].y / dp[l];
+      double t2x = alpha * (cp[l] * t1x + dp[l] * acc[l].x - cshift * t1x) - beta * (t0v.x / dp[l]);
+      double t2y = alpha * (cp[l] * t1y + dp[l] * acc[l].y - cshift * t1y) - beta * (t0v.y / dp[l]);
+      if (v < nvec) u2[row * nvp + v] = dp[l] * t2x;
+      if (v + 1 < nvec) u2[row * nvp + v + 1] = dp[l] * t2y;
+      dot.x = fma(r0, t2x, dot.x);
+      dot.y = fma(r1, t2y, dot.y);
+    }
+  }
+  // per (block, vector): the 8 sites of the wave, then the 4 waves in order
+  red[wave][sl][vp] = dot;
+  __syncthreads();
+  if (threadIdx.x < 8) {
+    d2 s2 = {0.0, 0.0};
+    for (int w = 0; w < BLOCK / WAVE; ++w)
+      for (int q = 0; q < 8; ++q) { s2.x += red[w][q][threadIdx.x].x; s2.y += red[w][q][threadIdx.x].y; }
+    const int vv = blockIdx.y * 16 + 2 * threadIdx.x;
+    double* pp = partial + ((size_t)(vv / 64) * gridDim.x + lb) * 64 + (vv % 64);
+    pp[0] = s2.x;
+    pp[1] = s2.y;
+  }
+}
+
 // mu_per[v][n] = (1/dim) sum_blocks partial, one block per vector, fixed order
 __global__ void __launch_bounds__(BLOCK) k_kpm_fin(const double* partial, int nbx, int vt, int nvec, double inv_dim,
                                                    double* mu_per, int nmom, int n) {
@@ -1474,7 +1550,8 @@ struct asr_gpu_op {
 
 struct asr_gpu_real {
   int nsite, nslot, nsp, vt = 0;
-  bool v2 = true;  // Chebyshev steps with two vectors per lane
+  bool v2 = true;    // Chebyshev steps with two vectors per lane
+  bool s16 = false;  // ... on 16-vector slices (8 sites per wave)
   int* nbr = nullptr;
   unsigned char* spec = nullptr;
   double *par = nullptr, *blk = nullptr;
@@ -2738,8 +2815,10 @@ static dim3 kpm_grid(const asr_gpu_real* R, int vt, int nvp) {
 
 // Vector tile of the KPM step (vectors per site in a wave: 8, 16, 32 or 64; 0 = automatic).
 int asr_gpu_real_set_tile(asr_gpu_real* R, int vt) {
-  if (vt != 0 && vt != 1 && vt != 2) return fail("asr_gpu_real_set_tile: 0 (automatic), 1 or 2 vectors per lane");
+  if (vt != 0 && vt != 1 && vt != 2 && vt != 3)
+    return fail("asr_gpu_real_set_tile: 0 (automatic), 1 or 2 vectors per lane, 3 = 16-vector slices");
   R->v2 = (vt != 1);
+  R->s16 = (vt == 3);
   return 0;
 }
 
@@ -2797,7 +2876,10 @@ int asr_gpu_kpm_moments(asr_gpu_real* R, double c, double s, int nmom, int nvec,
   for (int n = 2; n < nmom; ++n) {
     // t2 = 2 (H t1 - c t1) / s - t0
     HCHECK(hipEventRecord(ev[2 + 2 * (n - 2)], R->stream));
-    if (R->v2 && R->nslot == 18)
+    if (R->s16 && R->nslot == 18)
+      hipLaunchKernelGGL((k_kpm_s16<18>), dim3((R->nsite + 31) / 32, nvp / 16), dim3(BLOCK), (size_t)19 * 90 * 8, R->stream, rd,
+                         nvec, nvp, (const double*)t1, (const double*)t0, t2, c, 2.0 / s, 1.0, seed, part);
+    else if (R->v2 && R->nslot == 18)
       hipLaunchKernelGGL((k_kpm_v2<18>), dim3((R->nsite + 7) / 8, grid.y), dim3(BLOCK), 0, R->stream, rd,
                          (const double*)R->blk, nvec, nvp, (const double*)t1, (const double*)t0, t2, c, 2.0 / s, 1.0, seed,
                          part);
@@ -2812,8 +2894,8 @@ int asr_gpu_kpm_moments(asr_gpu_real* R, double c, double s, int nmom, int nvec,
       hipLaunchKernelGGL((k_kpm_u<false, 0>), grid, dim3(BLOCK), 0, R->stream, rd, (const double*)R->blk, nvec, nvp,
                          (const double*)t1, (const double*)t0, t2, c, 2.0 / s, 1.0, seed, part, 1);
     HCHECK(hipEventRecord(ev[3 + 2 * (n - 2)], R->stream));
-    hipLaunchKernelGGL(k_kpm_fin, dim3(nvec), dim3(BLOCK), 0, R->stream, part, R->v2 ? (int)((R->nsite + 7) / 8) : (int)grid.x,
-                       vt, nvec, inv_dim, mper, nmom, n);
+    const int nbx = (R->s16 && R->nslot == 18) ? (int)((R->nsite + 31) / 32) : R->v2 ? (int)((R->nsite + 7) / 8) : (int)grid.x;
+    hipLaunchKernelGGL(k_kpm_fin, dim3(nvec), dim3(BLOCK), 0, R->stream, part, nbx, vt, nvec, inv_dim, mper, nmom, n);
     double* tmp = t0; t0 = t1; t1 = t2; t2 = tmp;
   }
   HCHECK(hipEventRecord(ev[1], R->stream));

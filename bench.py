@@ -183,7 +183,12 @@ def main():
         by, fl = chain_kernel_cost(wl.op, nb, k["kind"], EDC_LAUNCH_READS.get(k["launch"], 1))
         tb, tf, tm = tb + by, tf + fl, tm + k["ms"]
     roof = roofline(tb / len(ops), tf / len(ops), tm / len(ops))
-    kname = {65: "k_pair<true, %d>" % wl.op.nslot, 64: "k_node<cplx, true, %d>" % wl.op.nslot}.get(out_tile, "k_fused<cplx>")
+    piid = bool(np.all(np.asarray(wl.op.pi) == np.arange(9)[None, :]))
+    if out_tile in (8, 16) and piid and wl.op.nslot in (14, 18):
+        kname = "k_ctile<%d, %d>" % (out_tile, wl.op.nslot)
+    else:
+        kname = {65: "k_pair<true, %d>" % wl.op.nslot, 64: "k_node<cplx, true, %d>" % wl.op.nslot}.get(out_tile,
+                                                                                                     "k_fused<cplx>")
     roof["kernel"] = kname
     roof["launches_per_level"] = len(ops)
     roof["avg_launch_ms"] = round(tm / len(ops), 4)
@@ -257,7 +262,7 @@ def main():
         cshift, sscale = 0.0, hb / 0.99
         R = engine.GpuRealization(g.nbr, species, par, block, block0)
         tiles = {}
-        for vt in (1, 2):  # vectors per lane, tuned on short recurrences
+        for vt in (1, 2, 3):  # one or two vectors per lane on 64-vector rows, or 16-vector slices; tuned on short runs
             R.set_tile(vt)
             tiles[vt] = R.kpm_moments(cshift, sscale, 12, 64, seed=5)["step_ms"]
         vt_best = min(tiles, key=tiles.get)
@@ -268,7 +273,7 @@ def main():
         wk = time.perf_counter() - t0
         kby, kfl = kpm_kernel_cost(g.n, g.nbr.shape[1], 64)
         kroof = roofline(kby, kfl, res["step_ms"])
-        kroof["kernel"] = "k_kpm_v2<%d>" % g.nbr.shape[1] if vt_best == 2 else "k_kpm_u<false, %d>" % g.nbr.shape[1]
+        kroof["kernel"] = {1: "k_kpm_u<false, %d>", 2: "k_kpm_v2<%d>", 3: "k_kpm_s16<%d>"}[vt_best] % g.nbr.shape[1]
         kroof["algorithmic_bytes_per_launch"] = round(kby)
         kroof["traffic"] = measured_traffic("kpm", kroof["kernel"], g.n, 64)
         out["kpm"] = {"value": round(64 * nmom / wk, 1), "unit": "vector-moments/s", "rows": g.n * 9, "nvec": 64,

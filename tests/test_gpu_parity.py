@@ -134,11 +134,14 @@ def test_edc_plain_equals_batched(ternary_small):
     assert rel(a1, a) < 1e-13 and rel(b1, b) < 1e-13
 
 
-def test_kpm_apply_and_moments():
+@pytest.mark.parametrize("tile", [1, 2, 3])
+def test_kpm_apply_and_moments(tile):
+    """Chebyshev steps with one or two vectors per lane (64-vector rows) or on 16-vector slices, against the oracle."""
     eng = _engine()
     from oracle.oracle import OracleReal
     g, species, par, block, block0 = W.kpm_supercell(L=3, seed=21)
     R = eng.GpuRealization(g.nbr, species, par, block, block0)
+    R.set_tile(tile)
     O = OracleReal(g.nbr, species, par[0], par[1], par[2], par[3], block, block0)
     rng = np.random.default_rng(3)
     x = rng.standard_normal((3, R.dim))
