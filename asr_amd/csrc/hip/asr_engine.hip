@@ -130,7 +130,7 @@ __device__ __forceinline__ void stage_blocks(const OpDev& op, double* lds) {
 template <class T>
 __device__ void s_accum(const OpDev& op, const double* __restrict__ blk, const int* __restrict__ pi,
                         const double* __restrict__ sg, uint64_t chim, int i, int t, int m, int nbp, const T* __restrict__ x,
-                        int xnorm, const double* dv, T acc[9]) {
+                        int xnorm, const double* dv, const double* __restrict__ dperm, T acc[9]) {
   T v[9];
   const double* b0 = blk + (size_t)t * (op.nslot + 1) * 81;
   {
@@ -159,7 +159,8 @@ __device__ void s_accum(const OpDev& op, const double* __restrict__ blk, const i
     for (int l = 0; l < 9; ++l) {
       int p = pg[l];
       T xv = ld(x, r, p, nbp, m);
-      double s = chi * sgg[l] * (dv ? dv[p] : 1.0);
+      // d of the gathered orbital pi(l): from the member's permuted table (a run-time index into dv would spill)
+      double s = chi * sgg[l] * (dv ? dperm[((size_t)g * 9 + l) * nbp + m] : 1.0);
       v[l] = (s * isr) * xv;
     }
     const double* bk = b0 + (size_t)(k + 1) * 81;
@@ -305,7 +306,7 @@ __global__ void __launch_bounds__(BLOCK) k_fused(OpDev op, MemDev md, FArgs fa) 
       for (int l = 0; l < 9; ++l) dv[l] = md.dvec[(size_t)l * md.nbp + m];
       T sacc[9];
       for (int l = 0; l < 9; ++l) sacc[l] = zero<T>();
-      s_accum<T>(op, blk, pi, sg, chim, i, (inf >> 12) & 255u, m, md.nbp, (const T*)fa.xs, fa.xs_norm, fa.s_din ? dv : nullptr,
+      s_accum<T>(op, blk, pi, sg, chim, i, (inf >> 12) & 255u, m, md.nbp, (const T*)fa.xs, fa.xs_norm, fa.s_din ? dv : nullptr, md.dperm,
                  sacc);
       for (int l = 0; l < 9; ++l) acc[l] = fma_s(fa.s_coef * (fa.s_dout ? dv[l] : 1.0), sacc[l], acc[l]);
     }
@@ -1170,7 +1171,13 @@ __global__ void __launch_bounds__(BLOCK) k_update(OpDev op, MemDev md, T* w, con
         for (int j = s0; j < s1; ++j) {
           int g = op.stab[j];
           double chi = ((chim >> g) & 1ull) ? -1.0 : 1.0;
-          for (int l = 0; l < 9; ++l) acc[l] = acc[l] + (chi * op.sg[g * 9 + l]) * x[op.pi[g * 9 + l]];
+          for (int l = 0; l < 9; ++l) {
+            // x[pi(l)] recomputed from memory (w is written below): a register array indexed at run time would live
+            // in scratch memory
+            const size_t q2 = ((size_t)i * 9 + op.pi[g * 9 + l]) * md.nbp + m;
+            const T xv = w[q2] - bilin(a, u[q2]) - bilin(b, up[q2]);
+            acc[l] = acc[l] + (chi * op.sg[g * 9 + l]) * xv;
+          }
         }
         for (int l = 0; l < 9; ++l) x[l] = acc[l] / (double)ns;
       }
