@@ -398,7 +398,7 @@ __device__ __forceinline__ void rand_ct(const OpDev& op, const MemDev& md, const
 }
 
 template <int MT, int NSLOT>
-__global__ void __launch_bounds__(BLOCK, 4) k_ctile(OpDev op, MemDev md, FArgs fa) {
+__global__ void __launch_bounds__(BLOCK, 4) k_ctile(OpDev op, MemDev md, FArgs fa, const double* __restrict__ sblk) {
   extern __shared__ double bls[];
   __shared__ cplx red[BLOCK];
   if (md.tact && !md.tact[blockIdx.y]) return;  // every member of this tile has left
@@ -467,14 +467,36 @@ __global__ void __launch_bounds__(BLOCK, 4) k_ctile(OpDev op, MemDev md, FArgs f
           const double c = sc * eg[l];
           v[l] = cplx{c * xre[l], c * xim[l]};
         }
-        const double* __restrict__ bk = bl + k * 90;
+        if (op.ntype == 1) {
+          // one site type: the block is uniform over the wave, its rows come through scalar loads
+          const double* __restrict__ bk = sblk + (size_t)k * 90;
+          double bc[9], bn[9];
 #pragma unroll
-        for (int l = 0; l < 9; ++l) {
-          cplx a = acc[l];
+          for (int l2 = 0; l2 < 9; ++l2) bc[l2] = bk[l2];
 #pragma unroll
-          for (int l2 = 0; l2 < 9; ++l2) a = fma_s(bk[l * 10 + l2], v[l2], a);
-          acc[l] = a;
-          __builtin_amdgcn_sched_barrier(0);
+          for (int l = 0; l < 9; ++l) {
+            if (l < 8) {
+#pragma unroll
+              for (int l2 = 0; l2 < 9; ++l2) bn[l2] = bk[(l + 1) * 10 + l2];
+            }
+            cplx a = acc[l];
+#pragma unroll
+            for (int l2 = 0; l2 < 9; ++l2) a = fma_s(bc[l2], v[l2], a);
+            acc[l] = a;
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int l2 = 0; l2 < 9; ++l2) bc[l2] = bn[l2];
+          }
+        } else {
+          const double* __restrict__ bk = bl + k * 90;
+#pragma unroll
+          for (int l = 0; l < 9; ++l) {
+            cplx a = acc[l];
+#pragma unroll
+            for (int l2 = 0; l2 < 9; ++l2) a = fma_s(bk[l * 10 + l2], v[l2], a);
+            acc[l] = a;
+            __builtin_amdgcn_sched_barrier(0);
+          }
         }
       }
       for (int l = 0; l < 9; ++l) acc[l] = (fa.s_coef * (fa.s_dout ? md.dvec[(size_t)l * nbp + m] : 1.0)) * acc[l];
@@ -1836,7 +1858,7 @@ int launch_fused(const asr_gpu_op* op, const MemDev& md, const FArgs& fa, hipStr
   } else if (sizeof(T) == sizeof(cplx) && (md.mt == 8 || md.mt == 16) && op->piid && (op->nslot == 18 || op->nslot == 14) &&
              (size_t)(op->nn + 1) * 9 * md.nbp * 16 < ((size_t)1 << 31)) {
 #define ASR_CT(MT, NS) hipLaunchKernelGGL((k_ctile<MT, NS>), grid, dim3(BLOCK), blds_bytes + (size_t)MT * (2 * op->nop + 1) * 9 * 8, \
-                                           strm, od, md, fa)
+                                           strm, od, md, fa, (const double*)op->blkp)
     if (md.mt == 8 && op->nslot == 18) ASR_CT(8, 18);
     else if (md.mt == 8) ASR_CT(8, 14);
     else if (op->nslot == 18) ASR_CT(16, 18);
