@@ -332,36 +332,45 @@ __global__ void __launch_bounds__(BLOCK) k_fused(OpDev op, MemDev md, FArgs fa) 
   }
 }
 
-// The fused operator pass for complex members with a small member tile MT (8 or 16; 64 / MT nodes per wave, one complex
-// member per lane), every operation mapping each orbital onto itself (pi = identity) and NSLOT links per node.  The S pass
-// reads each neighbour row (9 complex values of the lane's member) with raw buffer loads: the lane's offset (row of the
-// neighbour, member) is one integer operation per link, the orbital offset a scalar register; the padded bond blocks come
-// from LDS one row at a time.  The scale of a gathered value, dd(l) chi(g) sg(g, l) / sq(r), is one multiplication and a
-// sign flip per orbital and link, shared by the real and imaginary parts.  The rest of the pass (species-diagonal terms,
-// X(z), output scaling, the dot partials) is that of k_fused.  Member tiles of 8-16 keep one member slice of a 1-2 GB
-// vector small enough to stay in the last-level caches while all nodes are swept.
-// acc += coef X(x~) for complex members (apply_rand with pi = identity): the node's own state and its flip partners, the
+// The fused operator pass with a small member tile MT (8 or 16; 64 / MT nodes per wave, one member per lane, real or
+// complex members), every operation mapping each orbital onto itself (pi = identity) and NSLOT links per node.  The S pass
+// reads each neighbour row (9 values of the lane's member) with raw buffer loads: the lane's offset (row of the
+// neighbour, member) is one integer operation per link, the orbital offset a scalar register.  With one site type the
+// bond block is uniform over the wave and its rows are scalar operands of the FMAs (the next row prefetched); otherwise
+// the padded blocks come from LDS one row at a time.  The scale of a gathered value, dd(l) chi(g) sg(g, l) / sq(r), comes
+// from an LDS table and is shared by the real and imaginary parts.  The species-diagonal terms use the same loads and an
+// LDS sign table; X(z) = z X(pa) + X(pb) is one pass with a complex weight.  Member tiles of 8-16 keep one member slice
+// of a 1-2 GB vector small enough to stay in the last-level caches while all nodes are swept.
+// Row l of the lane's member at node offset voff: 16-byte (complex) or 8-byte (real) raw buffer load.
+template <class T>
+__device__ __forceinline__ T tile_row(__amdgpu_buffer_rsrc_t xr, uint32_t voff, uint32_t soff) {
+  if constexpr (std::is_same<T, cplx>::value) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(xr, (int)voff, (int)soff, 0);
+    return cplx{__builtin_bit_cast(double, (unsigned long long)v[0] | ((unsigned long long)v[1] << 32)),
+                __builtin_bit_cast(double, (unsigned long long)v[2] | ((unsigned long long)v[3] << 32))};
+  } else {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(xr, (int)voff, (int)soff, 0));
+  }
+}
+
+// acc += coef X(x~) for the members of a tile (apply_rand with pi = identity): the node's own state and its flip partners, the
 // member's local weights, the partner's sign chi(g) sg(g, l) from the LDS table cs (member x operation x orbital), raw
 // buffer loads for the rows.  LIN = false: X = X_q with the real weights wt(q, class, l0, k, l).  LIN = true: the
 // energy-dependent diagonal X(z) = z X(pa) + X(pb) with the complex weight z wt(pa) + wt(pb) (one pass, one accumulator).
 // Frozen nodes: the frozen species' parameters on the node's own row.
-template <bool LIN>
+template <class T, bool LIN>
 __device__ __forceinline__ void rand_ct(const OpDev& op, const MemDev& md, const double* __restrict__ cs, int i, uint32_t inf,
                                         int m, uint32_t node16, uint32_t vs16, uint32_t coff, __amdgpu_buffer_rsrc_t xr,
-                                        int xnorm, int q, double coef, cplx acc[9]) {
+                                        int xnorm, int q, double coef, T acc[9]) {
   const int nbp = md.nbp, nsm = op.nsmax;
   const bool fz = ((inf >> 20) & 1u) && md.frozen[m];
   const double isi = xnorm ? op.isq[i] : 1.0;
-  const cplx z = LIN ? md.z[m] : cplx{0.0, 0.0};
   const int qa = LIN ? QPA : q, qb = LIN ? QPB : q;
-  auto row = [&](uint32_t voff, int l) -> cplx {
-    const auto v = __builtin_amdgcn_raw_buffer_load_b128(xr, (int)voff, (int)(l * vs16), 0);
-    return cplx{__builtin_bit_cast(double, (unsigned long long)v[0] | ((unsigned long long)v[1] << 32)),
-                __builtin_bit_cast(double, (unsigned long long)v[2] | ((unsigned long long)v[3] << 32))};
-  };
+  auto row = [&](uint32_t voff, int l) -> T { return tile_row<T>(xr, voff, l * vs16); };
   // acc += (coef (z wa + wb) or coef wa) xv
-  auto add = [&](double wa, double wb, cplx xv, int l) {
-    if (LIN) {
+  auto add = [&](double wa, double wb, T xv, int l) {
+    if constexpr (LIN && std::is_same<T, cplx>::value) {
+      const cplx z = md.z[m];
       const cplx cw = cplx{coef * fma(z.re, wa, wb), coef * z.im * wa};
       acc[l] = acc[l] + cw * xv;
     } else {
@@ -397,10 +406,11 @@ __device__ __forceinline__ void rand_ct(const OpDev& op, const MemDev& md, const
   }
 }
 
-template <int MT, int NSLOT>
-__global__ void __launch_bounds__(BLOCK, 4) k_ctile(OpDev op, MemDev md, FArgs fa, const double* __restrict__ sblk) {
+template <class T, int MT, int NSLOT>
+__global__ void __launch_bounds__(BLOCK, 4) k_tile(OpDev op, MemDev md, FArgs fa, const double* __restrict__ sblk) {
   extern __shared__ double bls[];
-  __shared__ cplx red[BLOCK];
+  __shared__ T red[BLOCK];
+  constexpr uint32_t ES = sizeof(T);
   if (md.tact && !md.tact[blockIdx.y]) return;  // every member of this tile has left
   // LDS: the padded bond blocks, then the gather scales e[member][g][l] = dd(l) chi(g) sg(g, l) of the tile's members
   const int nblk = op.ntype * (op.nslot + 1) * 90;
@@ -432,14 +442,14 @@ __global__ void __launch_bounds__(BLOCK, 4) k_ctile(OpDev op, MemDev md, FArgs f
   const int i = (xcd_block(blockIdx.x, gridDim.x) * (BLOCK / WAVE) + wave) * npw + lane / MT;
   const bool live = (i < op.nn) && (m < md.nb);
   const int nbp = md.nbp;
-  cplx dotv = zero<cplx>();
+  T dotv = zero<T>();
   if (live) {
     const uint32_t inf = op.info[i];
     const uint64_t chim = md.chimask[m];
-    cplx acc[9];
-    for (int l = 0; l < 9; ++l) acc[l] = zero<cplx>();
+    T acc[9];
+    for (int l = 0; l < 9; ++l) acc[l] = zero<T>();
     if (fa.xs) {
-      const uint32_t vs16 = (uint32_t)nbp * 16u, node16 = 9u * vs16, coff = (uint32_t)m * 16u;
+      const uint32_t vs16 = (uint32_t)nbp * ES, node16 = 9u * vs16, coff = (uint32_t)m * ES;
       const __amdgpu_buffer_rsrc_t xr = __builtin_amdgcn_make_buffer_rsrc((void*)fa.xs, (short)0,
                                                                           (int)((uint32_t)(op.nn + 1) * node16), 0x00020000);
       const double* __restrict__ bl = bls + (size_t)((inf >> 12) & 255u) * (NSLOT + 1) * 90;
@@ -452,21 +462,13 @@ __global__ void __launch_bounds__(BLOCK, 4) k_ctile(OpDev op, MemDev md, FArgs f
         const int r = k > 0 ? (int)(e >> 6) - 1 : i;
         const int g = k > 0 ? (int)(e & 63u) : op.nop;
         const uint32_t voff = (uint32_t)r * node16 + coff;
-        double xre[9], xim[9];
+        T v[9];
 #pragma unroll
-        for (int l = 0; l < 9; ++l) {
-          const auto q = __builtin_amdgcn_raw_buffer_load_b128(xr, (int)voff, (int)(l * vs16), 0);
-          xre[l] = __builtin_bit_cast(double, (unsigned long long)q[0] | ((unsigned long long)q[1] << 32));
-          xim[l] = __builtin_bit_cast(double, (unsigned long long)q[2] | ((unsigned long long)q[3] << 32));
-        }
+        for (int l = 0; l < 9; ++l) v[l] = tile_row<T>(xr, voff, l * vs16);
         const double sc = fa.xs_norm ? op.isq[r] : 1.0;
         const double* __restrict__ eg = em + g * 9;
-        cplx v[9];
 #pragma unroll
-        for (int l = 0; l < 9; ++l) {
-          const double c = sc * eg[l];
-          v[l] = cplx{c * xre[l], c * xim[l]};
-        }
+        for (int l = 0; l < 9; ++l) v[l] = (sc * eg[l]) * v[l];
         if (op.ntype == 1) {
           // one site type: the block is uniform over the wave, its rows come through scalar loads
           const double* __restrict__ bk = sblk + (size_t)k * 90;
@@ -479,7 +481,7 @@ __global__ void __launch_bounds__(BLOCK, 4) k_ctile(OpDev op, MemDev md, FArgs f
 #pragma unroll
               for (int l2 = 0; l2 < 9; ++l2) bn[l2] = bk[(l + 1) * 10 + l2];
             }
-            cplx a = acc[l];
+            T a = acc[l];
 #pragma unroll
             for (int l2 = 0; l2 < 9; ++l2) a = fma_s(bc[l2], v[l2], a);
             acc[l] = a;
@@ -491,7 +493,7 @@ __global__ void __launch_bounds__(BLOCK, 4) k_ctile(OpDev op, MemDev md, FArgs f
           const double* __restrict__ bk = bl + k * 90;
 #pragma unroll
           for (int l = 0; l < 9; ++l) {
-            cplx a = acc[l];
+            T a = acc[l];
 #pragma unroll
             for (int l2 = 0; l2 < 9; ++l2) a = fma_s(bk[l * 10 + l2], v[l2], a);
             acc[l] = a;
@@ -502,35 +504,37 @@ __global__ void __launch_bounds__(BLOCK, 4) k_ctile(OpDev op, MemDev md, FArgs f
       for (int l = 0; l < 9; ++l) acc[l] = (fa.s_coef * (fa.s_dout ? md.dvec[(size_t)l * nbp + m] : 1.0)) * acc[l];
     }
     {
-      const uint32_t vs16 = (uint32_t)nbp * 16u, node16 = 9u * vs16, coff = (uint32_t)m * 16u;
+      const uint32_t vs16 = (uint32_t)nbp * ES, node16 = 9u * vs16, coff = (uint32_t)m * ES;
       const int nrec = (int)((uint32_t)(op.nn + 1) * node16);
       const double* __restrict__ cs = cst + (size_t)ml * op.nop * 9;
       for (int t = 0; t < fa.nr; ++t) {
         const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc((void*)fa.xr[t], (short)0, nrec, 0x00020000);
-        rand_ct<false>(op, md, cs, i, inf, m, node16, vs16, coff, rr, fa.r_norm[t], fa.qr[t], fa.r_coef[t], acc);
+        rand_ct<T, false>(op, md, cs, i, inf, m, node16, vs16, coff, rr, fa.r_norm[t], fa.qr[t], fa.r_coef[t], acc);
       }
-      if (fa.xl) {
-        // X(z) x~ = z X(pa) x~ + X(pb) x~ in one pass over x
-        const __amdgpu_buffer_rsrc_t rl = __builtin_amdgcn_make_buffer_rsrc((void*)fa.xl, (short)0, nrec, 0x00020000);
-        rand_ct<true>(op, md, cs, i, inf, m, node16, vs16, coff, rl, fa.l_norm, 0, fa.l_coef, acc);
+      if constexpr (std::is_same<T, cplx>::value) {
+        if (fa.xl) {
+          // X(z) x~ = z X(pa) x~ + X(pb) x~ in one pass over x
+          const __amdgpu_buffer_rsrc_t rl = __builtin_amdgcn_make_buffer_rsrc((void*)fa.xl, (short)0, nrec, 0x00020000);
+          rand_ct<T, true>(op, md, cs, i, inf, m, node16, vs16, coff, rl, fa.l_norm, 0, fa.l_coef, acc);
+        }
       }
     }
     for (int t = 0; t < fa.no; ++t) {
-      const cplx* xo = (const cplx*)fa.xo[t];
+      const T* xo = (const T*)fa.xo[t];
       for (int l = 0; l < 9; ++l) acc[l] = fma_s(fa.o_coef[t], ld(xo, i, l, nbp, m), acc[l]);
     }
     const double so = fa.out_sq ? op.sq[i] : 1.0;
-    cplx* out = (cplx*)fa.out;
-    const cplx* du = (const cplx*)fa.dotu;
+    T* out = (T*)fa.out;
+    const T* du = (const T*)fa.dotu;
     for (int l = 0; l < 9; ++l) {
-      const cplx y = so * acc[l];
+      const T y = so * acc[l];
       out[((size_t)i * 9 + l) * nbp + m] = y;
       if (du) dotv = dotv + bilin(ld(du, i, l, nbp, m), y);
     }
   }
   if (fa.dotu) {
-    cplx* part = (cplx*)fa.partial;
-    block_reduce_member<cplx>(dotv, MT, red, part + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * MT + threadIdx.x);
+    T* part = (T*)fa.partial;
+    block_reduce_member<T>(dotv, MT, red, part + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * MT + threadIdx.x);
   }
 }
 
@@ -1855,9 +1859,9 @@ int launch_fused(const asr_gpu_op* op, const MemDev& md, const FArgs& fa, hipStr
     else if (op->nslot == 18) ASR_NODE(false, 18);
     else ASR_NODE(false, 0);
 #undef ASR_NODE
-  } else if (sizeof(T) == sizeof(cplx) && (md.mt == 8 || md.mt == 16) && op->piid && (op->nslot == 18 || op->nslot == 14) &&
-             (size_t)(op->nn + 1) * 9 * md.nbp * 16 < ((size_t)1 << 31)) {
-#define ASR_CT(MT, NS) hipLaunchKernelGGL((k_ctile<MT, NS>), grid, dim3(BLOCK), blds_bytes + (size_t)MT * (2 * op->nop + 1) * 9 * 8, \
+  } else if ((md.mt == 8 || md.mt == 16) && op->piid && (op->nslot == 18 || op->nslot == 14) &&
+             (size_t)(op->nn + 1) * 9 * md.nbp * sizeof(T) < ((size_t)1 << 31)) {
+#define ASR_CT(MT, NS) hipLaunchKernelGGL((k_tile<T, MT, NS>), grid, dim3(BLOCK), blds_bytes + (size_t)MT * (2 * op->nop + 1) * 9 * 8, \
                                            strm, od, md, fa, (const double*)op->blkp)
     if (md.mt == 8 && op->nslot == 18) ASR_CT(8, 18);
     else if (md.mt == 8) ASR_CT(8, 14);

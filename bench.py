@@ -185,7 +185,7 @@ def main():
     roof = roofline(tb / len(ops), tf / len(ops), tm / len(ops))
     piid = bool(np.all(np.asarray(wl.op.pi) == np.arange(9)[None, :]))
     if out_tile in (8, 16) and piid and wl.op.nslot in (14, 18):
-        kname = "k_ctile<%d, %d>" % (out_tile, wl.op.nslot)
+        kname = "k_tile<cplx, %d, %d>" % (out_tile, wl.op.nslot)
     else:
         kname = {65: "k_pair<true, %d>" % wl.op.nslot, 64: "k_node<cplx, true, %d>" % wl.op.nslot}.get(out_tile,
                                                                                                      "k_fused<cplx>")
