@@ -15,7 +15,8 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 INCLUDE = os.path.join(ROOT, "include")
 HOST_SRC = [os.path.join(PKG, "csrc", "host", f) for f in ("augcls_build.cpp", "terminators.cpp")]
-HIP_SRC = [os.path.join(PKG, "csrc", "hip", f) for f in ("asr_engine.hip",)]
+HIP_SRC = [os.path.join(PKG, "csrc", "hip", f) for f in ("asr_engine.hip",)] + \
+    [os.path.join(PKG, "csrc", "host", "terminators.cpp")]  # cf_species for the adaptive depth test
 HOST_LIB = os.path.join(PKG, "libasr_host.so")
 HIP_LIB = os.path.join(PKG, "libasr_hip.so")
 

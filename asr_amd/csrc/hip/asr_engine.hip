@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "asr_hip.h"
+#include "asr_host.h"
 
 namespace {
 
@@ -1578,6 +1579,8 @@ struct asr_gpu_op {
   bool have_tables = false, have_ops = false, have_blocks = false, have_classes = false;
   bool piid = false;                              // every operation maps each orbital onto itself
   int tile_pref = 0;                              // member tile of the batched chains, 0 = autotuned
+  double depth_tol = 0.0;                         // adaptive depth of the edc chains (ed_depth_tol), 0 = off
+  int depth_tail = 0;                             // tail of that test (ed_tail): 0 hull, 1 bp, 2 avg2, 3 none
   std::vector<std::pair<long long, int>> tuned;   // (members, complex, uniform d, zero o) key -> member tile
 };
 
@@ -2021,6 +2024,8 @@ struct ChainRun {
   bool lin = false;
   std::vector<int> order;  // sorted member -> input chain
   int* tact = nullptr;     // per member tile: 1 while any member of the tile is active
+  int level_host = 0;      // levels issued
+  std::vector<cplx> gprev; // adaptive depth: G(z) of each chain at the previous tested level
 
   ~ChainRun() {
     if (tact) (void)hipFree(tact);
@@ -2099,14 +2104,71 @@ struct ChainRun {
     return 0;
   }
 
-  // the launches of one level on the group's stream (no host synchronisation)
+  // the launches of one level on the group's stream (no host synchronisation): the operator pass and a_k, then the
+  // update, b_k and the scaling.  With the adaptive depth the host looks at a_k between the two (adaptive_check).
   int step() {
+    if (step_apply()) return 1;
+    if (adaptive_check()) return 1;
+    return step_update();
+  }
+
+  int step_apply() {
     md.tact = tact;
     if (g_timer) g_timer->ord = 0;
     if (apply_chain_op<T>(op, M, md, B.u, B.w, B.t1, B.t2, B.h1, lin, B.u, B.part, stream)) return 1;
     if (timer_mark(stream, KIND_FIN_A, true)) return 1;
     hipLaunchKernelGGL(k_fin_a<T>, dim3(M.nb), dim3(BLOCK), 0, stream, cd, M.nb, B.part, nbx, apply_mt, B.a, B.a_eff);
-    if (timer_mark(stream, 0, false) || timer_mark(stream, KIND_UPDATE, true)) return 1;
+    if (timer_mark(stream, 0, false)) return 1;
+    return 0;
+  }
+
+  // Adaptive depth of edc_green (src/asr_edc.f90:121-127): at levels k = 8, 12, 16, ... (1-based) of a chain that has
+  // not reached its depth, G(z) of the chain terminated at k (cf_species with the operator's tail); from k = 12 on the
+  // chain leaves when |G_k - G_{k-4}| <= depth_tol |G_k|, keeping a_1..a_k and b_1..b_{k-1}.  One host synchronisation
+  // per tested level; the coefficients are a few KB.
+  int adaptive_check() {
+    if constexpr (!std::is_same<T, cplx>::value) return 0;
+    else {
+      const int k1 = level_host + 1;
+      if (!(op->depth_tol > 0.0) || !lin || k1 < 8 || (k1 % 4) != 0) return 0;
+      const int nb = M.nb;
+      std::vector<cplx> ah((size_t)nb * ld), bh((size_t)nb * ld);
+      std::vector<int> act(nb), dep(nb);
+      HCHECK(hipMemcpyAsync(ah.data(), B.a, ah.size() * sizeof(cplx), hipMemcpyDeviceToHost, stream));
+      HCHECK(hipMemcpyAsync(bh.data(), B.b, bh.size() * sizeof(cplx), hipMemcpyDeviceToHost, stream));
+      HCHECK(hipMemcpyAsync(act.data(), B.active, nb * sizeof(int), hipMemcpyDeviceToHost, stream));
+      HCHECK(hipMemcpyAsync(dep.data(), B.depth, nb * sizeof(int), hipMemcpyDeviceToHost, stream));
+      HCHECK(hipStreamSynchronize(stream));
+      if (gprev.size() != (size_t)nb) gprev.assign(nb, cplx{0.0, 0.0});
+      std::vector<double> cz(2 * k1, 0.0);
+      bool changed = false;
+      for (int q = 0; q < nb; ++q) {
+        if (!act[q] || k1 >= dep[q]) continue;
+        double g[2], f[2];
+        const double zz[2] = {M.z[q].re, M.z[q].im};
+        if (asr_host_cf_species((const double*)&ah[(size_t)q * ld], (const double*)&bh[(size_t)q * ld], cz.data(), k1, zz,
+                                op->depth_tail, g, f, nullptr, nullptr))
+          return fail("adaptive depth: cf_species");
+        const cplx gt{g[0], g[1]};
+        if (k1 > 8) {
+          const cplx dg = gt - gprev[q];
+          if (std::hypot(dg.re, dg.im) <= op->depth_tol * std::hypot(gt.re, gt.im)) {
+            act[q] = 0;
+            changed = true;
+          }
+        }
+        gprev[q] = gt;
+      }
+      if (changed) {
+        HCHECK(hipMemcpyAsync(B.active, act.data(), nb * sizeof(int), hipMemcpyHostToDevice, stream));
+        HCHECK(hipStreamSynchronize(stream));
+      }
+      return 0;
+    }
+  }
+
+  int step_update() {
+    if (timer_mark(stream, KIND_UPDATE, true)) return 1;
     hipLaunchKernelGGL(k_update<T>, grid, dim3(BLOCK), 0, stream, dev_of(op), md, B.w, B.u, B.up, B.a_eff, B.b_eff, project,
                        B.part);
     if (timer_mark(stream, 0, false) || timer_mark(stream, KIND_FIN_B, true)) return 1;
@@ -2129,6 +2191,7 @@ struct ChainRun {
     }
     std::swap(B.up, B.u);  // up <- u
     std::swap(B.u, B.w);   // u <- w / b ; w <- old up (overwritten next level)
+    ++level_host;
     return 0;
   }
 
@@ -2230,6 +2293,9 @@ int run_multi(int ngroup, asr_gpu_op* const* ops, const int* ngch, const int* st
     }
     return 0;
   }
+  // the adaptive depth decides on the host between levels: no graph capture then (levels issued from the host)
+  for (int g = 0; g < ngroup; ++g)
+    if (mode == 2 && ops[g]->depth_tol > 0.0 && std::is_same<T, cplx>::value) mode = 0;
   std::vector<hipStream_t> st(ngroup, nullptr);
   std::vector<ChainRun<T>> runs(ngroup);
   auto cleanup = [&]() {
@@ -2632,6 +2698,14 @@ int asr_gpu_op_set_tile(asr_gpu_op* op, int mt) {
 }
 
 // The member tile the autotuner picked for nchain members (0 if no batched run of that size has happened).
+// Adaptive depth of the edc chains (ed_depth_tol / ed_tail of edc_green, src/asr_edc.f90:121-127): 0 = fixed depth.
+int asr_gpu_op_set_depth_tol(asr_gpu_op* op, double depth_tol, int tail) {
+  if (!(depth_tol >= 0.0) || tail < 0 || tail > 3) return fail("asr_gpu_op_set_depth_tol: depth_tol >= 0, tail 0..3");
+  op->depth_tol = depth_tol;
+  op->depth_tail = tail;
+  return 0;
+}
+
 int asr_gpu_op_get_tile(asr_gpu_op* op, int nchain, int is_complex, int* mt) {
   *mt = 0;
   for (auto& kv : op->tuned)

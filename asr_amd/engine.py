@@ -38,6 +38,7 @@ def lib() -> C.CDLL:
         L.asr_gpu_op_set_blocks.argtypes = [P, P, P]
         L.asr_gpu_op_set_classes.argtypes = [P, P, P, P]
         L.asr_gpu_op_set_tile.argtypes = [P, C.c_int]
+        L.asr_gpu_op_set_depth_tol.argtypes = [P, C.c_double, C.c_int]
         L.asr_gpu_op_get_tile.argtypes = [P, C.c_int, C.c_int, P]
         L.asr_gpu_real_set_tile.argtypes = [P, C.c_int]
         L.asr_gpu_op_destroy.argtypes = [P]
@@ -137,6 +138,11 @@ class GpuOperator:
     def set_tile(self, mt: int):
         """Member tile of the batched chains (power of two <= 64; 0 = autotuned)."""
         _check(lib().asr_gpu_op_set_tile(self.h, int(mt)), "asr_gpu_op_set_tile")
+
+    def set_depth_tol(self, depth_tol: float, tail: str = "hull"):
+        """Adaptive depth of the edc chains (ed_depth_tol / ed_tail of edc_green); 0 = every chain to its depth."""
+        tails = {"hull": 0, "bp": 1, "avg2": 2, "none": 3}
+        _check(lib().asr_gpu_op_set_depth_tol(self.h, float(depth_tol), tails[tail]), "asr_gpu_op_set_depth_tol")
 
     def tuned_tile(self, nchain: int, is_complex: bool) -> int:
         mt = np.zeros(1, np.int32)

@@ -45,6 +45,12 @@ int asr_gpu_op_set_classes(asr_gpu_op* op, const int* nsc, const int* spc, const
  * complex members run as pairs of real lanes, or 0 = chosen by the engine's autotuner per member count (default).  Replaces the reference's chain_teams / blk_nbmax rules
  * (src/asr_chain.f90:140-155, src/asr_augblk.f90:32-36). */
 int asr_gpu_op_set_tile(asr_gpu_op* op, int mt);
+/* Adaptive depth of the complex-symmetric chains (ed_depth_tol and ed_tail, edc_green, src/asr_edc.f90:121-127): at
+ * levels 8, 12, 16, ... a chain that has not reached its depth is terminated with the given tail (0 hull, 1 bp, 2 avg2,
+ * 3 none; cf_species, src/asr_chain.f90:162-262) and leaves when |G_k - G_{k-4}| <= depth_tol |G_k| (k >= 12).
+ * depth_tol = 0 (the default): every chain runs to its depth.  With depth_tol > 0 the levels are issued from the host
+ * (one synchronisation per tested level; asr_gpu_chains_edc_multi mode 2 then runs as mode 0). */
+int asr_gpu_op_set_depth_tol(asr_gpu_op* op, double depth_tol, int tail);
 /* The tile the autotuner picked for nchain members of the given kind (0: none tuned yet). */
 int asr_gpu_op_get_tile(asr_gpu_op* op, int nchain, int is_complex, int* mt);
 void asr_gpu_op_destroy(asr_gpu_op* op);

@@ -222,3 +222,51 @@ def test_multi_group_graph_replays_agree(l21_small):
         assert np.array_equal(m0, m1) and np.array_equal(m1, m2)
         assert np.array_equal(a1, a2) and np.array_equal(b1, b2)
         assert np.array_equal(a0, a1) and np.array_equal(b0, b1)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_edc_adaptive_depth(mode):
+    """Adaptive depth of edc_green (src/asr_edc.f90:121-127): at levels 8, 12, ... the chain terminated with the hull tail;
+    a chain leaves at level k >= 12 when |G_k - G_{k-4}| <= depth_tol |G_k|.  The stopping levels are those of the rule
+    applied to the chains' own coefficients (host cf_species), the coefficients up to there are the oracle's, and the
+    tolerance (the median change at level 12) makes about half the chains leave early."""
+    eng = _engine()
+    from asr_amd import host
+    wl = W.ternary_edc(L=4, levels=2, depth=24, seed=13)
+    G = eng.GpuOperator(wl.op)
+    members = wl.members[:20]
+    nlev = 24
+    a0, b0, mu0, _ = G.chains_edc(members, nlev, mode=mode)
+    assert np.all(mu0 == nlev)
+
+    def g_at(a, b, k1, z):
+        return host.cf_species(a, b, np.zeros(k1), k1, z, "hull")[0]
+
+    ch = np.array([abs(g_at(a0[j], b0[j], 12, m.z) - g_at(a0[j], b0[j], 8, m.z)) / abs(g_at(a0[j], b0[j], 12, m.z))
+                   for j, m in enumerate(members)])
+    # a tolerance between two distinct changes near the median, well away from both (equivalent orbitals give equal
+    # chains, so a tolerance equal to one of the changes would leave the decision to rounding)
+    u = np.unique(np.round(ch, 12))
+    mid = len(u) // 2
+    lo, hi = u[mid - 1], u[mid]
+    assert hi > lo * (1 + 1e-6)
+    tol = float(np.sqrt(lo * hi))
+    G.set_depth_tol(tol, "hull")
+    a, b, mu, _ = G.chains_edc(members, nlev, mode=mode)
+    G.set_depth_tol(0.0)
+    O = _oracle(wl.op)
+    for j, m in enumerate(members):
+        stop, gprev = nlev, None
+        for k1 in range(8, nlev, 4):
+            g = g_at(a[j], b[j], k1, m.z)
+            if k1 > 8 and abs(g - gprev) <= tol * abs(g):
+                stop = k1
+                break
+            gprev = g
+        assert mu[j] == stop, (j, mu[j], stop)
+        ar, br, n = O.edc_chain(m, m.z * m.pa + m.pb, m.l0, nlev)
+        k = int(mu[j])
+        assert rel(a[j, :k], ar[:k]) < 1e-9
+        assert rel(b[j, :k - 1], br[:k - 1]) < 1e-9
+        assert np.all(a[j, k:] == 0.0)
+    assert np.any(mu == 12) and np.any(mu > 12)
