@@ -111,7 +111,7 @@ def test_node_reordering_keeps_the_chains():
     from asr_amd import workloads as W
     w0 = W.ternary_edc(L=4, levels=2, depth=10, reorder=False)
     w1 = W.ternary_edc(L=4, levels=2, depth=10, reorder=True)
-    assert w1.op.meta.get("node_order") == "rcm" and not np.array_equal(w0.op.hopr, w1.op.hopr)
+    assert w1.op.meta.get("node_order") == "rcm of the hop links" and not np.array_equal(w0.op.hopr, w1.op.hopr)
     O0, O1 = OracleOp(w0.op), OracleOp(w1.op)
     for k in (0, 17, 40):
         m = w0.members[k]
