@@ -10,6 +10,10 @@
  *   orc_project_c        src/asr_augcls.f90:329-348   augcls_project_c
  *   orc_lanczos          src/reference/asr_recursion.f90:17-58   lanczos (no reorthogonalisation)
  *   orc_edc_chain        src/asr_edc.f90:75-164       edc_green without the adaptive depth (depth_tol = 0): a, b, m
+ *   orc_hp_c             src/asr_augcls.f90:301-320   augcls_hp_c
+ *   orc_ed3_kmul         src/asr_ed3.f90:99-129       ed3_kmul
+ *   orc_ed3_start        src/asr_ed3.f90:131-144      ed3_start
+ *   orc_ed3_chain        src/asr_ed3.f90:148-211      chain of route ed3 without the adaptive depth: a, b, c, m, closure
  *   orc_real_apply       src/reference/asr_lmto_realization.f90:120-191   h_apply / real_apply (order 2), n species per site
  *   orc_kpm_moments      src/reference/asr_kpm.f90:39-65   kpm_moments (Rademacher vectors from a counter-based hash)
  *   orc_spectral_bounds  src/reference/asr_kpm.f90:14-36   Lanczos part (the eigenvalues are taken on the caller's side)
@@ -274,6 +278,112 @@ int orc_edc_chain(const orc_op *op, const orc_member *m, const cplx *ep, int iv,
     for (size_t q = 0; q < dim; ++q) { up[q] = u[q]; u[q] = w[q] / bk; }
   }
   free(u); free(up); free(w); free(dg); free(re); free(im); free(sre); free(sim); free(wk); free(btab);
+  return mm;
+}
+
+/* y = h x = sq (c x~ + D S D x~) for complex x in normalised coordinates: apply_h on the real and the imaginary part
+ * (augcls_hp_c).  wk: 8 work vectors of 9 nn. */
+void orc_hp_c(const orc_op *op, const orc_member *m, const double *btab, const cplx *x, cplx *y, double *wk) {
+  const size_t n = (size_t)9 * op->nn;
+  double *w1 = wk, *w2 = wk + n, *w4 = wk + 3 * n, *w5 = wk + 4 * n, *w6 = wk + 5 * n, *w7 = wk + 6 * n, *w8 = wk + 7 * n;
+  for (int i = 0; i < op->nn; ++i)
+    for (int l = 0; l < 9; ++l) {
+      w7[9 * i + l] = creal(x[9 * i + l]) / op->sq[i];
+      w8[9 * i + l] = cimag(x[9 * i + l]) / op->sq[i];
+    }
+  orc_apply_h(op, m, btab, w7, w1, w4, w5, w6);
+  orc_apply_h(op, m, btab, w8, w2, w4, w5, w6);
+  for (int i = 0; i < op->nn; ++i)
+    for (int l = 0; l < 9; ++l) y[9 * i + l] = op->sq[i] * CMPLX(w1[9 * i + l], w2[9 * i + l]);
+}
+
+/* y = K' x = et x - h'x + u h'x + h'(u x - Qt h'x) (ed3_kmul); et, uu, qt: nsp x 9 complex species coefficients at the
+ * chain's energy; the member carries h' (c = g, d = D_c^1/2, o = e = 0).  ct: 4 complex work vectors, wk: 8 real. */
+void orc_ed3_kmul(const orc_op *op, const orc_member *m, const double *btab, const cplx *et, const cplx *uu, const cplx *qt,
+                  const cplx *x, cplx *y, cplx *ct, double *wk) {
+  const size_t n = (size_t)9 * op->nn;
+  cplx *t1 = ct, *t2 = ct + n, *t3 = ct + 2 * n, *t4 = ct + 3 * n;
+  orc_hp_c(op, m, btab, x, t1, wk);
+  orc_rand_c(op, m, qt, t1, t2);
+  orc_rand_c(op, m, uu, x, t3);
+  for (size_t q = 0; q < n; ++q) t3[q] = t3[q] - t2[q];
+  orc_hp_c(op, m, btab, t3, t4, wk);
+  orc_rand_c(op, m, et, x, y);
+  orc_rand_c(op, m, uu, t1, t2);
+  for (size_t q = 0; q < n; ++q) y[q] = y[q] - t1[q] + t2[q] + t4[q];
+}
+
+/* Public wrapper of orc_ed3_kmul with its own work space. */
+int orc_ed3_apply(const orc_op *op, const orc_member *m, const cplx *et, const cplx *uu, const cplx *qt, const cplx *x,
+                  cplx *y) {
+  const size_t n = (size_t)9 * op->nn;
+  double *btab = make_btab(op);
+  double *wk = (double *)malloc(sizeof(double) * 8 * n);
+  cplx *ct = (cplx *)malloc(sizeof(cplx) * 4 * n);
+  orc_ed3_kmul(op, m, btab, et, uu, qt, x, y, ct, wk);
+  free(ct); free(wk); free(btab);
+  return 0;
+}
+
+/* Start vectors of orbital iv (ed3_start): |v> = h'|u> (projected if project), |a> = |u> + oc |v>; norms nv, na
+ * (sqrt of the real part of the bilinear square).  vv, av: 9 nn complex.  Returns 1 for a zero start vector. */
+int orc_ed3_start(const orc_op *op, const orc_member *m, int iv, double oc, int project, cplx *vv, cplx *av, double *nv,
+                  double *na) {
+  const size_t n = (size_t)9 * op->nn;
+  double *btab = make_btab(op);
+  double *wk = (double *)malloc(sizeof(double) * 8 * n);
+  cplx *s0 = (cplx *)calloc(n, sizeof(cplx));
+  s0[iv] = 1.0;
+  orc_hp_c(op, m, btab, s0, vv, wk);
+  if (project) orc_project_c(op, m, vv);
+  for (size_t q = 0; q < n; ++q) av[q] = s0[q] + oc * vv[q];
+  cplx sv = 0, sa = 0;
+  for (size_t q = 0; q < n; ++q) sv += vv[q] * vv[q];
+  for (size_t q = 0; q < n; ++q) sa += av[q] * av[q];
+  *nv = sqrt(creal(sv));
+  *na = sqrt(creal(sa));
+  free(s0); free(wk); free(btab);
+  return (*nv < 1.0e-14 || *na < 1.0e-14) ? 1 : 0;
+}
+
+/* One chain of route ed3 (chain, src/asr_ed3.f90:148-211) on H_eff = z - K' from u0 (u0^T u0 = 1) without the adaptive
+ * depth: a, b (nlev complex), c(k) = tv^T u_k (nlev complex); *closed = 1 when b_k < 1e-12 ended it.  Returns m. */
+int orc_ed3_chain(const orc_op *op, const orc_member *m, const cplx *et, const cplx *uu, const cplx *qt, cplx z,
+                  const cplx *u0, const cplx *tv, int nlev, int project, cplx *a, cplx *b, cplx *c, int *closed) {
+  const size_t n = (size_t)9 * op->nn;
+  double *btab = make_btab(op);
+  double *wk = (double *)malloc(sizeof(double) * 8 * n);
+  cplx *ct = (cplx *)malloc(sizeof(cplx) * 4 * n);
+  cplx *u = (cplx *)malloc(sizeof(cplx) * n), *up = (cplx *)calloc(n, sizeof(cplx)), *w = (cplx *)malloc(sizeof(cplx) * n);
+  for (int k = 0; k < nlev; ++k) { a[k] = 0; b[k] = 0; c[k] = 0; }
+  memcpy(u, u0, sizeof(cplx) * n);
+  int mm = 0;
+  *closed = 0;
+  for (int k = 0; k < nlev; ++k) {
+    cplx s = 0;
+    for (size_t q = 0; q < n; ++q) s += tv[q] * u[q];
+    c[k] = s;
+    orc_ed3_kmul(op, m, btab, et, uu, qt, u, w, ct, wk);
+    for (size_t q = 0; q < n; ++q) w[q] = z * u[q] - w[q];
+    s = 0;
+    for (size_t q = 0; q < n; ++q) s += u[q] * w[q];
+    a[k] = s;
+    mm = k + 1;
+    if (k == nlev - 1) break;
+    cplx ak = a[k], bk = (k > 0) ? b[k - 1] : 0;
+    for (size_t q = 0; q < n; ++q) {
+      w[q] = w[q] - ak * u[q];
+      if (k > 0) w[q] = w[q] - bk * up[q];
+    }
+    if (project) orc_project_c(op, m, w);
+    s = 0;
+    for (size_t q = 0; q < n; ++q) s += w[q] * w[q];
+    b[k] = csqrt(s);
+    if (cabs(b[k]) < 1.0e-12) { *closed = 1; break; }
+    bk = b[k];
+    for (size_t q = 0; q < n; ++q) { up[q] = u[q]; u[q] = w[q] / bk; }
+  }
+  free(u); free(up); free(w); free(ct); free(wk); free(btab);
   return mm;
 }
 

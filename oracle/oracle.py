@@ -3,6 +3,8 @@
 ctypes bindings of liborc.so (the C restatement in asr_oracle.c) plus small numpy restatements of the reference routines
 that call LAPACK on the chain matrices:
     np_cf_species       src/asr_chain.f90:67-167          (zgeev -> numpy.linalg.eigvals, the same LAPACK routine)
+    np_cf_closed        src/asr_ed3.f90:214-230           (the fraction of a closed chain, no tail)
+    OracleOp.ed3_blocks src/asr_ed3.f90:247-278           (two chains -> the central blocks bk(0:3))
     np_bp_asymptote     src/reference/asr_terminator.f90:39-79  (dsterf -> scipy.linalg.eigvalsh_tridiagonal)
     np_cf_green         src/reference/asr_greens.f90:207-217
     np_sqrt_tail        src/reference/asr_greens.f90:139-149
@@ -38,6 +40,9 @@ def lib():
         L.orc_project_c.argtypes = [P, P, P]
         L.orc_lanczos.argtypes = [P, P, C.c_int, C.c_int, C.c_double, P, P]
         L.orc_edc_chain.argtypes = [P, P, P, C.c_int, C.c_int, C.c_int, P, P]
+        L.orc_ed3_apply.argtypes = [P, P, P, P, P, P, P]
+        L.orc_ed3_start.argtypes = [P, P, C.c_int, C.c_double, C.c_int, P, P, P, P]
+        L.orc_ed3_chain.argtypes = [P, P, P, P, P, C.c_double, C.c_double, P, P, C.c_int, C.c_int, P, P, P, P]
         L.orc_lanczos_batch.argtypes = [P, P, P, P, C.c_int, C.c_int, P, P, P]
         L.orc_edc_batch.argtypes = [P, P, P, P, P, C.c_int, C.c_int, C.c_int, P, P, P]
         L.orc_real_apply.argtypes = [P, P, P, P]
@@ -131,6 +136,62 @@ class OracleOp:
         ms = self.member(m)
         mm = lib().orc_edc_chain(C.byref(self.s), C.byref(ms), _p(ep), int(iv), int(nlev), int(project), _p(a), _p(b))
         return a, b, mm
+
+    def ed3_apply(self, m, et, uu, qt, x: np.ndarray) -> np.ndarray:
+        """K' x (ed3_kmul) with the species coefficients et, uu, qt (nsp, 9) at the chain's energy."""
+        cs = [np.ascontiguousarray(v, dtype=np.complex128) for v in (et, uu, qt)]
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        y = np.zeros_like(x)
+        ms = self.member(m)
+        lib().orc_ed3_apply(C.byref(self.s), C.byref(ms), _p(cs[0]), _p(cs[1]), _p(cs[2]), _p(x), _p(y))
+        return y
+
+    def ed3_start(self, m, iv: int, oc: float, project: bool = True):
+        n = self.op.dim
+        vv = np.zeros(n, np.complex128)
+        av = np.zeros(n, np.complex128)
+        nv = C.c_double()
+        na = C.c_double()
+        ms = self.member(m)
+        st = lib().orc_ed3_start(C.byref(self.s), C.byref(ms), int(iv), float(oc), int(project), _p(vv), _p(av),
+                                 C.byref(nv), C.byref(na))
+        if st:
+            raise ValueError("ed3_start: zero start vector")
+        return vv, av, nv.value, na.value
+
+    def ed3_chain(self, m, et, uu, qt, z: complex, u0, tv, nlev: int, project: bool = True):
+        cs = [np.ascontiguousarray(v, dtype=np.complex128) for v in (et, uu, qt)]
+        u0 = np.ascontiguousarray(u0, dtype=np.complex128)
+        tv = np.ascontiguousarray(tv, dtype=np.complex128)
+        a = np.zeros(nlev, np.complex128)
+        b = np.zeros(nlev, np.complex128)
+        c = np.zeros(nlev, np.complex128)
+        closed = C.c_int()
+        ms = self.member(m)
+        mm = lib().orc_ed3_chain(C.byref(self.s), C.byref(ms), _p(cs[0]), _p(cs[1]), _p(cs[2]), complex(z).real,
+                                 complex(z).imag, _p(u0), _p(tv), int(nlev), int(project), _p(a), _p(b), _p(c),
+                                 C.byref(closed))
+        return a, b, c, mm, bool(closed.value)
+
+    def ed3_blocks(self, m, et, uu, qt, oc: float, z: complex, iv: int, nlev: int, project: bool = True,
+                   tail: str = "hull"):
+        """ed3_blocks: the chain from |a> / na projected on |v>, the chain from |v> / nv projected on |a>;
+        bk = (G_11, (hG)_11, (hGh)_11, q1a - q1b).  Returns bk and the two chains' (a, b, c, m, closed)."""
+        vv, av, nv, na = self.ed3_start(m, iv, oc, project)
+        chains = []
+        res = []
+        for u0, tv, nrm in ((av / na, vv, na), (vv / nv, av, nv)):
+            a, b, c, mm, closed = self.ed3_chain(m, et, uu, qt, z, u0, tv, nlev, project)
+            if closed:
+                g, f = np_cf_closed(a, b, c, mm, z)
+            else:
+                g, f = np_cf_species(a, b, c, mm, z, tail)
+            chains.append((a, b, c, mm, closed))
+            res.append((nrm ** 2 * g, nrm * f))
+        (q0, q1a), (q2, q1b) = res
+        fm = 0.5 * (q1a + q1b)
+        bk = np.array([q0 - 2.0 * oc * fm + oc ** 2 * q2, fm - oc * q2, q2, q1a - q1b], np.complex128)
+        return bk, chains, (na, nv)
 
     def lanczos_batch(self, members, iv, nlev):
         nch = len(members)
@@ -283,6 +344,21 @@ def np_cf_species(a, b, c, m, z, tail="hull"):
         tl = bb * bb * g1 if abs(bb * g1) <= abs(bb * g2) else bb * bb * g2
     gt = np.zeros(m + 1, dtype=np.complex128)
     gt[m - 1] = 1.0 / (z - a[m - 1] - tl)
+    for j in range(m - 2, -1, -1):
+        gt[j] = 1.0 / (z - a[j] - b[j] ** 2 * gt[j + 1])
+    f = 0.0
+    prod = gt[0]
+    for k in range(m):
+        f += c[k] * prod
+        if k < m - 1:
+            prod = prod * b[k] * gt[k + 1]
+    return gt[0], f
+
+
+def np_cf_closed(a, b, c, m, z):
+    """The fraction of a chain that closed (Krylov space exhausted): exact, no tail; g = G_11, f = sum_k c_k G_k1."""
+    gt = np.zeros(m, dtype=np.complex128)
+    gt[m - 1] = 1.0 / (z - a[m - 1])
     for j in range(m - 2, -1, -1):
         gt[j] = 1.0 / (z - a[j] - b[j] ** 2 * gt[j + 1])
     f = 0.0

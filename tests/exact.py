@@ -112,3 +112,32 @@ def dense_operator(apply, dim: int) -> np.ndarray:
         u[i] = 1.0
         m[:, i] = apply(u)
     return m
+
+
+def dense_asa(model, pp, species: np.ndarray):
+    """h, O and H of the ASA secular problem K = E O - H for one configuration, in the form of bands_core (alpha
+    representation): h = c + D^1/2 S D^1/2, O = 1 + o h + h o + h p h, H = E_nu + h + E_nu o h + h o E_nu + h (o + p E_nu) h.
+    pp: per species and orbital (nsp, 9) arrays c (= C - E_nu), dl (Delta), enu, o, p."""
+    s = dense_s(model)
+    c, dl, en, ov, pv = (pp[k][species].reshape(-1) for k in ("c", "dl", "enu", "o", "p"))
+    sd = np.sqrt(dl)
+    h = sd[:, None] * s * sd[None, :] + np.diag(c)
+    om = np.eye(h.shape[0]) + ov[:, None] * h + h * ov[None, :] + h @ np.diag(pv) @ h
+    hm = np.diag(en) + h + (en * ov)[:, None] * h + h * (en * ov)[None, :] + h @ np.diag(ov + pv * en) @ h
+    return h, om, hm
+
+
+def exact_average_asa(model, pp, z: complex, iv: int, frozen_species: int):
+    """Configuration averages of G_11, (h G)_11 and (h G h)_11 with G = (z O - H)^-1, the start site frozen (the brute
+    force of test/test_asr_ed3.f90:312-345 restated)."""
+    g11 = gh = ghh = 0.0
+    for sp, p in configurations(model, frozen_species):
+        h, om, hm = dense_asa(model, pp, sp)
+        rhs = np.zeros((h.shape[0], 2), np.complex128)
+        rhs[iv, 0] = 1.0
+        rhs[:, 1] = h[:, iv]
+        sol = np.linalg.solve(z * om - hm, rhs)
+        g11 += p * sol[iv, 0]
+        gh += p * (h[iv, :] @ sol[:, 0])
+        ghh += p * (h[:, iv] @ sol[:, 1])
+    return g11, gh, ghh

@@ -13,7 +13,7 @@ from asr_amd import augcls as A
 from asr_amd import host
 from oracle.oracle import OracleOp
 
-from exact import build_ring_operator, dense_operator, exact_average_g, ring_model
+from exact import build_ring_operator, dense_operator, exact_average_asa, exact_average_g, ring_model
 
 
 def rotated(model, ic):
@@ -121,3 +121,73 @@ def test_node_reordering_keeps_the_chains():
         assert n0 == n1
         assert np.max(np.abs(a1 - a0)) < 1e-12 * np.max(np.abs(a0))
         assert np.max(np.abs(b1 - b0)) < 1e-12 * np.max(np.abs(b0))
+
+
+def ring_ed3_params(seed: int = 77):
+    rng = np.random.default_rng(seed)
+    o = rng.uniform(-0.1, 0.1, (4, 3))
+    return dict(c=rng.uniform(-0.4, 0.4, (4, 3)), sqrdel=rng.uniform(0.5, 0.8, (4, 3)), enu=rng.uniform(-0.1, 0.1, (4, 3)),
+                o=o, p=o ** 2 + rng.uniform(0.0, 0.05, (4, 3)))
+
+
+@pytest.mark.parametrize("ic,cen", CASES[:4])
+def test_ring_ed3_blocks_equal_exact_average(ic, cen):
+    """Route ed3 on the ring (test/test_asr_ed3.f90): the augmented K'(z) is complex symmetric; its dense inverse gives the
+    configuration averages of G_11, (hG)_11, (hGh)_11 of K = E O - H (1e-10); the two projected chains of ed3_blocks
+    (exhausted Krylov space, no tail) give them too (1e-6), and the two cross terms agree."""
+    from asr_amd import ed3 as E3
+    model, _ = ring_model()
+    model = rotated(model, ic)
+    op = build_ring_operator(model, frozen_start=True)
+    O = OracleOp(op)
+    pp = ring_ed3_params()
+    ed = E3.ed3_setup(pp, cen)
+    orb = {k: E3.per_orbital(pp[k]) for k in ("c", "enu", "o", "p")}
+    orb["dl"] = E3.per_orbital(pp["sqrdel"]) ** 2
+    dim = op.dim
+    err_sym = err_dense = err_chain = err_cross = 0.0
+    for l0 in (0, 2, 6):
+        for e in (-0.3, 0.25):
+            z = complex(e, 0.3)
+            m = E3.ed3_member(ed, l0, z)
+            et, uu, qt = E3.ed3_coefs(ed, z)
+            k = np.zeros((dim, dim), np.complex128)
+            for j in range(dim):
+                u = np.zeros(dim, np.complex128)
+                u[j] = 1.0
+                k[:, j] = O.ed3_apply(m, et, uu, qt, u)
+            err_sym = max(err_sym, np.max(np.abs(k - k.T)) / np.max(np.abs(k)))
+            vv, av, nv, na = O.ed3_start(m, l0, E3.ed3_oc(ed, l0), project=True)
+            rhs = np.zeros((dim, 2), np.complex128)
+            rhs[l0, 0] = 1.0
+            rhs[:, 1] = vv
+            sol = np.linalg.solve(k, rhs)
+            dense = np.array([sol[l0, 0], vv @ sol[:, 0], vv @ sol[:, 1]])
+            exact = np.array(exact_average_asa(model, orb, z, l0, cen))
+            err_dense = max(err_dense, np.max(np.abs(dense - exact)))
+            bk, chains, _ = O.ed3_blocks(m, et, uu, qt, E3.ed3_oc(ed, l0), z, l0, 1500, project=True, tail="none")
+            err_chain = max(err_chain, np.max(np.abs(bk[:3] - exact)))
+            err_cross = max(err_cross, abs(bk[3]))
+    assert err_sym < 1.0e-13
+    assert err_dense < 1.0e-10
+    assert err_chain < 1.0e-6
+    assert err_cross < 1.0e-6
+
+
+def test_ed3_mutation_is_detected():
+    """A planted mistake in the oracle's K' (the sign of the Qt term's coefficient) breaks the exact-average agreement."""
+    from asr_amd import ed3 as E3
+    model, _ = ring_model()
+    model = rotated(model, 1)
+    op = build_ring_operator(model, frozen_start=True)
+    O = OracleOp(op)
+    pp = ring_ed3_params()
+    ed = E3.ed3_setup(pp, 1)
+    orb = {k: E3.per_orbital(pp[k]) for k in ("c", "enu", "o", "p")}
+    orb["dl"] = E3.per_orbital(pp["sqrdel"]) ** 2
+    z = complex(0.1, 0.3)
+    m = E3.ed3_member(ed, 2, z)
+    et, uu, qt = E3.ed3_coefs(ed, z)
+    bk, _, _ = O.ed3_blocks(m, et, uu, -qt, E3.ed3_oc(ed, 2), z, 2, 1500, project=True, tail="none")
+    exact = np.array(exact_average_asa(model, orb, z, 2, 1))
+    assert np.max(np.abs(bk[:3] - exact)) > 1.0e-4
