@@ -44,8 +44,10 @@ int fail(const std::string& msg) {
     if (e_ != hipSuccess) return fail(std::string(#call) + ": " + hipGetErrorString(e_));              \
   } while (0)
 
-constexpr int NQ = 6;  // parameter slots: c, d, o, e, pa, pb
-enum { QC = 0, QD = 1, QO = 2, QE = 3, QPA = 4, QPB = 5 };
+// parameter slots: c, d, o, e; then pairs (a, b) of energy-linear diagonals X(z) = z X(a) + X(b): pa/pb (route edc's X(z);
+// route ed3's z - et), ua/ub (ed3's u), qa/qb (ed3's Qt)
+constexpr int NQ = 10;
+enum { QC = 0, QD = 1, QO = 2, QE = 3, QPA = 4, QPB = 5, QUA = 6, QUB = 7, QQA = 8, QQB = 9 };
 constexpr int WAVE = 64;
 constexpr int BLOCK = 256;
 
@@ -219,15 +221,15 @@ __device__ void rand_accum(const OpDev& op, const MemDev& md, const int* __restr
 
 // X(z) x~ = z (X(pa) x~) + X(pb) x~ for complex members (blk_rand_lin / rand_lin_amp, src/asr_augblk.f90:127-165)
 __device__ void lin_accum(const OpDev& op, const MemDev& md, const int* pi, const double* sg, uint64_t chim, int i,
-                          uint32_t inf, int m, const cplx* x, int xnorm, double coef, cplx acc[9]) {
+                          uint32_t inf, int m, int qa, const cplx* x, int xnorm, double coef, cplx acc[9]) {
   cplx ta[9], tb[9];
   for (int l = 0; l < 9; ++l) { ta[l] = {0.0, 0.0}; tb[l] = {0.0, 0.0}; }
-  rand_accum<cplx>(op, md, pi, sg, chim, i, inf, m, QPA, x, xnorm, 1.0, ta);
-  rand_accum<cplx>(op, md, pi, sg, chim, i, inf, m, QPB, x, xnorm, 1.0, tb);
+  rand_accum<cplx>(op, md, pi, sg, chim, i, inf, m, qa, x, xnorm, 1.0, ta);
+  rand_accum<cplx>(op, md, pi, sg, chim, i, inf, m, qa + 1, x, xnorm, 1.0, tb);
   cplx z = md.z[m];
   for (int l = 0; l < 9; ++l) acc[l] = acc[l] + coef * (z * ta[l] + tb[l]);
 }
-__device__ inline void lin_accum(const OpDev&, const MemDev&, const int*, const double*, uint64_t, int, uint32_t, int,
+__device__ inline void lin_accum(const OpDev&, const MemDev&, const int*, const double*, uint64_t, int, uint32_t, int, int,
                                  const double*, int, double, double*) {}
 
 struct FArgs {
@@ -238,9 +240,10 @@ struct FArgs {
   int qr[3], r_norm[3];
   double r_coef[3];
   int nr;
-  const void* xl;  // complex X(z) term
-  int l_norm;
-  double l_coef;
+  const void* xl[2];  // complex energy-linear terms X(z) = z X(lq) + X(lq + 1)
+  int lq[2], l_norm[2];
+  double l_coef[2];
+  int nl;
   const void* xo[2];  // own-node terms (amplitude vectors)
   double o_coef[2];
   int no;
@@ -313,7 +316,8 @@ __global__ void __launch_bounds__(BLOCK) k_fused(OpDev op, MemDev md, FArgs fa) 
     }
     for (int t = 0; t < fa.nr; ++t)
       rand_accum<T>(op, md, pi, sg, chim, i, inf, m, fa.qr[t], (const T*)fa.xr[t], fa.r_norm[t], fa.r_coef[t], acc);
-    if (fa.xl) lin_accum(op, md, pi, sg, chim, i, inf, m, (const T*)fa.xl, fa.l_norm, fa.l_coef, acc);
+    if (fa.nl > 0) lin_accum(op, md, pi, sg, chim, i, inf, m, fa.lq[0], (const T*)fa.xl[0], fa.l_norm[0], fa.l_coef[0], acc);
+    if (fa.nl > 1) lin_accum(op, md, pi, sg, chim, i, inf, m, fa.lq[1], (const T*)fa.xl[1], fa.l_norm[1], fa.l_coef[1], acc);
     for (int t = 0; t < fa.no; ++t) {
       const T* xo = (const T*)fa.xo[t];
       for (int l = 0; l < 9; ++l) acc[l] = fma_s(fa.o_coef[t], ld(xo, i, l, md.nbp, m), acc[l]);
@@ -340,7 +344,7 @@ __global__ void __launch_bounds__(BLOCK) k_fused(OpDev op, MemDev md, FArgs fa) 
 // bond block is uniform over the wave and its rows are scalar operands of the FMAs (the next row prefetched); otherwise
 // the padded blocks come from LDS one row at a time.  The scale of a gathered value, dd(l) chi(g) sg(g, l) / sq(r), comes
 // from an LDS table and is shared by the real and imaginary parts.  The species-diagonal terms use the same loads and an
-// LDS sign table; X(z) = z X(pa) + X(pb) is one pass with a complex weight.  Member tiles of 8-16 keep one member slice
+// LDS sign table; an energy-linear term X(z) = z X(a) + X(b) is one pass with a complex weight.  Member tiles of 8-16 keep one member slice
 // of a 1-2 GB vector small enough to stay in the last-level caches while all nodes are swept.
 // Row l of the lane's member at node offset voff: 16-byte (complex) or 8-byte (real) raw buffer load.
 template <class T>
@@ -357,7 +361,7 @@ __device__ __forceinline__ T tile_row(__amdgpu_buffer_rsrc_t xr, uint32_t voff, 
 // acc += coef X(x~) for the members of a tile (apply_rand with pi = identity): the node's own state and its flip partners, the
 // member's local weights, the partner's sign chi(g) sg(g, l) from the LDS table cs (member x operation x orbital), raw
 // buffer loads for the rows.  LIN = false: X = X_q with the real weights wt(q, class, l0, k, l).  LIN = true: the
-// energy-dependent diagonal X(z) = z X(pa) + X(pb) with the complex weight z wt(pa) + wt(pb) (one pass, one accumulator).
+// energy-dependent diagonal X(z) = z X(q) + X(q + 1) with the complex weight z wt(q) + wt(q + 1) (one pass, one accumulator).
 // Frozen nodes: the frozen species' parameters on the node's own row.
 template <class T, bool LIN>
 __device__ __forceinline__ void rand_ct(const OpDev& op, const MemDev& md, const double* __restrict__ cs, int i, uint32_t inf,
@@ -366,7 +370,7 @@ __device__ __forceinline__ void rand_ct(const OpDev& op, const MemDev& md, const
   const int nbp = md.nbp, nsm = op.nsmax;
   const bool fz = ((inf >> 20) & 1u) && md.frozen[m];
   const double isi = xnorm ? op.isq[i] : 1.0;
-  const int qa = LIN ? QPA : q, qb = LIN ? QPB : q;
+  const int qa = q, qb = LIN ? q + 1 : q;
   auto row = [&](uint32_t voff, int l) -> T { return tile_row<T>(xr, voff, l * vs16); };
   // acc += (coef (z wa + wb) or coef wa) xv
   auto add = [&](double wa, double wb, T xv, int l) {
@@ -513,10 +517,10 @@ __global__ void __launch_bounds__(BLOCK, 4) k_tile(OpDev op, MemDev md, FArgs fa
         rand_ct<T, false>(op, md, cs, i, inf, m, node16, vs16, coff, rr, fa.r_norm[t], fa.qr[t], fa.r_coef[t], acc);
       }
       if constexpr (std::is_same<T, cplx>::value) {
-        if (fa.xl) {
-          // X(z) x~ = z X(pa) x~ + X(pb) x~ in one pass over x
-          const __amdgpu_buffer_rsrc_t rl = __builtin_amdgcn_make_buffer_rsrc((void*)fa.xl, (short)0, nrec, 0x00020000);
-          rand_ct<T, true>(op, md, cs, i, inf, m, node16, vs16, coff, rl, fa.l_norm, 0, fa.l_coef, acc);
+        for (int t = 0; t < fa.nl; ++t) {
+          // X(z) x~ = z X(a) x~ + X(b) x~ in one pass over x
+          const __amdgpu_buffer_rsrc_t rl = __builtin_amdgcn_make_buffer_rsrc((void*)fa.xl[t], (short)0, nrec, 0x00020000);
+          rand_ct<T, true>(op, md, cs, i, inf, m, node16, vs16, coff, rl, fa.l_norm[t], fa.lq[t], fa.l_coef[t], acc);
         }
       }
     }
@@ -689,7 +693,10 @@ __global__ void __launch_bounds__(BLOCK, 4) k_node(OpDev op, MemDev md, FArgs fa
     }
     for (int t = 0; t < fa.nr; ++t)
       rand_accum<T>(op, md, op.pi, op.sg, chim, i, inf, m, fa.qr[t], (const T*)fa.xr[t], fa.r_norm[t], fa.r_coef[t], acc);
-    if (fa.xl) lin_accum(op, md, op.pi, op.sg, chim, i, inf, m, (const T*)fa.xl, fa.l_norm, fa.l_coef, acc);
+    if (fa.nl > 0)
+      lin_accum(op, md, op.pi, op.sg, chim, i, inf, m, fa.lq[0], (const T*)fa.xl[0], fa.l_norm[0], fa.l_coef[0], acc);
+    if (fa.nl > 1)
+      lin_accum(op, md, op.pi, op.sg, chim, i, inf, m, fa.lq[1], (const T*)fa.xl[1], fa.l_norm[1], fa.l_coef[1], acc);
     for (int t = 0; t < fa.no; ++t) {
       const T* xo = (const T*)fa.xo[t];
       for (int l = 0; l < 9; ++l) acc[l] = fma_s(fa.o_coef[t], ld(xo, i, l, nbp, m), acc[l]);
@@ -826,16 +833,16 @@ __global__ void __launch_bounds__(BLOCK, 4) k_pair(OpDev op, MemDev md, FArgs fa
     }
     for (int t = 0; t < fa.nr; ++t)
       rand_accum_pair(op, md, chim, i, inf, m, vi, vs, fa.qr[t], (const double*)fa.xr[t], fa.r_norm[t], fa.r_coef[t], acc);
-    if (fa.xl) {
+    for (int t = 0; t < fa.nl; ++t) {
       double ta[9], tb[9];
       for (int l = 0; l < 9; ++l) { ta[l] = 0.0; tb[l] = 0.0; }
-      rand_accum_pair(op, md, chim, i, inf, m, vi, vs, QPA, (const double*)fa.xl, fa.l_norm, 1.0, ta);
-      rand_accum_pair(op, md, chim, i, inf, m, vi, vs, QPB, (const double*)fa.xl, fa.l_norm, 1.0, tb);
+      rand_accum_pair(op, md, chim, i, inf, m, vi, vs, fa.lq[t], (const double*)fa.xl[t], fa.l_norm[t], 1.0, ta);
+      rand_accum_pair(op, md, chim, i, inf, m, vi, vs, fa.lq[t] + 1, (const double*)fa.xl[t], fa.l_norm[t], 1.0, tb);
       const cplx z = md.z[m];
       const double sgn = part ? 1.0 : -1.0;
       for (int l = 0; l < 9; ++l) {
         const double oth = __shfl_xor(ta[l], 1);
-        acc[l] = fma(fa.l_coef, fma(z.re, ta[l], fma(sgn * z.im, oth, tb[l])), acc[l]);
+        acc[l] = fma(fa.l_coef[t], fma(z.re, ta[l], fma(sgn * z.im, oth, tb[l])), acc[l]);
       }
     }
     for (int t = 0; t < fa.no; ++t) {
@@ -1046,6 +1053,8 @@ struct ChainDev {
   int ld;            // leading dimension of a, b
   int real_chain;    // 1: real Lanczos conventions (b of the last level recorded)
   double tiny;
+  void* cvec;        // route ed3: c_k = <t|u_k> (nbp x ld, the chain's element type) or null
+  int* closed;       // route ed3: 1 where b_k < tiny ended the chain (Krylov space exhausted) or null
 };
 
 template <class T> __device__ inline double mag(T v);
@@ -1140,13 +1149,19 @@ __global__ void __launch_bounds__(BLOCK) k_fin_a(ChainDev cd, int nb, const T* p
   }
 }
 
-// b_k = sqrt(sum w w); members leave at their depth or when b_k < tiny; next u = w * inv_b
+// b_k = sqrt(sum w w); members leave at their depth or when b_k < tiny; next u = w * inv_b.  part2 (route ed3): the partials
+// of <t, w>, so that c_{k+1} = <t, u_{k+1}> = <t, w> / b_k.
 template <class T>
 __global__ void __launch_bounds__(BLOCK) k_fin_b(ChainDev cd, int nb, const T* part, int nbx, int mt, T* b, T* b_eff,
-                                                  T* inv_b) {
+                                                  T* inv_b, const T* part2) {
   __shared__ T lds[BLOCK / WAVE];
   const int m = blockIdx.x;
   T s = block_sum_partials(part, nbx, mt, m, lds);
+  T s2 = zero<T>();
+  if (part2) {
+    __syncthreads();
+    s2 = block_sum_partials(part2, nbx, mt, m, lds);
+  }
   if (threadIdx.x != 0) return;
   int k = *cd.level;
   T bk = dsqrt(s);
@@ -1154,11 +1169,40 @@ __global__ void __launch_bounds__(BLOCK) k_fin_b(ChainDev cd, int nb, const T* p
   if (act) {
     bool last = (k == cd.depth[m] - 1);
     if (!last || cd.real_chain) b[(size_t)m * cd.ld + k] = bk;
+    if (!last && mag(bk) < cd.tiny && cd.closed) cd.closed[m] = 1;
     if (last || mag(bk) < cd.tiny) act = 0;
   }
   cd.active[m] = act;
   b_eff[m] = act ? bk : zero<T>();
   inv_b[m] = act ? inv(bk) : zero<T>();
+  if (act && part2 && cd.cvec) ((T*)cd.cvec)[(size_t)m * cd.ld + k + 1] = bilin(s2, inv(bk));
+}
+
+// per-member sum of block partials (one block per member)
+template <class T>
+__global__ void __launch_bounds__(BLOCK) k_sum_member(const T* part, int nbx, int mt, T* out) {
+  __shared__ T lds[BLOCK / WAVE];
+  const int m = blockIdx.x;
+  T s = block_sum_partials(part, nbx, mt, m, lds);
+  if (threadIdx.x == 0) out[m] = s;
+}
+
+// Start vectors of route ed3 (ed3_start, src/asr_ed3.f90:131-144) from vv = h'|s0> (projected), s0 the unit vector at the
+// member's start component (node 0, orbital l0): u = su s0 + sv vv, t = tsu s0 + tsv vv, coef = (su, sv, tsu, tsv) per member.
+__global__ void k_ed3_start(const cplx* vv, cplx* u, cplx* tv, const double* coef, const int* l0, size_t n, int nbp) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int m = (int)(q % nbp);
+  const bool s0 = (q / nbp) == (size_t)l0[m];
+  const double* cf = coef + 4 * (size_t)m;
+  const cplx v = vv[q];
+  cplx uq = cf[1] * v, tq = cf[3] * v;
+  if (s0) {
+    uq.re += cf[0];
+    tq.re += cf[2];
+  }
+  u[q] = uq;
+  tv[q] = tq;
 }
 
 // end of a level: level += 1 and the activity of every member tile (a tile whose members have all left is skipped)
@@ -1171,17 +1215,17 @@ __global__ void k_level_end(int* level, const int* active, int nb, int mt, int* 
   if (threadIdx.x == 0) *level += 1;
 }
 
-// w = w - a u - b_prev up; optional projection onto the symmetric subspace; partial sum w w
+// w = w - a u - b_prev up; optional projection onto the symmetric subspace; partial sums w w and (tv non-null) tv w
 template <class T>
 __global__ void __launch_bounds__(BLOCK) k_update(OpDev op, MemDev md, T* w, const T* u, const T* up, const T* a_eff,
-                                                   const T* b_prev, int project, T* partial) {
+                                                   const T* b_prev, int project, T* partial, const T* tv, T* partial2) {
   __shared__ T red[BLOCK];
   if (md.tact && !md.tact[blockIdx.y]) return;  // every member of this tile has left
   const int mt = md.mt, npw = WAVE / mt;
   const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x >> 6;
   const int m = blockIdx.y * mt + (lane & (mt - 1));
   const int i = (xcd_block(blockIdx.x, gridDim.x) * (BLOCK / WAVE) + wave) * npw + lane / mt;
-  T dv = zero<T>();
+  T dv = zero<T>(), dv2 = zero<T>();
   if (i < op.nn && m < md.nb) {
     const T a = a_eff[m], b = b_prev[m];
     T x[9];
@@ -1210,11 +1254,21 @@ __global__ void __launch_bounds__(BLOCK) k_update(OpDev op, MemDev md, T* w, con
       }
     }
     for (int l = 0; l < 9; ++l) {
-      w[((size_t)i * 9 + l) * md.nbp + m] = x[l];
+      const size_t q = ((size_t)i * 9 + l) * md.nbp + m;
+      w[q] = x[l];
       dv = dv + bilin(x[l], x[l]);
+      if constexpr (std::is_same<T, cplx>::value) {
+        if (tv) dv2 = dv2 + bilin(tv[q], x[l]);
+      }
     }
   }
   block_reduce_member<T>(dv, mt, red, partial + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * mt + threadIdx.x);
+  if constexpr (std::is_same<T, cplx>::value) {
+    if (tv) {
+      __syncthreads();
+      block_reduce_member<T>(dv2, mt, red, partial2 + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * mt + threadIdx.x);
+    }
+  }
 }
 
 // u <- w * inv_b (in place), members fastest
@@ -1623,7 +1677,7 @@ int pow2_tile(int nb) {
 // Per-member host preparation: characters, local weights, frozen rows, uniform d.
 struct Members {
   int nb = 0, nbp = 0, mt = 1, pair = 0;
-  bool uniform_d = true, zero_o = true, zero_e = true, complex_lin = false;
+  bool uniform_d = true, zero_o = true, zero_e = true, complex_lin = false, ed3 = false;
   std::vector<uint64_t> chim;
   std::vector<double> wts, fpar, dvec, dperm, mtm;  // mtm: ncls x nsmax x nsmax x nsmax x nbp (class tables per member)
   std::vector<int> frozen, fsp;
@@ -1641,9 +1695,11 @@ int chi_mask(const asr_gpu_op* op, int l0, uint64_t* mask) {
   return 0;
 }
 
-// params: nchain x 4 x nsp x 9 (c, d, o, e); pab: nchain x 2 x nsp x 9 or null; conc: nchain x ncls x nsmax or null
+// params: nchain x 4 x nsp x 9 (c, d, o, e); pab: nchain x 2 x nsp x 9 or null; conc: nchain x ncls x nsmax or null;
+// pex (route ed3): nchain x 4 x nsp x 9 (ua, ub, qa, qb) or null
 int prepare_members(const asr_gpu_op* op, int nchain, const int* start_orb, const double* params, const double* pab,
-                    const int* frozen_sp, const double* conc, const double* z, Members& M, int tile = 0) {
+                    const int* frozen_sp, const double* conc, const double* z, Members& M, int tile = 0,
+                    const double* pex = nullptr) {
   const int nsm = op->nsmax, ncls = op->ncls, nsp = op->nsp;
   M.nb = nchain;
   M.pair = (tile == 65) ? 1 : 0;  // 65: member tile 64 with complex members as lane pairs
@@ -1662,6 +1718,7 @@ int prepare_members(const asr_gpu_op* op, int nchain, const int* start_orb, cons
   M.fsp.assign(nbp, -1);
   M.z.assign(nbp, cplx{0.0, 0.0});
   M.complex_lin = (pab != nullptr);
+  M.ed3 = (pex != nullptr);
   std::vector<double> mt((size_t)ncls * nsm * nsm * nsm);
   for (int m = 0; m < nchain; ++m) {
     if (start_orb[m] < 0 || start_orb[m] > 8) return fail("start orbital out of range");
@@ -1711,7 +1768,8 @@ int prepare_members(const asr_gpu_op* op, int nchain, const int* start_orb, cons
     for (int q = 0; q < NQ; ++q) {
       const double* par = nullptr;
       if (q < 4) par = pm + (size_t)q * nsp * 9;
-      else if (pab) par = pab + ((size_t)m * 2 + (q - 4)) * nsp * 9;
+      else if (q < 6 && pab) par = pab + ((size_t)m * 2 + (q - 4)) * nsp * 9;
+      else if (q >= 6 && pex) par = pex + ((size_t)m * 4 + (q - 6)) * nsp * 9;
       if (!par) continue;
       // local weights wt(lam, k, kk, kap) = sum_s par(lam, spc(s, kap)) mt(k, kk, s, kap)  (local_weights)
       for (int kap = 0; kap < ncls; ++kap)
@@ -1927,7 +1985,7 @@ int apply_chain_op(const asr_gpu_op* op, const Members& M, const MemDev& md, con
   }
   if (!M.zero_e) { f.xr[nr] = x; f.qr[nr] = QE; f.r_norm[nr] = 1; f.r_coef[nr] = 1.0; ++nr; }
   f.nr = nr;
-  if (lin) { f.xl = x; f.l_norm = 1; f.l_coef = 1.0; }
+  if (lin) { f.xl[0] = x; f.lq[0] = QPA; f.l_norm[0] = 1; f.l_coef[0] = 1.0; f.nl = 1; }
   f.xo[0] = h1; f.o_coef[0] = 1.0; f.no = 1;
   f.out_sq = 1;
   f.dotu = dotu; f.partial = partial;
@@ -1942,17 +2000,61 @@ int apply_chain_op(const asr_gpu_op* op, const Members& M, const MemDev& md, con
   return 0;
 }
 
+// w = z x - K'(z) x for the chains of route ed3 (ed3_kmul and the chain's w = z u - K' u, src/asr_ed3.f90:99-129,169-176):
+//   K' x = et x - h'x + u h'x + h'(u x - Qt h'x),  so  w = sq [ (z - et) x~ + h1 - u h1 - h'(t) ],
+//   h1 = h' x~ = g x~ + D_c S D_c x~,   t = u x~ - Qt h1,
+// in normalised coordinates (x~ = x / sq, h1 and t in amplitude coordinates).  The energy-linear diagonals are the slots
+// (QPA, QPB) = z - et, (QUA, QUB) = u, (QQA, QQB) = Qt.  Three launches: the S pass for h1, the diagonal pass for t, the S
+// pass on t with the other terms and the dot <x, w>.  h' has one d for every species (D_c^1/2).
+template <class T>
+int apply_ed3_op(const asr_gpu_op* op, const Members& M, const MemDev& md, const T* x, T* y, T* t1, T* h1, const T* dotu,
+                 T* partial, hipStream_t strm) {
+  if constexpr (!std::is_same<T, cplx>::value) {
+    return fail("route ed3: complex members only");
+  } else {
+    if (!M.uniform_d) return fail("route ed3: d must be the same for every species (D_c^1/2)");
+    FArgs f = fargs0();
+    f.xs = x; f.xs_norm = 1; f.s_din = 1; f.s_dout = 1; f.s_coef = 1.0;
+    f.xr[0] = x; f.qr[0] = QC; f.r_norm[0] = 1; f.r_coef[0] = 1.0; f.nr = 1;
+    f.out = h1;
+    if (launch_fused<T>(op, md, f, strm)) return 1;
+    f = fargs0();
+    f.xl[0] = x; f.lq[0] = QUA; f.l_norm[0] = 1; f.l_coef[0] = 1.0;
+    f.xl[1] = h1; f.lq[1] = QQA; f.l_norm[1] = 0; f.l_coef[1] = -1.0; f.nl = 2;
+    f.out = t1;
+    if (launch_fused<T>(op, md, f, strm)) return 1;
+    f = fargs0();
+    f.xs = t1; f.xs_norm = 0; f.s_din = 1; f.s_dout = 1; f.s_coef = -1.0;
+    f.xr[0] = t1; f.qr[0] = QC; f.r_norm[0] = 0; f.r_coef[0] = -1.0; f.nr = 1;
+    f.xl[0] = x; f.lq[0] = QPA; f.l_norm[0] = 1; f.l_coef[0] = 1.0;
+    f.xl[1] = h1; f.lq[1] = QUA; f.l_norm[1] = 0; f.l_coef[1] = -1.0; f.nl = 2;
+    f.xo[0] = h1; f.o_coef[0] = 1.0; f.no = 1;
+    f.out_sq = 1;
+    f.dotu = dotu; f.partial = partial;
+    f.out = y;
+    return launch_fused<T>(op, md, f, strm);
+  }
+}
+
+// the operator of one chain level: route ed3's z - K'(z), or H (+ X(z)) of the real and edc chains
+template <class T>
+int apply_level_op(const asr_gpu_op* op, const Members& M, const MemDev& md, const T* x, T* y, T* t1, T* t2, T* h1, bool lin,
+                   const T* dotu, T* partial, hipStream_t strm) {
+  if (M.ed3) return apply_ed3_op<T>(op, M, md, x, y, t1, h1, dotu, partial, strm);
+  return apply_chain_op<T>(op, M, md, x, y, t1, t2, h1, lin, dotu, partial, strm);
+}
+
 // The autotuner of the batched chains (the role of chain_teams / blk_nbmax, src/asr_chain.f90:140-155,
 // src/asr_augblk.f90:32-36): the member tile MT (members per wave, 64 / MT nodes per wave) sets the cache footprint of a
 // node (9 MT values) against the lane occupancy of ragged member counts.  Every candidate runs one operator application
 // of the level (same kernels, same members) between HIP events; the fastest is kept per (operator, member count, kind).
 template <class T>
 int autotune_tile(asr_gpu_op* op, int nchain, const int* start_orb, const double* params, const double* pab,
-                  const int* frozen_sp, const double* conc, const double* z, int* best) {
+                  const int* frozen_sp, const double* conc, const double* z, int* best, const double* pex = nullptr) {
   Members M0;
-  if (prepare_members(op, nchain, start_orb, params, pab, frozen_sp, conc, z, M0)) return 1;
+  if (prepare_members(op, nchain, start_orb, params, pab, frozen_sp, conc, z, M0, 0, pex)) return 1;
   long long key = ((long long)nchain << 8) | (sizeof(T) == 16 ? 1 : 0) | (M0.uniform_d ? 2 : 0) | (M0.zero_o ? 4 : 0) |
-                  (pab ? 8 : 0);
+                  (pab ? 8 : 0) | (pex ? 16 : 0);
   for (auto& kv : op->tuned)
     if (kv.first == key) { *best = kv.second; return 0; }
   const int top = pow2_tile(nchain);
@@ -1963,7 +2065,7 @@ int autotune_tile(asr_gpu_op* op, int nchain, const int* start_orb, const double
   if (top == WAVE && sizeof(T) == sizeof(cplx)) cands.push_back(65);
   for (int mt : cands) {
     Members M;
-    if (prepare_members(op, nchain, start_orb, params, pab, frozen_sp, conc, z, M, mt)) return 1;
+    if (prepare_members(op, nchain, start_orb, params, pab, frozen_sp, conc, z, M, mt, pex)) return 1;
     MemBuffers MB;
     MemDev md;
     if (upload_members(op, M, MB, md)) return 1;
@@ -1975,9 +2077,9 @@ int autotune_tile(asr_gpu_op* op, int nchain, const int* start_orb, const double
     st |= hipEventCreate(&e0) != hipSuccess;
     st |= hipEventCreate(&e1) != hipSuccess;
     for (T* p : {x, y, t1, t2, h1}) st |= hipMemsetAsync(p, 0, nv * sizeof(T), op->stream) != hipSuccess;
-    if (!st) st = apply_chain_op<T>(op, M, md, x, y, t1, t2, h1, pab != nullptr, nullptr, nullptr, op->stream);
+    if (!st) st = apply_level_op<T>(op, M, md, x, y, t1, t2, h1, pab != nullptr, nullptr, nullptr, op->stream);
     if (!st) st |= hipEventRecord(e0, op->stream) != hipSuccess;
-    for (int r = 0; r < 2 && !st; ++r) st = apply_chain_op<T>(op, M, md, x, y, t1, t2, h1, pab != nullptr, nullptr, nullptr, op->stream);
+    for (int r = 0; r < 2 && !st; ++r) st = apply_level_op<T>(op, M, md, x, y, t1, t2, h1, pab != nullptr, nullptr, nullptr, op->stream);
     if (!st) st |= hipEventRecord(e1, op->stream) != hipSuccess;
     if (!st) st |= hipEventSynchronize(e1) != hipSuccess;
     float ms = 0.f;
@@ -1998,12 +2100,22 @@ template <class T>
 struct ChainBuffers {
   T *u = nullptr, *up = nullptr, *w = nullptr, *t1 = nullptr, *t2 = nullptr, *h1 = nullptr, *part = nullptr;
   T *a = nullptr, *b = nullptr, *a_eff = nullptr, *b_eff = nullptr, *inv_b = nullptr;
-  int *level = nullptr, *depth = nullptr, *active = nullptr, *mused = nullptr;
+  T *tv = nullptr, *part2 = nullptr, *c = nullptr;  // route ed3: projection vector, its dot partials, c_k
+  int *level = nullptr, *depth = nullptr, *active = nullptr, *mused = nullptr, *closed = nullptr;
   ~ChainBuffers() {
     for (void* p : {(void*)u, (void*)up, (void*)w, (void*)t1, (void*)t2, (void*)h1, (void*)part, (void*)a, (void*)b,
-                    (void*)a_eff, (void*)b_eff, (void*)inv_b, (void*)level, (void*)depth, (void*)active, (void*)mused})
+                    (void*)a_eff, (void*)b_eff, (void*)inv_b, (void*)level, (void*)depth, (void*)active, (void*)mused,
+                    (void*)tv, (void*)part2, (void*)c, (void*)closed})
       if (p) (void)hipFree(p);
   }
+};
+
+// Route ed3 per member (member 2j: the chain of input chain j from |a> = |u> + o_c |v>, projected on |v>; member 2j + 1:
+// the chain from |v> = h'|u>, projected on |a>; src/asr_ed3.f90:247-278): the energy-linear diagonals u and Qt, o_c, side.
+struct Ed3In {
+  const double* pex;  // nmember x 4 x nsp x 9: ua, ub, qa, qb
+  const double* oc;   // nmember
+  const int* side;    // nmember: 0 from |a>, 1 from |v>
 };
 
 // One batch of chains on one operator: members sorted by depth (deepest first, so member tiles retire in order), device
@@ -2025,7 +2137,9 @@ struct ChainRun {
   std::vector<int> order;  // sorted member -> input chain
   int* tact = nullptr;     // per member tile: 1 while any member of the tile is active
   int level_host = 0;      // levels issued
-  std::vector<cplx> gprev; // adaptive depth: G(z) of each chain at the previous tested level
+  std::vector<cplx> gprev, fprev;  // adaptive depth: G(z) (and route ed3's projection f) at the previous tested level
+  bool ed3 = false;
+  std::vector<double> snorm, tnorm;  // route ed3: norms of the start vector and of the projection vector per member
 
   ~ChainRun() {
     if (tact) (void)hipFree(tact);
@@ -2033,8 +2147,10 @@ struct ChainRun {
 
   int setup(asr_gpu_op* op_, hipStream_t s, int nch, const int* start_orb, const double* params, const double* pab,
             const int* frozen_sp, const double* conc, const double* z, const int* nlev, int project_, int ldab, int tile,
-            double tiny, bool real_chain) {
+            double tiny, bool real_chain, const Ed3In* e3 = nullptr) {
     op = op_; stream = s; nchain = nch; project = project_; ld = ldab; lin = (pab != nullptr);
+    ed3 = (e3 != nullptr);
+    if (ed3 && (!std::is_same<T, cplx>::value || !pab || !z)) return fail("route ed3: complex chains with z - et and z");
     if (!op->have_tables || !op->have_ops || !op->have_blocks || !op->have_classes)
       return fail("asr_gpu chains: operator tables incomplete");
     // deepest chains first
@@ -2044,7 +2160,8 @@ struct ChainRun {
     const int nsp9 = op->nsp * 9, ncn = op->ncls * op->nsmax;
     std::vector<int> so(nch), fz(nch), nl(nch);
     std::vector<double> pp((size_t)nch * 4 * nsp9), pa(pab ? (size_t)nch * 2 * nsp9 : 0), cc(conc ? (size_t)nch * ncn : 0),
-        zz(z ? 2 * nch : 0);
+        zz(z ? 2 * nch : 0), px(ed3 ? (size_t)nch * 4 * nsp9 : 0), ocs(ed3 ? nch : 0);
+    std::vector<int> sides(ed3 ? nch : 0);
     for (int q = 0; q < nch; ++q) {
       const int c = order[q];
       so[q] = start_orb[c];
@@ -2054,12 +2171,18 @@ struct ChainRun {
       if (pab) std::memcpy(&pa[(size_t)q * 2 * nsp9], pab + (size_t)c * 2 * nsp9, sizeof(double) * 2 * nsp9);
       if (conc) std::memcpy(&cc[(size_t)q * ncn], conc + (size_t)c * ncn, sizeof(double) * ncn);
       if (z) { zz[2 * q] = z[2 * c]; zz[2 * q + 1] = z[2 * c + 1]; }
+      if (ed3) {
+        std::memcpy(&px[(size_t)q * 4 * nsp9], e3->pex + (size_t)c * 4 * nsp9, sizeof(double) * 4 * nsp9);
+        ocs[q] = e3->oc[c];
+        sides[q] = e3->side[c];
+      }
     }
     const double* pab_s = pab ? pa.data() : nullptr;
     const double* conc_s = conc ? cc.data() : nullptr;
     const double* z_s = z ? zz.data() : nullptr;
-    if (tile == 0 && autotune_tile<T>(op, nch, so.data(), pp.data(), pab_s, fz.data(), conc_s, z_s, &tile)) return 1;
-    if (prepare_members(op, nch, so.data(), pp.data(), pab_s, fz.data(), conc_s, z_s, M, tile)) return 1;
+    const double* px_s = ed3 ? px.data() : nullptr;
+    if (tile == 0 && autotune_tile<T>(op, nch, so.data(), pp.data(), pab_s, fz.data(), conc_s, z_s, &tile, px_s)) return 1;
+    if (prepare_members(op, nch, so.data(), pp.data(), pab_s, fz.data(), conc_s, z_s, M, tile, px_s)) return 1;
     if (upload_members(op, M, MB, md)) return 1;
     maxdepth = 0;
     for (int q = 0; q < nch; ++q) maxdepth = std::max(maxdepth, nl[q]);
@@ -2076,6 +2199,9 @@ struct ChainRun {
         dalloc(&B.b, (size_t)M.nbp * ldab) || dalloc(&B.a_eff, M.nbp) || dalloc(&B.b_eff, M.nbp) ||
         dalloc(&B.inv_b, M.nbp) || dalloc(&B.level, 1) || dalloc(&B.depth, M.nbp) || dalloc(&B.active, M.nbp) ||
         dalloc(&B.mused, M.nbp) || dalloc(&tact, std::max(ntile, 2)))
+      return 1;
+    if (ed3 && (dalloc(&B.tv, nv) || dalloc(&B.part2, npart) || dalloc(&B.c, (size_t)M.nbp * ldab) ||
+                dalloc(&B.closed, M.nbp)))
       return 1;
     for (T* q : {B.u, B.up, B.w, B.t1, B.t2, B.h1}) HCHECK(hipMemsetAsync(q, 0, nv * sizeof(T), stream));
     HCHECK(hipMemsetAsync(B.a, 0, (size_t)M.nbp * ldab * sizeof(T), stream));
@@ -2100,8 +2226,78 @@ struct ChainRun {
     HCHECK(hipStreamSynchronize(stream));
     cd.level = B.level; cd.depth = B.depth; cd.active = B.active; cd.mused = B.mused; cd.ld = ldab;
     cd.real_chain = real_chain ? 1 : 0; cd.tiny = tiny;
+    cd.cvec = B.c; cd.closed = B.closed;
     sgrid = dim3((unsigned)((nv + 255) / 256));
+    if (ed3 && ed3_start(so, ocs, sides)) return 1;
     return 0;
+  }
+
+  // Route ed3 start vectors (ed3_start, src/asr_ed3.f90:131-144): vv = h'|s0> from the unit vectors in u (one S pass,
+  // normalised coordinates), projected; nv^2 = vv.vv on the device, na^2 = 1 + 2 o_c vv(iv) + o_c^2 nv^2; then u and the
+  // projection vector t of each member, and c_1 = t.u.
+  int ed3_start(const std::vector<int>& so, const std::vector<double>& ocs, const std::vector<int>& sides) {
+    if constexpr (!std::is_same<T, cplx>::value) {
+      return fail("route ed3: complex chains only");
+    } else {
+      const int nb = M.nb, nbp = M.nbp;
+      md.tact = tact;
+      FArgs f = fargs0();
+      f.xs = B.u; f.xs_norm = 1; f.s_din = 1; f.s_dout = 1; f.s_coef = 1.0;
+      f.xr[0] = B.u; f.qr[0] = QC; f.r_norm[0] = 1; f.r_coef[0] = 1.0; f.nr = 1;
+      f.out = B.w; f.out_sq = 1;
+      if (launch_fused<T>(op, md, f, stream)) return 1;
+      hipLaunchKernelGGL(k_update<T>, grid, dim3(BLOCK), 0, stream, dev_of(op), md, B.w, B.u, B.up, B.a_eff, B.b_eff, project,
+                         B.part, (const T*)nullptr, (T*)nullptr);
+      hipLaunchKernelGGL(k_sum_member<T>, dim3(nb), dim3(BLOCK), 0, stream, B.part, nbx, md.mt, B.inv_b);
+      HCHECK(hipGetLastError());
+      std::vector<T> nv2(nbp), row0((size_t)9 * nbp);
+      HCHECK(hipMemcpyAsync(nv2.data(), B.inv_b, nbp * sizeof(T), hipMemcpyDeviceToHost, stream));
+      HCHECK(hipMemcpyAsync(row0.data(), B.w, row0.size() * sizeof(T), hipMemcpyDeviceToHost, stream));
+      HCHECK(hipStreamSynchronize(stream));
+      std::vector<double> coef((size_t)4 * nbp, 0.0);
+      std::vector<T> c1((size_t)nbp * ld);
+      std::memset(c1.data(), 0, c1.size() * sizeof(T));
+      snorm.assign(nb, 0.0);
+      tnorm.assign(nb, 0.0);
+      for (int q = 0; q < nb; ++q) {
+        const double oc = ocs[q];
+        const double vn2 = nv2[q].re;
+        const cplx vi = row0[(size_t)so[q] * nbp + q];
+        const double nvv = std::sqrt(vn2);
+        const double na = std::sqrt((1.0 + 2.0 * oc * vi.re + oc * oc * vn2));
+        if (!(nvv >= 1.0e-14) || !(na >= 1.0e-14)) return fail("ed3_start: zero start vector");
+        double su, sv, tsu, tsv;
+        if (sides[q] == 0) { su = 1.0 / na; sv = oc / na; tsu = 0.0; tsv = 1.0; snorm[q] = na; tnorm[q] = nvv; }
+        else { su = 0.0; sv = 1.0 / nvv; tsu = 1.0; tsv = oc; snorm[q] = nvv; tnorm[q] = na; }
+        coef[4 * q] = su; coef[4 * q + 1] = sv; coef[4 * q + 2] = tsu; coef[4 * q + 3] = tsv;
+        // c_1 = t.u = tsu su + (tsu sv + tsv su) vv(iv) + tsv sv vv.vv
+        const cplx vv2 = nv2[q];
+        c1[(size_t)q * ld] = cplx{tsu * su, 0.0} + (tsu * sv + tsv * su) * vi + (tsv * sv) * vv2;
+      }
+      double* dcoef = nullptr;
+      int* dl0 = nullptr;
+      std::vector<int> l0(nbp, -1);
+      for (int q = 0; q < nb; ++q) l0[q] = so[q];
+      if (dalloc(&dcoef, coef.size()) || dalloc(&dl0, l0.size())) return 1;
+      int st = 0;
+      st |= hipMemcpyAsync(dcoef, coef.data(), coef.size() * 8, hipMemcpyHostToDevice, stream) != hipSuccess;
+      st |= hipMemcpyAsync(dl0, l0.data(), l0.size() * 4, hipMemcpyHostToDevice, stream) != hipSuccess;
+      st |= hipMemcpyAsync(B.c, c1.data(), c1.size() * sizeof(T), hipMemcpyHostToDevice, stream) != hipSuccess;
+      if (!st) {
+        hipLaunchKernelGGL(k_ed3_start, sgrid, dim3(256), 0, stream, (const cplx*)B.w, (cplx*)B.u, (cplx*)B.tv,
+                           (const double*)dcoef, (const int*)dl0, nv, nbp);
+        st |= hipGetLastError() != hipSuccess;
+      }
+      st |= hipMemsetAsync(B.w, 0, nv * sizeof(T), stream) != hipSuccess;
+      st |= hipMemsetAsync(B.part, 0, (size_t)nbx * M.nbp * sizeof(T), stream) != hipSuccess;
+      st |= hipMemsetAsync(B.inv_b, 0, M.nbp * sizeof(T), stream) != hipSuccess;
+      st |= hipMemsetAsync(B.closed, 0, M.nbp * sizeof(int), stream) != hipSuccess;
+      st |= hipStreamSynchronize(stream) != hipSuccess;
+      (void)hipFree(dcoef);
+      (void)hipFree(dl0);
+      if (st) return fail("route ed3: start vectors");
+      return 0;
+    }
   }
 
   // the launches of one level on the group's stream (no host synchronisation): the operator pass and a_k, then the
@@ -2115,7 +2311,7 @@ struct ChainRun {
   int step_apply() {
     md.tact = tact;
     if (g_timer) g_timer->ord = 0;
-    if (apply_chain_op<T>(op, M, md, B.u, B.w, B.t1, B.t2, B.h1, lin, B.u, B.part, stream)) return 1;
+    if (apply_level_op<T>(op, M, md, B.u, B.w, B.t1, B.t2, B.h1, lin, B.u, B.part, stream)) return 1;
     if (timer_mark(stream, KIND_FIN_A, true)) return 1;
     hipLaunchKernelGGL(k_fin_a<T>, dim3(M.nb), dim3(BLOCK), 0, stream, cd, M.nb, B.part, nbx, apply_mt, B.a, B.a_eff);
     if (timer_mark(stream, 0, false)) return 1;
@@ -2124,40 +2320,49 @@ struct ChainRun {
 
   // Adaptive depth of edc_green (src/asr_edc.f90:121-127): at levels k = 8, 12, 16, ... (1-based) of a chain that has
   // not reached its depth, G(z) of the chain terminated at k (cf_species with the operator's tail); from k = 12 on the
-  // chain leaves when |G_k - G_{k-4}| <= depth_tol |G_k|, keeping a_1..a_k and b_1..b_{k-1}.  One host synchronisation
-  // per tested level; the coefficients are a few KB.
+  // chain leaves when |G_k - G_{k-4}| <= depth_tol |G_k|, keeping a_1..a_k and b_1..b_{k-1}.  Route ed3 (chain,
+  // src/asr_ed3.f90:177-184) also asks |f_k - f_{k-4}| <= depth_tol max(|f_k|, |t| |G_k|) of the projection f.  One host
+  // synchronisation per tested level; the coefficients are a few KB.
   int adaptive_check() {
     if constexpr (!std::is_same<T, cplx>::value) return 0;
     else {
       const int k1 = level_host + 1;
       if (!(op->depth_tol > 0.0) || !lin || k1 < 8 || (k1 % 4) != 0) return 0;
       const int nb = M.nb;
-      std::vector<cplx> ah((size_t)nb * ld), bh((size_t)nb * ld);
+      std::vector<cplx> ah((size_t)nb * ld), bh((size_t)nb * ld), ch(ed3 ? (size_t)nb * ld : 0);
       std::vector<int> act(nb), dep(nb);
       HCHECK(hipMemcpyAsync(ah.data(), B.a, ah.size() * sizeof(cplx), hipMemcpyDeviceToHost, stream));
       HCHECK(hipMemcpyAsync(bh.data(), B.b, bh.size() * sizeof(cplx), hipMemcpyDeviceToHost, stream));
+      if (ed3) HCHECK(hipMemcpyAsync(ch.data(), B.c, ch.size() * sizeof(cplx), hipMemcpyDeviceToHost, stream));
       HCHECK(hipMemcpyAsync(act.data(), B.active, nb * sizeof(int), hipMemcpyDeviceToHost, stream));
       HCHECK(hipMemcpyAsync(dep.data(), B.depth, nb * sizeof(int), hipMemcpyDeviceToHost, stream));
       HCHECK(hipStreamSynchronize(stream));
       if (gprev.size() != (size_t)nb) gprev.assign(nb, cplx{0.0, 0.0});
+      if (fprev.size() != (size_t)nb) fprev.assign(nb, cplx{0.0, 0.0});
       std::vector<double> cz(2 * k1, 0.0);
       bool changed = false;
       for (int q = 0; q < nb; ++q) {
         if (!act[q] || k1 >= dep[q]) continue;
         double g[2], f[2];
         const double zz[2] = {M.z[q].re, M.z[q].im};
-        if (asr_host_cf_species((const double*)&ah[(size_t)q * ld], (const double*)&bh[(size_t)q * ld], cz.data(), k1, zz,
-                                op->depth_tail, g, f, nullptr, nullptr))
-          return fail("adaptive depth: cf_species");
-        const cplx gt{g[0], g[1]};
+        const double* cq = ed3 ? (const double*)&ch[(size_t)q * ld] : cz.data();
+        // the status counts bp fallbacks (hull values then), not errors
+        (void)asr_host_cf_species((const double*)&ah[(size_t)q * ld], (const double*)&bh[(size_t)q * ld], cq, k1, zz,
+                                  op->depth_tail, g, f, nullptr, nullptr);
+        const cplx gt{g[0], g[1]}, ft{f[0], f[1]};
         if (k1 > 8) {
-          const cplx dg = gt - gprev[q];
-          if (std::hypot(dg.re, dg.im) <= op->depth_tol * std::hypot(gt.re, gt.im)) {
+          const cplx dg = gt - gprev[q], df = ft - fprev[q];
+          const double ag = std::hypot(gt.re, gt.im);
+          bool conv = std::hypot(dg.re, dg.im) <= op->depth_tol * ag;
+          if (ed3)
+            conv = conv && std::hypot(df.re, df.im) <= op->depth_tol * std::max(std::hypot(ft.re, ft.im), tnorm[q] * ag);
+          if (conv) {
             act[q] = 0;
             changed = true;
           }
         }
         gprev[q] = gt;
+        fprev[q] = ft;
       }
       if (changed) {
         HCHECK(hipMemcpyAsync(B.active, act.data(), nb * sizeof(int), hipMemcpyHostToDevice, stream));
@@ -2170,10 +2375,10 @@ struct ChainRun {
   int step_update() {
     if (timer_mark(stream, KIND_UPDATE, true)) return 1;
     hipLaunchKernelGGL(k_update<T>, grid, dim3(BLOCK), 0, stream, dev_of(op), md, B.w, B.u, B.up, B.a_eff, B.b_eff, project,
-                       B.part);
+                       B.part, (const T*)B.tv, B.part2);
     if (timer_mark(stream, 0, false) || timer_mark(stream, KIND_FIN_B, true)) return 1;
     hipLaunchKernelGGL(k_fin_b<T>, dim3(M.nb), dim3(BLOCK), 0, stream, cd, M.nb, B.part, nbx, md.mt, B.b, B.b_eff,
-                       B.inv_b);
+                       B.inv_b, (const T*)B.part2);
     if (timer_mark(stream, 0, false) || timer_mark(stream, KIND_SCALE, true)) return 1;
     hipLaunchKernelGGL(k_scale<T>, sgrid, dim3(256), 0, stream, B.w, B.inv_b, nv, M.nbp);
     if (timer_mark(stream, 0, false)) return 1;
@@ -2201,18 +2406,30 @@ struct ChainRun {
     return 0;
   }
 
-  int finish(T* a_out, T* b_out, int* mused_out) {
-    std::vector<T> ah((size_t)M.nbp * ld), bh((size_t)M.nbp * ld);
-    std::vector<int> mu(M.nbp);
+  // coefficients in the caller's chain order; route ed3 also c_k, the closure flags and the start / projection norms
+  int finish(T* a_out, T* b_out, int* mused_out, T* c_out = nullptr, int* closed_out = nullptr, double* snorm_out = nullptr,
+             double* tnorm_out = nullptr) {
+    std::vector<T> ah((size_t)M.nbp * ld), bh((size_t)M.nbp * ld), chv(ed3 ? (size_t)M.nbp * ld : 0);
+    std::vector<int> mu(M.nbp), cl(ed3 ? M.nbp : 0);
     HCHECK(hipMemcpyAsync(ah.data(), B.a, ah.size() * sizeof(T), hipMemcpyDeviceToHost, stream));
     HCHECK(hipMemcpyAsync(bh.data(), B.b, bh.size() * sizeof(T), hipMemcpyDeviceToHost, stream));
     HCHECK(hipMemcpyAsync(mu.data(), B.mused, mu.size() * sizeof(int), hipMemcpyDeviceToHost, stream));
+    if (ed3) {
+      HCHECK(hipMemcpyAsync(chv.data(), B.c, chv.size() * sizeof(T), hipMemcpyDeviceToHost, stream));
+      HCHECK(hipMemcpyAsync(cl.data(), B.closed, cl.size() * sizeof(int), hipMemcpyDeviceToHost, stream));
+    }
     HCHECK(hipStreamSynchronize(stream));
     for (int q = 0; q < nchain; ++q) {
       const int c = order[q];
       std::memcpy(a_out + (size_t)c * ld, ah.data() + (size_t)q * ld, sizeof(T) * ld);
       std::memcpy(b_out + (size_t)c * ld, bh.data() + (size_t)q * ld, sizeof(T) * ld);
       mused_out[c] = mu[q];
+      if (ed3) {
+        if (c_out) std::memcpy(c_out + (size_t)c * ld, chv.data() + (size_t)q * ld, sizeof(T) * ld);
+        if (closed_out) closed_out[c] = cl[q];
+        if (snorm_out) snorm_out[c] = snorm[q];
+        if (tnorm_out) tnorm_out[c] = tnorm[q];
+      }
     }
     return 0;
   }
@@ -2370,6 +2587,74 @@ int run_multi(int ngroup, asr_gpu_op* const* ops, const int* ngch, const int* st
   for (hipEvent_t e : {e0, e1, fork}) (void)hipEventDestroy(e);
   for (auto& e : join) (void)hipEventDestroy(e);
   cleanup();
+  return status;
+}
+
+// Route ed3 chains on one operator (the members of asr_gpu_ed3_blocks, two per energy and start orbital): mode 0 host-issued
+// levels, 1 the plain form (one chain per launch sequence, host read-back per level, as chain), 2 the whole run as one HIP
+// graph (fixed depths only: the adaptive depth decides on the host between levels, so it falls back to mode 0).
+int run_ed3(asr_gpu_op* op, int nmem, const int* so, const double* params, const double* pab, const int* fz,
+            const double* conc, const double* z, const int* nlev, const Ed3In& e3, int project, int ldab, cplx* a, cplx* b,
+            cplx* c, int* mused, int* closed, double* snorm, double* tnorm, int mode, double* elapsed_ms) {
+  const int nsp9 = op->nsp * 9, ncn = op->ncls * op->nsmax;
+  *elapsed_ms = 0.0;
+  if (mode == 2 && op->depth_tol > 0.0) mode = 0;
+  if (mode == 1 && nmem > 1) {
+    for (int q = 0; q < nmem; ++q) {
+      const Ed3In e1{e3.pex + (size_t)q * 4 * nsp9, e3.oc + q, e3.side + q};
+      double ms = 0.0;
+      if (run_ed3(op, 1, so + q, params + (size_t)q * 4 * nsp9, pab + (size_t)q * 2 * nsp9, fz ? fz + q : nullptr,
+                  conc ? conc + (size_t)q * ncn : nullptr, z + 2 * q, nlev + q, e1, project, ldab, a + (size_t)q * ldab,
+                  b + (size_t)q * ldab, c + (size_t)q * ldab, mused + q, closed + q, snorm + q, tnorm + q, 1, &ms))
+        return 1;
+      *elapsed_ms += ms;
+    }
+    return 0;
+  }
+  ChainRun<cplx> R;
+  if (R.setup(op, op->stream, nmem, so, params, pab, fz, conc, z, nlev, project, ldab, mode == 1 ? 1 : op->tile_pref, 1.0e-12,
+              false, &e3))
+    return 1;
+  hipEvent_t e0, e1;
+  HCHECK(hipEventCreate(&e0));
+  HCHECK(hipEventCreate(&e1));
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int status = 0;
+  if (mode == 2) {
+    HCHECK(hipStreamBeginCapture(op->stream, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < R.maxdepth && !status; ++k) status = R.step();
+    hipError_t ce = hipStreamEndCapture(op->stream, &graph);
+    if (!status && ce != hipSuccess) status = fail(std::string("graph capture: ") + hipGetErrorString(ce));
+    if (!status && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) status = fail("graph instantiate");
+    if (!status) {
+      HCHECK(hipEventRecord(e0, op->stream));
+      if (hipGraphLaunch(exec, op->stream) != hipSuccess) status = fail("graph launch");
+      HCHECK(hipEventRecord(e1, op->stream));
+    }
+  } else {
+    HCHECK(hipEventRecord(e0, op->stream));
+    for (int k = 0; k < R.maxdepth && !status; ++k) {
+      status = R.step();
+      if (!status && mode == 1) {
+        int actv = 0;
+        status = R.any_active(&actv);
+        if (!status && !actv) break;
+      }
+    }
+    HCHECK(hipEventRecord(e1, op->stream));
+  }
+  if (!status) {
+    HCHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HCHECK(hipEventElapsedTime(&ms, e0, e1));
+    *elapsed_ms = ms;
+    status = R.finish(a, b, mused, c, closed, snorm, tnorm);
+  }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   return status;
 }
 
@@ -2800,6 +3085,89 @@ int asr_gpu_chains_edc_multi(int ngroup, asr_gpu_op* const* ops, const int* ncha
   if (ngroup <= 0) return fail("chains_multi: no groups");
   return run_multi<cplx>(ngroup, ops, nchain, start_orb, params, pab, frozen_sp, conc, z, nlev, project, ldab, (cplx*)a,
                          (cplx*)b, mused, mode, 1e-12, false, elapsed_ms);
+}
+
+// Route ed3 (ed3_blocks / ed3_blocks_blk, src/asr_ed3.f90:247-340): per input chain (start orbital, energy, central species)
+// the chain from |a> / na projected on |v> and the chain from |v> / nv projected on |a>, run as one batch of 2 nchain
+// members; the central blocks bk = (q0 - 2 o_c q1 + o_c^2 q2, q1 - o_c q2, q2, q1a - q1b) with q1 = (q1a + q1b) / 2 from
+// cf_species (the operator's tail) or cf_closed.  params: c = g, d = D_c^1/2, o = e = 0 (ed3_setup); cf: nchain x 6 x nsp
+// x 9 (et = z cf1 + cf2, u = z cf3 + cf4, Qt = z cf5 + cf6).
+int asr_gpu_ed3_blocks(asr_gpu_op* op, int nchain, const int* start_orb, const double* params, const double* cf,
+                       const double* oc, const int* frozen_sp, const double* conc, const double* z, const int* nlev,
+                       int project, int ldab, double* bk, double* a, double* b, double* c, int* mused, int mode,
+                       double* elapsed_ms) {
+  if (nchain <= 0) return 0;
+  if (!op || !start_orb || !params || !cf || !oc || !z || !nlev || !bk || !a || !b || !c || !mused)
+    return fail("asr_gpu_ed3_blocks: null argument");
+  if (mode < 0 || mode > 2) return fail("asr_gpu_ed3_blocks: mode 0 (batched), 1 (plain) or 2 (graph)");
+  const int nsp9 = op->nsp * 9, ncn = op->ncls * op->nsmax, nm = 2 * nchain;
+  for (int j = 0; j < nchain; ++j) {
+    for (int q = 2 * nsp9; q < 4 * nsp9; ++q)
+      if (params[(size_t)j * 4 * nsp9 + q] != 0.0) return fail("asr_gpu_ed3_blocks: o and e must be zero (ed3_setup)");
+    if (nlev[j] < 1 || nlev[j] > ldab) return fail("asr_gpu_ed3_blocks: 1 <= nlev <= ldab");
+  }
+  std::vector<int> so(nm), fz(nm), nl(nm), side(nm);
+  std::vector<double> pp((size_t)nm * 4 * nsp9), pab((size_t)nm * 2 * nsp9), pex((size_t)nm * 4 * nsp9), ocm(nm), zz(2 * nm),
+      cc(conc ? (size_t)nm * ncn : 0);
+  for (int j = 0; j < nchain; ++j)
+    for (int sd = 0; sd < 2; ++sd) {
+      const int q = 2 * j + sd;
+      so[q] = start_orb[j];
+      fz[q] = frozen_sp ? frozen_sp[j] : -1;
+      nl[q] = nlev[j];
+      side[q] = sd;
+      ocm[q] = oc[j];
+      zz[2 * q] = z[2 * j];
+      zz[2 * q + 1] = z[2 * j + 1];
+      std::memcpy(&pp[(size_t)q * 4 * nsp9], params + (size_t)j * 4 * nsp9, sizeof(double) * 4 * nsp9);
+      if (conc) std::memcpy(&cc[(size_t)q * ncn], conc + (size_t)j * ncn, sizeof(double) * ncn);
+      const double* cj = cf + (size_t)j * 6 * nsp9;
+      // z - et = z (1 - cf1) - cf2; u = z cf3 + cf4; Qt = z cf5 + cf6
+      for (int t = 0; t < nsp9; ++t) {
+        pab[(size_t)q * 2 * nsp9 + t] = 1.0 - cj[t];
+        pab[(size_t)q * 2 * nsp9 + nsp9 + t] = -cj[nsp9 + t];
+      }
+      std::memcpy(&pex[(size_t)q * 4 * nsp9], cj + 2 * nsp9, sizeof(double) * 4 * nsp9);
+    }
+  std::vector<int> closed(nm, 0);
+  std::vector<double> snorm(nm), tnorm(nm);
+  const Ed3In e3{pex.data(), ocm.data(), side.data()};
+  double ms = 0.0;
+  cplx* ac = (cplx*)a;
+  cplx* bc = (cplx*)b;
+  cplx* ccv = (cplx*)c;
+  if (run_ed3(op, nm, so.data(), pp.data(), pab.data(), fz.data(), conc ? cc.data() : nullptr, zz.data(), nl.data(), e3,
+              project, ldab, ac, bc, ccv, mused, closed.data(), snorm.data(), tnorm.data(), mode, &ms))
+    return 1;
+  if (elapsed_ms) *elapsed_ms = ms;
+  for (int j = 0; j < nchain; ++j) {
+    cplx q[2][2];  // (norm^2 g, norm f) per side
+    for (int sd = 0; sd < 2; ++sd) {
+      const int m = 2 * j + sd;
+      const size_t o = (size_t)m * ldab;
+      double g[2], f[2];
+      int st;
+      if (closed[m])
+        st = asr_host_cf_closed((const double*)(ac + o), (const double*)(bc + o), (const double*)(ccv + o), mused[m], z + 2 * j,
+                                g, f);
+      else  // the status of cf_species counts bp fallbacks (hull values then), not errors
+        st = asr_host_cf_species((const double*)(ac + o), (const double*)(bc + o), (const double*)(ccv + o), mused[m],
+                                 z + 2 * j, op->depth_tail, g, f, nullptr, nullptr) < 0;
+      if (st) return fail("asr_gpu_ed3_blocks: continued fraction");
+      const double nr = snorm[m];
+      q[sd][0] = (nr * nr) * cplx{g[0], g[1]};
+      q[sd][1] = nr * cplx{f[0], f[1]};
+    }
+    const cplx q0 = q[0][0], q1a = q[0][1], q2 = q[1][0], q1b = q[1][1];
+    const double ocj = oc[j];
+    const cplx fm = 0.5 * (q1a + q1b);
+    const cplx r[4] = {q0 - (2.0 * ocj) * fm + (ocj * ocj) * q2, fm - ocj * q2, q2, q1a - q1b};
+    for (int t = 0; t < 4; ++t) {
+      bk[(size_t)j * 8 + 2 * t] = r[t].re;
+      bk[(size_t)j * 8 + 2 * t + 1] = r[t].im;
+    }
+  }
+  return 0;
 }
 
 int asr_gpu_lanczos_real_multi(int ngroup, asr_gpu_op* const* ops, const int* nchain, const int* start_orb,

@@ -160,6 +160,28 @@ inline int argmax_re(const std::vector<cplx>& v) {
 
 }  // namespace
 
+// g11 and f = sum_k c_k [g]_k1 of a complex-symmetric chain that closed (Krylov space exhausted): the finite fraction, no
+// tail (cf_closed, src/asr_ed3.f90:214-230).
+extern "C" int asr_host_cf_closed(const double* a_, const double* b_, const double* c_, int m, const double* z_, double* g11_,
+                                  double* f_) {
+  if (m < 1) return 1;
+  const cplx* a = reinterpret_cast<const cplx*>(a_);
+  const cplx* b = reinterpret_cast<const cplx*>(b_);
+  const cplx* c = reinterpret_cast<const cplx*>(c_);
+  const cplx z(z_[0], z_[1]);
+  std::vector<cplx> gt(m);
+  gt[m - 1] = 1.0 / (z - a[m - 1]);
+  for (int j = m - 2; j >= 0; --j) gt[j] = 1.0 / (z - a[j] - b[j] * b[j] * gt[j + 1]);
+  cplx f(0.0, 0.0), prod = gt[0];
+  for (int k = 0; k < m; ++k) {
+    f += c[k] * prod;
+    if (k < m - 1) prod = prod * b[k] * gt[k + 1];
+  }
+  g11_[0] = gt[0].real(); g11_[1] = gt[0].imag();
+  f_[0] = f.real(); f_[1] = f.imag();
+  return 0;
+}
+
 // g11 and f = sum_k c_k [g]_k1 of the complex-symmetric chain (a, b) at z with a square-root tail (src/asr_chain.f90:67-121).
 // tail: 0 hull, 1 bp, 2 avg2, 3 none.  a, b, c, z, g11, f, edges, pv are interleaved (re, im) pairs; edges and pv may be NULL.
 // Returns the number of fallbacks (bp fixed point not converged or eigen-solver failure), 0 normally.

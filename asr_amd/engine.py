@@ -49,6 +49,7 @@ def lib() -> C.CDLL:
         L.asr_gpu_chains_edc.argtypes = [P, C.c_int, P, P, P, P, P, P, P, C.c_int, C.c_int, P, P, P, C.c_int, P]
         L.asr_gpu_chains_edc_multi.argtypes = [C.c_int, P, P, P, P, P, P, P, P, P, C.c_int, C.c_int, P, P, P, C.c_int, P]
         L.asr_gpu_lanczos_real_multi.argtypes = [C.c_int, P, P, P, P, P, P, P, C.c_double, C.c_int, P, P, P, C.c_int, P]
+        L.asr_gpu_ed3_blocks.argtypes = [P, C.c_int, P, P, P, P, P, P, P, P, C.c_int, C.c_int, P, P, P, P, P, C.c_int, P]
         L.asr_gpu_lanczos_adjoint.argtypes = [P, C.c_int, P, P, P, P, C.c_int, C.c_int, P, P, P, P, P, P, P]
         L.asr_gpu_profile_chains.argtypes = [P, C.c_int, P, P, P, P, P, P, P, C.c_int, P, P, P]
         L.asr_gpu_real_create.argtypes = [C.c_int, C.c_int, P, C.c_int, P, P, P, P, C.POINTER(C.c_void_p)]
@@ -215,6 +216,29 @@ class GpuOperator:
         _check(lib().asr_gpu_chains_edc(self.h, nch, _p(s), _p(p), _p(pab), _p(f), _p(c), _p(z), _p(nlev), int(project), ld,
                                         _p(a), _p(b), _p(mu), int(mode), _p(ms)), "asr_gpu_chains_edc")
         return a, b, mu, float(ms[0])
+
+    def ed3_blocks(self, members: list[Member], cf, oc, nlev, project: bool = True, mode: int = 0):
+        """Route ed3 (ed3_blocks / ed3_blocks_blk): members from asr_amd.ed3.ed3_member (c = g, d = D_c^1/2, o = e = 0,
+        frozen central species, z), cf (nchain, 6, nsp, 9) from ed3_linear, oc (nchain) = o_c of the start orbital.
+        Returns bk (nchain, 4) = (G_11, (hG)_11, (hGh)_11, q1a - q1b), the chains' a, b, c (nchain, 2, ld), mused
+        (nchain, 2) and the device time in ms.  mode 0 batched, 1 plain, 2 one HIP graph."""
+        nch = len(members)
+        nlev = _i32(np.broadcast_to(np.asarray(nlev), (nch,)))
+        ld = int(nlev.max())
+        cf = _f64(np.broadcast_to(np.asarray(cf, np.float64), (nch, 6, self.op.nsp, 9)))
+        oc = _f64(np.broadcast_to(np.asarray(oc, np.float64), (nch,)))
+        bk = np.zeros((nch, 4), np.complex128)
+        a = np.zeros((nch, 2, ld), np.complex128)
+        b = np.zeros((nch, 2, ld), np.complex128)
+        c = np.zeros((nch, 2, ld), np.complex128)
+        mu = np.zeros((nch, 2), np.int32)
+        ms = np.zeros(1)
+        s, p, f, cc, _, _ = pack_members(self.op, members)
+        z = np.ascontiguousarray(np.array([complex(m.z) for m in members], dtype=np.complex128))
+        _check(lib().asr_gpu_ed3_blocks(self.h, nch, _p(s), _p(p), _p(cf), _p(oc), _p(f), _p(cc), _p(z), _p(nlev),
+                                        int(project), ld, _p(bk), _p(a), _p(b), _p(c), _p(mu), int(mode), _p(ms)),
+               "asr_gpu_ed3_blocks")
+        return bk, a, b, c, mu, float(ms[0])
 
     def profile(self, members: list[Member], nlev, project: bool = True, complex_route: bool = True):
         """Mean duration (ms) and kind of every kernel launch of one chain level."""

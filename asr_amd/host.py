@@ -27,6 +27,7 @@ def lib() -> C.CDLL:
         L.asr_host_tables_free.argtypes = [C.c_void_p]
         L.asr_host_tables_free.restype = None
         L.asr_host_cf_species.argtypes = [P, P, P, C.c_int, P, C.c_int, P, P, P, P]
+        L.asr_host_cf_closed.argtypes = [P, P, P, C.c_int, P, P, P]
         L.asr_host_bp_asymptote.argtypes = [P, P, C.c_int, P, P, P]
         L.asr_host_terminated_green.argtypes = [P, P, C.c_int, C.c_double, C.c_double, C.c_int, P, P]
         L.asr_host_kpm_dos.argtypes = [P, C.c_int, C.c_double, C.c_double, C.c_int, P, P]
@@ -89,6 +90,19 @@ def cf_species(a, b, c, m: int, z: complex, tail: str = "hull", want_pv: bool = 
     lib().asr_host_cf_species(_p(a), _p(b), _p(c), int(m), _p(zz), TAIL[tail], _p(g), _p(f), None, _p(pv))
     if want_pv:
         return g[0], f[0], pv[:m]
+    return g[0], f[0]
+
+
+def cf_closed(a, b, c, m: int, z: complex):
+    """(g11, f) of a complex-symmetric chain that closed: the finite fraction, no tail (asr_host_cf_closed)."""
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.complex128)[:m])
+    b = np.ascontiguousarray(np.asarray(b, dtype=np.complex128)[:m])
+    c = np.ascontiguousarray(np.asarray(c, dtype=np.complex128)[:m])
+    zz = np.array([z], dtype=np.complex128)
+    g = np.zeros(1, np.complex128)
+    f = np.zeros(1, np.complex128)
+    if lib().asr_host_cf_closed(_p(a), _p(b), _p(c), int(m), _p(zz), _p(g), _p(f)):
+        raise ValueError("cf_closed: m >= 1")
     return g[0], f[0]
 
 

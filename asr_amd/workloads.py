@@ -134,6 +134,39 @@ def ternary_edc(L: int = 8, levels: int = 4, depth: int = 30, seed: int = 2, gro
                     dict(L=L, levels=levels, group=group, seed=seed, nodes=op.nn, z=z))
 
 
+def ternary_ed3(L: int = 8, levels: int = 4, depth: int = 30, seed: int = 2, group: str = "d2h",
+                z: complex = complex(-0.2, 0.15), conc=(0.66, 0.14, 0.20), reorder: bool = True) -> Workload:
+    """Route ed3 on the fcc ternary operator: 2 spins x 3 central species x 9 start orbitals at one energy, each giving
+    the two chains of ed3_blocks.  info: cf (nchain, 6, 3, 9) energy-linear coefficients, oc (nchain) o_c of the start
+    orbital, ed (per chain the Ed3Species)."""
+    from . import ed3 as E3
+    rng = np.random.default_rng(seed)
+    g, block, block0 = graph_and_blocks("fcc", L)
+    cls = np.zeros(g.n, np.int32)
+    perm, mats = S.class_ops(g.coord, 2 * L, cls, group)
+    dmat = S.orbital_rep(mats)
+    fl = np.ones(g.n, np.uint8)
+    fl[0] = 0
+    op = build_operator(g, cls, [3], [[0, 1, 2]], [list(conc)], perm, dmat, 0, levels, maxrank=1, flippable=fl,
+                        block=block[None], block0=block0[None], nsp=3)
+    if reorder:
+        op = reorder_nodes(op)
+    spins = E3.ed3_params(rng, 3, 2)
+    members, cf, oc, eds = [], [], [], []
+    for sp in range(2):
+        for cen in range(3):
+            ed = E3.ed3_setup(spins[sp], cen)
+            lin = E3.ed3_linear(ed)
+            for l0 in range(9):
+                members.append(E3.ed3_member(ed, l0, z))
+                cf.append(lin)
+                oc.append(E3.ed3_oc(ed, l0))
+                eds.append(ed)
+    return Workload("fcc ternary ed3", op, members, np.full(len(members), depth, np.int32), True,
+                    dict(L=L, levels=levels, group=group, seed=seed, nodes=op.nn, z=z, cf=np.array(cf), oc=np.array(oc),
+                         ed=eds))
+
+
 def ternary_adjoint(L: int = 8, levels: int = 4, nchains: int = 256, depth: int = 40, seed: int = 5, group: str = "d2h",
                     conc=(0.66, 0.14, 0.20), reorder: bool = True, offset: float = 0.02) -> Workload:
     """Real chains of the ternary operator for gradients: theta = (c, d) of shape (2, 3 species, 3 l shells).  Chain j

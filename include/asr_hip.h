@@ -93,6 +93,22 @@ int asr_gpu_lanczos_real_multi(int ngroup, asr_gpu_op* const* ops, const int* nc
                                const double* params, const int* frozen_sp, const double* conc, const int* nlev, double tiny,
                                int ldab, double* a, double* b, int* mused, int mode, double* elapsed_ms);
 
+/* Route ed3, the full-ASA energy-dependent recursion: replaces ed3_blocks / ed3_blocks_blk (src/asr_ed3.f90:247-340) with
+ * their start vectors (ed3_start :131-144) and chains (chain / chain_blk :148-211, 307-392).  Per input chain j (start
+ * orbital l0, energy z, start site frozen to the central species) two complex-symmetric chains on z - K'(z),
+ * K' = et - h' + u h' + h' u - h' Qt h' (ed3_kmul :99-129): from |a> = |u> + o_c h'|u> (normalised) projected on |v> = h'|u>,
+ * and from |v> (normalised) projected on |a>.  params: nchain x 4 x nsp x 9 with c = g, d = D_c^1/2 and o = e = 0
+ * (ed3_setup :53-84); cf: nchain x 6 x nsp x 9, et = z cf1 + cf2, u = z cf3 + cf4, Qt = z cf5 + cf6 (the cf array of
+ * ed3_blocks_blk); oc: o_c of the start orbital per chain; z: nchain complex; nlev: depth per chain (<= ldab).
+ * bk: nchain x 4 complex = (G_11, (hG)_11, (hGh)_11, q1a - q1b) as ed3_blocks; a, b, c: nchain x 2 x ldab complex (the two
+ * chains' coefficients and projections c_k = <t|u_k>); mused: nchain x 2.  The continued fractions use the operator's tail
+ * (asr_gpu_op_set_depth_tol) or, for a chain that closed (b_k < 1e-12), the finite fraction.  mode 0: batched, host-issued
+ * levels; 1: plain (one chain per launch sequence); 2: one HIP graph (mode 0 when the adaptive depth is on). */
+int asr_gpu_ed3_blocks(asr_gpu_op* op, int nchain, const int* start_orb, const double* params, const double* cf,
+                       const double* oc, const int* frozen_sp, const double* conc, const double* z, const int* nlev,
+                       int project, int ldab, double* bk, double* a, double* b, double* c, int* mused, int mode,
+                       double* elapsed_ms);
+
 /* Gradient of a loss L(a, b) of real Lanczos chains with respect to the chain parameters (asr_gpu_adjoint of the
  * replacement interface).  The reference has no adjoint: its nearest analogue is the Hellmann-Feynman trace of EPI,
  * dOmega/dlambda = -(1/pi) Im sum_j w_j Tr[G(z_j) dH/dlambda] (src/asr_epi.f90:14-28), for site-0 parameters only.  Here the

@@ -31,6 +31,9 @@ void asr_host_tables_free(asr_augcls_tables* t);
  * tail: 0 hull, 1 bp, 2 avg2, 3 none.  Returns the number of bp fallbacks. */
 int asr_host_cf_species(const double* a, const double* b, const double* c, int m, const double* z, int tail, double* g11,
                         double* f, double* edges, double* pv);
+/* The same for a chain that closed (b_m < tiny, Krylov space exhausted): the finite fraction without a tail, cf_closed
+ * (src/asr_ed3.f90:214-230). */
+int asr_host_cf_closed(const double* a, const double* b, const double* c, int m, const double* z, double* g11, double* f);
 /* Beer-Pettifor asymptote: bp_asymptote (src/reference/asr_terminator.f90:39-67).  1 = not converged. */
 int asr_host_bp_asymptote(const double* a, const double* b, int n, double* ainf, double* binf, int* niter);
 /* G(z) of a real chain with a square-root tail: terminated_green (src/reference/asr_terminator.f90:117-126). */

@@ -230,3 +230,24 @@ def test_ordered_nearest_neighbour_band(kind, z):
     a, b, n = OracleOp(op).lanczos(m, 0, 3)
     assert abs(a[0]) < 1e-14
     assert abs(b[0] ** 2 - z) < 1e-12
+
+
+def test_cf_closed_is_the_resolvent_of_the_chain_matrix():
+    """cf_closed (src/asr_ed3.f90:214-230) of a complex-symmetric chain: g = [(z - J)^-1]_11 and f = sum_k c_k
+    [(z - J)^-1]_k1 of the tridiagonal chain matrix J (dense solve), host C++ and the numpy oracle alike."""
+    from oracle.oracle import np_cf_closed
+    rng = np.random.default_rng(5)
+    m = 11
+    a = rng.normal(size=m) + 0.2j * rng.normal(size=m)
+    b = 0.5 + rng.uniform(size=m) + 0.1j * rng.normal(size=m)
+    c = rng.normal(size=m) + 1j * rng.normal(size=m)
+    z = complex(0.3, 0.4)
+    j = np.diag(a) + np.diag(b[:m - 1], 1) + np.diag(b[:m - 1], -1)
+    col = np.linalg.solve(z * np.eye(m) - j, np.eye(m)[:, 0])
+    g, f = host.cf_closed(a, b, c, m, z)
+    assert abs(g - col[0]) < 1e-13 * abs(col[0])
+    assert abs(f - c @ col) < 1e-12 * np.max(np.abs(c)) * np.max(np.abs(col))
+    go, fo = np_cf_closed(a, b, c, m, z)
+    assert abs(go - g) < 1e-14 and abs(fo - f) < 1e-13
+    gs, fs = host.cf_species(a, b, c, m, z, "none")
+    assert abs(gs - g) < 1e-14 and abs(fs - f) < 1e-13
