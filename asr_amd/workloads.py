@@ -135,22 +135,25 @@ def ternary_edc(L: int = 8, levels: int = 4, depth: int = 30, seed: int = 2, gro
 
 
 def ternary_ed3(L: int = 8, levels: int = 4, depth: int = 30, seed: int = 2, group: str = "d2h",
-                z: complex = complex(-0.2, 0.15), conc=(0.66, 0.14, 0.20), reorder: bool = True) -> Workload:
+                z: complex = complex(-0.2, 0.15), conc=(0.66, 0.14, 0.20), reorder: bool = True,
+                op: AugclsOperator | None = None) -> Workload:
     """Route ed3 on the fcc ternary operator: 2 spins x 3 central species x 9 start orbitals at one energy, each giving
     the two chains of ed3_blocks.  info: cf (nchain, 6, 3, 9) energy-linear coefficients, oc (nchain) o_c of the start
-    orbital, ed (per chain the Ed3Species)."""
+    orbital, ed (per chain the Ed3Species).  op: the operator of ternary_edc with the same L, levels, group and conc
+    (its node tables do not depend on the species parameters), or None to build it."""
     from . import ed3 as E3
     rng = np.random.default_rng(seed)
-    g, block, block0 = graph_and_blocks("fcc", L)
-    cls = np.zeros(g.n, np.int32)
-    perm, mats = S.class_ops(g.coord, 2 * L, cls, group)
-    dmat = S.orbital_rep(mats)
-    fl = np.ones(g.n, np.uint8)
-    fl[0] = 0
-    op = build_operator(g, cls, [3], [[0, 1, 2]], [list(conc)], perm, dmat, 0, levels, maxrank=1, flippable=fl,
-                        block=block[None], block0=block0[None], nsp=3)
-    if reorder:
-        op = reorder_nodes(op)
+    if op is None:
+        g, block, block0 = graph_and_blocks("fcc", L)
+        cls = np.zeros(g.n, np.int32)
+        perm, mats = S.class_ops(g.coord, 2 * L, cls, group)
+        dmat = S.orbital_rep(mats)
+        fl = np.ones(g.n, np.uint8)
+        fl[0] = 0
+        op = build_operator(g, cls, [3], [[0, 1, 2]], [list(conc)], perm, dmat, 0, levels, maxrank=1, flippable=fl,
+                            block=block[None], block0=block0[None], nsp=3)
+        if reorder:
+            op = reorder_nodes(op)
     spins = E3.ed3_params(rng, 3, 2)
     members, cf, oc, eds = [], [], [], []
     for sp in range(2):

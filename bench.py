@@ -4,7 +4,8 @@ Headline: chain-levels/s of the batched complex-symmetric (route edc) recursion 
 (Fe0.66Ni0.14Cr0.20-like, spd, 2 shells = 18 slots, levels 4, rank 1, D2h-reduced node set, start site frozen to each
 species, 2 spins: 9 x 3 x 2 = 54 chains of depth 30, FP64), next to the plain form (one chain per launch sequence, host
 synchronisation per level) on the same chains, the KPM moments of the 40^3-cell fcc supercell (2,304,000 rows, 64
-Rademacher vectors, 2048 moments) and the oracle's CPU chains on the host cores.
+Rademacher vectors, 2048 moments), route ed3 on the same operator, the adjoint, and the oracle's CPU chains on the host
+cores.
 
 A step is one batched pass of all 54 chains to depth 30; inputs (operator tables, start vectors) are resident on the GPU
 when the timed region starts.  python bench.py [--steps K] [--warmup W] [--quick]
@@ -136,6 +137,7 @@ def main():
     ap.add_argument("--kpm-moments", type=int, default=2048)
     ap.add_argument("--no-l21", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-ed3", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the engine's multi-GPU form (chains dealt across GPUs, one all-gather of the "
@@ -227,6 +229,25 @@ def main():
         out["speedup_batched_vs_plain"] = round(value / plain, 2)
         out["plain_matches_batched"] = bool(np.max(np.abs(a1 - a)) <= 1e-12 * np.max(np.abs(a)))
         log(f"plain: {wall1 * 1e3:.0f} ms, {plain:.0f} chain-levels/s, batched/plain {value / plain:.2f}x")
+
+    if not args.no_ed3:
+        # route ed3 on the same operator: 54 (spin, central species, start orbital) blocks, two chains each, depth 30
+        we = W.ternary_ed3(L=L, levels=levels, depth=depth, op=wl.op)
+        cf, oc = we.info["cf"], we.info["oc"]
+        G.set_depth_tol(0.0, "hull")
+        G.ed3_blocks(we.members, cf, oc, we.nlev)  # warm-up (autotuning)
+        res = {}
+        for mode, name in ((0, "batched"), (2, "graph"), (1, "plain")):
+            t0 = time.perf_counter()
+            bk, a3, b3, c3, mu3, ms3 = G.ed3_blocks(we.members, cf, oc, we.nlev, mode=mode)
+            wall = time.perf_counter() - t0
+            lev = int(mu3.sum())
+            res[name] = {"value": round(lev / wall, 1), "wall_ms": round(wall * 1e3, 1), "device_ms": round(ms3, 1)}
+        out["ed3"] = {"workload": "fcc ternary, route ed3 (full-ASA energy-dependent), 54 blocks x 2 chains x 30, "
+                                  "the edc operator", "nodes": wl.op.nn, "chains": 2 * len(we.members), "depth": depth,
+                      "unit": "chain-levels/s", **res, "member_tile": G.tuned_tile(2 * len(we.members), True),
+                      "speedup_batched_vs_plain": round(res["batched"]["value"] / res["plain"]["value"], 2)}
+        log(f"ed3: {out['ed3']}")
 
     if not args.no_l21:
         # uneven chains over four sublattice operators (L2_1 partial disorder + SRO) in one batched run
