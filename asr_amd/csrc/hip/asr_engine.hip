@@ -1271,6 +1271,71 @@ __global__ void __launch_bounds__(BLOCK) k_update(OpDev op, MemDev md, T* w, con
   }
 }
 
+// Full reorthogonalisation of the real chains (lanczos with reorth, src/reference/asr_recursion.f90:44-47):
+// w = w - Q (Q^T w) against the stored vectors q_0..q_{nq-1} of every member (Q: nq vectors of qstride values).
+// k_qdots: per block partials of <q_j, w> for every j (plane j of part); k_qfin: their fixed-order sums (one block per
+// member); k_qsub: w -= sum_j c_j q_j, optionally with the partial sums w w of the result.
+__global__ void __launch_bounds__(BLOCK) k_qdots(OpDev op, MemDev md, const double* w, const double* Q, size_t qstride, int nq,
+                                                 double* part, size_t plane) {
+  __shared__ double red[BLOCK];
+  if (md.tact && !md.tact[blockIdx.y]) return;
+  const int mt = md.mt, npw = WAVE / mt;
+  const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x >> 6;
+  const int m = blockIdx.y * mt + (lane & (mt - 1));
+  const int i = (xcd_block(blockIdx.x, gridDim.x) * (BLOCK / WAVE) + wave) * npw + lane / mt;
+  const bool live = i < op.nn && m < md.nb;
+  double x[9];
+  for (int l = 0; l < 9; ++l) x[l] = live ? w[((size_t)i * 9 + l) * md.nbp + m] : 0.0;
+  for (int j = 0; j < nq; ++j) {
+    double d = 0.0;
+    if (live) {
+      const double* q = Q + (size_t)j * qstride;
+      for (int l = 0; l < 9; ++l) d = fma(q[((size_t)i * 9 + l) * md.nbp + m], x[l], d);
+    }
+    block_reduce_member<double>(d, mt, red, part + (size_t)j * plane + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * mt +
+                                                threadIdx.x);
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_qfin(const double* part, size_t plane, int nbx, int mt, int nq, int nbp,
+                                                double* coef) {
+  __shared__ double lds[BLOCK / WAVE];
+  const int m = blockIdx.x;
+  for (int j = 0; j < nq; ++j) {
+    const double v = block_sum_partials<double>(part + (size_t)j * plane, nbx, mt, m, lds);
+    if (threadIdx.x == 0) coef[(size_t)j * nbp + m] = v;
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_qsub(OpDev op, MemDev md, double* w, const double* Q, size_t qstride, int nq,
+                                                const double* coef, double* partial) {
+  __shared__ double red[BLOCK];
+  if (md.tact && !md.tact[blockIdx.y]) return;
+  const int mt = md.mt, npw = WAVE / mt;
+  const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x >> 6;
+  const int m = blockIdx.y * mt + (lane & (mt - 1));
+  const int i = (xcd_block(blockIdx.x, gridDim.x) * (BLOCK / WAVE) + wave) * npw + lane / mt;
+  double dv = 0.0;
+  if (i < op.nn && m < md.nb) {
+    double t[9];
+    for (int l = 0; l < 9; ++l) t[l] = 0.0;
+    for (int j = 0; j < nq; ++j) {
+      const double c = coef[(size_t)j * md.nbp + m];
+      const double* q = Q + (size_t)j * qstride;
+      for (int l = 0; l < 9; ++l) t[l] = fma(q[((size_t)i * 9 + l) * md.nbp + m], c, t[l]);
+    }
+    for (int l = 0; l < 9; ++l) {
+      const size_t q = ((size_t)i * 9 + l) * md.nbp + m;
+      const double v = w[q] - t[l];
+      w[q] = v;
+      dv = fma(v, v, dv);
+    }
+  }
+  if (partial) block_reduce_member<double>(dv, mt, red, partial + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * mt + threadIdx.x);
+}
+
 // u <- w * inv_b (in place), members fastest
 template <class T>
 __global__ void k_scale(T* w, const T* inv_b, size_t n, int nbp) {
@@ -1609,6 +1674,54 @@ __global__ void __launch_bounds__(BLOCK) k_kpm_s16(RealDev R, int nvec, int nvp,
 }
 
 // mu_per[v][n] = (1/dim) sum_blocks partial, one block per vector, fixed order
+// Lanczos on the supercell (spectral_bounds): nvp columns of nrow rows, column v = chain v.  Column sums of x .* y per
+// block (rows strided over the grid, fixed order), summed on the host.
+__global__ void __launch_bounds__(BLOCK) k_col_dot(const double* __restrict__ x, const double* __restrict__ y, size_t nrow,
+                                                   int nvp, double* partial) {
+  __shared__ double red[BLOCK];
+  const int v = threadIdx.x % nvp, rpb = BLOCK / nvp;
+  double s = 0.0;
+  for (size_t r = (size_t)blockIdx.x * rpb + threadIdx.x / nvp; r < nrow; r += (size_t)gridDim.x * rpb)
+    s = fma(x[r * nvp + v], y[r * nvp + v], s);
+  red[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x < (unsigned)nvp) {
+    double t = 0.0;
+    for (int q = threadIdx.x; q < BLOCK; q += nvp) t += red[q];
+    partial[(size_t)blockIdx.x * nvp + threadIdx.x] = t;
+  }
+}
+
+// random start columns: u(row, v) = U[-1/2, 1/2) from splitmix64(((seed + v) << 40) ^ row) (columns v >= nvec zero)
+__global__ void k_rl_start(double* u, size_t nrow, int nvp, int nvec, uint64_t seed) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nrow * nvp) return;
+  const int v = (int)(q % nvp);
+  const size_t r = q / nvp;
+  u[q] = v < nvec ? (double)(splitmix64(((seed + (uint64_t)v) << 40) ^ r) >> 11) * 0x1.0p-53 - 0.5 : 0.0;
+}
+
+// w = w - a u - b up per column (coef: a then b, nvp each)
+__global__ void k_rl_axpy(double* w, const double* u, const double* up, const double* coef, size_t n, int nvp) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int v = (int)(q % nvp);
+  w[q] = w[q] - coef[v] * u[q] - coef[nvp + v] * up[q];
+}
+
+// up = u; u = w * s per column (s: nvp); w == nullptr: u = u * s
+__global__ void k_rl_shift(double* u, double* up, const double* w, const double* sc, size_t n, int nvp) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int v = (int)(q % nvp);
+  if (w) {
+    up[q] = u[q];
+    u[q] = w[q] * sc[v];
+  } else {
+    u[q] = u[q] * sc[v];
+  }
+}
+
 __global__ void __launch_bounds__(BLOCK) k_kpm_fin(const double* partial, int nbx, int vt, int nvec, double inv_dim,
                                                    double* mu_per, int nmom, int n) {
   __shared__ double lds[BLOCK / WAVE];
@@ -1635,6 +1748,7 @@ struct asr_gpu_op {
   int tile_pref = 0;                              // member tile of the batched chains, 0 = autotuned
   double depth_tol = 0.0;                         // adaptive depth of the edc chains (ed_depth_tol), 0 = off
   int depth_tail = 0;                             // tail of that test (ed_tail): 0 hull, 1 bp, 2 avg2, 3 none
+  bool reorth = false;                            // full reorthogonalisation of the real chains (lanczos reorth)
   std::vector<std::pair<long long, int>> tuned;   // (members, complex, uniform d, zero o) key -> member tile
 };
 
@@ -2101,11 +2215,12 @@ struct ChainBuffers {
   T *u = nullptr, *up = nullptr, *w = nullptr, *t1 = nullptr, *t2 = nullptr, *h1 = nullptr, *part = nullptr;
   T *a = nullptr, *b = nullptr, *a_eff = nullptr, *b_eff = nullptr, *inv_b = nullptr;
   T *tv = nullptr, *part2 = nullptr, *c = nullptr;  // route ed3: projection vector, its dot partials, c_k
+  double *qv = nullptr, *qpart = nullptr, *qcoef = nullptr;  // real chains with full reorthogonalisation
   int *level = nullptr, *depth = nullptr, *active = nullptr, *mused = nullptr, *closed = nullptr;
   ~ChainBuffers() {
     for (void* p : {(void*)u, (void*)up, (void*)w, (void*)t1, (void*)t2, (void*)h1, (void*)part, (void*)a, (void*)b,
                     (void*)a_eff, (void*)b_eff, (void*)inv_b, (void*)level, (void*)depth, (void*)active, (void*)mused,
-                    (void*)tv, (void*)part2, (void*)c, (void*)closed})
+                    (void*)tv, (void*)part2, (void*)c, (void*)closed, (void*)qv, (void*)qpart, (void*)qcoef})
       if (p) (void)hipFree(p);
   }
 };
@@ -2140,6 +2255,8 @@ struct ChainRun {
   std::vector<cplx> gprev, fprev;  // adaptive depth: G(z) (and route ed3's projection f) at the previous tested level
   bool ed3 = false;
   std::vector<double> snorm, tnorm;  // route ed3: norms of the start vector and of the projection vector per member
+  bool reorth = false;               // real chains: full reorthogonalisation against every stored Lanczos vector
+  size_t qplane = 0;
 
   ~ChainRun() {
     if (tact) (void)hipFree(tact);
@@ -2203,6 +2320,16 @@ struct ChainRun {
     if (ed3 && (dalloc(&B.tv, nv) || dalloc(&B.part2, npart) || dalloc(&B.c, (size_t)M.nbp * ldab) ||
                 dalloc(&B.closed, M.nbp)))
       return 1;
+    reorth = op->reorth && real_chain && std::is_same<T, double>::value;
+    if (reorth) {
+      // the stored Lanczos vectors of every member: maxdepth x nv doubles (lanczos keeps q(dim, nmax) per chain)
+      if ((double)maxdepth * nv * 8.0 > 64.0 * (1ull << 30))
+        return fail("lanczos reorth: the stored vectors would exceed 64 GiB (full reorthogonalisation is for small cases)");
+      qplane = npart;
+      if (dalloc(&B.qv, (size_t)maxdepth * nv) || dalloc(&B.qpart, (size_t)maxdepth * npart) ||
+          dalloc(&B.qcoef, (size_t)maxdepth * M.nbp))
+        return 1;
+    }
     for (T* q : {B.u, B.up, B.w, B.t1, B.t2, B.h1}) HCHECK(hipMemsetAsync(q, 0, nv * sizeof(T), stream));
     HCHECK(hipMemsetAsync(B.a, 0, (size_t)M.nbp * ldab * sizeof(T), stream));
     HCHECK(hipMemsetAsync(B.b, 0, (size_t)M.nbp * ldab * sizeof(T), stream));
@@ -2311,6 +2438,7 @@ struct ChainRun {
   int step_apply() {
     md.tact = tact;
     if (g_timer) g_timer->ord = 0;
+    if (reorth) HCHECK(hipMemcpyAsync(B.qv + (size_t)level_host * nv, B.u, nv * sizeof(double), hipMemcpyDeviceToDevice, stream));
     if (apply_level_op<T>(op, M, md, B.u, B.w, B.t1, B.t2, B.h1, lin, B.u, B.part, stream)) return 1;
     if (timer_mark(stream, KIND_FIN_A, true)) return 1;
     hipLaunchKernelGGL(k_fin_a<T>, dim3(M.nb), dim3(BLOCK), 0, stream, cd, M.nb, B.part, nbx, apply_mt, B.a, B.a_eff);
@@ -2376,6 +2504,18 @@ struct ChainRun {
     if (timer_mark(stream, KIND_UPDATE, true)) return 1;
     hipLaunchKernelGGL(k_update<T>, grid, dim3(BLOCK), 0, stream, dev_of(op), md, B.w, B.u, B.up, B.a_eff, B.b_eff, project,
                        B.part, (const T*)B.tv, B.part2);
+    if (reorth) {
+      // w = w - Q (Q^T w), twice (classical Gram-Schmidt against q_0..q_k); the second pass leaves the partials of w w
+      const int nq = level_host + 1;
+      for (int pass = 0; pass < 2; ++pass) {
+        hipLaunchKernelGGL(k_qdots, grid, dim3(BLOCK), 0, stream, dev_of(op), md, (const double*)B.w, (const double*)B.qv, nv,
+                           nq, B.qpart, qplane);
+        hipLaunchKernelGGL(k_qfin, dim3(M.nb), dim3(BLOCK), 0, stream, (const double*)B.qpart, qplane, nbx, md.mt, nq, M.nbp,
+                           B.qcoef);
+        hipLaunchKernelGGL(k_qsub, grid, dim3(BLOCK), 0, stream, dev_of(op), md, (double*)B.w, (const double*)B.qv, nv, nq,
+                           (const double*)B.qcoef, pass == 1 ? (double*)B.part : (double*)nullptr);
+      }
+    }
     if (timer_mark(stream, 0, false) || timer_mark(stream, KIND_FIN_B, true)) return 1;
     hipLaunchKernelGGL(k_fin_b<T>, dim3(M.nb), dim3(BLOCK), 0, stream, cd, M.nb, B.part, nbx, md.mt, B.b, B.b_eff,
                        B.inv_b, (const T*)B.part2);
@@ -2983,6 +3123,13 @@ int asr_gpu_op_set_tile(asr_gpu_op* op, int mt) {
 }
 
 // The member tile the autotuner picked for nchain members (0 if no batched run of that size has happened).
+// Full reorthogonalisation of the real chains (lanczos reorth = .true., src/reference/asr_recursion.f90:17-58).
+int asr_gpu_op_set_reorth(asr_gpu_op* op, int reorth) {
+  if (!op) return fail("asr_gpu_op_set_reorth: null operator");
+  op->reorth = reorth != 0;
+  return 0;
+}
+
 // Adaptive depth of the edc chains (ed_depth_tol / ed_tail of edc_green, src/asr_edc.f90:121-127): 0 = fixed depth.
 int asr_gpu_op_set_depth_tol(asr_gpu_op* op, double depth_tol, int tail) {
   if (!(depth_tol >= 0.0) || tail < 0 || tail > 3) return fail("asr_gpu_op_set_depth_tol: depth_tol >= 0, tail 0..3");
@@ -3396,6 +3543,96 @@ int asr_gpu_kpm_moments(asr_gpu_real* R, double c, double s, int nmom, int nvec,
   if (total_ms) *total_ms = ms_all;
   for (double* p : {t0, t1, t2, part, mper}) (void)hipFree(p);
   for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+  return 0;
+}
+
+// Extremal eigenvalues of h (spectral_bounds, src/reference/asr_kpm.f90:14-36): nstep Lanczos steps from nvec random
+// start vectors at once (column v starts from splitmix64(((seed + v) << 40) ^ row), uniform in [-1/2, 1/2)); the
+// eigenvalues of each chain's tridiagonal matrix (the role of dsterf) give emin = min - margin, emax = max + margin over
+// the chains.  a, b: nvec x nstep or NULL (b(i) = |w_i| of every step, as the reference keeps it).
+int asr_gpu_spectral_bounds(asr_gpu_real* R, int nstep, int nvec, uint64_t seed, double margin, double* emin, double* emax,
+                            double* a_out, double* b_out) {
+  if (!R || nstep < 1 || nvec < 1 || nvec > 64 || !emin || !emax) return fail("asr_gpu_spectral_bounds: arguments");
+  const size_t nrow = (size_t)R->nsite * 9;
+  int vt = 8;
+  while (vt < nvec) vt <<= 1;
+  const int nvp = vt;
+  const size_t n = nrow * nvp;
+  double *u = nullptr, *up = nullptr, *w = nullptr, *part = nullptr, *coef = nullptr;
+  const int nbd = 1024;
+  if (dalloc(&u, n) || dalloc(&up, n) || dalloc(&w, n) || dalloc(&part, (size_t)nbd * nvp) ||
+      dalloc(&coef, 2 * nvp))
+    return 1;
+  auto cleanup = [&]() {
+    for (double* p : {u, up, w, part, coef}) (void)hipFree(p);
+  };
+  std::vector<double> ph((size_t)nbd * nvp), av((size_t)nvec * nstep), bv((size_t)nvec * nstep), cf(2 * nvp, 0.0);
+  const dim3 eg((unsigned)((n + 255) / 256));
+  // per-column sums of x .* y on the host, fixed order
+  auto coldot = [&](const double* x, const double* y, std::vector<double>& out) -> int {
+    hipLaunchKernelGGL(k_col_dot, dim3(nbd), dim3(BLOCK), 0, R->stream, x, y, nrow, nvp, part);
+    if (hipMemcpyAsync(ph.data(), part, ph.size() * 8, hipMemcpyDeviceToHost, R->stream) != hipSuccess ||
+        hipStreamSynchronize(R->stream) != hipSuccess)
+      return fail("spectral_bounds: reduction");
+    out.assign(nvp, 0.0);
+    for (int bq = 0; bq < nbd; ++bq)
+      for (int v = 0; v < nvp; ++v) out[v] += ph[(size_t)bq * nvp + v];
+    return 0;
+  };
+  auto upload = [&](const std::vector<double>& h) -> int {
+    return hipMemcpyAsync(coef, h.data(), h.size() * 8, hipMemcpyHostToDevice, R->stream) != hipSuccess;
+  };
+  std::vector<double> dd;
+  int st = 0;
+  hipLaunchKernelGGL(k_rl_start, eg, dim3(256), 0, R->stream, u, nrow, nvp, nvec, seed);
+  st |= hipMemsetAsync(up, 0, n * 8, R->stream) != hipSuccess;
+  st |= hipMemsetAsync(w, 0, n * 8, R->stream) != hipSuccess;  // the step writes the columns v < nvec only
+  if (!st) st = coldot(u, u, dd);
+  if (!st) {
+    for (int v = 0; v < nvp; ++v) cf[v] = v < nvec ? 1.0 / std::sqrt(dd[v]) : 0.0;
+    st = upload(cf);
+  }
+  if (!st) hipLaunchKernelGGL(k_rl_shift, eg, dim3(256), 0, R->stream, u, up, (const double*)nullptr, (const double*)coef, n, nvp);
+  std::vector<double> bn(nvp, 0.0);
+  for (int i = 0; i < nstep && !st; ++i) {
+    st = launch_kpm_step(R, vt, kpm_grid(R, vt, nvp), nvec, nvp, u, nullptr, w, 0.0, 1.0, 0.0, 0ull, nullptr, 0);
+    if (!st) st = coldot(u, w, dd);
+    if (st) break;
+    for (int v = 0; v < nvp; ++v) {
+      cf[v] = v < nvec ? dd[v] : 0.0;
+      cf[nvp + v] = v < nvec ? bn[v] : 0.0;
+      if (v < nvec) av[(size_t)v * nstep + i] = dd[v];
+    }
+    st = upload(cf);
+    if (st) break;
+    hipLaunchKernelGGL(k_rl_axpy, eg, dim3(256), 0, R->stream, w, (const double*)u, (const double*)up, (const double*)coef, n, nvp);
+    st = coldot(w, w, dd);
+    if (st) break;
+    for (int v = 0; v < nvp; ++v) {
+      bn[v] = v < nvec ? std::sqrt(dd[v]) : 0.0;
+      if (v < nvec) bv[(size_t)v * nstep + i] = bn[v];
+      cf[v] = v < nvec ? 1.0 / std::max(bn[v], 1.0e-300) : 0.0;
+    }
+    st = upload(cf);
+    if (st) break;
+    hipLaunchKernelGGL(k_rl_shift, eg, dim3(256), 0, R->stream, u, up, (const double*)w, (const double*)coef, n, nvp);
+  }
+  if (!st) st = hipGetLastError() != hipSuccess || hipStreamSynchronize(R->stream) != hipSuccess;
+  cleanup();
+  if (st) return fail("asr_gpu_spectral_bounds: a launch failed");
+  double lo = 1e300, hi = -1e300;
+  for (int v = 0; v < nvec; ++v) {
+    std::vector<double> d(av.begin() + (size_t)v * nstep, av.begin() + (size_t)(v + 1) * nstep);
+    std::vector<double> e(nstep, 0.0);
+    for (int i = 0; i + 1 < nstep; ++i) e[i] = bv[(size_t)v * nstep + i];
+    if (asr_host_tridiag_eigvals(nstep, d.data(), e.data())) return fail("asr_gpu_spectral_bounds: tridiagonal eigenvalues");
+    lo = std::min(lo, d[0]);
+    hi = std::max(hi, d[nstep - 1]);
+  }
+  *emin = lo - margin;
+  *emax = hi + margin;
+  if (a_out) std::memcpy(a_out, av.data(), av.size() * 8);
+  if (b_out) std::memcpy(b_out, bv.data(), bv.size() * 8);
   return 0;
 }
 

@@ -39,6 +39,7 @@ def lib() -> C.CDLL:
         L.asr_gpu_op_set_classes.argtypes = [P, P, P, P]
         L.asr_gpu_op_set_tile.argtypes = [P, C.c_int]
         L.asr_gpu_op_set_depth_tol.argtypes = [P, C.c_double, C.c_int]
+        L.asr_gpu_op_set_reorth.argtypes = [P, C.c_int]
         L.asr_gpu_op_get_tile.argtypes = [P, C.c_int, C.c_int, P]
         L.asr_gpu_real_set_tile.argtypes = [P, C.c_int]
         L.asr_gpu_op_destroy.argtypes = [P]
@@ -56,6 +57,7 @@ def lib() -> C.CDLL:
         L.asr_gpu_real_destroy.argtypes = [P]
         L.asr_gpu_real_destroy.restype = None
         L.asr_gpu_real_apply.argtypes = [P, C.c_int, P, P]
+        L.asr_gpu_spectral_bounds.argtypes = [P, C.c_int, C.c_int, C.c_uint64, C.c_double, P, P, P, P]
         L.asr_gpu_kpm_moments.argtypes = [P, C.c_double, C.c_double, C.c_int, C.c_int, C.c_uint64, P, P, P, P]
         _LIB = L
     return _LIB
@@ -139,6 +141,10 @@ class GpuOperator:
     def set_tile(self, mt: int):
         """Member tile of the batched chains (power of two <= 64; 0 = autotuned)."""
         _check(lib().asr_gpu_op_set_tile(self.h, int(mt)), "asr_gpu_op_set_tile")
+
+    def set_reorth(self, reorth: bool):
+        """Full reorthogonalisation of the real chains (lanczos reorth = .true.; small cases)."""
+        _check(lib().asr_gpu_op_set_reorth(self.h, int(bool(reorth))), "asr_gpu_op_set_reorth")
 
     def set_depth_tol(self, depth_tol: float, tail: str = "hull"):
         """Adaptive depth of the edc chains (ed_depth_tol / ed_tail of edc_green); 0 = every chain to its depth."""
@@ -337,6 +343,16 @@ class GpuRealization:
         y = np.zeros_like(x)
         _check(lib().asr_gpu_real_apply(self.h, x.shape[0], _p(x), _p(y)), "asr_gpu_real_apply")
         return y
+
+    def spectral_bounds(self, nstep: int = 80, margin: float = 0.0, nvec: int = 1, seed: int = 7):
+        """(emin, emax, a, b): extremal eigenvalues of h from nstep Lanczos steps of nvec random vectors, widened by
+        margin (spectral_bounds); a, b: (nvec, nstep) chain coefficients."""
+        lo, hi = np.zeros(1), np.zeros(1)
+        a = np.zeros((nvec, nstep))
+        b = np.zeros((nvec, nstep))
+        _check(lib().asr_gpu_spectral_bounds(self.h, int(nstep), int(nvec), int(seed), float(margin), _p(lo), _p(hi), _p(a),
+                                             _p(b)), "asr_gpu_spectral_bounds")
+        return float(lo[0]), float(hi[0]), a, b
 
     def kpm_moments(self, c: float, s: float, nmom: int, nvec: int, seed: int, per_vector: bool = False):
         mu = np.zeros(nmom)

@@ -277,11 +277,14 @@ def main():
         Lk = 6 if args.quick else 40
         nmom = 64 if args.quick else args.kpm_moments
         g, species, par, block, block0 = W.kpm_supercell(L=Lk)
-        # spectral bound of h = c + d S d (Gershgorin over the 9 x 9 blocks), scaled into [-0.99, 0.99]
-        sabs = np.abs(block).sum(axis=0).sum(axis=1) + np.abs(block0).sum(axis=1)
-        hb = float(np.max(np.abs(par[0]).max(axis=0) + par[1].max() * par[1].max() * sabs.max()))
-        cshift, sscale = 0.0, hb / 0.99
         R = engine.GpuRealization(g.nbr, species, par, block, block0)
+        # spectral bounds of h = c + d S d from 80 Lanczos steps (spectral_bounds), widened by 1 % of the width on each
+        # side, and the spectrum scaled into [-0.99, 0.99]
+        t0 = time.perf_counter()
+        elo, ehi, _, _ = R.spectral_bounds(80, margin=0.0, nvec=1, seed=3)
+        tb_ms = (time.perf_counter() - t0) * 1e3
+        marg = 0.01 * (ehi - elo)
+        cshift, sscale = 0.5 * (elo + ehi), 0.5 * (ehi - elo + 2.0 * marg) / 0.99
         tiles = {}
         for vt in (1, 2, 3):  # one or two vectors per lane on 64-vector rows, or 16-vector slices; tuned on short runs
             R.set_tile(vt)
@@ -300,6 +303,8 @@ def main():
         out["kpm"] = {"value": round(64 * nmom / wk, 1), "unit": "vector-moments/s", "rows": g.n * 9, "nvec": 64,
                       "nmom": nmom, "wall_s": round(wk, 3), "step_ms": round(res["step_ms"], 4), "roofline": kroof,
                       "vector_tile": vt_best,
+                      "bounds": {"emin": round(elo - marg, 6), "emax": round(ehi + marg, 6), "lanczos_steps": 80,
+                                 "ms": round(tb_ms, 1)},
                       "mu_head": [round(float(v), 8) for v in res["mu"][:4]]}
         log(f"kpm: {nmom} moments x 64 vectors on {g.n * 9} rows in {wk:.2f} s, step {res['step_ms']:.3f} ms, {kroof}")
         R.close()

@@ -51,6 +51,10 @@ int asr_gpu_op_set_tile(asr_gpu_op* op, int mt);
  * depth_tol = 0 (the default): every chain runs to its depth.  With depth_tol > 0 the levels are issued from the host
  * (one synchronisation per tested level; asr_gpu_chains_edc_multi mode 2 then runs as mode 0). */
 int asr_gpu_op_set_depth_tol(asr_gpu_op* op, double depth_tol, int tail);
+/* Full reorthogonalisation of the real chains (lanczos with reorth = .true., src/reference/asr_recursion.f90:17-58): every
+ * level projects the new vector twice against all stored Lanczos vectors of its chain, w = w - Q (Q^T w), before b_n is
+ * taken.  The stored vectors cost depth x (nn + 1) x 9 x members doubles (at most 64 GiB): for tests and small cases. */
+int asr_gpu_op_set_reorth(asr_gpu_op* op, int reorth);
 /* The tile the autotuner picked for nchain members of the given kind (0: none tuned yet). */
 int asr_gpu_op_get_tile(asr_gpu_op* op, int nchain, int is_complex, int* mt);
 void asr_gpu_op_destroy(asr_gpu_op* op);
@@ -147,6 +151,12 @@ int asr_gpu_real_apply(asr_gpu_real* r, int nvec, const double* x, double* y);
  * mu: nmom; mu_per: nvec x nmom or NULL; step_ms: mean duration of one Chebyshev step; total_ms: whole recurrence. */
 int asr_gpu_kpm_moments(asr_gpu_real* r, double c, double s, int nmom, int nvec, uint64_t seed, double* mu, double* mu_per,
                         double* step_ms, double* total_ms);
+/* Extremal eigenvalues of h = c + d S d for the KPM scaling (spectral_bounds, src/reference/asr_kpm.f90:14-36): nstep
+ * Lanczos steps from nvec (<= 64) random start vectors at once (vector v: uniform [-1/2, 1/2) entries from
+ * splitmix64(((seed + v) << 40) ^ row)), the eigenvalues of each chain's tridiagonal matrix (the role of dsterf);
+ * emin = smallest - margin, emax = largest + margin over the chains.  a, b: nvec x nstep coefficients or NULL. */
+int asr_gpu_spectral_bounds(asr_gpu_real* r, int nstep, int nvec, uint64_t seed, double margin, double* emin, double* emax,
+                            double* a, double* b);
 
 #ifdef __cplusplus
 }

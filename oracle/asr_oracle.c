@@ -8,7 +8,7 @@
  *   orc_augcls_apply     src/asr_augcls.f90:275-297   augcls_apply
  *   orc_rand_c           src/asr_augcls.f90:419-446   augcls_rand_c
  *   orc_project_c        src/asr_augcls.f90:329-348   augcls_project_c
- *   orc_lanczos          src/reference/asr_recursion.f90:17-58   lanczos (no reorthogonalisation)
+ *   orc_lanczos(_ro)     src/reference/asr_recursion.f90:17-58   lanczos (reorth = .false. / .true.)
  *   orc_edc_chain        src/asr_edc.f90:75-164       edc_green without the adaptive depth (depth_tol = 0): a, b, m
  *   orc_hp_c             src/asr_augcls.f90:301-320   augcls_hp_c
  *   orc_ed3_kmul         src/asr_ed3.f90:99-129       ed3_kmul
@@ -213,21 +213,34 @@ static double dot(const double *a, const double *b, size_t n) {
   return s;
 }
 
-/* Real Lanczos from the unit vector at component iv (lanczos, src/reference/asr_recursion.f90:17-58).
- * a(0..n-1), b(0..n-1): b(n-1) is the norm of the last residual when n == nmax.  Returns n. */
-int orc_lanczos(const orc_op *op, const orc_member *m, int iv, int nmax, double tiny, double *a, double *b) {
+/* Real Lanczos from the unit vector at component iv (lanczos, src/reference/asr_recursion.f90:17-58); reorth != 0: full
+ * reorthogonalisation, w = w - Q (Q^T w) twice against every stored vector q_1..q_i (classical Gram-Schmidt, as the
+ * reference's matmul form).  a(0..n-1), b(0..n-1): b(n-1) is the norm of the last residual when n == nmax.  Returns n. */
+int orc_lanczos_ro(const orc_op *op, const orc_member *m, int iv, int nmax, double tiny, int reorth, double *a, double *b) {
   const size_t dim = (size_t)9 * op->nn;
   double *btab = make_btab(op);
   double *u = (double *)calloc(dim, sizeof(double)), *up = (double *)calloc(dim, sizeof(double));
   double *w = (double *)malloc(sizeof(double) * dim), *wk = (double *)malloc(sizeof(double) * 8 * dim);
+  double *qs = reorth ? (double *)malloc(sizeof(double) * dim * nmax) : NULL;
+  double *cq = reorth ? (double *)malloc(sizeof(double) * nmax) : NULL;
   for (int i = 0; i < nmax; ++i) { a[i] = 0.0; b[i] = 0.0; }
   u[iv] = 1.0;
   double bn = 0.0;
   int n = 0;
   for (int i = 0; i < nmax; ++i) {
+    if (reorth) memcpy(qs + (size_t)i * dim, u, sizeof(double) * dim);
     orc_augcls_apply(op, m, btab, u, w, wk);
     a[i] = dot(u, w, dim);
     for (size_t q = 0; q < dim; ++q) w[q] = w[q] - a[i] * u[q] - bn * up[q];
+    if (reorth)
+      for (int pass = 0; pass < 2; ++pass) {
+        for (int j = 0; j <= i; ++j) cq[j] = dot(qs + (size_t)j * dim, w, dim);
+        for (size_t q = 0; q < dim; ++q) {
+          double t = 0.0;
+          for (int j = 0; j <= i; ++j) t += qs[(size_t)j * dim + q] * cq[j];
+          w[q] -= t;
+        }
+      }
     n = i + 1;
     bn = sqrt(dot(w, w, dim));
     if (i == nmax - 1) break;
@@ -237,7 +250,12 @@ int orc_lanczos(const orc_op *op, const orc_member *m, int iv, int nmax, double 
   }
   if (n == nmax) b[nmax - 1] = bn;
   free(u); free(up); free(w); free(wk); free(btab);
+  free(qs); free(cq);
   return n;
+}
+
+int orc_lanczos(const orc_op *op, const orc_member *m, int iv, int nmax, double tiny, double *a, double *b) {
+  return orc_lanczos_ro(op, m, iv, nmax, tiny, 0, a, b);
 }
 
 /* Complex-symmetric chain of route edc at one energy (edc_green, src/asr_edc.f90:75-164) with depth_tol = 0:

@@ -39,6 +39,7 @@ def lib():
         L.orc_rand_c.argtypes = [P, P, P, P, P]
         L.orc_project_c.argtypes = [P, P, P]
         L.orc_lanczos.argtypes = [P, P, C.c_int, C.c_int, C.c_double, P, P]
+        L.orc_lanczos_ro.argtypes = [P, P, C.c_int, C.c_int, C.c_double, C.c_int, P, P]
         L.orc_edc_chain.argtypes = [P, P, P, C.c_int, C.c_int, C.c_int, P, P]
         L.orc_ed3_apply.argtypes = [P, P, P, P, P, P, P]
         L.orc_ed3_start.argtypes = [P, P, C.c_int, C.c_double, C.c_int, P, P, P, P]
@@ -122,11 +123,11 @@ class OracleOp:
         lib().orc_project_c(C.byref(self.s), C.byref(ms), _p(x))
         return x
 
-    def lanczos(self, m, iv: int, nmax: int, tiny: float = 1.0e-12):
+    def lanczos(self, m, iv: int, nmax: int, tiny: float = 1.0e-12, reorth: bool = False):
         a = np.zeros(nmax)
         b = np.zeros(nmax)
         ms = self.member(m)
-        n = lib().orc_lanczos(C.byref(self.s), C.byref(ms), int(iv), int(nmax), float(tiny), _p(a), _p(b))
+        n = lib().orc_lanczos_ro(C.byref(self.s), C.byref(ms), int(iv), int(nmax), float(tiny), int(reorth), _p(a), _p(b))
         return a, b, n
 
     def edc_chain(self, m, ep: np.ndarray, iv: int, nlev: int, project: bool = True):

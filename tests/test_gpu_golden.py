@@ -72,3 +72,31 @@ def test_gpu_dos_of_plain_binary_matches_oracle():
         nint = [np.concatenate([[0.0], np.cumsum(0.5 * (r[1:] + r[:-1]) * np.diff(e))]) for r in rho]
         assert np.max(np.abs(nint[0] - nint[1])) < 1e-6, j
         assert 0.5 < nint[1][-1] <= 1.0 + 1e-9
+
+
+def test_gpu_spectral_bounds():
+    """spectral_bounds (src/reference/asr_kpm.f90:14-36) on the GPU: on the 12^3 sc band the Ritz extremes of 80 Lanczos
+    steps are the exact band edges -6 and 6 (interlacing keeps them inside, convergence to 1e-10); on a random ternary fcc
+    supercell the coefficients of each start vector are the oracle's (vector v = the oracle's seed + v) and the bounds are
+    the extremes of the oracle's tridiagonal matrices."""
+    eng = _engine()
+    from oracle.oracle import OracleReal
+    g, spec, par, block, block0 = _sc_band(12)
+    R = eng.GpuRealization(g.nbr, spec, par, block, block0)
+    lo, hi, a, b = R.spectral_bounds(80, margin=0.05, nvec=2, seed=11)
+    assert abs(lo + 0.05 + 6.0) < 1e-10 and abs(hi - 0.05 - 6.0) < 1e-10
+    R.close()
+    g, species, par, block, block0 = W.kpm_supercell(L=8, seed=6)
+    R = eng.GpuRealization(g.nbr, species, par, block, block0)
+    lo, hi, a, b = R.spectral_bounds(80, margin=0.0, nvec=3, seed=11)
+    O = OracleReal(g.nbr, species, par[0], par[1], par[2], par[3], block, block0)
+    los, his = [], []
+    for v in range(3):
+        ao, bo = O.lanczos(80, 11 + v)
+        assert np.max(np.abs(a[v, :30] - ao[:30])) < 1e-9 * np.max(np.abs(ao[:30]))
+        assert np.max(np.abs(b[v, :30] - bo[:30])) < 1e-9 * np.max(np.abs(bo[:30]))
+        ev = host.tridiag_eigvals(ao, bo[:79])
+        los.append(ev[0])
+        his.append(ev[-1])
+    assert abs(lo - min(los)) < 1e-10 and abs(hi - max(his)) < 1e-10
+    R.close()

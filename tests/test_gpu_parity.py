@@ -270,3 +270,38 @@ def test_edc_adaptive_depth(mode):
         assert rel(b[j, :k - 1], br[:k - 1]) < 1e-9
         assert np.all(a[j, k:] == 0.0)
     assert np.any(mu == 12) and np.any(mu > 12)
+
+
+def test_lanczos_full_reorthogonalisation():
+    """lanczos with reorth = .true. on the GPU: the chains of the plain binary workload to depth 40 equal the oracle's
+    reorthogonalised chains (1e-9); on the ring the reorthogonalised chain closes after dim steps with the exact spectrum
+    of the dense operator (multiplicities included)."""
+    eng = _engine()
+    from asr_amd import host
+    from exact import build_ring_operator, dense_operator, ring_model
+    from test_oracle_exact import rotated
+    wl = W.plain_binary(L=4, levels=2, depth=40, seed=11)
+    G = eng.GpuOperator(wl.op)
+    G.set_reorth(True)
+    a, b, mu, _ = G.lanczos(wl.members, 40)
+    O = _oracle(wl.op)
+    for j, m in enumerate(wl.members):
+        ao, bo, n = O.lanczos(m, m.l0, 40, reorth=True)
+        assert mu[j] == n
+        assert rel(a[j, :n], ao[:n]) < 1e-9 and rel(b[j, :n], bo[:n]) < 1e-9, j
+    G.close()
+    model, par = ring_model()
+    model = rotated(model, 2)
+    op = build_ring_operator(model, frozen_start=True)
+    m = A.Member(l0=4, c=par["c"], d=par["d"], o=par["o"], e=par["e"], frozen=0)
+    O = _oracle(op)
+    h = dense_operator(lambda x: O.apply(m, x), op.dim)
+    ev = np.linalg.eigvalsh(0.5 * (h + h.T))
+    G = eng.GpuOperator(op)
+    G.set_reorth(True)
+    a, b, mu, _ = G.lanczos([m], op.dim + 8, tiny=1.0e-9)
+    n = int(mu[0])
+    assert n == op.dim and b[0, n - 1] < 1.0e-9
+    ritz = host.tridiag_eigvals(a[0, :n], b[0, :n - 1])
+    assert np.max(np.abs(np.sort(ritz) - ev)) < 1.0e-9
+    G.close()

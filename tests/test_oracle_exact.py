@@ -191,3 +191,29 @@ def test_ed3_mutation_is_detected():
     bk, _, _ = O.ed3_blocks(m, et, uu, -qt, E3.ed3_oc(ed, 2), z, 2, 1500, project=True, tail="none")
     exact = np.array(exact_average_asa(model, orb, z, 2, 1))
     assert np.max(np.abs(bk[:3] - exact)) > 1.0e-4
+
+
+def test_lanczos_full_reorthogonalisation():
+    """lanczos with reorth = .true. (src/reference/asr_recursion.f90:17-58): with every stored vector projected out twice
+    the chain spans the whole space here (rounding splits the degenerate pairs of the ring's reflection) and closes
+    (b_n < tiny) after dim steps; the chain matrix then has exactly the spectrum of the dense operator, multiplicities
+    included, which a chain without reorthogonalisation (ghost copies) does not; the Green function is the exact
+    configuration average to 1e-10."""
+    model, par = ring_model()
+    model = rotated(model, 2)
+    op = build_ring_operator(model, frozen_start=True)
+    O = OracleOp(op)
+    m = A.Member(l0=4, c=par["c"], d=par["d"], o=par["o"], e=par["e"], frozen=0)
+    h = dense_operator(lambda x: O.apply(m, x), op.dim)
+    ev = np.linalg.eigvalsh(0.5 * (h + h.T))
+    a, b, n = O.lanczos(m, 4, op.dim + 8, tiny=1.0e-9, reorth=True)
+    assert n == op.dim and b[n - 1] < 1.0e-9
+    ritz = host.tridiag_eigvals(a[:n], b[:n - 1])
+    assert np.max(np.abs(np.sort(ritz) - ev)) < 1.0e-9
+    a0, b0, n0 = O.lanczos(m, 4, op.dim, tiny=1.0e-9)
+    ritz0 = host.tridiag_eigvals(a0[:n0], b0[:n0 - 1])
+    assert np.max(np.abs(np.sort(ritz0) - ev)) > 1.0e-3
+    for e in (-0.5, 0.2):
+        z = complex(e, 0.3)
+        g = host.terminated_green(a, b, n, 0.0, 0.0, z)[0]
+        assert abs(g - exact_average_g(model, par, z, 4, 0)) < 1.0e-10
