@@ -51,7 +51,7 @@ class AugclsOperator:
     conc: np.ndarray      # (ncls, nsmax)
     mt: np.ndarray        # (ncls, nsmax, nsmax, nsmax): mt[kap, l, k, s]
     start_site: int
-    fz: np.ndarray        # (nn,) uint8: node sits on the start site (a frozen species applies there)
+    fz: np.ndarray        # (nn,) uint8: 1 node on the start site, 2 node on a frozen pair partner, 0 elsewhere
     meta: dict = field(default_factory=dict)
 
     @property
@@ -74,8 +74,11 @@ class AugclsOperator:
 def build_operator(graph: structure.Graph, cls: np.ndarray, nsc, spc, conc, perm: np.ndarray, dmat: np.ndarray, start: int,
                    maxlevel: int, maxrank: int = 0, flippable: np.ndarray | None = None, stype: np.ndarray | None = None,
                    block: np.ndarray | None = None, block0: np.ndarray | None = None, nsp: int | None = None,
-                   nthreads: int = 0) -> AugclsOperator:
-    """Node set + tables (build_augcls_operator) and the orbital / class tables of the operator."""
+                   nthreads: int = 0, partner: int | None = None) -> AugclsOperator:
+    """Node set + tables (build_augcls_operator) and the orbital / class tables of the operator.  partner: a second
+    non-flippable site whose nodes take a frozen species of their own (the partner of an EPI pair, augcls_freeze_any)."""
+    if partner is not None and (flippable is None or flippable[partner] or partner == start):
+        raise ValueError("build_augcls_operator: the frozen partner must be a non-flippable site other than the start")
     t = host.build_tables(graph.nbr, cls, perm, start, maxlevel, nsc, maxrank=maxrank, flippable=flippable, nthreads=nthreads)
     nn = t["lst"].size
     nslot = graph.nbr.shape[1]
@@ -116,7 +119,7 @@ def build_operator(graph: structure.Graph, cls: np.ndarray, nsc, spc, conc, perm
         pi=pi, sg=sg, dmat=np.ascontiguousarray(dmat, dtype=np.float64),
         block=np.ascontiguousarray(block, dtype=np.float64), block0=np.ascontiguousarray(block0, dtype=np.float64),
         nsc=np.asarray(nsc, np.int32), spc=spc_a, conc=conc_a, mt=mt, start_site=int(start),
-        fz=(site == start).astype(np.uint8))
+        fz=(np.where(site == start, 1, 0) + (np.where(site == partner, 2, 0) if partner is not None else 0)).astype(np.uint8))
 
 
 def reorder_nodes(op: AugclsOperator) -> AugclsOperator:
@@ -196,6 +199,8 @@ class Member:
     pb: np.ndarray | None = None
     z: complex = 0.0
     depth: int = 0
+    frozen2: int = -1              # species of the frozen pair partner (fz = 2 nodes), -1: none
+    chi: np.ndarray | None = None  # characters of a symmetry-adapted start vector (nop,), None: those of orbital l0
 
     def mt(self, op: AugclsOperator) -> np.ndarray:
         if self.conc is None:

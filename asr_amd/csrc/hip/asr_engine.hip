@@ -76,7 +76,7 @@ struct OpDev {
   int nn, nslot, nop, nsmax, ncls, nsp, ntype, nstab;
   const uint32_t* hop;   // nn x nslot: (node + 1) << 6 | op, 0 = no link
   const uint32_t* flip;  // nn x nsmax, same packing
-  const uint32_t* info;  // nn: lst | ncl << 4 | ntyp << 12 | fz << 20
+  const uint32_t* info;  // nn: lst | ncl << 4 | ntyp << 12 | fz << 20 (fz: frozen-site slot 0 none, 1 start, 2 partner)
   const double* sq;      // nn + 1 (sq[nn] = 1: the padding node)
   const double* isq;     // nn + 1: 1 / sq
   const uint32_t* sgm;   // nop: bit l set when sg(l, g) < 0
@@ -94,16 +94,25 @@ struct MemDev {
   int nb, nbp, mt;            // members, padded stride, member tile
   const uint64_t* chimask;    // nbp: bit g set <=> chi(g) = -1
   const double* wts;          // NQ x ncls x nsmax x nsmax x 9 x nbp
-  const double* fpar;         // NQ x 9 x nbp  (parameters of the frozen species)
-  const int* frozen;          // nbp: 1 = the start site is frozen
+  const double* fpar;         // 2 x NQ x 9 x nbp: parameters of the frozen species of slot 1 (start), slot 2 (partner)
   const double* dvec;         // 9 x nbp: d per orbital (uniform-d members)
   const cplx* z;              // nbp
   const double* dperm;        // nop x 9 x nbp: d at the permuted orbital, d[pi(l, g)]
   int pair;                   // complex members as lane pairs (member tile 64 only)
   const int* tact;            // per member tile: 0 = every member has left (the tile's blocks return at once), or null
   const double* mtm;          // ncls x nsmax x nsmax x nsmax x nbp: class tables per member
-  const int* fsp;             // nbp: species of the frozen start site (-1: none)
+  const int* fsp;             // 2 x nbp: species of the frozen start site and of the frozen partner site (-1: none)
 };
+
+// The frozen-site slot of node info inf for member m: 1 (the start site), 2 (a frozen pair partner, augcls_freeze_any)
+// when the member freezes that site, else 0.
+__device__ inline int frozen_slot(uint32_t inf, const MemDev& md, int m) {
+  const int s = (int)((inf >> 20) & 3u);
+  return (s && md.fsp[(size_t)(s - 1) * md.nbp + m] >= 0) ? s : 0;
+}
+__device__ inline const double* frozen_row(const MemDev& md, int slot, int q) {
+  return md.fpar + ((size_t)(slot - 1) * NQ + q) * 9 * md.nbp;
+}
 
 __device__ inline void unpack_link(uint32_t e, int& r, int& g) {
   r = (int)(e >> 6) - 1;
@@ -181,10 +190,10 @@ __device__ void rand_accum(const OpDev& op, const MemDev& md, const int* __restr
                            uint64_t chim, int i, uint32_t inf, int m, int q, const T* __restrict__ x, int xnorm, double coef,
                            T acc[9]) {
   const int nbp = md.nbp, nsm = op.nsmax;
-  const bool fz = (inf >> 20) & 1u;
+  const int fs = frozen_slot(inf, md, m);
   const double isi = op.isq[i];
-  if (fz && md.frozen[m]) {
-    const double* fp = md.fpar + (size_t)q * 9 * nbp;
+  if (fs) {
+    const double* fp = frozen_row(md, fs, q);
     for (int l = 0; l < 9; ++l) {
       T xv = ld(x, i, l, nbp, m);
       if (xnorm) xv = isi * xv;
@@ -368,7 +377,7 @@ __device__ __forceinline__ void rand_ct(const OpDev& op, const MemDev& md, const
                                         int m, uint32_t node16, uint32_t vs16, uint32_t coff, __amdgpu_buffer_rsrc_t xr,
                                         int xnorm, int q, double coef, T acc[9]) {
   const int nbp = md.nbp, nsm = op.nsmax;
-  const bool fz = ((inf >> 20) & 1u) && md.frozen[m];
+  const int fs = frozen_slot(inf, md, m);
   const double isi = xnorm ? op.isq[i] : 1.0;
   const int qa = q, qb = LIN ? q + 1 : q;
   auto row = [&](uint32_t voff, int l) -> T { return tile_row<T>(xr, voff, l * vs16); };
@@ -383,9 +392,9 @@ __device__ __forceinline__ void rand_ct(const OpDev& op, const MemDev& md, const
     }
   };
   const uint32_t own = (uint32_t)i * node16 + coff;
-  if (fz) {
-    const double* fa_ = md.fpar + (size_t)qa * 9 * nbp + m;
-    const double* fb_ = md.fpar + (size_t)qb * 9 * nbp + m;
+  if (fs) {
+    const double* fa_ = frozen_row(md, fs, qa) + m;
+    const double* fb_ = frozen_row(md, fs, qb) + m;
 #pragma unroll
     for (int l = 0; l < 9; ++l) add(fa_[(size_t)l * nbp], LIN ? fb_[(size_t)l * nbp] : 0.0, isi * row(own, l), l);
     return;
@@ -724,10 +733,10 @@ __global__ void __launch_bounds__(BLOCK, 4) k_node(OpDev op, MemDev md, FArgs fa
 __device__ void rand_accum_pair(const OpDev& op, const MemDev& md, uint64_t chim, int i, uint32_t inf, int m, int vi, int vs,
                                 int q, const double* __restrict__ x, int xnorm, double coef, double acc[9]) {
   const int nbp = md.nbp, nsm = op.nsmax;
-  const bool fz = (inf >> 20) & 1u;
+  const int fs = frozen_slot(inf, md, m);
   const double isi = op.isq[i];
-  if (fz && md.frozen[m]) {
-    const double* fp = md.fpar + (size_t)q * 9 * nbp;
+  if (fs) {
+    const double* fp = frozen_row(md, fs, q);
     for (int l = 0; l < 9; ++l) {
       double xv = x[((size_t)i * 9 + l) * vs + vi];
       if (xnorm) xv = isi * xv;
@@ -949,7 +958,8 @@ __global__ void __launch_bounds__(BLOCK) k_adj_grad(OpDev op, MemDev md, const d
     }
     // one pass per gathered term (the node's own state, then every flip partner): px = R-term of x~, py = of y~, and the
     // per-shell products A = y . px, B = t2x . py + t2y . px spread over the species slots with the weights mt(l0, k, s)
-    const bool fz = ((inf >> 20) & 1u) && md.frozen[m];
+    const int fs = frozen_slot(inf, md, m);
+    const bool fz = fs != 0;
     const int l0 = inf & 15u, kap = (inf >> 4) & 255u;
     const int nsc = fz ? 1 : op.nsc[kap];
     const double* mtk = md.mtm + ((size_t)kap * nsm + l0) * nsm * nsm * nbp;  // [k][sl][nbp]
@@ -983,8 +993,8 @@ __global__ void __launch_bounds__(BLOCK) k_adj_grad(OpDev op, MemDev md, const d
         B[L] = fma(tx[l], pyl, fma(ty[l], pxl, B[L]));
       }
       if (fz) {
-        // the frozen start site carries its species: R_s is the identity there for that species alone
-        const int s = md.fsp[m];
+        // a frozen site carries its species: R_s is the identity there for that species alone
+        const int s = md.fsp[(size_t)(fs - 1) * nbp + m];
 #pragma unroll
         for (int q = 0; q < NSP; ++q)
           if (q == s)
@@ -1187,20 +1197,20 @@ __global__ void __launch_bounds__(BLOCK) k_sum_member(const T* part, int nbx, in
   if (threadIdx.x == 0) out[m] = s;
 }
 
-// Start vectors of route ed3 (ed3_start, src/asr_ed3.f90:131-144) from vv = h'|s0> (projected), s0 the unit vector at the
-// member's start component (node 0, orbital l0): u = su s0 + sv vv, t = tsu s0 + tsv vv, coef = (su, sv, tsu, tsv) per member.
-__global__ void k_ed3_start(const cplx* vv, cplx* u, cplx* tv, const double* coef, const int* l0, size_t n, int nbp) {
+// Start vectors of the two-chain blocks (ed3_start, src/asr_ed3.f90:131-144; epi_start, src/asr_epi.f90:578-587) from
+// vv = h'|s0> (projected), s0 the member's start vector on the start node (s0v: 9 x nbp, the unit vector of the start
+// orbital or a symmetry-adapted combination): u = su s0 + sv vv, t = tsu s0 + tsv vv, coef = (su, sv, tsu, tsv) per member.
+__global__ void k_block_start(const cplx* vv, cplx* u, cplx* tv, const double* coef, const double* s0v, size_t n, int nbp) {
   const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n) return;
   const int m = (int)(q % nbp);
-  const bool s0 = (q / nbp) == (size_t)l0[m];
+  const size_t row = q / nbp;  // node * 9 + l: rows 0..8 are the start node
+  const double s0 = row < 9 ? s0v[row * nbp + m] : 0.0;
   const double* cf = coef + 4 * (size_t)m;
   const cplx v = vv[q];
   cplx uq = cf[1] * v, tq = cf[3] * v;
-  if (s0) {
-    uq.re += cf[0];
-    tq.re += cf[2];
-  }
+  uq.re += cf[0] * s0;
+  tq.re += cf[2] * s0;
   u[q] = uq;
   tv[q] = tq;
 }
@@ -1749,6 +1759,7 @@ struct asr_gpu_op {
   double depth_tol = 0.0;                         // adaptive depth of the edc chains (ed_depth_tol), 0 = off
   int depth_tail = 0;                             // tail of that test (ed_tail): 0 hull, 1 bp, 2 avg2, 3 none
   bool reorth = false;                            // full reorthogonalisation of the real chains (lanczos reorth)
+  bool slot_flippable[2] = {false, false};        // the site of frozen slot 1 / 2 was flippable (no species may be frozen)
   std::vector<std::pair<long long, int>> tuned;   // (members, complex, uniform d, zero o) key -> member tile
 };
 
@@ -1794,7 +1805,7 @@ struct Members {
   bool uniform_d = true, zero_o = true, zero_e = true, complex_lin = false, ed3 = false;
   std::vector<uint64_t> chim;
   std::vector<double> wts, fpar, dvec, dperm, mtm;  // mtm: ncls x nsmax x nsmax x nsmax x nbp (class tables per member)
-  std::vector<int> frozen, fsp;
+  std::vector<int> fsp;                             // 2 x nbp: frozen species of the start site and of the partner site
   std::vector<cplx> z;
 };
 
@@ -1810,10 +1821,12 @@ int chi_mask(const asr_gpu_op* op, int l0, uint64_t* mask) {
 }
 
 // params: nchain x 4 x nsp x 9 (c, d, o, e); pab: nchain x 2 x nsp x 9 or null; conc: nchain x ncls x nsmax or null;
-// pex (route ed3): nchain x 4 x nsp x 9 (ua, ub, qa, qb) or null
+// pex (route ed3): nchain x 4 x nsp x 9 (ua, ub, qa, qb) or null; frozen_sp2: species of the frozen partner site (nodes
+// of fz slot 2, augcls_freeze_any) or null; chim_in: the characters of a symmetry-adapted start vector (bit g set <=>
+// chi(g) = -1, augcls_set_chi) or null for those of the start orbital
 int prepare_members(const asr_gpu_op* op, int nchain, const int* start_orb, const double* params, const double* pab,
                     const int* frozen_sp, const double* conc, const double* z, Members& M, int tile = 0,
-                    const double* pex = nullptr) {
+                    const double* pex = nullptr, const int* frozen_sp2 = nullptr, const uint64_t* chim_in = nullptr) {
   const int nsm = op->nsmax, ncls = op->ncls, nsp = op->nsp;
   M.nb = nchain;
   M.pair = (tile == 65) ? 1 : 0;  // 65: member tile 64 with complex members as lane pairs
@@ -1824,19 +1837,23 @@ int prepare_members(const asr_gpu_op* op, int nchain, const int* start_orb, cons
   const int nbp = M.nbp;
   M.chim.assign(nbp, 0);
   M.wts.assign((size_t)NQ * ncls * nsm * nsm * 9 * nbp, 0.0);
-  M.fpar.assign((size_t)NQ * 9 * nbp, 0.0);
+  M.fpar.assign((size_t)2 * NQ * 9 * nbp, 0.0);
   M.dvec.assign((size_t)9 * nbp, 0.0);
   M.dperm.assign((size_t)op->nop * 9 * nbp, 0.0);
   M.mtm.assign((size_t)ncls * nsm * nsm * nsm * nbp, 0.0);
-  M.frozen.assign(nbp, 0);
-  M.fsp.assign(nbp, -1);
+  M.fsp.assign((size_t)2 * nbp, -1);
   M.z.assign(nbp, cplx{0.0, 0.0});
   M.complex_lin = (pab != nullptr);
   M.ed3 = (pex != nullptr);
   std::vector<double> mt((size_t)ncls * nsm * nsm * nsm);
   for (int m = 0; m < nchain; ++m) {
     if (start_orb[m] < 0 || start_orb[m] > 8) return fail("start orbital out of range");
-    if (chi_mask(op, start_orb[m], &M.chim[m])) return 1;
+    if (chim_in) {
+      if (op->nop < 64 && (chim_in[m] >> op->nop) != 0) return fail("augcls_set_chi: character bits beyond the group");
+      M.chim[m] = chim_in[m];
+    } else if (chi_mask(op, start_orb[m], &M.chim[m])) {
+      return 1;
+    }
     // class tables: the operator's, or the member's concentrations (augcls_set_class_conc)
     if (conc) {
       std::fill(mt.begin(), mt.end(), 0.0);
@@ -1895,13 +1912,16 @@ int prepare_members(const asr_gpu_op* op, int nchain, const int* start_orb, cons
                 s += par[op->spc_h[kap * nsm + sp] * 9 + lam] * mt[(((size_t)kap * nsm + k) * nsm + kk) * nsm + sp];
               M.wts[(((((size_t)q * ncls + kap) * nsm + k) * nsm + kk) * 9 + lam) * nbp + m] = s;
             }
-      if (frozen_sp && frozen_sp[m] >= 0) {
-        if (frozen_sp[m] >= nsp) return fail("augcls_freeze_site: species index out of range");
-        for (int l = 0; l < 9; ++l) M.fpar[((size_t)q * 9 + l) * nbp + m] = par[frozen_sp[m] * 9 + l];
+      for (int slot = 0; slot < 2; ++slot) {
+        const int* fsp = slot == 0 ? frozen_sp : frozen_sp2;
+        if (!fsp || fsp[m] < 0) continue;
+        if (fsp[m] >= nsp) return fail("augcls_freeze_site: species index out of range");
+        if (op->slot_flippable[slot]) return fail("augcls_freeze_site: the frozen site was flippable");
+        for (int l = 0; l < 9; ++l) M.fpar[(((size_t)slot * NQ + q) * 9 + l) * nbp + m] = par[fsp[m] * 9 + l];
       }
     }
-    M.frozen[m] = (frozen_sp && frozen_sp[m] >= 0) ? 1 : 0;
-    M.fsp[m] = M.frozen[m] ? frozen_sp[m] : -1;
+    M.fsp[m] = (frozen_sp && frozen_sp[m] >= 0) ? frozen_sp[m] : -1;
+    M.fsp[(size_t)nbp + m] = (frozen_sp2 && frozen_sp2[m] >= 0) ? frozen_sp2[m] : -1;
     for (size_t q = 0; q < mt.size(); ++q) M.mtm[q * nbp + m] = mt[q];
     const double* dp = pm + (size_t)nsp * 9;
     for (int s = 1; s < nsp; ++s)
@@ -1922,7 +1942,7 @@ int prepare_members(const asr_gpu_op* op, int nchain, const int* start_orb, cons
 struct MemBuffers {
   uint64_t* chim = nullptr;
   double *wts = nullptr, *fpar = nullptr, *dvec = nullptr, *dperm = nullptr, *mtm = nullptr;
-  int *frozen = nullptr, *fsp = nullptr;
+  int* fsp = nullptr;
   cplx* z = nullptr;
   ~MemBuffers() {
     if (mtm) (void)hipFree(mtm);
@@ -1932,14 +1952,13 @@ struct MemBuffers {
     if (wts) (void)hipFree(wts);
     if (fpar) (void)hipFree(fpar);
     if (dvec) (void)hipFree(dvec);
-    if (frozen) (void)hipFree(frozen);
     if (z) (void)hipFree(z);
   }
 };
 
 int upload_members(const asr_gpu_op* op, const Members& M, MemBuffers& B, MemDev& md) {
   if (dalloc(&B.chim, M.chim.size()) || dalloc(&B.wts, M.wts.size()) || dalloc(&B.fpar, M.fpar.size()) ||
-      dalloc(&B.dvec, M.dvec.size()) || dalloc(&B.frozen, M.frozen.size()) || dalloc(&B.z, M.z.size()) ||
+      dalloc(&B.dvec, M.dvec.size()) || dalloc(&B.z, M.z.size()) ||
       dalloc(&B.dperm, M.dperm.size()) || dalloc(&B.mtm, M.mtm.size()) || dalloc(&B.fsp, M.fsp.size()))
     return 1;
   HCHECK(hipMemcpyAsync(B.fsp, M.fsp.data(), M.fsp.size() * 4, hipMemcpyHostToDevice, op->stream));
@@ -1949,10 +1968,9 @@ int upload_members(const asr_gpu_op* op, const Members& M, MemBuffers& B, MemDev
   HCHECK(hipMemcpyAsync(B.wts, M.wts.data(), M.wts.size() * 8, hipMemcpyHostToDevice, op->stream));
   HCHECK(hipMemcpyAsync(B.fpar, M.fpar.data(), M.fpar.size() * 8, hipMemcpyHostToDevice, op->stream));
   HCHECK(hipMemcpyAsync(B.dvec, M.dvec.data(), M.dvec.size() * 8, hipMemcpyHostToDevice, op->stream));
-  HCHECK(hipMemcpyAsync(B.frozen, M.frozen.data(), M.frozen.size() * 4, hipMemcpyHostToDevice, op->stream));
   HCHECK(hipMemcpyAsync(B.z, M.z.data(), M.z.size() * sizeof(cplx), hipMemcpyHostToDevice, op->stream));
   md.nb = M.nb; md.nbp = M.nbp; md.mt = M.mt; md.pair = M.pair; md.tact = nullptr;
-  md.chimask = B.chim; md.wts = B.wts; md.fpar = B.fpar; md.frozen = B.frozen; md.dvec = B.dvec; md.z = B.z;
+  md.chimask = B.chim; md.wts = B.wts; md.fpar = B.fpar; md.dvec = B.dvec; md.z = B.z;
   md.dperm = B.dperm;
   md.mtm = B.mtm;
   md.fsp = B.fsp;
@@ -2164,9 +2182,10 @@ int apply_level_op(const asr_gpu_op* op, const Members& M, const MemDev& md, con
 // of the level (same kernels, same members) between HIP events; the fastest is kept per (operator, member count, kind).
 template <class T>
 int autotune_tile(asr_gpu_op* op, int nchain, const int* start_orb, const double* params, const double* pab,
-                  const int* frozen_sp, const double* conc, const double* z, int* best, const double* pex = nullptr) {
+                  const int* frozen_sp, const double* conc, const double* z, int* best, const double* pex = nullptr,
+                  const int* fsp2 = nullptr, const uint64_t* chim = nullptr) {
   Members M0;
-  if (prepare_members(op, nchain, start_orb, params, pab, frozen_sp, conc, z, M0, 0, pex)) return 1;
+  if (prepare_members(op, nchain, start_orb, params, pab, frozen_sp, conc, z, M0, 0, pex, fsp2, chim)) return 1;
   long long key = ((long long)nchain << 8) | (sizeof(T) == 16 ? 1 : 0) | (M0.uniform_d ? 2 : 0) | (M0.zero_o ? 4 : 0) |
                   (pab ? 8 : 0) | (pex ? 16 : 0);
   for (auto& kv : op->tuned)
@@ -2179,7 +2198,7 @@ int autotune_tile(asr_gpu_op* op, int nchain, const int* start_orb, const double
   if (top == WAVE && sizeof(T) == sizeof(cplx)) cands.push_back(65);
   for (int mt : cands) {
     Members M;
-    if (prepare_members(op, nchain, start_orb, params, pab, frozen_sp, conc, z, M, mt, pex)) return 1;
+    if (prepare_members(op, nchain, start_orb, params, pab, frozen_sp, conc, z, M, mt, pex, fsp2, chim)) return 1;
     MemBuffers MB;
     MemDev md;
     if (upload_members(op, M, MB, md)) return 1;
@@ -2225,12 +2244,17 @@ struct ChainBuffers {
   }
 };
 
-// Route ed3 per member (member 2j: the chain of input chain j from |a> = |u> + o_c |v>, projected on |v>; member 2j + 1:
-// the chain from |v> = h'|u>, projected on |a>; src/asr_ed3.f90:247-278): the energy-linear diagonals u and Qt, o_c, side.
-struct Ed3In {
-  const double* pex;  // nmember x 4 x nsp x 9: ua, ub, qa, qb
-  const double* oc;   // nmember
-  const int* side;    // nmember: 0 from |a>, 1 from |v>
+// The two-chain blocks of route ed3 (ed3_blocks, src/asr_ed3.f90:247-278) and of the EPI pairs (epi_blocks,
+// src/asr_epi.f90:628-647), per member: member 2j is the chain of block j from |a> = |s0> + o_c |v> (normalised), projected
+// on |v> = h'|s0>; member 2j + 1 the chain from |v> / |v|, projected on |a>.  The operator is K'(z) of route ed3 when pex
+// is given, else H(z) of route edc (o_c = 0 there).
+struct BlockIn {
+  const double* pex;   // nmember x 4 x nsp x 9: ua, ub, qa, qb (route ed3) or null
+  const double* oc;    // nmember
+  const int* side;     // nmember: 0 from |a>, 1 from |v>
+  const double* v9;    // nmember x 9: the start vector on the start node, or null for the unit vector of the start orbital
+  const uint64_t* chim;  // nmember: characters of v9 (bit g set <=> chi(g) = -1) or null (those of the start orbital)
+  const int* fsp2;     // nmember: species of the frozen partner site (fz slot 2) or null
 };
 
 // One batch of chains on one operator: members sorted by depth (deepest first, so member tiles retire in order), device
@@ -2252,9 +2276,10 @@ struct ChainRun {
   std::vector<int> order;  // sorted member -> input chain
   int* tact = nullptr;     // per member tile: 1 while any member of the tile is active
   int level_host = 0;      // levels issued
-  std::vector<cplx> gprev, fprev;  // adaptive depth: G(z) (and route ed3's projection f) at the previous tested level
-  bool ed3 = false;
-  std::vector<double> snorm, tnorm;  // route ed3: norms of the start vector and of the projection vector per member
+  std::vector<cplx> gprev, fprev;  // adaptive depth: G(z) (and the projection f of the blocks) at the previous tested level
+  bool ed3 = false;                  // two-chain blocks (start vectors, projections c_k, closure flags)
+  std::vector<double> snorm, tnorm;  // blocks: norms of the start vector and of the projection vector per member
+  std::vector<double> s0h;           // blocks: 9 x nbp start vectors on the start node
   bool reorth = false;               // real chains: full reorthogonalisation against every stored Lanczos vector
   size_t qplane = 0;
 
@@ -2264,10 +2289,10 @@ struct ChainRun {
 
   int setup(asr_gpu_op* op_, hipStream_t s, int nch, const int* start_orb, const double* params, const double* pab,
             const int* frozen_sp, const double* conc, const double* z, const int* nlev, int project_, int ldab, int tile,
-            double tiny, bool real_chain, const Ed3In* e3 = nullptr) {
+            double tiny, bool real_chain, const BlockIn* e3 = nullptr) {
     op = op_; stream = s; nchain = nch; project = project_; ld = ldab; lin = (pab != nullptr);
     ed3 = (e3 != nullptr);
-    if (ed3 && (!std::is_same<T, cplx>::value || !pab || !z)) return fail("route ed3: complex chains with z - et and z");
+    if (ed3 && (!std::is_same<T, cplx>::value || !pab || !z)) return fail("chain blocks: complex chains with X(z) and z");
     if (!op->have_tables || !op->have_ops || !op->have_blocks || !op->have_classes)
       return fail("asr_gpu chains: operator tables incomplete");
     // deepest chains first
@@ -2277,8 +2302,10 @@ struct ChainRun {
     const int nsp9 = op->nsp * 9, ncn = op->ncls * op->nsmax;
     std::vector<int> so(nch), fz(nch), nl(nch);
     std::vector<double> pp((size_t)nch * 4 * nsp9), pa(pab ? (size_t)nch * 2 * nsp9 : 0), cc(conc ? (size_t)nch * ncn : 0),
-        zz(z ? 2 * nch : 0), px(ed3 ? (size_t)nch * 4 * nsp9 : 0), ocs(ed3 ? nch : 0);
-    std::vector<int> sides(ed3 ? nch : 0);
+        zz(z ? 2 * nch : 0), px(ed3 && e3->pex ? (size_t)nch * 4 * nsp9 : 0), ocs(ed3 ? nch : 0),
+        v9s(ed3 && e3->v9 ? (size_t)nch * 9 : 0);
+    std::vector<int> sides(ed3 ? nch : 0), fz2(ed3 && e3->fsp2 ? nch : 0);
+    std::vector<uint64_t> chs(ed3 && e3->chim ? nch : 0);
     for (int q = 0; q < nch; ++q) {
       const int c = order[q];
       so[q] = start_orb[c];
@@ -2289,7 +2316,10 @@ struct ChainRun {
       if (conc) std::memcpy(&cc[(size_t)q * ncn], conc + (size_t)c * ncn, sizeof(double) * ncn);
       if (z) { zz[2 * q] = z[2 * c]; zz[2 * q + 1] = z[2 * c + 1]; }
       if (ed3) {
-        std::memcpy(&px[(size_t)q * 4 * nsp9], e3->pex + (size_t)c * 4 * nsp9, sizeof(double) * 4 * nsp9);
+        if (e3->pex) std::memcpy(&px[(size_t)q * 4 * nsp9], e3->pex + (size_t)c * 4 * nsp9, sizeof(double) * 4 * nsp9);
+        if (e3->v9) std::memcpy(&v9s[(size_t)q * 9], e3->v9 + (size_t)c * 9, sizeof(double) * 9);
+        if (e3->chim) chs[q] = e3->chim[c];
+        if (e3->fsp2) fz2[q] = e3->fsp2[c];
         ocs[q] = e3->oc[c];
         sides[q] = e3->side[c];
       }
@@ -2297,9 +2327,12 @@ struct ChainRun {
     const double* pab_s = pab ? pa.data() : nullptr;
     const double* conc_s = conc ? cc.data() : nullptr;
     const double* z_s = z ? zz.data() : nullptr;
-    const double* px_s = ed3 ? px.data() : nullptr;
-    if (tile == 0 && autotune_tile<T>(op, nch, so.data(), pp.data(), pab_s, fz.data(), conc_s, z_s, &tile, px_s)) return 1;
-    if (prepare_members(op, nch, so.data(), pp.data(), pab_s, fz.data(), conc_s, z_s, M, tile, px_s)) return 1;
+    const double* px_s = px.empty() ? nullptr : px.data();
+    const int* fz2_s = fz2.empty() ? nullptr : fz2.data();
+    const uint64_t* ch_s = chs.empty() ? nullptr : chs.data();
+    if (tile == 0 && autotune_tile<T>(op, nch, so.data(), pp.data(), pab_s, fz.data(), conc_s, z_s, &tile, px_s, fz2_s, ch_s))
+      return 1;
+    if (prepare_members(op, nch, so.data(), pp.data(), pab_s, fz.data(), conc_s, z_s, M, tile, px_s, fz2_s, ch_s)) return 1;
     if (upload_members(op, M, MB, md)) return 1;
     maxdepth = 0;
     for (int q = 0; q < nch; ++q) maxdepth = std::max(maxdepth, nl[q]);
@@ -2343,8 +2376,16 @@ struct ChainRun {
       dep[q] = nl[q];
       act[q] = nl[q] > 0 ? 1 : 0;
       if (act[q]) ta[q / M.mt] = 1;
-      // start vector: unit at component iv = 9 * node 0 + l0
-      *(double*)&ustart[(size_t)so[q] * M.nbp + q] = 1.0;
+      // start vector: unit at component iv = 9 * node 0 + l0, or the block's combination on the start node
+      if (ed3 && !v9s.empty())
+        for (int l = 0; l < 9; ++l) *(double*)&ustart[(size_t)l * M.nbp + q] = v9s[(size_t)q * 9 + l];
+      else
+        *(double*)&ustart[(size_t)so[q] * M.nbp + q] = 1.0;
+    }
+    if (ed3) {
+      s0h.assign((size_t)9 * M.nbp, 0.0);
+      for (int q = 0; q < nch; ++q)
+        for (int l = 0; l < 9; ++l) s0h[(size_t)l * M.nbp + q] = *(const double*)&ustart[(size_t)l * M.nbp + q];
     }
     HCHECK(hipMemcpyAsync(B.u, ustart.data(), ustart.size() * sizeof(T), hipMemcpyHostToDevice, stream));
     HCHECK(hipMemcpyAsync(B.depth, dep.data(), M.nbp * sizeof(int), hipMemcpyHostToDevice, stream));
@@ -2355,18 +2396,20 @@ struct ChainRun {
     cd.real_chain = real_chain ? 1 : 0; cd.tiny = tiny;
     cd.cvec = B.c; cd.closed = B.closed;
     sgrid = dim3((unsigned)((nv + 255) / 256));
-    if (ed3 && ed3_start(so, ocs, sides)) return 1;
+    if (ed3 && block_start(ocs, sides)) return 1;
     return 0;
   }
 
-  // Route ed3 start vectors (ed3_start, src/asr_ed3.f90:131-144): vv = h'|s0> from the unit vectors in u (one S pass,
-  // normalised coordinates), projected; nv^2 = vv.vv on the device, na^2 = 1 + 2 o_c vv(iv) + o_c^2 nv^2; then u and the
-  // projection vector t of each member, and c_1 = t.u.
-  int ed3_start(const std::vector<int>& so, const std::vector<double>& ocs, const std::vector<int>& sides) {
+  // Start vectors of the blocks (ed3_start, src/asr_ed3.f90:131-144; epi_start, src/asr_epi.f90:578-587): vv = h'|s0>
+  // from the start vectors in u (one S pass, normalised coordinates), projected; nv^2 = vv.vv on the device,
+  // na^2 = s0.s0 + 2 o_c s0.vv + o_c^2 nv^2 (s0 lives on the start node); then u and the projection vector t of each
+  // member, and c_1 = t.u.
+  int block_start(const std::vector<double>& ocs, const std::vector<int>& sides) {
     if constexpr (!std::is_same<T, cplx>::value) {
-      return fail("route ed3: complex chains only");
+      return fail("chain blocks: complex chains only");
     } else {
       const int nb = M.nb, nbp = M.nbp;
+      if (!M.uniform_d) return fail("chain blocks: h' needs one d for every species (D_c^1/2)");
       md.tact = tact;
       FArgs f = fargs0();
       f.xs = B.u; f.xs_norm = 1; f.s_din = 1; f.s_dout = 1; f.s_coef = 1.0;
@@ -2389,30 +2432,34 @@ struct ChainRun {
       for (int q = 0; q < nb; ++q) {
         const double oc = ocs[q];
         const double vn2 = nv2[q].re;
-        const cplx vi = row0[(size_t)so[q] * nbp + q];
+        cplx vi{0.0, 0.0};
+        double s00 = 0.0;
+        for (int l = 0; l < 9; ++l) {
+          const double sl = s0h[(size_t)l * nbp + q];
+          vi = vi + sl * row0[(size_t)l * nbp + q];
+          s00 += sl * sl;
+        }
+        if (std::fabs(s00 - 1.0) > 1.0e-12) return fail("chain blocks: the start vector must have unit norm");
         const double nvv = std::sqrt(vn2);
-        const double na = std::sqrt((1.0 + 2.0 * oc * vi.re + oc * oc * vn2));
-        if (!(nvv >= 1.0e-14) || !(na >= 1.0e-14)) return fail("ed3_start: zero start vector");
+        const double na = std::sqrt((s00 + 2.0 * oc * vi.re + oc * oc * vn2));
+        if (!(nvv >= 1.0e-14) || !(na >= 1.0e-14)) return fail("ed3_start / epi_start: zero start vector");
         double su, sv, tsu, tsv;
         if (sides[q] == 0) { su = 1.0 / na; sv = oc / na; tsu = 0.0; tsv = 1.0; snorm[q] = na; tnorm[q] = nvv; }
         else { su = 0.0; sv = 1.0 / nvv; tsu = 1.0; tsv = oc; snorm[q] = nvv; tnorm[q] = na; }
         coef[4 * q] = su; coef[4 * q + 1] = sv; coef[4 * q + 2] = tsu; coef[4 * q + 3] = tsv;
-        // c_1 = t.u = tsu su + (tsu sv + tsv su) vv(iv) + tsv sv vv.vv
+        // c_1 = t.u = tsu su s0.s0 + (tsu sv + tsv su) s0.vv + tsv sv vv.vv
         const cplx vv2 = nv2[q];
-        c1[(size_t)q * ld] = cplx{tsu * su, 0.0} + (tsu * sv + tsv * su) * vi + (tsv * sv) * vv2;
+        c1[(size_t)q * ld] = cplx{tsu * su * s00, 0.0} + (tsu * sv + tsv * su) * vi + (tsv * sv) * vv2;
       }
-      double* dcoef = nullptr;
-      int* dl0 = nullptr;
-      std::vector<int> l0(nbp, -1);
-      for (int q = 0; q < nb; ++q) l0[q] = so[q];
-      if (dalloc(&dcoef, coef.size()) || dalloc(&dl0, l0.size())) return 1;
+      double *dcoef = nullptr, *ds0 = nullptr;
+      if (dalloc(&dcoef, coef.size()) || dalloc(&ds0, s0h.size())) return 1;
       int st = 0;
       st |= hipMemcpyAsync(dcoef, coef.data(), coef.size() * 8, hipMemcpyHostToDevice, stream) != hipSuccess;
-      st |= hipMemcpyAsync(dl0, l0.data(), l0.size() * 4, hipMemcpyHostToDevice, stream) != hipSuccess;
+      st |= hipMemcpyAsync(ds0, s0h.data(), s0h.size() * 8, hipMemcpyHostToDevice, stream) != hipSuccess;
       st |= hipMemcpyAsync(B.c, c1.data(), c1.size() * sizeof(T), hipMemcpyHostToDevice, stream) != hipSuccess;
       if (!st) {
-        hipLaunchKernelGGL(k_ed3_start, sgrid, dim3(256), 0, stream, (const cplx*)B.w, (cplx*)B.u, (cplx*)B.tv,
-                           (const double*)dcoef, (const int*)dl0, nv, nbp);
+        hipLaunchKernelGGL(k_block_start, sgrid, dim3(256), 0, stream, (const cplx*)B.w, (cplx*)B.u, (cplx*)B.tv,
+                           (const double*)dcoef, (const double*)ds0, nv, nbp);
         st |= hipGetLastError() != hipSuccess;
       }
       st |= hipMemsetAsync(B.w, 0, nv * sizeof(T), stream) != hipSuccess;
@@ -2421,8 +2468,8 @@ struct ChainRun {
       st |= hipMemsetAsync(B.closed, 0, M.nbp * sizeof(int), stream) != hipSuccess;
       st |= hipStreamSynchronize(stream) != hipSuccess;
       (void)hipFree(dcoef);
-      (void)hipFree(dl0);
-      if (st) return fail("route ed3: start vectors");
+      (void)hipFree(ds0);
+      if (st) return fail("chain blocks: start vectors");
       return 0;
     }
   }
@@ -2730,20 +2777,23 @@ int run_multi(int ngroup, asr_gpu_op* const* ops, const int* ngch, const int* st
   return status;
 }
 
-// Route ed3 chains on one operator (the members of asr_gpu_ed3_blocks, two per energy and start orbital): mode 0 host-issued
-// levels, 1 the plain form (one chain per launch sequence, host read-back per level, as chain), 2 the whole run as one HIP
-// graph (fixed depths only: the adaptive depth decides on the host between levels, so it falls back to mode 0).
-int run_ed3(asr_gpu_op* op, int nmem, const int* so, const double* params, const double* pab, const int* fz,
-            const double* conc, const double* z, const int* nlev, const Ed3In& e3, int project, int ldab, cplx* a, cplx* b,
-            cplx* c, int* mused, int* closed, double* snorm, double* tnorm, int mode, double* elapsed_ms) {
+// The chains of two-chain blocks on one operator (the members of asr_gpu_ed3_blocks and asr_gpu_epi_blocks, two per
+// block): mode 0 host-issued levels, 1 the plain form (one chain per launch sequence, host read-back per level, as chain),
+// 2 the whole run as one HIP graph (fixed depths only: the adaptive depth decides on the host between levels, so it
+// falls back to mode 0).
+int run_blocks(asr_gpu_op* op, int nmem, const int* so, const double* params, const double* pab, const int* fz,
+               const double* conc, const double* z, const int* nlev, const BlockIn& e3, int project, int ldab, cplx* a, cplx* b,
+               cplx* c, int* mused, int* closed, double* snorm, double* tnorm, int mode, double* elapsed_ms) {
   const int nsp9 = op->nsp * 9, ncn = op->ncls * op->nsmax;
   *elapsed_ms = 0.0;
   if (mode == 2 && op->depth_tol > 0.0) mode = 0;
   if (mode == 1 && nmem > 1) {
     for (int q = 0; q < nmem; ++q) {
-      const Ed3In e1{e3.pex + (size_t)q * 4 * nsp9, e3.oc + q, e3.side + q};
+      const BlockIn e1{e3.pex ? e3.pex + (size_t)q * 4 * nsp9 : nullptr, e3.oc + q, e3.side + q,
+                       e3.v9 ? e3.v9 + (size_t)q * 9 : nullptr, e3.chim ? e3.chim + q : nullptr,
+                       e3.fsp2 ? e3.fsp2 + q : nullptr};
       double ms = 0.0;
-      if (run_ed3(op, 1, so + q, params + (size_t)q * 4 * nsp9, pab + (size_t)q * 2 * nsp9, fz ? fz + q : nullptr,
+      if (run_blocks(op, 1, so + q, params + (size_t)q * 4 * nsp9, pab + (size_t)q * 2 * nsp9, fz ? fz + q : nullptr,
                   conc ? conc + (size_t)q * ncn : nullptr, z + 2 * q, nlev + q, e1, project, ldab, a + (size_t)q * ldab,
                   b + (size_t)q * ldab, c + (size_t)q * ldab, mused + q, closed + q, snorm + q, tnorm + q, 1, &ms))
         return 1;
@@ -3021,10 +3071,18 @@ int asr_gpu_op_set_tables(asr_gpu_op* op, const int* hopr, const int* hopg, cons
       return fail("set_tables: flip link out of range");
     flip[q] = flipr[q] < 0 ? 0u : ((uint32_t)(flipr[q] + 1) << 6) | (uint32_t)flipg[q];
   }
+  op->slot_flippable[0] = op->slot_flippable[1] = false;
   for (int i = 0; i < nn; ++i) {
     if (lst[i] < 0 || lst[i] >= nsm || ncl[i] < 0 || ncl[i] >= op->ncls || ntyp[i] < 0 || ntyp[i] >= op->ntype)
       return fail("set_tables: node state, class or type out of range");
-    info[i] = (uint32_t)lst[i] | ((uint32_t)ncl[i] << 4) | ((uint32_t)ntyp[i] << 12) | ((uint32_t)(fz && fz[i] ? 1 : 0) << 20);
+    if (fz && fz[i] > 2) return fail("set_tables: frozen-site slot 0 (none), 1 (start site) or 2 (pair partner)");
+    // a site that can be frozen must have been non-flippable: its nodes keep the reference state, no flip partner
+    if (fz && fz[i]) {
+      bool flips = lst[i] != 0;
+      for (int k = 0; k < nsm; ++k) flips |= (k != lst[i] && flipr[(size_t)i * nsm + k] >= 0);
+      if (flips) op->slot_flippable[fz[i] - 1] = true;
+    }
+    info[i] = (uint32_t)lst[i] | ((uint32_t)ncl[i] << 4) | ((uint32_t)ntyp[i] << 12) | ((uint32_t)(fz ? fz[i] : 0) << 20);
   }
   for (int i = 0; i <= nn; ++i)
     if (stab_ptr[i] < 0 || stab_ptr[i] > nstab || (i > 0 && stab_ptr[i] < stab_ptr[i - 1]))
@@ -3234,6 +3292,43 @@ int asr_gpu_chains_edc_multi(int ngroup, asr_gpu_op* const* ops, const int* ncha
                          (cplx*)b, mused, mode, 1e-12, false, elapsed_ms);
 }
 
+// The blocks of the two chains of every input block j (members 2j, 2j + 1): q0 = na^2 g_a, q1a = na f_a, q2 = nv^2 g_v,
+// q1b = nv f_v with (g, f) from cf_species (the operator's tail) or, for a chain that closed, the finite fraction;
+// bk = (q0 - 2 o_c q1 + o_c^2 q2, q1 - o_c q2, q2, q1a - q1b), q1 = (q1a + q1b) / 2 (ed3_blocks, src/asr_ed3.f90:247-278;
+// with o_c = 0 and na = 1 the x of epi_blocks, src/asr_epi.f90:628-647).
+static int finish_blocks(const asr_gpu_op* op, int nchain, const double* oc, const double* z, int ldab, const cplx* ac,
+                         const cplx* bc, const cplx* ccv, const int* mused, const int* closed, const double* snorm,
+                         double* bk) {
+  for (int j = 0; j < nchain; ++j) {
+    cplx q[2][2];  // (norm^2 g, norm f) per side
+    for (int sd = 0; sd < 2; ++sd) {
+      const int m = 2 * j + sd;
+      const size_t o = (size_t)m * ldab;
+      double g[2], f[2];
+      int st;
+      if (closed[m])
+        st = asr_host_cf_closed((const double*)(ac + o), (const double*)(bc + o), (const double*)(ccv + o), mused[m], z + 2 * j,
+                                g, f);
+      else  // the status of cf_species counts bp fallbacks (hull values then), not errors
+        st = asr_host_cf_species((const double*)(ac + o), (const double*)(bc + o), (const double*)(ccv + o), mused[m],
+                                 z + 2 * j, op->depth_tail, g, f, nullptr, nullptr) < 0;
+      if (st) return fail("chain blocks: continued fraction");
+      const double nr = snorm[m];
+      q[sd][0] = (nr * nr) * cplx{g[0], g[1]};
+      q[sd][1] = nr * cplx{f[0], f[1]};
+    }
+    const cplx q0 = q[0][0], q1a = q[0][1], q2 = q[1][0], q1b = q[1][1];
+    const double ocj = oc ? oc[j] : 0.0;
+    const cplx fm = 0.5 * (q1a + q1b);
+    const cplx r[4] = {q0 - (2.0 * ocj) * fm + (ocj * ocj) * q2, fm - ocj * q2, q2, q1a - q1b};
+    for (int t = 0; t < 4; ++t) {
+      bk[(size_t)j * 8 + 2 * t] = r[t].re;
+      bk[(size_t)j * 8 + 2 * t + 1] = r[t].im;
+    }
+  }
+  return 0;
+}
+
 // Route ed3 (ed3_blocks / ed3_blocks_blk, src/asr_ed3.f90:247-340): per input chain (start orbital, energy, central species)
 // the chain from |a> / na projected on |v> and the chain from |v> / nv projected on |a>, run as one batch of 2 nchain
 // members; the central blocks bk = (q0 - 2 o_c q1 + o_c^2 q2, q1 - o_c q2, q2, q1a - q1b) with q1 = (q1a + q1b) / 2 from
@@ -3278,43 +3373,73 @@ int asr_gpu_ed3_blocks(asr_gpu_op* op, int nchain, const int* start_orb, const d
     }
   std::vector<int> closed(nm, 0);
   std::vector<double> snorm(nm), tnorm(nm);
-  const Ed3In e3{pex.data(), ocm.data(), side.data()};
+  const BlockIn e3{pex.data(), ocm.data(), side.data(), nullptr, nullptr, nullptr};
   double ms = 0.0;
   cplx* ac = (cplx*)a;
   cplx* bc = (cplx*)b;
   cplx* ccv = (cplx*)c;
-  if (run_ed3(op, nm, so.data(), pp.data(), pab.data(), fz.data(), conc ? cc.data() : nullptr, zz.data(), nl.data(), e3,
+  if (run_blocks(op, nm, so.data(), pp.data(), pab.data(), fz.data(), conc ? cc.data() : nullptr, zz.data(), nl.data(), e3,
               project, ldab, ac, bc, ccv, mused, closed.data(), snorm.data(), tnorm.data(), mode, &ms))
     return 1;
   if (elapsed_ms) *elapsed_ms = ms;
+  return finish_blocks(op, nchain, oc, z, ldab, ac, bc, ccv, mused, closed.data(), snorm.data(), bk);
+}
+
+// EPI pair blocks on the route-edc operator (epi_blocks / epi_blocks_blk, src/asr_epi.f90:628-647,740-759; the chains
+// chain / chain_blk :590-626,650-737; epi_start :578-587): per block j the symmetry-adapted start vector v9 (unit norm,
+// characters chi under the pair group) on the start node, the chain from it projected on |v> = h'|v9>, and the chain
+// from |v> / nv projected on it; x = (X0, X1, X2, X1a - X1b) with X0 = <v9|G|v9>, X1 = <v9|h'G|v9>, X2 = <v9|h'Gh'|v9>.
+// params, pab, z: the edc chain parameters of epi_setup (site 0 on the path point: its species and the partner's frozen
+// through frozen_sp (fz slot 1) and frozen_sp2 (fz slot 2)).
+int asr_gpu_epi_blocks(asr_gpu_op* op, int nchain, const double* v9, const double* chi, const double* params,
+                       const double* pab, const int* frozen_sp, const int* frozen_sp2, const double* conc, const double* z,
+                       const int* nlev, int project, int ldab, double* x, double* a, double* b, double* c, int* mused,
+                       int mode, double* elapsed_ms) {
+  if (nchain <= 0) return 0;
+  if (!op || !v9 || !chi || !params || !pab || !z || !nlev || !x || !a || !b || !c || !mused)
+    return fail("asr_gpu_epi_blocks: null argument");
+  if (mode < 0 || mode > 2) return fail("asr_gpu_epi_blocks: mode 0 (batched), 1 (plain) or 2 (graph)");
+  const int nsp9 = op->nsp * 9, ncn = op->ncls * op->nsmax, nm = 2 * nchain;
+  std::vector<int> so(nm, 0), fz(nm), fz2(nm), nl(nm), side(nm);
+  std::vector<double> pp((size_t)nm * 4 * nsp9), pb((size_t)nm * 2 * nsp9), ocm(nm, 0.0), zz(2 * nm), vv((size_t)nm * 9),
+      cc(conc ? (size_t)nm * ncn : 0);
+  std::vector<uint64_t> chm(nm, 0);
   for (int j = 0; j < nchain; ++j) {
-    cplx q[2][2];  // (norm^2 g, norm f) per side
-    for (int sd = 0; sd < 2; ++sd) {
-      const int m = 2 * j + sd;
-      const size_t o = (size_t)m * ldab;
-      double g[2], f[2];
-      int st;
-      if (closed[m])
-        st = asr_host_cf_closed((const double*)(ac + o), (const double*)(bc + o), (const double*)(ccv + o), mused[m], z + 2 * j,
-                                g, f);
-      else  // the status of cf_species counts bp fallbacks (hull values then), not errors
-        st = asr_host_cf_species((const double*)(ac + o), (const double*)(bc + o), (const double*)(ccv + o), mused[m],
-                                 z + 2 * j, op->depth_tail, g, f, nullptr, nullptr) < 0;
-      if (st) return fail("asr_gpu_ed3_blocks: continued fraction");
-      const double nr = snorm[m];
-      q[sd][0] = (nr * nr) * cplx{g[0], g[1]};
-      q[sd][1] = nr * cplx{f[0], f[1]};
+    if (nlev[j] < 1 || nlev[j] > ldab) return fail("asr_gpu_epi_blocks: 1 <= nlev <= ldab");
+    uint64_t mk = 0;
+    for (int g = 0; g < op->nop; ++g) {
+      const double ch = chi[(size_t)j * op->nop + g];
+      if (std::fabs(std::fabs(ch) - 1.0) > 1e-12) return fail("augcls_set_chi: characters must be +-1");
+      if (ch < 0.0) mk |= 1ull << g;
     }
-    const cplx q0 = q[0][0], q1a = q[0][1], q2 = q[1][0], q1b = q[1][1];
-    const double ocj = oc[j];
-    const cplx fm = 0.5 * (q1a + q1b);
-    const cplx r[4] = {q0 - (2.0 * ocj) * fm + (ocj * ocj) * q2, fm - ocj * q2, q2, q1a - q1b};
-    for (int t = 0; t < 4; ++t) {
-      bk[(size_t)j * 8 + 2 * t] = r[t].re;
-      bk[(size_t)j * 8 + 2 * t + 1] = r[t].im;
+    if (chi[(size_t)j * op->nop] < 0.0) return fail("augcls_set_chi: chi(identity) must be 1");
+    for (int sd = 0; sd < 2; ++sd) {
+      const int q = 2 * j + sd;
+      fz[q] = frozen_sp ? frozen_sp[j] : -1;
+      fz2[q] = frozen_sp2 ? frozen_sp2[j] : -1;
+      nl[q] = nlev[j];
+      side[q] = sd;
+      chm[q] = mk;
+      zz[2 * q] = z[2 * j];
+      zz[2 * q + 1] = z[2 * j + 1];
+      std::memcpy(&vv[(size_t)q * 9], v9 + (size_t)j * 9, sizeof(double) * 9);
+      std::memcpy(&pp[(size_t)q * 4 * nsp9], params + (size_t)j * 4 * nsp9, sizeof(double) * 4 * nsp9);
+      std::memcpy(&pb[(size_t)q * 2 * nsp9], pab + (size_t)j * 2 * nsp9, sizeof(double) * 2 * nsp9);
+      if (conc) std::memcpy(&cc[(size_t)q * ncn], conc + (size_t)j * ncn, sizeof(double) * ncn);
     }
   }
-  return 0;
+  std::vector<int> closed(nm, 0);
+  std::vector<double> snorm(nm), tnorm(nm);
+  const BlockIn e3{nullptr, ocm.data(), side.data(), vv.data(), chm.data(), fz2.data()};
+  double ms = 0.0;
+  cplx* ac = (cplx*)a;
+  cplx* bc = (cplx*)b;
+  cplx* ccv = (cplx*)c;
+  if (run_blocks(op, nm, so.data(), pp.data(), pb.data(), fz.data(), conc ? cc.data() : nullptr, zz.data(), nl.data(), e3,
+                 project, ldab, ac, bc, ccv, mused, closed.data(), snorm.data(), tnorm.data(), mode, &ms))
+    return 1;
+  if (elapsed_ms) *elapsed_ms = ms;
+  return finish_blocks(op, nchain, nullptr, z, ldab, ac, bc, ccv, mused, closed.data(), snorm.data(), x);
 }
 
 int asr_gpu_lanczos_real_multi(int ngroup, asr_gpu_op* const* ops, const int* nchain, const int* start_orb,

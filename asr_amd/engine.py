@@ -51,6 +51,7 @@ def lib() -> C.CDLL:
         L.asr_gpu_chains_edc_multi.argtypes = [C.c_int, P, P, P, P, P, P, P, P, P, C.c_int, C.c_int, P, P, P, C.c_int, P]
         L.asr_gpu_lanczos_real_multi.argtypes = [C.c_int, P, P, P, P, P, P, P, C.c_double, C.c_int, P, P, P, C.c_int, P]
         L.asr_gpu_ed3_blocks.argtypes = [P, C.c_int, P, P, P, P, P, P, P, P, C.c_int, C.c_int, P, P, P, P, P, C.c_int, P]
+        L.asr_gpu_epi_blocks.argtypes = [P, C.c_int, P, P, P, P, P, P, P, P, P, C.c_int, C.c_int, P, P, P, P, P, C.c_int, P]
         L.asr_gpu_lanczos_adjoint.argtypes = [P, C.c_int, P, P, P, P, C.c_int, C.c_int, P, P, P, P, P, P, P]
         L.asr_gpu_profile_chains.argtypes = [P, C.c_int, P, P, P, P, P, P, P, C.c_int, P, P, P]
         L.asr_gpu_real_create.argtypes = [C.c_int, C.c_int, P, C.c_int, P, P, P, P, C.POINTER(C.c_void_p)]
@@ -245,6 +246,29 @@ class GpuOperator:
                                         int(project), ld, _p(bk), _p(a), _p(b), _p(c), _p(mu), int(mode), _p(ms)),
                "asr_gpu_ed3_blocks")
         return bk, a, b, c, mu, float(ms[0])
+
+    def epi_blocks(self, members: list[Member], v9, nlev, project: bool = True, mode: int = 0):
+        """EPI blocks (epi_blocks / epi_blocks_blk): members from asr_amd.epi.epi_member (edc parameters of epi_setup, site
+        0 frozen to the path point, frozen2 the partner's species, chi the characters of the start vector, z), v9 (nchain, 9)
+        unit start combinations on the start node.  Returns x (nchain, 4) = (X0, X1, X2, X1a - X1b), the chains' a, b, c
+        (nchain, 2, ld), mused (nchain, 2) and the device time in ms."""
+        nch = len(members)
+        nlev = _i32(np.broadcast_to(np.asarray(nlev), (nch,)))
+        ld = int(nlev.max())
+        v9 = _f64(np.broadcast_to(np.asarray(v9, np.float64), (nch, 9)))
+        chi = _f64(np.stack([np.asarray(m.chi, np.float64).reshape(self.op.nop) for m in members]))
+        fz2 = _i32([m.frozen2 for m in members])
+        x = np.zeros((nch, 4), np.complex128)
+        a = np.zeros((nch, 2, ld), np.complex128)
+        b = np.zeros((nch, 2, ld), np.complex128)
+        c = np.zeros((nch, 2, ld), np.complex128)
+        mu = np.zeros((nch, 2), np.int32)
+        ms = np.zeros(1)
+        _, p, f, cc, pab, z = pack_members(self.op, members, complex_route=True)
+        _check(lib().asr_gpu_epi_blocks(self.h, nch, _p(v9), _p(chi), _p(p), _p(pab), _p(f), _p(fz2), _p(cc), _p(z),
+                                        _p(nlev), int(project), ld, _p(x), _p(a), _p(b), _p(c), _p(mu), int(mode), _p(ms)),
+               "asr_gpu_epi_blocks")
+        return x, a, b, c, mu, float(ms[0])
 
     def profile(self, members: list[Member], nlev, project: bool = True, complex_route: bool = True):
         """Mean duration (ms) and kind of every kernel launch of one chain level."""

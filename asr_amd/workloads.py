@@ -170,6 +170,41 @@ def ternary_ed3(L: int = 8, levels: int = 4, depth: int = 30, seed: int = 2, gro
                          ed=eds))
 
 
+def fcc_epi_pair(L: int = 8, levels: int = 4, seed: int = 31, z: complex = complex(-0.2, 0.15), lam: float = 0.5,
+                 conc=(0.66, 0.14, 0.20), reorder: bool = True) -> Workload:
+    """EPI blocks of a first-shell pair on the fcc ternary: site 0 on the path between species 0 and 1 at lam (species 3
+    of the operator), the partner (first neighbour) frozen to species 0 or 2, the group the D2h operations that keep the
+    partner (a class of its own), the nine symmetry-adapted start vectors (epi_adapted) per partner species.  info: v9
+    (nchain, 9), par, pt, partner."""
+    from . import epi as EP
+    rng = np.random.default_rng(seed)
+    g, block, block0 = graph_and_blocks("fcc", L)
+    partner = int(g.nbr[0, 0])
+    cls = np.zeros(g.n, np.int32)
+    cls[partner] = 1
+    perm, mats = S.class_ops(g.coord, 2 * L, cls, "d2h")
+    dmat = S.orbital_rep(mats)
+    fl = np.ones(g.n, np.uint8)
+    fl[0] = 0
+    fl[partner] = 0
+    op = build_operator(g, cls, [3, 3], [[0, 1, 2], [0, 1, 2]], [list(conc), list(conc)], perm, dmat, 0, levels, maxrank=1,
+                        flippable=fl, block=block[None], block0=block0[None], nsp=4, partner=partner)
+    if reorder:
+        op = reorder_nodes(op)
+    par = [EP.epi_species(dict(enu=rng.uniform(-0.1, 0.1, 3), c=rng.uniform(-0.5, 0.5, 3), sqrdel=rng.uniform(0.05, 0.15, 3),
+                               o=rng.uniform(-0.05, 0.05, 3))) for _ in range(3)]
+    pt = EP.epi_path(par[0], par[1], lam)
+    vec, chi, lo = EP.epi_adapted(dmat)
+    members, v9 = [], []
+    for sp in (0, 2):
+        for a in range(9):
+            members.append(EP.epi_member(par, pt, int(lo[a]), chi[:, a], sp, z))
+            v9.append(vec[:, a])
+    return Workload("fcc ternary EPI pair", op, members, np.full(len(members), 30, np.int32), True,
+                    dict(L=L, levels=levels, seed=seed, nodes=op.nn, z=z, v9=np.array(v9), par=par, pt=pt, partner=partner,
+                         nop=op.nop))
+
+
 def ternary_adjoint(L: int = 8, levels: int = 4, nchains: int = 256, depth: int = 40, seed: int = 5, group: str = "d2h",
                     conc=(0.66, 0.14, 0.20), reorder: bool = True, offset: float = 0.02) -> Workload:
     """Real chains of the ternary operator for gradients: theta = (c, d) of shape (2, 3 species, 3 l shells).  Chain j

@@ -30,7 +30,9 @@ int asr_gpu_device_count(int* n);
  * ntype site types.  Replaces the allocation part of build_augcls_operator (src/asr_augcls.f90:709-727). */
 int asr_gpu_op_create(int nn, int nslot, int nop, int nsmax, int ncls, int nsp, int ntype, asr_gpu_op** out);
 /* Node tables (src/asr_augcls.f90:55-63): hopr/hopg nn x nslot, flipr/flipg nn x nsmax, lst/ncl/ntyp/fz/sq nn,
- * stab_ptr nn + 1, stab nstab.  fz[i] = 1: node i sits on the start site (augcls_freeze_site applies there, :129-141). */
+ * stab_ptr nn + 1, stab nstab.  fz[i] = 1: node i sits on the start site (augcls_freeze_site applies there, :129-141);
+ * fz[i] = 2: node i sits on a frozen pair partner (augcls_freeze_any, :147-161; the EPI pair operator); 0 elsewhere.
+ * Frozen sites must have been non-flippable when the node set was built. */
 int asr_gpu_op_set_tables(asr_gpu_op* op, const int* hopr, const int* hopg, const int* flipr, const int* flipg, const int* lst,
                           const int* ncl, const int* ntyp, const unsigned char* fz, const double* sq, const int* stab_ptr,
                           const int* stab, int nstab);
@@ -112,6 +114,22 @@ int asr_gpu_ed3_blocks(asr_gpu_op* op, int nchain, const int* start_orb, const d
                        const double* oc, const int* frozen_sp, const double* conc, const double* z, const int* nlev,
                        int project, int ldab, double* bk, double* a, double* b, double* c, int* mused, int mode,
                        double* elapsed_ms);
+
+/* Effective pair interactions: replaces epi_blocks / epi_blocks_blk (src/asr_epi.f90:628-647,740-759) with their start
+ * vectors (epi_start :578-587) and chains (chain / chain_blk :590-626,650-737) on the route-edc operator
+ * H(z) = E_nu + h - h o h + X(z) (augcls_apply + augcls_rand_c).  Per block j: the symmetry-adapted start combination
+ * v9 (9 values on the start node, unit norm) with its characters chi (nop values +-1 under the operator's group,
+ * augcls_set_chi), the chain from it projected on |v> = h'|v9> and the chain from |v> / |v| projected on it, both to full
+ * depth nlev.  params: nchain x 4 x nsp x 9 and pab: nchain x 2 x nsp x 9 as epi_setup (c, d = D_c^1/2, o, e = 0 and
+ * e_s(z) = z pa + pb, species nsp - 1 the path point); frozen_sp: species of the start site (fz slot 1, the path point);
+ * frozen_sp2: species of the pair partner (fz slot 2) or NULL; z: nchain complex.  x: nchain x 4 complex = (X0 = G_00,
+ * X1 = (hG)_00, X2 = (hGh)_00, X1a - X1b) for the trace of epi_integrand; a, b, c: nchain x 2 x ldab complex; mused:
+ * nchain x 2.  A chain that closed (b_k < 1e-12) gets no tail, the others the operator's tail.  mode as
+ * asr_gpu_ed3_blocks. */
+int asr_gpu_epi_blocks(asr_gpu_op* op, int nchain, const double* v9, const double* chi, const double* params,
+                       const double* pab, const int* frozen_sp, const int* frozen_sp2, const double* conc, const double* z,
+                       const int* nlev, int project, int ldab, double* x, double* a, double* b, double* c, int* mused,
+                       int mode, double* elapsed_ms);
 
 /* Gradient of a loss L(a, b) of real Lanczos chains with respect to the chain parameters (asr_gpu_adjoint of the
  * replacement interface).  The reference has no adjoint: its nearest analogue is the Hellmann-Feynman trace of EPI,

@@ -14,6 +14,8 @@
  *   orc_ed3_kmul         src/asr_ed3.f90:99-129       ed3_kmul
  *   orc_ed3_start        src/asr_ed3.f90:131-144      ed3_start
  *   orc_ed3_chain        src/asr_ed3.f90:148-211      chain of route ed3 without the adaptive depth: a, b, c, m, closure
+ *   orc_epi_start        src/asr_epi.f90:578-587      epi_start
+ *   orc_epi_chain        src/asr_epi.f90:590-626      chain of the EPI blocks (route edc operator, full depth)
  *   orc_real_apply       src/reference/asr_lmto_realization.f90:120-191   h_apply / real_apply (order 2), n species per site
  *   orc_kpm_moments      src/reference/asr_kpm.f90:39-65   kpm_moments (Rademacher vectors from a counter-based hash)
  *   orc_spectral_bounds  src/reference/asr_kpm.f90:14-36   Lanczos part (the eigenvalues are taken on the caller's side)
@@ -36,11 +38,17 @@ typedef struct {
 } orc_op;
 
 /* One chain's parameters on an operator: chi (nop), c/d/o/e (nsp x 9, row-major [species][orbital]), frozen species of the
- * start site (-1: none), mt (NULL: the operator's class tables). */
+ * start site (-1: none) and of a frozen pair partner (-1: none), mt (NULL: the operator's class tables).  fz[i] of the
+ * operator: 1 = node on the start site, 2 = node on the pair partner (augcls_freeze_any), 0 = neither. */
 typedef struct {
   const double *chi, *c, *d, *o, *e, *mt;
-  int frozen;
+  int frozen, frozen2;
 } orc_member;
+
+/* the frozen species that applies at node i (-1: none): op%frozen(i) of the reference */
+static inline int frozen_at(const orc_op *op, const orc_member *m, int i) {
+  return op->fz[i] == 1 ? m->frozen : op->fz[i] == 2 ? m->frozen2 : -1;
+}
 
 static inline const double *mt_of(const orc_op *op, const orc_member *m) { return m->mt ? m->mt : op->mt; }
 
@@ -64,8 +72,9 @@ void orc_apply_rand(const orc_op *op, const orc_member *m, const double *par, co
   orc_local_weights(op, mt_of(op, m), par, wt);
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < op->nn; ++i) {
-    if (m->frozen >= 0 && op->fz[i]) {
-      for (int l = 0; l < 9; ++l) w[9 * i + l] = par[m->frozen * 9 + l] * x[9 * i + l];
+    const int fsp = frozen_at(op, m, i);
+    if (fsp >= 0) {
+      for (int l = 0; l < 9; ++l) w[9 * i + l] = par[fsp * 9 + l] * x[9 * i + l];
       continue;
     }
     int kap = op->ncl[i], l0 = op->lst[i];
@@ -167,8 +176,9 @@ void orc_rand_c(const orc_op *op, const orc_member *m, const cplx *par, const cp
   for (int i = 0; i < op->nn; ++i) {
     cplx xi[9], acc[9];
     for (int l = 0; l < 9; ++l) xi[l] = x[9 * i + l] / op->sq[i];
-    if (m->frozen >= 0 && op->fz[i]) {
-      for (int l = 0; l < 9; ++l) w[9 * i + l] = par[m->frozen * 9 + l] * x[9 * i + l];
+    const int fsp = frozen_at(op, m, i);
+    if (fsp >= 0) {
+      for (int l = 0; l < 9; ++l) w[9 * i + l] = par[fsp * 9 + l] * x[9 * i + l];
       continue;
     }
     int kap = op->ncl[i], l0 = op->lst[i];
@@ -402,6 +412,68 @@ int orc_ed3_chain(const orc_op *op, const orc_member *m, const cplx *et, const c
     for (size_t q = 0; q < n; ++q) { up[q] = u[q]; u[q] = w[q] / bk; }
   }
   free(u); free(up); free(w); free(ct); free(wk); free(btab);
+  return mm;
+}
+
+/* Start vector of an EPI block (epi_start, src/asr_epi.f90:578-587): s0 = v9 on the start node, |v> = h'|s0> (projected
+ * if project), nv = sqrt(Re v.v).  s0, vv: 9 nn complex.  Returns 1 for a zero |v>. */
+int orc_epi_start(const orc_op *op, const orc_member *m, const double *v9, int project, cplx *s0, cplx *vv, double *nv) {
+  const size_t n = (size_t)9 * op->nn;
+  double *btab = make_btab(op);
+  double *wk = (double *)malloc(sizeof(double) * 8 * n);
+  for (size_t q = 0; q < n; ++q) s0[q] = 0;
+  for (int l = 0; l < 9; ++l) s0[l] = v9[l];
+  orc_hp_c(op, m, btab, s0, vv, wk);
+  if (project) orc_project_c(op, m, vv);
+  cplx sv = 0;
+  for (size_t q = 0; q < n; ++q) sv += vv[q] * vv[q];
+  *nv = sqrt(creal(sv));
+  free(wk); free(btab);
+  return *nv < 1.0e-14 ? 1 : 0;
+}
+
+/* One chain of an EPI block (chain, src/asr_epi.f90:590-626): complex-symmetric Lanczos on H(z) of route edc (augcls_apply
+ * on the real and imaginary parts plus augcls_rand_c with ep = E - D_c (E - E_nu,s) / D_s) from u0 (u0^T u0 = 1), full depth
+ * nlev: a, b, c(k) = tv^T u_k; *closed = 1 when b_k < 1e-12 ended it.  Returns m. */
+int orc_epi_chain(const orc_op *op, const orc_member *m, const cplx *ep, const cplx *u0, const cplx *tv, int nlev, int project,
+                  cplx *a, cplx *b, cplx *c, int *closed) {
+  const size_t dim = (size_t)9 * op->nn;
+  double *btab = make_btab(op);
+  cplx *u = (cplx *)malloc(sizeof(cplx) * dim), *up = (cplx *)calloc(dim, sizeof(cplx)), *w = (cplx *)malloc(sizeof(cplx) * dim);
+  cplx *dg = (cplx *)malloc(sizeof(cplx) * dim);
+  double *re = (double *)malloc(sizeof(double) * dim), *im = (double *)malloc(sizeof(double) * dim);
+  double *sre = (double *)malloc(sizeof(double) * dim), *sim = (double *)malloc(sizeof(double) * dim);
+  double *wk = (double *)malloc(sizeof(double) * 8 * dim);
+  for (int k = 0; k < nlev; ++k) { a[k] = 0; b[k] = 0; c[k] = 0; }
+  memcpy(u, u0, sizeof(cplx) * dim);
+  int mm = 0;
+  *closed = 0;
+  for (int k = 0; k < nlev; ++k) {
+    cplx s = 0;
+    for (size_t q = 0; q < dim; ++q) s += tv[q] * u[q];
+    c[k] = s;
+    for (size_t q = 0; q < dim; ++q) { re[q] = creal(u[q]); im[q] = cimag(u[q]); }
+    orc_augcls_apply(op, m, btab, re, sre, wk);
+    orc_augcls_apply(op, m, btab, im, sim, wk);
+    orc_rand_c(op, m, ep, u, dg);
+    for (size_t q = 0; q < dim; ++q) w[q] = CMPLX(sre[q], sim[q]) + dg[q];
+    s = 0;
+    for (size_t q = 0; q < dim; ++q) s += u[q] * w[q];
+    a[k] = s;
+    mm = k + 1;
+    if (k == nlev - 1) break;
+    for (size_t q = 0; q < dim; ++q) {
+      w[q] = w[q] - a[k] * u[q];
+      if (k > 0) w[q] = w[q] - b[k - 1] * up[q];
+    }
+    if (project) orc_project_c(op, m, w);
+    s = 0;
+    for (size_t q = 0; q < dim; ++q) s += w[q] * w[q];
+    b[k] = csqrt(s);
+    if (cabs(b[k]) < 1.0e-12) { *closed = 1; break; }
+    for (size_t q = 0; q < dim; ++q) { up[q] = u[q]; u[q] = w[q] / b[k]; }
+  }
+  free(u); free(up); free(w); free(dg); free(re); free(im); free(sre); free(sim); free(wk); free(btab);
   return mm;
 }
 

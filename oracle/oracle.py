@@ -44,6 +44,8 @@ def lib():
         L.orc_ed3_apply.argtypes = [P, P, P, P, P, P, P]
         L.orc_ed3_start.argtypes = [P, P, C.c_int, C.c_double, C.c_int, P, P, P, P]
         L.orc_ed3_chain.argtypes = [P, P, P, P, P, C.c_double, C.c_double, P, P, C.c_int, C.c_int, P, P, P, P]
+        L.orc_epi_start.argtypes = [P, P, P, C.c_int, P, P, P]
+        L.orc_epi_chain.argtypes = [P, P, P, P, P, C.c_int, C.c_int, P, P, P, P]
         L.orc_lanczos_batch.argtypes = [P, P, P, P, C.c_int, C.c_int, P, P, P]
         L.orc_edc_batch.argtypes = [P, P, P, P, P, C.c_int, C.c_int, C.c_int, P, P, P]
         L.orc_real_apply.argtypes = [P, P, P, P]
@@ -66,7 +68,7 @@ class _Op(C.Structure):
 
 
 class _Member(C.Structure):
-    _fields_ = [(n, C.c_void_p) for n in ("chi", "c", "d", "o", "e", "mt")] + [("frozen", C.c_int)]
+    _fields_ = [(n, C.c_void_p) for n in ("chi", "c", "d", "o", "e", "mt")] + [("frozen", C.c_int), ("frozen2", C.c_int)]
 
 
 class _Real(C.Structure):
@@ -97,8 +99,9 @@ class OracleOp:
             a = np.ascontiguousarray(a, dtype=np.float64)
             keep.append(a)
             return _p(a)
-        s = _Member(arr(self.op.chi_for(m.l0)), arr(m.c), arr(m.d), arr(m.o), arr(m.e),
-                    arr(m.mt(self.op)) if m.conc is not None else None, int(m.frozen))
+        chi = self.op.chi_for(m.l0) if getattr(m, "chi", None) is None else np.asarray(m.chi, float)
+        s = _Member(arr(chi), arr(m.c), arr(m.d), arr(m.o), arr(m.e),
+                    arr(m.mt(self.op)) if m.conc is not None else None, int(m.frozen), int(getattr(m, "frozen2", -1)))
         s._keep = keep
         return s
 
@@ -193,6 +196,37 @@ class OracleOp:
         fm = 0.5 * (q1a + q1b)
         bk = np.array([q0 - 2.0 * oc * fm + oc ** 2 * q2, fm - oc * q2, q2, q1a - q1b], np.complex128)
         return bk, chains, (na, nv)
+
+    def epi_blocks(self, m, ep, v9, z: complex, nlev: int, project: bool = True, tail: str = "hull"):
+        """epi_blocks (src/asr_epi.f90:628-647): chain from s0 = v9 projected on |v> = h'|s0>, chain from |v> / nv projected
+        on s0; x = (X0, X1, X2, X1a - X1b).  A chain that closed gets no tail.  Returns x and the chains' (a, b, c, m,
+        closed)."""
+        n = self.op.dim
+        ms = self.member(m)
+        v9 = np.ascontiguousarray(v9, dtype=np.float64)
+        s0 = np.zeros(n, np.complex128)
+        vv = np.zeros(n, np.complex128)
+        nv = C.c_double()
+        if lib().orc_epi_start(C.byref(self.s), C.byref(ms), _p(v9), int(project), _p(s0), _p(vv), C.byref(nv)):
+            raise ValueError("epi_start: zero start vector")
+        nv = nv.value
+        ep = np.ascontiguousarray(ep, dtype=np.complex128)
+        res, chains = [], []
+        for u0, tv in ((s0, vv), (vv / nv, s0)):
+            a = np.zeros(nlev, np.complex128)
+            b = np.zeros(nlev, np.complex128)
+            c = np.zeros(nlev, np.complex128)
+            closed = C.c_int()
+            mm = lib().orc_epi_chain(C.byref(self.s), C.byref(ms), _p(ep), _p(np.ascontiguousarray(u0)),
+                                     _p(np.ascontiguousarray(tv)), int(nlev), int(project), _p(a), _p(b), _p(c),
+                                     C.byref(closed))
+            g, f = np_cf_species(a, b, c, mm, z, "none" if closed.value else tail)
+            chains.append((a, b, c, mm, bool(closed.value)))
+            res.append((g, f))
+        (ga, fa), (gv, fv) = res
+        fb = nv * fv
+        x = np.array([ga, 0.5 * (fa + fb), nv ** 2 * gv, fa - fb], np.complex128)
+        return x, chains
 
     def lanczos_batch(self, members, iv, nlev):
         nch = len(members)

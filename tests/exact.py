@@ -141,3 +141,32 @@ def exact_average_asa(model, pp, z: complex, iv: int, frozen_species: int):
         gh += p * (h[iv, :] @ sol[:, 0])
         ghh += p * (h[:, iv] @ sol[:, 1])
     return g11, gh, ghh
+
+
+def pair_configurations(model, start_species: int, partner: int, partner_species: int):
+    """(species per site, probability) with site 0 and the pair partner frozen (the conditional average of the EPI)."""
+    for sp, p in configurations(model, start_species):
+        if sp[partner] != partner_species:
+            continue
+        kap = model["cls"][partner]
+        pr = model["conc"][kap][list(model["spc"][kap]).index(partner_species)]
+        yield sp, p / pr
+
+
+def exact_average_pair_blocks(model, sites_par, z: complex, v9, partner: int, partner_species: int, start_species: int):
+    """Conditional averages of X0 = v9^T G_00 v9, X1 = v9^T (hG)_00 v9, X2 = v9^T (hGh)_00 v9 with G = (z - H)^-1,
+    H = E_nu + h - h o h, h = c + d S d (unscaled, per-site species parameters sites_par[k] = (nsp, 9) arrays for
+    k = c, d, o, e; e = E_nu); site 0 carries start_species, the partner partner_species (brute force over all
+    configurations of the other sites)."""
+    s = dense_s(model)
+    x = np.zeros(3, np.complex128)
+    v = np.zeros(s.shape[0])
+    v[:9] = v9
+    for sp, p in pair_configurations(model, start_species, partner, partner_species):
+        c, d, o, e = (sites_par[k][sp].reshape(-1) for k in ("c", "d", "o", "e"))
+        h = d[:, None] * s * d[None, :] + np.diag(c)
+        hm = np.diag(e) + h - h @ np.diag(o) @ h
+        hv = h @ v
+        sol = np.linalg.solve(z * np.eye(h.shape[0]) - hm, np.stack([v, hv], axis=1))
+        x += p * np.array([v @ sol[:, 0], hv @ sol[:, 0], hv @ sol[:, 1]])
+    return x

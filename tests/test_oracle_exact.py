@@ -13,7 +13,7 @@ from asr_amd import augcls as A
 from asr_amd import host
 from oracle.oracle import OracleOp
 
-from exact import build_ring_operator, dense_operator, exact_average_asa, exact_average_g, ring_model
+from exact import build_ring_operator, dense_operator, exact_average_asa, exact_average_g, exact_average_pair_blocks, ring_model
 
 
 def rotated(model, ic):
@@ -217,3 +217,85 @@ def test_lanczos_full_reorthogonalisation():
         z = complex(e, 0.3)
         g = host.terminated_green(a, b, n, 0.0, 0.0, z)[0]
         assert abs(g - exact_average_g(model, par, z, 4, 0)) < 1.0e-10
+
+
+def _ring_pair_setup(seed: int = 91):
+    """Ring with site 0 (class {1, 2}) on a path between species 1 and 2 and the pair partner site 2 (class {1, 2}) frozen
+    to species 2; site 1 (ternary class) random; the path point is species 4."""
+    from asr_amd import epi as EP
+    from exact import ring_graph
+    model, _ = ring_model()
+    model = rotated(model, 1)
+    partner = 2
+    g = ring_graph(4)
+    fl = np.ones(4, np.uint8)
+    fl[0] = 0
+    fl[partner] = 0
+    op = A.build_operator(g, model["cls"], model["nsc"], model["spc"], model["conc"], np.arange(4, dtype=np.int32)[None, :],
+                          np.eye(9)[None], 0, 80, flippable=fl, block=model["block"], block0=model["block0"], nsp=5,
+                          partner=partner)
+    rng = np.random.default_rng(seed)
+    par = [EP.epi_species(dict(enu=rng.uniform(-0.1, 0.1, 3), c=rng.uniform(-0.4, 0.4, 3), sqrdel=rng.uniform(0.5, 0.8, 3),
+                               o=rng.uniform(-0.1, 0.1, 3))) for _ in range(4)]
+    pt = EP.epi_path(par[1], par[2], 0.37)
+    sites = {k: np.stack([getattr(p, k) for p in par] + [getattr(pt, k)]) for k in ("c", "d", "o", "e")}
+    return model, op, par, pt, sites, partner
+
+
+def test_ring_epi_blocks_equal_exact_average():
+    """The EPI blocks (epi_blocks, src/asr_epi.f90:628-647) on the ring with two frozen sites (site 0 on the path point,
+    the partner frozen to a species: augcls_freeze_any): the dense resolvent of the scaled augmented operator gives the
+    conditional average of X0 = v9^T G_00 v9 of the unscaled H = E_nu + h - h o h (1e-10), and the two projected chains
+    give X0, X1 = (hG)_00, X2 = (hGh)_00 (1e-6, exhausted Krylov space) with a vanishing cross term, for a random unit
+    start combination v9 (the identity group: chi = 1)."""
+    from asr_amd import epi as EP
+    model, op, par, pt, sites, partner = _ring_pair_setup()
+    O = OracleOp(op)
+    rng = np.random.default_rng(5)
+    assert set(np.unique(op.fz)) == {0, 1, 2}
+    err_dense = err_chain = err_cross = 0.0
+    for v9 in (np.eye(9)[4], rng.standard_normal(9)):
+        v9 = v9 / np.linalg.norm(v9)
+        for e in (-0.25, 0.2):
+            z = complex(e, 0.3)
+            m = EP.epi_member(par, pt, 4, np.ones(1), 2, z)
+            ep = z * m.pa + m.pb
+            dim = op.dim
+            k = np.zeros((dim, dim), np.complex128)
+            for j in range(dim):
+                u = np.zeros(dim)
+                u[j] = 1.0
+                k[:, j] = O.apply(m, u) + O.rand_c(m, ep, u.astype(np.complex128))
+            s0 = np.zeros(dim)
+            s0[:9] = v9
+            x0 = s0 @ np.linalg.solve(z * np.eye(dim) - k, s0)
+            exact = exact_average_pair_blocks(model, sites, z, v9, partner, 2, 4)
+            err_dense = max(err_dense, abs(x0 - exact[0]))
+            x, chains = O.epi_blocks(m, ep, v9, z, 1500, tail="none")
+            err_chain = max(err_chain, np.max(np.abs(x[:3] - exact)))
+            err_cross = max(err_cross, abs(x[3]))
+    assert err_dense < 1.0e-10
+    assert err_chain < 1.0e-6
+    assert err_cross < 1.0e-6
+    # a planted mistake (the partner frozen to the other species of its class) is detected
+    m = EP.epi_member(par, pt, 4, np.ones(1), 1, z)
+    x, _ = O.epi_blocks(m, z * m.pa + m.pb, v9, z, 1500, tail="none")
+    assert np.max(np.abs(x[:3] - exact)) > 1.0e-4
+
+
+def test_epi_adapted_vectors():
+    """epi_adapted (src/asr_epi.f90:290-328): for the abelian D2h the nine vectors are orthonormal, one l shell each, and
+    common eigenvectors of every operation with characters +-1; with the identity alone they are the harmonics."""
+    from asr_amd import epi as EP
+    from asr_amd import structure as S
+    g = S.periodic_graph("fcc", 2)
+    perm, mats = S.class_ops(g.coord, 4, np.zeros(g.n, np.int32), "d2h")
+    d = S.orbital_rep(mats)
+    vec, chi, lo = EP.epi_adapted(d)
+    assert np.allclose(vec.T @ vec, np.eye(9), atol=1e-12)
+    for a in range(9):
+        for gq in range(d.shape[0]):
+            assert np.allclose(d[gq] @ vec[:, a], chi[gq, a] * vec[:, a], atol=1e-10)
+    assert list(lo) == [0, 1, 1, 1, 4, 4, 4, 4, 4]
+    vec1, chi1, _ = EP.epi_adapted(np.eye(9)[None])
+    assert np.allclose(np.abs(vec1), np.eye(9)) and np.all(chi1 == 1.0)
