@@ -74,12 +74,13 @@ class AugclsOperator:
 def build_operator(graph: structure.Graph, cls: np.ndarray, nsc, spc, conc, perm: np.ndarray, dmat: np.ndarray, start: int,
                    maxlevel: int, maxrank: int = 0, flippable: np.ndarray | None = None, stype: np.ndarray | None = None,
                    block: np.ndarray | None = None, block0: np.ndarray | None = None, nsp: int | None = None,
-                   nthreads: int = 0, partner: int | None = None) -> AugclsOperator:
+                   nthreads: int = 0, partner: int | None = None, tsite=None) -> AugclsOperator:
     """Node set + tables (build_augcls_operator) and the orbital / class tables of the operator.  partner: a second
     non-flippable site whose nodes take a frozen species of their own (the partner of an EPI pair, augcls_freeze_any)."""
     if partner is not None and (flippable is None or flippable[partner] or partner == start):
         raise ValueError("build_augcls_operator: the frozen partner must be a non-flippable site other than the start")
-    t = host.build_tables(graph.nbr, cls, perm, start, maxlevel, nsc, maxrank=maxrank, flippable=flippable, nthreads=nthreads)
+    t = host.build_tables(graph.nbr, cls, perm, start, maxlevel, nsc, maxrank=maxrank, flippable=flippable, nthreads=nthreads,
+                          tsite=tsite)
     nn = t["lst"].size
     nslot = graph.nbr.shape[1]
     nop = perm.shape[0]
@@ -111,6 +112,10 @@ def build_operator(graph: structure.Graph, cls: np.ndarray, nsc, spc, conc, perm
     if nsp is None:
         nsp = int(max(max(s[:n]) for s, n in zip(spc, nsc))) + 1
     site = t["site"]
+    meta = {}
+    if tsite is not None:
+        # target nodes of the projected chains (tnode, tgop: (nt, nsmax), -1 = absent)
+        meta.update(tsite=np.asarray(tsite, np.int32), tnode=t["tnode"], tgop=t["tgop"])
     return AugclsOperator(
         nn=nn, nslot=nslot, nop=nop, nsmax=nsm, ncls=ncls, nsp=nsp, ntype=ntype,
         hopr=t["hopr"], hopg=t["hopg"], flipr=t["flipr"], flipg=t["flipg"], lst=t["lst"],
@@ -119,7 +124,8 @@ def build_operator(graph: structure.Graph, cls: np.ndarray, nsc, spc, conc, perm
         pi=pi, sg=sg, dmat=np.ascontiguousarray(dmat, dtype=np.float64),
         block=np.ascontiguousarray(block, dtype=np.float64), block0=np.ascontiguousarray(block0, dtype=np.float64),
         nsc=np.asarray(nsc, np.int32), spc=spc_a, conc=conc_a, mt=mt, start_site=int(start),
-        fz=(np.where(site == start, 1, 0) + (np.where(site == partner, 2, 0) if partner is not None else 0)).astype(np.uint8))
+        fz=(np.where(site == start, 1, 0) + (np.where(site == partner, 2, 0) if partner is not None else 0)).astype(np.uint8),
+        meta=meta)
 
 
 def reorder_nodes(op: AugclsOperator) -> AugclsOperator:
@@ -156,6 +162,9 @@ def reorder_nodes(op: AugclsOperator) -> AugclsOperator:
     import dataclasses
     meta = dict(op.meta)
     meta["node_order"] = "rcm of the hop links"
+    if "tnode" in meta:
+        tn = meta["tnode"]
+        meta["tnode"] = np.where(tn >= 0, inv[np.maximum(tn, 0)], -1).astype(np.int32)
     return dataclasses.replace(op, hopr=relink(op.hopr), hopg=np.ascontiguousarray(op.hopg[p]), flipr=relink(op.flipr),
                                flipg=np.ascontiguousarray(op.flipg[p]), lst=np.ascontiguousarray(op.lst[p]),
                                ncl=np.ascontiguousarray(op.ncl[p]), ntyp=np.ascontiguousarray(op.ntyp[p]),

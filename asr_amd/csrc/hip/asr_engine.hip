@@ -1281,6 +1281,29 @@ __global__ void __launch_bounds__(BLOCK) k_update(OpDev op, MemDev md, T* w, con
   }
 }
 
+// Projections of the chain vector on the target states (edc_green_proj, src/asr_edc.f90:197-204): for target t and
+// orbital l, cpr(t, l) = sum_ks tw(t, ks, l) chi(g) sg(l, g) u(r, pi(l, g)) / sq(r) with (r, g) = (tnode, tgop)(t, ks)
+// (augcls_amp_c).  One thread per (member, t, l); members fastest, so the reads of a node row coalesce.
+template <class T>
+__global__ void k_proj(OpDev op, MemDev md, const T* __restrict__ u, int nt, const int* __restrict__ tnode,
+                       const int* __restrict__ tgop, const double* __restrict__ tw, T* __restrict__ cpr) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  const int tl = blockIdx.y, t = tl / 9, l = tl % 9;
+  if (m >= md.nb) return;
+  const int nsm = op.nsmax, nbp = md.nbp;
+  const uint64_t chim = md.chimask[m];
+  T acc = zero<T>();
+  for (int ks = 0; ks < nsm; ++ks) {
+    const int r = tnode[t * nsm + ks];
+    if (r < 0) continue;
+    const int g = tgop[t * nsm + ks];
+    const double cs = (((chim >> g) & 1ull) ? -1.0 : 1.0) * op.sg[g * 9 + l] * op.isq[r];
+    const double w = tw[((size_t)(t * nsm + ks) * 9 + l) * nbp + m];
+    acc = fma_s(w * cs, u[((size_t)r * 9 + op.pi[g * 9 + l]) * nbp + m], acc);
+  }
+  cpr[(size_t)tl * nbp + m] = acc;
+}
+
 // Full reorthogonalisation of the real chains (lanczos with reorth, src/reference/asr_recursion.f90:44-47):
 // w = w - Q (Q^T w) against the stored vectors q_0..q_{nq-1} of every member (Q: nq vectors of qstride values).
 // k_qdots: per block partials of <q_j, w> for every j (plane j of part); k_qfin: their fixed-order sums (one block per
@@ -1752,6 +1775,8 @@ struct asr_gpu_op {
   double *sq = nullptr, *isq = nullptr, *sg = nullptr, *blk = nullptr, *blkp = nullptr;
   uint32_t* sgm = nullptr;
   int *stab_ptr = nullptr, *stab = nullptr, *pi = nullptr, *nsc = nullptr;
+  int ntarget = 0;                  // target states of the projected chains (asr_gpu_op_set_targets)
+  int *tnode = nullptr, *tgop = nullptr;  // ntarget x nsmax: node (-1 none) and operation of target t in state k
   hipStream_t stream = nullptr;
   bool have_tables = false, have_ops = false, have_blocks = false, have_classes = false;
   bool piid = false;                              // every operation maps each orbital onto itself
@@ -2235,11 +2260,14 @@ struct ChainBuffers {
   T *a = nullptr, *b = nullptr, *a_eff = nullptr, *b_eff = nullptr, *inv_b = nullptr;
   T *tv = nullptr, *part2 = nullptr, *c = nullptr;  // route ed3: projection vector, its dot partials, c_k
   double *qv = nullptr, *qpart = nullptr, *qcoef = nullptr;  // real chains with full reorthogonalisation
+  double* tw = nullptr;  // projected chains: dressing weights ntarget x nsmax x 9 x nbp
+  T* cpr = nullptr;      // projected chains: projections per level, ld x ntarget x 9 x nbp
   int *level = nullptr, *depth = nullptr, *active = nullptr, *mused = nullptr, *closed = nullptr;
   ~ChainBuffers() {
     for (void* p : {(void*)u, (void*)up, (void*)w, (void*)t1, (void*)t2, (void*)h1, (void*)part, (void*)a, (void*)b,
                     (void*)a_eff, (void*)b_eff, (void*)inv_b, (void*)level, (void*)depth, (void*)active, (void*)mused,
-                    (void*)tv, (void*)part2, (void*)c, (void*)closed, (void*)qv, (void*)qpart, (void*)qcoef})
+                    (void*)tv, (void*)part2, (void*)c, (void*)closed, (void*)qv, (void*)qpart, (void*)qcoef, (void*)tw,
+                    (void*)cpr})
       if (p) (void)hipFree(p);
   }
 };
@@ -2282,6 +2310,7 @@ struct ChainRun {
   std::vector<double> s0h;           // blocks: 9 x nbp start vectors on the start node
   bool reorth = false;               // real chains: full reorthogonalisation against every stored Lanczos vector
   size_t qplane = 0;
+  bool proj = false;                 // projected chains: projections of every level's vector on the target states
 
   ~ChainRun() {
     if (tact) (void)hipFree(tact);
@@ -2289,8 +2318,10 @@ struct ChainRun {
 
   int setup(asr_gpu_op* op_, hipStream_t s, int nch, const int* start_orb, const double* params, const double* pab,
             const int* frozen_sp, const double* conc, const double* z, const int* nlev, int project_, int ldab, int tile,
-            double tiny, bool real_chain, const BlockIn* e3 = nullptr) {
+            double tiny, bool real_chain, const BlockIn* e3 = nullptr, const double* tw_in = nullptr) {
     op = op_; stream = s; nchain = nch; project = project_; ld = ldab; lin = (pab != nullptr);
+    proj = (tw_in != nullptr);
+    if (proj && (op->ntarget <= 0 || !std::is_same<T, cplx>::value)) return fail("projected chains: no target states set");
     ed3 = (e3 != nullptr);
     if (ed3 && (!std::is_same<T, cplx>::value || !pab || !z)) return fail("chain blocks: complex chains with X(z) and z");
     if (!op->have_tables || !op->have_ops || !op->have_blocks || !op->have_classes)
@@ -2362,6 +2393,18 @@ struct ChainRun {
       if (dalloc(&B.qv, (size_t)maxdepth * nv) || dalloc(&B.qpart, (size_t)maxdepth * npart) ||
           dalloc(&B.qcoef, (size_t)maxdepth * M.nbp))
         return 1;
+    }
+    if (proj) {
+      // dressing weights of the sorted members, [t][ks][l][member]; projections [level][t][l][member]
+      const int nt = op->ntarget, nsm = op->nsmax;
+      std::vector<double> twd((size_t)nt * nsm * 9 * M.nbp, 0.0);
+      for (int q = 0; q < nch; ++q) {
+        const double* src = tw_in + (size_t)order[q] * nt * nsm * 9;
+        for (size_t e = 0; e < (size_t)nt * nsm * 9; ++e) twd[e * M.nbp + q] = src[e];
+      }
+      if (dalloc(&B.tw, twd.size()) || dalloc(&B.cpr, (size_t)ldab * nt * 9 * M.nbp)) return 1;
+      HCHECK(hipMemcpyAsync(B.tw, twd.data(), twd.size() * 8, hipMemcpyHostToDevice, stream));
+      HCHECK(hipMemsetAsync(B.cpr, 0, (size_t)ldab * nt * 9 * M.nbp * sizeof(T), stream));
     }
     for (T* q : {B.u, B.up, B.w, B.t1, B.t2, B.h1}) HCHECK(hipMemsetAsync(q, 0, nv * sizeof(T), stream));
     HCHECK(hipMemsetAsync(B.a, 0, (size_t)M.nbp * ldab * sizeof(T), stream));
@@ -2486,6 +2529,13 @@ struct ChainRun {
     md.tact = tact;
     if (g_timer) g_timer->ord = 0;
     if (reorth) HCHECK(hipMemcpyAsync(B.qv + (size_t)level_host * nv, B.u, nv * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    if (proj) {
+      const int nt = op->ntarget;
+      hipLaunchKernelGGL(k_proj<T>, dim3((M.nb + 255) / 256, nt * 9), dim3(256), 0, stream, dev_of(op), md, (const T*)B.u, nt,
+                         (const int*)op->tnode, (const int*)op->tgop, (const double*)B.tw,
+                         B.cpr + (size_t)level_host * nt * 9 * M.nbp);
+      HCHECK(hipGetLastError());
+    }
     if (apply_level_op<T>(op, M, md, B.u, B.w, B.t1, B.t2, B.h1, lin, B.u, B.part, stream)) return 1;
     if (timer_mark(stream, KIND_FIN_A, true)) return 1;
     hipLaunchKernelGGL(k_fin_a<T>, dim3(M.nb), dim3(BLOCK), 0, stream, cd, M.nb, B.part, nbx, apply_mt, B.a, B.a_eff);
@@ -2502,7 +2552,8 @@ struct ChainRun {
     if constexpr (!std::is_same<T, cplx>::value) return 0;
     else {
       const int k1 = level_host + 1;
-      if (!(op->depth_tol > 0.0) || !lin || k1 < 8 || (k1 % 4) != 0) return 0;
+      // the projected chains run all levels (edc_green_proj has no depth test)
+      if (proj || !(op->depth_tol > 0.0) || !lin || k1 < 8 || (k1 % 4) != 0) return 0;
       const int nb = M.nb;
       std::vector<cplx> ah((size_t)nb * ld), bh((size_t)nb * ld), ch(ed3 ? (size_t)nb * ld : 0);
       std::vector<int> act(nb), dep(nb);
@@ -2595,7 +2646,18 @@ struct ChainRun {
 
   // coefficients in the caller's chain order; route ed3 also c_k, the closure flags and the start / projection norms
   int finish(T* a_out, T* b_out, int* mused_out, T* c_out = nullptr, int* closed_out = nullptr, double* snorm_out = nullptr,
-             double* tnorm_out = nullptr) {
+             double* tnorm_out = nullptr, T* cpr_out = nullptr) {
+    if (proj && cpr_out) {
+      // projections in the caller's order: chain x level x target x orbital
+      const size_t per = (size_t)op->ntarget * 9;
+      std::vector<T> cp((size_t)ld * per * M.nbp);
+      HCHECK(hipMemcpyAsync(cp.data(), B.cpr, cp.size() * sizeof(T), hipMemcpyDeviceToHost, stream));
+      HCHECK(hipStreamSynchronize(stream));
+      for (int q = 0; q < nchain; ++q)
+        for (int k = 0; k < ld; ++k)
+          for (size_t e = 0; e < per; ++e)
+            cpr_out[((size_t)order[q] * ld + k) * per + e] = cp[((size_t)k * per + e) * M.nbp + q];
+    }
     std::vector<T> ah((size_t)M.nbp * ld), bh((size_t)M.nbp * ld), chv(ed3 ? (size_t)M.nbp * ld : 0);
     std::vector<int> mu(M.nbp), cl(ed3 ? M.nbp : 0);
     HCHECK(hipMemcpyAsync(ah.data(), B.a, ah.size() * sizeof(T), hipMemcpyDeviceToHost, stream));
@@ -2840,6 +2902,71 @@ int run_blocks(asr_gpu_op* op, int nmem, const int* so, const double* params, co
     HCHECK(hipEventElapsedTime(&ms, e0, e1));
     *elapsed_ms = ms;
     status = R.finish(a, b, mused, c, closed, snorm, tnorm);
+  }
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return status;
+}
+
+// Projected chains of route edc on one operator (the chains of asr_gpu_chains_edc_proj): mode 0 host-issued levels, 1 the
+// plain form (one chain per launch sequence), 2 the whole run as one HIP graph.  cpr: nchain x ldab x ntarget x 9.
+int run_proj(asr_gpu_op* op, int nchain, const int* so, const double* params, const double* pab, const int* fz,
+             const double* conc, const double* z, const int* nlev, const double* tw, int project, int ldab, cplx* a, cplx* b,
+             int* mused, cplx* cpr, int mode, double* elapsed_ms) {
+  const int nsp9 = op->nsp * 9, ncn = op->ncls * op->nsmax, ntw = op->ntarget * op->nsmax * 9;
+  *elapsed_ms = 0.0;
+  if (mode == 1 && nchain > 1) {
+    for (int q = 0; q < nchain; ++q) {
+      double ms = 0.0;
+      if (run_proj(op, 1, so + q, params + (size_t)q * 4 * nsp9, pab + (size_t)q * 2 * nsp9, fz ? fz + q : nullptr,
+                   conc ? conc + (size_t)q * ncn : nullptr, z + 2 * q, nlev + q, tw + (size_t)q * ntw, project, ldab,
+                   a + (size_t)q * ldab, b + (size_t)q * ldab, mused + q, cpr + (size_t)q * ldab * op->ntarget * 9, 1, &ms))
+        return 1;
+      *elapsed_ms += ms;
+    }
+    return 0;
+  }
+  ChainRun<cplx> R;
+  if (R.setup(op, op->stream, nchain, so, params, pab, fz, conc, z, nlev, project, ldab, mode == 1 ? 1 : op->tile_pref,
+              1.0e-12, false, nullptr, tw))
+    return 1;
+  hipEvent_t e0, e1;
+  HCHECK(hipEventCreate(&e0));
+  HCHECK(hipEventCreate(&e1));
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int status = 0;
+  if (mode == 2) {
+    HCHECK(hipStreamBeginCapture(op->stream, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < R.maxdepth && !status; ++k) status = R.step();
+    hipError_t ce = hipStreamEndCapture(op->stream, &graph);
+    if (!status && ce != hipSuccess) status = fail(std::string("graph capture: ") + hipGetErrorString(ce));
+    if (!status && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) status = fail("graph instantiate");
+    if (!status) {
+      HCHECK(hipEventRecord(e0, op->stream));
+      if (hipGraphLaunch(exec, op->stream) != hipSuccess) status = fail("graph launch");
+      HCHECK(hipEventRecord(e1, op->stream));
+    }
+  } else {
+    HCHECK(hipEventRecord(e0, op->stream));
+    for (int k = 0; k < R.maxdepth && !status; ++k) {
+      status = R.step();
+      if (!status && mode == 1) {
+        int actv = 0;
+        status = R.any_active(&actv);
+        if (!status && !actv) break;
+      }
+    }
+    HCHECK(hipEventRecord(e1, op->stream));
+  }
+  if (!status) {
+    HCHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HCHECK(hipEventElapsedTime(&ms, e0, e1));
+    *elapsed_ms = ms;
+    status = R.finish(a, b, mused, nullptr, nullptr, nullptr, nullptr, cpr);
   }
   if (exec) (void)hipGraphExecDestroy(exec);
   if (graph) (void)hipGraphDestroy(graph);
@@ -3181,6 +3308,73 @@ int asr_gpu_op_set_tile(asr_gpu_op* op, int mt) {
 }
 
 // The member tile the autotuner picked for nchain members (0 if no batched run of that size has happened).
+// Target states of the projected chains (tnode / tgop of build_augcls_operator with tsite, src/asr_augcls.f90:805-819):
+// ntarget x nsmax node indices (-1: none) and operations.
+int asr_gpu_op_set_targets(asr_gpu_op* op, int ntarget, const int* tnode, const int* tgop) {
+  if (!op || ntarget < 0 || (ntarget > 0 && (!tnode || !tgop))) return fail("asr_gpu_op_set_targets: arguments");
+  const size_t n = (size_t)ntarget * op->nsmax;
+  for (size_t q = 0; q < n; ++q)
+    if (tnode[q] >= op->nn || (tnode[q] >= 0 && (tgop[q] < 0 || tgop[q] >= op->nop)))
+      return fail("asr_gpu_op_set_targets: target node or operation out of range");
+  if (op->tnode) (void)hipFree(op->tnode);
+  if (op->tgop) (void)hipFree(op->tgop);
+  op->tnode = op->tgop = nullptr;
+  op->ntarget = 0;
+  if (ntarget == 0) return 0;
+  if (dalloc(&op->tnode, n) || dalloc(&op->tgop, n)) return 1;
+  HCHECK(hipMemcpy(op->tnode, tnode, n * 4, hipMemcpyHostToDevice));
+  HCHECK(hipMemcpy(op->tgop, tgop, n * 4, hipMemcpyHostToDevice));
+  op->ntarget = ntarget;
+  return 0;
+}
+
+// Projected chains of route edc (edc_green_proj, src/asr_edc.f90:174-232): the chains of asr_gpu_chains_edc run all nlev
+// levels, every level's vector projected on the target states with the dressing weights tw (nchain x ntarget x nsmax x 9,
+// spectral_dressing); on the host g = cf_species (the operator's tail) and gp = sum_k cpr_k pv_k, gp4 the same for the
+// chain cut four levels earlier.  g: nchain complex; gp, gp4: nchain x ntarget x 9 complex.
+int asr_gpu_chains_edc_proj(asr_gpu_op* op, int nchain, const int* start_orb, const double* params, const double* pab,
+                            const int* frozen_sp, const double* conc, const double* z, const int* nlev, int project, int ldab,
+                            const double* tw, double* a, double* b, int* mused, double* g, double* gp, double* gp4, int mode,
+                            double* elapsed_ms) {
+  if (nchain <= 0) return 0;
+  if (!op || !start_orb || !params || !pab || !z || !nlev || !tw || !a || !b || !mused || !g || !gp || !gp4)
+    return fail("asr_gpu_chains_edc_proj: null argument");
+  if (op->ntarget <= 0) return fail("asr_gpu_chains_edc_proj: no target states (asr_gpu_op_set_targets)");
+  if (mode < 0 || mode > 2) return fail("asr_gpu_chains_edc_proj: mode 0 (batched), 1 (plain) or 2 (graph)");
+  for (int j = 0; j < nchain; ++j)
+    if (nlev[j] < 1 || nlev[j] > ldab) return fail("asr_gpu_chains_edc_proj: 1 <= nlev <= ldab");
+  const int nt = op->ntarget;
+  std::vector<cplx> cpr((size_t)nchain * ldab * nt * 9);
+  double ms = 0.0;
+  cplx* ac = (cplx*)a;
+  cplx* bc = (cplx*)b;
+  if (run_proj(op, nchain, start_orb, params, pab, frozen_sp, conc, z, nlev, tw, project, ldab, ac, bc, mused, cpr.data(),
+               mode, &ms))
+    return 1;
+  if (elapsed_ms) *elapsed_ms = ms;
+  std::vector<double> c0(2 * (size_t)ldab, 0.0), pv(2 * (size_t)ldab);
+  for (int j = 0; j < nchain; ++j) {
+    const int m = mused[j];
+    const double* aj = (const double*)(ac + (size_t)j * ldab);
+    const double* bj = (const double*)(bc + (size_t)j * ldab);
+    for (int pass = 0; pass < 2; ++pass) {
+      const int mm = pass == 0 ? m : std::max(1, m - 4);
+      double gg[2], f[2];
+      if (asr_host_cf_species(aj, bj, c0.data(), mm, z + 2 * j, op->depth_tail, gg, f, nullptr, pv.data()) < 0)
+        return fail("asr_gpu_chains_edc_proj: continued fraction");
+      if (pass == 0) { g[2 * j] = gg[0]; g[2 * j + 1] = gg[1]; }
+      double* out = (pass == 0 ? gp : gp4) + (size_t)j * nt * 9 * 2;
+      for (int e = 0; e < nt * 9; ++e) {
+        cplx s{0.0, 0.0};
+        for (int k = 0; k < mm; ++k) s = s + cpr[((size_t)j * ldab + k) * nt * 9 + e] * cplx{pv[2 * k], pv[2 * k + 1]};
+        out[2 * e] = s.re;
+        out[2 * e + 1] = s.im;
+      }
+    }
+  }
+  return 0;
+}
+
 // Full reorthogonalisation of the real chains (lanczos reorth = .true., src/reference/asr_recursion.f90:17-58).
 int asr_gpu_op_set_reorth(asr_gpu_op* op, int reorth) {
   if (!op) return fail("asr_gpu_op_set_reorth: null operator");
@@ -3204,6 +3398,8 @@ int asr_gpu_op_get_tile(asr_gpu_op* op, int nchain, int is_complex, int* mt) {
 }
 
 void asr_gpu_op_destroy(asr_gpu_op* op) {
+  if (op && op->tnode) (void)hipFree(op->tnode);
+  if (op && op->tgop) (void)hipFree(op->tgop);
   if (!op) return;
   for (void* p : {(void*)op->isq, (void*)op->sgm, (void*)op->blkp}) if (p) (void)hipFree(p);
   for (void* p : {(void*)op->hop, (void*)op->flip, (void*)op->info, (void*)op->sq, (void*)op->sg, (void*)op->blk,

@@ -162,6 +162,10 @@ inline void set_state(const std::vector<int>& cfg, int r, int k, int nsm, std::v
 struct asr_augcls_tables {
   int nn = 0, nslot = 0, nop = 0, nsm = 0, nstab = 0;
   std::vector<int> hopr, hopg, flipr, flipg, lst, site, level, orb, stab_ptr, stab;
+  // the node store and the group, kept for the target lookups (tnode / tgop)
+  Store st;
+  int nsite = 0, start = 0;
+  std::vector<int> perm, nst;
 };
 
 extern "C" int asr_host_build_augcls(int nsite, const int* nbr_ptr, const int* nbr, const int* cls, int nop, const int* perm,
@@ -367,7 +371,40 @@ extern "C" int asr_host_build_augcls(int nsite, const int* nbr_ptr, const int* n
       }
     }
   }
+  T->nsite = nsite;
+  T->start = start;
+  T->perm.assign(perm, perm + (size_t)nop * nsite);
+  T->nst = nst;
+  T->st = std::move(st);
   *out = T;
+  return 0;
+}
+
+// Target nodes of build_augcls_operator (tsite / tnode / tgop, src/asr_augcls.f90:805-819): for target site tsite[t] and
+// local state k = 0 .. nsm - 1 (the site alone in state k, every other site in its reference state) tnode[t * nsm + k] is the
+// representative node and tgop[t * nsm + k] the operation that maps the node onto it; -1 when the node is not in the node
+// set, the state does not exist on the site, or the site cannot flip (the start site, a non-flippable site).
+extern "C" int asr_host_tables_targets(const asr_augcls_tables* T, int nt, const int* tsite, int* tnode, int* tgop) {
+  if (!T) return 1;
+  Canon cn;
+  std::vector<int> c2, cfg, tmp;
+  for (int t = 0; t < nt; ++t) {
+    const int r = tsite[t];
+    if (r < 0 || r >= T->nsite) return 1;
+    for (int k = 0; k < T->nsm; ++k) {
+      tnode[(size_t)t * T->nsm + k] = -1;
+      tgop[(size_t)t * T->nsm + k] = 0;
+      if (k > 0 && (k >= T->nst[r] || r == T->start)) continue;  // nst = 1 on a site that cannot flip
+      cfg.clear();
+      set_state(cfg, r, k, T->nsm, tmp);
+      canon_node(T->perm.data(), T->nsite, T->nop, T->nsm, r, tmp.data(), (int)tmp.size(), cn, c2);
+      const int f = T->st.find(cn.csite, cn.ccfg.data(), (int)cn.ccfg.size());
+      if (f >= 0) {
+        tnode[(size_t)t * T->nsm + k] = f;
+        tgop[(size_t)t * T->nsm + k] = cn.gbest;
+      }
+    }
+  }
   return 0;
 }
 

@@ -40,6 +40,8 @@ def lib() -> C.CDLL:
         L.asr_gpu_op_set_tile.argtypes = [P, C.c_int]
         L.asr_gpu_op_set_depth_tol.argtypes = [P, C.c_double, C.c_int]
         L.asr_gpu_op_set_reorth.argtypes = [P, C.c_int]
+        L.asr_gpu_op_set_targets.argtypes = [P, C.c_int, P, P]
+        L.asr_gpu_chains_edc_proj.argtypes = [P, C.c_int, P, P, P, P, P, P, P, C.c_int, C.c_int, P, P, P, P, P, P, P, C.c_int, P]
         L.asr_gpu_op_get_tile.argtypes = [P, C.c_int, C.c_int, P]
         L.asr_gpu_real_set_tile.argtypes = [P, C.c_int]
         L.asr_gpu_op_destroy.argtypes = [P]
@@ -142,6 +144,33 @@ class GpuOperator:
     def set_tile(self, mt: int):
         """Member tile of the batched chains (power of two <= 64; 0 = autotuned)."""
         _check(lib().asr_gpu_op_set_tile(self.h, int(mt)), "asr_gpu_op_set_tile")
+
+    def set_targets(self, tnode: np.ndarray, tgop: np.ndarray):
+        """Target states of the projected chains (tnode / tgop: (ntarget, nsmax), -1 = none)."""
+        tn, tg = _i32(tnode), _i32(tgop)
+        _check(lib().asr_gpu_op_set_targets(self.h, tn.shape[0], _p(tn), _p(tg)), "asr_gpu_op_set_targets")
+        self.ntarget = tn.shape[0]
+
+    def chains_edc_proj(self, members: list[Member], nlev, tw, project: bool = True, mode: int = 0):
+        """Projected chains (edc_green_proj) on the target states of set_targets: tw (nchain, ntarget, nsmax, 9)
+        dressing weights.  Returns g (nchain,), gp, gp4 (nchain, ntarget, 9), a, b, mused and the device time in ms."""
+        nch = len(members)
+        nlev = _i32(np.broadcast_to(np.asarray(nlev), (nch,)))
+        ld = int(nlev.max())
+        tw = _f64(tw)
+        nt = tw.shape[1]
+        a = np.zeros((nch, ld), np.complex128)
+        b = np.zeros((nch, ld), np.complex128)
+        mu = np.zeros(nch, np.int32)
+        g = np.zeros(nch, np.complex128)
+        gp = np.zeros((nch, nt, 9), np.complex128)
+        gp4 = np.zeros((nch, nt, 9), np.complex128)
+        ms = np.zeros(1)
+        s, p, f, c, pab, z = pack_members(self.op, members, complex_route=True)
+        _check(lib().asr_gpu_chains_edc_proj(self.h, nch, _p(s), _p(p), _p(pab), _p(f), _p(c), _p(z), _p(nlev), int(project),
+                                             ld, _p(tw), _p(a), _p(b), _p(mu), _p(g), _p(gp), _p(gp4), int(mode), _p(ms)),
+               "asr_gpu_chains_edc_proj")
+        return g, gp, gp4, a, b, mu, float(ms[0])
 
     def set_reorth(self, reorth: bool):
         """Full reorthogonalisation of the real chains (lanczos reorth = .true.; small cases)."""

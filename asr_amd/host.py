@@ -28,6 +28,7 @@ def lib() -> C.CDLL:
         L.asr_host_tables_free.restype = None
         L.asr_host_cf_species.argtypes = [P, P, P, C.c_int, P, C.c_int, P, P, P, P]
         L.asr_host_cf_closed.argtypes = [P, P, P, C.c_int, P, P, P]
+        L.asr_host_tables_targets.argtypes = [P, C.c_int, P, P, P]
         L.asr_host_bp_asymptote.argtypes = [P, P, C.c_int, P, P, P]
         L.asr_host_terminated_green.argtypes = [P, P, C.c_int, C.c_double, C.c_double, C.c_int, P, P]
         L.asr_host_kpm_dos.argtypes = [P, C.c_int, C.c_double, C.c_double, C.c_int, P, P]
@@ -44,8 +45,9 @@ def _p(a: np.ndarray):
 
 
 def build_tables(nbr: np.ndarray, cls: np.ndarray, perm: np.ndarray, start: int, maxlevel: int, nsc, maxrank: int = 0,
-                 flippable: np.ndarray | None = None, nthreads: int = 0) -> dict:
-    """Node set and node tables (asr_host_build_augcls).  nbr (nsite, nslot) with -1 = no neighbour."""
+                 flippable: np.ndarray | None = None, nthreads: int = 0, tsite=None) -> dict:
+    """Node set and node tables (asr_host_build_augcls).  nbr (nsite, nslot) with -1 = no neighbour.  tsite: target sites
+    whose single-site nodes are looked up (tnode, tgop: (nt, nsmax), -1 = absent; asr_host_tables_targets)."""
     nbr = np.ascontiguousarray(nbr, dtype=np.int32)
     nsite, nslot = nbr.shape
     ptr = (np.arange(nsite + 1, dtype=np.int32) * nslot).astype(np.int32)
@@ -73,6 +75,12 @@ def build_tables(nbr: np.ndarray, cls: np.ndarray, perm: np.ndarray, start: int,
                                                                "stab_ptr", "stab")])
         t["stab"] = t["stab"][:nstab]
         t["nsmax"] = nsm
+        if tsite is not None:
+            ts = np.ascontiguousarray(tsite, dtype=np.int32)
+            t["tnode"] = np.zeros((ts.size, nsm), np.int32)
+            t["tgop"] = np.zeros((ts.size, nsm), np.int32)
+            if lib().asr_host_tables_targets(handle, ts.size, _p(ts), _p(t["tnode"]), _p(t["tgop"])) != 0:
+                raise RuntimeError("asr_host_tables_targets: target site out of range")
         return t
     finally:
         lib().asr_host_tables_free(handle)

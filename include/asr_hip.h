@@ -53,6 +53,21 @@ int asr_gpu_op_set_tile(asr_gpu_op* op, int mt);
  * depth_tol = 0 (the default): every chain runs to its depth.  With depth_tol > 0 the levels are issued from the host
  * (one synchronisation per tested level; asr_gpu_chains_edc_multi mode 2 then runs as mode 0). */
 int asr_gpu_op_set_depth_tol(asr_gpu_op* op, double depth_tol, int tail);
+/* Target states of the projected chains: tnode / tgop of build_augcls_operator with tsite (src/asr_augcls.f90:805-819,
+ * asr_host_tables_targets): ntarget x nsmax representative nodes (-1: none) and the operations that map the target
+ * states onto them. */
+int asr_gpu_op_set_targets(asr_gpu_op* op, int ntarget, const int* tnode, const int* tgop);
+/* Projected chains of route edc for spectral functions: replaces edc_green_proj (src/asr_edc.f90:174-232).  The chains
+ * of asr_gpu_chains_edc (same arguments) run all nlev levels (no depth test); at every level the vector is projected on the
+ * target states with the dressing weights tw (nchain x ntarget x nsmax x 9, spectral_dressing, src/asr_spectral.f90:
+ * 208-228): cpr_k(t, l) = sum_ks tw(t, ks, l) psi_{u_k}(tnode(t, ks), l) (augcls_amp_c).  On the host g = G(z) of the
+ * chain with the operator's tail, gp(t, l) = sum_k cpr_k(t, l) pv_k (pv the first column of the terminated resolvent) and
+ * gp4 the same for the chain cut four levels earlier.  g: nchain complex; gp, gp4: nchain x ntarget x 9 complex; a, b:
+ * nchain x ldab complex.  mode 0 batched, 1 plain, 2 one HIP graph. */
+int asr_gpu_chains_edc_proj(asr_gpu_op* op, int nchain, const int* start_orb, const double* params, const double* pab,
+                            const int* frozen_sp, const double* conc, const double* z, const int* nlev, int project, int ldab,
+                            const double* tw, double* a, double* b, int* mused, double* g, double* gp, double* gp4, int mode,
+                            double* elapsed_ms);
 /* Full reorthogonalisation of the real chains (lanczos with reorth = .true., src/reference/asr_recursion.f90:17-58): every
  * level projects the new vector twice against all stored Lanczos vectors of its chain, w = w - Q (Q^T w), before b_n is
  * taken.  The stored vectors cost depth x (nn + 1) x 9 x members doubles (at most 64 GiB): for tests and small cases. */

@@ -25,6 +25,10 @@ int asr_host_tables_dims(const asr_augcls_tables* t, int* dims);
 /* hopr/hopg: nn x nslot, flipr/flipg: nn x nsmax, lst/site/level/orb: nn, stab_ptr: nn + 1, stab: nstab */
 int asr_host_tables_get(const asr_augcls_tables* t, int* hopr, int* hopg, int* flipr, int* flipg, int* lst, int* site,
                         int* level, int* orb, int* stab_ptr, int* stab);
+/* Target nodes (build_augcls_operator with tsite, src/asr_augcls.f90:805-819): for target site tsite[t] and local state k
+ * (the site alone in state k, every other site in its reference state) tnode[t * nsmax + k] = representative node,
+ * tgop[t * nsmax + k] = the operation that maps the node onto it (augcls_amp_c); -1 = not in the node set. */
+int asr_host_tables_targets(const asr_augcls_tables* t, int nt, const int* tsite, int* tnode, int* tgop);
 void asr_host_tables_free(asr_augcls_tables* t);
 
 /* Terminated continued fraction of a complex-symmetric chain: cf_species (src/asr_chain.f90:67-167).

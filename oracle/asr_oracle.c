@@ -16,6 +16,7 @@
  *   orc_ed3_chain        src/asr_ed3.f90:148-211      chain of route ed3 without the adaptive depth: a, b, c, m, closure
  *   orc_epi_start        src/asr_epi.f90:578-587      epi_start
  *   orc_epi_chain        src/asr_epi.f90:590-626      chain of the EPI blocks (route edc operator, full depth)
+ *   orc_edc_chain_proj   src/asr_edc.f90:174-232      edc_green_proj: the chain and its projections on target states
  *   orc_real_apply       src/reference/asr_lmto_realization.f90:120-191   h_apply / real_apply (order 2), n species per site
  *   orc_kpm_moments      src/reference/asr_kpm.f90:39-65   kpm_moments (Rademacher vectors from a counter-based hash)
  *   orc_spectral_bounds  src/reference/asr_kpm.f90:14-36   Lanczos part (the eigenvalues are taken on the caller's side)
@@ -471,6 +472,59 @@ int orc_epi_chain(const orc_op *op, const orc_member *m, const cplx *ep, const c
     for (size_t q = 0; q < dim; ++q) s += w[q] * w[q];
     b[k] = csqrt(s);
     if (cabs(b[k]) < 1.0e-12) { *closed = 1; break; }
+    for (size_t q = 0; q < dim; ++q) { up[q] = u[q]; u[q] = w[q] / b[k]; }
+  }
+  free(u); free(up); free(w); free(dg); free(re); free(im); free(sre); free(sim); free(wk); free(btab);
+  return mm;
+}
+
+/* Projected chain of route edc (edc_green_proj, src/asr_edc.f90:174-232) without the fraction: the chain of edc_green run
+ * all nlev steps (no depth test) and, at every level k before the operator, the projections of u_k on the target states
+ *   cpr[k][t][l] = sum_ks tw[t][ks][l] amp_l(u_k, tnode[t][ks], tgop[t][ks]),  amp_l = chi(g) sg(l, g) u(9 r + pi(l, g)) / sq(r)
+ * (augcls_amp_c; tnode < 0: no such node).  cpr: nlev x nt x 9 complex.  Returns m. */
+int orc_edc_chain_proj(const orc_op *op, const orc_member *m, const cplx *ep, int iv, int nlev, int project, int nt,
+                       const int *tnode, const int *tgop, const double *tw, cplx *a, cplx *b, cplx *cpr) {
+  const size_t dim = (size_t)9 * op->nn;
+  const int nsm = op->nsmax;
+  double *btab = make_btab(op);
+  cplx *u = (cplx *)calloc(dim, sizeof(cplx)), *up = (cplx *)calloc(dim, sizeof(cplx)), *w = (cplx *)malloc(sizeof(cplx) * dim);
+  cplx *dg = (cplx *)malloc(sizeof(cplx) * dim);
+  double *re = (double *)malloc(sizeof(double) * dim), *im = (double *)malloc(sizeof(double) * dim);
+  double *sre = (double *)malloc(sizeof(double) * dim), *sim = (double *)malloc(sizeof(double) * dim);
+  double *wk = (double *)malloc(sizeof(double) * 8 * dim);
+  for (int k = 0; k < nlev; ++k) { a[k] = 0; b[k] = 0; }
+  for (size_t q = 0; q < (size_t)nlev * nt * 9; ++q) cpr[q] = 0;
+  u[iv] = 1.0;
+  int mm = 0;
+  for (int k = 0; k < nlev; ++k) {
+    for (int t = 0; t < nt; ++t)
+      for (int ks = 0; ks < nsm; ++ks) {
+        const int r = tnode[t * nsm + ks], g = tgop[t * nsm + ks];
+        if (r < 0) continue;
+        for (int l = 0; l < 9; ++l) {
+          const cplx amp = m->chi[g] * op->sg[g * 9 + l] * u[9 * (size_t)r + op->pi[g * 9 + l]] / op->sq[r];
+          cpr[((size_t)k * nt + t) * 9 + l] += tw[((size_t)t * nsm + ks) * 9 + l] * amp;
+        }
+      }
+    for (size_t q = 0; q < dim; ++q) { re[q] = creal(u[q]); im[q] = cimag(u[q]); }
+    orc_augcls_apply(op, m, btab, re, sre, wk);
+    orc_augcls_apply(op, m, btab, im, sim, wk);
+    orc_rand_c(op, m, ep, u, dg);
+    for (size_t q = 0; q < dim; ++q) w[q] = CMPLX(sre[q], sim[q]) + dg[q];
+    cplx s = 0;
+    for (size_t q = 0; q < dim; ++q) s += u[q] * w[q];
+    a[k] = s;
+    mm = k + 1;
+    if (k == nlev - 1) break;
+    for (size_t q = 0; q < dim; ++q) {
+      w[q] = w[q] - a[k] * u[q];
+      if (k > 0) w[q] = w[q] - b[k - 1] * up[q];
+    }
+    if (project) orc_project_c(op, m, w);
+    s = 0;
+    for (size_t q = 0; q < dim; ++q) s += w[q] * w[q];
+    b[k] = csqrt(s);
+    if (cabs(b[k]) < 1.0e-12) break;
     for (size_t q = 0; q < dim; ++q) { up[q] = u[q]; u[q] = w[q] / b[k]; }
   }
   free(u); free(up); free(w); free(dg); free(re); free(im); free(sre); free(sim); free(wk); free(btab);

@@ -46,6 +46,7 @@ def lib():
         L.orc_ed3_chain.argtypes = [P, P, P, P, P, C.c_double, C.c_double, P, P, C.c_int, C.c_int, P, P, P, P]
         L.orc_epi_start.argtypes = [P, P, P, C.c_int, P, P, P]
         L.orc_epi_chain.argtypes = [P, P, P, P, P, C.c_int, C.c_int, P, P, P, P]
+        L.orc_edc_chain_proj.argtypes = [P, P, P, C.c_int, C.c_int, C.c_int, C.c_int, P, P, P, P, P, P]
         L.orc_lanczos_batch.argtypes = [P, P, P, P, C.c_int, C.c_int, P, P, P]
         L.orc_edc_batch.argtypes = [P, P, P, P, P, C.c_int, C.c_int, C.c_int, P, P, P]
         L.orc_real_apply.argtypes = [P, P, P, P]
@@ -228,6 +229,28 @@ class OracleOp:
         x = np.array([ga, 0.5 * (fa + fb), nv ** 2 * gv, fa - fb], np.complex128)
         return x, chains
 
+    def edc_green_proj(self, m, ep, iv: int, nlev: int, tnode, tgop, tw, z: complex, project: bool = True,
+                       tail: str = "hull"):
+        """edc_green_proj (src/asr_edc.f90:174-232): g and the projections gp[t, l] = sum_k cpr[k, t, l] pv_k of the chain
+        terminated at m, gp4 the same at max(1, m - 4).  tnode, tgop: (nt, nsmax); tw: (nt, nsmax, 9) dressing weights."""
+        ep = np.ascontiguousarray(ep, dtype=np.complex128)
+        tn = np.ascontiguousarray(tnode, dtype=np.int32)
+        tg = np.ascontiguousarray(tgop, dtype=np.int32)
+        twa = np.ascontiguousarray(tw, dtype=np.float64)
+        nt = tn.shape[0]
+        a = np.zeros(nlev, np.complex128)
+        b = np.zeros(nlev, np.complex128)
+        cpr = np.zeros((nlev, nt, 9), np.complex128)
+        ms = self.member(m)
+        mm = lib().orc_edc_chain_proj(C.byref(self.s), C.byref(ms), _p(ep), int(iv), int(nlev), int(project), nt, _p(tn),
+                                      _p(tg), _p(twa), _p(a), _p(b), _p(cpr))
+        g, _, pv = np_cf_species(a, b, np.zeros(nlev), mm, z, tail, want_pv=True)
+        gp = np.einsum("ktl,k->tl", cpr[:mm], pv[:mm])
+        m4 = max(1, mm - 4)
+        _, _, pv4 = np_cf_species(a, b, np.zeros(nlev), m4, z, tail, want_pv=True)
+        gp4 = np.einsum("ktl,k->tl", cpr[:m4], pv4[:m4])
+        return g, gp, gp4, (a, b, mm, cpr)
+
     def lanczos_batch(self, members, iv, nlev):
         nch = len(members)
         ld = int(max(nlev))
@@ -337,7 +360,7 @@ def _chain_matrix(a, b, m, am=None):
     return t
 
 
-def np_cf_species(a, b, c, m, z, tail="hull"):
+def np_cf_species(a, b, c, m, z, tail="hull", want_pv=False):
     a = np.asarray(a, dtype=np.complex128)
     b = np.asarray(b, dtype=np.complex128)
     c = np.asarray(c, dtype=np.complex128)
@@ -383,10 +406,14 @@ def np_cf_species(a, b, c, m, z, tail="hull"):
         gt[j] = 1.0 / (z - a[j] - b[j] ** 2 * gt[j + 1])
     f = 0.0
     prod = gt[0]
+    pv = np.zeros(m, np.complex128)
     for k in range(m):
+        pv[k] = prod
         f += c[k] * prod
         if k < m - 1:
             prod = prod * b[k] * gt[k + 1]
+    if want_pv:
+        return gt[0], f, pv
     return gt[0], f
 
 

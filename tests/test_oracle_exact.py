@@ -299,3 +299,49 @@ def test_epi_adapted_vectors():
     assert list(lo) == [0, 1, 1, 1, 4, 4, 4, 4, 4]
     vec1, chi1, _ = EP.epi_adapted(np.eye(9)[None])
     assert np.allclose(np.abs(vec1), np.eye(9)) and np.all(chi1 == 1.0)
+
+
+def test_ring_projected_chain_equals_exact_off_site_average():
+    """edc_green_proj (src/asr_edc.f90:174-232) with the dressing weights of spectral_dressing: on the ring (start site
+    frozen to each species of its class, every target site and state) the projections of the exhausted chain give the
+    conditional averages <[(z - H)^-1]_{0 iv, t l}> of the unscaled H = E_nu + h - h o h (brute force, 1e-7), the
+    target = centre reproduces g; a planted mistake (the dressing without T_s) is detected."""
+    from asr_amd import spectral as SP
+    from asr_amd import workloads as W
+    from exact import ring_graph, configurations, dense_s
+    model, _ = ring_model()
+    model = rotated(model, 1)       # site 0: binary class {1, 2}; sites 1 (ternary), 2 (binary), 3 (ordered)
+    g = ring_graph(4)
+    fl = np.ones(4, np.uint8)
+    fl[0] = 0
+    op = A.build_operator(g, model["cls"], model["nsc"], model["spc"], model["conc"], np.arange(4, dtype=np.int32)[None, :],
+                          np.eye(9)[None], 0, 80, flippable=fl, block=model["block"], block0=model["block0"], nsp=4,
+                          tsite=[0, 1, 2, 3])
+    O = OracleOp(op)
+    rng = np.random.default_rng(17)
+    pp = dict(c=rng.uniform(-0.4, 0.4, (4, 3)), sqrdel=rng.uniform(0.5, 0.8, (4, 3)), enu=rng.uniform(-0.1, 0.1, (4, 3)),
+              o=rng.uniform(-0.1, 0.05, (4, 3)))
+    orb = {k: W.per_orbital(v) for k, v in pp.items()}
+    s = dense_s(model)
+    iv, z = 2, complex(0.1, 0.35)
+    for cen in (1, 2):
+        m = W.edc_member(pp, cen, iv, z)
+        dc = orb["sqrdel"][cen] ** 2
+        tw = np.stack([SP.spectral_dressing(dc, orb["sqrdel"] ** 2, op.spc[op.ncl[0] if t == 0 else model["cls"][t]],
+                                            op.conc[model["cls"][t]], t == 0, op.nsmax) for t in range(4)])
+        gg, gp, gp4, _ = O.edc_green_proj(m, z * m.pa + m.pb, iv, 1500, op.meta["tnode"], op.meta["tgop"], tw, z, tail="none")
+        exact = np.zeros((4, 9), np.complex128)
+        for sp, p in configurations(model, cen):
+            c, d, o, e = (orb[k][sp].reshape(-1) for k in ("c", "sqrdel", "o", "enu"))
+            h = d[:, None] * s * d[None, :] + np.diag(c)
+            hm = np.diag(e) + h - h @ np.diag(o) @ h
+            rhs = np.zeros(36, np.complex128)
+            rhs[iv] = 1.0
+            exact += p * np.linalg.solve(z * np.eye(36) - hm, rhs).reshape(4, 9)
+        assert np.max(np.abs(gp - exact)) < 1e-7, (cen, np.max(np.abs(gp - exact)))
+        assert abs(gp[0, iv] - gg) < 1e-9
+        bad = tw.copy()
+        bad[1:, :, :] = np.stack([SP.spectral_dressing(np.ones(9), np.ones((4, 9)), op.spc[model["cls"][t]],
+                                                       op.conc[model["cls"][t]], False, op.nsmax) for t in range(1, 4)])
+        _, gpb, _, _ = O.edc_green_proj(m, z * m.pa + m.pb, iv, 1500, op.meta["tnode"], op.meta["tgop"], bad, z, tail="none")
+        assert np.max(np.abs(gpb - exact)) > 1e-4
