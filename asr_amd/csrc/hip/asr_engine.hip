@@ -74,6 +74,7 @@ __device__ inline cplx shfl_xor(cplx v, int m) { return {__shfl_xor(v.re, m), __
 // ------------------------------------------------------------------------------------------------ device operator
 struct OpDev {
   int nn, nslot, nop, nsmax, ncls, nsp, ntype, nstab;
+  int piid;              // every operation maps each orbital onto itself (pi = identity)
   const uint32_t* hop;   // nn x nslot: (node + 1) << 6 | op, 0 = no link
   const uint32_t* flip;  // nn x nsmax, same packing
   const uint32_t* info;  // nn: lst | ncl << 4 | ntyp << 12 | fz << 20 (fz: frozen-site slot 0 none, 1 start, 2 partner)
@@ -1245,7 +1246,18 @@ __global__ void __launch_bounds__(BLOCK) k_update(OpDev op, MemDev md, T* w, con
     }
     if (project) {
       int s0 = op.stab_ptr[i], s1 = op.stab_ptr[i + 1], ns = s1 - s0;
-      if (ns > 1) {
+      if (std::is_same<T, cplx>::value && ns > 1 && op.piid) {
+        // every operation keeps each orbital: (P x)_l = x_l sum_g chi(g) sg(g, l) / |S| (no re-read of the row)
+        const uint64_t chim = md.chimask[m];
+        double f[9];
+        for (int l = 0; l < 9; ++l) f[l] = 0.0;
+        for (int j = s0; j < s1; ++j) {
+          const int g = op.stab[j];
+          const double chi = ((chim >> g) & 1ull) ? -1.0 : 1.0;
+          for (int l = 0; l < 9; ++l) f[l] += chi * op.sg[g * 9 + l];
+        }
+        for (int l = 0; l < 9; ++l) x[l] = (f[l] / (double)ns) * x[l];
+      } else if (ns > 1) {
         const uint64_t chim = md.chimask[m];
         T acc[9];
         for (int l = 0; l < 9; ++l) acc[l] = zero<T>();
@@ -1815,6 +1827,7 @@ OpDev dev_of(const asr_gpu_op* op) {
   d.hop = op->hop; d.flip = op->flip; d.info = op->info; d.sq = op->sq; d.stab_ptr = op->stab_ptr; d.stab = op->stab;
   d.isq = op->isq; d.sgm = op->sgm; d.blkp = op->blkp;
   d.pi = op->pi; d.sg = op->sg; d.blk = op->blk; d.nsc = op->nsc;
+  d.piid = op->piid ? 1 : 0;
   return d;
 }
 

@@ -295,6 +295,13 @@ def main():
         t0 = time.perf_counter()
         res = R.kpm_moments(cshift, sscale, nmom, 64, seed=5)
         wk = time.perf_counter() - t0
+        # the Jackson-kernel DOS on 4001 points inside the scaled window (kpm_dos, on the host)
+        from asr_amd import host as H
+        t0 = time.perf_counter()
+        egrid = cshift + sscale * 0.99 * np.linspace(-1.0, 1.0, 4001)
+        rho = H.kpm_dos(res["mu"], cshift, sscale, egrid)
+        tdos = time.perf_counter() - t0
+        dos_norm = float(np.sum(rho) * (egrid[1] - egrid[0]))
         kby, kfl = kpm_kernel_cost(g.n, g.nbr.shape[1], 64)
         kroof = roofline(kby, kfl, res["step_ms"])
         kroof["kernel"] = {1: "k_kpm_u<false, %d>", 2: "k_kpm_v2<%d>", 3: "k_kpm_s16<%d>"}[vt_best] % g.nbr.shape[1]
@@ -305,7 +312,8 @@ def main():
                       "vector_tile": vt_best,
                       "bounds": {"emin": round(elo - marg, 6), "emax": round(ehi + marg, 6), "lanczos_steps": 80,
                                  "ms": round(tb_ms, 1)},
-                      "mu_head": [round(float(v), 8) for v in res["mu"][:4]]}
+                      "mu_head": [round(float(v), 8) for v in res["mu"][:4]],
+                      "dos": {"points": 4001, "ms": round(tdos * 1e3, 2), "integral": round(dos_norm, 6)}}
         log(f"kpm: {nmom} moments x 64 vectors on {g.n * 9} rows in {wk:.2f} s, step {res['step_ms']:.3f} ms, {kroof}")
         R.close()
 
