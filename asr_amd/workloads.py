@@ -116,7 +116,11 @@ def edc_member(pp: dict, central: int, l0: int, z: complex) -> Member:
 
 
 def ternary_edc(L: int = 8, levels: int = 4, depth: int = 30, seed: int = 2, group: str = "d2h",
-                z: complex = complex(-0.2, 0.15), conc=(0.66, 0.14, 0.20), reorder: bool = True) -> Workload:
+                z: complex = complex(-0.2, 0.15), conc=(0.66, 0.14, 0.20), reorder: bool = True,
+                targets: bool = False) -> Workload:
+    """Route edc on the fcc ternary operator: 2 spins x 3 central species x 9 start orbitals.  targets: also the target
+    nodes of the start site and its 18 neighbours (op.meta tnode / tgop) and per chain the dressing weights
+    (info tw: (nchain, 19, nsmax, 9)) for the projected chains."""
     rng = np.random.default_rng(seed)
     g, block, block0 = graph_and_blocks("fcc", L)
     cls = np.zeros(g.n, np.int32)
@@ -124,14 +128,25 @@ def ternary_edc(L: int = 8, levels: int = 4, depth: int = 30, seed: int = 2, gro
     dmat = S.orbital_rep(mats)
     fl = np.ones(g.n, np.uint8)
     fl[0] = 0                                    # the start site carries the central species (never flips)
+    tsite = [0] + [int(v) for v in g.nbr[0]] if targets else None
     op = build_operator(g, cls, [3], [[0, 1, 2]], [list(conc)], perm, dmat, 0, levels, maxrank=1, flippable=fl,
-                        block=block[None], block0=block0[None], nsp=3)
+                        block=block[None], block0=block0[None], nsp=3, tsite=tsite)
     if reorder:
         op = reorder_nodes(op)
     spins = edc_params(rng, 3, 2)
     members = [edc_member(spins[sp], cen, l0, z) for sp in range(2) for cen in range(3) for l0 in range(9)]
-    return Workload("fcc ternary edc", op, members, np.full(len(members), depth, np.int32), True,
-                    dict(L=L, levels=levels, group=group, seed=seed, nodes=op.nn, z=z))
+    info = dict(L=L, levels=levels, group=group, seed=seed, nodes=op.nn, z=z)
+    if targets:
+        from . import spectral as SP
+        tw = []
+        for sp in range(2):
+            dl = per_orbital(spins[sp]["sqrdel"] ** 2)
+            for cen in range(3):
+                dc = dl[cen]
+                w = np.stack([SP.spectral_dressing(dc, dl, op.spc[0], op.conc[0], t == 0, op.nsmax) for t in range(len(tsite))])
+                tw += [w] * 9
+        info.update(tsite=np.asarray(tsite), tw=np.array(tw))
+    return Workload("fcc ternary edc", op, members, np.full(len(members), depth, np.int32), True, info)
 
 
 def ternary_ed3(L: int = 8, levels: int = 4, depth: int = 30, seed: int = 2, group: str = "d2h",

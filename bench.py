@@ -4,8 +4,8 @@ Headline: chain-levels/s of the batched complex-symmetric (route edc) recursion 
 (Fe0.66Ni0.14Cr0.20-like, spd, 2 shells = 18 slots, levels 4, rank 1, D2h-reduced node set, start site frozen to each
 species, 2 spins: 9 x 3 x 2 = 54 chains of depth 30, FP64), next to the plain form (one chain per launch sequence, host
 synchronisation per level) on the same chains, the KPM moments of the 40^3-cell fcc supercell (2,304,000 rows, 64
-Rademacher vectors, 2048 moments), route ed3 on the same operator, the adjoint, and the oracle's CPU chains on the host
-cores.
+Rademacher vectors, 2048 moments), route ed3 on the same operator, the EPI pair blocks, the projected chains of the
+spectral functions, the adjoint, and the oracle's CPU chains on the host cores.
 
 A step is one batched pass of all 54 chains to depth 30; inputs (operator tables, start vectors) are resident on the GPU
 when the timed region starts.  python bench.py [--steps K] [--warmup W] [--quick]
@@ -138,6 +138,8 @@ def main():
     ap.add_argument("--no-l21", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-ed3", action="store_true")
+    ap.add_argument("--no-epi", action="store_true")
+    ap.add_argument("--no-spectral", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the engine's multi-GPU form (chains dealt across GPUs, one all-gather of the "
@@ -248,6 +250,51 @@ def main():
                       "unit": "chain-levels/s", **res, "member_tile": G.tuned_tile(2 * len(we.members), True),
                       "speedup_batched_vs_plain": round(res["batched"]["value"] / res["plain"]["value"], 2)}
         log(f"ed3: {out['ed3']}")
+
+    if not args.no_epi:
+        # EPI pair blocks: first-shell pair on the fcc ternary (levels 3, D2h stabiliser of the partner), nine adapted
+        # start vectors x two partner species, two chains each, depth 30
+        t = time.perf_counter()
+        wp = W.fcc_epi_pair(L=4, levels=2) if args.quick else W.fcc_epi_pair(L=8, levels=3)
+        GP = engine.GpuOperator(wp.op)
+        log(f"epi: {wp.op.nn} nodes, {len(wp.members)} blocks, build {time.perf_counter() - t:.1f} s")
+        GP.epi_blocks(wp.members, wp.info["v9"], 30)  # warm-up (autotuning)
+        res = {}
+        for mode, name in ((0, "batched"), (2, "graph"), (1, "plain")):
+            t0 = time.perf_counter()
+            x, _, _, _, mu3, ms3 = GP.epi_blocks(wp.members, wp.info["v9"], 30, mode=mode)
+            wall = time.perf_counter() - t0
+            res[name] = {"value": round(int(mu3.sum()) / wall, 1), "wall_ms": round(wall * 1e3, 1), "device_ms": round(ms3, 1)}
+        out["epi"] = {"workload": "fcc ternary first-shell EPI pair, levels 3, 18 blocks x 2 chains x 30", "nodes": wp.op.nn,
+                      "chains": 2 * len(wp.members), "unit": "chain-levels/s", **res,
+                      "member_tile": GP.tuned_tile(2 * len(wp.members), True),
+                      "speedup_batched_vs_plain": round(res["batched"]["value"] / res["plain"]["value"], 2)}
+        log(f"epi: {out['epi']}")
+        GP.close()
+
+    if not args.no_spectral:
+        # projected chains (spectral functions): the headline chains with their projections on the 19 sites of the
+        # start site's first two shells at every level
+        t = time.perf_counter()
+        ws = W.ternary_edc(L=L, levels=levels, depth=depth, targets=True)
+        GS = engine.GpuOperator(ws.op)
+        GS.set_targets(ws.op.meta["tnode"], ws.op.meta["tgop"])
+        log(f"spectral: {ws.op.nn} nodes, {len(ws.members)} chains x {len(ws.info['tsite'])} targets, build "
+            f"{time.perf_counter() - t:.1f} s")
+        GS.chains_edc_proj(ws.members, ws.nlev, ws.info["tw"])  # warm-up (autotuning)
+        res = {}
+        for mode, name in ((0, "batched"), (2, "graph"), (1, "plain")):
+            t0 = time.perf_counter()
+            r = GS.chains_edc_proj(ws.members, ws.nlev, ws.info["tw"], mode=mode)
+            wall = time.perf_counter() - t0
+            res[name] = {"value": round(int(r[5].sum()) / wall, 1), "wall_ms": round(wall * 1e3, 1),
+                         "device_ms": round(r[6], 1)}
+        out["spectral"] = {"workload": "fcc ternary projected edc chains, 54 chains x 30, 19 target sites",
+                           "nodes": ws.op.nn, "chains": len(ws.members), "targets": len(ws.info["tsite"]),
+                           "unit": "chain-levels/s", **res,
+                           "speedup_batched_vs_plain": round(res["batched"]["value"] / res["plain"]["value"], 2)}
+        log(f"spectral: {out['spectral']}")
+        GS.close()
 
     if not args.no_l21:
         # uneven chains over four sublattice operators (L2_1 partial disorder + SRO) in one batched run
