@@ -1144,12 +1144,15 @@ __global__ void __launch_bounds__(BLOCK) k_adj_fin_grad(int nb, int nq, const do
   }
 }
 
+// sc (lagged normalisation): the chain vector is stored as s_k u_k, so the partials are those of s_k^2 <u, H u>
 template <class T>
-__global__ void __launch_bounds__(BLOCK) k_fin_a(ChainDev cd, int nb, const T* part, int nbx, int mt, T* a, T* a_eff) {
+__global__ void __launch_bounds__(BLOCK) k_fin_a(ChainDev cd, int nb, const T* part, int nbx, int mt, T* a, T* a_eff,
+                                                  const T* sc) {
   __shared__ T lds[BLOCK / WAVE];
   const int m = blockIdx.x;
   T s = block_sum_partials(part, nbx, mt, m, lds);
   if (threadIdx.x != 0) return;
+  if (sc) s = bilin(s, inv(bilin(sc[m], sc[m])));
   int k = *cd.level;
   if (cd.active[m]) {
     a[(size_t)m * cd.ld + k] = s;
@@ -1161,10 +1164,12 @@ __global__ void __launch_bounds__(BLOCK) k_fin_a(ChainDev cd, int nb, const T* p
 }
 
 // b_k = sqrt(sum w w); members leave at their depth or when b_k < tiny; next u = w * inv_b.  part2 (route ed3): the partials
-// of <t, w>, so that c_{k+1} = <t, u_{k+1}> = <t, w> / b_k.
+// of <t, w>, so that c_{k+1} = <t, u_{k+1}> = <t, w> / b_k.  Lagged normalisation (sc non-null): w holds s_k w_k, so
+// b_k = sqrt(sum / s_k^2) (the principal root of the same number as the unscaled chain) and the next vector is w itself
+// with the scale s_{k+1} = s_k b_k (snext; 1 for a member that left, whose vector the update zeroes).
 template <class T>
 __global__ void __launch_bounds__(BLOCK) k_fin_b(ChainDev cd, int nb, const T* part, int nbx, int mt, T* b, T* b_eff,
-                                                  T* inv_b, const T* part2) {
+                                                  T* inv_b, const T* part2, const T* sc, T* snext) {
   __shared__ T lds[BLOCK / WAVE];
   const int m = blockIdx.x;
   T s = block_sum_partials(part, nbx, mt, m, lds);
@@ -1175,6 +1180,13 @@ __global__ void __launch_bounds__(BLOCK) k_fin_b(ChainDev cd, int nb, const T* p
   }
   if (threadIdx.x != 0) return;
   int k = *cd.level;
+  T s_m = zero<T>();
+  if (sc) {
+    s_m = sc[m];
+    const T is2 = inv(bilin(s_m, s_m));
+    s = bilin(s, is2);
+    s2 = bilin(s2, inv(s_m));
+  }
   T bk = dsqrt(s);
   int act = cd.active[m];
   if (act) {
@@ -1185,8 +1197,30 @@ __global__ void __launch_bounds__(BLOCK) k_fin_b(ChainDev cd, int nb, const T* p
   }
   cd.active[m] = act;
   b_eff[m] = act ? bk : zero<T>();
-  inv_b[m] = act ? inv(bk) : zero<T>();
+  if (inv_b) inv_b[m] = act ? inv(bk) : zero<T>();
+  if (snext) {
+    T one = zero<T>();
+    *(double*)&one = 1.0;
+    snext[m] = act ? bilin(s_m, bk) : one;
+  }
   if (act && part2 && cd.cvec) ((T*)cd.cvec)[(size_t)m * cd.ld + k + 1] = bilin(s2, inv(bk));
+}
+
+// lagged normalisation: w <- w / s (the vector of the next level normalised), then s <- 1 (k_scale_one)
+template <class T>
+__global__ void k_rescale(T* w, const T* s, size_t n, int nbp) {
+  size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int m = (int)(q % nbp);
+  w[q] = bilin(inv(s[m]), w[q]);
+}
+template <class T>
+__global__ void k_scale_one(T* s, int nbp) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nbp) return;
+  T one = zero<T>();
+  *(double*)&one = 1.0;
+  s[m] = one;
 }
 
 // per-member sum of block partials (one block per member)
@@ -1229,7 +1263,8 @@ __global__ void k_level_end(int* level, const int* active, int nb, int mt, int* 
 // w = w - a u - b_prev up; optional projection onto the symmetric subspace; partial sums w w and (tv non-null) tv w
 template <class T>
 __global__ void __launch_bounds__(BLOCK) k_update(OpDev op, MemDev md, T* w, const T* u, const T* up, const T* a_eff,
-                                                   const T* b_prev, int project, T* partial, const T* tv, T* partial2) {
+                                                   const T* b_prev, int project, T* partial, const T* tv, T* partial2,
+                                                   const T* sc, const T* scp, const int* active) {
   __shared__ T red[BLOCK];
   if (md.tact && !md.tact[blockIdx.y]) return;  // every member of this tile has left
   const int mt = md.mt, npw = WAVE / mt;
@@ -1238,11 +1273,20 @@ __global__ void __launch_bounds__(BLOCK) k_update(OpDev op, MemDev md, T* w, con
   const int i = (xcd_block(blockIdx.x, gridDim.x) * (BLOCK / WAVE) + wave) * npw + lane / mt;
   T dv = zero<T>(), dv2 = zero<T>();
   if (i < op.nn && m < md.nb) {
-    const T a = a_eff[m], b = b_prev[m];
+    // lagged normalisation (sc, scp: scales of u and up): w - a u - b_prev (s_k / s_{k-1}) up keeps the scale s_k; a member
+    // that has left gets a zero vector
+    // (complex chains only: the real chains keep k_scale and their register budget)
+    const T a = a_eff[m];
+    T b = b_prev[m];
+    bool off = false;
+    if constexpr (std::is_same<T, cplx>::value) {
+      if (sc) b = bilin(b, bilin(sc[m], inv(scp[m])));
+      off = active && !active[m];
+    }
     T x[9];
     for (int l = 0; l < 9; ++l) {
       size_t q = ((size_t)i * 9 + l) * md.nbp + m;
-      x[l] = w[q] - bilin(a, u[q]) - bilin(b, up[q]);
+      x[l] = off ? zero<T>() : w[q] - bilin(a, u[q]) - bilin(b, up[q]);
     }
     if (project) {
       int s0 = op.stab_ptr[i], s1 = op.stab_ptr[i + 1], ns = s1 - s0;
@@ -1268,7 +1312,7 @@ __global__ void __launch_bounds__(BLOCK) k_update(OpDev op, MemDev md, T* w, con
             // x[pi(l)] recomputed from memory (w is written below): a register array indexed at run time would live
             // in scratch memory
             const size_t q2 = ((size_t)i * 9 + op.pi[g * 9 + l]) * md.nbp + m;
-            const T xv = w[q2] - bilin(a, u[q2]) - bilin(b, up[q2]);
+            const T xv = off ? zero<T>() : w[q2] - bilin(a, u[q2]) - bilin(b, up[q2]);
             acc[l] = acc[l] + (chi * op.sg[g * 9 + l]) * xv;
           }
         }
@@ -1298,7 +1342,8 @@ __global__ void __launch_bounds__(BLOCK) k_update(OpDev op, MemDev md, T* w, con
 // (augcls_amp_c).  One thread per (member, t, l); members fastest, so the reads of a node row coalesce.
 template <class T>
 __global__ void k_proj(OpDev op, MemDev md, const T* __restrict__ u, int nt, const int* __restrict__ tnode,
-                       const int* __restrict__ tgop, const double* __restrict__ tw, T* __restrict__ cpr) {
+                       const int* __restrict__ tgop, const double* __restrict__ tw, T* __restrict__ cpr,
+                       const T* __restrict__ sc) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   const int tl = blockIdx.y, t = tl / 9, l = tl % 9;
   if (m >= md.nb) return;
@@ -1313,7 +1358,7 @@ __global__ void k_proj(OpDev op, MemDev md, const T* __restrict__ u, int nt, con
     const double w = tw[((size_t)(t * nsm + ks) * 9 + l) * nbp + m];
     acc = fma_s(w * cs, u[((size_t)r * 9 + op.pi[g * 9 + l]) * nbp + m], acc);
   }
-  cpr[(size_t)tl * nbp + m] = acc;
+  cpr[(size_t)tl * nbp + m] = sc ? bilin(acc, inv(sc[m])) : acc;  // the vector is s_k u_k under lagged normalisation
 }
 
 // Full reorthogonalisation of the real chains (lanczos with reorth, src/reference/asr_recursion.f90:44-47):
@@ -2324,9 +2369,20 @@ struct ChainRun {
   bool reorth = false;               // real chains: full reorthogonalisation against every stored Lanczos vector
   size_t qplane = 0;
   bool proj = false;                 // projected chains: projections of every level's vector on the target states
+  // lagged normalisation of the complex chains: the vector of level k is kept as s_k u_k (w' of the previous update, not
+  // divided by b), the per-member scale s_k = s_{k-1} b_{k-1} enters only the scalar bookkeeping (a_k, b_k, the update's
+  // coefficients, c_k, the projections); every 16 levels the vector is normalised once (k_rescale).  The scales rotate
+  // over three buffers: sc3[k % 3] = s_k, sc3[(k + 2) % 3] = s_{k-1}, sc3[(k + 1) % 3] receives s_{k+1}.
+  bool lag = false;
+  T* sc3[3] = {nullptr, nullptr, nullptr};
+  const T* s_cur() const { return lag ? sc3[level_host % 3] : nullptr; }
+  const T* s_prev() const { return lag ? sc3[(level_host + 2) % 3] : nullptr; }
+  T* s_next() const { return lag ? sc3[(level_host + 1) % 3] : nullptr; }
 
   ~ChainRun() {
     if (tact) (void)hipFree(tact);
+    for (T* p : sc3)
+      if (p) (void)hipFree(p);
   }
 
   int setup(asr_gpu_op* op_, hipStream_t s, int nch, const int* start_orb, const double* params, const double* pab,
@@ -2419,6 +2475,17 @@ struct ChainRun {
       HCHECK(hipMemcpyAsync(B.tw, twd.data(), twd.size() * 8, hipMemcpyHostToDevice, stream));
       HCHECK(hipMemsetAsync(B.cpr, 0, (size_t)ldab * nt * 9 * M.nbp * sizeof(T), stream));
     }
+    lag = std::is_same<T, cplx>::value && !real_chain;
+    std::vector<T> one;  // lives until the stream synchronisation below (source of asynchronous copies)
+    if (lag) {
+      one.resize(M.nbp);
+      std::memset(one.data(), 0, one.size() * sizeof(T));
+      for (int q = 0; q < M.nbp; ++q) *(double*)&one[q] = 1.0;
+      for (T*& p : sc3) {
+        if (dalloc(&p, M.nbp)) return 1;
+        HCHECK(hipMemcpyAsync(p, one.data(), M.nbp * sizeof(T), hipMemcpyHostToDevice, stream));
+      }
+    }
     for (T* q : {B.u, B.up, B.w, B.t1, B.t2, B.h1}) HCHECK(hipMemsetAsync(q, 0, nv * sizeof(T), stream));
     HCHECK(hipMemsetAsync(B.a, 0, (size_t)M.nbp * ldab * sizeof(T), stream));
     HCHECK(hipMemsetAsync(B.b, 0, (size_t)M.nbp * ldab * sizeof(T), stream));
@@ -2473,7 +2540,7 @@ struct ChainRun {
       f.out = B.w; f.out_sq = 1;
       if (launch_fused<T>(op, md, f, stream)) return 1;
       hipLaunchKernelGGL(k_update<T>, grid, dim3(BLOCK), 0, stream, dev_of(op), md, B.w, B.u, B.up, B.a_eff, B.b_eff, project,
-                         B.part, (const T*)nullptr, (T*)nullptr);
+                         B.part, (const T*)nullptr, (T*)nullptr, (const T*)nullptr, (const T*)nullptr, (const int*)nullptr);
       hipLaunchKernelGGL(k_sum_member<T>, dim3(nb), dim3(BLOCK), 0, stream, B.part, nbx, md.mt, B.inv_b);
       HCHECK(hipGetLastError());
       std::vector<T> nv2(nbp), row0((size_t)9 * nbp);
@@ -2546,12 +2613,12 @@ struct ChainRun {
       const int nt = op->ntarget;
       hipLaunchKernelGGL(k_proj<T>, dim3((M.nb + 255) / 256, nt * 9), dim3(256), 0, stream, dev_of(op), md, (const T*)B.u, nt,
                          (const int*)op->tnode, (const int*)op->tgop, (const double*)B.tw,
-                         B.cpr + (size_t)level_host * nt * 9 * M.nbp);
+                         B.cpr + (size_t)level_host * nt * 9 * M.nbp, s_cur());
       HCHECK(hipGetLastError());
     }
     if (apply_level_op<T>(op, M, md, B.u, B.w, B.t1, B.t2, B.h1, lin, B.u, B.part, stream)) return 1;
     if (timer_mark(stream, KIND_FIN_A, true)) return 1;
-    hipLaunchKernelGGL(k_fin_a<T>, dim3(M.nb), dim3(BLOCK), 0, stream, cd, M.nb, B.part, nbx, apply_mt, B.a, B.a_eff);
+    hipLaunchKernelGGL(k_fin_a<T>, dim3(M.nb), dim3(BLOCK), 0, stream, cd, M.nb, B.part, nbx, apply_mt, B.a, B.a_eff, s_cur());
     if (timer_mark(stream, 0, false)) return 1;
     return 0;
   }
@@ -2614,7 +2681,7 @@ struct ChainRun {
   int step_update() {
     if (timer_mark(stream, KIND_UPDATE, true)) return 1;
     hipLaunchKernelGGL(k_update<T>, grid, dim3(BLOCK), 0, stream, dev_of(op), md, B.w, B.u, B.up, B.a_eff, B.b_eff, project,
-                       B.part, (const T*)B.tv, B.part2);
+                       B.part, (const T*)B.tv, B.part2, s_cur(), s_prev(), lag ? (const int*)B.active : (const int*)nullptr);
     if (reorth) {
       // w = w - Q (Q^T w), twice (classical Gram-Schmidt against q_0..q_k); the second pass leaves the partials of w w
       const int nq = level_host + 1;
@@ -2629,10 +2696,19 @@ struct ChainRun {
     }
     if (timer_mark(stream, 0, false) || timer_mark(stream, KIND_FIN_B, true)) return 1;
     hipLaunchKernelGGL(k_fin_b<T>, dim3(M.nb), dim3(BLOCK), 0, stream, cd, M.nb, B.part, nbx, md.mt, B.b, B.b_eff,
-                       B.inv_b, (const T*)B.part2);
-    if (timer_mark(stream, 0, false) || timer_mark(stream, KIND_SCALE, true)) return 1;
-    hipLaunchKernelGGL(k_scale<T>, sgrid, dim3(256), 0, stream, B.w, B.inv_b, nv, M.nbp);
+                       B.inv_b, (const T*)B.part2, s_cur(), s_next());
     if (timer_mark(stream, 0, false)) return 1;
+    if (!lag) {
+      if (timer_mark(stream, KIND_SCALE, true)) return 1;
+      hipLaunchKernelGGL(k_scale<T>, sgrid, dim3(256), 0, stream, B.w, B.inv_b, nv, M.nbp);
+      if (timer_mark(stream, 0, false)) return 1;
+    } else if (level_host % 16 == 15) {
+      // normalise once every 16 levels, so that the scale (a product of b's) stays far from under- and overflow
+      if (timer_mark(stream, KIND_SCALE, true)) return 1;
+      hipLaunchKernelGGL(k_rescale<T>, sgrid, dim3(256), 0, stream, B.w, (const T*)s_next(), nv, M.nbp);
+      hipLaunchKernelGGL(k_scale_one<T>, dim3((M.nbp + 255) / 256), dim3(256), 0, stream, s_next(), M.nbp);
+      if (timer_mark(stream, 0, false)) return 1;
+    }
     hipLaunchKernelGGL(k_level_end, dim3(1), dim3(64), 0, stream, B.level, B.active, M.nb, M.mt, tact, ntile);
     HCHECK(hipGetLastError());
     if (g_timer) {
