@@ -22,6 +22,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -1824,8 +1825,52 @@ __global__ void __launch_bounds__(BLOCK) k_kpm_fin(const double* partial, int nb
 }  // namespace
 
 // ================================================================================================ host side
+// Cached device blocks of the chain vectors: a batched run returns its level vectors here instead of freeing them, and the
+// next run of the same operator (the next batch of a contour pass) takes them back without hipMalloc / hipFree, whose
+// cost grows with the size (tens of GB per batch).  A free block serves a request of up to its size and at least 4/5 of
+// it.  One host thread per operator (the calls of an operator are not concurrent), so no lock.
+struct BlockPool {
+  std::multimap<size_t, void*> free_;
+  std::map<void*, size_t> size_;
+  void* get(size_t bytes) {
+    auto it = free_.lower_bound(bytes);
+    if (it != free_.end() && it->first - bytes <= it->first / 5) {
+      void* p = it->second;
+      free_.erase(it);
+      return p;
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      trim();  // out of memory: give the cached blocks back and try once more
+      if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+    }
+    size_[p] = bytes;
+    return p;
+  }
+  void put(void* p) {
+    auto it = size_.find(p);
+    if (it == size_.end()) {
+      (void)hipFree(p);
+      return;
+    }
+    free_.insert({it->second, p});
+  }
+  void trim() {
+    for (auto& kv : free_) {
+      size_.erase(kv.second);
+      (void)hipFree(kv.second);
+    }
+    free_.clear();
+  }
+  ~BlockPool() {
+    for (auto& kv : size_) (void)hipFree(kv.first);
+  }
+};
+
 struct asr_gpu_op {
   int nn, nslot, nop, nsmax, ncls, nsp, ntype, nstab;
+  BlockPool pool;  // cached level-vector blocks of the batched runs (asr_gpu_op_trim releases them)
   std::vector<int> pi_h, nsc_h, spc_h;
   std::vector<double> sg_h, mt_h;  // mt: ncls x nsmax x nsmax x nsmax
   uint32_t *hop = nullptr, *flip = nullptr, *info = nullptr;
@@ -1862,6 +1907,15 @@ int dalloc(P** p, size_t n) {
   if (n == 0) n = 1;
   hipError_t e = hipMalloc((void**)p, n * sizeof(P));
   if (e != hipSuccess) return fail(std::string("hipMalloc: ") + hipGetErrorString(e));
+  return 0;
+}
+
+// a level-vector block from the operator's cache (returned with pool->put)
+template <class P>
+int palloc(BlockPool* pool, P** p, size_t n) {
+  if (n == 0) n = 1;
+  *p = (P*)pool->get(n * sizeof(P));
+  if (!*p) return fail("hipMalloc: out of device memory for the level vectors");
   return 0;
 }
 
@@ -2287,7 +2341,10 @@ int autotune_tile(asr_gpu_op* op, int nchain, const int* start_orb, const double
     if (upload_members(op, M, MB, md)) return 1;
     const size_t nv = (size_t)(op->nn + 1) * 9 * M.nbp;  // + the zero padding node
     T *x = nullptr, *y = nullptr, *t1 = nullptr, *t2 = nullptr, *h1 = nullptr;
-    if (dalloc(&x, nv) || dalloc(&y, nv) || dalloc(&t1, nv) || dalloc(&t2, nv) || dalloc(&h1, nv)) return 1;
+    BlockPool* pool = &op->pool;
+    if (palloc(pool, &x, nv) || palloc(pool, &y, nv) || palloc(pool, &t1, nv) || palloc(pool, &t2, nv) ||
+        palloc(pool, &h1, nv))
+      return 1;
     int st = 0;
     hipEvent_t e0, e1;
     st |= hipEventCreate(&e0) != hipSuccess;
@@ -2302,7 +2359,8 @@ int autotune_tile(asr_gpu_op* op, int nchain, const int* start_orb, const double
     if (!st) st |= hipEventElapsedTime(&ms, e0, e1) != hipSuccess;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    for (T* p : {x, y, t1, t2, h1}) (void)hipFree(p);
+    (void)hipStreamSynchronize(op->stream);
+    for (T* p : {x, y, t1, t2, h1}) pool->put(p);
     if (st) return fail("autotune: a timing launch failed");
     // a tile that pads many idle members is charged for them
     if (ms < best_ms) { best_ms = ms; best_mt = mt; }
@@ -2321,11 +2379,16 @@ struct ChainBuffers {
   double* tw = nullptr;  // projected chains: dressing weights ntarget x nsmax x 9 x nbp
   T* cpr = nullptr;      // projected chains: projections per level, ld x ntarget x 9 x nbp
   int *level = nullptr, *depth = nullptr, *active = nullptr, *mused = nullptr, *closed = nullptr;
+  BlockPool* pool = nullptr;  // the level vectors u .. h1, tv come from the operator's cache and go back there
   ~ChainBuffers() {
-    for (void* p : {(void*)u, (void*)up, (void*)w, (void*)t1, (void*)t2, (void*)h1, (void*)part, (void*)a, (void*)b,
-                    (void*)a_eff, (void*)b_eff, (void*)inv_b, (void*)level, (void*)depth, (void*)active, (void*)mused,
-                    (void*)tv, (void*)part2, (void*)c, (void*)closed, (void*)qv, (void*)qpart, (void*)qcoef, (void*)tw,
-                    (void*)cpr})
+    for (void* p : {(void*)u, (void*)up, (void*)w, (void*)t1, (void*)t2, (void*)h1, (void*)tv})
+      if (p) {
+        if (pool) pool->put(p);
+        else (void)hipFree(p);
+      }
+    for (void* p : {(void*)part, (void*)a, (void*)b, (void*)a_eff, (void*)b_eff, (void*)inv_b, (void*)level, (void*)depth,
+                    (void*)active, (void*)mused, (void*)part2, (void*)c, (void*)closed, (void*)qv, (void*)qpart,
+                    (void*)qcoef, (void*)tw, (void*)cpr})
       if (p) (void)hipFree(p);
   }
 };
@@ -2380,6 +2443,8 @@ struct ChainRun {
   T* s_next() const { return lag ? sc3[(level_host + 1) % 3] : nullptr; }
 
   ~ChainRun() {
+    // work of an aborted run may still be queued: the cached blocks go back only once the stream is idle
+    if (stream) (void)hipStreamSynchronize(stream);
     if (tact) (void)hipFree(tact);
     for (T* p : sc3)
       if (p) (void)hipFree(p);
@@ -2444,13 +2509,14 @@ struct ChainRun {
     const size_t npart = (size_t)grid.x * grid.y * md.mt;
     // the dot partials of the apply pass: complex members with tile 64 run as lane pairs, 32 members per wave
     apply_mt = (md.mt == WAVE && md.pair && sizeof(T) == sizeof(cplx)) ? WAVE / 2 : md.mt;
-    if (dalloc(&B.u, nv) || dalloc(&B.up, nv) || dalloc(&B.w, nv) || dalloc(&B.t1, nv) || dalloc(&B.t2, nv) ||
-        dalloc(&B.h1, nv) || dalloc(&B.part, npart) || dalloc(&B.a, (size_t)M.nbp * ldab) ||
+    B.pool = &op->pool;
+    if (palloc(B.pool, &B.u, nv) || palloc(B.pool, &B.up, nv) || palloc(B.pool, &B.w, nv) || palloc(B.pool, &B.t1, nv) ||
+        palloc(B.pool, &B.t2, nv) || palloc(B.pool, &B.h1, nv) || dalloc(&B.part, npart) || dalloc(&B.a, (size_t)M.nbp * ldab) ||
         dalloc(&B.b, (size_t)M.nbp * ldab) || dalloc(&B.a_eff, M.nbp) || dalloc(&B.b_eff, M.nbp) ||
         dalloc(&B.inv_b, M.nbp) || dalloc(&B.level, 1) || dalloc(&B.depth, M.nbp) || dalloc(&B.active, M.nbp) ||
         dalloc(&B.mused, M.nbp) || dalloc(&tact, std::max(ntile, 2)))
       return 1;
-    if (ed3 && (dalloc(&B.tv, nv) || dalloc(&B.part2, npart) || dalloc(&B.c, (size_t)M.nbp * ldab) ||
+    if (ed3 && (palloc(B.pool, &B.tv, nv) || dalloc(&B.part2, npart) || dalloc(&B.c, (size_t)M.nbp * ldab) ||
                 dalloc(&B.closed, M.nbp)))
       return 1;
     reorth = op->reorth && real_chain && std::is_same<T, double>::value;
@@ -3483,6 +3549,13 @@ int asr_gpu_op_get_tile(asr_gpu_op* op, int nchain, int is_complex, int* mt) {
   *mt = 0;
   for (auto& kv : op->tuned)
     if ((kv.first >> 8) == nchain && ((kv.first & 1) != 0) == (is_complex != 0)) *mt = kv.second;
+  return 0;
+}
+
+int asr_gpu_op_trim(asr_gpu_op* op) {
+  if (!op) return fail("asr_gpu_op_trim: null operator");
+  if (op->stream) (void)hipStreamSynchronize(op->stream);
+  op->pool.trim();
   return 0;
 }
 

@@ -376,3 +376,74 @@ extern "C" int asr_host_contour_closed(double ebot, double ef, double eta, int n
   }
   return 0;
 }
+
+// Semicircle from E_bot to E_F through the upper half plane, optionally stopped at Im z = eta_end (eta_end <= 0: down to the
+// real axis) (contour_nodes, src/asr_contour.f90:17-34).
+extern "C" int asr_host_contour_nodes(double ebot, double ef, int n, double eta_end, double* z_, double* w_) {
+  const double pi = 3.14159265358979323846;
+  if (n < 1 || !(ef > ebot)) return 1;
+  std::vector<double> x(n), wg(n);
+  asr_host_gauss_legendre(n, x.data(), wg.data());
+  const double c = 0.5 * (ebot + ef), r = 0.5 * (ef - ebot);
+  const double th0 = eta_end > 0.0 ? std::asin(std::min(eta_end / r, 0.5)) : 0.0;
+  for (int j = 0; j < n; ++j) {
+    const double th = th0 + 0.5 * (pi - th0) * (1.0 - x[j]);  // pi at E_bot down to th0 near E_F
+    const cplx z(c + r * std::cos(th), r * std::sin(th));
+    const cplx w = wg[j] * (-0.5 * (pi - th0)) * cplx(0.0, r) * std::exp(cplx(0.0, th));
+    z_[2 * j] = z.real(); z_[2 * j + 1] = z.imag(); w_[2 * j] = w.real(); w_[2 * j + 1] = w.imag();
+  }
+  return 0;
+}
+
+// Matsubara poles of the Fermi contour, Gamma = 2 npole pi kT ~ 0.1 Ry, 2 <= npole <= 40 (fermi_npole,
+// src/asr_contour.f90:61-64), and the node count of the contour (fermi_nodes_count :66-69).
+extern "C" int asr_host_fermi_npole(double kt) {
+  const double pi = 3.14159265358979323846;
+  const long n = std::lround(0.1 / (2.0 * pi * kt));
+  return (int)std::max(2L, std::min(40L, n));
+}
+extern "C" int asr_host_fermi_nodes_count(int npole, int n_main, int n_step) { return 4 + n_main + n_step + 8 + npole; }
+
+// Fermi-smeared contour (contour_nodes_fermi, src/asr_contour.f90:77-121; Wildberger et al., PRB 52, 11502 (1995)):
+//   int f(E) [-(1/pi) Im h(E + i0)] dE = -(1/pi) Im sum_j w_j h(z_j)   for h analytic in the upper half plane.
+// Legs: the vertical E_bot -> E_bot + i Gamma (4 nodes, weight times the complex Fermi function), the line Im z = Gamma from
+// E_bot to mu - 10 kT (n_main), across the step to mu + 10 kT (n_step), to mu + 40 kT (8), where f(x + i Gamma) = f(x) is
+// real; then the npole Matsubara poles mu + i (2n + 1) pi kT with w = -2 pi i kT.  z, w: fermi_nodes_count (re, im) pairs.
+extern "C" int asr_host_contour_fermi(double ebot, double mu, double kt, int npole, int n_main, int n_step, double* z_,
+                                      double* w_) {
+  const double pi = 3.14159265358979323846;
+  if (!(kt > 0.0) || npole < 0 || n_main < 1 || n_step < 1) return 1;
+  auto fermi_r = [&](double e) {
+    const double t = (e - mu) / kt;
+    return t > 0.0 ? std::exp(-t) / (1.0 + std::exp(-t)) : 1.0 / (1.0 + std::exp(t));
+  };
+  auto fermi_c = [&](cplx zz) {
+    const cplx t = (zz - mu) / kt;
+    return t.real() < -600.0 ? cplx(1.0, 0.0) : 1.0 / (std::exp(t) + 1.0);
+  };
+  const double gam = 2.0 * npole * pi * kt;
+  int k = 0;
+  auto put = [&](cplx z, cplx w) {
+    z_[2 * k] = z.real(); z_[2 * k + 1] = z.imag(); w_[2 * k] = w.real(); w_[2 * k + 1] = w.imag();
+    ++k;
+  };
+  std::vector<double> xv(4), wv(4);
+  asr_host_gauss_legendre(4, xv.data(), wv.data());
+  for (int j = 0; j < 4; ++j) {
+    const cplx z(ebot, 0.5 * gam * (1.0 + xv[j]));
+    put(z, cplx(0.0, 0.5 * gam * wv[j]) * fermi_c(z));
+  }
+  auto line = [&](int n, double lo, double hi) {
+    std::vector<double> x(n), w(n);
+    asr_host_gauss_legendre(n, x.data(), w.data());
+    for (int j = 0; j < n; ++j) {
+      const double e = 0.5 * (lo + hi) + 0.5 * (hi - lo) * x[j];
+      put(cplx(e, gam), cplx(0.5 * (hi - lo) * w[j] * fermi_r(e), 0.0));
+    }
+  };
+  line(n_main, ebot, mu - 10.0 * kt);
+  line(n_step, mu - 10.0 * kt, mu + 10.0 * kt);
+  line(8, mu + 10.0 * kt, mu + 40.0 * kt);
+  for (int j = 0; j < npole; ++j) put(cplx(mu, (2 * j + 1) * pi * kt), cplx(0.0, -2.0 * pi * kt));
+  return 0;
+}

@@ -44,6 +44,7 @@ def lib() -> C.CDLL:
         L.asr_gpu_chains_edc_proj.argtypes = [P, C.c_int, P, P, P, P, P, P, P, C.c_int, C.c_int, P, P, P, P, P, P, P, C.c_int, P]
         L.asr_gpu_op_get_tile.argtypes = [P, C.c_int, C.c_int, P]
         L.asr_gpu_real_set_tile.argtypes = [P, C.c_int]
+        L.asr_gpu_op_trim.argtypes = [P]
         L.asr_gpu_op_destroy.argtypes = [P]
         L.asr_gpu_op_destroy.restype = None
         L.asr_gpu_apply_batched.argtypes = [P, C.c_int, P, P, P, P, P, P]
@@ -171,6 +172,10 @@ class GpuOperator:
                                              ld, _p(tw), _p(a), _p(b), _p(mu), _p(g), _p(gp), _p(gp4), int(mode), _p(ms)),
                "asr_gpu_chains_edc_proj")
         return g, gp, gp4, a, b, mu, float(ms[0])
+
+    def trim(self):
+        """Release the level-vector blocks cached between batched runs."""
+        _check(lib().asr_gpu_op_trim(self.h), "asr_gpu_op_trim")
 
     def set_reorth(self, reorth: bool):
         """Full reorthogonalisation of the real chains (lanczos reorth = .true.; small cases)."""

@@ -36,6 +36,10 @@ def lib() -> C.CDLL:
         L.asr_host_chain_eigvals.argtypes = [C.c_int, P, P, P]
         L.asr_host_gauss_legendre.argtypes = [C.c_int, P, P]
         L.asr_host_contour_closed.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, P, P]
+        L.asr_host_contour_nodes.argtypes = [C.c_double, C.c_double, C.c_int, C.c_double, P, P]
+        L.asr_host_fermi_npole.argtypes = [C.c_double]
+        L.asr_host_fermi_nodes_count.argtypes = [C.c_int, C.c_int, C.c_int]
+        L.asr_host_contour_fermi.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int, P, P]
         _LIB = L
     return _LIB
 
@@ -164,4 +168,36 @@ def contour_closed(ebot: float, ef: float, eta: float, n_arc: int, n_vert: int):
     z = np.zeros(n_arc + n_vert, np.complex128)
     w = np.zeros(n_arc + n_vert, np.complex128)
     lib().asr_host_contour_closed(float(ebot), float(ef), float(eta), int(n_arc), int(n_vert), _p(z), _p(w))
+    return z, w
+
+
+def gauss_legendre(n: int):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    lib().asr_host_gauss_legendre(int(n), _p(x), _p(w))
+    return x, w
+
+
+def contour_nodes(ebot: float, ef: float, n: int, eta_end: float | None = None):
+    z = np.zeros(n, np.complex128)
+    w = np.zeros(n, np.complex128)
+    if lib().asr_host_contour_nodes(float(ebot), float(ef), int(n), float(eta_end or 0.0), _p(z), _p(w)):
+        raise ValueError("contour_nodes: n >= 1 and ef > ebot")
+    return z, w
+
+
+def fermi_npole(kt: float) -> int:
+    return int(lib().asr_host_fermi_npole(float(kt)))
+
+
+def fermi_nodes_count(npole: int, n_main: int, n_step: int) -> int:
+    return int(lib().asr_host_fermi_nodes_count(int(npole), int(n_main), int(n_step)))
+
+
+def contour_fermi(ebot: float, mu: float, kt: float, npole: int, n_main: int, n_step: int):
+    n = fermi_nodes_count(npole, n_main, n_step)
+    z = np.zeros(n, np.complex128)
+    w = np.zeros(n, np.complex128)
+    if lib().asr_host_contour_fermi(float(ebot), float(mu), float(kt), int(npole), int(n_main), int(n_step), _p(z), _p(w)):
+        raise ValueError("contour_fermi: kt > 0, npole >= 0, n_main >= 1, n_step >= 1")
     return z, w

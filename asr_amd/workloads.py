@@ -135,7 +135,7 @@ def ternary_edc(L: int = 8, levels: int = 4, depth: int = 30, seed: int = 2, gro
         op = reorder_nodes(op)
     spins = edc_params(rng, 3, 2)
     members = [edc_member(spins[sp], cen, l0, z) for sp in range(2) for cen in range(3) for l0 in range(9)]
-    info = dict(L=L, levels=levels, group=group, seed=seed, nodes=op.nn, z=z)
+    info = dict(L=L, levels=levels, group=group, seed=seed, nodes=op.nn, z=z, params=spins)
     if targets:
         from . import spectral as SP
         tw = []

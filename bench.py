@@ -140,6 +140,7 @@ def main():
     ap.add_argument("--no-ed3", action="store_true")
     ap.add_argument("--no-epi", action="store_true")
     ap.add_argument("--no-spectral", action="store_true")
+    ap.add_argument("--no-scf", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the engine's multi-GPU form (chains dealt across GPUs, one all-gather of the "
@@ -295,6 +296,31 @@ def main():
                            "speedup_batched_vs_plain": round(res["batched"]["value"] / res["plain"]["value"], 2)}
         log(f"spectral: {out['spectral']}")
         GS.close()
+
+    if not args.no_scf:
+        # one occupation pass of the SCF loop on the headline operator (chain_greens + ed_moments): the Fermi contour
+        # with the reference's defaults (32 main-line nodes, 16 across the step, kT = 1 mRy: 16 poles) plus the E_F probe,
+        # every (energy, start orbital, component, spin) task a chain of depth 30 with the adaptive depth (1e-5, hull),
+        # in energy-major batches of at most 112 members that reuse the operator's cached level vectors
+        from asr_amd import scf as SC
+        params = wl.info["params"]
+        enu = np.stack([np.stack([p["enu"][pp] for p in params]) for pp in range(3)])
+        cp = SC.contour(0.0, enu)
+        classes, comps = [SC.ChainClass(wl.op)], [SC.Component(0) for _ in range(3)]
+        t0 = time.perf_counter()
+        cg = SC.chain_greens(cp.zq, classes, comps, params, depth, route="edc", depth_tol=1.0e-5, tail="hull",
+                             gpu_ops=[G])
+        mom = SC.ed_moments(cp, cg, classes, comps, enu)
+        wall = time.perf_counter() - t0
+        G.set_depth_tol(0.0, "hull")
+        out["scf_pass"] = {"workload": "fcc ternary edc occupation pass: Fermi contour (kT 1 mRy, 32 + 16 + 4 + 8 nodes, "
+                                       "16 poles) + E_F probe, 2 spins x 3 components x 9 orbitals, depth 30, "
+                                       "adaptive depth 1e-5", "nodes": wl.op.nn, "energies": int(cp.zq.size),
+                           "chains": cg.nchain, "chain_levels": cg.nlev, "unit": "chain-levels/s",
+                           "value": round(cg.nlev / wall, 1), "wall_s": round(wall, 2),
+                           "device_s": round(cg.device_ms * 1e-3, 2), "causality_failures": [mom.npsd, mom.nnode],
+                           "charge_per_spin": [round(float(mom.qc[:, g].sum()), 6) for g in range(2)]}
+        log(f"scf pass: {out['scf_pass']}")
 
     if not args.no_l21:
         # uneven chains over four sublattice operators (L2_1 partial disorder + SRO) in one batched run

@@ -74,6 +74,10 @@ int asr_gpu_chains_edc_proj(asr_gpu_op* op, int nchain, const int* start_orb, co
 int asr_gpu_op_set_reorth(asr_gpu_op* op, int reorth);
 /* The tile the autotuner picked for nchain members of the given kind (0: none tuned yet). */
 int asr_gpu_op_get_tile(asr_gpu_op* op, int nchain, int is_complex, int* mt);
+/* Release the level-vector blocks the operator keeps cached between batched runs (the batches of a contour pass reuse
+ * them instead of allocating tens of GB per call).  No counterpart in the reference (its work arrays are per thread and
+ * allocated once, chain_worker src/asr_class_scf.f90:1126-1136); asr_gpu_op_destroy releases them too. */
+int asr_gpu_op_trim(asr_gpu_op* op);
 void asr_gpu_op_destroy(asr_gpu_op* op);
 
 /* y = H x, H = E_nu + h - h o h in normalised coordinates (augcls_apply, src/asr_augcls.f90:275-297) for nb chains at once

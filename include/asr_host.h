@@ -52,6 +52,13 @@ int asr_host_chain_eigvals(int n, const double* a, const double* b, double* ev);
 /* Gauss-Legendre rule (src/asr_contour.f90:132-151) and the closed occupation contour (src/asr_contour.f90:39-58). */
 int asr_host_gauss_legendre(int n, double* x, double* w);
 int asr_host_contour_closed(double ebot, double ef, double eta, int n_arc, int n_vert, double* z, double* w);
+/* Semicircle E_bot -> E_F (contour_nodes, src/asr_contour.f90:17-34; eta_end <= 0: no eta_end).  1 = bad arguments. */
+int asr_host_contour_nodes(double ebot, double ef, int n, double eta_end, double* z, double* w);
+/* Fermi-smeared contour: fermi_npole, fermi_nodes_count, contour_nodes_fermi (src/asr_contour.f90:61-121).  z, w hold
+   fermi_nodes_count(npole, n_main, n_step) (re, im) pairs.  1 = bad arguments. */
+int asr_host_fermi_npole(double kt);
+int asr_host_fermi_nodes_count(int npole, int n_main, int n_step);
+int asr_host_contour_fermi(double ebot, double mu, double kt, int npole, int n_main, int n_step, double* z, double* w);
 
 #ifdef __cplusplus
 }
