@@ -1611,15 +1611,28 @@ __global__ void __launch_bounds__(BLOCK) k_kpm_u(RealDev R, const double* __rest
 }
 
 // The same step with two vectors per lane (16-byte loads: half the L1 requests of one vector per lane) and two sites per
-// wave (32 lanes x 2 vectors = the 64-vector tile of one site).
+// wave (32 lanes x 2 vectors = the 64-vector tile of one site).  DBL (moment doubling): instead of <r|t_{n+1}> the partials
+// of <t_{n+1}|t_{n+1}> (partial) and <t_{n+1}|t_n> (partial2), from which mu_{2n+2} and mu_{2n+1} follow; no random
+// vector is regenerated.
 struct d2 {
   double x, y;
 };
-template <int NSLOT>
-__global__ void __launch_bounds__(BLOCK) k_kpm_v2(RealDev R, const double* __restrict__ blk, int nvec, int nvp,
-                                                  const double* __restrict__ u1, const double* __restrict__ u0,
-                                                  double* __restrict__ u2, double cshift, double alpha, double beta,
-                                                  uint64_t seed, double* partial) {
+// fixed-order sum of a (vector pair) value over the two sites of each wave and the 4 waves; 64 values out per block
+__device__ inline void kpm_pair_reduce(d2 v, d2 (*red)[32], int wave, int half, int vl, double* out) {
+  v.x += __shfl_xor(v.x, 32);
+  v.y += __shfl_xor(v.y, 32);
+  if (half == 0) red[wave][vl] = v;
+  __syncthreads();
+  if (threadIdx.x < 32) {
+    d2 s2 = red[0][threadIdx.x];
+    for (int w = 1; w < BLOCK / WAVE; ++w) { s2.x += red[w][threadIdx.x].x; s2.y += red[w][threadIdx.x].y; }
+    out[2 * threadIdx.x] = s2.x;
+    out[2 * threadIdx.x + 1] = s2.y;
+  }
+}
+
+// partials of <t|t> of a stored (d-scaled) vector, laid out as k_kpm_v2's (moment doubling: mu_2 = 2 <t_1|t_1> - mu_0)
+__global__ void __launch_bounds__(BLOCK) k_kpm_sq(RealDev R, int nvp, const double* __restrict__ u, double* partial) {
   __shared__ d2 red[BLOCK / WAVE][32];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, half = lane >> 5, vl = lane & 31;
   const int lb = xcd_block(blockIdx.x, gridDim.x);
@@ -1627,22 +1640,49 @@ __global__ void __launch_bounds__(BLOCK) k_kpm_v2(RealDev R, const double* __res
   const int v = blockIdx.y * 64 + 2 * vl;
   d2 dot = {0.0, 0.0};
   if (site < R.nsite) {
+    const double* dp = R.par + (size_t)(R.nsp + R.spec[site]) * 9;
+    for (int l = 0; l < 9; ++l) {
+      const d2 x = ((const d2*)(u + ((size_t)site * 9 + l) * nvp + v))[0];
+      const double tx = x.x / dp[l], ty = x.y / dp[l];
+      dot.x = fma(tx, tx, dot.x);
+      dot.y = fma(ty, ty, dot.y);
+    }
+  }
+  kpm_pair_reduce(dot, red, wave, half, vl, partial + ((size_t)blockIdx.y * gridDim.x + lb) * 64);
+}
+
+template <int NSLOT, bool DBL>
+__global__ void __launch_bounds__(BLOCK) k_kpm_v2(RealDev R, const double* __restrict__ blk, int nvec, int nvp,
+                                                  const double* __restrict__ u1, const double* __restrict__ u0,
+                                                  double* __restrict__ u2, double cshift, double alpha, double beta,
+                                                  uint64_t seed, double* partial, double* partial2) {
+  __shared__ d2 red[BLOCK / WAVE][32];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, half = lane >> 5, vl = lane & 31;
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int site = (lb * (BLOCK / WAVE) + wave) * 2 + half;
+  const int v = blockIdx.y * 64 + 2 * vl;
+  d2 dot = {0.0, 0.0}, dot2 = {0.0, 0.0};
+  if (site < R.nsite) {
     const int sp = R.spec[site];
     const double* cp = R.par + (size_t)sp * 9;
     const double* dp = R.par + (size_t)(R.nsp + sp) * 9;
     const int nslot = NSLOT > 0 ? NSLOT : R.nslot;
-    d2 acc[9], xo[9];
+    // acc = block0 x + ((c - cshift) / d^2) x: the own-row term of t2 folded into the start of the sum (x = d t1 is
+    // the stored row), so that the row is dead after this and does not hold 18 registers through the neighbour loop
+    d2 acc[9];
     {
       const d2* __restrict__ row = (const d2*)(u1 + (size_t)site * 9 * nvp + v);
+      d2 xo[9];
       for (int l = 0; l < 9; ++l) xo[l] = row[(size_t)l * (nvp / 2)];
-    }
-    for (int l = 0; l < 9; ++l) {
-      d2 a = {0.0, 0.0};
-      for (int l2 = 0; l2 < 9; ++l2) {
-        a.x = fma(blk[l * 9 + l2], xo[l2].x, a.x);
-        a.y = fma(blk[l * 9 + l2], xo[l2].y, a.y);
+      for (int l = 0; l < 9; ++l) {
+        const double f = (cp[l] - cshift) / (dp[l] * dp[l]);
+        d2 a = {f * xo[l].x, f * xo[l].y};
+        for (int l2 = 0; l2 < 9; ++l2) {
+          a.x = fma(blk[l * 9 + l2], xo[l2].x, a.x);
+          a.y = fma(blk[l * 9 + l2], xo[l2].y, a.y);
+        }
+        acc[l] = a;
       }
-      acc[l] = a;
     }
     const int* __restrict__ nbl = R.nbr + (size_t)site * R.nslot;
 #pragma unroll 2
@@ -1663,28 +1703,30 @@ __global__ void __launch_bounds__(BLOCK) k_kpm_v2(RealDev R, const double* __res
     }
     for (int l = 0; l < 9; ++l) {
       const uint64_t row = (uint64_t)site * 9 + l;
-      const double r0 = rademacher(seed, (uint64_t)v, row), r1 = rademacher(seed, (uint64_t)v + 1, row);
       const d2 t0v = ((const d2*)(u0 + row * nvp + v))[0];
-      const double t1x = xo[l].x / dp[l], t1y = xo[l].y / dp[l];
-      double t2x = alpha * (cp[l] * t1x + dp[l] * acc[l].x - cshift * t1x) - beta * (t0v.x / dp[l]);
-      double t2y = alpha * (cp[l] * t1y + dp[l] * acc[l].y - cshift * t1y) - beta * (t0v.y / dp[l]);
+      double t2x = alpha * (dp[l] * acc[l].x) - beta * (t0v.x / dp[l]);
+      double t2y = alpha * (dp[l] * acc[l].y) - beta * (t0v.y / dp[l]);
       if (v < nvec) u2[row * nvp + v] = dp[l] * t2x;
       if (v + 1 < nvec) u2[row * nvp + v + 1] = dp[l] * t2y;
-      dot.x = fma(r0, t2x, dot.x);
-      dot.y = fma(r1, t2y, dot.y);
+      if constexpr (DBL) {
+        // the own row again (an L1/L2 hit) rather than held through the neighbour loop
+        const d2 xo = ((const d2*)(u1 + row * nvp + v))[0];
+        dot.x = fma(t2x, t2x, dot.x);
+        dot.y = fma(t2y, t2y, dot.y);
+        dot2.x = fma(t2x, xo.x / dp[l], dot2.x);
+        dot2.y = fma(t2y, xo.y / dp[l], dot2.y);
+      } else {
+        const double r0 = rademacher(seed, (uint64_t)v, row), r1 = rademacher(seed, (uint64_t)v + 1, row);
+        dot.x = fma(r0, t2x, dot.x);
+        dot.y = fma(r1, t2y, dot.y);
+      }
     }
   }
   // per (block, vector): the two sites of the wave, then the 4 waves in order
-  dot.x += __shfl_xor(dot.x, 32);
-  dot.y += __shfl_xor(dot.y, 32);
-  if (half == 0) red[wave][vl] = dot;
-  __syncthreads();
-  if (threadIdx.x < 32) {
-    d2 s2 = red[0][threadIdx.x];
-    for (int w = 1; w < BLOCK / WAVE; ++w) { s2.x += red[w][threadIdx.x].x; s2.y += red[w][threadIdx.x].y; }
-    double* pp = partial + ((size_t)blockIdx.y * gridDim.x + lb) * 64 + 2 * threadIdx.x;
-    pp[0] = s2.x;
-    pp[1] = s2.y;
+  kpm_pair_reduce(dot, red, wave, half, vl, partial + ((size_t)blockIdx.y * gridDim.x + lb) * 64);
+  if constexpr (DBL) {
+    __syncthreads();
+    kpm_pair_reduce(dot2, red, wave, half, vl, partial2 + ((size_t)blockIdx.y * gridDim.x + lb) * 64);
   }
 }
 
@@ -1822,6 +1864,23 @@ __global__ void __launch_bounds__(BLOCK) k_kpm_fin(const double* partial, int nb
   if (threadIdx.x == 0) mu_per[(size_t)v * nmom + n] = s * inv_dim;
 }
 
+// moment doubling (T_2m = 2 T_m^2 - T_0, T_2m-1 = 2 T_m T_m-1 - T_1): from the partials of <t_m|t_m> and <t_m|t_m-1>,
+// mu_2m = 2 <t_m|t_m> / dim - mu_0 and mu_2m-1 = 2 <t_m|t_m-1> / dim - mu_1 (partial2 null: only mu_2m)
+__global__ void __launch_bounds__(BLOCK) k_kpm_fin2(const double* partial, const double* partial2, int nbx, int vt, int nvec,
+                                                    double inv_dim, double* mu_per, int nmom, int m) {
+  __shared__ double lds[BLOCK / WAVE];
+  const int v = blockIdx.x;
+  if (v >= nvec) return;
+  double* mv = mu_per + (size_t)v * nmom;
+  const double s1 = block_sum_partials<double>(partial, nbx, vt, v, lds);
+  __syncthreads();
+  if (threadIdx.x == 0 && 2 * m < nmom) mv[2 * m] = 2.0 * s1 * inv_dim - mv[0];
+  if (partial2) {
+    const double s2 = block_sum_partials<double>(partial2, nbx, vt, v, lds);
+    if (threadIdx.x == 0 && 2 * m - 1 < nmom) mv[2 * m - 1] = 2.0 * s2 * inv_dim - mv[1];
+  }
+}
+
 }  // namespace
 
 // ================================================================================================ host side
@@ -1894,6 +1953,7 @@ struct asr_gpu_real {
   int nsite, nslot, nsp, vt = 0;
   bool v2 = true;    // Chebyshev steps with two vectors per lane
   bool s16 = false;  // ... on 16-vector slices (8 sites per wave)
+  bool dbl = false;  // kpm_moments by moment doubling (two moments per Chebyshev step)
   int* nbr = nullptr;
   unsigned char* spec = nullptr;
   double *par = nullptr, *blk = nullptr;
@@ -3918,6 +3978,13 @@ static dim3 kpm_grid(const asr_gpu_real* R, int vt, int nvp) {
   return dim3((R->nsite + sites_per_block - 1) / sites_per_block, nvp / vt);
 }
 
+// kpm_moments by moment doubling (two moments per Chebyshev step)
+int asr_gpu_real_set_doubling(asr_gpu_real* R, int on) {
+  if (!R) return fail("asr_gpu_real_set_doubling: null handle");
+  R->dbl = on != 0;
+  return 0;
+}
+
 // Vector tile of the KPM step (vectors per site in a wave: 8, 16, 32 or 64; 0 = automatic).
 int asr_gpu_real_set_tile(asr_gpu_real* R, int vt) {
   if (vt != 0 && vt != 1 && vt != 2 && vt != 3)
@@ -3958,12 +4025,14 @@ int asr_gpu_kpm_moments(asr_gpu_real* R, double c, double s, int nmom, int nvec,
   const int vt = 64;
   const int nvp = ((nvec + vt - 1) / vt) * vt;
   const dim3 grid = kpm_grid(R, vt, nvp);
-  double *t0, *t1, *t2, *part, *mper;
+  double *t0, *t1, *t2, *part, *mper, *part2 = nullptr;
   if (dalloc(&t0, dim * nvp) || dalloc(&t1, dim * nvp) || dalloc(&t2, dim * nvp) ||
       dalloc(&part, (size_t)grid.x * grid.y * vt) || dalloc(&mper, (size_t)nvec * nmom))
     return 1;
+  if (R->dbl && dalloc(&part2, (size_t)((R->nsite + 7) / 8) * grid.y * 64)) return 1;
   const double inv_dim = 1.0 / (double)dim;
-  const int nstep_ev = nmom > 2 ? nmom - 2 : 0;
+  // Chebyshev steps timed: t_2 .. t_(nmom-1), or t_2 .. t_(nmom/2) with the doubling
+  const int nstep_ev = R->dbl ? std::max(nmom / 2 - 1, 0) : (nmom > 2 ? nmom - 2 : 0);
   std::vector<hipEvent_t> ev(2 + 2 * nstep_ev);
   for (auto& e : ev) HCHECK(hipEventCreate(&e));
   const RealDev rd = real_dev(R);
@@ -3978,20 +4047,44 @@ int asr_gpu_kpm_moments(asr_gpu_real* R, double c, double s, int nmom, int nvec,
                        (const double*)nullptr, (const double*)nullptr, t1, c, 1.0 / s, 0.0, seed, part, 1);
     hipLaunchKernelGGL(k_kpm_fin, dim3(nvec), dim3(BLOCK), 0, R->stream, part, (int)grid.x, vt, nvec, inv_dim, mper, nmom, 1);
   }
-  for (int n = 2; n < nmom; ++n) {
+  if (R->dbl) {
+    // moment doubling: the step that makes t_m gives mu_2m and mu_2m-1, so t_2 .. t_M with 2 M >= nmom - 1 suffice
+    const int nbx2 = (int)((R->nsite + 7) / 8);
+    const dim3 g2((unsigned)nbx2, grid.y);
+    if (nmom > 2) {
+      hipLaunchKernelGGL(k_kpm_sq, g2, dim3(BLOCK), 0, R->stream, rd, nvp, (const double*)t1, part);
+      hipLaunchKernelGGL(k_kpm_fin2, dim3(nvec), dim3(BLOCK), 0, R->stream, (const double*)part, (const double*)nullptr,
+                         nbx2, vt, nvec, inv_dim, mper, nmom, 1);
+    }
+    const int M = nmom / 2;  // the largest m with 2 m - 1 <= nmom - 1
+    for (int m = 2; m <= M; ++m) {
+      HCHECK(hipEventRecord(ev[2 + 2 * (m - 2)], R->stream));
+      if (R->nslot == 18)
+        hipLaunchKernelGGL((k_kpm_v2<18, true>), g2, dim3(BLOCK), 0, R->stream, rd, (const double*)R->blk, nvec, nvp,
+                           (const double*)t1, (const double*)t0, t2, c, 2.0 / s, 1.0, seed, part, part2);
+      else
+        hipLaunchKernelGGL((k_kpm_v2<0, true>), g2, dim3(BLOCK), 0, R->stream, rd, (const double*)R->blk, nvec, nvp,
+                           (const double*)t1, (const double*)t0, t2, c, 2.0 / s, 1.0, seed, part, part2);
+      HCHECK(hipEventRecord(ev[3 + 2 * (m - 2)], R->stream));
+      hipLaunchKernelGGL(k_kpm_fin2, dim3(nvec), dim3(BLOCK), 0, R->stream, (const double*)part, (const double*)part2, nbx2,
+                         vt, nvec, inv_dim, mper, nmom, m);
+      double* tmp = t0; t0 = t1; t1 = t2; t2 = tmp;
+    }
+  }
+  for (int n = 2; n < nmom && !R->dbl; ++n) {
     // t2 = 2 (H t1 - c t1) / s - t0
     HCHECK(hipEventRecord(ev[2 + 2 * (n - 2)], R->stream));
     if (R->s16 && R->nslot == 18)
       hipLaunchKernelGGL((k_kpm_s16<18>), dim3((R->nsite + 31) / 32, nvp / 16), dim3(BLOCK), (size_t)19 * 90 * 8, R->stream, rd,
                          nvec, nvp, (const double*)t1, (const double*)t0, t2, c, 2.0 / s, 1.0, seed, part);
     else if (R->v2 && R->nslot == 18)
-      hipLaunchKernelGGL((k_kpm_v2<18>), dim3((R->nsite + 7) / 8, grid.y), dim3(BLOCK), 0, R->stream, rd,
+      hipLaunchKernelGGL((k_kpm_v2<18, false>), dim3((R->nsite + 7) / 8, grid.y), dim3(BLOCK), 0, R->stream, rd,
                          (const double*)R->blk, nvec, nvp, (const double*)t1, (const double*)t0, t2, c, 2.0 / s, 1.0, seed,
-                         part);
+                         part, (double*)nullptr);
     else if (R->v2)
-      hipLaunchKernelGGL((k_kpm_v2<0>), dim3((R->nsite + 7) / 8, grid.y), dim3(BLOCK), 0, R->stream, rd,
+      hipLaunchKernelGGL((k_kpm_v2<0, false>), dim3((R->nsite + 7) / 8, grid.y), dim3(BLOCK), 0, R->stream, rd,
                          (const double*)R->blk, nvec, nvp, (const double*)t1, (const double*)t0, t2, c, 2.0 / s, 1.0, seed,
-                         part);
+                         part, (double*)nullptr);
     else if (R->nslot == 18)
       hipLaunchKernelGGL((k_kpm_u<false, 18>), grid, dim3(BLOCK), 0, R->stream, rd, (const double*)R->blk, nvec, nvp,
                          (const double*)t1, (const double*)t0, t2, c, 2.0 / s, 1.0, seed, part, 1);
@@ -4025,6 +4118,7 @@ int asr_gpu_kpm_moments(asr_gpu_real* R, double c, double s, int nmom, int nvec,
   if (step_ms) *step_ms = nstep_ev ? ms_steps / nstep_ev : 0.0;
   if (total_ms) *total_ms = ms_all;
   for (double* p : {t0, t1, t2, part, mper}) (void)hipFree(p);
+  if (part2) (void)hipFree(part2);
   for (hipEvent_t e : ev) (void)hipEventDestroy(e);
   return 0;
 }

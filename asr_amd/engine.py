@@ -44,6 +44,7 @@ def lib() -> C.CDLL:
         L.asr_gpu_chains_edc_proj.argtypes = [P, C.c_int, P, P, P, P, P, P, P, C.c_int, C.c_int, P, P, P, P, P, P, P, C.c_int, P]
         L.asr_gpu_op_get_tile.argtypes = [P, C.c_int, C.c_int, P]
         L.asr_gpu_real_set_tile.argtypes = [P, C.c_int]
+        L.asr_gpu_real_set_doubling.argtypes = [P, C.c_int]
         L.asr_gpu_op_trim.argtypes = [P]
         L.asr_gpu_op_destroy.argtypes = [P]
         L.asr_gpu_op_destroy.restype = None
@@ -395,6 +396,10 @@ class GpuRealization:
 
     def set_tile(self, vt: int):
         _check(lib().asr_gpu_real_set_tile(self.h, int(vt)), "asr_gpu_real_set_tile")
+
+    def set_doubling(self, on: bool):
+        """kpm_moments by moment doubling (two moments per Chebyshev step; equal to the direct moments up to rounding)."""
+        _check(lib().asr_gpu_real_set_doubling(self.h, int(bool(on))), "asr_gpu_real_set_doubling")
 
     def apply(self, x: np.ndarray) -> np.ndarray:
         x = _f64(np.atleast_2d(x))

@@ -388,6 +388,22 @@ def main():
                       "mu_head": [round(float(v), 8) for v in res["mu"][:4]],
                       "dos": {"points": 4001, "ms": round(tdos * 1e3, 2), "integral": round(dos_norm, 6)}}
         log(f"kpm: {nmom} moments x 64 vectors on {g.n * 9} rows in {wk:.2f} s, step {res['step_ms']:.3f} ms, {kroof}")
+        # the same moments by moment doubling (mu_2m = 2 <t_m|t_m> - mu_0, mu_2m-1 = 2 <t_m|t_m-1> - mu_1): nmom / 2
+        # Chebyshev steps, no random vector in the step; the difference from the direct moments is rounding
+        R.set_doubling(True)
+        R.kpm_moments(cshift, sscale, 12, 64, seed=5)
+        t0 = time.perf_counter()
+        rd = R.kpm_moments(cshift, sscale, nmom, 64, seed=5)
+        wd = time.perf_counter() - t0
+        R.set_doubling(False)
+        dby = g.n * 9 * 64 * 8 * 3.0  # read t_m (its 19 gathers counted once), read t_m-1, write t_m+1
+        droof = roofline(dby, kfl, rd["step_ms"])
+        droof["kernel"] = "k_kpm_v2<%d, DBL>" % g.nbr.shape[1]
+        droof["algorithmic_bytes_per_launch"] = round(dby)
+        out["kpm"]["doubling"] = {"value": round(64 * nmom / wd, 1), "unit": "vector-moments/s", "wall_s": round(wd, 3),
+                                  "steps": nmom // 2 - 1, "step_ms": round(rd["step_ms"], 4), "roofline": droof,
+                                  "max_abs_diff_vs_direct": float(np.max(np.abs(rd["mu"] - res["mu"])))}
+        log(f"kpm doubling: {out['kpm']['doubling']}")
         R.close()
 
     if not args.no_adjoint:

@@ -181,6 +181,11 @@ int asr_gpu_real_create(int nsite, int nslot, const int* nbr, int nsp, const uns
 void asr_gpu_real_destroy(asr_gpu_real* r);
 /* Vectors per lane in the Chebyshev steps: 1, 2, or 0 = automatic (2: 16-byte loads, two sites per wave). */
 int asr_gpu_real_set_tile(asr_gpu_real* r, int vt);
+/* Moment doubling for asr_gpu_kpm_moments (on != 0): the Chebyshev step that makes t_m = T_m(H~) r also gives
+ * mu_2m = 2 <t_m|t_m> / dim - mu_0 and mu_2m-1 = 2 <t_m|t_m-1> / dim - mu_1 (T_2m = 2 T_m^2 - T_0, T_2m-1 = 2 T_m T_m-1 - T_1),
+ * so nmom moments take nmom / 2 steps instead of nmom - 1.  The moments equal those of kpm_moments
+ * (src/reference/asr_kpm.f90:39-65, mu_n = <r|t_n> / dim) up to rounding; the default (off) computes them as it does. */
+int asr_gpu_real_set_doubling(asr_gpu_real* r, int on);
 /* y = h x for nvec vectors (h_apply, src/reference/asr_lmto_realization.f90:120-147); x, y: nvec x 9 nsite. */
 int asr_gpu_real_apply(asr_gpu_real* r, int nvec, const double* x, double* y);
 /* Chebyshev moments with nvec Rademacher vectors as one batched SpMM per moment (kpm_moments,

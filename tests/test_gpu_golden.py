@@ -25,11 +25,12 @@ def test_gpu_kpm_reproduces_exact_sc_spectrum():
     eng = _engine()
     g, spec, par, block, block0 = _sc_band(12)
     R = eng.GpuRealization(g.nbr, spec, par, block, block0)
-    for vt in (1, 2):
+    for vt, dbl in ((1, False), (2, False), (2, True)):
         R.set_tile(vt)
+        R.set_doubling(dbl)  # moment doubling: two moments per Chebyshev step
         res = R.kpm_moments(0.0, 6.3, 400, 60, seed=5)
         assert abs(res["mu"][0] - 1.0) < 1e-14
-        assert kpm_dos_error(res["mu"]) < 2.5e-2, vt
+        assert kpm_dos_error(res["mu"]) < 2.5e-2, (vt, dbl)
     R.close()
 
 

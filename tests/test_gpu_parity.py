@@ -154,6 +154,13 @@ def test_kpm_apply_and_moments(tile):
     assert np.max(np.abs(res["mu_per"] - per_o)) < 1e-10
     assert np.max(np.abs(res["mu"] - mu_o)) < 1e-10
     assert abs(res["mu"][0] - 1.0) < 1e-14
+    # moment doubling (mu_2m = 2 <t_m|t_m> - mu_0, mu_2m-1 = 2 <t_m|t_m-1> - mu_1): the direct moments up to rounding, for
+    # an even and an odd count
+    R.set_doubling(True)
+    for nm in (48, 47, 3):
+        rd = R.kpm_moments(c, s, nm, 5, seed=7, per_vector=True)
+        assert np.max(np.abs(rd["mu_per"] - per_o[:, :nm])) < 1e-10, nm
+    R.set_doubling(False)
 
 
 @pytest.mark.parametrize("tile", [8, 16, 64, 65])
