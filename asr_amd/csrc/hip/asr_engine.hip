@@ -2980,6 +2980,7 @@ int run_multi(int ngroup, asr_gpu_op* const* ops, const int* ngch, const int* st
   std::vector<hipStream_t> st(ngroup, nullptr);
   std::vector<ChainRun<T>> runs(ngroup);
   auto cleanup = [&]() {
+    runs.clear();  // the runs synchronise on their streams when they release their buffers: before the streams go
     for (auto s2 : st)
       if (s2) (void)hipStreamDestroy(s2);
   };
