@@ -79,6 +79,7 @@ struct OpDev {
   const uint32_t* hop;   // nn x nslot: (node + 1) << 6 | op, 0 = no link
   const uint32_t* flip;  // nn x nsmax, same packing
   const uint32_t* info;  // nn: lst | ncl << 4 | ntyp << 12 | fz << 20 (fz: frozen-site slot 0 none, 1 start, 2 partner)
+                         //     | INFO_PARTNERS (the node has a flip partner other than itself)
   const double* sq;      // nn + 1 (sq[nn] = 1: the padding node)
   const double* isq;     // nn + 1: 1 / sq
   const uint32_t* sgm;   // nop: bit l set when sg(l, g) < 0
@@ -108,6 +109,8 @@ struct MemDev {
 
 // The frozen-site slot of node info inf for member m: 1 (the start site), 2 (a frozen pair partner, augcls_freeze_any)
 // when the member freezes that site, else 0.
+constexpr uint32_t INFO_PARTNERS = 1u << 24;
+
 __device__ inline int frozen_slot(uint32_t inf, const MemDev& md, int m) {
   const int s = (int)((inf >> 20) & 3u);
   return (s && md.fsp[(size_t)(s - 1) * md.nbp + m] >= 0) ? s : 0;
@@ -262,6 +265,11 @@ struct FArgs {
   int out_sq;
   const void* dotu;  // partial dot <dotu, out> per member (bilinear)
   void* partial;
+  // k_tile only: out2 = c2 X_q2(own state) out at the nodes without flip partners (the species-diagonal term of the
+  // output when it is node-local; the nodes with partners are left to k_rand_nodes)
+  void* out2;
+  int q2;
+  double c2;
 };
 
 template <class T>
@@ -542,16 +550,65 @@ __global__ void __launch_bounds__(BLOCK, 4) k_tile(OpDev op, MemDev md, FArgs fa
     const double so = fa.out_sq ? op.sq[i] : 1.0;
     T* out = (T*)fa.out;
     const T* du = (const T*)fa.dotu;
+    // out2: the own-state term of rand_ct on the output row (amplitude coordinates, no partner rows)
+    T* out2 = (fa.out2 && !(inf & INFO_PARTNERS)) ? (T*)fa.out2 : nullptr;
+    const double* w2 = nullptr;
+    if (out2) {
+      const int fs = frozen_slot(inf, md, m);
+      const int l0 = inf & 15u, kap = (inf >> 4) & 255u, nsm = op.nsmax;
+      w2 = fs ? frozen_row(md, fs, fa.q2) + m : md.wts + ((((size_t)fa.q2 * op.ncls + kap) * nsm + l0) * nsm + l0) * 9 * nbp + m;
+    }
     for (int l = 0; l < 9; ++l) {
       const T y = so * acc[l];
       out[((size_t)i * 9 + l) * nbp + m] = y;
       if (du) dotv = dotv + bilin(ld(du, i, l, nbp, m), y);
+      if (out2) out2[((size_t)i * 9 + l) * nbp + m] = fma_s(fa.c2 * w2[(size_t)l * nbp], 1.0 * y, zero<T>());
     }
   }
   if (fa.dotu) {
     T* part = (T*)fa.partial;
     block_reduce_member<T>(dotv, MT, red, part + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * MT + threadIdx.x);
   }
+}
+
+// The species-diagonal term y = coef X_q x (rand_ct without normalisation) at the listed nodes only: the nodes with flip
+// partners, whose rows k_tile's out2 leaves out.  One thread per (listed node, member), members fastest.  Same operation
+// order as rand_ct, so the rows equal those of a full pass.
+template <class T>
+__global__ void k_rand_nodes(OpDev op, MemDev md, const int* __restrict__ list, int nlist, const T* __restrict__ x,
+                             T* __restrict__ y, int q, double coef) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int nbp = md.nbp, nsm = op.nsmax;
+  const int j = (int)(t / nbp), m = (int)(t % nbp);
+  if (j >= nlist || m >= md.nb) return;
+  if (md.tact && !md.tact[m / md.mt]) return;
+  const int i = list[j];
+  const uint32_t inf = op.info[i];
+  T acc[9];
+  for (int l = 0; l < 9; ++l) acc[l] = zero<T>();
+  const int fs = frozen_slot(inf, md, m);
+  if (fs) {
+    const double* fa_ = frozen_row(md, fs, q) + m;
+    for (int l = 0; l < 9; ++l) acc[l] = fma_s(coef * fa_[(size_t)l * nbp], 1.0 * x[((size_t)i * 9 + l) * nbp + m], acc[l]);
+  } else {
+    const int l0 = inf & 15u, kap = (inf >> 4) & 255u;
+    const double* wa_ = md.wts + (((size_t)q * op.ncls + kap) * nsm + l0) * nsm * 9 * nbp + m;
+    for (int l = 0; l < 9; ++l)
+      acc[l] = fma_s(coef * wa_[((size_t)l0 * 9 + l) * nbp], 1.0 * x[((size_t)i * 9 + l) * nbp + m], acc[l]);
+    const uint64_t chim = md.chimask[m];
+    const int nsc = op.nsc[kap];
+    const uint32_t* fl = op.flip + (size_t)i * nsm;
+    for (int k = 0; k < nsc; ++k) {
+      const uint32_t e = fl[k];
+      if (k == l0 || e == 0u) continue;
+      const int r = (int)(e >> 6) - 1, g = (int)(e & 63u);
+      const double chi = ((chim >> g) & 1ull) ? -1.0 : 1.0;
+      for (int l = 0; l < 9; ++l)
+        acc[l] = fma_s(coef * wa_[((size_t)k * 9 + l) * nbp], (1.0 * (chi * op.sg[g * 9 + l])) * x[((size_t)r * 9 + l) * nbp + m],
+                       acc[l]);
+    }
+  }
+  for (int l = 0; l < 9; ++l) y[((size_t)i * 9 + l) * nbp + m] = acc[l];
 }
 
 // The same fused operator pass with one node per wave and 64 members per wave (member tile 64): the node's links, weights
@@ -1936,6 +1993,8 @@ struct asr_gpu_op {
   double *sq = nullptr, *isq = nullptr, *sg = nullptr, *blk = nullptr, *blkp = nullptr;
   uint32_t* sgm = nullptr;
   int *stab_ptr = nullptr, *stab = nullptr, *pi = nullptr, *nsc = nullptr;
+  int nplist = 0;                   // nodes with flip partners (INFO_PARTNERS) and their list on the device
+  int* plist = nullptr;
   int ntarget = 0;                  // target states of the projected chains (asr_gpu_op_set_targets)
   int *tnode = nullptr, *tgop = nullptr;  // ntarget x nsmax: node (-1 none) and operation of target t in state k
   hipStream_t stream = nullptr;
@@ -2196,7 +2255,7 @@ KTimer* g_timer = nullptr;
 std::vector<double> g_acc;  // summed duration per kernel ordinal of a level
 int g_levels = 0;
 
-enum { KIND_SPMM = 0, KIND_RAND = 1, KIND_FIN_A = 2, KIND_UPDATE = 3, KIND_FIN_B = 4, KIND_SCALE = 5 };
+enum { KIND_SPMM = 0, KIND_RAND = 1, KIND_FIN_A = 2, KIND_UPDATE = 3, KIND_FIN_B = 4, KIND_SCALE = 5, KIND_FIX = 6 };
 
 int timer_mark(hipStream_t s, int kind, bool start) {
   if (!g_timer) return 0;
@@ -2222,8 +2281,15 @@ FArgs fargs0() {
   return f;
 }
 
+// the operator pass runs k_tile (member tile 8 or 16, orbital-preserving group, 14 or 18 slots, vectors under 2 GB)
+bool tile_kernel(const asr_gpu_op* op, const MemDev& md, size_t es) {
+  return !(md.mt == WAVE) && (md.mt == 8 || md.mt == 16) && op->piid && (op->nslot == 18 || op->nslot == 14) &&
+         (size_t)(op->nn + 1) * 9 * md.nbp * es < ((size_t)1 << 31);
+}
+
 template <class T>
 int launch_fused(const asr_gpu_op* op, const MemDev& md, const FArgs& fa, hipStream_t strm) {
+  if (fa.out2 && !tile_kernel(op, md, sizeof(T))) return fail("operator pass: out2 needs the tile kernel");
   dim3 grid = node_grid(op, md);
   size_t lds = fused_lds_bytes(op, sizeof(T));
   if (timer_mark(strm, fa.xs ? KIND_SPMM : KIND_RAND, true)) return 1;
@@ -2249,8 +2315,7 @@ int launch_fused(const asr_gpu_op* op, const MemDev& md, const FArgs& fa, hipStr
     else if (op->nslot == 18) ASR_NODE(false, 18);
     else ASR_NODE(false, 0);
 #undef ASR_NODE
-  } else if ((md.mt == 8 || md.mt == 16) && op->piid && (op->nslot == 18 || op->nslot == 14) &&
-             (size_t)(op->nn + 1) * 9 * md.nbp * sizeof(T) < ((size_t)1 << 31)) {
+  } else if (tile_kernel(op, md, sizeof(T))) {
 #define ASR_CT(MT, NS) hipLaunchKernelGGL((k_tile<T, MT, NS>), grid, dim3(BLOCK), blds_bytes + (size_t)MT * (2 * op->nop + 1) * 9 * 8, \
                                            strm, od, md, fa, (const double*)op->blkp)
     if (md.mt == 8 && op->nslot == 18) ASR_CT(8, 18);
@@ -2271,13 +2336,27 @@ int launch_fused(const asr_gpu_op* op, const MemDev& md, const FArgs& fa, hipStr
 template <class T>
 int apply_chain_op(const asr_gpu_op* op, const Members& M, const MemDev& md, const T* x, T* y, T* t1, T* t2, T* h1, bool lin,
                    const T* dotu, T* partial, hipStream_t strm) {
-  // h1 = h x~ = rand_c(x~) + D S D x~
+  // h1 = h x~ = rand_c(x~) + D S D x~; with the tile kernel the same launch writes t1 = rand_o(h1) at the nodes without
+  // flip partners (99 % of them), and k_rand_nodes the rest, in place of a full rand_o pass
+  bool t1_done = false;
   if (M.uniform_d) {
     FArgs f = fargs0();
     f.xs = x; f.xs_norm = 1; f.s_din = 1; f.s_dout = 1; f.s_coef = 1.0;
     f.xr[0] = x; f.qr[0] = QC; f.r_norm[0] = 1; f.r_coef[0] = 1.0; f.nr = 1;
     f.out = h1;
+    if (!M.zero_o && tile_kernel(op, md, sizeof(T))) {
+      f.out2 = t1; f.q2 = QO; f.c2 = 1.0;
+      t1_done = true;
+    }
     if (launch_fused<T>(op, md, f, strm)) return 1;
+    if (t1_done && op->nplist > 0) {
+      if (timer_mark(strm, KIND_FIX, true)) return 1;
+      const size_t nt = (size_t)op->nplist * md.nbp;
+      hipLaunchKernelGGL(k_rand_nodes<T>, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, strm, dev_of(op), md,
+                         (const int*)op->plist, op->nplist, (const T*)h1, t1, (int)QO, 1.0);
+      HCHECK(hipGetLastError());
+      if (timer_mark(strm, 0, false)) return 1;
+    }
   } else {
     FArgs f = fargs0();
     f.xr[0] = x; f.qr[0] = QD; f.r_norm[0] = 1; f.r_coef[0] = 1.0; f.nr = 1; f.out = t1;
@@ -2297,7 +2376,7 @@ int apply_chain_op(const asr_gpu_op* op, const Members& M, const MemDev& md, con
   if (!M.zero_o) {
     FArgs g = fargs0();
     g.xr[0] = h1; g.qr[0] = QO; g.r_norm[0] = 0; g.r_coef[0] = 1.0; g.nr = 1; g.out = t1;  // r = rand_o(h1)
-    if (launch_fused<T>(op, md, g, strm)) return 1;
+    if (!t1_done && launch_fused<T>(op, md, g, strm)) return 1;
     if (M.uniform_d) {
       f.xs = t1; f.s_din = 1; f.s_dout = 1; f.s_coef = -1.0;
     } else {
@@ -3404,6 +3483,7 @@ int asr_gpu_op_set_tables(asr_gpu_op* op, const int* hopr, const int* hopg, cons
                           const int* stab, int nstab) {
   const int nn = op->nn, ns = op->nslot, nsm = op->nsmax;
   std::vector<uint32_t> hop((size_t)nn * ns), flip((size_t)nn * nsm), info(nn);
+  std::vector<int> plist;  // the nodes with a flip partner other than themselves
   for (size_t q = 0; q < hop.size(); ++q) {
     if (hopr[q] >= nn || (hopr[q] >= 0 && (hopg[q] < 0 || hopg[q] >= op->nop))) return fail("set_tables: hop link out of range");
     // a missing link points at the padding node nn, whose amplitudes are zero in every vector
@@ -3426,6 +3506,12 @@ int asr_gpu_op_set_tables(asr_gpu_op* op, const int* hopr, const int* hopg, cons
       if (flips) op->slot_flippable[fz[i] - 1] = true;
     }
     info[i] = (uint32_t)lst[i] | ((uint32_t)ncl[i] << 4) | ((uint32_t)ntyp[i] << 12) | ((uint32_t)(fz ? fz[i] : 0) << 20);
+    bool partners = false;
+    for (int k = 0; k < nsm; ++k) partners |= (k != lst[i] && flipr[(size_t)i * nsm + k] >= 0);
+    if (partners) {
+      info[i] |= INFO_PARTNERS;
+      plist.push_back(i);
+    }
   }
   for (int i = 0; i <= nn; ++i)
     if (stab_ptr[i] < 0 || stab_ptr[i] > nstab || (i > 0 && stab_ptr[i] < stab_ptr[i - 1]))
@@ -3442,6 +3528,11 @@ int asr_gpu_op_set_tables(asr_gpu_op* op, const int* hopr, const int* hopg, cons
   HCHECK(hipMemcpy(op->hop, hop.data(), hop.size() * 4, hipMemcpyHostToDevice));
   HCHECK(hipMemcpy(op->flip, flip.data(), flip.size() * 4, hipMemcpyHostToDevice));
   HCHECK(hipMemcpy(op->info, info.data(), info.size() * 4, hipMemcpyHostToDevice));
+  if (op->plist) (void)hipFree(op->plist);
+  op->plist = nullptr;
+  op->nplist = (int)plist.size();
+  if (dalloc(&op->plist, std::max(op->nplist, 1))) return 1;
+  if (op->nplist) HCHECK(hipMemcpy(op->plist, plist.data(), plist.size() * 4, hipMemcpyHostToDevice));
   HCHECK(hipMemcpy(op->sq, sq, (size_t)nn * 8, hipMemcpyHostToDevice));
   {
     const double one = 1.0;
@@ -3621,6 +3712,7 @@ int asr_gpu_op_trim(asr_gpu_op* op) {
 }
 
 void asr_gpu_op_destroy(asr_gpu_op* op) {
+  if (op && op->plist) (void)hipFree(op->plist);
   if (op && op->tnode) (void)hipFree(op->tnode);
   if (op && op->tgop) (void)hipFree(op->tgop);
   if (!op) return;

@@ -35,10 +35,13 @@ def log(*a):
 # vectors read by the operator launches of one level of the route-edc chains (uniform d, o != 0, X(z) diagonal):
 # launch 0: h1 = rand_c(x~) + D S D x~ (reads x); launch 1: t1 = rand_o(h1) (reads h1); launch 2: w = sq (h1 - h t1
 # + X(z) x~) with the dot <u, w> (reads t1, h1 and x = u).  Each writes one vector.
-EDC_LAUNCH_READS = {0: 1, 1: 1, 2: 3}
+# (vectors read, vectors written) per launch of the level's operator pass: with the tile kernel the first S pass also
+# writes the node-local species-diagonal term (launch 1 is the small pass over the nodes with flip partners, kind 6)
+EDC_LAUNCH_IO = {0: (1, 1), 1: (1, 1), 2: (3, 1)}
+EDC_LAUNCH_IO_FUSED = {0: (1, 2), 2: (3, 1)}
 
 
-def chain_kernel_cost(op, nb: int, kind: int, nread: int = 1):
+def chain_kernel_cost(op, nb: int, kind: int, nread: int = 1, nwrite: int = 1):
     """Algorithmic bytes and flops of one launch of a chain-level kernel over nn nodes and nb complex members.
 
     Operator pass (kind 0, S pass + flip terms; kind 1, flip terms only): read nread vectors once (9 nb complex per node
@@ -49,14 +52,20 @@ def chain_kernel_cost(op, nb: int, kind: int, nread: int = 1):
     vec = nn * 9 * nb * 16
     tables = nn * (4 * ns + 4 * nsm + 4 + 8)
     if kind == 0:
-        bytes_ = (nread + 1) * vec + tables
-        flops = nn * nb * ((1 + ns) * 81 * 4 + nsm * 9 * 4 + (1 + ns) * 9 * 2 * 2 + 9 * 4)
+        bytes_ = (nread + nwrite) * vec + tables
+        flops = nn * nb * ((1 + ns) * 81 * 4 + nsm * 9 * 4 + (1 + ns) * 9 * 2 * 2 + 9 * 4 + (nwrite - 1) * 9 * 2)
     elif kind == 1:
         bytes_ = (nread + 1) * vec + nn * (4 * nsm + 4 + 8)
         flops = nn * nb * (nsm * 9 * 4)
     elif kind == 3:
         bytes_ = 4 * vec
         flops = nn * nb * 9 * 16
+    elif kind == 6:
+        # the species-diagonal term at the nodes with flip partners only: own and partner rows read, one row written
+        fr = np.asarray(op.flipr).reshape(nn, nsm)
+        npf = int(np.sum(((fr >= 0) & (fr != np.arange(nn)[:, None])).any(axis=1)))
+        bytes_ = npf * 9 * nb * 16 * (nsm + 1)
+        flops = npf * nb * nsm * 9 * 4
     else:
         bytes_ = 2 * vec
         flops = nn * nb * 9 * 6
@@ -175,8 +184,9 @@ def main():
     # per-kernel durations of the level step (HIP events on the engine's stream)
     kms, kinds = G.profile(wl.members, wl.nlev)
     per_kernel = []
+    io = EDC_LAUNCH_IO_FUSED if 6 in [int(k) for k in kinds] else EDC_LAUNCH_IO
     for q, (t_ms, kd) in enumerate(zip(kms, kinds)):
-        by, fl = chain_kernel_cost(wl.op, nb, int(kd), EDC_LAUNCH_READS.get(q, 1))
+        by, fl = chain_kernel_cost(wl.op, nb, int(kd), *io.get(q, (1, 1)))
         per_kernel.append(dict(launch=q, kind=int(kd), ms=round(float(t_ms), 4), gbs=round(by / (t_ms * 1e-3) / 1e9, 1),
                                tfs=round(fl / (t_ms * 1e-3) / 1e12, 3)))
     # the dominant kernel is the operator pass (every launch of kinds 0 and 1 is the same kernel: the S pass with the flip
@@ -185,7 +195,7 @@ def main():
     ops = [k for k in per_kernel if k["kind"] in (0, 1)]
     tb = tf = tm = 0.0
     for k in ops:
-        by, fl = chain_kernel_cost(wl.op, nb, k["kind"], EDC_LAUNCH_READS.get(k["launch"], 1))
+        by, fl = chain_kernel_cost(wl.op, nb, k["kind"], *io.get(k["launch"], (1, 1)))
         tb, tf, tm = tb + by, tf + fl, tm + k["ms"]
     roof = roofline(tb / len(ops), tf / len(ops), tm / len(ops))
     piid = bool(np.all(np.asarray(wl.op.pi) == np.arange(9)[None, :]))
