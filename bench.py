@@ -95,7 +95,7 @@ def roofline(bytes_, flops, ms):
 
 def measured_traffic(workload: str, kernel: str, nodes: int, members: int):
     """HBM bytes per launch of a kernel measured with rocprofv3 --pmc (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled
-    by the same counter's reading for a streaming kernel of known size: k_scale, which reads and writes one chain vector),
+    by the same counter's reading for a streaming kernel of known size: k_rescale, which reads and writes one chain vector),
     kept as text under profiles/ for the workload it was measured on; None when there is no measurement for this kernel
     and size."""
     path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
@@ -387,7 +387,7 @@ def main():
         dos_norm = float(np.sum(rho) * (egrid[1] - egrid[0]))
         kby, kfl = kpm_kernel_cost(g.n, g.nbr.shape[1], 64)
         kroof = roofline(kby, kfl, res["step_ms"])
-        kroof["kernel"] = {1: "k_kpm_u<false, %d>", 2: "k_kpm_v2<%d>", 3: "k_kpm_s16<%d>"}[vt_best] % g.nbr.shape[1]
+        kroof["kernel"] = {1: "k_kpm_u<false, %d>", 2: "k_kpm_v2<%d, false>", 3: "k_kpm_s16<%d>"}[vt_best] % g.nbr.shape[1]
         kroof["algorithmic_bytes_per_launch"] = round(kby)
         kroof["traffic"] = measured_traffic("kpm", kroof["kernel"], g.n, 64)
         out["kpm"] = {"value": round(64 * nmom / wk, 1), "unit": "vector-moments/s", "rows": g.n * 9, "nvec": 64,
@@ -408,8 +408,9 @@ def main():
         R.set_doubling(False)
         dby = g.n * 9 * 64 * 8 * 3.0  # read t_m (its 19 gathers counted once), read t_m-1, write t_m+1
         droof = roofline(dby, kfl, rd["step_ms"])
-        droof["kernel"] = "k_kpm_v2<%d, DBL>" % g.nbr.shape[1]
+        droof["kernel"] = "k_kpm_v2<%d, true>" % g.nbr.shape[1]
         droof["algorithmic_bytes_per_launch"] = round(dby)
+        droof["traffic"] = measured_traffic("kpm", droof["kernel"], g.n, 64)
         out["kpm"]["doubling"] = {"value": round(64 * nmom / wd, 1), "unit": "vector-moments/s", "wall_s": round(wd, 3),
                                   "steps": nmom // 2 - 1, "step_ms": round(rd["step_ms"], 4), "roofline": droof,
                                   "max_abs_diff_vs_direct": float(np.max(np.abs(rd["mu"] - res["mu"])))}
