@@ -24,6 +24,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -2851,16 +2852,36 @@ struct ChainRun {
       if (gprev.size() != (size_t)nb) gprev.assign(nb, cplx{0.0, 0.0});
       if (fprev.size() != (size_t)nb) fprev.assign(nb, cplx{0.0, 0.0});
       std::vector<double> cz(2 * k1, 0.0);
+      // G_k and f_k of every member still running, on a few host threads (independent continued fractions)
+      std::vector<int> idx;
+      for (int q = 0; q < nb; ++q)
+        if (act[q] && k1 < dep[q]) idx.push_back(q);
+      std::vector<cplx> gv(nb), fv(nb);
+      auto eval = [&](size_t lo, size_t hi) {
+        for (size_t t = lo; t < hi; ++t) {
+          const int q = idx[t];
+          double g[2], f[2];
+          const double zz[2] = {M.z[q].re, M.z[q].im};
+          const double* cq = ed3 ? (const double*)&ch[(size_t)q * ld] : cz.data();
+          // the status counts bp fallbacks (hull values then), not errors
+          (void)asr_host_cf_species((const double*)&ah[(size_t)q * ld], (const double*)&bh[(size_t)q * ld], cq, k1, zz,
+                                    op->depth_tail, g, f, nullptr, nullptr);
+          gv[q] = cplx{g[0], g[1]};
+          fv[q] = cplx{f[0], f[1]};
+        }
+      };
+      const size_t nidx = idx.size();
+      const int nth = (int)std::min<size_t>(std::max(1u, std::min(16u, std::thread::hardware_concurrency())), nidx / 16 + 1);
+      if (nth > 1) {
+        std::vector<std::thread> th;
+        for (int w = 0; w < nth; ++w) th.emplace_back(eval, nidx * w / nth, nidx * (w + 1) / nth);
+        for (auto& t : th) t.join();
+      } else {
+        eval(0, nidx);
+      }
       bool changed = false;
-      for (int q = 0; q < nb; ++q) {
-        if (!act[q] || k1 >= dep[q]) continue;
-        double g[2], f[2];
-        const double zz[2] = {M.z[q].re, M.z[q].im};
-        const double* cq = ed3 ? (const double*)&ch[(size_t)q * ld] : cz.data();
-        // the status counts bp fallbacks (hull values then), not errors
-        (void)asr_host_cf_species((const double*)&ah[(size_t)q * ld], (const double*)&bh[(size_t)q * ld], cq, k1, zz,
-                                  op->depth_tail, g, f, nullptr, nullptr);
-        const cplx gt{g[0], g[1]}, ft{f[0], f[1]};
+      for (const int q : idx) {
+        const cplx gt = gv[q], ft = fv[q];
         if (k1 > 8) {
           const cplx dg = gt - gprev[q], df = ft - fprev[q];
           const double ag = std::hypot(gt.re, gt.im);

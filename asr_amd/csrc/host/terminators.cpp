@@ -377,6 +377,22 @@ extern "C" int asr_host_contour_closed(double ebot, double ef, double eta, int n
   return 0;
 }
 
+// G(z) of n complex-symmetric chains at once (cf_species per chain, the tail of every chain the same; the chains are
+// rows of a, b of leading dimension ld with m[j] levels): the host end of a contour pass.  Chains in parallel.
+extern "C" int asr_host_cf_species_batch(int n, const double* a, const double* b, int ld, const int* m, const double* z,
+                                         int tail, double* g11) {
+  if (n < 0 || ld < 1) return -1;
+  std::vector<double> zero((size_t)2 * ld, 0.0);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int j = 0; j < n; ++j) {
+    const int mj = std::max(1, std::min(m[j], ld));
+    double f[2];
+    (void)asr_host_cf_species(a + (size_t)2 * ld * j, b + (size_t)2 * ld * j, zero.data(), mj, z + 2 * (size_t)j, tail,
+                              g11 + 2 * (size_t)j, f, nullptr, nullptr);
+  }
+  return 0;
+}
+
 // Semicircle from E_bot to E_F through the upper half plane, optionally stopped at Im z = eta_end (eta_end <= 0: down to the
 // real axis) (contour_nodes, src/asr_contour.f90:17-34).
 extern "C" int asr_host_contour_nodes(double ebot, double ef, int n, double eta_end, double* z_, double* w_) {

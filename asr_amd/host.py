@@ -28,6 +28,7 @@ def lib() -> C.CDLL:
         L.asr_host_tables_free.restype = None
         L.asr_host_cf_species.argtypes = [P, P, P, C.c_int, P, C.c_int, P, P, P, P]
         L.asr_host_cf_closed.argtypes = [P, P, P, C.c_int, P, P, P]
+        L.asr_host_cf_species_batch.argtypes = [C.c_int, P, P, C.c_int, P, P, C.c_int, P]
         L.asr_host_tables_targets.argtypes = [P, C.c_int, P, P, P]
         L.asr_host_bp_asymptote.argtypes = [P, P, C.c_int, P, P, P]
         L.asr_host_terminated_green.argtypes = [P, P, C.c_int, C.c_double, C.c_double, C.c_int, P, P]
@@ -103,6 +104,19 @@ def cf_species(a, b, c, m: int, z: complex, tail: str = "hull", want_pv: bool = 
     if want_pv:
         return g[0], f[0], pv[:m]
     return g[0], f[0]
+
+
+def cf_species_batch(a, b, m, z, tail: str = "hull") -> np.ndarray:
+    """g11 of every chain (rows of a, b; m[j] levels; energies z), cf_species with the same tail, in parallel on the host
+    (asr_host_cf_species_batch)."""
+    a = np.ascontiguousarray(np.atleast_2d(a), dtype=np.complex128)
+    b = np.ascontiguousarray(np.atleast_2d(b), dtype=np.complex128)
+    n, ld = a.shape
+    m = np.ascontiguousarray(np.broadcast_to(np.asarray(m), (n,)), dtype=np.int32)
+    z = np.ascontiguousarray(np.broadcast_to(np.asarray(z, dtype=np.complex128), (n,)))
+    g = np.zeros(n, np.complex128)
+    lib().asr_host_cf_species_batch(int(n), _p(a), _p(b), int(ld), _p(m), _p(z), TAIL[tail], _p(g))
+    return g
 
 
 def cf_closed(a, b, c, m: int, z: complex):

@@ -211,9 +211,9 @@ def chain_greens(zs, classes: list[ChainClass], comps: list[Component], params: 
                     nlevs += int(mu.sum())
                 else:
                     a, b, mu, ms = G.chains_edc(members, nlev, project=project, mode=mode)
+                    g = host.cf_species_batch(a, b, mu, [zs[t[0]] for t in chunk], tail)
                     for q, (j, o, pp, gg) in enumerate(chunk):
-                        m = int(mu[q])
-                        gz[j, o, pp, gg] = host.cf_species(a[q], b[q], np.zeros(max(m, 1)), m, zs[j], tail)[0]
+                        gz[j, o, pp, gg] = g[q]
                     nlevs += int(mu.sum())
                 nchain += len(chunk) * (2 if ed3 else 1)
                 dev_ms += ms

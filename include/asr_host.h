@@ -38,6 +38,11 @@ int asr_host_cf_species(const double* a, const double* b, const double* c, int m
 /* The same for a chain that closed (b_m < tiny, Krylov space exhausted): the finite fraction without a tail, cf_closed
  * (src/asr_ed3.f90:214-230). */
 int asr_host_cf_closed(const double* a, const double* b, const double* c, int m, const double* z, double* g11, double* f);
+/* G(z) of n chains at once (cf_species per chain with the same tail, src/asr_chain.f90:162-262): a, b n x ld complex,
+   m[j] levels of chain j, z n complex, g11 n complex out.  The host end of a contour pass (chain_greens,
+   src/asr_class_scf.f90:1127-1150).  Returns 0 (a tail that cannot be formed falls back as in cf_species). */
+int asr_host_cf_species_batch(int n, const double* a, const double* b, int ld, const int* m, const double* z, int tail,
+                              double* g11);
 /* Beer-Pettifor asymptote: bp_asymptote (src/reference/asr_terminator.f90:39-67).  1 = not converged. */
 int asr_host_bp_asymptote(const double* a, const double* b, int n, double* ainf, double* binf, int* niter);
 /* G(z) of a real chain with a square-root tail: terminated_green (src/reference/asr_terminator.f90:117-126). */

@@ -251,3 +251,19 @@ def test_cf_closed_is_the_resolvent_of_the_chain_matrix():
     assert abs(go - g) < 1e-14 and abs(fo - f) < 1e-13
     gs, fs = host.cf_species(a, b, c, m, z, "none")
     assert abs(gs - g) < 1e-14 and abs(fs - f) < 1e-13
+
+
+def test_cf_species_batch_equals_single():
+    """The batched host terminator (asr_host_cf_species_batch, the end of a contour pass) gives each chain's
+    cf_species value bit for bit, for chains of different lengths and energies and every tail."""
+    rng = np.random.default_rng(8)
+    n, ld = 37, 24
+    a = rng.uniform(-0.3, 0.3, (n, ld)) + 1j * rng.uniform(-0.05, 0.0, (n, ld))
+    b = rng.uniform(0.2, 0.5, (n, ld)) + 1j * rng.uniform(-0.02, 0.02, (n, ld))
+    m = rng.integers(1, ld + 1, n)
+    z = rng.uniform(-0.6, 0.6, n) + 1j * rng.uniform(0.01, 0.3, n)
+    for tail in ("hull", "avg2", "none"):
+        g = host.cf_species_batch(a, b, m, z, tail)
+        for j in range(n):
+            gj, _ = host.cf_species(a[j], b[j], np.zeros(max(m[j], 1)), int(m[j]), z[j], tail)
+            assert g[j] == gj, (tail, j)
