@@ -266,11 +266,15 @@ struct FArgs {
   int out_sq;
   const void* dotu;  // partial dot <dotu, out> per member (bilinear)
   void* partial;
-  // k_tile only: out2 = c2 X_q2(own state) out at the nodes without flip partners (the species-diagonal term of the
-  // output when it is node-local; the nodes with partners are left to k_rand_nodes)
+  // k_tile only: out2 = the node-local species-diagonal terms at the nodes without flip partners (the nodes with partners
+  // are left to k_rand_nodes): c2 X_q2 out (x2 null), or, for complex members, c2x X_q2x(z) x2~ + c2 X_q2(z) out with
+  // the energy-linear pairs (q, q + 1) and x2~ = x2 / sq when x2norm
   void* out2;
   int q2;
   double c2;
+  const void* x2;
+  int q2x, x2norm;
+  double c2x;
 };
 
 template <class T>
@@ -431,7 +435,8 @@ __device__ __forceinline__ void rand_ct(const OpDev& op, const MemDev& md, const
   }
 }
 
-template <class T, int MT, int NSLOT>
+// O2: the out2 form compiled in (0 none, 1 c2 X_q2 out, 2 the two energy-linear terms of route ed3's t pass)
+template <class T, int MT, int NSLOT, int O2>
 __global__ void __launch_bounds__(BLOCK, 4) k_tile(OpDev op, MemDev md, FArgs fa, const double* __restrict__ sblk) {
   extern __shared__ double bls[];
   __shared__ T red[BLOCK];
@@ -551,19 +556,47 @@ __global__ void __launch_bounds__(BLOCK, 4) k_tile(OpDev op, MemDev md, FArgs fa
     const double so = fa.out_sq ? op.sq[i] : 1.0;
     T* out = (T*)fa.out;
     const T* du = (const T*)fa.dotu;
-    // out2: the own-state term of rand_ct on the output row (amplitude coordinates, no partner rows)
-    T* out2 = (fa.out2 && !(inf & INFO_PARTNERS)) ? (T*)fa.out2 : nullptr;
-    const double* w2 = nullptr;
-    if (out2) {
-      const int fs = frozen_slot(inf, md, m);
-      const int l0 = inf & 15u, kap = (inf >> 4) & 255u, nsm = op.nsmax;
-      w2 = fs ? frozen_row(md, fs, fa.q2) + m : md.wts + ((((size_t)fa.q2 * op.ncls + kap) * nsm + l0) * nsm + l0) * 9 * nbp + m;
-    }
-    for (int l = 0; l < 9; ++l) {
-      const T y = so * acc[l];
-      out[((size_t)i * 9 + l) * nbp + m] = y;
-      if (du) dotv = dotv + bilin(ld(du, i, l, nbp, m), y);
-      if (out2) out2[((size_t)i * 9 + l) * nbp + m] = fma_s(fa.c2 * w2[(size_t)l * nbp], 1.0 * y, zero<T>());
+    if constexpr (O2 == 0) {
+      for (int l = 0; l < 9; ++l) {
+        const T y = so * acc[l];
+        out[((size_t)i * 9 + l) * nbp + m] = y;
+        if (du) dotv = dotv + bilin(ld(du, i, l, nbp, m), y);
+      }
+    } else {
+      // out2: the own-state terms of rand_ct on the output row (and on x2), in rand_ct's operation order, no partner rows
+      T* out2 = !(inf & INFO_PARTNERS) ? (T*)fa.out2 : nullptr;
+      const double *wa = nullptr, *wb = nullptr, *xa = nullptr, *xb = nullptr;
+      if (out2) {
+        const int fs = frozen_slot(inf, md, m);
+        const int l0 = inf & 15u, kap = (inf >> 4) & 255u, nsm = op.nsmax;
+        auto wrow = [&](int q) -> const double* {
+          return fs ? frozen_row(md, fs, q) + m : md.wts + ((((size_t)q * op.ncls + kap) * nsm + l0) * nsm + l0) * 9 * nbp + m;
+        };
+        wa = wrow(fa.q2);
+        if constexpr (O2 == 2) {
+          wb = wrow(fa.q2 + 1);
+          xa = wrow(fa.q2x);
+          xb = wrow(fa.q2x + 1);
+        }
+      }
+      for (int l = 0; l < 9; ++l) {
+        const T y = so * acc[l];
+        out[((size_t)i * 9 + l) * nbp + m] = y;
+        if (du) dotv = dotv + bilin(ld(du, i, l, nbp, m), y);
+        if (out2) {
+          if constexpr (O2 == 1) {
+            out2[((size_t)i * 9 + l) * nbp + m] = fma_s(fa.c2 * wa[(size_t)l * nbp], 1.0 * y, zero<T>());
+          } else if constexpr (std::is_same<T, cplx>::value) {
+            const cplx z = md.z[m];
+            const double sx = fa.x2norm ? op.isq[i] : 1.0;
+            const cplx xv = sx * ((const cplx*)fa.x2)[((size_t)i * 9 + l) * nbp + m];
+            const double xal = xa[(size_t)l * nbp], wal = wa[(size_t)l * nbp];
+            const cplx cx = cplx{fa.c2x * fma(z.re, xal, xb[(size_t)l * nbp]), fa.c2x * z.im * xal};
+            const cplx cy = cplx{fa.c2 * fma(z.re, wal, wb[(size_t)l * nbp]), fa.c2 * z.im * wal};
+            out2[((size_t)i * 9 + l) * nbp + m] = (zero<cplx>() + cx * xv) + cy * (1.0 * y);
+          }
+        }
+      }
     }
   }
   if (fa.dotu) {
@@ -576,8 +609,57 @@ __global__ void __launch_bounds__(BLOCK, 4) k_tile(OpDev op, MemDev md, FArgs fa
 // partners, whose rows k_tile's out2 leaves out.  One thread per (listed node, member), members fastest.  Same operation
 // order as rand_ct, so the rows equal those of a full pass.
 template <class T>
+__device__ inline void rand_node_term(const OpDev& op, const MemDev& md, int i, uint32_t inf, int m, const T* __restrict__ x,
+                                      int q, double coef, int xnorm, bool lin, T acc[9]) {
+  // rand_ct for one (node, member) with plain loads (acc += coef X_q x; lin: coef (z X_q + X_q+1) x, complex members)
+  const int nbp = md.nbp, nsm = op.nsmax;
+  const int fs = frozen_slot(inf, md, m);
+  const double isi = xnorm ? op.isq[i] : 1.0;
+  auto add = [&](double wa, double wb, T xv, int l) {
+    if constexpr (std::is_same<T, cplx>::value) {
+      if (lin) {
+        const cplx z = md.z[m];
+        const cplx cw = cplx{coef * fma(z.re, wa, wb), coef * z.im * wa};
+        acc[l] = acc[l] + cw * xv;
+        return;
+      }
+    }
+    acc[l] = fma_s(coef * wa, xv, acc[l]);
+  };
+  const int qb = lin ? q + 1 : q;
+  if (fs) {
+    const double* fa_ = frozen_row(md, fs, q) + m;
+    const double* fb_ = frozen_row(md, fs, qb) + m;
+    for (int l = 0; l < 9; ++l)
+      add(fa_[(size_t)l * nbp], lin ? fb_[(size_t)l * nbp] : 0.0, isi * x[((size_t)i * 9 + l) * nbp + m], l);
+    return;
+  }
+  const int l0 = inf & 15u, kap = (inf >> 4) & 255u;
+  const double* wa_ = md.wts + (((size_t)q * op.ncls + kap) * nsm + l0) * nsm * 9 * nbp + m;
+  const double* wb_ = md.wts + (((size_t)qb * op.ncls + kap) * nsm + l0) * nsm * 9 * nbp + m;
+  for (int l = 0; l < 9; ++l)
+    add(wa_[((size_t)l0 * 9 + l) * nbp], lin ? wb_[((size_t)l0 * 9 + l) * nbp] : 0.0, isi * x[((size_t)i * 9 + l) * nbp + m], l);
+  const uint64_t chim = md.chimask[m];
+  const int nsc = op.nsc[kap];
+  const uint32_t* fl = op.flip + (size_t)i * nsm;
+  for (int k = 0; k < nsc; ++k) {
+    const uint32_t e = fl[k];
+    if (k == l0 || e == 0u) continue;
+    const int r = (int)(e >> 6) - 1, g = (int)(e & 63u);
+    const double isr = xnorm ? op.isq[r] : 1.0;
+    const double chi = ((chim >> g) & 1ull) ? -1.0 : 1.0;
+    for (int l = 0; l < 9; ++l)
+      add(wa_[((size_t)k * 9 + l) * nbp], lin ? wb_[((size_t)k * 9 + l) * nbp] : 0.0,
+          (isr * (chi * op.sg[g * 9 + l])) * x[((size_t)r * 9 + l) * nbp + m], l);
+  }
+}
+
+// y = terms at the listed nodes: c2x X_q2x x2 (when x2) + c2 X_q2 x, the form of k_tile's out2 (O2 = 1: x2 null,
+// plain weights; O2 = 2: energy-linear pairs)
+template <class T>
 __global__ void k_rand_nodes(OpDev op, MemDev md, const int* __restrict__ list, int nlist, const T* __restrict__ x,
-                             T* __restrict__ y, int q, double coef) {
+                             T* __restrict__ y, int q, double coef, const T* __restrict__ x2, int q2x, int x2norm,
+                             double c2x) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int nbp = md.nbp, nsm = op.nsmax;
   const int j = (int)(t / nbp), m = (int)(t % nbp);
@@ -587,28 +669,8 @@ __global__ void k_rand_nodes(OpDev op, MemDev md, const int* __restrict__ list, 
   const uint32_t inf = op.info[i];
   T acc[9];
   for (int l = 0; l < 9; ++l) acc[l] = zero<T>();
-  const int fs = frozen_slot(inf, md, m);
-  if (fs) {
-    const double* fa_ = frozen_row(md, fs, q) + m;
-    for (int l = 0; l < 9; ++l) acc[l] = fma_s(coef * fa_[(size_t)l * nbp], 1.0 * x[((size_t)i * 9 + l) * nbp + m], acc[l]);
-  } else {
-    const int l0 = inf & 15u, kap = (inf >> 4) & 255u;
-    const double* wa_ = md.wts + (((size_t)q * op.ncls + kap) * nsm + l0) * nsm * 9 * nbp + m;
-    for (int l = 0; l < 9; ++l)
-      acc[l] = fma_s(coef * wa_[((size_t)l0 * 9 + l) * nbp], 1.0 * x[((size_t)i * 9 + l) * nbp + m], acc[l]);
-    const uint64_t chim = md.chimask[m];
-    const int nsc = op.nsc[kap];
-    const uint32_t* fl = op.flip + (size_t)i * nsm;
-    for (int k = 0; k < nsc; ++k) {
-      const uint32_t e = fl[k];
-      if (k == l0 || e == 0u) continue;
-      const int r = (int)(e >> 6) - 1, g = (int)(e & 63u);
-      const double chi = ((chim >> g) & 1ull) ? -1.0 : 1.0;
-      for (int l = 0; l < 9; ++l)
-        acc[l] = fma_s(coef * wa_[((size_t)k * 9 + l) * nbp], (1.0 * (chi * op.sg[g * 9 + l])) * x[((size_t)r * 9 + l) * nbp + m],
-                       acc[l]);
-    }
-  }
+  if (x2) rand_node_term<T>(op, md, i, inf, m, x2, q2x, c2x, x2norm, true, acc);
+  rand_node_term<T>(op, md, i, inf, m, x, q, coef, 0, x2 != nullptr, acc);
   for (int l = 0; l < 9; ++l) y[((size_t)i * 9 + l) * nbp + m] = acc[l];
 }
 
@@ -2291,6 +2353,7 @@ bool tile_kernel(const asr_gpu_op* op, const MemDev& md, size_t es) {
 template <class T>
 int launch_fused(const asr_gpu_op* op, const MemDev& md, const FArgs& fa, hipStream_t strm) {
   if (fa.out2 && !tile_kernel(op, md, sizeof(T))) return fail("operator pass: out2 needs the tile kernel");
+  if (fa.x2 && !std::is_same<T, cplx>::value) return fail("operator pass: energy-linear out2 terms need complex members");
   dim3 grid = node_grid(op, md);
   size_t lds = fused_lds_bytes(op, sizeof(T));
   if (timer_mark(strm, fa.xs ? KIND_SPMM : KIND_RAND, true)) return 1;
@@ -2317,13 +2380,17 @@ int launch_fused(const asr_gpu_op* op, const MemDev& md, const FArgs& fa, hipStr
     else ASR_NODE(false, 0);
 #undef ASR_NODE
   } else if (tile_kernel(op, md, sizeof(T))) {
-#define ASR_CT(MT, NS) hipLaunchKernelGGL((k_tile<T, MT, NS>), grid, dim3(BLOCK), blds_bytes + (size_t)MT * (2 * op->nop + 1) * 9 * 8, \
-                                           strm, od, md, fa, (const double*)op->blkp)
+#define ASR_CT3(MT, NS, O2) hipLaunchKernelGGL((k_tile<T, MT, NS, O2>), grid, dim3(BLOCK), \
+                                                blds_bytes + (size_t)MT * (2 * op->nop + 1) * 9 * 8, strm, od, md, fa, \
+                                                (const double*)op->blkp)
+#define ASR_CT(MT, NS) do { if (!fa.out2) ASR_CT3(MT, NS, 0); else if (!fa.x2) ASR_CT3(MT, NS, 1); \
+                            else ASR_CT3(MT, NS, 2); } while (0)
     if (md.mt == 8 && op->nslot == 18) ASR_CT(8, 18);
     else if (md.mt == 8) ASR_CT(8, 14);
     else if (op->nslot == 18) ASR_CT(16, 18);
     else ASR_CT(16, 14);
 #undef ASR_CT
+#undef ASR_CT3
   } else {
     hipLaunchKernelGGL(k_fused<T>, grid, dim3(BLOCK), lds, strm, dev_of(op), md, fa);
   }
@@ -2354,7 +2421,7 @@ int apply_chain_op(const asr_gpu_op* op, const Members& M, const MemDev& md, con
       if (timer_mark(strm, KIND_FIX, true)) return 1;
       const size_t nt = (size_t)op->nplist * md.nbp;
       hipLaunchKernelGGL(k_rand_nodes<T>, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, strm, dev_of(op), md,
-                         (const int*)op->plist, op->nplist, (const T*)h1, t1, (int)QO, 1.0);
+                         (const int*)op->plist, op->nplist, (const T*)h1, t1, (int)QO, 1.0, (const T*)nullptr, 0, 0, 0.0);
       HCHECK(hipGetLastError());
       if (timer_mark(strm, 0, false)) return 1;
     }
@@ -2426,12 +2493,29 @@ int apply_ed3_op(const asr_gpu_op* op, const Members& M, const MemDev& md, const
     f.xs = x; f.xs_norm = 1; f.s_din = 1; f.s_dout = 1; f.s_coef = 1.0;
     f.xr[0] = x; f.qr[0] = QC; f.r_norm[0] = 1; f.r_coef[0] = 1.0; f.nr = 1;
     f.out = h1;
+    // t1 = u x~ - Qt h1: node-local at the nodes without flip partners, so the first S pass writes it there from the
+    // registers that hold h1 (and the own row of x) and k_rand_nodes writes the listed nodes
+    const bool fuse = tile_kernel(op, md, sizeof(T));
+    if (fuse) {
+      f.out2 = t1; f.q2 = QQA; f.c2 = -1.0;
+      f.x2 = x; f.q2x = QUA; f.x2norm = 1; f.c2x = 1.0;
+    }
     if (launch_fused<T>(op, md, f, strm)) return 1;
-    f = fargs0();
-    f.xl[0] = x; f.lq[0] = QUA; f.l_norm[0] = 1; f.l_coef[0] = 1.0;
-    f.xl[1] = h1; f.lq[1] = QQA; f.l_norm[1] = 0; f.l_coef[1] = -1.0; f.nl = 2;
-    f.out = t1;
-    if (launch_fused<T>(op, md, f, strm)) return 1;
+    if (fuse && op->nplist > 0) {
+      if (timer_mark(strm, KIND_FIX, true)) return 1;
+      const size_t nt = (size_t)op->nplist * md.nbp;
+      hipLaunchKernelGGL(k_rand_nodes<T>, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, strm, dev_of(op), md,
+                         (const int*)op->plist, op->nplist, (const T*)h1, t1, (int)QQA, -1.0, x, (int)QUA, 1, 1.0);
+      HCHECK(hipGetLastError());
+      if (timer_mark(strm, 0, false)) return 1;
+    }
+    if (!fuse) {
+      f = fargs0();
+      f.xl[0] = x; f.lq[0] = QUA; f.l_norm[0] = 1; f.l_coef[0] = 1.0;
+      f.xl[1] = h1; f.lq[1] = QQA; f.l_norm[1] = 0; f.l_coef[1] = -1.0; f.nl = 2;
+      f.out = t1;
+      if (launch_fused<T>(op, md, f, strm)) return 1;
+    }
     f = fargs0();
     f.xs = t1; f.xs_norm = 0; f.s_din = 1; f.s_dout = 1; f.s_coef = -1.0;
     f.xr[0] = t1; f.qr[0] = QC; f.r_norm[0] = 0; f.r_coef[0] = -1.0; f.nr = 1;
