@@ -7,8 +7,9 @@ vector come from two complex-symmetric chains (asr_gpu_epi_blocks), and
 U(c, s) = int_0^1 dlambda [dOmega/dlambda (c0 -> c | s) - dOmega/dlambda (c0 -> c | s0)].
 
 This module holds the species parameters (epi_species without the representation change, epi_path, epi_setup
-src/asr_epi.f90:124-178), the trace formula (epi_integrand :182-189), the symmetry-adapted start vectors of an abelian pair
-group (epi_adapted :290-328) and the pair-table algebra (epi_exchange :257-266).  The chains run on the GPU
+src/asr_epi.f90:124-178), the trace formula (epi_integrand :182-189), dOmega/dlambda over a contour (epi_domega), the pair
+table entry by the path quadrature (epi_pair_u), the symmetry-adapted start vectors of an abelian pair group (epi_adapted
+:290-328) and the pair-table algebra (epi_exchange :257-266).  The chains run on the GPU
 (asr_amd.engine.GpuOperator.epi_blocks).
 """
 from __future__ import annotations
@@ -95,6 +96,43 @@ def epi_integrand(pt: EpiPath, l: int, z: complex, x) -> complex:
     constant -2 kappa (epi_integrand, src/asr_epi.f90:182-189)."""
     return ((pt.de[l] + pt.dc[l]) * x[0] - 2.0 * pt.dc[l] * pt.o[l] * x[1] - pt.dox[l] * x[2]
             + 2.0 * pt.dk[l] * ((z - pt.e[l] - pt.c[l]) * x[0] + 2.0 * pt.c[l] * pt.o[l] * x[1] - pt.o[l] * x[2]))
+
+
+def epi_domega(G, par: list[EpiPar], pt: EpiPath, partner_species: int, zs, ws, vec: np.ndarray, chi: np.ndarray,
+               lo: np.ndarray, nlev: int, mode: int = 0, max_blocks: int = 56) -> float:
+    """dOmega/dlambda at the path point pt with the pair partner frozen to partner_species, without the constant -2 kappa
+    term of epi_integrand (the same for every partner species, so it cancels in the pair table):
+        -(1/pi) Im sum_j ws_j sum_a epi_integrand(pt, lo_a, zs_j, X_a(zs_j))
+    over the contour (zs, ws) and the nine adapted start vectors (vec[:, a], characters chi[:, a], shell lo[a] of
+    epi_adapted), the blocks X_a from the GPU (GpuOperator.epi_blocks of the pair operator G; calls of at most max_blocks
+    blocks) (the dOmega/dlambda of the EPI driver, src/asr_epi.f90:241 with epi_integrand :182-189)."""
+    zs = np.asarray(zs, np.complex128).ravel()
+    ws = np.asarray(ws, np.complex128).ravel()
+    tasks = [(a, j) for j in range(zs.size) for a in range(9)]
+    tot = 0.0 + 0.0j
+    for q0 in range(0, len(tasks), max_blocks):
+        chunk = tasks[q0:q0 + max_blocks]
+        members = [epi_member(par, pt, int(lo[a]), chi[:, a], partner_species, zs[j]) for a, j in chunk]
+        v9 = np.stack([vec[:, a] for a, _ in chunk])
+        x, *_ = G.epi_blocks(members, v9, nlev, mode=mode)
+        for q, (a, j) in enumerate(chunk):
+            tot += ws[j] * epi_integrand(pt, int(lo[a]), zs[j], x[q])
+    return float(-tot.imag / np.pi)
+
+
+def epi_pair_u(G, par: list[EpiPar], c0: int, c: int, s: int, s0: int, zs, ws, vec, chi, lo, nlev: int, nlam: int = 2,
+               mode: int = 0, max_blocks: int = 56) -> float:
+    """U(c, s) = int_0^1 dlambda [dOmega/dlambda (c0 -> c | s) - dOmega/dlambda (c0 -> c | s0)] by nlam Gauss-Legendre
+    points on the straight path of site 0 from species c0 to c (the pair table entry against the first species c0, s0,
+    src/asr_epi.f90:4-24)."""
+    from .host import gauss_legendre
+    x, w = gauss_legendre(nlam)
+    u = 0.0
+    for xk, wk in zip(x, w):
+        pt = epi_path(par[c0], par[c], 0.5 * (1.0 + xk))
+        u += 0.5 * wk * (epi_domega(G, par, pt, s, zs, ws, vec, chi, lo, nlev, mode, max_blocks)
+                         - epi_domega(G, par, pt, s0, zs, ws, vec, chi, lo, nlev, mode, max_blocks))
+    return u
 
 
 def epi_exchange(u: np.ndarray) -> np.ndarray:

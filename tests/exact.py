@@ -170,3 +170,40 @@ def exact_average_pair_blocks(model, sites_par, z: complex, v9, partner: int, pa
         sol = np.linalg.solve(z * np.eye(h.shape[0]) - hm, np.stack([v, hv], axis=1))
         x += p * np.array([v @ sol[:, 0], hv @ sol[:, 0], hv @ sol[:, 1]])
     return x
+
+
+def pair_hamiltonians(model, sites_par, partner: int, partner_species: int, start_species: int):
+    """(H, probability) per configuration with site 0 on start_species and the partner frozen: H = E_nu + h - h o h with the
+    per-species parameter rows sites_par[k] (k = c, d, o, e)."""
+    for sp, p in pair_configurations(model, start_species, partner, partner_species):
+        yield dense_h(model, sites_par, sp), p
+
+
+def fermi(e, mu: float, kt: float):
+    t = (np.asarray(e) - mu) / kt
+    return np.where(t > 0, np.exp(-np.abs(t)) / (1.0 + np.exp(-np.abs(t))), 1.0 / (1.0 + np.exp(-np.abs(t))))
+
+
+def exact_pair_domega(model, sites_of, lam: float, partner: int, partner_species: int, mu: float, kt: float,
+                      dl: float = 1.0e-5) -> float:
+    """dOmega/dlambda = sum_configs p sum_i f(e_i) <psi_i|dH/dlambda|psi_i> (Hellmann-Feynman, brute force), site 0 on
+    the path point at lambda (sites_of(lam): parameter rows with the path point as the last species), dH/dlambda by a
+    central difference."""
+    tot = 0.0
+    hp = list(pair_hamiltonians(model, sites_of(lam + dl), partner, partner_species, len(sites_of(lam)["c"]) - 1))
+    hm = list(pair_hamiltonians(model, sites_of(lam - dl), partner, partner_species, len(sites_of(lam)["c"]) - 1))
+    for (h0, p), (h1, _), (h2, _) in zip(pair_hamiltonians(model, sites_of(lam), partner, partner_species,
+                                                           len(sites_of(lam)["c"]) - 1), hp, hm):
+        e, v = np.linalg.eigh(h0)
+        dh = (h1 - h2) / (2.0 * dl)
+        tot += p * float(np.sum(fermi(e, mu, kt) * np.einsum("ji,jk,ki->i", v, dh, v)))
+    return tot
+
+
+def exact_pair_omega(model, sites_par, partner: int, partner_species: int, start_species: int, mu: float, kt: float):
+    """Omega = sum_configs p sum_i -kT log(1 + exp(-(e_i - mu) / kT)) (brute force, fixed E_F)."""
+    tot = 0.0
+    for h, p in pair_hamiltonians(model, sites_par, partner, partner_species, start_species):
+        t = -(np.linalg.eigvalsh(h) - mu) / kt
+        tot += p * float(np.sum(-kt * np.logaddexp(0.0, t)))
+    return tot

@@ -101,3 +101,51 @@ def test_epi_integrand_closes_on_the_blocks(epi_small):
             tg += EP.epi_integrand(pt, m.l0, m.z, x[j])
             to += EP.epi_integrand(pt, m.l0, m.z, xo)
         assert abs(tg - to) < 1e-8 * abs(to)
+
+
+def test_epi_thermodynamic_integration_on_ring():
+    """The EPI driver on the ring (src/asr_epi.f90:4-24): the difference of dOmega/dlambda between two partner species
+    (epi_domega over a Fermi contour, exhausted chains) equals the brute-force Hellmann-Feynman difference
+    sum_configs p sum_i f(e_i) <psi_i|dH/dlambda|psi_i> (the constant -2 kappa term cancels); the pair table entry by four
+    Gauss-Legendre path points (epi_pair_u) equals the same quadrature of the brute-force difference; and that quadrature
+    converges to the second difference of the brute-force grand potentials Omega(c, s) - Omega(c0, s) - Omega(c, s0) +
+    Omega(c0, s0) (eight points: the thermodynamic-integration identity the driver rests on)."""
+    eng = _engine()
+    from asr_amd import host
+    from exact import exact_pair_domega, exact_pair_omega, pair_hamiltonians
+    from test_oracle_exact import _ring_pair_setup
+    model, op, par, _, _, partner = _ring_pair_setup()
+
+    def sites_of(lam):
+        pt = EP.epi_path(par[1], par[2], lam)
+        return {k: np.stack([getattr(p, k) for p in par] + [getattr(pt, k)]) for k in ("c", "d", "o", "e")}
+
+    def exact_diff(lam):
+        return (exact_pair_domega(model, sites_of, lam, partner, 2, mu, kt)
+                - exact_pair_domega(model, sites_of, lam, partner, 1, mu, kt))
+
+    eig = np.concatenate([np.linalg.eigvalsh(h) for lam in (0.0, 1.0) for s in (1, 2)
+                          for h, _ in pair_hamiltonians(model, sites_of(lam), partner, s, 4)])
+    mu, kt = float(np.quantile(eig, 0.45)), 0.01
+    zs, ws = host.contour_fermi(float(eig.min()) - 0.3, mu, kt, host.fermi_npole(kt), 96, 16)
+    vec, chi, lo = EP.epi_adapted(np.eye(9)[None])
+    G = eng.GpuOperator(op)
+    G.set_depth_tol(0.0, "none")
+    nlev, nb = 400, 9 * zs.size  # the ring's chains close well before 400 levels; one call per species and path point
+    print(f"epi TI ring: {zs.size} contour energies x 9 start vectors, {nlev} levels", flush=True)
+    lam = 0.37
+    pt = EP.epi_path(par[1], par[2], lam)
+    d_gpu = (EP.epi_domega(G, par, pt, 2, zs, ws, vec, chi, lo, nlev, max_blocks=nb)
+             - EP.epi_domega(G, par, pt, 1, zs, ws, vec, chi, lo, nlev, max_blocks=nb))
+    d_ex = exact_diff(lam)
+    assert abs(d_gpu - d_ex) < 1e-6 * max(1.0, abs(d_ex)), (d_gpu, d_ex)
+    u_gpu = EP.epi_pair_u(G, par, 1, 2, 2, 1, zs, ws, vec, chi, lo, nlev, nlam=4, max_blocks=nb)
+    x, w = host.gauss_legendre(4)
+    u_quad = sum(0.5 * wk * exact_diff(0.5 * (1.0 + xk)) for xk, wk in zip(x, w))
+    assert abs(u_gpu - u_quad) < 1e-6, (u_gpu, u_quad)
+    om = {(lm, s): exact_pair_omega(model, sites_of(lm), partner, s, 4, mu, kt) for lm in (0.0, 1.0) for s in (1, 2)}
+    u_ex = om[1.0, 2] - om[0.0, 2] - om[1.0, 1] + om[0.0, 1]
+    x8, w8 = host.gauss_legendre(8)
+    u8 = sum(0.5 * wk * exact_diff(0.5 * (1.0 + xk)) for xk, wk in zip(x8, w8))
+    assert abs(u8 - u_ex) < 2e-5, (u8, u_ex)
+    assert abs(u_quad - u_ex) > 10 * abs(u8 - u_ex)  # the path quadrature converges (4 points are coarse here)
