@@ -189,26 +189,41 @@ def main():
         by, fl = chain_kernel_cost(wl.op, nb, int(kd), *io.get(q, (1, 1)))
         per_kernel.append(dict(launch=q, kind=int(kd), ms=round(float(t_ms), 4), gbs=round(by / (t_ms * 1e-3) / 1e9, 1),
                                tfs=round(fl / (t_ms * 1e-3) / 1e12, 3)))
-    # the dominant kernel is the operator pass (every launch of kinds 0 and 1 is the same kernel: the S pass with the flip
-    # terms, or the flip terms alone); its roofline uses the summed algorithmic cost over the level's launches of it
-    # divided by their summed duration, so that the mean duration per launch is comparable with a kernel-trace average
-    ops = [k for k in per_kernel if k["kind"] in (0, 1)]
-    tb = tf = tm = 0.0
-    for k in ops:
-        by, fl = chain_kernel_cost(wl.op, nb, k["kind"], *io.get(k["launch"], (1, 1)))
-        tb, tf, tm = tb + by, tf + fl, tm + k["ms"]
-    roof = roofline(tb / len(ops), tf / len(ops), tm / len(ops))
+    # the dominant kernel is the operator pass.  With the tile kernel its two launches per level are two instantiations
+    # (k_tile<..., 1>: the first S pass, which also writes the node-local o term; k_tile<..., 0>: the second S pass with
+    # the output terms and the a_n dot), so each gets its own roofline, the longer one as "roofline"; otherwise every
+    # launch of kinds 0 and 1 is the same kernel and the roofline is over their summed cost and duration
     piid = bool(np.all(np.asarray(wl.op.pi) == np.arange(9)[None, :]))
-    if out_tile in (8, 16) and piid and wl.op.nslot in (14, 18):
-        kname = "k_tile<cplx, %d, %d>" % (out_tile, wl.op.nslot)
+    tile_kernel = out_tile in (8, 16) and piid and wl.op.nslot in (14, 18)
+    fused = 6 in [int(k) for k in kinds]
+
+    def launch_roofline(launches, kname):
+        tb = tf = tm = 0.0
+        for k in launches:
+            by, fl = chain_kernel_cost(wl.op, nb, k["kind"], *io.get(k["launch"], (1, 1)))
+            tb, tf, tm = tb + by, tf + fl, tm + k["ms"]
+        r = roofline(tb / len(launches), tf / len(launches), tm / len(launches))
+        r["kernel"] = kname
+        r["launches_per_level"] = len(launches)
+        r["avg_launch_ms"] = round(tm / len(launches), 4)
+        r["algorithmic_bytes_per_launch"] = round(tb / len(launches))
+        r["traffic"] = measured_traffic("chain_edc", kname, wl.op.nn, nb)
+        return r
+
+    roof_first = None
+    if tile_kernel and fused:
+        first = [k for k in per_kernel if k["launch"] == 0]
+        second = [k for k in per_kernel if k["launch"] == 2]
+        roof = launch_roofline(second, "k_tile<cplx, %d, %d, 0>" % (out_tile, wl.op.nslot))
+        roof_first = launch_roofline(first, "k_tile<cplx, %d, %d, 1>" % (out_tile, wl.op.nslot))
     else:
-        kname = {65: "k_pair<true, %d>" % wl.op.nslot, 64: "k_node<cplx, true, %d>" % wl.op.nslot}.get(out_tile,
-                                                                                                     "k_fused<cplx>")
-    roof["kernel"] = kname
-    roof["launches_per_level"] = len(ops)
-    roof["avg_launch_ms"] = round(tm / len(ops), 4)
-    roof["algorithmic_bytes_per_launch"] = round(tb / len(ops))
-    roof["traffic"] = measured_traffic("chain_edc", kname, wl.op.nn, nb)
+        ops = [k for k in per_kernel if k["kind"] in (0, 1)]
+        if tile_kernel:
+            kname = "k_tile<cplx, %d, %d, 0>" % (out_tile, wl.op.nslot)
+        else:
+            kname = {65: "k_pair<true, %d>" % wl.op.nslot, 64: "k_node<cplx, true, %d>" % wl.op.nslot}.get(out_tile,
+                                                                                                         "k_fused<cplx>")
+        roof = launch_roofline(ops, kname)
     log("level kernels:", per_kernel)
 
     out = {
@@ -229,6 +244,7 @@ def main():
                    "device_ms_per_step": round(float(np.mean(dev_ms)), 2), "member_tile": out_tile,
                    "node_order": wl.op.meta.get("node_order", "bfs")},
         "roofline": roof,
+        "roofline_first_pass": roof_first,
         "level_kernels": per_kernel,
     }
 
