@@ -435,6 +435,17 @@ __device__ __forceinline__ void rand_ct(const OpDev& op, const MemDev& md, const
   }
 }
 
+// streaming (non-temporal) stores and loads of values a pass writes or reads once: those lines need not take L2 room from
+// the gathered rows (measured on the headline level: 143.5 -> 134.9 ms per 30 levels, the coefficients unchanged)
+typedef double nt_d2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void nt_store(cplx* p, cplx v) { __builtin_nontemporal_store(nt_d2{v.re, v.im}, (nt_d2*)p); }
+__device__ __forceinline__ void nt_store(double* p, double v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ cplx nt_load(const cplx* p) {
+  const nt_d2 v = __builtin_nontemporal_load((const nt_d2*)p);
+  return cplx{v.x, v.y};
+}
+__device__ __forceinline__ double nt_load(const double* p) { return __builtin_nontemporal_load(p); }
+
 // O2: the out2 form compiled in (0 none, 1 c2 X_q2 out, 2 the two energy-linear terms of route ed3's t pass)
 template <class T, int MT, int NSLOT, int O2>
 __global__ void __launch_bounds__(BLOCK, 4) k_tile(OpDev op, MemDev md, FArgs fa, const double* __restrict__ sblk) {
@@ -551,7 +562,7 @@ __global__ void __launch_bounds__(BLOCK, 4) k_tile(OpDev op, MemDev md, FArgs fa
     }
     for (int t = 0; t < fa.no; ++t) {
       const T* xo = (const T*)fa.xo[t];
-      for (int l = 0; l < 9; ++l) acc[l] = fma_s(fa.o_coef[t], ld(xo, i, l, nbp, m), acc[l]);
+      for (int l = 0; l < 9; ++l) acc[l] = fma_s(fa.o_coef[t], nt_load(&xo[((size_t)i * 9 + l) * nbp + m]), acc[l]);
     }
     const double so = fa.out_sq ? op.sq[i] : 1.0;
     T* out = (T*)fa.out;
@@ -559,7 +570,7 @@ __global__ void __launch_bounds__(BLOCK, 4) k_tile(OpDev op, MemDev md, FArgs fa
     if constexpr (O2 == 0) {
       for (int l = 0; l < 9; ++l) {
         const T y = so * acc[l];
-        out[((size_t)i * 9 + l) * nbp + m] = y;
+        nt_store(&out[((size_t)i * 9 + l) * nbp + m], y);
         if (du) dotv = dotv + bilin(ld(du, i, l, nbp, m), y);
       }
     } else {
@@ -581,11 +592,11 @@ __global__ void __launch_bounds__(BLOCK, 4) k_tile(OpDev op, MemDev md, FArgs fa
       }
       for (int l = 0; l < 9; ++l) {
         const T y = so * acc[l];
-        out[((size_t)i * 9 + l) * nbp + m] = y;
+        nt_store(&out[((size_t)i * 9 + l) * nbp + m], y);
         if (du) dotv = dotv + bilin(ld(du, i, l, nbp, m), y);
         if (out2) {
           if constexpr (O2 == 1) {
-            out2[((size_t)i * 9 + l) * nbp + m] = fma_s(fa.c2 * wa[(size_t)l * nbp], 1.0 * y, zero<T>());
+            nt_store(&out2[((size_t)i * 9 + l) * nbp + m], fma_s(fa.c2 * wa[(size_t)l * nbp], 1.0 * y, zero<T>()));
           } else if constexpr (std::is_same<T, cplx>::value) {
             const cplx z = md.z[m];
             const double sx = fa.x2norm ? op.isq[i] : 1.0;
@@ -593,7 +604,7 @@ __global__ void __launch_bounds__(BLOCK, 4) k_tile(OpDev op, MemDev md, FArgs fa
             const double xal = xa[(size_t)l * nbp], wal = wa[(size_t)l * nbp];
             const cplx cx = cplx{fa.c2x * fma(z.re, xal, xb[(size_t)l * nbp]), fa.c2x * z.im * xal};
             const cplx cy = cplx{fa.c2 * fma(z.re, wal, wb[(size_t)l * nbp]), fa.c2 * z.im * wal};
-            out2[((size_t)i * 9 + l) * nbp + m] = (zero<cplx>() + cx * xv) + cy * (1.0 * y);
+            nt_store(&out2[((size_t)i * 9 + l) * nbp + m], (zero<cplx>() + cx * xv) + cy * (1.0 * y));
           }
         }
       }
@@ -1407,7 +1418,7 @@ __global__ void __launch_bounds__(BLOCK) k_update(OpDev op, MemDev md, T* w, con
     T x[9];
     for (int l = 0; l < 9; ++l) {
       size_t q = ((size_t)i * 9 + l) * md.nbp + m;
-      x[l] = off ? zero<T>() : w[q] - bilin(a, u[q]) - bilin(b, up[q]);
+      x[l] = off ? zero<T>() : nt_load(&w[q]) - bilin(a, nt_load(&u[q])) - bilin(b, nt_load(&up[q]));
     }
     if (project) {
       int s0 = op.stab_ptr[i], s1 = op.stab_ptr[i + 1], ns = s1 - s0;
@@ -1442,7 +1453,7 @@ __global__ void __launch_bounds__(BLOCK) k_update(OpDev op, MemDev md, T* w, con
     }
     for (int l = 0; l < 9; ++l) {
       const size_t q = ((size_t)i * 9 + l) * md.nbp + m;
-      w[q] = x[l];
+      nt_store(&w[q], x[l]);  // the next level gathers it: no use in L2 now
       dv = dv + bilin(x[l], x[l]);
       if constexpr (std::is_same<T, cplx>::value) {
         if (tv) dv2 = dv2 + bilin(tv[q], x[l]);
