@@ -1834,11 +1834,15 @@ __global__ void __launch_bounds__(BLOCK) k_kpm_v2(RealDev R, const double* __res
     }
     for (int l = 0; l < 9; ++l) {
       const uint64_t row = (uint64_t)site * 9 + l;
-      const d2 t0v = ((const d2*)(u0 + row * nvp + v))[0];
+      const nt_d2 t0n = __builtin_nontemporal_load((const nt_d2*)(u0 + row * nvp + v));
+      const d2 t0v = {t0n.x, t0n.y};
       double t2x = alpha * (dp[l] * acc[l].x) - beta * (t0v.x / dp[l]);
       double t2y = alpha * (dp[l] * acc[l].y) - beta * (t0v.y / dp[l]);
-      if (v < nvec) u2[row * nvp + v] = dp[l] * t2x;
-      if (v + 1 < nvec) u2[row * nvp + v + 1] = dp[l] * t2y;
+      if (v + 1 < nvec) {
+        __builtin_nontemporal_store(nt_d2{dp[l] * t2x, dp[l] * t2y}, (nt_d2*)(u2 + row * nvp + v));
+      } else if (v < nvec) {
+        u2[row * nvp + v] = dp[l] * t2x;
+      }
       if constexpr (DBL) {
         // the own row again (an L1/L2 hit) rather than held through the neighbour loop
         const d2 xo = ((const d2*)(u1 + row * nvp + v))[0];
