@@ -486,7 +486,6 @@ __global__ void __launch_bounds__(BLOCK, 4) k_tile(OpDev op, MemDev md, FArgs fa
   T dotv = zero<T>();
   if (live) {
     const uint32_t inf = op.info[i];
-    const uint64_t chim = md.chimask[m];
     T acc[9];
     for (int l = 0; l < 9; ++l) acc[l] = zero<T>();
     if (fa.xs) {
@@ -672,7 +671,7 @@ __global__ void k_rand_nodes(OpDev op, MemDev md, const int* __restrict__ list, 
                              T* __restrict__ y, int q, double coef, const T* __restrict__ x2, int q2x, int x2norm,
                              double c2x) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int nbp = md.nbp, nsm = op.nsmax;
+  const int nbp = md.nbp;
   const int j = (int)(t / nbp), m = (int)(t % nbp);
   if (j >= nlist || m >= md.nb) return;
   if (md.tact && !md.tact[m / md.mt]) return;
