@@ -440,6 +440,12 @@ __device__ __forceinline__ void rand_ct(const OpDev& op, const MemDev& md, const
 typedef double nt_d2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ void nt_store(cplx* p, cplx v) { __builtin_nontemporal_store(nt_d2{v.re, v.im}, (nt_d2*)p); }
 __device__ __forceinline__ void nt_store(double* p, double v) { __builtin_nontemporal_store(v, p); }
+// an output value: streamed when a wave's row segment fills whole 128-byte lines, else a plain store
+template <bool NT, class T>
+__device__ __forceinline__ void out_store(T* p, T v) {
+  if constexpr (NT) nt_store(p, v);
+  else *p = v;
+}
 __device__ __forceinline__ cplx nt_load(const cplx* p) {
   const nt_d2 v = __builtin_nontemporal_load((const nt_d2*)p);
   return cplx{v.x, v.y};
@@ -569,7 +575,7 @@ __global__ void __launch_bounds__(BLOCK, 4) k_tile(OpDev op, MemDev md, FArgs fa
     if constexpr (O2 == 0) {
       for (int l = 0; l < 9; ++l) {
         const T y = so * acc[l];
-        nt_store(&out[((size_t)i * 9 + l) * nbp + m], y);
+        out_store<(MT * sizeof(T) >= 128)>(&out[((size_t)i * 9 + l) * nbp + m], y);
         if (du) dotv = dotv + bilin(ld(du, i, l, nbp, m), y);
       }
     } else {
@@ -591,11 +597,11 @@ __global__ void __launch_bounds__(BLOCK, 4) k_tile(OpDev op, MemDev md, FArgs fa
       }
       for (int l = 0; l < 9; ++l) {
         const T y = so * acc[l];
-        nt_store(&out[((size_t)i * 9 + l) * nbp + m], y);
+        out_store<(MT * sizeof(T) >= 128)>(&out[((size_t)i * 9 + l) * nbp + m], y);
         if (du) dotv = dotv + bilin(ld(du, i, l, nbp, m), y);
         if (out2) {
           if constexpr (O2 == 1) {
-            nt_store(&out2[((size_t)i * 9 + l) * nbp + m], fma_s(fa.c2 * wa[(size_t)l * nbp], 1.0 * y, zero<T>()));
+            out_store<(MT * sizeof(T) >= 128)>(&out2[((size_t)i * 9 + l) * nbp + m], fma_s(fa.c2 * wa[(size_t)l * nbp], 1.0 * y, zero<T>()));
           } else if constexpr (std::is_same<T, cplx>::value) {
             const cplx z = md.z[m];
             const double sx = fa.x2norm ? op.isq[i] : 1.0;
@@ -603,7 +609,7 @@ __global__ void __launch_bounds__(BLOCK, 4) k_tile(OpDev op, MemDev md, FArgs fa
             const double xal = xa[(size_t)l * nbp], wal = wa[(size_t)l * nbp];
             const cplx cx = cplx{fa.c2x * fma(z.re, xal, xb[(size_t)l * nbp]), fa.c2x * z.im * xal};
             const cplx cy = cplx{fa.c2 * fma(z.re, wal, wb[(size_t)l * nbp]), fa.c2 * z.im * wal};
-            nt_store(&out2[((size_t)i * 9 + l) * nbp + m], (zero<cplx>() + cx * xv) + cy * (1.0 * y));
+            out_store<(MT * sizeof(T) >= 128)>(&out2[((size_t)i * 9 + l) * nbp + m], (zero<cplx>() + cx * xv) + cy * (1.0 * y));
           }
         }
       }
@@ -1392,7 +1398,9 @@ __global__ void k_level_end(int* level, const int* active, int nb, int mt, int* 
 }
 
 // w = w - a u - b_prev up; optional projection onto the symmetric subspace; partial sums w w and (tv non-null) tv w
-template <class T>
+// NT: streaming loads and stores (the member tile fills whole 128-byte lines per row segment; at smaller tiles the partial
+// lines are better left to the caches)
+template <class T, bool NT>
 __global__ void __launch_bounds__(BLOCK) k_update(OpDev op, MemDev md, T* w, const T* u, const T* up, const T* a_eff,
                                                    const T* b_prev, int project, T* partial, const T* tv, T* partial2,
                                                    const T* sc, const T* scp, const int* active) {
@@ -1417,7 +1425,8 @@ __global__ void __launch_bounds__(BLOCK) k_update(OpDev op, MemDev md, T* w, con
     T x[9];
     for (int l = 0; l < 9; ++l) {
       size_t q = ((size_t)i * 9 + l) * md.nbp + m;
-      x[l] = off ? zero<T>() : nt_load(&w[q]) - bilin(a, nt_load(&u[q])) - bilin(b, nt_load(&up[q]));
+      if constexpr (NT) x[l] = off ? zero<T>() : nt_load(&w[q]) - bilin(a, nt_load(&u[q])) - bilin(b, nt_load(&up[q]));
+      else x[l] = off ? zero<T>() : w[q] - bilin(a, u[q]) - bilin(b, up[q]);
     }
     if (project) {
       int s0 = op.stab_ptr[i], s1 = op.stab_ptr[i + 1], ns = s1 - s0;
@@ -1452,7 +1461,8 @@ __global__ void __launch_bounds__(BLOCK) k_update(OpDev op, MemDev md, T* w, con
     }
     for (int l = 0; l < 9; ++l) {
       const size_t q = ((size_t)i * 9 + l) * md.nbp + m;
-      nt_store(&w[q], x[l]);  // the next level gathers it: no use in L2 now
+      if constexpr (NT) nt_store(&w[q], x[l]);  // the next level gathers it: no use in L2 now
+      else w[q] = x[l];
       dv = dv + bilin(x[l], x[l]);
       if constexpr (std::is_same<T, cplx>::value) {
         if (tv) dv2 = dv2 + bilin(tv[q], x[l]);
@@ -2843,8 +2853,9 @@ struct ChainRun {
       f.xr[0] = B.u; f.qr[0] = QC; f.r_norm[0] = 1; f.r_coef[0] = 1.0; f.nr = 1;
       f.out = B.w; f.out_sq = 1;
       if (launch_fused<T>(op, md, f, stream)) return 1;
-      hipLaunchKernelGGL(k_update<T>, grid, dim3(BLOCK), 0, stream, dev_of(op), md, B.w, B.u, B.up, B.a_eff, B.b_eff, project,
-                         B.part, (const T*)nullptr, (T*)nullptr, (const T*)nullptr, (const T*)nullptr, (const int*)nullptr);
+      hipLaunchKernelGGL((k_update<T, false>), grid, dim3(BLOCK), 0, stream, dev_of(op), md, B.w, B.u, B.up, B.a_eff, B.b_eff,
+                         project, B.part, (const T*)nullptr, (T*)nullptr, (const T*)nullptr, (const T*)nullptr,
+                         (const int*)nullptr);
       hipLaunchKernelGGL(k_sum_member<T>, dim3(nb), dim3(BLOCK), 0, stream, B.part, nbx, md.mt, B.inv_b);
       HCHECK(hipGetLastError());
       std::vector<T> nv2(nbp), row0((size_t)9 * nbp);
@@ -3004,8 +3015,14 @@ struct ChainRun {
 
   int step_update() {
     if (timer_mark(stream, KIND_UPDATE, true)) return 1;
-    hipLaunchKernelGGL(k_update<T>, grid, dim3(BLOCK), 0, stream, dev_of(op), md, B.w, B.u, B.up, B.a_eff, B.b_eff, project,
-                       B.part, (const T*)B.tv, B.part2, s_cur(), s_prev(), lag ? (const int*)B.active : (const int*)nullptr);
+    if (md.mt * (int)sizeof(T) >= 128)
+      hipLaunchKernelGGL((k_update<T, true>), grid, dim3(BLOCK), 0, stream, dev_of(op), md, B.w, B.u, B.up, B.a_eff, B.b_eff,
+                         project, B.part, (const T*)B.tv, B.part2, s_cur(), s_prev(),
+                         lag ? (const int*)B.active : (const int*)nullptr);
+    else
+      hipLaunchKernelGGL((k_update<T, false>), grid, dim3(BLOCK), 0, stream, dev_of(op), md, B.w, B.u, B.up, B.a_eff, B.b_eff,
+                         project, B.part, (const T*)B.tv, B.part2, s_cur(), s_prev(),
+                         lag ? (const int*)B.active : (const int*)nullptr);
     if (reorth) {
       // w = w - Q (Q^T w), twice (classical Gram-Schmidt against q_0..q_k); the second pass leaves the partials of w w
       const int nq = level_host + 1;
