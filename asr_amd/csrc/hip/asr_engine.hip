@@ -501,7 +501,7 @@ __global__ void __launch_bounds__(BLOCK, 4) k_tile(OpDev op, MemDev md, FArgs fa
       const double* __restrict__ bl = bls + (size_t)((inf >> 12) & 255u) * (NSLOT + 1) * 90;
       const double* __restrict__ em = etab + (size_t)ml * (op.nop + 1) * 9;
       const uint32_t* __restrict__ hp = op.hop + (size_t)i * NSLOT;
-#pragma unroll 1
+#pragma unroll 2
       for (int k = 0; k <= NSLOT; ++k) {
         // k = 0: the node itself with the on-site block; k > 0: link k - 1
         const uint32_t e = hp[k > 0 ? k - 1 : 0];
@@ -531,7 +531,6 @@ __global__ void __launch_bounds__(BLOCK, 4) k_tile(OpDev op, MemDev md, FArgs fa
 #pragma unroll
             for (int l2 = 0; l2 < 9; ++l2) a = fma_s(bc[l2], v[l2], a);
             acc[l] = a;
-            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int l2 = 0; l2 < 9; ++l2) bc[l2] = bn[l2];
           }
