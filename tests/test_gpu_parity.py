@@ -163,6 +163,40 @@ def test_kpm_apply_and_moments(tile):
     R.set_doubling(False)
 
 
+@pytest.mark.parametrize("tile", [8, 16, 64])
+def test_edc_two_site_types(tile):
+    """Two site types (type 1: the bond blocks x 1.1 and the on-site block shifted by 0.01): the operator pass takes each
+    node's blocks by its type (the member-tile kernel's LDS rows, not its one-type scalar path) and gives the oracle's
+    chains; the type-1 blocks change the chains by about 0.1, so a pass that ignores the type fails."""
+    eng = _engine()
+    g, block, block0 = W.graph_and_blocks("fcc", 4)
+    cls = np.zeros(g.n, np.int32)
+    perm, mats = S.class_ops(g.coord, 8, cls, "d2h")
+    dmat = S.orbital_rep(mats)
+    fl = np.ones(g.n, np.uint8)
+    fl[0] = 0
+    stype = (np.arange(g.n) % 2).astype(np.int32)
+    op = A.reorder_nodes(A.build_operator(g, cls, [3], [[0, 1, 2]], [[0.66, 0.14, 0.2]], perm, dmat, 0, 2, maxrank=1,
+                                          flippable=fl, stype=stype, block=np.stack([block, 1.1 * block]),
+                                          block0=np.stack([block0, block0 + 0.01 * np.eye(9)]), nsp=3))
+    assert op.ntype == 2
+    spins = W.edc_params(np.random.default_rng(5), 3, 2)
+    members = [W.edc_member(spins[sp], cen, l0, complex(-0.2, 0.15)) for sp in range(2) for cen in range(3)
+               for l0 in range(0, 9, 2)]
+    nlev = np.array([8 + (k % 4) for k in range(len(members))], np.int32)
+    print(f"two site types: {op.nn} nodes, tile {tile}, {len(members)} chains", flush=True)
+    G = eng.GpuOperator(op)
+    G.set_tile(tile)
+    a, b, mu, _ = G.chains_edc(members, nlev)
+    O = _oracle(op)
+    for j in (0, 7, 18, len(members) - 1):
+        m = members[j]
+        ar, br, n = O.edc_chain(m, m.z * m.pa + m.pb, m.l0, int(nlev[j]))
+        assert mu[j] == n
+        assert rel(a[j, :n], ar[:n]) < 1e-9, f"chain {j}"
+        assert rel(b[j, :n - 1], br[:n - 1]) < 1e-9, f"chain {j}"
+
+
 @pytest.mark.parametrize("tile", [8, 16, 64, 65])
 def test_edc_member_tiles(ternary_small, tile):
     """Every member tile (several nodes per wave, or one node per wave with 64 members) gives the oracle's chains."""
