@@ -2417,10 +2417,11 @@ FArgs fargs0() {
   return f;
 }
 
-// the operator pass runs k_tile (member tile 8 or 16, orbital-preserving group, 14 or 18 slots, vectors under 2 GB)
+// the operator pass runs k_tile (member tile 8 or 16, orbital-preserving group, 14 or 18 slots, vectors under 4 GB: its
+// buffer offsets and record counts are unsigned 32-bit)
 bool tile_kernel(const asr_gpu_op* op, const MemDev& md, size_t es) {
   return !(md.mt == WAVE) && (md.mt == 8 || md.mt == 16) && op->piid && (op->nslot == 18 || op->nslot == 14) &&
-         (size_t)(op->nn + 1) * 9 * md.nbp * es < ((size_t)1 << 31);
+         (size_t)(op->nn + 1) * 9 * md.nbp * es < ((size_t)1 << 32);
 }
 
 template <class T>

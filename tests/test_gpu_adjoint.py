@@ -119,3 +119,30 @@ def test_torch_harness_contour_objective(wl):
     scale = np.max(np.abs(fd))
     assert scale > 1e-4
     assert np.max(np.abs(grad - fd)) < 1e-6 * scale, np.max(np.abs(grad - fd)) / scale
+
+
+def test_real_chains_beyond_2gb_vectors():
+    """The adjoint workload at full size: 256 real members on 127,517 nodes, vectors of 2.35 GB (beyond 2^31 bytes).  The
+    member-tile kernel (tile 16; unsigned 32-bit buffer offsets) gives the chains of the one-node-per-wave kernel (tile
+    64) and of the oracle (two chains, one at each end of the batch)."""
+    eng = _engine()
+    from oracle.oracle import OracleOp
+    wa = W.ternary_adjoint()
+    mem = AD.members_at(wa.members, wa.info["theta"], wa.info["c_offset"])
+    assert (wa.op.nn + 1) * 9 * len(mem) * 8 > 2 ** 31
+    nlev = 6
+    print(f"real chains beyond 2 GB: {wa.op.nn} nodes, {len(mem)} chains x {nlev}, tiles 16 and 64", flush=True)
+    G = eng.GpuOperator(wa.op)
+    res = {}
+    for tile in (16, 64):
+        G.set_tile(tile)
+        res[tile] = G.lanczos(mem, nlev, tiny=0.0)
+    a16, b16, mu16 = res[16][:3]
+    a64, b64, mu64 = res[64][:3]
+    assert np.array_equal(mu16, mu64)
+    assert np.max(np.abs(a16 - a64)) < 1e-12 and np.max(np.abs(b16 - b64)) < 1e-12
+    O = OracleOp(wa.op)
+    for j in (0, len(mem) - 1):
+        ar, br, n = O.lanczos(mem[j], mem[j].l0, nlev, tiny=0.0)
+        assert np.max(np.abs(a16[j, :n] - ar[:n])) < 1e-9 * np.max(np.abs(ar[:n]))
+        assert np.max(np.abs(b16[j, :n] - br[:n])) < 1e-9 * np.max(np.abs(br[:n]))
