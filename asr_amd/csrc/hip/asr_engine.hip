@@ -370,7 +370,8 @@ __global__ void __launch_bounds__(BLOCK) k_fused(OpDev op, MemDev md, FArgs fa) 
 // gathered row's registers take the next link's row as soon as its FMAs are issued (the next slot's gathers are in flight
 // during this slot's FMAs; the next column's block values and scale are requested one column ahead).  At 3 waves per
 // SIMD (168 VGPRs) the headline level ran 10 % faster than with the row-order loop at 4 waves, and 3 % faster than this
-// loop at 4 waves; 2 waves ran as fast as 3.  Otherwise the padded blocks come from LDS one row at a time.  The scale of a gathered value, dd(l) chi(g) sg(g, l) / sq(r), comes
+// loop at 4 waves; 2 waves ran as fast as 3.  With several site types the same loop reads each block column from the node
+// type's padded rows in LDS.  The scale of a gathered value, dd(l) chi(g) sg(g, l) / sq(r), comes
 // from an LDS table and is shared by the real and imaginary parts.  The species-diagonal terms use the same loads and an
 // LDS sign table; an energy-linear term X(z) = z X(a) + X(b) is one pass with a complex weight.  Member tiles of 8-16 keep one member slice
 // of a 1-2 GB vector small enough to stay in the last-level caches while all nodes are swept.
@@ -505,19 +506,17 @@ __global__ void __launch_bounds__(BLOCK, 3) k_tile(OpDev op, MemDev md, FArgs fa
       const double* __restrict__ bl = bls + (size_t)((inf >> 12) & 255u) * (NSLOT + 1) * 90;
       const double* __restrict__ em = etab + (size_t)ml * (op.nop + 1) * 9;
       const uint32_t* __restrict__ hp = op.hop + (size_t)i * NSLOT;
-      // slot k = 0: the node itself with the on-site block; k > 0: link k - 1
-      auto slot_src = [&](int k, uint32_t& voff, int& g, double& sc) {
-        const uint32_t e = hp[k > 0 ? k - 1 : 0];
-        const int r = k > 0 ? (int)(e >> 6) - 1 : i;
-        g = k > 0 ? (int)(e & 63u) : op.nop;
-        voff = (uint32_t)r * node16 + coff;
-        sc = fa.xs_norm ? op.isq[r] : 1.0;
-      };
-      if (op.ntype == 1) {
-        // one site type: the block is uniform over the wave and comes through scalar loads, transposed (sblk holds column
-        // l2 of slot k at k * 90 + l2 * 10).  Column order: gathered row l2 of slot k is dead after its 9 FMAs, and the
-        // same registers take row l2 of slot k + 1 at once, so the next slot's gathers are in flight while this slot's
-        // FMAs run, in the registers of one slot.  Each acc[l] still takes its terms in the order (k, l2).
+      // Slot k = 0: the node itself with the on-site block; k > 0: link k - 1.  Column order: gathered row l2 of slot k is
+      // dead after its 9 FMAs, and the same registers take row l2 of slot k + 1 at once, so the next slot's gathers are
+      // in flight while this slot's FMAs run, in the registers of one slot.  Each acc[l] takes its terms in the order
+      // (k, l2) of a row-order sum.  The block value B_k[l][l2]: with one site type uniform over the wave, from the
+      // transposed copy through scalar loads (sblk holds column l2 of slot k at k * 90 + l2 * 10); otherwise from the
+      // node type's padded rows in LDS.
+      auto run = [&](auto onetype) {
+        auto bv = [&](int k, int l2, int l) -> double {
+          if constexpr (decltype(onetype)::value) return sblk[(size_t)k * 90 + l2 * 10 + l];
+          else return bl[k * 90 + l * 10 + l2];
+        };
         // per slot, in the order of the loop body: 1/sq of the next slot's node (raw), the link after it, then the rows
         int g = op.nop;
         double q = op.isq[i];
@@ -532,7 +531,7 @@ __global__ void __launch_bounds__(BLOCK, 3) k_tile(OpDev op, MemDev md, FArgs fa
         // the block column and the gather scale of the next column are requested before this column's FMAs
         double bc[9], ec;
 #pragma unroll
-        for (int l = 0; l < 9; ++l) bc[l] = sblk[l];
+        for (int l = 0; l < 9; ++l) bc[l] = bv(0, 0, l);
         ec = em[op.nop * 9];
         auto slot = [&](int k, auto more) {
           const double* __restrict__ eg = em + g * 9;
@@ -544,17 +543,16 @@ __global__ void __launch_bounds__(BLOCK, 3) k_tile(OpDev op, MemDev md, FArgs fa
             q = op.isq[r];
             en = hp[k + 1 < NSLOT ? k + 1 : k];
           }
-          const double* __restrict__ bt = sblk + (size_t)k * 90;
 #pragma unroll
           for (int l2 = 0; l2 < 9; ++l2) {
             double bn[9], egn = 0.0;
             if (l2 < 8) {
 #pragma unroll
-              for (int l = 0; l < 9; ++l) bn[l] = bt[(l2 + 1) * 10 + l];
+              for (int l = 0; l < 9; ++l) bn[l] = bv(k, l2 + 1, l);
               egn = eg[l2 + 1];
             } else if constexpr (decltype(more)::value) {
 #pragma unroll
-              for (int l = 0; l < 9; ++l) bn[l] = bt[90 + l];
+              for (int l = 0; l < 9; ++l) bn[l] = bv(k + 1, 0, l);
               egn = em[g * 9];
             }
             const T x = (sck * ec) * v[l2];
@@ -572,30 +570,9 @@ __global__ void __launch_bounds__(BLOCK, 3) k_tile(OpDev op, MemDev md, FArgs fa
 #pragma unroll 1
         for (int k = 0; k < NSLOT; ++k) slot(k, std::true_type{});
         slot(NSLOT, std::false_type{});
-      } else {
-#pragma unroll 1
-        for (int k = 0; k <= NSLOT; ++k) {
-          uint32_t voff;
-          int g;
-          double sc;
-          slot_src(k, voff, g, sc);
-          T v[9];
-#pragma unroll
-          for (int l = 0; l < 9; ++l) v[l] = tile_row<T>(xr, voff, l * vs16);
-          const double* __restrict__ eg = em + g * 9;
-#pragma unroll
-          for (int l = 0; l < 9; ++l) v[l] = (sc * eg[l]) * v[l];
-          const double* __restrict__ bk = bl + k * 90;
-#pragma unroll
-          for (int l = 0; l < 9; ++l) {
-            T a = acc[l];
-#pragma unroll
-            for (int l2 = 0; l2 < 9; ++l2) a = fma_s(bk[l * 10 + l2], v[l2], a);
-            acc[l] = a;
-            __builtin_amdgcn_sched_barrier(0);
-          }
-        }
-      }
+      };
+      if (op.ntype == 1) run(std::true_type{});
+      else run(std::false_type{});
       for (int l = 0; l < 9; ++l) acc[l] = (fa.s_coef * (fa.s_dout ? md.dvec[(size_t)l * nbp + m] : 1.0)) * acc[l];
     }
     {
