@@ -730,7 +730,7 @@ __global__ void k_rand_nodes(OpDev op, MemDev md, const int* __restrict__ list, 
 // come from LDS (rows padded to 10 doubles), one row at a time next to its nine FMAs (a scheduling barrier per row keeps
 // the block out of the registers).  The orbital signs sg_k are sign-bit flips with a uniform mask; chi(g_k) / sq(r_k) enters
 // once per orbital.  Per link and column: 9 loads, 81 FMA, 18 multiplications, 9 sign flips.
-// The caller guarantees that the vector is below 4 GiB (32-bit buffer offsets).
+// launch_fused takes this path only for vectors below 4 GiB (buffer_ok: 32-bit buffer offsets and record counts).
 template <int NSLOT>
 __device__ __forceinline__ void s_link_load(__amdgpu_buffer_rsrc_t xr, uint32_t base, uint32_t vs8, uint32_t coff, double x[9]) {
 #pragma unroll
@@ -2394,11 +2394,17 @@ FArgs fargs0() {
   return f;
 }
 
-// the operator pass runs k_tile (member tile 8 or 16, orbital-preserving group, 14 or 18 slots, vectors under 4 GB: its
-// buffer offsets and record counts are unsigned 32-bit)
+// A chain vector of the operator's node set (nn + 1 rows of 9 x nbp values of es bytes) is below 4 GiB: the raw buffer
+// loads of k_tile and s_links_piid address it with unsigned 32-bit offsets and record counts.  Larger vectors take the
+// kernels that index through 64-bit pointers.
+bool buffer_ok(const asr_gpu_op* op, const MemDev& md, size_t es) {
+  return (size_t)(op->nn + 1) * 9 * md.nbp * es < ((size_t)1 << 32);
+}
+
+// the operator pass runs k_tile (member tile 8 or 16, orbital-preserving group, 14 or 18 slots, vectors under 4 GiB)
 bool tile_kernel(const asr_gpu_op* op, const MemDev& md, size_t es) {
   return !(md.mt == WAVE) && (md.mt == 8 || md.mt == 16) && op->piid && (op->nslot == 18 || op->nslot == 14) &&
-         (size_t)(op->nn + 1) * 9 * md.nbp * es < ((size_t)1 << 32);
+         buffer_ok(op, md, es);
 }
 
 template <class T>
@@ -2411,12 +2417,15 @@ int launch_fused(const asr_gpu_op* op, const MemDev& md, const FArgs& fa, hipStr
   const OpDev od = dev_of(op);
   const double* bl = op->blk;
   const size_t blds_bytes = (size_t)op->ntype * (op->nslot + 1) * 90 * 8;
+  // the buffer-load S pass of k_pair and of the real k_node (s_links_piid) needs a vector below 4 GiB; NS = 0 instances
+  // take the pointer path
+  const bool bok = buffer_ok(op, md, sizeof(T));
   if (md.mt == WAVE && md.pair && sizeof(T) == sizeof(cplx)) {
     grid.y = md.nbp / (WAVE / 2);
 #define ASR_PAIR(PI, NS) hipLaunchKernelGGL((k_pair<PI, NS>), grid, dim3(BLOCK), (PI && NS > 0) ? blds_bytes : 0, strm, od, md, \
                                          fa, bl, md.dperm)
-    if (op->piid && op->nslot == 18) ASR_PAIR(true, 18);
-    else if (op->piid && op->nslot == 14) ASR_PAIR(true, 14);
+    if (op->piid && op->nslot == 18 && bok) ASR_PAIR(true, 18);
+    else if (op->piid && op->nslot == 14 && bok) ASR_PAIR(true, 14);
     else if (op->piid) ASR_PAIR(true, 0);
     else if (op->nslot == 18) ASR_PAIR(false, 18);
     else ASR_PAIR(false, 0);
@@ -2424,8 +2433,9 @@ int launch_fused(const asr_gpu_op* op, const MemDev& md, const FArgs& fa, hipStr
   } else if (md.mt == WAVE) {
 #define ASR_NODE(PI, NS) hipLaunchKernelGGL((k_node<T, PI, NS>), grid, dim3(BLOCK), \
                                          (PI && NS > 0 && sizeof(T) == 8) ? blds_bytes : 0, strm, od, md, fa, bl, md.dperm)
-    if (op->piid && op->nslot == 18) ASR_NODE(true, 18);
-    else if (op->piid && op->nslot == 14) ASR_NODE(true, 14);
+    const bool nbok = bok || sizeof(T) != sizeof(double);  // complex members of k_node index through pointers
+    if (op->piid && op->nslot == 18 && nbok) ASR_NODE(true, 18);
+    else if (op->piid && op->nslot == 14 && nbok) ASR_NODE(true, 14);
     else if (op->piid) ASR_NODE(true, 0);
     else if (op->nslot == 18) ASR_NODE(false, 18);
     else ASR_NODE(false, 0);
