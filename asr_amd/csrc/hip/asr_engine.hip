@@ -72,6 +72,11 @@ __device__ inline double bilin(double a, double b) { return a * b; }
 __device__ inline cplx bilin(cplx a, cplx b) { return a * b; }
 __device__ inline double shfl_xor(double v, int m) { return __shfl_xor(v, m); }
 __device__ inline cplx shfl_xor(cplx v, int m) { return {__shfl_xor(v.re, m), __shfl_xor(v.im, m)}; }
+__device__ inline double inv(double v) { return 1.0 / v; }
+__device__ inline cplx inv(cplx v) {
+  double d = v.re * v.re + v.im * v.im;
+  return {v.re / d, -v.im / d};
+}
 
 // ------------------------------------------------------------------------------------------------ device operator
 struct OpDev {
@@ -276,6 +281,18 @@ struct FArgs {
   const void* x2;
   int q2x, x2norm;
   double c2x;
+  // k_tile, complex members, the fused level of the edc chains (one pass fewer per level):
+  // apart: partials of a_k = sum_i sq_i^2 [x~ (E + X(z)) x~ + x~ h1 - h1 (o h1)]_i into `partial`, at the nodes without
+  //   flip partners (uo: the chain vector u, h1 = the pass's output, o h1 = its out2 or zero; a_e: the E term is present);
+  // upd: the three-term update on the output, w = y - a u - b (s / s_prev) u_prev, the projection onto the symmetric
+  //   subspace (proj) and the partials of <w, w> into `partial` (aeff, beff, scv, scpv: per member; act: activity)
+  //   partial2: the partials of <u, y> before the update (the direct a_k, checked against the fused one by k_fin_b)
+  int apart, a_e, upd, proj;
+  void* partial2;
+  const void* uo;
+  const void* upv;
+  const void *aeff, *beff, *scv, *scpv;
+  const int* act;
 };
 
 template <class T>
@@ -411,20 +428,23 @@ __device__ __forceinline__ void rand_ct(const OpDev& op, const MemDev& md, const
       acc[l] = fma_s(coef * wa, xv, acc[l]);
     }
   };
-  const uint32_t own = (uint32_t)i * node16 + coff;
+  const uint32_t own_off = (uint32_t)i * node16 + coff;
   if (fs) {
     const double* fa_ = frozen_row(md, fs, qa) + m;
     const double* fb_ = frozen_row(md, fs, qb) + m;
 #pragma unroll
-    for (int l = 0; l < 9; ++l) add(fa_[(size_t)l * nbp], LIN ? fb_[(size_t)l * nbp] : 0.0, isi * row(own, l), l);
+    for (int l = 0; l < 9; ++l) {
+      add(fa_[(size_t)l * nbp], LIN ? fb_[(size_t)l * nbp] : 0.0, isi * row(own_off, l), l);
+    }
     return;
   }
   const int l0 = inf & 15u, kap = (inf >> 4) & 255u;
   const double* wa_ = md.wts + (((size_t)qa * op.ncls + kap) * nsm + l0) * nsm * 9 * nbp + m;
   const double* wb_ = md.wts + (((size_t)qb * op.ncls + kap) * nsm + l0) * nsm * 9 * nbp + m;
 #pragma unroll
-  for (int l = 0; l < 9; ++l)
-    add(wa_[((size_t)l0 * 9 + l) * nbp], LIN ? wb_[((size_t)l0 * 9 + l) * nbp] : 0.0, isi * row(own, l), l);
+  for (int l = 0; l < 9; ++l) {
+    add(wa_[((size_t)l0 * 9 + l) * nbp], LIN ? wb_[((size_t)l0 * 9 + l) * nbp] : 0.0, isi * row(own_off, l), l);
+  }
   const int nsc = op.nsc[kap];
   const uint32_t* fl = op.flip + (size_t)i * nsm;
   for (int k = 0; k < nsc; ++k) {
@@ -494,7 +514,7 @@ __global__ void __launch_bounds__(BLOCK, 3) k_tile(OpDev op, MemDev md, FArgs fa
   const int i = (xcd_block(blockIdx.x, gridDim.x) * (BLOCK / WAVE) + wave) * npw + lane / MT;
   const bool live = (i < op.nn) && (m < md.nb);
   const int nbp = md.nbp;
-  T dotv = zero<T>();
+  T dotv = zero<T>(), dotv2 = zero<T>();
   if (live) {
     const uint32_t inf = op.info[i];
     T acc[9];
@@ -598,22 +618,87 @@ __global__ void __launch_bounds__(BLOCK, 3) k_tile(OpDev op, MemDev md, FArgs fa
     const double so = fa.out_sq ? op.sq[i] : 1.0;
     T* out = (T*)fa.out;
     const T* du = (const T*)fa.dotu;
+    constexpr bool NTO = MT * sizeof(T) >= 128;
+    // the own-state weights of the node for parameter slot q (the frozen species' row on a frozen site)
+    auto wrow = [&](int q) -> const double* {
+      const int fs = frozen_slot(inf, md, m);
+      const int l0 = inf & 15u, kap = (inf >> 4) & 255u, nsm = op.nsmax;
+      return fs ? frozen_row(md, fs, q) + m : md.wts + ((((size_t)q * op.ncls + kap) * nsm + l0) * nsm + l0) * 9 * nbp + m;
+    };
+    if constexpr (std::is_same<T, cplx>::value && O2 < 2) {
+      // a_k at a node without flip partners: sq^2 [x~ (E + X(z)) x~ + x~ h1 - h1 t1] with h1 = y (out_sq = 0) and
+      // t1 = c2 X_q2 h1 (O2 = 1) or zero; (E + X(z)) and X_q2 are node-local there
+      if (fa.apart && !(inf & INFO_PARTNERS)) {
+        const cplx z = md.z[m];
+        const double* we = fa.a_e ? wrow(QE) : nullptr;
+        const double *wpa = wrow(QPA), *wpb = wrow(QPB), *wo = O2 == 1 ? wrow(fa.q2) : nullptr;
+        const double isi = op.isq[i], sq2 = op.sq[i] * op.sq[i];
+        const cplx* uo = (const cplx*)fa.uo;
+        cplx pa = zero<cplx>();
+        for (int l = 0; l < 9; ++l) {
+          const cplx xt = isi * uo[((size_t)i * 9 + l) * nbp + m];
+          const double wal = wpa[(size_t)l * nbp];
+          const cplx cw = cplx{fma(z.re, wal, wpb[(size_t)l * nbp]) + (we ? we[(size_t)l * nbp] : 0.0), z.im * wal};
+          const cplx h = so * acc[l];
+          cplx term = bilin(xt, cw * xt) + bilin(xt, h);
+          if constexpr (O2 == 1) term = term - bilin(h, fma_s(fa.c2 * wo[(size_t)l * nbp], 1.0 * h, zero<T>()));
+          pa = pa + term;
+        }
+        dotv = sq2 * pa;
+      }
+    }
     if constexpr (O2 == 0) {
-      for (int l = 0; l < 9; ++l) {
-        const T y = so * acc[l];
-        out_store<(MT * sizeof(T) >= 128)>(&out[((size_t)i * 9 + l) * nbp + m], y);
-        if (du) dotv = dotv + bilin(ld(du, i, l, nbp, m), y);
+      bool done = false;
+      if constexpr (std::is_same<T, cplx>::value) {
+        if (fa.upd) {
+          // w = y - a u - b (s_k / s_{k-1}) u_prev (lagged normalisation; zero for a member that has left), projected
+          // onto the symmetric subspace (every operation keeps each orbital: a factor per orbital), then <w, w>
+          const cplx a = ((const cplx*)fa.aeff)[m];
+          const cplx b = bilin(((const cplx*)fa.beff)[m], bilin(((const cplx*)fa.scv)[m], inv(((const cplx*)fa.scpv)[m])));
+          const bool off = !fa.act[m];
+          double f[9];
+          int ns = 1;
+          if (fa.proj) {
+            const int s0 = op.stab_ptr[i], s1 = op.stab_ptr[i + 1];
+            ns = s1 - s0;
+            if (ns > 1) {
+              const double* __restrict__ cs = cst + (size_t)ml * op.nop * 9;
+              for (int l = 0; l < 9; ++l) f[l] = 0.0;
+              for (int j = s0; j < s1; ++j) {
+                const int g = op.stab[j];
+                for (int l = 0; l < 9; ++l) f[l] += cs[g * 9 + l];
+              }
+            }
+          }
+          const cplx* uo = (const cplx*)fa.uo;
+          const cplx* up = (const cplx*)fa.upv;
+          cplx ww = zero<cplx>(), uy = zero<cplx>();
+          for (int l = 0; l < 9; ++l) {
+            const size_t q = ((size_t)i * 9 + l) * nbp + m;
+            const cplx y = so * acc[l], uq = uo[q];
+            uy = uy + bilin(uq, y);
+            cplx w = off ? zero<cplx>() : y - bilin(a, uq) - bilin(b, NTO ? nt_load(&up[q]) : up[q]);
+            if (ns > 1) w = (f[l] / (double)ns) * w;
+            out_store<NTO>(&out[q], w);
+            ww = ww + bilin(w, w);
+          }
+          dotv = ww;
+          dotv2 = uy;
+          done = true;
+        }
+      }
+      if (!done) {
+        for (int l = 0; l < 9; ++l) {
+          const T y = so * acc[l];
+          out_store<NTO>(&out[((size_t)i * 9 + l) * nbp + m], y);
+          if (du) dotv = dotv + bilin(ld(du, i, l, nbp, m), y);
+        }
       }
     } else {
       // out2: the own-state terms of rand_ct on the output row (and on x2), in rand_ct's operation order, no partner rows
       T* out2 = !(inf & INFO_PARTNERS) ? (T*)fa.out2 : nullptr;
       const double *wa = nullptr, *wb = nullptr, *xa = nullptr, *xb = nullptr;
       if (out2) {
-        const int fs = frozen_slot(inf, md, m);
-        const int l0 = inf & 15u, kap = (inf >> 4) & 255u, nsm = op.nsmax;
-        auto wrow = [&](int q) -> const double* {
-          return fs ? frozen_row(md, fs, q) + m : md.wts + ((((size_t)q * op.ncls + kap) * nsm + l0) * nsm + l0) * 9 * nbp + m;
-        };
         wa = wrow(fa.q2);
         if constexpr (O2 == 2) {
           wb = wrow(fa.q2 + 1);
@@ -641,9 +726,14 @@ __global__ void __launch_bounds__(BLOCK, 3) k_tile(OpDev op, MemDev md, FArgs fa
       }
     }
   }
-  if (fa.dotu) {
+  if (fa.dotu || fa.apart || fa.upd) {
     T* part = (T*)fa.partial;
     block_reduce_member<T>(dotv, MT, red, part + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * MT + threadIdx.x);
+  }
+  if (fa.partial2) {
+    __syncthreads();
+    T* part = (T*)fa.partial2;
+    block_reduce_member<T>(dotv2, MT, red, part + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * MT + threadIdx.x);
   }
 }
 
@@ -714,6 +804,36 @@ __global__ void k_rand_nodes(OpDev op, MemDev md, const int* __restrict__ list, 
   if (x2) rand_node_term<T>(op, md, i, inf, m, x2, q2x, c2x, x2norm, true, acc);
   rand_node_term<T>(op, md, i, inf, m, x, q, coef, 0, x2 != nullptr, acc);
   for (int l = 0; l < 9; ++l) y[((size_t)i * 9 + l) * nbp + m] = acc[l];
+}
+
+// The listed nodes (those with flip partners) of the fused edc level, one thread per (listed node, member): t1 = X_o h1
+// (when t1 is non-null, as k_rand_nodes) and the node's term of a_k, sq^2 [x~ (E + X(z)) x~ + x~ h1 - h1 t1] with the
+// partner rows (lpart: nlist x nbp, summed by k_fin_a after the block partials of the tile pass).
+__global__ void k_list_terms(OpDev op, MemDev md, const int* __restrict__ list, int nlist, const cplx* __restrict__ h1,
+                             cplx* __restrict__ t1, const cplx* __restrict__ x, int a_e, cplx* __restrict__ lpart) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int nbp = md.nbp;
+  const int j = (int)(t / nbp), m = (int)(t % nbp);
+  if (j >= nlist || m >= md.nb) return;
+  if (md.tact && !md.tact[m / md.mt]) return;
+  const int i = list[j];
+  const uint32_t inf = op.info[i];
+  cplx to[9], ex[9];
+  for (int l = 0; l < 9; ++l) { to[l] = zero<cplx>(); ex[l] = zero<cplx>(); }
+  if (t1) {
+    rand_node_term<cplx>(op, md, i, inf, m, h1, QO, 1.0, 0, false, to);
+    for (int l = 0; l < 9; ++l) t1[((size_t)i * 9 + l) * nbp + m] = to[l];
+  }
+  if (a_e) rand_node_term<cplx>(op, md, i, inf, m, x, QE, 1.0, 1, false, ex);
+  rand_node_term<cplx>(op, md, i, inf, m, x, QPA, 1.0, 1, true, ex);
+  const double isi = op.isq[i], sq2 = op.sq[i] * op.sq[i];
+  cplx pa = zero<cplx>();
+  for (int l = 0; l < 9; ++l) {
+    const size_t q = ((size_t)i * 9 + l) * nbp + m;
+    const cplx xt = isi * x[q], h = h1[q];
+    pa = pa + ((bilin(xt, ex[l]) + bilin(xt, h)) - bilin(h, to[l]));
+  }
+  lpart[(size_t)j * nbp + m] = sq2 * pa;
 }
 
 // The same fused operator pass with one node per wave and 64 members per wave (member tile 64): the node's links, weights
@@ -1245,11 +1365,6 @@ __device__ inline cplx csqrt_c(cplx v) {
 }
 __device__ inline double dsqrt(double v) { return csqrt_re(v); }
 __device__ inline cplx dsqrt(cplx v) { return csqrt_c(v); }
-__device__ inline double inv(double v) { return 1.0 / v; }
-__device__ inline cplx inv(cplx v) {
-  double d = v.re * v.re + v.im * v.im;
-  return {v.re / d, -v.im / d};
-}
 
 // fixed-order sum of the per-block partials of member m by one block of BLOCK threads: strided per-thread sums, then a
 // shuffle tree per wave and the wave sums in order (deterministic for a given launch shape)
@@ -1308,12 +1423,25 @@ __global__ void __launch_bounds__(BLOCK) k_adj_fin_grad(int nb, int nq, const do
 }
 
 // sc (lagged normalisation): the chain vector is stored as s_k u_k, so the partials are those of s_k^2 <u, H u>
+// lpart (the fused edc level): nlist further partials per member (stride nbp) added after the block partials
 template <class T>
 __global__ void __launch_bounds__(BLOCK) k_fin_a(ChainDev cd, int nb, const T* part, int nbx, int mt, T* a, T* a_eff,
-                                                  const T* sc) {
+                                                  const T* sc, const T* lpart, int nlist, int nbp) {
   __shared__ T lds[BLOCK / WAVE];
   const int m = blockIdx.x;
   T s = block_sum_partials(part, nbx, mt, m, lds);
+  if (lpart) {
+    __syncthreads();
+    T sl = zero<T>();
+    for (int j = threadIdx.x; j < nlist; j += BLOCK) sl = sl + lpart[(size_t)j * nbp + m];
+    for (int off = WAVE / 2; off > 0; off >>= 1) sl = sl + shfl_xor(sl, off);
+    const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x >> 6;
+    if (lane == 0) lds[wave] = sl;
+    __syncthreads();
+    T tot = lds[0];
+    for (int w = 1; w < BLOCK / WAVE; ++w) tot = tot + lds[w];
+    s = s + tot;
+  }
   if (threadIdx.x != 0) return;
   if (sc) s = bilin(s, inv(bilin(sc[m], sc[m])));
   int k = *cd.level;
@@ -1330,16 +1458,23 @@ __global__ void __launch_bounds__(BLOCK) k_fin_a(ChainDev cd, int nb, const T* p
 // of <t, w>, so that c_{k+1} = <t, u_{k+1}> = <t, w> / b_k.  Lagged normalisation (sc non-null): w holds s_k w_k, so
 // b_k = sqrt(sum / s_k^2) (the principal root of the same number as the unscaled chain) and the next vector is w itself
 // with the scale s_{k+1} = s_k b_k (snext; 1 for a member that left, whose vector the update zeroes).
+// pchk (the fused edc level): the partials of the direct <u, H u>; a member whose a_k (taken from the first pass on the
+// symmetry of the operator) differs from it beyond rounding sets *asym (the operator is not symmetric on the subspace)
 template <class T>
 __global__ void __launch_bounds__(BLOCK) k_fin_b(ChainDev cd, int nb, const T* part, int nbx, int mt, T* b, T* b_eff,
-                                                  T* inv_b, const T* part2, const T* sc, T* snext) {
+                                                  T* inv_b, const T* part2, const T* sc, T* snext, const T* pchk, int* asym,
+                                                  const T* a) {
   __shared__ T lds[BLOCK / WAVE];
   const int m = blockIdx.x;
   T s = block_sum_partials(part, nbx, mt, m, lds);
-  T s2 = zero<T>();
+  T s2 = zero<T>(), sd = zero<T>();
   if (part2) {
     __syncthreads();
     s2 = block_sum_partials(part2, nbx, mt, m, lds);
+  }
+  if (pchk) {
+    __syncthreads();
+    sd = block_sum_partials(pchk, nbx, mt, m, lds);
   }
   if (threadIdx.x != 0) return;
   int k = *cd.level;
@@ -1349,6 +1484,11 @@ __global__ void __launch_bounds__(BLOCK) k_fin_b(ChainDev cd, int nb, const T* p
     const T is2 = inv(bilin(s_m, s_m));
     s = bilin(s, is2);
     s2 = bilin(s2, inv(s_m));
+    sd = bilin(sd, is2);
+  }
+  if (pchk && cd.active[m]) {
+    const T af = a[(size_t)m * cd.ld + k];
+    if (!(mag(sd - af) <= 1.0e-9 * mag(sd) + 1.0e-12)) *asym = 1;
   }
   T bk = dsqrt(s);
   int act = cd.active[m];
@@ -2117,6 +2257,9 @@ struct asr_gpu_op {
   double depth_tol = 0.0;                         // adaptive depth of the edc chains (ed_depth_tol), 0 = off
   int depth_tail = 0;                             // tail of that test (ed_tail): 0 hull, 1 bp, 2 avg2, 3 none
   bool reorth = false;                            // full reorthogonalisation of the real chains (lanczos reorth)
+  int level_form = 0;                             // edc chains: 0 the fused level (a_k from the first pass, the update
+                                                  // in the second), 1 the separate update pass
+  bool asym = false;                              // the fused level's check found the operator not symmetric: form 1
   bool slot_flippable[2] = {false, false};        // the site of frozen slot 1 / 2 was flippable (no species may be frozen)
   std::vector<std::pair<long long, int>> tuned;   // (members, complex, uniform d, zero o) key -> member tile
 };
@@ -2368,7 +2511,8 @@ KTimer* g_timer = nullptr;
 std::vector<double> g_acc;  // summed duration per kernel ordinal of a level
 int g_levels = 0;
 
-enum { KIND_SPMM = 0, KIND_RAND = 1, KIND_FIN_A = 2, KIND_UPDATE = 3, KIND_FIN_B = 4, KIND_SCALE = 5, KIND_FIX = 6 };
+enum { KIND_SPMM = 0, KIND_RAND = 1, KIND_FIN_A = 2, KIND_UPDATE = 3, KIND_FIN_B = 4, KIND_SCALE = 5, KIND_FIX = 6,
+       KIND_SPMM_UPD = 7 };
 
 int timer_mark(hipStream_t s, int kind, bool start) {
   if (!g_timer) return 0;
@@ -2413,7 +2557,8 @@ int launch_fused(const asr_gpu_op* op, const MemDev& md, const FArgs& fa, hipStr
   if (fa.x2 && !std::is_same<T, cplx>::value) return fail("operator pass: energy-linear out2 terms need complex members");
   dim3 grid = node_grid(op, md);
   size_t lds = fused_lds_bytes(op, sizeof(T));
-  if (timer_mark(strm, fa.xs ? KIND_SPMM : KIND_RAND, true)) return 1;
+  if ((fa.upd || fa.apart) && !tile_kernel(op, md, sizeof(T))) return fail("operator pass: the fused level needs the tile kernel");
+  if (timer_mark(strm, fa.upd ? KIND_SPMM_UPD : fa.xs ? KIND_SPMM : KIND_RAND, true)) return 1;
   const OpDev od = dev_of(op);
   const double* bl = op->blk;
   const size_t blds_bytes = (size_t)op->ntype * (op->nslot + 1) * 90 * 8;
@@ -2655,11 +2800,17 @@ int autotune_tile(asr_gpu_op* op, int nchain, const int* start_orb, const double
   return 0;
 }
 
+// status of a chain run that must be repeated with the separate update pass (ChainRun::finish)
+constexpr int RERUN_ASYM = 2;
+
 template <class T>
 struct ChainBuffers {
   T *u = nullptr, *up = nullptr, *w = nullptr, *t1 = nullptr, *t2 = nullptr, *h1 = nullptr, *part = nullptr;
   T *a = nullptr, *b = nullptr, *a_eff = nullptr, *b_eff = nullptr, *inv_b = nullptr;
   T *tv = nullptr, *part2 = nullptr, *c = nullptr;  // route ed3: projection vector, its dot partials, c_k
+  T* lpart = nullptr;  // the fused edc level: a_k terms of the nodes with flip partners (nplist x nbp)
+  T* pchk = nullptr;   // ... the partials of the direct <u, H u> of its check
+  int* asym = nullptr; // ... set on the device when the check fails
   double *qv = nullptr, *qpart = nullptr, *qcoef = nullptr;  // real chains with full reorthogonalisation
   double* tw = nullptr;  // projected chains: dressing weights ntarget x nsmax x 9 x nbp
   T* cpr = nullptr;      // projected chains: projections per level, ld x ntarget x 9 x nbp
@@ -2672,7 +2823,8 @@ struct ChainBuffers {
         else (void)hipFree(p);
       }
     for (void* p : {(void*)part, (void*)a, (void*)b, (void*)a_eff, (void*)b_eff, (void*)inv_b, (void*)level, (void*)depth,
-                    (void*)active, (void*)mused, (void*)part2, (void*)c, (void*)closed, (void*)qv, (void*)qpart,
+                    (void*)active, (void*)mused, (void*)part2, (void*)c, (void*)closed, (void*)lpart, (void*)pchk,
+                    (void*)asym, (void*)qv, (void*)qpart,
                     (void*)qcoef, (void*)tw, (void*)cpr})
       if (p) (void)hipFree(p);
   }
@@ -2722,6 +2874,12 @@ struct ChainRun {
   // coefficients, c_k, the projections); every 16 levels the vector is normalised once (k_rescale).  The scales rotate
   // over three buffers: sc3[k % 3] = s_k, sc3[(k + 2) % 3] = s_{k-1}, sc3[(k + 1) % 3] receives s_{k+1}.
   bool lag = false;
+  // The fused level (complex edc chains on the member-tile pass): a_k = <x, H x> / s_k^2 from the first operator pass, by
+  // the symmetry of Sq^2 h and Sq^2 X_o on the symmetric subspace,
+  //   <x~, Sq^2 (E + X(z) + h - h X_o h) x~> = sum_i sq_i^2 [x~ (E + X(z)) x~ + x~ h1 - h1 (X_o h1)]_i,  h1 = h x~,
+  // so the second pass applies the three-term update, the projection and the partials of <w, w> to its output in
+  // registers, and the separate update pass (a read of w, u, u_prev and a write of w per level) goes away.
+  bool fused = false;
   T* sc3[3] = {nullptr, nullptr, nullptr};
   const T* s_cur() const { return lag ? sc3[level_host % 3] : nullptr; }
   const T* s_prev() const { return lag ? sc3[(level_host + 2) % 3] : nullptr; }
@@ -2827,6 +2985,14 @@ struct ChainRun {
       HCHECK(hipMemsetAsync(B.cpr, 0, (size_t)ldab * nt * 9 * M.nbp * sizeof(T), stream));
     }
     lag = std::is_same<T, cplx>::value && !real_chain;
+    // the fused level of the edc chains: the member-tile operator pass with one d for every species
+    fused = lag && lin && project && !ed3 && !reorth && M.uniform_d && op->level_form == 0 && !op->asym &&
+            tile_kernel(op, md, sizeof(T));
+    if (fused) {
+      if (op->nplist > 0 && dalloc(&B.lpart, (size_t)op->nplist * M.nbp)) return 1;
+      if (dalloc(&B.pchk, npart) || dalloc(&B.asym, 1)) return 1;
+      HCHECK(hipMemsetAsync(B.asym, 0, sizeof(int), stream));
+    }
     std::vector<T> one;  // lives until the stream synchronisation below (source of asynchronous copies)
     if (lag) {
       one.resize(M.nbp);
@@ -2968,11 +3134,61 @@ struct ChainRun {
                          B.cpr + (size_t)level_host * nt * 9 * M.nbp, s_cur());
       HCHECK(hipGetLastError());
     }
-    if (apply_level_op<T>(op, M, md, B.u, B.w, B.t1, B.t2, B.h1, lin, B.u, B.part, stream)) return 1;
+    if (fused) {
+      if (fused_apply()) return 1;
+    } else if (apply_level_op<T>(op, M, md, B.u, B.w, B.t1, B.t2, B.h1, lin, B.u, B.part, stream)) {
+      return 1;
+    }
     if (timer_mark(stream, KIND_FIN_A, true)) return 1;
-    hipLaunchKernelGGL(k_fin_a<T>, dim3(M.nb), dim3(BLOCK), 0, stream, cd, M.nb, B.part, nbx, apply_mt, B.a, B.a_eff, s_cur());
+    hipLaunchKernelGGL(k_fin_a<T>, dim3(M.nb), dim3(BLOCK), 0, stream, cd, M.nb, B.part, nbx, apply_mt, B.a, B.a_eff, s_cur(),
+                       (const T*)B.lpart, fused ? op->nplist : 0, M.nbp);
     if (timer_mark(stream, 0, false)) return 1;
     return 0;
+  }
+
+  // the fused level, first half: h1 = rand_c(u~) + D S D u~, t1 = X_o h1 and the partials of a_k (tile pass at the nodes
+  // without flip partners, k_list_terms at the others)
+  int fused_apply() {
+    if constexpr (!std::is_same<T, cplx>::value) {
+      return fail("fused level: complex chains only");
+    } else {
+      FArgs f = fargs0();
+      f.xs = B.u; f.xs_norm = 1; f.s_din = 1; f.s_dout = 1; f.s_coef = 1.0;
+      f.xr[0] = B.u; f.qr[0] = QC; f.r_norm[0] = 1; f.r_coef[0] = 1.0; f.nr = 1;
+      f.out = B.h1;
+      if (!M.zero_o) { f.out2 = B.t1; f.q2 = QO; f.c2 = 1.0; }
+      f.apart = 1; f.a_e = M.zero_e ? 0 : 1; f.uo = B.u; f.partial = B.part;
+      if (launch_fused<T>(op, md, f, stream)) return 1;
+      if (op->nplist > 0) {
+        if (timer_mark(stream, KIND_FIX, true)) return 1;
+        const size_t nt = (size_t)op->nplist * md.nbp;
+        hipLaunchKernelGGL(k_list_terms, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, stream, dev_of(op), md,
+                           (const int*)op->plist, op->nplist, (const cplx*)B.h1, M.zero_o ? (cplx*)nullptr : (cplx*)B.t1,
+                           (const cplx*)B.u, M.zero_e ? 0 : 1, (cplx*)B.lpart);
+        HCHECK(hipGetLastError());
+        if (timer_mark(stream, 0, false)) return 1;
+      }
+      return 0;
+    }
+  }
+
+  // the fused level, second half: w = sq (E u~ + h1 - h t1 + X(z) u~) - a u - b u_prev, projected, and <w, w>
+  int fused_update() {
+    FArgs f = fargs0();
+    int nr = 0;
+    if (!M.zero_o) {
+      f.xs = B.t1; f.s_din = 1; f.s_dout = 1; f.s_coef = -1.0;
+      f.xr[nr] = B.t1; f.qr[nr] = QC; f.r_norm[nr] = 0; f.r_coef[nr] = -1.0; ++nr;
+    }
+    if (!M.zero_e) { f.xr[nr] = B.u; f.qr[nr] = QE; f.r_norm[nr] = 1; f.r_coef[nr] = 1.0; ++nr; }
+    f.nr = nr;
+    f.xl[0] = B.u; f.lq[0] = QPA; f.l_norm[0] = 1; f.l_coef[0] = 1.0; f.nl = 1;
+    f.xo[0] = B.h1; f.o_coef[0] = 1.0; f.no = 1;
+    f.out_sq = 1;
+    f.upd = 1; f.proj = project; f.uo = B.u; f.upv = B.up; f.aeff = B.a_eff; f.beff = B.b_eff; f.scv = s_cur();
+    f.scpv = s_prev(); f.act = B.active;
+    f.out = B.w; f.partial = B.part; f.partial2 = B.pchk;
+    return launch_fused<T>(op, md, f, stream);
   }
 
   // Adaptive depth of edc_green (src/asr_edc.f90:121-127): at levels k = 8, 12, 16, ... (1-based) of a chain that has
@@ -3051,6 +3267,9 @@ struct ChainRun {
   }
 
   int step_update() {
+    if (fused) {
+      if (fused_update()) return 1;
+    } else {
     if (timer_mark(stream, KIND_UPDATE, true)) return 1;
     if (md.mt * (int)sizeof(T) >= 128)
       hipLaunchKernelGGL((k_update<T, true>), grid, dim3(BLOCK), 0, stream, dev_of(op), md, B.w, B.u, B.up, B.a_eff, B.b_eff,
@@ -3072,9 +3291,11 @@ struct ChainRun {
                            (const double*)B.qcoef, pass == 1 ? (double*)B.part : (double*)nullptr);
       }
     }
-    if (timer_mark(stream, 0, false) || timer_mark(stream, KIND_FIN_B, true)) return 1;
+    if (timer_mark(stream, 0, false)) return 1;
+    }
+    if (timer_mark(stream, KIND_FIN_B, true)) return 1;
     hipLaunchKernelGGL(k_fin_b<T>, dim3(M.nb), dim3(BLOCK), 0, stream, cd, M.nb, B.part, nbx, md.mt, B.b, B.b_eff,
-                       B.inv_b, (const T*)B.part2, s_cur(), s_next());
+                       B.inv_b, (const T*)B.part2, s_cur(), s_next(), (const T*)B.pchk, B.asym, (const T*)B.a);
     if (timer_mark(stream, 0, false)) return 1;
     if (!lag) {
       if (timer_mark(stream, KIND_SCALE, true)) return 1;
@@ -3114,6 +3335,18 @@ struct ChainRun {
   // coefficients in the caller's chain order; route ed3 also c_k, the closure flags and the start / projection norms
   int finish(T* a_out, T* b_out, int* mused_out, T* c_out = nullptr, int* closed_out = nullptr, double* snorm_out = nullptr,
              double* tnorm_out = nullptr, T* cpr_out = nullptr) {
+    if (fused) {
+      // the fused level assumes an operator symmetric on the symmetric subspace (S_RR' = S_R'R^T); when its a_k and the
+      // direct <u, H u> disagree the run is void: the operator keeps the separate update pass from now on
+      int bad = 0;
+      HCHECK(hipMemcpyAsync(&bad, B.asym, sizeof(int), hipMemcpyDeviceToHost, stream));
+      HCHECK(hipStreamSynchronize(stream));
+      if (bad) {
+        op->asym = true;
+        fail("fused level: the operator is not symmetric on the symmetric subspace; rerun with the separate update pass");
+        return RERUN_ASYM;
+      }
+    }
     if (proj && cpr_out) {
       // projections in the caller's order: chain x level x target x orbital
       const size_t per = (size_t)op->ntarget * 9;
@@ -3154,7 +3387,13 @@ struct ChainRun {
 template <class T>
 int run_chains(asr_gpu_op* op, int nchain, const int* start_orb, const double* params, const double* pab,
                const int* frozen_sp, const double* conc, const double* z, const int* nlev, int project, int ldab, T* a_out,
-               T* b_out, int* mused_out, int mode, double tiny, bool real_chain, double* elapsed_ms) {
+               T* b_out, int* mused_out, int mode, double tiny, bool real_chain, double* elapsed_ms);
+
+// a run of the fused level that found the operator not symmetric is repeated with the separate update pass
+template <class T>
+int run_chains_once(asr_gpu_op* op, int nchain, const int* start_orb, const double* params, const double* pab,
+                    const int* frozen_sp, const double* conc, const double* z, const int* nlev, int project, int ldab,
+                    T* a_out, T* b_out, int* mused_out, int mode, double tiny, bool real_chain, double* elapsed_ms) {
   if (nchain <= 0) return 0;
   const int nsp9 = op->nsp * 9, ncn = op->ncls * op->nsmax;
   if (mode == 1 && nchain > 1) {
@@ -3197,15 +3436,33 @@ int run_chains(asr_gpu_op* op, int nchain, const int* start_orb, const double* p
   return R.finish(a_out, b_out, mused_out);
 }
 
+template <class T>
+int run_chains(asr_gpu_op* op, int nchain, const int* start_orb, const double* params, const double* pab,
+               const int* frozen_sp, const double* conc, const double* z, const int* nlev, int project, int ldab, T* a_out,
+               T* b_out, int* mused_out, int mode, double tiny, bool real_chain, double* elapsed_ms) {
+  int st = RERUN_ASYM;
+  for (int pass = 0; pass < 2 && st == RERUN_ASYM; ++pass) {
+    if (pass && g_timer) {  // the per-kernel timings start over with the separate update pass
+      g_acc.clear();
+      g_levels = 0;
+      g_timer->kind.clear();
+    }
+    st = run_chains_once<T>(op, nchain, start_orb, params, pab, frozen_sp, conc, z, nlev, project, ldab, a_out, b_out,
+                            mused_out, mode, tiny, real_chain, elapsed_ms);
+  }
+  return st;
+}
+
 // Chains on several operators at once (one group per operator: the sublattices of a multi-sublattice alloy).  Every group
 // runs on its own stream so that small operators fill the GPU together; members of a group are sorted by depth and tiles
 // whose members have all left are skipped.  Per-chain arrays are concatenated over the groups in order; conc (optional)
 // holds per chain the ncls x nsmax concentrations of its own operator.  mode 0: host-issued levels on the group streams;
 // mode 2: the whole run captured into one HIP graph and launched once; mode 1: the plain form, one chain per launch sequence.
 template <class T>
-int run_multi(int ngroup, asr_gpu_op* const* ops, const int* ngch, const int* start_orb, const double* params,
-              const double* pab, const int* frozen_sp, const double* conc, const double* z, const int* nlev, int project,
-              int ldab, T* a, T* b, int* mused, int mode, double tiny, bool real_chain, double* elapsed_ms) {
+int run_multi_once(int ngroup, asr_gpu_op* const* ops, const int* ngch, const int* start_orb, const double* params,
+                   const double* pab, const int* frozen_sp, const double* conc, const double* z, const int* nlev,
+                   int project, int ldab, T* a, T* b, int* mused, int mode, double tiny, bool real_chain,
+                   double* elapsed_ms) {
   if (elapsed_ms) *elapsed_ms = 0.0;
   std::vector<size_t> off_c(ngroup + 1, 0), off_p(ngroup + 1, 0), off_pa(ngroup + 1, 0), off_cc(ngroup + 1, 0);
   for (int g = 0; g < ngroup; ++g) {
@@ -3307,6 +3564,17 @@ int run_multi(int ngroup, asr_gpu_op* const* ops, const int* ngch, const int* st
   return status;
 }
 
+template <class T>
+int run_multi(int ngroup, asr_gpu_op* const* ops, const int* ngch, const int* start_orb, const double* params,
+              const double* pab, const int* frozen_sp, const double* conc, const double* z, const int* nlev, int project,
+              int ldab, T* a, T* b, int* mused, int mode, double tiny, bool real_chain, double* elapsed_ms) {
+  int st = RERUN_ASYM;
+  for (int pass = 0; pass < 2 && st == RERUN_ASYM; ++pass)
+    st = run_multi_once<T>(ngroup, ops, ngch, start_orb, params, pab, frozen_sp, conc, z, nlev, project, ldab, a, b, mused,
+                           mode, tiny, real_chain, elapsed_ms);
+  return st;
+}
+
 // The chains of two-chain blocks on one operator (the members of asr_gpu_ed3_blocks and asr_gpu_epi_blocks, two per
 // block): mode 0 host-issued levels, 1 the plain form (one chain per launch sequence, host read-back per level, as chain),
 // 2 the whole run as one HIP graph (fixed depths only: the adaptive depth decides on the host between levels, so it
@@ -3382,7 +3650,11 @@ int run_blocks(asr_gpu_op* op, int nmem, const int* so, const double* params, co
 // plain form (one chain per launch sequence), 2 the whole run as one HIP graph.  cpr: nchain x ldab x ntarget x 9.
 int run_proj(asr_gpu_op* op, int nchain, const int* so, const double* params, const double* pab, const int* fz,
              const double* conc, const double* z, const int* nlev, const double* tw, int project, int ldab, cplx* a, cplx* b,
-             int* mused, cplx* cpr, int mode, double* elapsed_ms) {
+             int* mused, cplx* cpr, int mode, double* elapsed_ms);
+
+int run_proj_once(asr_gpu_op* op, int nchain, const int* so, const double* params, const double* pab, const int* fz,
+                  const double* conc, const double* z, const int* nlev, const double* tw, int project, int ldab, cplx* a,
+                  cplx* b, int* mused, cplx* cpr, int mode, double* elapsed_ms) {
   const int nsp9 = op->nsp * 9, ncn = op->ncls * op->nsmax, ntw = op->ntarget * op->nsmax * 9;
   *elapsed_ms = 0.0;
   if (mode == 1 && nchain > 1) {
@@ -3441,6 +3713,15 @@ int run_proj(asr_gpu_op* op, int nchain, const int* so, const double* params, co
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return status;
+}
+
+int run_proj(asr_gpu_op* op, int nchain, const int* so, const double* params, const double* pab, const int* fz,
+             const double* conc, const double* z, const int* nlev, const double* tw, int project, int ldab, cplx* a, cplx* b,
+             int* mused, cplx* cpr, int mode, double* elapsed_ms) {
+  int st = RERUN_ASYM;
+  for (int pass = 0; pass < 2 && st == RERUN_ASYM; ++pass)
+    st = run_proj_once(op, nchain, so, params, pab, fz, conc, z, nlev, tw, project, ldab, a, b, mused, cpr, mode, elapsed_ms);
+  return st;
 }
 
 // Adjoint of the real Lanczos recursion (the reference has none; the forward is lanczos, src/reference/asr_recursion.f90:17-58)
@@ -3861,6 +4142,13 @@ int asr_gpu_chains_edc_proj(asr_gpu_op* op, int nchain, const int* start_orb, co
 }
 
 // Full reorthogonalisation of the real chains (lanczos reorth = .true., src/reference/asr_recursion.f90:17-58).
+int asr_gpu_op_set_level_form(asr_gpu_op* op, int form) {
+  if (!op) return fail("asr_gpu_op_set_level_form: null operator");
+  if (form != 0 && form != 1) return fail("asr_gpu_op_set_level_form: form 0 (fused level) or 1 (separate update pass)");
+  op->level_form = form;
+  return 0;
+}
+
 int asr_gpu_op_set_reorth(asr_gpu_op* op, int reorth) {
   if (!op) return fail("asr_gpu_op_set_reorth: null operator");
   op->reorth = reorth != 0;

@@ -40,6 +40,7 @@ def lib() -> C.CDLL:
         L.asr_gpu_op_set_tile.argtypes = [P, C.c_int]
         L.asr_gpu_op_set_depth_tol.argtypes = [P, C.c_double, C.c_int]
         L.asr_gpu_op_set_reorth.argtypes = [P, C.c_int]
+        L.asr_gpu_op_set_level_form.argtypes = [P, C.c_int]
         L.asr_gpu_op_set_targets.argtypes = [P, C.c_int, P, P]
         L.asr_gpu_chains_edc_proj.argtypes = [P, C.c_int, P, P, P, P, P, P, P, C.c_int, C.c_int, P, P, P, P, P, P, P, C.c_int, P]
         L.asr_gpu_op_get_tile.argtypes = [P, C.c_int, C.c_int, P]
@@ -181,6 +182,11 @@ class GpuOperator:
     def set_reorth(self, reorth: bool):
         """Full reorthogonalisation of the real chains (lanczos reorth = .true.; small cases)."""
         _check(lib().asr_gpu_op_set_reorth(self.h, int(bool(reorth))), "asr_gpu_op_set_reorth")
+
+    def set_level_form(self, form: int):
+        """Form of an edc chain level: 0 the fused level (a_n from the first operator pass, the update folded into the
+        second), 1 the separate update pass in the reference's order (edc_green)."""
+        _check(lib().asr_gpu_op_set_level_form(self.h, int(form)), "asr_gpu_op_set_level_form")
 
     def set_depth_tol(self, depth_tol: float, tail: str = "hull"):
         """Adaptive depth of the edc chains (ed_depth_tol / ed_tail of edc_green); 0 = every chain to its depth."""

@@ -32,28 +32,35 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-# vectors read by the operator launches of one level of the route-edc chains (uniform d, o != 0, X(z) diagonal):
-# launch 0: h1 = rand_c(x~) + D S D x~ (reads x); launch 1: t1 = rand_o(h1) (reads h1); launch 2: w = sq (h1 - h t1
-# + X(z) x~) with the dot <u, w> (reads t1, h1 and x = u).  Each writes one vector.
-# (vectors read, vectors written) per launch of the level's operator pass: with the tile kernel the first S pass also
-# writes the node-local species-diagonal term (launch 1 is the small pass over the nodes with flip partners, kind 6)
-EDC_LAUNCH_IO = {0: (1, 1), 1: (1, 1), 2: (3, 1)}
-EDC_LAUNCH_IO_FUSED = {0: (1, 2), 2: (3, 1)}
+# (vectors read, vectors written) per launch of a level of the route-edc chains (uniform d, o != 0, X(z) diagonal), by
+# launch kind.  Separate update pass: 0 h1 = rand_c(x~) + D S D x~ (reads x; with the tile kernel it also writes the
+# node-local t1 = X_o h1: then 6 writes t1 at the nodes with flip partners), [1 t1 = X_o h1], 0 w = sq (h1 - h t1 + X(z) x~)
+# with the dot <u, w> (reads t1, h1, x), 3 the update (reads w, u, u_prev, writes w).  Fused level: 0 as above with the
+# partials of a_n, 6 also the a_n terms of the listed nodes, 7 the second operator pass with the update (reads t1, h1, x and
+# u_prev, writes the updated w).
+EDC_IO_FIRST = (1, 2)
+EDC_IO_SECOND = (3, 1)
+EDC_IO_UPD = (4, 1)
 
 
-def chain_kernel_cost(op, nb: int, kind: int, nread: int = 1, nwrite: int = 1):
+def chain_kernel_cost(op, nb: int, kind: int, tile: int = 8, nread: int = 1, nwrite: int = 1):
     """Algorithmic bytes and flops of one launch of a chain-level kernel over nn nodes and nb complex members.
 
-    Operator pass (kind 0, S pass + flip terms; kind 1, flip terms only): read nread vectors once (9 nb complex per node
-    each), write the result once, read the node tables once (hop 4 nslot + flip 4 nsmax + info 4 + sq 8 bytes per node);
-    flops: (1 + nslot) 9x9 blocks on complex members (4 flops per complex-by-real FMA pair) plus the flip terms and the
-    D scalings."""
+    Operator passes (kind 0, S pass + flip terms; kind 7, the same with the three-term update; kind 1, flip terms only):
+    read nread vectors once (9 nb complex per node each), write nwrite once, read the node tables once (hop 4 nslot +
+    flip 4 nsmax + info 4 + sq 8 bytes per node); flops: (1 + nslot) 9x9 blocks on complex members (4 flops per
+    complex-by-real FMA pair) plus the flip terms and the D scalings (kind 7: the update, 9 complex a u + b u_prev and the
+    norm).  Reductions (kinds 2, 4): the per-block partials, one complex value per block and member (plus the listed
+    nodes' terms of a_n).  Update (3): w, u, u_prev read, w written.  Rescale (5): one vector read and written."""
     nn, ns, nsm = op.nn, op.nslot, op.nsmax
     vec = nn * 9 * nb * 16
     tables = nn * (4 * ns + 4 * nsm + 4 + 8)
-    if kind == 0:
+    nbx = -(-nn // ((64 // min(tile, 64)) * 4))
+    if kind in (0, 7):
         bytes_ = (nread + nwrite) * vec + tables
         flops = nn * nb * ((1 + ns) * 81 * 4 + nsm * 9 * 4 + (1 + ns) * 9 * 2 * 2 + 9 * 4 + (nwrite - 1) * 9 * 2)
+        if kind == 7:
+            flops += nn * nb * 9 * (8 + 8 + 4)
     elif kind == 1:
         bytes_ = (nread + 1) * vec + nn * (4 * nsm + 4 + 8)
         flops = nn * nb * (nsm * 9 * 4)
@@ -66,10 +73,35 @@ def chain_kernel_cost(op, nb: int, kind: int, nread: int = 1, nwrite: int = 1):
         npf = int(np.sum(((fr >= 0) & (fr != np.arange(nn)[:, None])).any(axis=1)))
         bytes_ = npf * 9 * nb * 16 * (nsm + 1)
         flops = npf * nb * nsm * 9 * 4
+    elif kind in (2, 4):
+        bytes_ = nbx * -(-nb // tile) * tile * 16 + nb * 16 * 4
+        flops = nbx * nb * 2
     else:
         bytes_ = 2 * vec
         flops = nn * nb * 9 * 6
     return float(bytes_), float(flops)
+
+
+def level_io(kinds):
+    """(vectors read, written) of each launch of a level from the sequence of launch kinds."""
+    out, first = [], True
+    for k in kinds:
+        if k == 0:
+            out.append(EDC_IO_FIRST if first and 6 in kinds else (EDC_IO_SECOND if not first else (1, 1)))
+            first = False
+        elif k == 7:
+            out.append(EDC_IO_UPD)
+        else:
+            out.append((1, 1))
+    return out
+
+
+def compulsory_level_bytes(op, nb: int, s: int = 2):
+    """The compulsory HBM bytes of one level of nb chains sharing one operator pass: per chain-level
+    N [s 9 8 3 + T / n_share] (read u_k, read u_k-1, write u_k+1; the node tables T once per pass, shared by n_share = nb
+    chains), T = the bytes per node of this engine's tables (hop 4 nslot + flip 4 nsmax + info 4 + sq 8)."""
+    t = 4 * op.nslot + 4 * op.nsmax + 4 + 8
+    return float(nb * op.nn * (s * 9 * 8 * 3) + op.nn * t)
 
 
 def kpm_kernel_cost(nsite: int, nslot: int, nvec: int):
@@ -81,14 +113,17 @@ def kpm_kernel_cost(nsite: int, nslot: int, nvec: int):
     return float(bytes_), float(flops)
 
 
-def roofline(bytes_, flops, ms):
+def roofline(bytes_, flops, ms, mfma: bool = False):
+    """Roofline of a launch: FP64-bound when its intensity exceeds the ridge (bound "fp64" for VALU FMA kernels, "mfma"
+    for kernels that contract on the matrix cores; the FP64 peak of MI355X is 78.6 TFLOP/s for both), else HBM-bound."""
     t = ms * 1e-3
     gbs = bytes_ / t / 1e9
     tfs = flops / t / 1e12
     intensity = flops / bytes_
     if intensity > (FP64_PEAK_TFS * 1e12) / (HBM_PEAK_GBS * 1e9):
-        return dict(bound="mfma", achieved=round(tfs, 3), peak=FP64_PEAK_TFS, unit="TFLOP/s", frac=round(tfs / FP64_PEAK_TFS, 4),
-                    hbm_gbs=round(gbs, 1), hbm_frac=round(gbs / HBM_PEAK_GBS, 4), flop_per_byte=round(intensity, 2))
+        return dict(bound="mfma" if mfma else "fp64", achieved=round(tfs, 3), peak=FP64_PEAK_TFS, unit="TFLOP/s",
+                    frac=round(tfs / FP64_PEAK_TFS, 4), hbm_gbs=round(gbs, 1), hbm_frac=round(gbs / HBM_PEAK_GBS, 4),
+                    flop_per_byte=round(intensity, 2))
     return dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(gbs / HBM_PEAK_GBS, 4),
                 fp64_tfs=round(tfs, 3), flop_per_byte=round(intensity, 2))
 
@@ -183,24 +218,26 @@ def main():
     log(f"autotuned member tile: {out_tile}")
     # per-kernel durations of the level step (HIP events on the engine's stream)
     kms, kinds = G.profile(wl.members, wl.nlev)
+    kinds = [int(k) for k in kinds]
+    io = level_io(kinds)
     per_kernel = []
-    io = EDC_LAUNCH_IO_FUSED if 6 in [int(k) for k in kinds] else EDC_LAUNCH_IO
     for q, (t_ms, kd) in enumerate(zip(kms, kinds)):
-        by, fl = chain_kernel_cost(wl.op, nb, int(kd), *io.get(q, (1, 1)))
-        per_kernel.append(dict(launch=q, kind=int(kd), ms=round(float(t_ms), 4), gbs=round(by / (t_ms * 1e-3) / 1e9, 1),
+        by, fl = chain_kernel_cost(wl.op, nb, kd, out_tile, *io[q])
+        per_kernel.append(dict(launch=q, kind=kd, ms=round(float(t_ms), 4), gbs=round(by / (t_ms * 1e-3) / 1e9, 1),
                                tfs=round(fl / (t_ms * 1e-3) / 1e12, 3)))
     # the dominant kernel is the operator pass.  With the tile kernel its two launches per level are two instantiations
-    # (k_tile<..., 1>: the first S pass, which also writes the node-local o term; k_tile<..., 0>: the second S pass with
-    # the output terms and the a_n dot), so each gets its own roofline, the longer one as "roofline"; otherwise every
-    # launch of kinds 0 and 1 is the same kernel and the roofline is over their summed cost and duration
+    # (k_tile<..., 1>: the first S pass, which also writes the node-local o term and, in the fused level, the partials of
+    # a_n; k_tile<..., 0>: the second S pass with the output terms and, in the fused level, the three-term update), so each
+    # gets its own roofline, the longer one as "roofline"; otherwise every launch of kinds 0 and 1 is the same kernel and
+    # the roofline is over their summed cost and duration
     piid = bool(np.all(np.asarray(wl.op.pi) == np.arange(9)[None, :]))
     tile_kernel = out_tile in (8, 16) and piid and wl.op.nslot in (14, 18)
-    fused = 6 in [int(k) for k in kinds]
+    split = 6 in kinds
 
     def launch_roofline(launches, kname):
         tb = tf = tm = 0.0
         for k in launches:
-            by, fl = chain_kernel_cost(wl.op, nb, k["kind"], *io.get(k["launch"], (1, 1)))
+            by, fl = chain_kernel_cost(wl.op, nb, k["kind"], out_tile, *io[k["launch"]])
             tb, tf, tm = tb + by, tf + fl, tm + k["ms"]
         r = roofline(tb / len(launches), tf / len(launches), tm / len(launches))
         r["kernel"] = kname
@@ -211,11 +248,11 @@ def main():
         return r
 
     roof_first = None
-    if tile_kernel and fused:
-        first = [k for k in per_kernel if k["launch"] == 0]
-        second = [k for k in per_kernel if k["launch"] == 2]
-        roof = launch_roofline(second, "k_tile<cplx, %d, %d, 0>" % (out_tile, wl.op.nslot))
-        roof_first = launch_roofline(first, "k_tile<cplx, %d, %d, 1>" % (out_tile, wl.op.nslot))
+    if tile_kernel and split:
+        ops = [k for k in per_kernel if k["kind"] in (0, 7)]
+        first, second = ops[0], ops[-1]
+        roof = launch_roofline([second], "k_tile<cplx, %d, %d, 0>" % (out_tile, wl.op.nslot))
+        roof_first = launch_roofline([first], "k_tile<cplx, %d, %d, 1>" % (out_tile, wl.op.nslot))
     else:
         ops = [k for k in per_kernel if k["kind"] in (0, 1)]
         if tile_kernel:
@@ -224,6 +261,14 @@ def main():
             kname = {65: "k_pair<true, %d>" % wl.op.nslot, 64: "k_node<cplx, true, %d>" % wl.op.nslot}.get(out_tile,
                                                                                                          "k_fused<cplx>")
         roof = launch_roofline(ops, kname)
+    # the whole level against the compulsory traffic of the recursion (read u_k, u_k-1, write u_k+1, the tables once)
+    lev_ms = float(np.mean(dev_ms)) / depth
+    lev_b = compulsory_level_bytes(wl.op, nb)
+    roof_level = dict(bound="hbm", achieved=round(lev_b / (lev_ms * 1e-3) / 1e9, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                      frac=round(lev_b / (lev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), level_ms=round(lev_ms, 4),
+                      compulsory_bytes_per_level=round(lev_b), launches_per_level=len(kinds),
+                      form="fused level (a_n from the first pass, update in the second)" if 7 in kinds else
+                      "separate update pass")
     log("level kernels:", per_kernel)
 
     out = {
@@ -245,6 +290,7 @@ def main():
                    "node_order": wl.op.meta.get("node_order", "bfs")},
         "roofline": roof,
         "roofline_first_pass": roof_first,
+        "roofline_level": roof_level,
         "level_kernels": per_kernel,
     }
 

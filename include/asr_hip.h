@@ -72,6 +72,14 @@ int asr_gpu_chains_edc_proj(asr_gpu_op* op, int nchain, const int* start_orb, co
  * level projects the new vector twice against all stored Lanczos vectors of its chain, w = w - Q (Q^T w), before b_n is
  * taken.  The stored vectors cost depth x (nn + 1) x 9 x members doubles (at most 64 GiB): for tests and small cases. */
 int asr_gpu_op_set_reorth(asr_gpu_op* op, int reorth);
+/* Form of one level of the complex-symmetric edc chains (edc_green, src/asr_edc.f90:95-155, whose loop applies the
+ * operator, takes a_n = sum(u w), then updates and projects w): form 0 (the default) takes a_n from the first of the
+ * level's two operator passes, a_n = sum_i sq_i^2 [u~ (E + X(z)) u~ + u~ h u~ - (h u~) o (h u~)]_i, equal to sum(u w)
+ * on the symmetric subspace, and folds the three-term update, the projection and the norm into the second pass; form 1
+ * runs the update as a pass of its own after the operator, in the reference's order.  The coefficients of the two forms
+ * agree to rounding.  Form 0 applies to chains on the member-tile operator pass (member tile 8 or 16, an operator whose
+ * group keeps every orbital, one d for every species); other chains take form 1. */
+int asr_gpu_op_set_level_form(asr_gpu_op* op, int form);
 /* The tile the autotuner picked for nchain members of the given kind (0: none tuned yet). */
 int asr_gpu_op_get_tile(asr_gpu_op* op, int nchain, int is_complex, int* mt);
 /* Release the level-vector blocks the operator keeps cached between batched runs (the batches of a contour pass reuse
