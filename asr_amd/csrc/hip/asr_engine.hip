@@ -2509,6 +2509,7 @@ struct KTimer {
 };
 KTimer* g_timer = nullptr;
 std::vector<double> g_acc;  // summed duration per kernel ordinal of a level
+std::vector<int> g_cnt;     // levels in which that ordinal ran (the rescale runs every 16 levels only)
 int g_levels = 0;
 
 enum { KIND_SPMM = 0, KIND_RAND = 1, KIND_FIN_A = 2, KIND_UPDATE = 3, KIND_FIN_B = 4, KIND_SCALE = 5, KIND_FIX = 6,
@@ -3312,11 +3313,15 @@ struct ChainRun {
     HCHECK(hipGetLastError());
     if (g_timer) {
       HCHECK(hipStreamSynchronize(stream));
-      if (g_acc.size() < (size_t)g_timer->ord) g_acc.resize(g_timer->ord, 0.0);
+      if (g_acc.size() < (size_t)g_timer->ord) {
+        g_acc.resize(g_timer->ord, 0.0);
+        g_cnt.resize(g_timer->ord, 0);
+      }
       for (int q = 0; q < g_timer->ord; ++q) {
         float t = 0.f;
         HCHECK(hipEventElapsedTime(&t, g_timer->ev[2 * q], g_timer->ev[2 * q + 1]));
         g_acc[q] += t;
+        g_cnt[q] += 1;
       }
       g_levels++;
     }
@@ -3444,6 +3449,7 @@ int run_chains(asr_gpu_op* op, int nchain, const int* start_orb, const double* p
   for (int pass = 0; pass < 2 && st == RERUN_ASYM; ++pass) {
     if (pass && g_timer) {  // the per-kernel timings start over with the separate update pass
       g_acc.clear();
+      g_cnt.clear();
       g_levels = 0;
       g_timer->kind.clear();
     }
@@ -4444,6 +4450,7 @@ int asr_gpu_profile_chains(asr_gpu_op* op, int nchain, const int* start_orb, con
   KTimer T;
   g_timer = &T;
   g_acc.clear();
+  g_cnt.clear();
   g_levels = 0;
   int maxd = 0;
   for (int c = 0; c < nchain; ++c) maxd = std::max(maxd, nlev[c]);
@@ -4463,7 +4470,7 @@ int asr_gpu_profile_chains(asr_gpu_op* op, int nchain, const int* start_orb, con
   int n = std::min((int)g_acc.size(), 16);
   *nk = n;
   for (int q = 0; q < n; ++q) {
-    kernel_ms[q] = g_levels ? g_acc[q] / g_levels : 0.0;
+    kernel_ms[q] = g_cnt[q] ? g_acc[q] / g_cnt[q] : 0.0;
     kind[q] = T.kind[q];
   }
   return 0;

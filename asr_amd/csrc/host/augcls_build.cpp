@@ -1,7 +1,7 @@
 // Node-set builder of the site-class, point-group reduced augmented-space operator.
 //
 // Follows build_augcls_operator (src/asr_augcls.f90:596-823) with canon_node (:494-526), set_state / get_state (:208-235)
-// and the hashed state store (src/asr_augspace.f90:48-170):
+// (src/asr_augspace.f90:48-170 keeps the nodes of the closure in a hashed store; NodeStore below is this builder's own):
 //   * breadth-first closure over paths R -> R1 -> R2 with every local-state combination of the visited sites, rank-limited;
 //   * every candidate is canonicalised over the group (minimum over the operations of (site, sorted config codes));
 //   * candidates are inserted serially in the order (node, k1, k2, f) of the plain serial loop, so node numbering does not depend
@@ -24,68 +24,78 @@
 
 namespace {
 
-struct Store {
-  std::vector<int> site, start, len, level, orb, data;
-  std::vector<int> table;  // 0 = empty, else node index + 1
-  uint32_t tmask = 0;
+// The node store: every node's key (its site and sorted config codes) in one arena, looked up through an open-addressed
+// index.  Node numbers are the insertion order; the index only finds keys, so the hash function does not affect the node
+// numbering.  The index keeps the 64-bit hash of each entry beside it (most probes end on a hash mismatch without
+// touching the arena), holds at most half as many entries as slots, and doubles when it reaches that.
+struct NodeStore {
+  std::vector<int> site, start, len, level, orb, data;  // per node; data: the config codes of every node, back to back
+  std::vector<uint64_t> khash;                          // per node: hash of its key
+  std::vector<int32_t> slot;                            // index: node number or -1
+  std::vector<uint64_t> slot_hash;                      // index: hash of the entry in that slot
+  size_t mask = 0;
   int n = 0;
 
+  static uint64_t mix(uint64_t h) {  // a 64-bit finaliser (xor-shift and multiply rounds)
+    h ^= h >> 31;
+    h *= 0x7fb5d329728ea185ull;
+    h ^= h >> 27;
+    h *= 0x81dadef4bc2dd44dull;
+    return h ^ (h >> 33);
+  }
+  static uint64_t key_hash(int s, const int* cfg, int nc) {
+    uint64_t h = mix(0x243f6a8885a308d3ull ^ (uint64_t)(uint32_t)s);
+    for (int i = 0; i < nc; ++i) h = mix(h + 0x9e3779b97f4a7c15ull * (uint64_t)((uint32_t)cfg[i] + 1u));
+    return h;
+  }
   void init(int cap) {
-    site.reserve(cap); start.reserve(cap); len.reserve(cap); level.reserve(cap); orb.reserve(cap); data.reserve(8 * cap);
-    uint32_t tsz = 1024;
-    while (tsz < 4u * (uint32_t)cap) tsz *= 2;
-    table.assign(tsz, 0);
-    tmask = tsz - 1;
+    site.reserve(cap); start.reserve(cap); len.reserve(cap); level.reserve(cap); orb.reserve(cap);
+    khash.reserve(cap); data.reserve(8 * (size_t)cap);
+    size_t sz = 64;
+    while (sz < 2 * (size_t)cap) sz <<= 1;
+    slot.assign(sz, -1);
+    slot_hash.assign(sz, 0);
+    mask = sz - 1;
   }
-  static uint32_t hash(int s, const int* cfg, int nc) {
-    uint64_t h = 1469598103ull + (uint64_t)(uint32_t)s * 1099511ull;
-    for (int i = 0; i < nc; ++i) {
-      h ^= (uint64_t)(uint32_t)cfg[i] + 1234567ull;
-      h = (h * 16777619ull) & 0x7fffffffull;
-    }
-    return (uint32_t)h;
-  }
-  bool same(int j, int s, const int* cfg, int nc) const {
+  bool key_is(int j, int s, const int* cfg, int nc) const {
     if (site[j] != s || len[j] != nc) return false;
-    for (int k = 0; k < nc; ++k)
-      if (data[start[j] + k] != cfg[k]) return false;
-    return true;
+    return std::equal(cfg, cfg + nc, data.begin() + start[j]);
   }
-  int find(int s, const int* cfg, int nc) const {
-    uint32_t h = hash(s, cfg, nc) & tmask;
-    for (;;) {
-      int j = table[h];
-      if (j == 0) return -1;
-      if (same(j - 1, s, cfg, nc)) return j - 1;
-      h = (h + 1) & tmask;
-    }
+  // the slot holding the key, or the empty slot where it would go
+  size_t probe(uint64_t h, int s, const int* cfg, int nc) const {
+    size_t q = (size_t)h & mask;
+    while (slot[q] >= 0 && !(slot_hash[q] == h && key_is(slot[q], s, cfg, nc))) q = (q + 1) & mask;
+    return q;
   }
-  void rehash() {
-    uint32_t tsz = 2 * (tmask + 1);
-    table.assign(tsz, 0);
-    tmask = tsz - 1;
-    for (int i = 0; i < n; ++i) {
-      uint32_t h = hash(site[i], &data[start[i]], len[i]) & tmask;
-      while (table[h] != 0) h = (h + 1) & tmask;
-      table[h] = i + 1;
+  int find(int s, const int* cfg, int nc) const { return slot[probe(key_hash(s, cfg, nc), s, cfg, nc)]; }
+  void grow() {
+    const size_t sz = 2 * (mask + 1);
+    slot.assign(sz, -1);
+    slot_hash.assign(sz, 0);
+    mask = sz - 1;
+    for (int j = 0; j < n; ++j) {
+      size_t q = (size_t)khash[j] & mask;
+      while (slot[q] >= 0) q = (q + 1) & mask;
+      slot[q] = j;
+      slot_hash[q] = khash[j];
     }
   }
   int find_or_add(int s, const int* cfg, int nc, int lev, bool* isnew) {
-    uint32_t h = hash(s, cfg, nc) & tmask;
-    for (;;) {
-      int j = table[h];
-      if (j == 0) break;
-      if (same(j - 1, s, cfg, nc)) { *isnew = false; return j - 1; }
-      h = (h + 1) & tmask;
+    const uint64_t h = key_hash(s, cfg, nc);
+    const size_t q = probe(h, s, cfg, nc);
+    if (slot[q] >= 0) {
+      *isnew = false;
+      return slot[q];
     }
-    site.push_back(s); len.push_back(nc); level.push_back(lev); orb.push_back(1);
+    const int j = n++;
+    site.push_back(s); len.push_back(nc); level.push_back(lev); orb.push_back(1); khash.push_back(h);
     start.push_back((int)data.size());
     data.insert(data.end(), cfg, cfg + nc);
-    table[h] = n + 1;
-    int idx = n++;
+    slot[q] = j;
+    slot_hash[q] = h;
     *isnew = true;
-    if (4u * (uint32_t)n > tmask) rehash();
-    return idx;
+    if (2 * (size_t)n > mask) grow();
+    return j;
   }
 };
 
@@ -163,7 +173,7 @@ struct asr_augcls_tables {
   int nn = 0, nslot = 0, nop = 0, nsm = 0, nstab = 0;
   std::vector<int> hopr, hopg, flipr, flipg, lst, site, level, orb, stab_ptr, stab;
   // the node store and the group, kept for the target lookups (tnode / tgop)
-  Store st;
+  NodeStore st;
   int nsite = 0, start = 0;
   std::vector<int> perm, nst;
 };
@@ -176,14 +186,14 @@ extern "C" int asr_host_build_augcls(int nsite, const int* nbr_ptr, const int* n
     return 1;
   };
   *out = nullptr;
-  if (nsite <= 0 || nop <= 0) return fail("build_augcls: empty graph or group");
+  if (nsite <= 0 || nop <= 0) return fail("node-set builder: the site graph or the group is empty");
   for (int i = 0; i < nsite; ++i)
-    if (perm[i] != i) return fail("build_augcls_operator: operation 1 must be the identity");
+    if (perm[i] != i) return fail("node-set builder: the first group operation is not the identity permutation");
   for (int g = 0; g < nop; ++g) {
     const int* p = perm + (size_t)g * nsite;
-    if (p[start] != start) return fail("build_augcls_operator: the group does not fix the start site");
+    if (p[start] != start) return fail("node-set builder: a group operation moves the start site");
     for (int i = 0; i < nsite; ++i)
-      if (cls[p[i]] != cls[i]) return fail("build_augcls_operator: an operation does not preserve the site classes");
+      if (cls[p[i]] != cls[i]) return fail("node-set builder: a group operation maps a site onto one of another class");
   }
   int nsm = 0;
   for (int c = 0; c < ncls; ++c) nsm = std::max(nsm, nsc[c]);
@@ -194,13 +204,13 @@ extern "C" int asr_host_build_augcls(int nsite, const int* nbr_ptr, const int* n
   for (int r = 0; r < nsite; ++r) nslot = std::max(nslot, nbr_ptr[r + 1] - nbr_ptr[r]);
 
   if (nthreads > 0) omp_set_num_threads(nthreads);
-  Store st;
+  NodeStore st;
   st.init(4096);
   {
     Canon cn;
     std::vector<int> c2;
     canon_node(perm, nsite, nop, nsm, start, nullptr, 0, cn, c2);
-    if (cn.osize != 1) return fail("build_augcls_operator: start site is not invariant under the group");
+    if (cn.osize != 1) return fail("node-set builder: the start node has an orbit of more than one node under the group");
     bool isnew;
     int idx = st.find_or_add(cn.csite, cn.ccfg.data(), 0, 0, &isnew);
     st.orb[idx] = 1;
@@ -262,7 +272,7 @@ extern "C" int asr_host_build_augcls(int nsite, const int* nbr_ptr, const int* n
         }
       }
     }
-    if (edge) return fail("build_augcls_operator: the closure reaches the edge of the site cluster");
+    if (edge) return fail("node-set builder: a path of the closure leaves the site graph (a neighbour index is missing)");
     for (int ii = i; ii <= i2; ++ii) {  // serial insertion, in the order of the serial loop
       int lev = st.level[ii];
       const std::vector<int>& rec = cb[ii - i];
@@ -311,7 +321,7 @@ extern "C" int asr_host_build_augcls(int nsite, const int* nbr_ptr, const int* n
       if (cnt * st.orb[n] != nop) bad_stab = true;
     }
   }
-  if (bad_stab) { delete T; return fail("build_augcls_operator: stabiliser size /= group order / orbit size"); }
+  if (bad_stab) { delete T; return fail("node-set builder: a stabiliser does not have group order / orbit size operations"); }
   for (int n = 0; n < nn; ++n) T->stab_ptr[n + 1] = T->stab_ptr[n] + nstb[n];
   T->nstab = T->stab_ptr[nn];
   T->stab.assign(T->nstab, 0);

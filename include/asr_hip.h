@@ -172,11 +172,12 @@ int asr_gpu_lanczos_adjoint(asr_gpu_op* op, int nchain, const int* start_orb, co
                             const double* conc, int nlev, int ckpt, const double* dl_da, const double* dl_db, double* a,
                             double* b, double* grad, double* fwd_ms, double* adj_ms);
 
-/* Per-kernel durations of the chain level step (HIP events around every launch, averaged over the levels): the chains of
- * asr_gpu_chains_edc (pab != NULL) or asr_gpu_lanczos_real (pab == NULL).  On return *nk (<= 16) launches per level,
- * kernel_ms[q] their mean duration and kind[q] their kind: 0 fused SpMM (S pass + flip terms), 1 flip-gather terms only,
- * 2 a_n reduction, 3 Lanczos update + projection, 4 b_n reduction, 5 scaling.  Measurement support; no reference
- * counterpart. */
+/* Per-kernel durations of the chain level step (HIP events around every launch, each averaged over the levels in which
+ * it ran): the chains of asr_gpu_chains_edc (pab != NULL) or asr_gpu_lanczos_real (pab == NULL).  On return *nk (<= 16)
+ * launches per level, kernel_ms[q] their mean duration and kind[q] their kind: 0 operator pass (S pass + flip terms),
+ * 1 flip-gather terms only, 2 a_n reduction, 3 Lanczos update + projection, 4 b_n reduction, 5 scaling (every 16 levels
+ * for the complex chains), 6 the terms of the nodes with flip partners, 7 operator pass with the three-term update (the
+ * fused edc level).  Measurement support; no reference counterpart. */
 int asr_gpu_profile_chains(asr_gpu_op* op, int nchain, const int* start_orb, const double* params, const double* pab,
                            const int* frozen_sp, const double* conc, const double* z, const int* nlev, int project, int* nk,
                            double* kernel_ms, int* kind);
