@@ -23,6 +23,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <type_traits>
@@ -30,6 +31,12 @@
 
 #include "asr_hip.h"
 #include "asr_host.h"
+
+// Device scratch of asr_gpu_kpm_moments_noalloc (64 MiB per device, held from the load of the library to the end of the
+// process): the entry point that allocates nothing carves its work arrays from it.  In host code `scratch` names a
+// host-side stand-in, not device memory: the device address comes from hipGetSymbolAddress, per device (scratch_ptr).
+#define ASR_NOALLOC_SCRATCH_DOUBLES ((size_t)8 << 20)
+static __device__ __align__(256) double scratch[ASR_NOALLOC_SCRATCH_DOUBLES];
 
 namespace {
 
@@ -45,6 +52,28 @@ int fail(const std::string& msg) {
     hipError_t e_ = (call);                                                                            \
     if (e_ != hipSuccess) return fail(std::string(#call) + ": " + hipGetErrorString(e_));              \
   } while (0)
+
+// The device address of `scratch` on a device (looked up once per device, with that device current; every call on the
+// device shares the array, so calls that overlap on one GPU race).
+static std::mutex g_scratch_mu;
+static double* g_scratch_addr[64] = {};
+static int scratch_ptr(int device, double** out) {
+  if (device < 0 || device >= 64) return fail("device scratch: device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_addr[device]) {
+    int cur = 0;
+    HCHECK(hipGetDevice(&cur));
+    if (cur != device) HCHECK(hipSetDevice(device));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) HCHECK(hipSetDevice(cur));
+    if (e != hipSuccess || !p) return fail(std::string("device scratch: hipGetSymbolAddress: ") + hipGetErrorString(e));
+    g_scratch_addr[device] = (double*)p;
+  }
+  *out = g_scratch_addr[device];
+  return 0;
+}
+
 
 // parameter slots: c, d, o, e; then pairs (a, b) of energy-linear diagonals X(z) = z X(a) + X(b): pa/pb (route edc's X(z);
 // route ed3's z - et), ua/ub (ed3's u), qa/qb (ed3's Qt)
@@ -2174,6 +2203,15 @@ __global__ void __launch_bounds__(BLOCK) k_kpm_fin(const double* partial, int nb
   if (threadIdx.x == 0) mu_per[(size_t)v * nmom + n] = s * inv_dim;
 }
 
+// mu[n] = sum_v mu_per[v][n] / nvec, the vectors in order (the host's sum of asr_gpu_kpm_moments, on the device)
+__global__ void k_kpm_mean(const double* __restrict__ mu_per, int nvec, int nmom, double* __restrict__ mu) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nmom) return;
+  double acc = 0.0;
+  for (int v = 0; v < nvec; ++v) acc += mu_per[(size_t)v * nmom + n];
+  mu[n] = acc / (double)nvec;
+}
+
 // moment doubling (T_2m = 2 T_m^2 - T_0, T_2m-1 = 2 T_m T_m-1 - T_1): from the partials of <t_m|t_m> and <t_m|t_m-1>,
 // mu_2m = 2 <t_m|t_m> / dim - mu_0 and mu_2m-1 = 2 <t_m|t_m-1> / dim - mu_1 (partial2 null: only mu_2m)
 __global__ void __launch_bounds__(BLOCK) k_kpm_fin2(const double* partial, const double* partial2, int nbx, int vt, int nvec,
@@ -2266,6 +2304,7 @@ struct asr_gpu_op {
 
 struct asr_gpu_real {
   int nsite, nslot, nsp, vt = 0;
+  int device = 0;    // the device current at asr_gpu_real_create
   bool v2 = true;    // Chebyshev steps with two vectors per lane
   bool s16 = false;  // ... on 16-vector slices (8 sites per wave)
   bool dbl = false;  // kpm_moments by moment doubling (two moments per Chebyshev step)
@@ -4498,6 +4537,9 @@ int asr_gpu_real_create(int nsite, int nslot, const int* nbr, int nsp, const uns
     return 1;
   }
   HCHECK(hipStreamCreateWithFlags(&R->stream, hipStreamNonBlocking));
+  HCHECK(hipGetDevice(&R->device));
+  double* sp = nullptr;  // the device address of the no-alloc entry's scratch, looked up at set-up
+  if (scratch_ptr(R->device, &sp)) return 1;
   HCHECK(hipMemcpy(R->nbr, nbr, (size_t)nsite * nslot * 4, hipMemcpyHostToDevice));
   HCHECK(hipMemcpy(R->spec, species, nsite, hipMemcpyHostToDevice));
   HCHECK(hipMemcpy(R->par, par, (size_t)4 * nsp * 9 * 8, hipMemcpyHostToDevice));
@@ -4579,30 +4621,49 @@ int asr_gpu_real_apply(asr_gpu_real* R, int nvec, const double* x, double* y) {
   return 0;
 }
 
-// mu_n = (1/R) sum_r <r|T_n((H - c)/s)|r> / dim (kpm_moments, src/reference/asr_kpm.f90:39-65) for nvec Rademacher vectors
-// in one batched SpMM per moment.  mu_per (optional): nvec x nmom per-vector terms.  step_ms: mean duration of the
-// Chebyshev step kernel alone (HIP events around each launch on the engine's stream); total_ms: the whole recurrence.
-int asr_gpu_kpm_moments(asr_gpu_real* R, double c, double s, int nmom, int nvec, uint64_t seed, double* mu, double* mu_per,
-                        double* step_ms, double* total_ms) {
-  if (nmom < 1 || nvec < 1) return fail("kpm_moments: nmom and nvec must be positive");
-  if (seed >= (1ull << 24) || nvec > 4096) return fail("kpm_moments: seed < 2^24 and nvec <= 4096");
+// The device buffers of one KPM recurrence: three Chebyshev vectors (dim x nvp), the per-block dot partials (and those of
+// moment doubling), the per-vector moments (nvec x nmom).
+struct KpmBufs {
+  double *t0 = nullptr, *t1 = nullptr, *t2 = nullptr, *part = nullptr, *part2 = nullptr, *mper = nullptr;
+};
+struct KpmSizes {
+  size_t vec, part, part2, mper;
+  size_t total() const { return 3 * vec + part + part2 + mper; }
+};
+static KpmSizes kpm_sizes(const asr_gpu_real* R, int nmom, int nvec) {
   const size_t dim = (size_t)R->nsite * 9;
-  if (dim >= (1ull << 28)) return fail("kpm_moments: more than 2^28 rows");
   const int vt = 64;
   const int nvp = ((nvec + vt - 1) / vt) * vt;
   const dim3 grid = kpm_grid(R, vt, nvp);
-  double *t0, *t1, *t2, *part, *mper, *part2 = nullptr;
-  if (dalloc(&t0, dim * nvp) || dalloc(&t1, dim * nvp) || dalloc(&t2, dim * nvp) ||
-      dalloc(&part, (size_t)grid.x * grid.y * vt) || dalloc(&mper, (size_t)nvec * nmom))
-    return 1;
-  if (R->dbl && dalloc(&part2, (size_t)((R->nsite + 7) / 8) * grid.y * 64)) return 1;
+  KpmSizes z;
+  z.vec = dim * nvp;
+  z.part = (size_t)grid.x * grid.y * vt;
+  z.part2 = R->dbl ? (size_t)((R->nsite + 7) / 8) * grid.y * 64 : 0;
+  z.mper = (size_t)nvec * nmom;
+  return z;
+}
+static int kpm_check(int nmom, int nvec, uint64_t seed, const asr_gpu_real* R) {
+  if (nmom < 1 || nvec < 1) return fail("kpm_moments: nmom and nvec must be positive");
+  if (seed >= (1ull << 24) || nvec > 4096) return fail("kpm_moments: seed < 2^24 and nvec <= 4096");
+  if ((size_t)R->nsite * 9 >= (1ull << 28)) return fail("kpm_moments: more than 2^28 rows");
+  return 0;
+}
+
+// The Chebyshev recurrence of kpm_moments on the handle's stream into the given buffers: mper[v][n] = <r_v|t_n> / dim.
+// ev (2 + 2 x steps events, or null): the whole recurrence and each timed Chebyshev step.
+static int kpm_run(asr_gpu_real* R, double c, double s, int nmom, int nvec, uint64_t seed, KpmBufs B, hipEvent_t* ev) {
+  double *t0 = B.t0, *t1 = B.t1, *t2 = B.t2, *part = B.part, *part2 = B.part2, *mper = B.mper;
+  const size_t dim = (size_t)R->nsite * 9;
+  const int vt = 64;
+  const int nvp = ((nvec + vt - 1) / vt) * vt;
+  const dim3 grid = kpm_grid(R, vt, nvp);
   const double inv_dim = 1.0 / (double)dim;
-  // Chebyshev steps timed: t_2 .. t_(nmom-1), or t_2 .. t_(nmom/2) with the doubling
-  const int nstep_ev = R->dbl ? std::max(nmom / 2 - 1, 0) : (nmom > 2 ? nmom - 2 : 0);
-  std::vector<hipEvent_t> ev(2 + 2 * nstep_ev);
-  for (auto& e : ev) HCHECK(hipEventCreate(&e));
   const RealDev rd = real_dev(R);
-  HCHECK(hipEventRecord(ev[0], R->stream));
+  auto mark = [&](int q) -> int {
+    if (ev) HCHECK(hipEventRecord(ev[q], R->stream));
+    return 0;
+  };
+  if (mark(0)) return 1;
   // t0 = r (stored as d o r), mu_0 = <r|r>/dim: the step with alpha = 0, beta = -1
   hipLaunchKernelGGL((k_kpm_u<true, 0>), grid, dim3(BLOCK), 0, R->stream, rd, (const double*)R->blk, nvec, nvp,
                      (const double*)nullptr, (const double*)nullptr, t0, 0.0, 0.0, -1.0, seed, part, 1);
@@ -4624,14 +4685,14 @@ int asr_gpu_kpm_moments(asr_gpu_real* R, double c, double s, int nmom, int nvec,
     }
     const int M = nmom / 2;  // the largest m with 2 m - 1 <= nmom - 1
     for (int m = 2; m <= M; ++m) {
-      HCHECK(hipEventRecord(ev[2 + 2 * (m - 2)], R->stream));
+      if (mark(2 + 2 * (m - 2))) return 1;
       if (R->nslot == 18)
         hipLaunchKernelGGL((k_kpm_v2<18, true>), g2, dim3(BLOCK), 0, R->stream, rd, (const double*)R->blk, nvec, nvp,
                            (const double*)t1, (const double*)t0, t2, c, 2.0 / s, 1.0, seed, part, part2);
       else
         hipLaunchKernelGGL((k_kpm_v2<0, true>), g2, dim3(BLOCK), 0, R->stream, rd, (const double*)R->blk, nvec, nvp,
                            (const double*)t1, (const double*)t0, t2, c, 2.0 / s, 1.0, seed, part, part2);
-      HCHECK(hipEventRecord(ev[3 + 2 * (m - 2)], R->stream));
+      if (mark(3 + 2 * (m - 2))) return 1;
       hipLaunchKernelGGL(k_kpm_fin2, dim3(nvec), dim3(BLOCK), 0, R->stream, (const double*)part, (const double*)part2, nbx2,
                          vt, nvec, inv_dim, mper, nmom, m);
       double* tmp = t0; t0 = t1; t1 = t2; t2 = tmp;
@@ -4639,7 +4700,7 @@ int asr_gpu_kpm_moments(asr_gpu_real* R, double c, double s, int nmom, int nvec,
   }
   for (int n = 2; n < nmom && !R->dbl; ++n) {
     // t2 = 2 (H t1 - c t1) / s - t0
-    HCHECK(hipEventRecord(ev[2 + 2 * (n - 2)], R->stream));
+    if (mark(2 + 2 * (n - 2))) return 1;
     if (R->s16 && R->nslot == 18)
       hipLaunchKernelGGL((k_kpm_s16<18>), dim3((R->nsite + 31) / 32, nvp / 16), dim3(BLOCK), (size_t)19 * 90 * 8, R->stream, rd,
                          nvec, nvp, (const double*)t1, (const double*)t0, t2, c, 2.0 / s, 1.0, seed, part);
@@ -4657,13 +4718,33 @@ int asr_gpu_kpm_moments(asr_gpu_real* R, double c, double s, int nmom, int nvec,
     else
       hipLaunchKernelGGL((k_kpm_u<false, 0>), grid, dim3(BLOCK), 0, R->stream, rd, (const double*)R->blk, nvec, nvp,
                          (const double*)t1, (const double*)t0, t2, c, 2.0 / s, 1.0, seed, part, 1);
-    HCHECK(hipEventRecord(ev[3 + 2 * (n - 2)], R->stream));
+    if (mark(3 + 2 * (n - 2))) return 1;
     const int nbx = (R->s16 && R->nslot == 18) ? (int)((R->nsite + 31) / 32) : R->v2 ? (int)((R->nsite + 7) / 8) : (int)grid.x;
     hipLaunchKernelGGL(k_kpm_fin, dim3(nvec), dim3(BLOCK), 0, R->stream, part, nbx, vt, nvec, inv_dim, mper, nmom, n);
     double* tmp = t0; t0 = t1; t1 = t2; t2 = tmp;
   }
-  HCHECK(hipEventRecord(ev[1], R->stream));
+  if (mark(1)) return 1;
   HCHECK(hipGetLastError());
+  return 0;
+}
+
+// mu_n = (1/R) sum_r <r|T_n((H - c)/s)|r> / dim (kpm_moments, src/reference/asr_kpm.f90:39-65) for nvec Rademacher vectors
+// in one batched SpMM per moment.  mu_per (optional): nvec x nmom per-vector terms.  step_ms: mean duration of the
+// Chebyshev step kernel alone (HIP events around each launch on the engine's stream); total_ms: the whole recurrence.
+int asr_gpu_kpm_moments(asr_gpu_real* R, double c, double s, int nmom, int nvec, uint64_t seed, double* mu, double* mu_per,
+                        double* step_ms, double* total_ms) {
+  if (kpm_check(nmom, nvec, seed, R)) return 1;
+  const KpmSizes z = kpm_sizes(R, nmom, nvec);
+  KpmBufs B;
+  if (dalloc(&B.t0, z.vec) || dalloc(&B.t1, z.vec) || dalloc(&B.t2, z.vec) || dalloc(&B.part, z.part) ||
+      dalloc(&B.mper, z.mper))
+    return 1;
+  if (R->dbl && dalloc(&B.part2, z.part2)) return 1;
+  // Chebyshev steps timed: t_2 .. t_(nmom-1), or t_2 .. t_(nmom/2) with the doubling
+  const int nstep_ev = R->dbl ? std::max(nmom / 2 - 1, 0) : (nmom > 2 ? nmom - 2 : 0);
+  std::vector<hipEvent_t> ev(2 + 2 * nstep_ev);
+  for (auto& e : ev) HCHECK(hipEventCreate(&e));
+  if (kpm_run(R, c, s, nmom, nvec, seed, B, ev.data())) return 1;
   HCHECK(hipEventSynchronize(ev[1]));
   float ms_all = 0.f;
   HCHECK(hipEventElapsedTime(&ms_all, ev[0], ev[1]));
@@ -4674,7 +4755,7 @@ int asr_gpu_kpm_moments(asr_gpu_real* R, double c, double s, int nmom, int nvec,
     ms_steps += t;
   }
   std::vector<double> per((size_t)nvec * nmom);
-  HCHECK(hipMemcpy(per.data(), mper, per.size() * 8, hipMemcpyDeviceToHost));
+  HCHECK(hipMemcpy(per.data(), B.mper, per.size() * 8, hipMemcpyDeviceToHost));
   for (int n = 0; n < nmom; ++n) {
     double acc = 0.0;
     for (int v = 0; v < nvec; ++v) acc += per[(size_t)v * nmom + n];
@@ -4683,9 +4764,33 @@ int asr_gpu_kpm_moments(asr_gpu_real* R, double c, double s, int nmom, int nvec,
   if (mu_per) std::memcpy(mu_per, per.data(), per.size() * 8);
   if (step_ms) *step_ms = nstep_ev ? ms_steps / nstep_ev : 0.0;
   if (total_ms) *total_ms = ms_all;
-  for (double* p : {t0, t1, t2, part, mper}) (void)hipFree(p);
-  if (part2) (void)hipFree(part2);
+  for (double* p : {B.t0, B.t1, B.t2, B.part, B.mper}) (void)hipFree(p);
+  if (B.part2) (void)hipFree(B.part2);
   for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+  return 0;
+}
+
+int asr_gpu_kpm_moments_noalloc(asr_gpu_real* R, double c, double s, int nmom, int nvec, uint64_t seed, double* mu_dev) {
+  if (!R || !mu_dev) return fail("asr_gpu_kpm_moments_noalloc: null handle or output");
+  if (kpm_check(nmom, nvec, seed, R)) return 1;
+  const KpmSizes z = kpm_sizes(R, nmom, nvec);
+  if (z.total() > ASR_NOALLOC_SCRATCH_DOUBLES)
+    return fail("asr_gpu_kpm_moments_noalloc: the work arrays (" + std::to_string(z.total() * 8) +
+                " bytes: three Chebyshev vectors of 9 x sites x vectors rounded up to 64, partials, moments) exceed the "
+                "fixed device scratch of " + std::to_string((size_t)ASR_NOALLOC_SCRATCH_DOUBLES * 8) + " bytes");
+  double* base = nullptr;
+  if (scratch_ptr(R->device, &base)) return 1;
+  KpmBufs B;
+  B.t0 = base;
+  B.t1 = B.t0 + z.vec;
+  B.t2 = B.t1 + z.vec;
+  B.part = B.t2 + z.vec;
+  B.part2 = R->dbl ? B.part + z.part : nullptr;
+  B.mper = B.part + z.part + z.part2;
+  if (kpm_run(R, c, s, nmom, nvec, seed, B, nullptr)) return 1;
+  hipLaunchKernelGGL(k_kpm_mean, dim3((nmom + 255) / 256), dim3(256), 0, R->stream, (const double*)B.mper, nvec, nmom, mu_dev);
+  HCHECK(hipGetLastError());
+  HCHECK(hipStreamSynchronize(R->stream));
   return 0;
 }
 

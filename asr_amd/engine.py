@@ -65,6 +65,7 @@ def lib() -> C.CDLL:
         L.asr_gpu_real_apply.argtypes = [P, C.c_int, P, P]
         L.asr_gpu_spectral_bounds.argtypes = [P, C.c_int, C.c_int, C.c_uint64, C.c_double, P, P, P, P]
         L.asr_gpu_kpm_moments.argtypes = [P, C.c_double, C.c_double, C.c_int, C.c_int, C.c_uint64, P, P, P, P]
+        L.asr_gpu_kpm_moments_noalloc.argtypes = [P, C.c_double, C.c_double, C.c_int, C.c_int, C.c_uint64, P]
         _LIB = L
     return _LIB
 
@@ -422,6 +423,12 @@ class GpuRealization:
         _check(lib().asr_gpu_spectral_bounds(self.h, int(nstep), int(nvec), int(seed), float(margin), _p(lo), _p(hi), _p(a),
                                              _p(b)), "asr_gpu_spectral_bounds")
         return float(lo[0]), float(hi[0]), a, b
+
+    def kpm_moments_noalloc(self, c: float, s: float, nmom: int, nvec: int, seed: int, mu_dev: int):
+        """The moments into nmom doubles of device memory at address mu_dev (asr_gpu_kpm_moments_noalloc: the work arrays
+        in the library's fixed device scratch, no device allocation)."""
+        _check(lib().asr_gpu_kpm_moments_noalloc(self.h, float(c), float(s), int(nmom), int(nvec), int(seed),
+                                                 C.c_void_p(int(mu_dev))), "asr_gpu_kpm_moments_noalloc")
 
     def kpm_moments(self, c: float, s: float, nmom: int, nvec: int, seed: int, per_vector: bool = False):
         mu = np.zeros(nmom)

@@ -202,6 +202,14 @@ int asr_gpu_real_apply(asr_gpu_real* r, int nvec, const double* x, double* y);
  * mu: nmom; mu_per: nvec x nmom or NULL; step_ms: mean duration of one Chebyshev step; total_ms: whole recurrence. */
 int asr_gpu_kpm_moments(asr_gpu_real* r, double c, double s, int nmom, int nvec, uint64_t seed, double* mu, double* mu_per,
                         double* step_ms, double* total_ms);
+/* asr_gpu_kpm_moments for callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): no counterpart in the reference.  The same recurrence, vectors and moments (mu
+ * bitwise equal to asr_gpu_kpm_moments'), with its work arrays (three Chebyshev vectors of 9 nsite x nvec rounded up to
+ * 64, the block partials and the per-vector moments) in a fixed device scratch of 64 MiB that the library holds on each
+ * device until the process ends; a size beyond it returns an error.  mu_dev: nmom doubles of device memory (output).  No
+ * timing, no per-vector output; the call returns when the moments are in mu_dev.  Every call on one GPU shares the
+ * scratch: calls that overlap on one GPU (several host threads, several handles) race. */
+int asr_gpu_kpm_moments_noalloc(asr_gpu_real* r, double c, double s, int nmom, int nvec, uint64_t seed, double* mu_dev);
 /* Extremal eigenvalues of h = c + d S d for the KPM scaling (spectral_bounds, src/reference/asr_kpm.f90:14-36): nstep
  * Lanczos steps from nvec (<= 64) random start vectors at once (vector v: uniform [-1/2, 1/2) entries from
  * splitmix64(((seed + v) << 40) ^ row)), the eigenvalues of each chain's tridiagonal matrix (the role of dsterf);
