@@ -2,6 +2,7 @@
 
     libasr_host.so   C++ host side (node-set builder, terminators), g++ -O3 -fopenmp
     libasr_hip.so    HIP kernels for gfx950 and the chain-engine C ABI, hipcc --offload-arch=gfx950
+    fortran/asr_gpu_demo  the Fortran binding (module asr_gpu) and a short caller of the C ABI, amdflang
 
 `python -m asr_amd.build` compiles both every time it is called.
 """
@@ -42,12 +43,40 @@ def build_hip() -> str:
     return HIP_LIB
 
 
+FORTRAN_DIR = os.path.join(PKG, "fortran")
+FORTRAN_BUILD = os.path.join(FORTRAN_DIR, "build")
+FORTRAN_DEMO = os.path.join(FORTRAN_DIR, "asr_gpu_demo")
+
+
+def amdflang() -> str | None:
+    for c in ("/opt/rocm/bin/amdflang", "/opt/rocm/llvm/bin/flang"):
+        if os.path.exists(c):
+            return c
+    return None
+
+
+def build_fortran() -> str | None:
+    """The Fortran binding (module asr_gpu, asr_amd/fortran/asr_gpu.f90) and its short caller asr_gpu_demo, linked
+    against libasr_hip.so (found next to the binary's directory at run time).  Skipped when no Fortran compiler is
+    present."""
+    fc = amdflang()
+    if fc is None:
+        print("build_fortran: no amdflang, skipped", flush=True)
+        return None
+    os.makedirs(FORTRAN_BUILD, exist_ok=True)
+    _run([fc, "-O2", "-J", FORTRAN_BUILD, os.path.join(FORTRAN_DIR, "asr_gpu.f90"), os.path.join(FORTRAN_DIR, "asr_gpu_demo.f90"),
+          "-L", PKG, "-lasr_hip", "-Wl,-rpath,$ORIGIN/..", "-o", FORTRAN_DEMO + ".tmp"])
+    os.replace(FORTRAN_DEMO + ".tmp", FORTRAN_DEMO)
+    return FORTRAN_DEMO
+
+
 def build_all() -> None:
     build_host()
     if all(os.path.exists(s) for s in HIP_SRC):
         build_hip()
+        build_fortran()
 
 
 if __name__ == "__main__":
     what = sys.argv[1] if len(sys.argv) > 1 else "all"
-    {"host": build_host, "hip": build_hip, "all": build_all}[what]()
+    {"host": build_host, "hip": build_hip, "fortran": build_fortran, "all": build_all}[what]()

@@ -3978,8 +3978,9 @@ int asr_gpu_op_create(int nn, int nslot, int nop, int nsmax, int ncls, int nsp, 
 }
 
 int asr_gpu_op_set_tables(asr_gpu_op* op, const int* hopr, const int* hopg, const int* flipr, const int* flipg, const int* lst,
-                          const int* ncl, const int* ntyp, const unsigned char* fz, const double* sq, const int* stab_ptr,
+                          const int* ncl, const int* ntyp, const signed char* fz, const double* sq, const int* stab_ptr,
                           const int* stab, int nstab) {
+  if (!op) return fail("asr_gpu_op_set_tables: null operator");
   const int nn = op->nn, ns = op->nslot, nsm = op->nsmax;
   std::vector<uint32_t> hop((size_t)nn * ns), flip((size_t)nn * nsm), info(nn);
   std::vector<int> plist;  // the nodes with a flip partner other than themselves
@@ -3997,7 +3998,7 @@ int asr_gpu_op_set_tables(asr_gpu_op* op, const int* hopr, const int* hopg, cons
   for (int i = 0; i < nn; ++i) {
     if (lst[i] < 0 || lst[i] >= nsm || ncl[i] < 0 || ncl[i] >= op->ncls || ntyp[i] < 0 || ntyp[i] >= op->ntype)
       return fail("set_tables: node state, class or type out of range");
-    if (fz && fz[i] > 2) return fail("set_tables: frozen-site slot 0 (none), 1 (start site) or 2 (pair partner)");
+    if (fz && (fz[i] < 0 || fz[i] > 2)) return fail("set_tables: frozen-site slot 0 (none), 1 (start site) or 2 (pair partner)");
     // a site that can be frozen must have been non-flippable: its nodes keep the reference state, no flip partner
     if (fz && fz[i]) {
       bool flips = lst[i] != 0;
@@ -4050,6 +4051,7 @@ int asr_gpu_op_set_tables(asr_gpu_op* op, const int* hopr, const int* hopg, cons
 }
 
 int asr_gpu_op_set_orbital_ops(asr_gpu_op* op, const int* pi, const double* sg) {
+  if (!op) return fail("asr_gpu_op_set_orbital_ops: null operator");
   for (int q = 0; q < op->nop * 9; ++q)
     if (pi[q] < 0 || pi[q] > 8) return fail("set_orbital_ops: orbital index out of range");
   op->pi_h.assign(pi, pi + op->nop * 9);
@@ -4072,6 +4074,7 @@ int asr_gpu_op_set_orbital_ops(asr_gpu_op* op, const int* pi, const double* sg) 
 }
 
 int asr_gpu_op_set_blocks(asr_gpu_op* op, const double* block, const double* block0) {
+  if (!op) return fail("asr_gpu_op_set_blocks: null operator");
   const int ns = op->nslot;
   if ((size_t)op->ntype * (ns + 1) * 81 * 8 > 120 * 1024) return fail("set_blocks: bond tables exceed the LDS budget");
   std::vector<double> blk((size_t)op->ntype * (ns + 1) * 81);
@@ -4096,6 +4099,7 @@ int asr_gpu_op_set_blocks(asr_gpu_op* op, const double* block, const double* blo
 }
 
 int asr_gpu_op_set_classes(asr_gpu_op* op, const int* nsc, const int* spc, const double* mt) {
+  if (!op) return fail("asr_gpu_op_set_classes: null operator");
   for (int k = 0; k < op->ncls; ++k) {
     if (nsc[k] < 1 || nsc[k] > op->nsmax) return fail("set_classes: species count of a class out of range");
     for (int s = 0; s < nsc[k]; ++s)
@@ -4112,6 +4116,7 @@ int asr_gpu_op_set_classes(asr_gpu_op* op, const int* nsc, const int* spc, const
 
 // Member tile of the batched chains: 1, 2, ..., 64 members per wave (0 = autotuned per member count, the default).
 int asr_gpu_op_set_tile(asr_gpu_op* op, int mt) {
+  if (!op) return fail("asr_gpu_op_set_tile: null operator");
   if (mt != 65 && (mt < 0 || mt > 64 || (mt & (mt - 1)) != 0))
     return fail("asr_gpu_op_set_tile: 0, a power of two <= 64, or 65 (64 with complex members as lane pairs)");
   op->tile_pref = mt;
@@ -4147,6 +4152,7 @@ int asr_gpu_chains_edc_proj(asr_gpu_op* op, int nchain, const int* start_orb, co
                             const int* frozen_sp, const double* conc, const double* z, const int* nlev, int project, int ldab,
                             const double* tw, double* a, double* b, int* mused, double* g, double* gp, double* gp4, int mode,
                             double* elapsed_ms) {
+  if (!op) return fail("asr_gpu_chains_edc_proj: null operator");
   if (nchain <= 0) return 0;
   if (!op || !start_orb || !params || !pab || !z || !nlev || !tw || !a || !b || !mused || !g || !gp || !gp4)
     return fail("asr_gpu_chains_edc_proj: null argument");
@@ -4202,6 +4208,7 @@ int asr_gpu_op_set_reorth(asr_gpu_op* op, int reorth) {
 
 // Adaptive depth of the edc chains (ed_depth_tol / ed_tail of edc_green, src/asr_edc.f90:121-127): 0 = fixed depth.
 int asr_gpu_op_set_depth_tol(asr_gpu_op* op, double depth_tol, int tail) {
+  if (!op) return fail("asr_gpu_op_set_depth_tol: null operator");
   if (!(depth_tol >= 0.0) || tail < 0 || tail > 3) return fail("asr_gpu_op_set_depth_tol: depth_tol >= 0, tail 0..3");
   op->depth_tol = depth_tol;
   op->depth_tail = tail;
@@ -4209,6 +4216,7 @@ int asr_gpu_op_set_depth_tol(asr_gpu_op* op, double depth_tol, int tail) {
 }
 
 int asr_gpu_op_get_tile(asr_gpu_op* op, int nchain, int is_complex, int* mt) {
+  if (!op) return fail("asr_gpu_op_get_tile: null operator");
   *mt = 0;
   for (auto& kv : op->tuned)
     if ((kv.first >> 8) == nchain && ((kv.first & 1) != 0) == (is_complex != 0)) *mt = kv.second;
@@ -4292,6 +4300,7 @@ int asr_gpu_apply_complex(asr_gpu_op* op, int nb, const int* start_orb, const do
 int asr_gpu_lanczos_real(asr_gpu_op* op, int nchain, const int* start_orb, const double* params, const int* frozen_sp,
                          const double* conc, const int* nlev, double tiny, int ldab, double* a, double* b, int* mused, int mode,
                          double* elapsed_ms) {
+  if (!op) return fail("asr_gpu_lanczos_real: null operator");
   if (elapsed_ms) *elapsed_ms = 0.0;
   return run_chains<double>(op, nchain, start_orb, params, nullptr, frozen_sp, conc, nullptr, nlev, 0, ldab, a, b, mused, mode,
                             tiny > 0 ? tiny : 1e-12, true, elapsed_ms);
@@ -4300,6 +4309,7 @@ int asr_gpu_lanczos_real(asr_gpu_op* op, int nchain, const int* start_orb, const
 int asr_gpu_chains_edc(asr_gpu_op* op, int nchain, const int* start_orb, const double* params, const double* pab,
                        const int* frozen_sp, const double* conc, const double* z, const int* nlev, int project, int ldab,
                        double* a, double* b, int* mused, int mode, double* elapsed_ms) {
+  if (!op) return fail("asr_gpu_chains_edc: null operator");
   if (elapsed_ms) *elapsed_ms = 0.0;
   return run_chains<cplx>(op, nchain, start_orb, params, pab, frozen_sp, conc, z, nlev, project, ldab, (cplx*)a, (cplx*)b,
                           mused, mode, 1e-12, false, elapsed_ms);
@@ -4309,6 +4319,8 @@ int asr_gpu_chains_edc_multi(int ngroup, asr_gpu_op* const* ops, const int* ncha
                              const double* params, const double* pab, const int* frozen_sp, const double* conc, const double* z,
                              const int* nlev, int project, int ldab, double* a, double* b, int* mused, int mode,
                              double* elapsed_ms) {
+  for (int g = 0; g < ngroup; ++g)
+    if (!ops || !ops[g]) return fail("asr_gpu_chains_edc_multi: null operator");
   if (ngroup <= 0) return fail("chains_multi: no groups");
   return run_multi<cplx>(ngroup, ops, nchain, start_orb, params, pab, frozen_sp, conc, z, nlev, project, ldab, (cplx*)a,
                          (cplx*)b, mused, mode, 1e-12, false, elapsed_ms);
@@ -4360,6 +4372,7 @@ int asr_gpu_ed3_blocks(asr_gpu_op* op, int nchain, const int* start_orb, const d
                        const double* oc, const int* frozen_sp, const double* conc, const double* z, const int* nlev,
                        int project, int ldab, double* bk, double* a, double* b, double* c, int* mused, int mode,
                        double* elapsed_ms) {
+  if (!op) return fail("asr_gpu_ed3_blocks: null operator");
   if (nchain <= 0) return 0;
   if (!op || !start_orb || !params || !cf || !oc || !z || !nlev || !bk || !a || !b || !c || !mused)
     return fail("asr_gpu_ed3_blocks: null argument");
@@ -4417,6 +4430,7 @@ int asr_gpu_epi_blocks(asr_gpu_op* op, int nchain, const double* v9, const doubl
                        const double* pab, const int* frozen_sp, const int* frozen_sp2, const double* conc, const double* z,
                        const int* nlev, int project, int ldab, double* x, double* a, double* b, double* c, int* mused,
                        int mode, double* elapsed_ms) {
+  if (!op) return fail("asr_gpu_epi_blocks: null operator");
   if (nchain <= 0) return 0;
   if (!op || !v9 || !chi || !params || !pab || !z || !nlev || !x || !a || !b || !c || !mused)
     return fail("asr_gpu_epi_blocks: null argument");
@@ -4467,6 +4481,8 @@ int asr_gpu_epi_blocks(asr_gpu_op* op, int nchain, const double* v9, const doubl
 int asr_gpu_lanczos_real_multi(int ngroup, asr_gpu_op* const* ops, const int* nchain, const int* start_orb,
                                const double* params, const int* frozen_sp, const double* conc, const int* nlev, double tiny,
                                int ldab, double* a, double* b, int* mused, int mode, double* elapsed_ms) {
+  for (int g = 0; g < ngroup; ++g)
+    if (!ops || !ops[g]) return fail("asr_gpu_lanczos_real_multi: null operator");
   if (ngroup <= 0) return fail("lanczos_multi: no groups");
   return run_multi<double>(ngroup, ops, nchain, start_orb, params, nullptr, frozen_sp, conc, nullptr, nlev, 0, ldab, a, b,
                            mused, mode, tiny > 0 ? tiny : 1e-12, true, elapsed_ms);
@@ -4475,6 +4491,7 @@ int asr_gpu_lanczos_real_multi(int ngroup, asr_gpu_op* const* ops, const int* nc
 int asr_gpu_lanczos_adjoint(asr_gpu_op* op, int nchain, const int* start_orb, const double* params, const int* frozen_sp,
                             const double* conc, int nlev, int ckpt, const double* dl_da, const double* dl_db, double* a,
                             double* b, double* grad, double* fwd_ms, double* adj_ms) {
+  if (!op) return fail("asr_gpu_lanczos_adjoint: null operator");
   if (nchain <= 0 || nlev <= 0) return fail("lanczos_adjoint: no chains");
   return run_adjoint(op, nchain, start_orb, params, frozen_sp, conc, nlev, ckpt, dl_da, dl_db, a, b, grad, fwd_ms, adj_ms);
 }
@@ -4486,6 +4503,7 @@ int asr_gpu_lanczos_adjoint(asr_gpu_op* op, int nchain, const int* start_orb, co
 int asr_gpu_profile_chains(asr_gpu_op* op, int nchain, const int* start_orb, const double* params, const double* pab,
                            const int* frozen_sp, const double* conc, const double* z, const int* nlev, int project, int* nk,
                            double* kernel_ms, int* kind) {
+  if (!op) return fail("asr_gpu_profile_chains: null operator");
   KTimer T;
   g_timer = &T;
   g_acc.clear();
@@ -4516,14 +4534,14 @@ int asr_gpu_profile_chains(asr_gpu_op* op, int nchain, const int* start_orb, con
 }
 
 // ------------------------------------------------------------------------------------------------ KPM
-int asr_gpu_real_create(int nsite, int nslot, const int* nbr, int nsp, const unsigned char* species, const double* par,
+int asr_gpu_real_create(int nsite, int nslot, const int* nbr, int nsp, const signed char* species, const double* par,
                         const double* block, const double* block0, asr_gpu_real** out) {
   *out = nullptr;
   if (nslot > 18) return fail("asr_gpu_real_create: at most 18 neighbour slots");
   for (size_t q = 0; q < (size_t)nsite * nslot; ++q)
     if (nbr[q] < 0 || nbr[q] >= nsite) return fail("asr_gpu_real_create: neighbour index out of range");
   for (int i = 0; i < nsite; ++i)
-    if (species[i] >= nsp) return fail("asr_gpu_real_create: species out of range");
+    if (species[i] < 0 || species[i] >= nsp) return fail("asr_gpu_real_create: species out of range");
   for (int q = 2 * nsp * 9; q < 4 * nsp * 9; ++q)
     if (par[q] != 0.0) return fail("asr_gpu_real_create: the KPM kernel applies h = c + d S d (o = E_nu = 0)");
   auto* R = new asr_gpu_real();
@@ -4595,6 +4613,7 @@ int asr_gpu_real_set_doubling(asr_gpu_real* R, int on) {
 
 // Vector tile of the KPM step (vectors per site in a wave: 8, 16, 32 or 64; 0 = automatic).
 int asr_gpu_real_set_tile(asr_gpu_real* R, int vt) {
+  if (!R) return fail("asr_gpu_real_set_tile: null handle");
   if (vt != 0 && vt != 1 && vt != 2 && vt != 3)
     return fail("asr_gpu_real_set_tile: 0 (automatic), 1 or 2 vectors per lane, 3 = 16-vector slices");
   R->v2 = (vt != 1);
@@ -4604,6 +4623,7 @@ int asr_gpu_real_set_tile(asr_gpu_real* R, int vt) {
 
 // y = h x for nvec vectors (host layout [vector][9 nsite])
 int asr_gpu_real_apply(asr_gpu_real* R, int nvec, const double* x, double* y) {
+  if (!R) return fail("asr_gpu_real_apply: null handle");
   const size_t dim = (size_t)R->nsite * 9;
   const int vt = kpm_vt(R, nvec);
   const int nvp = ((nvec + vt - 1) / vt) * vt;
@@ -4733,6 +4753,7 @@ static int kpm_run(asr_gpu_real* R, double c, double s, int nmom, int nvec, uint
 // Chebyshev step kernel alone (HIP events around each launch on the engine's stream); total_ms: the whole recurrence.
 int asr_gpu_kpm_moments(asr_gpu_real* R, double c, double s, int nmom, int nvec, uint64_t seed, double* mu, double* mu_per,
                         double* step_ms, double* total_ms) {
+  if (!R) return fail("asr_gpu_kpm_moments: null handle");
   if (kpm_check(nmom, nvec, seed, R)) return 1;
   const KpmSizes z = kpm_sizes(R, nmom, nvec);
   KpmBufs B;

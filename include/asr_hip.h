@@ -30,11 +30,12 @@ int asr_gpu_device_count(int* n);
  * ntype site types.  Replaces the allocation part of build_augcls_operator (src/asr_augcls.f90:709-727). */
 int asr_gpu_op_create(int nn, int nslot, int nop, int nsmax, int ncls, int nsp, int ntype, asr_gpu_op** out);
 /* Node tables (src/asr_augcls.f90:55-63): hopr/hopg nn x nslot, flipr/flipg nn x nsmax, lst/ncl/ntyp/fz/sq nn,
- * stab_ptr nn + 1, stab nstab.  fz[i] = 1: node i sits on the start site (augcls_freeze_site applies there, :129-141);
- * fz[i] = 2: node i sits on a frozen pair partner (augcls_freeze_any, :147-161; the EPI pair operator); 0 elsewhere.
+ * stab_ptr nn + 1, stab nstab.  fz: one byte per node (signed char, Fortran's integer(c_signed_char)): 1 the node sits on
+ * the start site (augcls_freeze_site applies there, :129-141), 2 on a frozen pair partner (augcls_freeze_any, :147-161;
+ * the EPI pair operator), 0 elsewhere.
  * Frozen sites must have been non-flippable when the node set was built. */
 int asr_gpu_op_set_tables(asr_gpu_op* op, const int* hopr, const int* hopg, const int* flipr, const int* flipg, const int* lst,
-                          const int* ncl, const int* ntyp, const unsigned char* fz, const double* sq, const int* stab_ptr,
+                          const int* ncl, const int* ntyp, const signed char* fz, const double* sq, const int* stab_ptr,
                           const int* stab, int nstab);
 /* Orbital operation tables pi, sg (nop x 9): D(g)[pi(l), l] = sg(l) (src/asr_augcls.f90:727-736). */
 int asr_gpu_op_set_orbital_ops(asr_gpu_op* op, const int* pi, const double* sg);
@@ -183,9 +184,9 @@ int asr_gpu_profile_chains(asr_gpu_op* op, int nchain, const int* start_orb, con
                            double* kernel_ms, int* kind);
 
 /* ---- KPM on a supercell realisation (lmto_real_op, src/reference/asr_lmto_realization.f90:14-23) */
-/* nbr: nsite x nslot neighbour sites (nslot <= 18), species: nsite, par: 4 x nsp x 9 (c, d, o, e; o = e = 0 required: the
- * kernel applies h = c + d S d), block: nslot x 9 x 9, block0: 9 x 9. */
-int asr_gpu_real_create(int nsite, int nslot, const int* nbr, int nsp, const unsigned char* species, const double* par,
+/* nbr: nsite x nslot neighbour sites (nslot <= 18), species: nsite bytes (signed char, 0 .. nsp - 1), par: 4 x nsp x 9
+ * (c, d, o, e; o = e = 0 required: the kernel applies h = c + d S d), block: nslot x 9 x 9, block0: 9 x 9. */
+int asr_gpu_real_create(int nsite, int nslot, const int* nbr, int nsp, const signed char* species, const double* par,
                         const double* block, const double* block0, asr_gpu_real** out);
 void asr_gpu_real_destroy(asr_gpu_real* r);
 /* Vectors per lane in the Chebyshev steps: 1, 2, or 0 = automatic (2: 16-byte loads, two sites per wave). */

@@ -24,6 +24,9 @@ _LIB = None
 
 def build() -> str:
     subprocess.check_call(["make", "-s", "-C", _HERE])
+    if os.path.isdir("/root/reference/src") and os.path.exists("/opt/rocm/bin/amdflang"):
+        # the reference's chain modules compile-only and the Fortran binding against them (oracle/_ref/fmod)
+        subprocess.check_call(["make", "-s", "-C", _HERE, "ref"])
     return os.path.join(_HERE, "liborc.so")
 
 
