@@ -1,0 +1,115 @@
+"""Parity at the measured sizes, not only at reduced ones.
+
+* Headline (fcc ternary, 127,517 nodes, levels 4, D2h): all 54 chains x 30 levels of the GPU run against the oracle's
+  chains, 1e-9 relative.
+* The L2_1 uneven batch (four sublattice operators with SRO classes, 256 chains of depth 20-60, one HIP graph): the
+  deepest and the shallowest chain of every operator against the oracle at full depth, 1e-9.
+* KPM on the 40^3-cell supercell (2,304,000 rows, 64 vectors, 2048 moments): size-independent properties, mu_0 = 1,
+  |mu_n| <= 1, moment doubling = direct moments to 1e-12, the Jackson DOS integrates to 1.
+* The adjoint (256 real chains x 40 on 127,517 nodes): gradient components of a contour objective against central finite
+  differences of the GPU forward (h = 1e-5), 1e-6 relative.
+"""
+import numpy as np
+import pytest
+
+from asr_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _engine():
+    from asr_amd import engine
+    assert engine.device_count() >= 1, "no GPU visible"
+    return engine
+
+
+def test_headline_all_chains_against_the_oracle():
+    eng = _engine()
+    from oracle.oracle import OracleOp
+    wl = W.ternary_edc(L=8, levels=4, depth=30)
+    print(f"headline: {wl.op.nn} nodes, {len(wl.members)} chains x 30", flush=True)
+    G = eng.GpuOperator(wl.op)
+    a, b, mu, _ = G.chains_edc(wl.members, wl.nlev)
+    G.close()
+    eps = np.stack([m.z * m.pa + m.pb for m in wl.members])
+    ar, br, mr = OracleOp(wl.op).edc_batch(wl.members, eps, [m.l0 for m in wl.members], list(wl.nlev), project=True)
+    assert np.array_equal(mu, mr)
+    for j in range(len(wl.members)):
+        n = int(mu[j])
+        assert np.max(np.abs(a[j, :n] - ar[j, :n])) < 1e-9 * np.max(np.abs(ar[j, :n])), f"chain {j} a"
+        assert np.max(np.abs(b[j, :n - 1] - br[j, :n - 1])) < 1e-9 * np.max(np.abs(br[j, :n - 1])), f"chain {j} b"
+
+
+def test_l21_uneven_batch_full_depth_sample():
+    eng = _engine()
+    from oracle.oracle import OracleOp
+    wls = W.l21_batch()
+    groups = [(eng.GpuOperator(w.op), w.members, w.nlev) for w in wls]
+    print(f"l21: {[(w.op.nn, len(w.members)) for w in wls]}", flush=True)
+    res, _ = eng.chains_multi(groups, mode=2)
+    for (G, members, nlev), w, (a, b, mu) in zip(groups, wls, res):
+        O = OracleOp(w.op)
+        for j in {int(np.argmax(nlev)), int(np.argmin(nlev))}:
+            m = members[j]
+            ea, eb, n = O.edc_chain(m, m.z * m.pa + m.pb, m.l0, int(nlev[j]))
+            assert mu[j] == n == nlev[j]
+            assert np.max(np.abs(a[j, :n] - ea[:n])) < 1e-9 * np.max(np.abs(ea[:n])), (w.name, j)
+            assert np.max(np.abs(b[j, :n - 1] - eb[:n - 1])) < 1e-9 * np.max(np.abs(eb[:n - 1])), (w.name, j)
+        G.close()
+
+
+def test_kpm_full_size_properties():
+    eng = _engine()
+    from asr_amd import host as H
+    g, species, par, block, block0 = W.kpm_supercell(L=40)
+    R = eng.GpuRealization(g.nbr, species, par, block, block0)
+    elo, ehi, _, _ = R.spectral_bounds(80, margin=0.0, nvec=1, seed=3)
+    marg = 0.01 * (ehi - elo)
+    c, s = 0.5 * (elo + ehi), 0.5 * (ehi - elo + 2.0 * marg) / 0.99
+    print(f"kpm: {g.n * 9} rows, 64 vectors, 2048 moments", flush=True)
+    mu = R.kpm_moments(c, s, 2048, 64, seed=5)["mu"]
+    R.set_doubling(True)
+    md = R.kpm_moments(c, s, 2048, 64, seed=5)["mu"]
+    R.close()
+    assert abs(mu[0] - 1.0) < 1e-14
+    assert np.max(np.abs(mu)) <= 1.0 + 1e-12
+    assert np.max(np.abs(md - mu)) < 1e-12
+    # the Jackson DOS over the scaled window integrates to g_0 mu_0 = 1 (Chebyshev-Gauss nodes: exact for the polynomial
+    # part; the weight 1/sqrt(1 - x^2) is absorbed by the nodes)
+    nq = 8192
+    x = np.cos(np.pi * (np.arange(nq) + 0.5) / nq)
+    rho = H.kpm_dos(mu, c, s, c + s * x)
+    integral = float(np.sum(rho * np.sqrt(1.0 - x * x)) * np.pi / nq * s)
+    assert abs(integral - 1.0) < 1e-10, integral
+
+
+def test_adjoint_full_size_gradient_components():
+    eng = _engine()
+    import torch
+    from asr_amd import adjoint as AD, host
+    wa = W.ternary_adjoint()
+    nlev = int(wa.nlev[0])
+    nch = len(wa.members)
+    G = eng.GpuOperator(wa.op)
+    z, w = host.contour_closed(-1.6, -0.05, 0.6, 24, 8)
+    zt, wt = torch.from_numpy(z), torch.from_numpy(w)
+    weight = torch.from_numpy(np.asarray(wa.info["conc"])[[(j // 9) % 3 for j in range(nch)]] / 9.0)
+    q0 = torch.full((nch,), 0.5, dtype=torch.float64)
+    off = wa.info["c_offset"]
+
+    def L(th):
+        a, b = AD.chain_coefficients(th, G, wa.members, nlev, off)
+        return AD.contour_objective(a, b, zt, wt, weight, q0, lam=3.0)
+
+    print(f"adjoint: {wa.op.nn} nodes, {nch} chains x {nlev}", flush=True)
+    theta = torch.tensor(wa.info["theta"], requires_grad=True)
+    L(theta).backward()
+    grad = theta.grad.numpy()
+    h = 1e-5
+    with torch.no_grad():
+        for idx in [(0, 0, 0), (0, 2, 1), (1, 1, 2), (1, 0, 1)]:
+            tp = wa.info["theta"].copy(); tp[idx] += h
+            tm = wa.info["theta"].copy(); tm[idx] -= h
+            fd = (float(L(torch.from_numpy(tp))) - float(L(torch.from_numpy(tm)))) / (2 * h)
+            assert abs(grad[idx] - fd) < 1e-6 * max(abs(fd), np.max(np.abs(grad)) * 1e-2), (idx, grad[idx], fd)
+    G.close()
