@@ -1538,6 +1538,126 @@ __global__ void __launch_bounds__(BLOCK) k_fin_b(ChainDev cd, int nb, const T* p
   if (act && part2 && cd.cvec) ((T*)cd.cvec)[(size_t)m * cd.ld + k + 1] = bilin(s2, inv(bk));
 }
 
+// ---------------------------------------------------------------------------------------------- adaptive depth on the device
+// Eigenvalues of the complex symmetric tridiagonal chain matrix (diagonal d[0..n-1], off-diagonal e[0..n-2], both
+// overwritten; the eigenvalues come back in d): the implicit QL iteration with shifts (the method of EISPACK tql1) in
+// complex arithmetic with complex orthogonal rotations (c^2 + s^2 = 1), which keep a complex symmetric tridiagonal matrix
+// tridiagonal.  Returns 0, or 1 when a rotation breaks down (c^2 + s^2 = 0) or the iteration does not converge.
+__device__ inline cplx csq(cplx v) { return csqrt_c(v); }
+__device__ int cs_tridiag_eigs(int n, cplx* d, cplx* e) {
+  const double eps = 2.220446049250313e-16;
+  e[n - 1] = zero<cplx>();
+  for (int l = 0; l < n; ++l) {
+    int iter = 0, m;
+    for (;;) {
+      for (m = l; m < n - 1; ++m)
+        if (mag(e[m]) <= eps * (mag(d[m]) + mag(d[m + 1]))) break;
+      if (m == l) break;
+      if (iter++ == 60) return 1;
+      cplx g = (d[l + 1] - d[l]) * inv(2.0 * e[l]);
+      cplx r = csq(g * g + cplx{1.0, 0.0});
+      // the root that makes |g + r| large (no cancellation)
+      const cplx gp = g + r, gm = g - r;
+      g = d[m] - d[l] + e[l] * inv(mag(gp) >= mag(gm) ? gp : gm);
+      cplx sn = {1.0, 0.0}, c = {1.0, 0.0}, p = zero<cplx>();
+      bool split = false;
+      for (int i = m - 1; i >= l; --i) {
+        const cplx f = sn * e[i], b = c * e[i];
+        r = csq(f * f + g * g);
+        e[i + 1] = r;
+        if (mag(r) == 0.0) {
+          if (mag(f) + mag(g) != 0.0) return 1;  // an isotropic pair: the rotation does not exist
+          d[i + 1] = d[i + 1] - p;
+          e[m] = zero<cplx>();
+          split = true;
+          break;
+        }
+        const cplx ir = inv(r);
+        sn = f * ir;
+        c = g * ir;
+        g = d[i + 1] - p;
+        r = (d[i] - g) * sn + 2.0 * (c * b);
+        p = sn * r;
+        d[i + 1] = g + p;
+        g = c * r - b;
+      }
+      if (split) continue;
+      d[l] = d[l] - p;
+      e[l] = g;
+      e[m] = zero<cplx>();
+    }
+  }
+  return 0;
+}
+
+constexpr int DEPTH_MAX = 128;  // chains of the device depth test: at most this many levels
+
+// G_11 (and f = sum_k c_k G_k1) of a chain of m levels terminated with the tail of cf_species (src/asr_chain.f90:67-167):
+// tail 0 hull (the extreme real parts of the chain matrix's eigenvalues give a_inf, b_inf), 2 avg2 (the last two levels),
+// 3 none.  Returns 1 when the hull's eigenvalues fail (the caller then keeps the chain running).
+__device__ int cf_species_dev(const cplx* a, const cplx* b, const cplx* c, int m, cplx z, int tail, cplx* g11, cplx* f) {
+  cplx aa = a[m - 1], bb = m > 1 ? b[m - 2] : zero<cplx>();
+  if (tail == 0 && m >= 2) {
+    cplx d[DEPTH_MAX], e[DEPTH_MAX];
+    for (int j = 0; j < m; ++j) { d[j] = a[j]; e[j] = j < m - 1 ? b[j] : zero<cplx>(); }
+    if (cs_tridiag_eigs(m, d, e)) return 1;
+    int lo = 0, hi = 0;
+    for (int j = 1; j < m; ++j) {
+      if (d[j].re < d[lo].re) lo = j;
+      if (d[j].re > d[hi].re) hi = j;
+    }
+    aa = 0.5 * (d[lo] + d[hi]);
+    bb = 0.25 * (d[hi] - d[lo]);
+  }
+  if (tail == 2 && m >= 3) {
+    aa = 0.5 * (a[m - 1] + a[m - 2]);
+    bb = csq(0.5 * (b[m - 2] * b[m - 2] + b[m - 3] * b[m - 3]));
+  }
+  if (tail == 3) { aa = zero<cplx>(); bb = zero<cplx>(); }
+  const cplx dz = z - aa;
+  cplx tl = zero<cplx>();
+  if (mag(bb) >= 1e-14) {
+    const cplx s = csq(dz * dz - 4.0 * (bb * bb));
+    const cplx ib = inv(2.0 * (bb * bb));
+    const cplx g1 = (dz - s) * ib, g2 = (dz + s) * ib;
+    tl = mag(bb * g1) <= mag(bb * g2) ? (bb * bb) * g1 : (bb * bb) * g2;
+  }
+  // G_k down the chain, then f with the first column pv_k = prod G
+  cplx gt[DEPTH_MAX];
+  gt[m - 1] = inv(z - a[m - 1] - tl);
+  for (int j = m - 2; j >= 0; --j) gt[j] = inv(z - a[j] - (b[j] * b[j]) * gt[j + 1]);
+  cplx prod = gt[0], ff = zero<cplx>();
+  for (int k = 0; k < m; ++k) {
+    if (c) ff = ff + c[k] * prod;
+    if (k < m - 1) prod = prod * b[k] * gt[k + 1];
+  }
+  *g11 = gt[0];
+  *f = ff;
+  return 0;
+}
+
+// The adaptive depth of edc_green (src/asr_edc.f90:121-127) at level k1 (1-based; k1 = 8, 12, ...) on the device, one
+// thread per member: a member still running below its depth gets G_k1 of its chain (and route ed3's projection f, with
+// the check of ed3's chain, src/asr_ed3.f90:177-184), and leaves when |G - G_prev| <= tol |G| (k1 > 8).  gprev / fprev keep
+// the previous tested level's values.  Run between the a_k reduction and the update, as the host test.
+__global__ void k_depth_test(ChainDev cd, int nb, const cplx* a, const cplx* b, const cplx* cvec, const cplx* z, int k1,
+                             double tol, int tail, cplx* gprev, cplx* fprev, const double* tnorm) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nb || !cd.active[q] || k1 >= cd.depth[q] || k1 > DEPTH_MAX) return;
+  cplx g, f;
+  if (cf_species_dev(a + (size_t)q * cd.ld, b + (size_t)q * cd.ld, cvec ? cvec + (size_t)q * cd.ld : nullptr, k1, z[q], tail,
+                     &g, &f))
+    return;
+  if (k1 > 8) {
+    const double ag = mag(g);
+    bool conv = mag(g - gprev[q]) <= tol * ag;
+    if (cvec) conv = conv && mag(f - fprev[q]) <= tol * fmax(mag(f), tnorm[q] * ag);
+    if (conv) cd.active[q] = 0;
+  }
+  gprev[q] = g;
+  fprev[q] = f;
+}
+
 // lagged normalisation: w <- w / s (the vector of the next level normalised), then s <- 1 (k_scale_one)
 template <class T>
 __global__ void k_rescale(T* w, const T* s, size_t n, int nbp) {
@@ -2903,6 +3023,10 @@ struct ChainRun {
   int* tact = nullptr;     // per member tile: 1 while any member of the tile is active
   int level_host = 0;      // levels issued
   std::vector<cplx> gprev, fprev;  // adaptive depth: G(z) (and the projection f of the blocks) at the previous tested level
+  // the adaptive depth on the device (tails hull, avg2, none): no host synchronisation, so the run can be one HIP graph
+  bool dev_depth = false;
+  cplx *gprev_d = nullptr, *fprev_d = nullptr;
+  double* tnorm_d = nullptr;
   bool ed3 = false;                  // two-chain blocks (start vectors, projections c_k, closure flags)
   std::vector<double> snorm, tnorm;  // blocks: norms of the start vector and of the projection vector per member
   std::vector<double> s0h;           // blocks: 9 x nbp start vectors on the start node
@@ -2929,6 +3053,8 @@ struct ChainRun {
     // work of an aborted run may still be queued: the cached blocks go back only once the stream is idle
     if (stream) (void)hipStreamSynchronize(stream);
     if (tact) (void)hipFree(tact);
+    for (void* p : {(void*)gprev_d, (void*)fprev_d, (void*)tnorm_d})
+      if (p) (void)hipFree(p);
     for (T* p : sc3)
       if (p) (void)hipFree(p);
   }
@@ -3077,6 +3203,18 @@ struct ChainRun {
     cd.cvec = B.c; cd.closed = B.closed;
     sgrid = dim3((unsigned)((nv + 255) / 256));
     if (ed3 && block_start(ocs, sides)) return 1;
+    if constexpr (std::is_same<T, cplx>::value) {
+      dev_depth = lin && !proj && op->depth_tol > 0.0 && op->depth_tail != 1 && maxdepth <= DEPTH_MAX;
+      if (dev_depth) {
+        if (dalloc(&gprev_d, M.nbp) || dalloc(&fprev_d, M.nbp) || dalloc(&tnorm_d, M.nbp)) return 1;
+        std::vector<double> tn(M.nbp, 0.0);
+        for (size_t q = 0; q < tnorm.size(); ++q) tn[q] = tnorm[q];
+        HCHECK(hipMemcpyAsync(tnorm_d, tn.data(), M.nbp * 8, hipMemcpyHostToDevice, stream));
+        HCHECK(hipMemsetAsync(gprev_d, 0, M.nbp * sizeof(cplx), stream));
+        HCHECK(hipMemsetAsync(fprev_d, 0, M.nbp * sizeof(cplx), stream));
+        HCHECK(hipStreamSynchronize(stream));
+      }
+    }
     return 0;
   }
 
@@ -3159,8 +3297,27 @@ struct ChainRun {
   // update, b_k and the scaling.  With the adaptive depth the host looks at a_k between the two (adaptive_check).
   int step() {
     if (step_apply()) return 1;
-    if (adaptive_check()) return 1;
+    if (dev_depth) {
+      if (depth_test_dev()) return 1;
+    } else if (adaptive_check()) {
+      return 1;
+    }
     return step_update();
+  }
+
+  // the adaptive depth test of this level on the device (k_depth_test): at levels 8, 12, 16, ... (1-based)
+  int depth_test_dev() {
+    if constexpr (!std::is_same<T, cplx>::value) {
+      return 0;
+    } else {
+      const int k1 = level_host + 1;
+      if (k1 < 8 || (k1 % 4) != 0) return 0;
+      hipLaunchKernelGGL(k_depth_test, dim3((M.nb + 63) / 64), dim3(64), 0, stream, cd, M.nb, (const cplx*)B.a,
+                         (const cplx*)B.b, ed3 ? (const cplx*)B.c : (const cplx*)nullptr, (const cplx*)md.z, k1,
+                         op->depth_tol, op->depth_tail, gprev_d, fprev_d, (const double*)tnorm_d);
+      HCHECK(hipGetLastError());
+      return 0;
+    }
   }
 
   int step_apply() {
@@ -3528,9 +3685,10 @@ int run_multi_once(int ngroup, asr_gpu_op* const* ops, const int* ngch, const in
     }
     return 0;
   }
-  // the adaptive depth decides on the host between levels: no graph capture then (levels issued from the host)
+  // the adaptive depth with the bp tail decides on the host between levels: no graph capture then (the other tails are
+  // tested on the device, k_depth_test)
   for (int g = 0; g < ngroup; ++g)
-    if (mode == 2 && ops[g]->depth_tol > 0.0 && std::is_same<T, cplx>::value) mode = 0;
+    if (mode == 2 && ops[g]->depth_tol > 0.0 && ops[g]->depth_tail == 1 && std::is_same<T, cplx>::value) mode = 0;
   std::vector<hipStream_t> st(ngroup, nullptr);
   std::vector<ChainRun<T>> runs(ngroup);
   auto cleanup = [&]() {
@@ -3622,14 +3780,14 @@ int run_multi(int ngroup, asr_gpu_op* const* ops, const int* ngch, const int* st
 
 // The chains of two-chain blocks on one operator (the members of asr_gpu_ed3_blocks and asr_gpu_epi_blocks, two per
 // block): mode 0 host-issued levels, 1 the plain form (one chain per launch sequence, host read-back per level, as chain),
-// 2 the whole run as one HIP graph (fixed depths only: the adaptive depth decides on the host between levels, so it
+// 2 the whole run as one HIP graph (with the adaptive depth on the device; the bp tail's test runs on the host, so it
 // falls back to mode 0).
 int run_blocks(asr_gpu_op* op, int nmem, const int* so, const double* params, const double* pab, const int* fz,
                const double* conc, const double* z, const int* nlev, const BlockIn& e3, int project, int ldab, cplx* a, cplx* b,
                cplx* c, int* mused, int* closed, double* snorm, double* tnorm, int mode, double* elapsed_ms) {
   const int nsp9 = op->nsp * 9, ncn = op->ncls * op->nsmax;
   *elapsed_ms = 0.0;
-  if (mode == 2 && op->depth_tol > 0.0) mode = 0;
+  if (mode == 2 && op->depth_tol > 0.0 && op->depth_tail == 1) mode = 0;
   if (mode == 1 && nmem > 1) {
     for (int q = 0; q < nmem; ++q) {
       const BlockIn e1{e3.pex ? e3.pex + (size_t)q * 4 * nsp9 : nullptr, e3.oc + q, e3.side + q,

@@ -51,8 +51,10 @@ int asr_gpu_op_set_tile(asr_gpu_op* op, int mt);
 /* Adaptive depth of the complex-symmetric chains (ed_depth_tol and ed_tail, edc_green, src/asr_edc.f90:121-127): at
  * levels 8, 12, 16, ... a chain that has not reached its depth is terminated with the given tail (0 hull, 1 bp, 2 avg2,
  * 3 none; cf_species, src/asr_chain.f90:162-262) and leaves when |G_k - G_{k-4}| <= depth_tol |G_k| (k >= 12).
- * depth_tol = 0 (the default): every chain runs to its depth.  With depth_tol > 0 the levels are issued from the host
- * (one synchronisation per tested level; asr_gpu_chains_edc_multi mode 2 then runs as mode 0). */
+ * depth_tol = 0 (the default): every chain runs to its depth.  With the tails hull, avg2 and none the test runs on the
+ * device (the chain matrix's eigenvalues by a complex-symmetric QL iteration, one thread per chain, chains of at most 128
+ * levels), so a batch stays one stream of launches or one HIP graph; with bp (or deeper chains) the host evaluates it
+ * (one synchronisation per tested level; mode 2 of the batched calls then runs as mode 0). */
 int asr_gpu_op_set_depth_tol(asr_gpu_op* op, double depth_tol, int tail);
 /* Target states of the projected chains: tnode / tgop of build_augcls_operator with tsite (src/asr_augcls.f90:805-819,
  * asr_host_tables_targets): ntarget x nsmax representative nodes (-1: none) and the operations that map the target

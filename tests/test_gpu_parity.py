@@ -313,6 +313,48 @@ def test_edc_adaptive_depth(mode):
     assert np.any(mu == 12) and np.any(mu > 12)
 
 
+@pytest.mark.parametrize("tail,mode", [("hull", 2), ("avg2", 2), ("none", 0), ("bp", 0)])
+def test_edc_adaptive_depth_tails_and_graph(tail, mode):
+    """The adaptive depth with every tail of cf_species: hull, avg2 and none are tested on the device (the chain
+    matrix's eigenvalues by the complex-symmetric QL iteration), so mode 2 stays one HIP graph; bp is tested on the host.
+    The stopping levels are those of the rule applied with the host's cf_species (its own eigen-solver) to the chains'
+    coefficients."""
+    eng = _engine()
+    from asr_amd import host
+    wl = W.ternary_edc(L=4, levels=2, depth=24, seed=13)
+    G = eng.GpuOperator(wl.op)
+    members = wl.members[20:44]
+    nlev = 24
+
+    def g_at(a, b, k1, z):
+        return host.cf_species(a, b, np.zeros(k1), k1, z, tail)[0]
+
+    a0, b0, _, _ = G.chains_edc(members, nlev)
+    ch = np.array([abs(g_at(a0[j], b0[j], 12, m.z) - g_at(a0[j], b0[j], 8, m.z)) / abs(g_at(a0[j], b0[j], 12, m.z))
+                   for j, m in enumerate(members)])
+    u = np.unique(np.round(ch, 12))
+    mid = len(u) // 2
+    lo, hi = u[mid - 1], u[mid]
+    assert hi > lo * (1 + 1e-6)
+    tol = float(np.sqrt(lo * hi))
+    G.set_depth_tol(tol, tail)
+    res, _ = eng.chains_multi([(G, members, np.full(len(members), nlev, np.int32))], mode=mode)
+    a, b, mu = res[0]
+    G.set_depth_tol(0.0)
+    for j, m in enumerate(members):
+        stop, gprev = nlev, None
+        for k1 in range(8, nlev, 4):
+            g = g_at(a[j], b[j], k1, m.z)
+            if k1 > 8 and abs(g - gprev) <= tol * abs(g):
+                stop = k1
+                break
+            gprev = g
+        assert mu[j] == stop, (j, mu[j], stop)
+        k = int(mu[j])
+        assert rel(a[j, :k], a0[j, :k]) < 1e-12 and np.all(a[j, k:] == 0.0)
+    assert np.any(mu == 12) and np.any(mu > 12)
+
+
 def test_lanczos_full_reorthogonalisation():
     """lanczos with reorth = .true. on the GPU: the chains of the plain binary workload to depth 40 equal the oracle's
     reorthogonalised chains (1e-9); on the ring the reorthogonalised chain closes after dim steps with the exact spectrum
