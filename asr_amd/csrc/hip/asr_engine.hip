@@ -1898,6 +1898,8 @@ struct RealDev {
   const unsigned char* spec;    // nsite
   const double* par;            // 4 x nsp x 9: c, d, o, e
   const double* blk;            // (nslot + 1) x 81, slot 0 = on-site
+  const double* idv;            // nsp x 9: 1 / d (no division in the steps)
+  const double* fsh;            // nsp x 9: (c - cshift) / d^2 of the current recurrence's shift
 };
 
 __device__ inline uint64_t splitmix64(uint64_t z) {
@@ -2008,6 +2010,7 @@ __global__ void __launch_bounds__(BLOCK) k_kpm_u(RealDev R, const double* __rest
     const int sp = R.spec[site];
     const double* cp = R.par + (size_t)sp * 9;
     const double* dp = R.par + (size_t)(R.nsp + sp) * 9;
+    const double* id = R.idv + (size_t)sp * 9;
     const int nslot = NSLOT > 0 ? NSLOT : R.nslot;
     double acc[9], xo[9];
     {
@@ -2045,10 +2048,10 @@ __global__ void __launch_bounds__(BLOCK) k_kpm_u(RealDev R, const double* __rest
       for (int l = 0; l < 9; ++l) {
         const uint64_t row = (uint64_t)site * 9 + l;
         const double r = rademacher(seed, (uint64_t)v, row);
-        const double t1 = GEN ? r : xo[l] / dp[l];
+        const double t1 = GEN ? r : xo[l] * id[l];
         const double hx = cp[l] * t1 + dp[l] * acc[l];
         double t2 = alpha * (hx - cshift * t1);
-        if (beta != 0.0) t2 -= beta * (u0 ? u0[row * nvp + v] / dp[l] : r);
+        if (beta != 0.0) t2 -= beta * (u0 ? u0[row * nvp + v] * id[l] : r);
         u2[row * nvp + v] = dp[l] * t2;
         if (want_dot) dot = fma(r, t2, dot);
       }
@@ -2095,10 +2098,10 @@ __global__ void __launch_bounds__(BLOCK) k_kpm_sq(RealDev R, int nvp, const doub
   const int v = blockIdx.y * 64 + 2 * vl;
   d2 dot = {0.0, 0.0};
   if (site < R.nsite) {
-    const double* dp = R.par + (size_t)(R.nsp + R.spec[site]) * 9;
+    const double* id = R.idv + (size_t)R.spec[site] * 9;
     for (int l = 0; l < 9; ++l) {
       const d2 x = ((const d2*)(u + ((size_t)site * 9 + l) * nvp + v))[0];
-      const double tx = x.x / dp[l], ty = x.y / dp[l];
+      const double tx = x.x * id[l], ty = x.y * id[l];
       dot.x = fma(tx, tx, dot.x);
       dot.y = fma(ty, ty, dot.y);
     }
@@ -2119,8 +2122,9 @@ __global__ void __launch_bounds__(BLOCK) k_kpm_v2(RealDev R, const double* __res
   d2 dot = {0.0, 0.0}, dot2 = {0.0, 0.0};
   if (site < R.nsite) {
     const int sp = R.spec[site];
-    const double* cp = R.par + (size_t)sp * 9;
     const double* dp = R.par + (size_t)(R.nsp + sp) * 9;
+    const double* id = R.idv + (size_t)sp * 9;
+    const double* fs = R.fsh + (size_t)sp * 9;
     const int nslot = NSLOT > 0 ? NSLOT : R.nslot;
     // acc = block0 x + ((c - cshift) / d^2) x: the own-row term of t2 folded into the start of the sum (x = d t1 is
     // the stored row), so that the row is dead after this and does not hold 18 registers through the neighbour loop
@@ -2130,7 +2134,7 @@ __global__ void __launch_bounds__(BLOCK) k_kpm_v2(RealDev R, const double* __res
       d2 xo[9];
       for (int l = 0; l < 9; ++l) xo[l] = row[(size_t)l * (nvp / 2)];
       for (int l = 0; l < 9; ++l) {
-        const double f = (cp[l] - cshift) / (dp[l] * dp[l]);
+        const double f = fs[l];
         d2 a = {f * xo[l].x, f * xo[l].y};
         for (int l2 = 0; l2 < 9; ++l2) {
           a.x = fma(blk[l * 9 + l2], xo[l2].x, a.x);
@@ -2160,8 +2164,8 @@ __global__ void __launch_bounds__(BLOCK) k_kpm_v2(RealDev R, const double* __res
       const uint64_t row = (uint64_t)site * 9 + l;
       const nt_d2 t0n = __builtin_nontemporal_load((const nt_d2*)(u0 + row * nvp + v));
       const d2 t0v = {t0n.x, t0n.y};
-      double t2x = alpha * (dp[l] * acc[l].x) - beta * (t0v.x / dp[l]);
-      double t2y = alpha * (dp[l] * acc[l].y) - beta * (t0v.y / dp[l]);
+      double t2x = alpha * (dp[l] * acc[l].x) - beta * (t0v.x * id[l]);
+      double t2y = alpha * (dp[l] * acc[l].y) - beta * (t0v.y * id[l]);
       if (v + 1 < nvec) {
         __builtin_nontemporal_store(nt_d2{dp[l] * t2x, dp[l] * t2y}, (nt_d2*)(u2 + row * nvp + v));
       } else if (v < nvec) {
@@ -2172,8 +2176,8 @@ __global__ void __launch_bounds__(BLOCK) k_kpm_v2(RealDev R, const double* __res
         const d2 xo = ((const d2*)(u1 + row * nvp + v))[0];
         dot.x = fma(t2x, t2x, dot.x);
         dot.y = fma(t2y, t2y, dot.y);
-        dot2.x = fma(t2x, xo.x / dp[l], dot2.x);
-        dot2.y = fma(t2y, xo.y / dp[l], dot2.y);
+        dot2.x = fma(t2x, xo.x * id[l], dot2.x);
+        dot2.y = fma(t2y, xo.y * id[l], dot2.y);
       } else {
         const double r0 = rademacher(seed, (uint64_t)v, row), r1 = rademacher(seed, (uint64_t)v + 1, row);
         dot.x = fma(r0, t2x, dot.x);
@@ -2213,6 +2217,7 @@ __global__ void __launch_bounds__(BLOCK) k_kpm_s16(RealDev R, int nvec, int nvp,
     const int sp = R.spec[site];
     const double* cp = R.par + (size_t)sp * 9;
     const double* dp = R.par + (size_t)(R.nsp + sp) * 9;
+    const double* id = R.idv + (size_t)sp * 9;
     d2 acc[9], xo[9];
     for (int l = 0; l < 9; ++l) acc[l] = d2{0.0, 0.0};
     const int* __restrict__ nbl = R.nbr + (size_t)site * NSLOT;
@@ -2242,9 +2247,9 @@ __global__ void __launch_bounds__(BLOCK) k_kpm_s16(RealDev R, int nvec, int nvp,
       const uint64_t row = (uint64_t)site * 9 + l;
       const double r0 = rademacher(seed, (uint64_t)v, row), r1 = rademacher(seed, (uint64_t)v + 1, row);
       const d2 t0v = ((const d2*)(u0 + row * nvp + v))[0];
-      const double t1x = xo[l].x / dp[l], t1y = xo[l].y / dp[l];
-      double t2x = alpha * (cp[l] * t1x + dp[l] * acc[l].x - cshift * t1x) - beta * (t0v.x / dp[l]);
-      double t2y = alpha * (cp[l] * t1y + dp[l] * acc[l].y - cshift * t1y) - beta * (t0v.y / dp[l]);
+      const double t1x = xo[l].x * id[l], t1y = xo[l].y * id[l];
+      double t2x = alpha * (cp[l] * t1x + dp[l] * acc[l].x - cshift * t1x) - beta * (t0v.x * id[l]);
+      double t2y = alpha * (cp[l] * t1y + dp[l] * acc[l].y - cshift * t1y) - beta * (t0v.y * id[l]);
       if (v < nvec) u2[row * nvp + v] = dp[l] * t2x;
       if (v + 1 < nvec) u2[row * nvp + v + 1] = dp[l] * t2y;
       dot.x = fma(r0, t2x, dot.x);
@@ -2431,6 +2436,9 @@ struct asr_gpu_real {
   int* nbr = nullptr;
   unsigned char* spec = nullptr;
   double *par = nullptr, *blk = nullptr;
+  double* tab = nullptr;       // 2 x nsp x 9: 1 / d, then (c - cshift) / d^2 of the last recurrence
+  std::vector<double> tab_h;   // its host copy
+  std::vector<double> par_h;   // par on the host
   hipStream_t stream = nullptr;
 };
 
@@ -4692,6 +4700,20 @@ int asr_gpu_profile_chains(asr_gpu_op* op, int nchain, const int* start_orb, con
 }
 
 // ------------------------------------------------------------------------------------------------ KPM
+// the per-species tables of the Chebyshev steps, 1 / d and (c - cshift) / d^2 for the shift cshift, copied to the device
+// (ordered after the handle's earlier work, the host copy read before the call returns)
+static int real_shift(asr_gpu_real* R, double cshift) {
+  const int n = R->nsp * 9;
+  for (int q = 0; q < n; ++q) {
+    const double c = R->par_h[q], d = R->par_h[n + q];
+    R->tab_h[q] = 1.0 / d;
+    R->tab_h[n + q] = (c - cshift) / (d * d);
+  }
+  HCHECK(hipMemcpyAsync(R->tab, R->tab_h.data(), (size_t)2 * n * 8, hipMemcpyHostToDevice, R->stream));
+  HCHECK(hipStreamSynchronize(R->stream));
+  return 0;
+}
+
 int asr_gpu_real_create(int nsite, int nslot, const int* nbr, int nsp, const signed char* species, const double* par,
                         const double* block, const double* block0, asr_gpu_real** out) {
   *out = nullptr;
@@ -4708,7 +4730,7 @@ int asr_gpu_real_create(int nsite, int nslot, const int* nbr, int nsp, const sig
   std::memcpy(blk.data(), block0, 81 * 8);
   std::memcpy(blk.data() + 81, block, (size_t)nslot * 81 * 8);
   if (dalloc(&R->nbr, (size_t)nsite * nslot) || dalloc(&R->spec, nsite) || dalloc(&R->par, (size_t)4 * nsp * 9) ||
-      dalloc(&R->blk, blk.size())) {
+      dalloc(&R->blk, blk.size()) || dalloc(&R->tab, (size_t)2 * nsp * 9)) {
     delete R;
     return 1;
   }
@@ -4720,13 +4742,18 @@ int asr_gpu_real_create(int nsite, int nslot, const int* nbr, int nsp, const sig
   HCHECK(hipMemcpy(R->spec, species, nsite, hipMemcpyHostToDevice));
   HCHECK(hipMemcpy(R->par, par, (size_t)4 * nsp * 9 * 8, hipMemcpyHostToDevice));
   HCHECK(hipMemcpy(R->blk, blk.data(), blk.size() * 8, hipMemcpyHostToDevice));
+  R->par_h.assign(par, par + (size_t)4 * nsp * 9);
+  R->tab_h.assign((size_t)2 * nsp * 9, 0.0);
+  for (int q = 0; q < nsp * 9; ++q)
+    if (!(par[nsp * 9 + q] != 0.0)) return fail("asr_gpu_real_create: d must be non-zero");
+  if (real_shift(R, 0.0)) return 1;
   *out = R;
   return 0;
 }
 
 void asr_gpu_real_destroy(asr_gpu_real* R) {
   if (!R) return;
-  for (void* p : {(void*)R->nbr, (void*)R->spec, (void*)R->par, (void*)R->blk})
+  for (void* p : {(void*)R->nbr, (void*)R->spec, (void*)R->par, (void*)R->blk, (void*)R->tab})
     if (p) (void)hipFree(p);
   if (R->stream) (void)hipStreamDestroy(R->stream);
   delete R;
@@ -4735,6 +4762,8 @@ void asr_gpu_real_destroy(asr_gpu_real* R) {
 static RealDev real_dev(const asr_gpu_real* R) {
   RealDev d;
   d.nsite = R->nsite; d.nslot = R->nslot; d.nsp = R->nsp; d.nbr = R->nbr; d.spec = R->spec; d.par = R->par; d.blk = R->blk;
+  d.idv = R->tab;
+  d.fsh = R->tab + (size_t)R->nsp * 9;
   return d;
 }
 
@@ -4841,6 +4870,7 @@ static int kpm_run(asr_gpu_real* R, double c, double s, int nmom, int nvec, uint
     if (ev) HCHECK(hipEventRecord(ev[q], R->stream));
     return 0;
   };
+  if (real_shift(R, c)) return 1;
   if (mark(0)) return 1;
   // t0 = r (stored as d o r), mu_0 = <r|r>/dim: the step with alpha = 0, beta = -1
   hipLaunchKernelGGL((k_kpm_u<true, 0>), grid, dim3(BLOCK), 0, R->stream, rd, (const double*)R->blk, nvec, nvp,
