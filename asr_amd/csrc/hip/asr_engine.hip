@@ -2193,6 +2193,102 @@ __global__ void __launch_bounds__(BLOCK) k_kpm_v2(RealDev R, const double* __res
   }
 }
 
+// The Chebyshev step with the block contraction on the matrix cores (FP64 MFMA 16x16x4): one wave per site and 64 vectors
+// (four 16-vector tiles).  The contraction Y = sum_k B_k X_nbr(k) over the on-site block and the NSLOT bond blocks is one
+// product of a 16 x 171 matrix (the nine output orbitals, padded; the 19 blocks side by side) by a 171 x 16 matrix per
+// tile (the gathered rows stacked), in 43 steps of k = 4: the A operand (blocks, the same for every site) from LDS, the
+// B operand one gathered value per lane (rows of the neighbour in slot (4 s + g) / 9, orbital (4 s + g) % 9 for the lane
+// group g = lane / 16), so a k-step mixes the end of one block with the start of the next and no k is padded but the last
+// one.  The MFMA result holds D[g + 4 r][lane % 16] in register r.  The own-row term ((c - cshift) / d^2) x, the
+// recurrence, the stores and the moment partials follow per lane for its rows (g, g + 4, and 8 in group 0), as k_kpm_v2.
+typedef double kd4 __attribute__((ext_vector_type(4)));
+template <int NSLOT, bool DBL>
+__global__ void __launch_bounds__(BLOCK) k_kpm_mf(RealDev R, const double* __restrict__ blk, int nvec, int nvp,
+                                                  const double* __restrict__ u1, const double* __restrict__ u0,
+                                                  double* __restrict__ u2, double cshift, double alpha, double beta,
+                                                  uint64_t seed, double* partial, double* partial2) {
+  constexpr int KT = (NSLOT + 1) * 9;  // 171 for 18 slots
+  constexpr int NKS = (KT + 3) / 4;    // 43 k-steps
+  __shared__ double at[NKS * WAVE];    // A operand of k-step s for lane l at s * 64 + l
+  __shared__ int roff[BLOCK / WAVE][NSLOT + 1];
+  __shared__ double red[BLOCK / WAVE][WAVE];
+  for (int q = threadIdx.x; q < NKS * WAVE; q += BLOCK) {
+    const int st = q / WAVE, l = q % WAVE, row = l % 16, kk = 4 * st + l / 16;
+    at[q] = (row < 9 && kk < KT) ? blk[(kk / 9) * 81 + row * 9 + kk % 9] : 0.0;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4, col = lane & 15;
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int site = lb * (BLOCK / WAVE) + wave;
+  const bool live = site < R.nsite;
+  if (lane <= NSLOT) roff[wave][lane] = live ? (lane == 0 ? site : R.nbr[(size_t)site * R.nslot + lane - 1]) * 9 : 0;
+  __syncthreads();
+  const int v0 = blockIdx.y * 64;
+  double dsum[4] = {0.0, 0.0, 0.0, 0.0}, dsum2[4] = {0.0, 0.0, 0.0, 0.0};
+  if (live) {
+    kd4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = kd4{0.0, 0.0, 0.0, 0.0};
+    const double* __restrict__ xb = u1 + v0 + col;
+#pragma unroll 4
+    for (int st = 0; st < NKS; ++st) {
+      const int kk = 4 * st + g;
+      const bool ok = kk < KT;
+      const size_t r = (size_t)(roff[wave][ok ? kk / 9 : 0] + (ok ? kk % 9 : 0)) * nvp;
+      const double a = at[st * WAVE + lane];
+      double b[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) b[t] = ok ? xb[r + 16 * t] : 0.0;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[t], acc[t], 0, 0, 0);
+    }
+    const int sp = R.spec[site];
+    const double* dp = R.par + (size_t)(R.nsp + sp) * 9;
+    const double* id = R.idv + (size_t)sp * 9;
+    const double* fs = R.fsh + (size_t)sp * 9;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int v = v0 + 16 * t + col;
+#pragma unroll
+      for (int rr = 0; rr < 3; ++rr) {
+        const int l = g + 4 * rr;
+        if (l >= 9 || v >= nvec) continue;
+        const uint64_t row = (uint64_t)site * 9 + l;
+        const double xo = u1[row * nvp + v];
+        const double hsum = acc[t][rr] + fs[l] * xo;
+        const double t2 = alpha * (dp[l] * hsum) - beta * (__builtin_nontemporal_load(u0 + row * nvp + v) * id[l]);
+        __builtin_nontemporal_store(dp[l] * t2, u2 + row * nvp + v);
+        if constexpr (DBL) {
+          dsum[t] = fma(t2, t2, dsum[t]);
+          dsum2[t] = fma(t2, xo * id[l], dsum2[t]);
+        } else {
+          dsum[t] = fma(rademacher(seed, (uint64_t)v, row), t2, dsum[t]);
+        }
+      }
+    }
+  }
+  // per vector: the four lane groups, then the four waves (sites) of the block in order
+  auto reduce = [&](double (&ds)[4], double* out) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      double vsum = ds[t];
+      vsum += __shfl_xor(vsum, 16);
+      vsum += __shfl_xor(vsum, 32);
+      if (g == 0) red[wave][16 * t + col] = vsum;
+    }
+    __syncthreads();
+    if (threadIdx.x < WAVE) {
+      double s2 = red[0][threadIdx.x];
+      for (int w = 1; w < BLOCK / WAVE; ++w) s2 += red[w][threadIdx.x];
+      out[((size_t)blockIdx.y * gridDim.x + lb) * 64 + threadIdx.x] = s2;
+    }
+  };
+  reduce(dsum, partial);
+  if constexpr (DBL) {
+    __syncthreads();
+    reduce(dsum2, partial2);
+  }
+}
+
 // The Chebyshev step on vector slices: the 64-vector rows are processed as slices of 16 vectors (one 128-byte line per
 // site and orbital), 8 lanes per site with two vectors each, 8 sites per wave; blockIdx.y is the slice, so one slice of
 // all sites is swept before the next and the slice of the input (a quarter of the vector) stays in the last-level caches
@@ -2433,6 +2529,7 @@ struct asr_gpu_real {
   bool v2 = true;    // Chebyshev steps with two vectors per lane
   bool s16 = false;  // ... on 16-vector slices (8 sites per wave)
   bool dbl = false;  // kpm_moments by moment doubling (two moments per Chebyshev step)
+  bool mf = false;   // ... the steps with the block contraction on the matrix cores (k_kpm_mf, 18 slots)
   int* nbr = nullptr;
   unsigned char* spec = nullptr;
   double *par = nullptr, *blk = nullptr;
@@ -4801,10 +4898,11 @@ int asr_gpu_real_set_doubling(asr_gpu_real* R, int on) {
 // Vector tile of the KPM step (vectors per site in a wave: 8, 16, 32 or 64; 0 = automatic).
 int asr_gpu_real_set_tile(asr_gpu_real* R, int vt) {
   if (!R) return fail("asr_gpu_real_set_tile: null handle");
-  if (vt != 0 && vt != 1 && vt != 2 && vt != 3)
-    return fail("asr_gpu_real_set_tile: 0 (automatic), 1 or 2 vectors per lane, 3 = 16-vector slices");
+  if (vt < 0 || vt > 4)
+    return fail("asr_gpu_real_set_tile: 0 (automatic), 1 or 2 vectors per lane, 3 = 16-vector slices, 4 = MFMA contraction");
   R->v2 = (vt != 1);
   R->s16 = (vt == 3);
+  R->mf = (vt == 4);
   return 0;
 }
 
@@ -4845,7 +4943,7 @@ static KpmSizes kpm_sizes(const asr_gpu_real* R, int nmom, int nvec) {
   KpmSizes z;
   z.vec = dim * nvp;
   z.part = (size_t)grid.x * grid.y * vt;
-  z.part2 = R->dbl ? (size_t)((R->nsite + 7) / 8) * grid.y * 64 : 0;
+  z.part2 = R->dbl ? (size_t)((R->nsite + 3) / 4) * grid.y * 64 : 0;
   z.mper = (size_t)nvec * nmom;
   return z;
 }
@@ -4892,16 +4990,22 @@ static int kpm_run(asr_gpu_real* R, double c, double s, int nmom, int nvec, uint
                          nbx2, vt, nvec, inv_dim, mper, nmom, 1);
     }
     const int M = nmom / 2;  // the largest m with 2 m - 1 <= nmom - 1
+    const bool mfd = R->mf && R->nslot == 18;
+    const int nbxm = mfd ? (int)((R->nsite + 3) / 4) : nbx2;
     for (int m = 2; m <= M; ++m) {
       if (mark(2 + 2 * (m - 2))) return 1;
-      if (R->nslot == 18)
+      if (mfd)
+        hipLaunchKernelGGL((k_kpm_mf<18, true>), dim3((unsigned)nbxm, grid.y), dim3(BLOCK), 0, R->stream, rd,
+                           (const double*)R->blk, nvec, nvp, (const double*)t1, (const double*)t0, t2, c, 2.0 / s, 1.0, seed,
+                           part, part2);
+      else if (R->nslot == 18)
         hipLaunchKernelGGL((k_kpm_v2<18, true>), g2, dim3(BLOCK), 0, R->stream, rd, (const double*)R->blk, nvec, nvp,
                            (const double*)t1, (const double*)t0, t2, c, 2.0 / s, 1.0, seed, part, part2);
       else
         hipLaunchKernelGGL((k_kpm_v2<0, true>), g2, dim3(BLOCK), 0, R->stream, rd, (const double*)R->blk, nvec, nvp,
                            (const double*)t1, (const double*)t0, t2, c, 2.0 / s, 1.0, seed, part, part2);
       if (mark(3 + 2 * (m - 2))) return 1;
-      hipLaunchKernelGGL(k_kpm_fin2, dim3(nvec), dim3(BLOCK), 0, R->stream, (const double*)part, (const double*)part2, nbx2,
+      hipLaunchKernelGGL(k_kpm_fin2, dim3(nvec), dim3(BLOCK), 0, R->stream, (const double*)part, (const double*)part2, nbxm,
                          vt, nvec, inv_dim, mper, nmom, m);
       double* tmp = t0; t0 = t1; t1 = t2; t2 = tmp;
     }
@@ -4909,7 +5013,11 @@ static int kpm_run(asr_gpu_real* R, double c, double s, int nmom, int nvec, uint
   for (int n = 2; n < nmom && !R->dbl; ++n) {
     // t2 = 2 (H t1 - c t1) / s - t0
     if (mark(2 + 2 * (n - 2))) return 1;
-    if (R->s16 && R->nslot == 18)
+    if (R->mf && R->nslot == 18)
+      hipLaunchKernelGGL((k_kpm_mf<18, false>), dim3((R->nsite + 3) / 4, grid.y), dim3(BLOCK), 0, R->stream, rd,
+                         (const double*)R->blk, nvec, nvp, (const double*)t1, (const double*)t0, t2, c, 2.0 / s, 1.0, seed,
+                         part, (double*)nullptr);
+    else if (R->s16 && R->nslot == 18)
       hipLaunchKernelGGL((k_kpm_s16<18>), dim3((R->nsite + 31) / 32, nvp / 16), dim3(BLOCK), (size_t)19 * 90 * 8, R->stream, rd,
                          nvec, nvp, (const double*)t1, (const double*)t0, t2, c, 2.0 / s, 1.0, seed, part);
     else if (R->v2 && R->nslot == 18)
@@ -4927,7 +5035,9 @@ static int kpm_run(asr_gpu_real* R, double c, double s, int nmom, int nvec, uint
       hipLaunchKernelGGL((k_kpm_u<false, 0>), grid, dim3(BLOCK), 0, R->stream, rd, (const double*)R->blk, nvec, nvp,
                          (const double*)t1, (const double*)t0, t2, c, 2.0 / s, 1.0, seed, part, 1);
     if (mark(3 + 2 * (n - 2))) return 1;
-    const int nbx = (R->s16 && R->nslot == 18) ? (int)((R->nsite + 31) / 32) : R->v2 ? (int)((R->nsite + 7) / 8) : (int)grid.x;
+    const int nbx = (R->mf && R->nslot == 18) ? (int)((R->nsite + 3) / 4)
+                    : (R->s16 && R->nslot == 18) ? (int)((R->nsite + 31) / 32)
+                    : R->v2 ? (int)((R->nsite + 7) / 8) : (int)grid.x;
     hipLaunchKernelGGL(k_kpm_fin, dim3(nvec), dim3(BLOCK), 0, R->stream, part, nbx, vt, nvec, inv_dim, mper, nmom, n);
     double* tmp = t0; t0 = t1; t1 = t2; t2 = tmp;
   }

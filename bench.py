@@ -431,7 +431,9 @@ def main():
         marg = 0.01 * (ehi - elo)
         cshift, sscale = 0.5 * (elo + ehi), 0.5 * (ehi - elo + 2.0 * marg) / 0.99
         tiles = {}
-        for vt in (1, 2, 3):  # one or two vectors per lane on 64-vector rows, or 16-vector slices; tuned on short runs
+        # one or two vectors per lane on 64-vector rows, 16-vector slices, or the block contraction on the FP64 matrix
+        # cores; tuned on short runs
+        for vt in (1, 2, 3, 4):
             R.set_tile(vt)
             tiles[vt] = R.kpm_moments(cshift, sscale, 12, 64, seed=5)["step_ms"]
         vt_best = min(tiles, key=tiles.get)

@@ -191,7 +191,9 @@ int asr_gpu_profile_chains(asr_gpu_op* op, int nchain, const int* start_orb, con
 int asr_gpu_real_create(int nsite, int nslot, const int* nbr, int nsp, const signed char* species, const double* par,
                         const double* block, const double* block0, asr_gpu_real** out);
 void asr_gpu_real_destroy(asr_gpu_real* r);
-/* Vectors per lane in the Chebyshev steps: 1, 2, or 0 = automatic (2: 16-byte loads, two sites per wave). */
+/* Form of the Chebyshev steps: 1 or 2 vectors per lane on 64-vector rows (2: 16-byte loads, two sites per wave), 3 = 16-vector
+ * slices, 4 = the block contraction on the FP64 matrix cores (one wave per site, 18 neighbour slots; other slot counts take
+ * form 2), or 0 = automatic (2). */
 int asr_gpu_real_set_tile(asr_gpu_real* r, int vt);
 /* Moment doubling for asr_gpu_kpm_moments (on != 0): the Chebyshev step that makes t_m = T_m(H~) r also gives
  * mu_2m = 2 <t_m|t_m> / dim - mu_0 and mu_2m-1 = 2 <t_m|t_m-1> / dim - mu_1 (T_2m = 2 T_m^2 - T_0, T_2m-1 = 2 T_m T_m-1 - T_1),

@@ -25,7 +25,7 @@ def test_gpu_kpm_reproduces_exact_sc_spectrum():
     eng = _engine()
     g, spec, par, block, block0 = _sc_band(12)
     R = eng.GpuRealization(g.nbr, spec, par, block, block0)
-    for vt, dbl in ((1, False), (2, False), (2, True)):
+    for vt, dbl in ((1, False), (2, False), (2, True), (4, False), (4, True)):
         R.set_tile(vt)
         R.set_doubling(dbl)  # moment doubling: two moments per Chebyshev step
         res = R.kpm_moments(0.0, 6.3, 400, 60, seed=5)

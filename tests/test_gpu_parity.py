@@ -134,9 +134,10 @@ def test_edc_plain_equals_batched(ternary_small):
     assert rel(a1, a) < 1e-13 and rel(b1, b) < 1e-13
 
 
-@pytest.mark.parametrize("tile", [1, 2, 3])
+@pytest.mark.parametrize("tile", [1, 2, 3, 4])
 def test_kpm_apply_and_moments(tile):
-    """Chebyshev steps with one or two vectors per lane (64-vector rows) or on 16-vector slices, against the oracle."""
+    """Chebyshev steps with one or two vectors per lane (64-vector rows), on 16-vector slices or with the block contraction
+    on the FP64 matrix cores (tile 4), against the oracle."""
     eng = _engine()
     from oracle.oracle import OracleReal
     g, species, par, block, block0 = W.kpm_supercell(L=3, seed=21)

@@ -171,14 +171,16 @@ def test_gpu_realisation_equals_dense(L, tile):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("tile", [2, 4])
 @pytest.mark.parametrize("doubling", [False, True])
-def test_gpu_moments_equal_exact(doubling):
+def test_gpu_moments_equal_exact(doubling, tile):
     eng = _engine()
     g, species, par, block, block0 = supercell(2, 7)
     h = dense_h(g.nbr, species, par, block, block0)
     lam = np.linalg.eigvalsh(h)
     c, s = 0.5 * (lam[0] + lam[-1]), 0.55 * (lam[-1] - lam[0])
     R = eng.GpuRealization(g.nbr, species, par, block, block0)
+    R.set_tile(tile)
     R.set_doubling(doubling)
     ex = exact_moments(h, c, s, 40, 5, 9)
     res = R.kpm_moments(c, s, 40, 5, 9, per_vector=True)
