@@ -433,9 +433,9 @@ __device__ __forceinline__ T tile_row(__amdgpu_buffer_rsrc_t xr, uint32_t voff, 
   }
 }
 
-// acc += coef X(x~) for the members of a tile (apply_rand with pi = identity): the node's own state and its flip partners, the
-// member's local weights, the partner's sign chi(g) sg(g, l) from the LDS table cs (member x operation x orbital), raw
-// buffer loads for the rows.  LIN = false: X = X_q with the real weights wt(q, class, l0, k, l).  LIN = true: the
+// acc += coef X(x~) for the members of a tile (apply_rand): the node's own state and its flip partners, the member's local
+// weights, the partner's sign chi(g) sg(g, l) from the LDS table cs (member x operation x orbital) and its row pi(g, l),
+// raw buffer loads for the rows.  LIN = false: X = X_q with the real weights wt(q, class, l0, k, l).  LIN = true: the
 // energy-dependent diagonal X(z) = z X(q) + X(q + 1) with the complex weight z wt(q) + wt(q + 1) (one pass, one accumulator).
 // Frozen nodes: the frozen species' parameters on the node's own row.
 template <class T, bool LIN>
@@ -482,10 +482,13 @@ __device__ __forceinline__ void rand_ct(const OpDev& op, const MemDev& md, const
     const int r = (int)(e >> 6) - 1, g = (int)(e & 63u);
     const double isr = xnorm ? op.isq[r] : 1.0;
     const double* __restrict__ cg = cs + g * 9;
+    const int* __restrict__ pg = op.pi + g * 9;
     const uint32_t voff = (uint32_t)r * node16 + coff;
+    // orbital l of the partner's image is its row pi(g, l) (the identity for an orbital-preserving group)
 #pragma unroll
     for (int l = 0; l < 9; ++l)
-      add(wa_[((size_t)k * 9 + l) * nbp], LIN ? wb_[((size_t)k * 9 + l) * nbp] : 0.0, (isr * cg[l]) * row(voff, l), l);
+      add(wa_[((size_t)k * 9 + l) * nbp], LIN ? wb_[((size_t)k * 9 + l) * nbp] : 0.0,
+          (isr * cg[l]) * tile_row<T>(xr, voff + (uint32_t)pg[l] * vs16, 0u), l);
   }
 }
 
@@ -507,10 +510,21 @@ __device__ __forceinline__ cplx nt_load(const cplx* p) {
 __device__ __forceinline__ double nt_load(const double* p) { return __builtin_nontemporal_load(p); }
 
 // O2: the out2 form compiled in (0 none, 1 c2 X_q2 out, 2 the two energy-linear terms of route ed3's t pass)
-template <class T, int MT, int NSLOT, int O2>
+// PI: every operation of the group maps each orbital onto itself (pi = identity, e.g. D2h); otherwise (e.g. the D4h of
+// class_ops) orbital l2 of a gathered image is row pi(g, l2) of the neighbour, whose nine 4-bit row indices per operation
+// come packed from LDS, and the gather scale carries d of that row.
+template <class T, int MT, int NSLOT, int O2, bool PI>
 __global__ void __launch_bounds__(BLOCK, 3) k_tile(OpDev op, MemDev md, FArgs fa, const double* __restrict__ sblk) {
   extern __shared__ double bls[];
   __shared__ T red[BLOCK];
+  __shared__ unsigned long long pik[PI ? 1 : 64];
+  if constexpr (!PI) {
+    for (int g = threadIdx.x; g < op.nop; g += BLOCK) {
+      unsigned long long pk = 0ull;
+      for (int l = 0; l < 9; ++l) pk |= (unsigned long long)op.pi[g * 9 + l] << (4 * l);
+      pik[g] = pk;
+    }
+  }
   constexpr uint32_t ES = sizeof(T);
   if (md.tact && !md.tact[blockIdx.y]) return;  // every member of this tile has left
   // LDS: the padded bond blocks, then the gather scales e[member][g][l] = dd(l) chi(g) sg(g, l) of the tile's members
@@ -530,7 +544,9 @@ __global__ void __launch_bounds__(BLOCK, 3) k_tile(OpDev op, MemDev md, FArgs fa
     for (int q = threadIdx.x; q < ne; q += BLOCK) {
       const int ml = q / (ng * 9), g = (q / 9) % ng, l = q % 9;
       const int mm = min(blockIdx.y * MT + ml, md.nb - 1);
-      const double d = fa.s_din ? md.dvec[(size_t)l * md.nbp + mm] : 1.0;
+      const double d = !fa.s_din ? 1.0
+                       : (PI || g == op.nop) ? md.dvec[(size_t)l * md.nbp + mm]
+                                             : md.dperm[((size_t)g * 9 + l) * md.nbp + mm];
       const double chi = (g < op.nop && ((md.chimask[mm] >> g) & 1ull)) ? -1.0 : 1.0;
       etab[q] = d * chi * (g < op.nop ? op.sg[g * 9 + l] : 1.0);
     }
@@ -571,6 +587,7 @@ __global__ void __launch_bounds__(BLOCK, 3) k_tile(OpDev op, MemDev md, FArgs fa
         double q = op.isq[i];
         uint32_t en = hp[0];  // the link of slot k + 1, read one slot ahead
         uint32_t voff = (uint32_t)i * node16 + coff;
+        unsigned long long pm = 0ull;  // !PI: the packed rows pi(g, .) of the slot whose rows are being gathered
         T v[9];
 #pragma unroll
         for (int l = 0; l < 9; ++l) {
@@ -591,6 +608,7 @@ __global__ void __launch_bounds__(BLOCK, 3) k_tile(OpDev op, MemDev md, FArgs fa
             voff = (uint32_t)r * node16 + coff;
             q = op.isq[r];
             en = hp[k + 1 < NSLOT ? k + 1 : k];
+            if constexpr (!PI) pm = pik[g];
           }
 #pragma unroll
           for (int l2 = 0; l2 < 9; ++l2) {
@@ -607,7 +625,10 @@ __global__ void __launch_bounds__(BLOCK, 3) k_tile(OpDev op, MemDev md, FArgs fa
             const T x = (sck * ec) * v[l2];
 #pragma unroll
             for (int l = 0; l < 9; ++l) acc[l] = fma_s(bc[l], x, acc[l]);
-            if constexpr (decltype(more)::value) v[l2] = tile_row<T>(xr, voff, l2 * vs16);
+            if constexpr (decltype(more)::value) {
+              if constexpr (PI) v[l2] = tile_row<T>(xr, voff, l2 * vs16);
+              else v[l2] = tile_row<T>(xr, voff + (uint32_t)((pm >> (4 * l2)) & 15ull) * vs16, 0u);
+            }
             if (l2 < 8 || decltype(more)::value) {
 #pragma unroll
               for (int l = 0; l < 9; ++l) bc[l] = bn[l];
@@ -690,7 +711,7 @@ __global__ void __launch_bounds__(BLOCK, 3) k_tile(OpDev op, MemDev md, FArgs fa
           if (fa.proj) {
             const int s0 = op.stab_ptr[i], s1 = op.stab_ptr[i + 1];
             ns = s1 - s0;
-            if (ns > 1) {
+            if (PI && ns > 1) {
               const double* __restrict__ cs = cst + (size_t)ml * op.nop * 9;
               for (int l = 0; l < 9; ++l) f[l] = 0.0;
               for (int j = s0; j < s1; ++j) {
@@ -702,14 +723,52 @@ __global__ void __launch_bounds__(BLOCK, 3) k_tile(OpDev op, MemDev md, FArgs fa
           const cplx* uo = (const cplx*)fa.uo;
           const cplx* up = (const cplx*)fa.upv;
           cplx ww = zero<cplx>(), uy = zero<cplx>();
-          for (int l = 0; l < 9; ++l) {
-            const size_t q = ((size_t)i * 9 + l) * nbp + m;
-            const cplx y = so * acc[l], uq = uo[q];
-            uy = uy + bilin(uq, y);
-            cplx w = off ? zero<cplx>() : y - bilin(a, uq) - bilin(b, NTO ? nt_load(&up[q]) : up[q]);
-            if (ns > 1) w = (f[l] / (double)ns) * w;
-            out_store<NTO>(&out[q], w);
-            ww = ww + bilin(w, w);
+          if constexpr (PI) {
+            for (int l = 0; l < 9; ++l) {
+              const size_t q = ((size_t)i * 9 + l) * nbp + m;
+              const cplx y = so * acc[l], uq = uo[q];
+              uy = uy + bilin(uq, y);
+              cplx w = off ? zero<cplx>() : y - bilin(a, uq) - bilin(b, NTO ? nt_load(&up[q]) : up[q]);
+              if (ns > 1) w = (f[l] / (double)ns) * w;
+              out_store<NTO>(&out[q], w);
+              ww = ww + bilin(w, w);
+            }
+          } else {
+            // (P w)_l = sum_{g in S} chi(g) sg(g, l) w_pi(g, l) / |S|: the nine w of the node first (in acc), then each
+            // w_pi from the registers (a run-time index into a register array would live in scratch memory)
+#pragma unroll
+            for (int l = 0; l < 9; ++l) {
+              const size_t q = ((size_t)i * 9 + l) * nbp + m;
+              const cplx y = so * acc[l], uq = uo[q];
+              uy = uy + bilin(uq, y);
+              acc[l] = off ? zero<cplx>() : y - bilin(a, uq) - bilin(b, NTO ? nt_load(&up[q]) : up[q]);
+            }
+            if (ns > 1) {
+              // w_p as a sum of the nine values weighted 1 or 0 (a select chain or an index becomes an indexed load from
+              // scratch memory); exact for finite values
+              const double* __restrict__ cs = cst + (size_t)ml * op.nop * 9;
+              const int s0 = op.stab_ptr[i];
+#pragma unroll 1
+              for (int l = 0; l < 9; ++l) {
+                cplx sacc = zero<cplx>();
+                for (int j = s0; j < s0 + ns; ++j) {
+                  const int g = op.stab[j], p = op.pi[g * 9 + l];
+                  cplx wv = zero<cplx>();
+#pragma unroll
+                  for (int l2 = 0; l2 < 9; ++l2) wv = fma_s(p == l2 ? 1.0 : 0.0, acc[l2], wv);
+                  sacc = sacc + cs[g * 9 + l] * wv;
+                }
+                const cplx w = (1.0 / (double)ns) * sacc;
+                out_store<NTO>(&out[((size_t)i * 9 + l) * nbp + m], w);
+                ww = ww + bilin(w, w);
+              }
+            } else {
+#pragma unroll
+              for (int l = 0; l < 9; ++l) {
+                out_store<NTO>(&out[((size_t)i * 9 + l) * nbp + m], acc[l]);
+                ww = ww + bilin(acc[l], acc[l]);
+              }
+            }
           }
           dotv = ww;
           dotv2 = uy;
@@ -811,7 +870,7 @@ __device__ inline void rand_node_term(const OpDev& op, const MemDev& md, int i, 
     const double chi = ((chim >> g) & 1ull) ? -1.0 : 1.0;
     for (int l = 0; l < 9; ++l)
       add(wa_[((size_t)k * 9 + l) * nbp], lin ? wb_[((size_t)k * 9 + l) * nbp] : 0.0,
-          (isr * (chi * op.sg[g * 9 + l])) * x[((size_t)r * 9 + l) * nbp + m], l);
+          (isr * (chi * op.sg[g * 9 + l])) * x[((size_t)r * 9 + op.pi[g * 9 + l]) * nbp + m], l);
   }
 }
 
@@ -2810,9 +2869,9 @@ bool buffer_ok(const asr_gpu_op* op, const MemDev& md, size_t es) {
   return (size_t)(op->nn + 1) * 9 * md.nbp * es < ((size_t)1 << 32);
 }
 
-// the operator pass runs k_tile (member tile 8 or 16, orbital-preserving group, 14 or 18 slots, vectors under 4 GiB)
+// the operator pass runs k_tile (member tile 8 or 16, 14 or 18 slots, at most 64 operations, vectors under 4 GiB)
 bool tile_kernel(const asr_gpu_op* op, const MemDev& md, size_t es) {
-  return !(md.mt == WAVE) && (md.mt == 8 || md.mt == 16) && op->piid && (op->nslot == 18 || op->nslot == 14) &&
+  return !(md.mt == WAVE) && (md.mt == 8 || md.mt == 16) && (op->nslot == 18 || op->nslot == 14) && op->nop <= 64 &&
          buffer_ok(op, md, es);
 }
 
@@ -2851,9 +2910,10 @@ int launch_fused(const asr_gpu_op* op, const MemDev& md, const FArgs& fa, hipStr
     else ASR_NODE(false, 0);
 #undef ASR_NODE
   } else if (tile_kernel(op, md, sizeof(T))) {
-#define ASR_CT3(MT, NS, O2) hipLaunchKernelGGL((k_tile<T, MT, NS, O2>), grid, dim3(BLOCK), \
+#define ASR_CT4(MT, NS, O2, PI) hipLaunchKernelGGL((k_tile<T, MT, NS, O2, PI>), grid, dim3(BLOCK), \
                                                 blds_bytes + (size_t)MT * (2 * op->nop + 1) * 9 * 8, strm, od, md, fa, \
                                                 (const double*)op->blkp + (size_t)op->ntype * (op->nslot + 1) * 90)
+#define ASR_CT3(MT, NS, O2) do { if (op->piid) ASR_CT4(MT, NS, O2, true); else ASR_CT4(MT, NS, O2, false); } while (0)
 #define ASR_CT(MT, NS) do { if (!fa.out2) ASR_CT3(MT, NS, 0); else if (!fa.x2) ASR_CT3(MT, NS, 1); \
                             else ASR_CT3(MT, NS, 2); } while (0)
     if (md.mt == 8 && op->nslot == 18) ASR_CT(8, 18);
@@ -2862,6 +2922,7 @@ int launch_fused(const asr_gpu_op* op, const MemDev& md, const FArgs& fa, hipStr
     else ASR_CT(16, 14);
 #undef ASR_CT
 #undef ASR_CT3
+#undef ASR_CT4
   } else {
     hipLaunchKernelGGL(k_fused<T>, grid, dim3(BLOCK), lds, strm, dev_of(op), md, fa);
   }

@@ -182,6 +182,7 @@ def main():
     ap.add_argument("--no-l21", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-ed3", action="store_true")
+    ap.add_argument("--no-d4h", action="store_true")
     ap.add_argument("--no-epi", action="store_true")
     ap.add_argument("--no-spectral", action="store_true")
     ap.add_argument("--no-scf", action="store_true")
@@ -304,6 +305,28 @@ def main():
         out["speedup_batched_vs_plain"] = round(value / plain, 2)
         out["plain_matches_batched"] = bool(np.max(np.abs(a1 - a)) <= 1e-12 * np.max(np.abs(a)))
         log(f"plain: {wall1 * 1e3:.0f} ms, {plain:.0f} chain-levels/s, batched/plain {value / plain:.2f}x")
+
+    if not args.no_d4h:
+        # the reference's own point group (class_ops' D4h, which swaps p_x / p_y and d_xz / d_yz): the same fcc ternary
+        # reduced by D4h (about half the nodes of D2h), the 30 chains whose start orbital is one-dimensional under the
+        # group (2 spins x 3 central species x s, p_z, d_z2, d_xy, d_x2-y2), depth 30; batched and plain
+        wd = W.ternary_edc(L=L, levels=levels, depth=depth, group="d4h")
+        md4 = [m for m in wd.members if m.l0 in (0, 2, 4, 6, 8)]
+        G4 = engine.GpuOperator(wd.op)
+        G4.chains_edc(md4, depth)  # warm-up (autotuning)
+        res = {}
+        for mode, name in ((0, "batched"), (1, "plain")):
+            t0 = time.perf_counter()
+            a4, b4, mu4, ms4 = G4.chains_edc(md4, depth, mode=mode)
+            wall = time.perf_counter() - t0
+            res[name] = {"value": round(len(md4) * depth / wall, 1), "wall_ms": round(wall * 1e3, 1),
+                         "device_ms": round(ms4, 1)}
+        out["d4h"] = {"workload": "fcc ternary, route edc, levels 4, D4h (orbital-permuting), 30 chains x 30",
+                      "nodes": wd.op.nn, "chains": len(md4), "depth": depth, "unit": "chain-levels/s", **res,
+                      "member_tile": G4.tuned_tile(len(md4), True),
+                      "speedup_batched_vs_plain": round(res["batched"]["value"] / res["plain"]["value"], 2)}
+        log(f"d4h: {out['d4h']}")
+        G4.close()
 
     if not args.no_ed3:
         # route ed3 on the same operator: 54 (spin, central species, start orbital) blocks, two chains each, depth 30

@@ -1,8 +1,10 @@
 """GPU parity on the reference's own point group and the reduced-against-unreduced identity on the device.
 
 * D4h (class_ops default, src/asr_symmetry.f90:67-92) permutes p_x / p_y and d_xz / d_yz, so pi is not the identity:
-  the operator passes take their orbital-permuting forms (k_fused at member tiles 8-32, k_node<T, false> at 64, k_pair<false>
-  at 65) and the update its general projector.  Chains and applications against the oracle at tiles 8, 16, 64 and 65.
+  the operator passes take their orbital-permuting forms (k_tile<.., PI = false> at member tiles 8 and 16, with the fused
+  level's projection over the stabiliser in registers; k_node<T, false> at 64, k_pair<false> at 65, with the update's
+  general projector).  Chains and applications against the oracle at tiles 8, 16, 64 and 65; the fused level against the
+  separate update pass.
 * On the GPU alone: the D4h- and D2h-reduced chains equal the chains of the unreduced operator (identity group), and the
   144-step chain of the reduced ordered B2 cell equals the dense solve on the unreduced one, 1e-11
   (test/test_augcls_project.f90:100-150).
@@ -59,6 +61,26 @@ def test_d4h_edc_chains_every_tile(d4h_ternary, tile):
             assert mu[j] == n
             assert rel(a[j, :n], ar[:n]) < 1e-9, f"chain {j} project {project}"
             assert rel(b[j, :n - 1], br[:n - 1]) < 1e-9, f"chain {j} project {project}"
+
+
+@pytest.mark.parametrize("tile", [8, 16])
+def test_d4h_fused_level_against_separate_pass(d4h_ternary, tile):
+    """The fused level on the orbital-permuting group (the stabiliser projection sum_g chi sg w_pi(g, l) / |S| on the
+    registers of the second operator pass) equals the separate update pass (k_update's general projector) to rounding,
+    and is the path taken."""
+    eng = _engine()
+    wl, members = d4h_ternary
+    G = eng.GpuOperator(wl.op)
+    G.set_tile(tile)
+    out = {}
+    for form in (0, 1):
+        G.set_level_form(form)
+        out[form] = G.chains_edc(members, 14)
+    assert np.array_equal(out[0][2], out[1][2])
+    assert rel(out[0][0], out[1][0]) < 1e-12 and rel(out[0][1], out[1][1]) < 1e-12
+    G.set_level_form(0)
+    kinds = list(G.profile(members, 4)[1])
+    assert 7 in kinds and 3 not in kinds, kinds
 
 
 @pytest.mark.parametrize("tile", [8, 16, 64])
