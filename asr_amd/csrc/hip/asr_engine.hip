@@ -141,6 +141,7 @@ struct MemDev {
   const int* tact;            // per member tile: 0 = every member has left (the tile's blocks return at once), or null
   const double* mtm;          // ncls x nsmax x nsmax x nsmax x nbp: class tables per member
   const int* fsp;             // 2 x nbp: species of the frozen start site and of the frozen partner site (-1: none)
+  int ntl;                    // member tiles launched (the leading ones; 0 = all): after a compaction the running tiles
 };
 
 // The frozen-site slot of node info inf for member m: 1 (the start site), 2 (a frozen pair partner, augcls_freeze_any)
@@ -1437,6 +1438,7 @@ struct ChainDev {
   double tiny;
   void* cvec;        // route ed3: c_k = <t|u_k> (nbp x ld, the chain's element type) or null
   int* closed;       // route ed3: 1 where b_k < tiny ended the chain (Krylov space exhausted) or null
+  const int* s0;     // window runs: the level at which the member of each slot started (its level is level - s0), or null
 };
 
 template <class T> __device__ inline double mag(T v);
@@ -1512,11 +1514,45 @@ __global__ void __launch_bounds__(BLOCK) k_adj_fin_grad(int nb, int nq, const do
 
 // sc (lagged normalisation): the chain vector is stored as s_k u_k, so the partials are those of s_k^2 <u, H u>
 // lpart (the fused edc level): nlist further partials per member (stride nbp) added after the block partials
+// A member that the device depth test of the previous level found converged (conv, set beside that level's update) leaves
+// now: the b_k (and route ed3's c_k+1) that the previous level recorded for it are taken back, as if it had left before
+// that level's update (the test ran on its own stream, off the level's critical path).
+template <class T>
+__device__ inline void apply_conv(const ChainDev& cd, int* conv, T* b, int m) {
+  conv[m] = 0;
+  cd.active[m] = 0;
+  const int kp = *cd.level - (cd.s0 ? cd.s0[m] : 0) - 1;
+  if (kp >= 0) {
+    b[(size_t)m * cd.ld + kp] = zero<T>();
+    if (cd.cvec && kp + 1 < cd.ld) ((T*)cd.cvec)[(size_t)m * cd.ld + kp + 1] = zero<T>();
+  }
+  if (cd.closed) cd.closed[m] = 0;
+}
+
+template <class T>
+__global__ void k_conv_apply(ChainDev cd, int nb, int* conv, T* b) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m < nb && conv[m]) apply_conv<T>(cd, conv, b, m);
+}
+
 template <class T>
 __global__ void __launch_bounds__(BLOCK) k_fin_a(ChainDev cd, int nb, const T* part, int nbx, int mt, T* a, T* a_eff,
-                                                  const T* sc, const T* lpart, int nlist, int nbp) {
+                                                  const T* sc, const T* lpart, int nlist, int nbp, int* conv, T* bk) {
   __shared__ T lds[BLOCK / WAVE];
   const int m = blockIdx.x;
+  int act = cd.active[m];
+  if (conv) {
+    const int cv = conv[m];
+    __syncthreads();  // every thread has read the flag before it is cleared
+    if (cv) {
+      if (threadIdx.x == 0) apply_conv<T>(cd, conv, bk, m);
+      act = 0;
+    }
+  }
+  if (!act) {  // a member that has left: its partials are not read (its tile may not have run)
+    if (threadIdx.x == 0) a_eff[m] = zero<T>();
+    return;
+  }
   T s = block_sum_partials(part, nbx, mt, m, lds);
   if (lpart) {
     __syncthreads();
@@ -1532,14 +1568,13 @@ __global__ void __launch_bounds__(BLOCK) k_fin_a(ChainDev cd, int nb, const T* p
   }
   if (threadIdx.x != 0) return;
   if (sc) s = bilin(s, inv(bilin(sc[m], sc[m])));
-  int k = *cd.level;
-  if (cd.active[m]) {
-    a[(size_t)m * cd.ld + k] = s;
-    cd.mused[m] = k + 1;
-    a_eff[m] = s;
-  } else {
-    a_eff[m] = zero<T>();
-  }
+  const int k = *cd.level - (cd.s0 ? cd.s0[m] : 0);
+  a[(size_t)m * cd.ld + k] = s;
+  cd.mused[m] = k + 1;
+  a_eff[m] = s;
+  // a complex chain (no projection c_k) at its last level leaves now: its b_k is not recorded, so the update gives it a
+  // zero vector (the slot's vector is clean for the next chain of a window run)
+  if (!cd.real_chain && !cd.cvec && k == cd.depth[m] - 1) cd.active[m] = 0;
 }
 
 // b_k = sqrt(sum w w); members leave at their depth or when b_k < tiny; next u = w * inv_b.  part2 (route ed3): the partials
@@ -1554,6 +1589,18 @@ __global__ void __launch_bounds__(BLOCK) k_fin_b(ChainDev cd, int nb, const T* p
                                                   const T* a) {
   __shared__ T lds[BLOCK / WAVE];
   const int m = blockIdx.x;
+  if (!cd.active[m]) {  // a member that has left: no partials read, a zero update and a unit scale
+    if (threadIdx.x == 0) {
+      b_eff[m] = zero<T>();
+      if (inv_b) inv_b[m] = zero<T>();
+      if (snext) {
+        T one = zero<T>();
+        *(double*)&one = 1.0;
+        snext[m] = one;
+      }
+    }
+    return;
+  }
   T s = block_sum_partials(part, nbx, mt, m, lds);
   T s2 = zero<T>(), sd = zero<T>();
   if (part2) {
@@ -1565,7 +1612,7 @@ __global__ void __launch_bounds__(BLOCK) k_fin_b(ChainDev cd, int nb, const T* p
     sd = block_sum_partials(pchk, nbx, mt, m, lds);
   }
   if (threadIdx.x != 0) return;
-  int k = *cd.level;
+  const int k = *cd.level - (cd.s0 ? cd.s0[m] : 0);
   T s_m = zero<T>();
   if (sc) {
     s_m = sc[m];
@@ -1598,75 +1645,81 @@ __global__ void __launch_bounds__(BLOCK) k_fin_b(ChainDev cd, int nb, const T* p
 }
 
 // ---------------------------------------------------------------------------------------------- adaptive depth on the device
-// Eigenvalues of the complex symmetric tridiagonal chain matrix (diagonal d[0..n-1], off-diagonal e[0..n-2], both
-// overwritten; the eigenvalues come back in d): the implicit QL iteration with shifts (the method of EISPACK tql1) in
-// complex arithmetic with complex orthogonal rotations (c^2 + s^2 = 1), which keep a complex symmetric tridiagonal matrix
-// tridiagonal.  Returns 0, or 1 when a rotation breaks down (c^2 + s^2 = 0) or the iteration does not converge.
+// Eigenvalues of the complex symmetric tridiagonal chain matrix (diagonal d[0..n-1], off-diagonal e[0..n-2], elements
+// st apart, both overwritten; the eigenvalues come back in d): the implicit QL iteration with shifts (the method of
+// EISPACK tql1) in complex arithmetic with complex orthogonal rotations (c^2 + s^2 = 1), which keep a complex symmetric
+// tridiagonal matrix tridiagonal.  Returns 0, or 1 when a rotation breaks down (c^2 + s^2 = 0) or the iteration does not
+// converge.
 __device__ inline cplx csq(cplx v) { return csqrt_c(v); }
-__device__ int cs_tridiag_eigs(int n, cplx* d, cplx* e) {
+__device__ int cs_tridiag_eigs(int n, cplx* dv, cplx* ev, int st) {
+#define d(x) dv[(x) * st]
+#define e(x) ev[(x) * st]
   const double eps = 2.220446049250313e-16;
-  e[n - 1] = zero<cplx>();
+  e(n - 1) = zero<cplx>();
   for (int l = 0; l < n; ++l) {
     int iter = 0, m;
     for (;;) {
       for (m = l; m < n - 1; ++m)
-        if (mag(e[m]) <= eps * (mag(d[m]) + mag(d[m + 1]))) break;
+        if (mag(e(m)) <= eps * (mag(d(m)) + mag(d(m + 1)))) break;
       if (m == l) break;
       if (iter++ == 60) return 1;
-      cplx g = (d[l + 1] - d[l]) * inv(2.0 * e[l]);
+      cplx g = (d(l + 1) - d(l)) * inv(2.0 * e(l));
       cplx r = csq(g * g + cplx{1.0, 0.0});
       // the root that makes |g + r| large (no cancellation)
       const cplx gp = g + r, gm = g - r;
-      g = d[m] - d[l] + e[l] * inv(mag(gp) >= mag(gm) ? gp : gm);
+      g = d(m) - d(l) + e(l) * inv(mag(gp) >= mag(gm) ? gp : gm);
       cplx sn = {1.0, 0.0}, c = {1.0, 0.0}, p = zero<cplx>();
       bool split = false;
       for (int i = m - 1; i >= l; --i) {
-        const cplx f = sn * e[i], b = c * e[i];
+        const cplx f = sn * e(i), b = c * e(i);
         r = csq(f * f + g * g);
-        e[i + 1] = r;
+        e(i + 1) = r;
         if (mag(r) == 0.0) {
           if (mag(f) + mag(g) != 0.0) return 1;  // an isotropic pair: the rotation does not exist
-          d[i + 1] = d[i + 1] - p;
-          e[m] = zero<cplx>();
+          d(i + 1) = d(i + 1) - p;
+          e(m) = zero<cplx>();
           split = true;
           break;
         }
         const cplx ir = inv(r);
         sn = f * ir;
         c = g * ir;
-        g = d[i + 1] - p;
-        r = (d[i] - g) * sn + 2.0 * (c * b);
+        g = d(i + 1) - p;
+        r = (d(i) - g) * sn + 2.0 * (c * b);
         p = sn * r;
-        d[i + 1] = g + p;
+        d(i + 1) = g + p;
         g = c * r - b;
       }
       if (split) continue;
-      d[l] = d[l] - p;
-      e[l] = g;
-      e[m] = zero<cplx>();
+      d(l) = d(l) - p;
+      e(l) = g;
+      e(m) = zero<cplx>();
     }
   }
   return 0;
 }
+#undef d
+#undef e
 
 constexpr int DEPTH_MAX = 128;  // chains of the device depth test: at most this many levels
 
 // G_11 (and f = sum_k c_k G_k1) of a chain of m levels terminated with the tail of cf_species (src/asr_chain.f90:67-167):
 // tail 0 hull (the extreme real parts of the chain matrix's eigenvalues give a_inf, b_inf), 2 avg2 (the last two levels),
 // 3 none.  Returns 1 when the hull's eigenvalues fail (the caller then keeps the chain running).
-__device__ int cf_species_dev(const cplx* a, const cplx* b, const cplx* c, int m, cplx z, int tail, cplx* g11, cplx* f) {
+// wd, we: work arrays of DEPTH_MAX values at stride st (LDS rows of the calling thread)
+__device__ int cf_species_dev(const cplx* a, const cplx* b, const cplx* c, int m, cplx z, int tail, cplx* g11, cplx* f,
+                              cplx* wd, cplx* we, int st) {
   cplx aa = a[m - 1], bb = m > 1 ? b[m - 2] : zero<cplx>();
   if (tail == 0 && m >= 2) {
-    cplx d[DEPTH_MAX], e[DEPTH_MAX];
-    for (int j = 0; j < m; ++j) { d[j] = a[j]; e[j] = j < m - 1 ? b[j] : zero<cplx>(); }
-    if (cs_tridiag_eigs(m, d, e)) return 1;
+    for (int j = 0; j < m; ++j) { wd[j * st] = a[j]; we[j * st] = j < m - 1 ? b[j] : zero<cplx>(); }
+    if (cs_tridiag_eigs(m, wd, we, st)) return 1;
     int lo = 0, hi = 0;
     for (int j = 1; j < m; ++j) {
-      if (d[j].re < d[lo].re) lo = j;
-      if (d[j].re > d[hi].re) hi = j;
+      if (wd[j * st].re < wd[lo * st].re) lo = j;
+      if (wd[j * st].re > wd[hi * st].re) hi = j;
     }
-    aa = 0.5 * (d[lo] + d[hi]);
-    bb = 0.25 * (d[hi] - d[lo]);
+    aa = 0.5 * (wd[lo * st] + wd[hi * st]);
+    bb = 0.25 * (wd[hi * st] - wd[lo * st]);
   }
   if (tail == 2 && m >= 3) {
     aa = 0.5 * (a[m - 1] + a[m - 2]);
@@ -1681,14 +1734,14 @@ __device__ int cf_species_dev(const cplx* a, const cplx* b, const cplx* c, int m
     const cplx g1 = (dz - s) * ib, g2 = (dz + s) * ib;
     tl = mag(bb * g1) <= mag(bb * g2) ? (bb * bb) * g1 : (bb * bb) * g2;
   }
-  // G_k down the chain, then f with the first column pv_k = prod G
-  cplx gt[DEPTH_MAX];
-  gt[m - 1] = inv(z - a[m - 1] - tl);
-  for (int j = m - 2; j >= 0; --j) gt[j] = inv(z - a[j] - (b[j] * b[j]) * gt[j + 1]);
+  // G_k down the chain (in wd), then f with the first column pv_k = prod G
+  cplx* gt = wd;
+  gt[(m - 1) * st] = inv(z - a[m - 1] - tl);
+  for (int j = m - 2; j >= 0; --j) gt[j * st] = inv(z - a[j] - (b[j] * b[j]) * gt[(j + 1) * st]);
   cplx prod = gt[0], ff = zero<cplx>();
   for (int k = 0; k < m; ++k) {
     if (c) ff = ff + c[k] * prod;
-    if (k < m - 1) prod = prod * b[k] * gt[k + 1];
+    if (k < m - 1) prod = prod * b[k] * gt[(k + 1) * st];
   }
   *g11 = gt[0];
   *f = ff;
@@ -1698,20 +1751,34 @@ __device__ int cf_species_dev(const cplx* a, const cplx* b, const cplx* c, int m
 // The adaptive depth of edc_green (src/asr_edc.f90:121-127) at level k1 (1-based; k1 = 8, 12, ...) on the device, one
 // thread per member: a member still running below its depth gets G_k1 of its chain (and route ed3's projection f, with
 // the check of ed3's chain, src/asr_ed3.f90:177-184), and leaves when |G - G_prev| <= tol |G| (k1 > 8).  gprev / fprev keep
-// the previous tested level's values.  Run between the a_k reduction and the update, as the host test.
-__global__ void k_depth_test(ChainDev cd, int nb, const cplx* a, const cplx* b, const cplx* cvec, const cplx* z, int k1,
-                             double tol, int tail, cplx* gprev, cplx* fprev, const double* tnorm) {
+// the previous tested level's values.  It runs on a stream of its own after the a_k reduction, beside the level's update;
+// a converged member is flagged (conv) and leaves at the next level's k_fin_a, which takes back what the level recorded
+// for it after a_k: the coefficients and depths are those of the test run before the update.
+constexpr int DT_TPB = 16;  // threads (members) per block of the device depth test: 2 x 128 x 16 values of LDS
+__global__ void __launch_bounds__(DT_TPB) k_depth_test(ChainDev cd, int nb, const cplx* a, const cplx* b, const cplx* cvec,
+                                                       const cplx* z, int k1, double tol, int tail, cplx* gprev, cplx* fprev,
+                                                       const double* tnorm, int* conv) {
+  // the QL iteration's work rows in LDS (a run-time indexed private array would live in scratch memory)
+  __shared__ cplx wsd[DEPTH_MAX * DT_TPB], wse[DEPTH_MAX * DT_TPB];
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= nb || !cd.active[q] || k1 >= cd.depth[q] || k1 > DEPTH_MAX) return;
+  if (q >= nb) return;
+  // window runs: the member's own level count (the host launches the test at the levels where some member is due)
+  if (cd.s0) {
+    k1 -= cd.s0[q];
+    if (k1 < 8 || (k1 % 4) != 0) return;
+  }
+  // the members that took part in this level's a_k (mused, written by k_fin_a, is not touched by the level's k_fin_b,
+  // which may run at the same time)
+  if (cd.mused[q] != k1 || k1 >= cd.depth[q] || k1 > DEPTH_MAX) return;
   cplx g, f;
   if (cf_species_dev(a + (size_t)q * cd.ld, b + (size_t)q * cd.ld, cvec ? cvec + (size_t)q * cd.ld : nullptr, k1, z[q], tail,
-                     &g, &f))
+                     &g, &f, wsd + threadIdx.x, wse + threadIdx.x, DT_TPB))
     return;
   if (k1 > 8) {
     const double ag = mag(g);
-    bool conv = mag(g - gprev[q]) <= tol * ag;
-    if (cvec) conv = conv && mag(f - fprev[q]) <= tol * fmax(mag(f), tnorm[q] * ag);
-    if (conv) cd.active[q] = 0;
+    bool cv = mag(g - gprev[q]) <= tol * ag;
+    if (cvec) cv = cv && mag(f - fprev[q]) <= tol * fmax(mag(f), tnorm[q] * ag);
+    if (cv) conv[q] = 1;
   }
   gprev[q] = g;
   fprev[q] = f;
@@ -1762,6 +1829,110 @@ __global__ void k_block_start(const cplx* vv, cplx* u, cplx* tv, const double* c
 }
 
 // end of a level: level += 1 and the activity of every member tile (a tile whose members have all left is skipped)
+// Member compaction: the permutation map (new slot -> old slot) applied along the member axis of a member-fastest array
+// (nrows rows of nbp values), dst[r][q] = src[r][map[q]] for the first ncol slots q of every row, and along the rows of a
+// member-major array (nbp rows of ld values), dst[q][k] = src[map[q]][k].
+template <class E>
+__global__ void k_perm_cols(E* __restrict__ dst, const E* __restrict__ src, size_t nrows, int nbp, int ncol,
+                            const int* __restrict__ map) {
+  const size_t n = nrows * (size_t)ncol;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+    const size_t r = t / (size_t)ncol;
+    const int q = (int)(t - r * (size_t)ncol);
+    dst[r * (size_t)nbp + q] = src[r * (size_t)nbp + map[q]];
+  }
+}
+template <class E>
+__global__ void k_perm_rows(E* __restrict__ dst, const E* __restrict__ src, int ld, int nbp, const int* __restrict__ map) {
+  const size_t n = (size_t)nbp * ld;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+    const int q = (int)(t / (size_t)ld), k = (int)(t % (size_t)ld);
+    dst[t] = src[(size_t)map[q] * ld + k];
+  }
+}
+
+// Window runs: the chains wait in a queue and take the slots of members that have left.  k_extract copies the results of
+// leaving members (their relative levels 0 .. ld - 1) to the rows of their chains; k_fill_cols copies the member tables
+// of the starting chains from the all-chain tables (stride nbps) into their slots (stride nbpd); k_start_cols writes their
+// start vectors (u = the unit vector of the start orbital on the start node, rows 0..8; u_prev = 0); k_member_start
+// resets their recursion state.
+template <class T>
+__global__ void k_extract(T* __restrict__ a_all, T* __restrict__ b_all, int* __restrict__ mused_all, const T* __restrict__ a,
+                          const T* __restrict__ b, const int* __restrict__ mused, int ld, const int* __restrict__ slots,
+                          const int* __restrict__ chains, int n) {
+  const int j = blockIdx.x;
+  if (j >= n) return;
+  const int sl = slots[j], c = chains[j];
+  for (int k = threadIdx.x; k < ld; k += blockDim.x) {
+    a_all[(size_t)c * ld + k] = a[(size_t)sl * ld + k];
+    b_all[(size_t)c * ld + k] = b[(size_t)sl * ld + k];
+  }
+  if (threadIdx.x == 0) mused_all[c] = mused[sl];
+}
+template <class E>
+__global__ void k_fill_cols(E* __restrict__ dst, const E* __restrict__ src, size_t nrows, int nbpd, int nbps,
+                            const int* __restrict__ slots, const int* __restrict__ chains, int n) {
+  const size_t tot = nrows * (size_t)n;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += (size_t)gridDim.x * blockDim.x) {
+    const size_t r = t / (size_t)n;
+    const int j = (int)(t - r * (size_t)n);
+    dst[r * (size_t)nbpd + slots[j]] = src[r * (size_t)nbps + chains[j]];
+  }
+}
+template <class T>
+__global__ void k_start_cols(T* __restrict__ u, T* __restrict__ up, size_t nrows, int nbp, const int* __restrict__ slots,
+                             const int* __restrict__ l0, int n) {
+  const size_t tot = nrows * (size_t)n;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += (size_t)gridDim.x * blockDim.x) {
+    const size_t r = t / (size_t)n;
+    const int j = (int)(t - r * (size_t)n);
+    T v = zero<T>();
+    if (r == (size_t)l0[j]) *(double*)&v = 1.0;
+    u[r * (size_t)nbp + slots[j]] = v;
+    up[r * (size_t)nbp + slots[j]] = zero<T>();
+  }
+}
+template <class T>
+__global__ void k_member_start(ChainDev cd, const int* __restrict__ slots, const int* __restrict__ depth, int n, int s0,
+                               T* a, T* b, T* a_eff, T* b_eff, T* inv_b, T* sc0, T* sc1, T* sc2, cplx* gprev, cplx* fprev,
+                               int* s0v, int* closed, int* conv) {
+  const int j = blockIdx.x;
+  if (j >= n) return;
+  const int sl = slots[j];
+  for (int k = threadIdx.x; k < cd.ld; k += blockDim.x) {
+    a[(size_t)sl * cd.ld + k] = zero<T>();
+    b[(size_t)sl * cd.ld + k] = zero<T>();
+  }
+  if (threadIdx.x != 0) return;
+  T one = zero<T>();
+  *(double*)&one = 1.0;
+  a_eff[sl] = zero<T>();
+  b_eff[sl] = zero<T>();
+  inv_b[sl] = zero<T>();
+  sc0[sl] = one;
+  sc1[sl] = one;
+  sc2[sl] = one;
+  if (gprev) gprev[sl] = zero<cplx>();
+  if (fprev) fprev[sl] = zero<cplx>();
+  cd.depth[sl] = depth[j];
+  cd.active[sl] = depth[j] > 0 ? 1 : 0;
+  cd.mused[sl] = 0;
+  s0v[sl] = s0;
+  if (closed) closed[sl] = 0;
+  if (conv) conv[sl] = 0;
+}
+
+// the start vectors of new members in slots whose vectors are zero (the member before left by the depth test or at its
+// depth, so its last update wrote zeros): u = 1 on the start orbital's row of the start node
+template <class T>
+__global__ void k_start_unit(T* __restrict__ u, int nbp, const int* __restrict__ slots, const int* __restrict__ l0, int n) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  T v = zero<T>();
+  *(double*)&v = 1.0;
+  u[(size_t)l0[j] * nbp + slots[j]] = v;
+}
+
 __global__ void k_level_end(int* level, const int* active, int nb, int mt, int* tact, int ntile) {
   for (int t = threadIdx.x; t < ntile; t += blockDim.x) {
     int any = 0;
@@ -2575,6 +2746,9 @@ struct asr_gpu_op {
   double depth_tol = 0.0;                         // adaptive depth of the edc chains (ed_depth_tol), 0 = off
   int depth_tail = 0;                             // tail of that test (ed_tail): 0 hull, 1 bp, 2 avg2, 3 none
   bool reorth = false;                            // full reorthogonalisation of the real chains (lanczos reorth)
+  bool compact = true;                            // member compaction between host-issued levels (adaptive depth)
+  int window = 0;                                 // member slots of a window run (0: every chain holds a slot)
+  int ncompact = 0;                               // compactions of the last batched run
   int level_form = 0;                             // edc chains: 0 the fused level (a_k from the first pass, the update
                                                   // in the second), 1 the separate update pass
   bool asym = false;                              // the fused level's check found the operator not symmetric: form 1
@@ -2804,7 +2978,7 @@ int upload_members(const asr_gpu_op* op, const Members& M, MemBuffers& B, MemDev
   HCHECK(hipMemcpyAsync(B.fpar, M.fpar.data(), M.fpar.size() * 8, hipMemcpyHostToDevice, op->stream));
   HCHECK(hipMemcpyAsync(B.dvec, M.dvec.data(), M.dvec.size() * 8, hipMemcpyHostToDevice, op->stream));
   HCHECK(hipMemcpyAsync(B.z, M.z.data(), M.z.size() * sizeof(cplx), hipMemcpyHostToDevice, op->stream));
-  md.nb = M.nb; md.nbp = M.nbp; md.mt = M.mt; md.pair = M.pair; md.tact = nullptr;
+  md.nb = M.nb; md.nbp = M.nbp; md.mt = M.mt; md.pair = M.pair; md.tact = nullptr; md.ntl = 0;
   md.chimask = B.chim; md.wts = B.wts; md.fpar = B.fpar; md.dvec = B.dvec; md.z = B.z;
   md.dperm = B.dperm;
   md.mtm = B.mtm;
@@ -2821,7 +2995,7 @@ size_t fused_lds_bytes(const asr_gpu_op* op, size_t tsize) {
 dim3 node_grid(const asr_gpu_op* op, const MemDev& md) {
   int npw = WAVE / md.mt;
   int nodes_per_block = npw * (BLOCK / WAVE);
-  return dim3((op->nn + nodes_per_block - 1) / nodes_per_block, md.nbp / md.mt, 1);
+  return dim3((op->nn + nodes_per_block - 1) / nodes_per_block, md.ntl > 0 ? md.ntl : md.nbp / md.mt, 1);
 }
 
 // Optional per-kernel timing (HIP events on the engine's stream), used by asr_gpu_profile_level.
@@ -3193,6 +3367,10 @@ struct ChainRun {
   bool dev_depth = false;
   cplx *gprev_d = nullptr, *fprev_d = nullptr;
   double* tnorm_d = nullptr;
+  int* conv_d = nullptr;                 // members the last test found converged (they leave at the next k_fin_a)
+  hipStream_t tstream = nullptr;         // the test's own stream (default priority)
+  hipEvent_t ev_a = nullptr, ev_t = nullptr;
+  bool test_pending = false;
   bool ed3 = false;                  // two-chain blocks (start vectors, projections c_k, closure flags)
   std::vector<double> snorm, tnorm;  // blocks: norms of the start vector and of the projection vector per member
   std::vector<double> s0h;           // blocks: 9 x nbp start vectors on the start node
@@ -3215,10 +3393,39 @@ struct ChainRun {
   const T* s_prev() const { return lag ? sc3[(level_host + 2) % 3] : nullptr; }
   T* s_next() const { return lag ? sc3[(level_host + 1) % 3] : nullptr; }
 
+  // member compaction (host-issued levels with the adaptive depth): the permutation of the last compaction and a staging
+  // buffer for the in-place permutations of the member tables
+  int* perm_d = nullptr;
+  void* ptmp = nullptr;
+  size_t ptmp_bytes = 0;
+  int ncompact = 0;
+  // window run (route edc with the adaptive depth on the device, more chains than op->window): the run holds
+  // op->window member slots; the chains wait in a queue (sorted order) and take the slots of members that have left, whose
+  // results go to the rows of their chains (a_all, b_all, mused_all, relative levels); a slot's member started at level
+  // slot_s0 (s0_d on the device), and the tables of every chain stay on the device (Mall / MBall / mdall)
+  bool win = false;
+  Members Mall;
+  MemBuffers MBall;
+  MemDev mdall{};
+  std::vector<int> so_all, nl_all, slot_chain, slot_s0, slot_dirty;
+  int qnext = 0;
+  T *a_all = nullptr, *b_all = nullptr;
+  int *mused_all = nullptr, *s0_d = nullptr, *wslots = nullptr, *wchains = nullptr, *wvals = nullptr, *wl0 = nullptr;
+
   ~ChainRun() {
     // work of an aborted run may still be queued: the cached blocks go back only once the stream is idle
     if (stream) (void)hipStreamSynchronize(stream);
     if (tact) (void)hipFree(tact);
+    if (perm_d) (void)hipFree(perm_d);
+    if (ptmp) (void)hipFree(ptmp);
+    if (tstream) (void)hipStreamSynchronize(tstream);
+    if (conv_d) (void)hipFree(conv_d);
+    if (ev_a) (void)hipEventDestroy(ev_a);
+    if (ev_t) (void)hipEventDestroy(ev_t);
+    if (tstream) (void)hipStreamDestroy(tstream);
+    for (void* p : {(void*)a_all, (void*)b_all, (void*)mused_all, (void*)s0_d, (void*)wslots, (void*)wchains, (void*)wvals,
+                    (void*)wl0})
+      if (p) (void)hipFree(p);
     for (void* p : {(void*)gprev_d, (void*)fprev_d, (void*)tnorm_d})
       if (p) (void)hipFree(p);
     for (T* p : sc3)
@@ -3227,8 +3434,10 @@ struct ChainRun {
 
   int setup(asr_gpu_op* op_, hipStream_t s, int nch, const int* start_orb, const double* params, const double* pab,
             const int* frozen_sp, const double* conc, const double* z, const int* nlev, int project_, int ldab, int tile,
-            double tiny, bool real_chain, const BlockIn* e3 = nullptr, const double* tw_in = nullptr) {
+            double tiny, bool real_chain, const BlockIn* e3 = nullptr, const double* tw_in = nullptr,
+            bool allow_window = false) {
     op = op_; stream = s; nchain = nch; project = project_; ld = ldab; lin = (pab != nullptr);
+    op->ncompact = 0;
     proj = (tw_in != nullptr);
     if (proj && (op->ntarget <= 0 || !std::is_same<T, cplx>::value)) return fail("projected chains: no target states set");
     ed3 = (e3 != nullptr);
@@ -3270,13 +3479,42 @@ struct ChainRun {
     const double* px_s = px.empty() ? nullptr : px.data();
     const int* fz2_s = fz2.empty() ? nullptr : fz2.data();
     const uint64_t* ch_s = chs.empty() ? nullptr : chs.data();
-    if (tile == 0 && autotune_tile<T>(op, nch, so.data(), pp.data(), pab_s, fz.data(), conc_s, z_s, &tile, px_s, fz2_s, ch_s))
-      return 1;
-    if (prepare_members(op, nch, so.data(), pp.data(), pab_s, fz.data(), conc_s, z_s, M, tile, px_s, fz2_s, ch_s)) return 1;
-    if (upload_members(op, M, MB, md)) return 1;
     maxdepth = 0;
     for (int q = 0; q < nch; ++q) maxdepth = std::max(maxdepth, nl[q]);
     if (maxdepth > ldab) return fail("chains: depth larger than the leading dimension of a, b");
+    // a window run: the route-edc chains with the adaptive depth on the device hold op->window slots, the others queue
+    win = allow_window && std::is_same<T, cplx>::value && op->window > 0 && nch > op->window && lin && !proj && !ed3 &&
+          !real_chain && op->depth_tol > 0.0 && op->depth_tail != 1 && maxdepth <= DEPTH_MAX;
+    const int nmem = win ? op->window : nch;  // members held at once
+    if (tile == 0 && autotune_tile<T>(op, nmem, so.data(), pp.data(), pab_s, fz.data(), conc_s, z_s, &tile, px_s, fz2_s, ch_s))
+      return 1;
+    if (!win) {
+      if (prepare_members(op, nch, so.data(), pp.data(), pab_s, fz.data(), conc_s, z_s, M, tile, px_s, fz2_s, ch_s)) return 1;
+    } else {
+      // the tables of every chain stay on the device (stride Mall.nbp); the slots take the first nmem chains
+      if (prepare_members(op, nch, so.data(), pp.data(), pab_s, fz.data(), conc_s, z_s, Mall, tile, px_s, fz2_s, ch_s))
+        return 1;
+      if (prepare_members(op, nmem, so.data(), pp.data(), pab_s, fz.data(), conc_s, z_s, M, tile, px_s, fz2_s, ch_s))
+        return 1;
+      if (M.mt != Mall.mt) return fail("window run: member tile mismatch");
+      if (upload_members(op, Mall, MBall, mdall)) return 1;
+      so_all = so;
+      nl_all = nl;
+      slot_chain.assign(M.nbp, -1);
+      slot_s0.assign(M.nbp, 0);
+      slot_dirty.assign(M.nbp, 0);
+      for (int q = 0; q < nmem; ++q) slot_chain[q] = q;
+      qnext = nmem;
+      if (dalloc(&a_all, (size_t)nch * ldab) || dalloc(&b_all, (size_t)nch * ldab) || dalloc(&mused_all, nch) ||
+          dalloc(&s0_d, M.nbp) || dalloc(&wslots, M.nbp) || dalloc(&wchains, M.nbp) || dalloc(&wvals, M.nbp) ||
+          dalloc(&wl0, M.nbp))
+        return 1;
+      HCHECK(hipMemsetAsync(a_all, 0, (size_t)nch * ldab * sizeof(T), stream));
+      HCHECK(hipMemsetAsync(b_all, 0, (size_t)nch * ldab * sizeof(T), stream));
+      HCHECK(hipMemsetAsync(mused_all, 0, nch * sizeof(int), stream));
+      HCHECK(hipMemsetAsync(s0_d, 0, M.nbp * sizeof(int), stream));
+    }
+    if (upload_members(op, M, MB, md)) return 1;
     nv = (size_t)(op->nn + 1) * 9 * M.nbp;  // + the zero padding node (target of missing links)
     grid = node_grid(op, md);
     nbx = grid.x;
@@ -3294,6 +3532,11 @@ struct ChainRun {
     if (ed3 && (palloc(B.pool, &B.tv, nv) || dalloc(&B.part2, npart) || dalloc(&B.c, (size_t)M.nbp * ldab) ||
                 dalloc(&B.closed, M.nbp)))
       return 1;
+    if (win) {
+      // the chains that end early (b_k < tiny) are flagged: their slot's vector is not zero when the next chain takes it
+      if (dalloc(&B.closed, M.nbp)) return 1;
+      HCHECK(hipMemsetAsync(B.closed, 0, M.nbp * sizeof(int), stream));
+    }
     reorth = op->reorth && real_chain && std::is_same<T, double>::value;
     if (reorth) {
       // the stored Lanczos vectors of every member: maxdepth x nv doubles (lanczos keeps q(dim, nmax) per chain)
@@ -3344,7 +3587,7 @@ struct ChainRun {
     std::vector<int> dep(M.nbp, 0), act(M.nbp, 0), ta(std::max(ntile, 2), 0);
     std::vector<T> ustart((size_t)9 * M.nbp);
     std::memset(ustart.data(), 0, ustart.size() * sizeof(T));
-    for (int q = 0; q < nch; ++q) {
+    for (int q = 0; q < M.nb; ++q) {
       dep[q] = nl[q];
       act[q] = nl[q] > 0 ? 1 : 0;
       if (act[q]) ta[q / M.mt] = 1;
@@ -3367,12 +3610,17 @@ struct ChainRun {
     cd.level = B.level; cd.depth = B.depth; cd.active = B.active; cd.mused = B.mused; cd.ld = ldab;
     cd.real_chain = real_chain ? 1 : 0; cd.tiny = tiny;
     cd.cvec = B.c; cd.closed = B.closed;
+    cd.s0 = win ? s0_d : nullptr;
     sgrid = dim3((unsigned)((nv + 255) / 256));
     if (ed3 && block_start(ocs, sides)) return 1;
     if constexpr (std::is_same<T, cplx>::value) {
       dev_depth = lin && !proj && op->depth_tol > 0.0 && op->depth_tail != 1 && maxdepth <= DEPTH_MAX;
       if (dev_depth) {
-        if (dalloc(&gprev_d, M.nbp) || dalloc(&fprev_d, M.nbp) || dalloc(&tnorm_d, M.nbp)) return 1;
+        if (dalloc(&gprev_d, M.nbp) || dalloc(&fprev_d, M.nbp) || dalloc(&tnorm_d, M.nbp) || dalloc(&conv_d, M.nbp)) return 1;
+        HCHECK(hipMemsetAsync(conv_d, 0, M.nbp * sizeof(int), stream));
+        HCHECK(hipStreamCreateWithFlags(&tstream, hipStreamNonBlocking));
+        HCHECK(hipEventCreateWithFlags(&ev_a, hipEventDisableTiming));
+        HCHECK(hipEventCreateWithFlags(&ev_t, hipEventDisableTiming));
         std::vector<double> tn(M.nbp, 0.0);
         for (size_t q = 0; q < tnorm.size(); ++q) tn[q] = tnorm[q];
         HCHECK(hipMemcpyAsync(tnorm_d, tn.data(), M.nbp * 8, hipMemcpyHostToDevice, stream));
@@ -3477,13 +3725,46 @@ struct ChainRun {
       return 0;
     } else {
       const int k1 = level_host + 1;
-      if (k1 < 8 || (k1 % 4) != 0) return 0;
-      hipLaunchKernelGGL(k_depth_test, dim3((M.nb + 63) / 64), dim3(64), 0, stream, cd, M.nb, (const cplx*)B.a,
-                         (const cplx*)B.b, ed3 ? (const cplx*)B.c : (const cplx*)nullptr, (const cplx*)md.z, k1,
-                         op->depth_tol, op->depth_tail, gprev_d, fprev_d, (const double*)tnorm_d);
+      if (!win && (k1 < 8 || (k1 % 4) != 0)) return 0;
+      if (win) {
+        // window runs: the kernel takes each member's own level count; no launch when no member is at a test level
+        bool any = false;
+        for (int q = 0; q < M.nb && !any; ++q) {
+          const int c = slot_chain[q];
+          if (c < 0) continue;
+          const int rel = k1 - slot_s0[q];
+          any = rel >= 8 && rel % 4 == 0 && rel < nl_all[c];
+        }
+        if (!any) return 0;
+      }
+      // on the test stream, after this level's a_k (k_fin_a) and beside its update; joined before the next k_fin_a
+      HCHECK(hipEventRecord(ev_a, stream));
+      HCHECK(hipStreamWaitEvent(tstream, ev_a, 0));
+      hipLaunchKernelGGL(k_depth_test, dim3((M.nb + DT_TPB - 1) / DT_TPB), dim3(DT_TPB), 0, tstream, cd, M.nb,
+                         (const cplx*)B.a, (const cplx*)B.b, ed3 ? (const cplx*)B.c : (const cplx*)nullptr,
+                         (const cplx*)md.z, k1, op->depth_tol, op->depth_tail, gprev_d, fprev_d, (const double*)tnorm_d,
+                         conv_d);
       HCHECK(hipGetLastError());
+      HCHECK(hipEventRecord(ev_t, tstream));
+      test_pending = true;
       return 0;
     }
+  }
+
+  // the main stream waits for the last depth test (before a k_fin_a, a compaction, or the end of the run)
+  int join_test() {
+    if (!test_pending) return 0;
+    HCHECK(hipStreamWaitEvent(stream, ev_t, 0));
+    test_pending = false;
+    return 0;
+  }
+  // the end of a run: the decisions of the last test take effect
+  int flush_test() {
+    if (!test_pending) return 0;
+    if (join_test()) return 1;
+    hipLaunchKernelGGL(k_conv_apply<T>, dim3((M.nb + 63) / 64), dim3(64), 0, stream, cd, M.nb, conv_d, B.b);
+    HCHECK(hipGetLastError());
+    return 0;
   }
 
   int step_apply() {
@@ -3502,9 +3783,10 @@ struct ChainRun {
     } else if (apply_level_op<T>(op, M, md, B.u, B.w, B.t1, B.t2, B.h1, lin, B.u, B.part, stream)) {
       return 1;
     }
+    if (join_test()) return 1;
     if (timer_mark(stream, KIND_FIN_A, true)) return 1;
     hipLaunchKernelGGL(k_fin_a<T>, dim3(M.nb), dim3(BLOCK), 0, stream, cd, M.nb, B.part, nbx, apply_mt, B.a, B.a_eff, s_cur(),
-                       (const T*)B.lpart, fused ? op->nplist : 0, M.nbp);
+                       (const T*)B.lpart, fused ? op->nplist : 0, M.nbp, conv_d, B.b);
     if (timer_mark(stream, 0, false)) return 1;
     return 0;
   }
@@ -3693,6 +3975,284 @@ struct ChainRun {
     return 0;
   }
 
+  // Member compaction between host-issued levels (the reference's edc_green_blk lets every member leave at its own depth;
+  // here a member tile runs while any of its members does).  After a level of the adaptive-depth test (levels 12, 16, ...)
+  // the members still running move to the front, in order, and those that have left behind them, when that frees at
+  // least one member tile for at least two more levels.  Every per-member array moves alike: the chain vectors u, u_prev
+  // (and route ed3's projection vector), the member tables of the operator pass, the coefficients a, b (c) with the
+  // scalars of the recursion, the lagged scales and the depth-test state, and on the host the chain order and the
+  // per-member values the host reads.  *alive = 0 when no member runs any more.
+  bool can_compact() const { return lag && lin && !proj && !reorth && op->depth_tol > 0.0; }
+
+  template <class E>
+  int perm_cols(E* arr, size_t nrows) {
+    const size_t bytes = nrows * (size_t)M.nbp * sizeof(E);
+    if (bytes > ptmp_bytes) {
+      if (ptmp) (void)hipFree(ptmp);
+      ptmp = nullptr;
+      ptmp_bytes = 0;
+      HCHECK(hipMalloc(&ptmp, bytes));
+      ptmp_bytes = bytes;
+    }
+    HCHECK(hipMemcpyAsync(ptmp, arr, bytes, hipMemcpyDeviceToDevice, stream));
+    const size_t n = nrows * (size_t)M.nbp;
+    const unsigned nblk = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_perm_cols<E>, dim3(nblk), dim3(256), 0, stream, arr, (const E*)ptmp, nrows, M.nbp, M.nbp,
+                       (const int*)perm_d);
+    HCHECK(hipGetLastError());
+    return 0;
+  }
+  template <class E>
+  int perm_rows(E* arr) {
+    const size_t bytes = (size_t)M.nbp * ld * sizeof(E);
+    if (bytes > ptmp_bytes) {
+      if (ptmp) (void)hipFree(ptmp);
+      ptmp = nullptr;
+      ptmp_bytes = 0;
+      HCHECK(hipMalloc(&ptmp, bytes));
+      ptmp_bytes = bytes;
+    }
+    HCHECK(hipMemcpyAsync(ptmp, arr, bytes, hipMemcpyDeviceToDevice, stream));
+    const size_t n = (size_t)M.nbp * ld;
+    hipLaunchKernelGGL(k_perm_rows<E>, dim3((unsigned)std::min<size_t>((n + 255) / 256, 8192)), dim3(256), 0, stream, arr,
+                       (const E*)ptmp, ld, M.nbp, (const int*)perm_d);
+    HCHECK(hipGetLastError());
+    return 0;
+  }
+  // a chain vector into a free level buffer, which then takes its place: only the slots of the member tiles that still
+  // run (ncol) are written, the vectors of members that have left are not needed any more
+  int perm_vector(T*& v, T*& spare, int ncol) {
+    const size_t n = nv / M.nbp * ncol;
+    hipLaunchKernelGGL(k_perm_cols<T>, dim3((unsigned)std::min<size_t>((n + 255) / 256, 16384)), dim3(256), 0, stream, spare,
+                       (const T*)v, nv / M.nbp, M.nbp, ncol, (const int*)perm_d);
+    HCHECK(hipGetLastError());
+    std::swap(v, spare);
+    return 0;
+  }
+
+  int compact(int* alive) {
+    *alive = 1;
+    if (win) return refill(alive);
+    // members leave at the test levels 12, 16, ...; with the test on the device its decision takes effect one level later
+    const int ph = dev_depth ? 1 : 0;
+    if (!can_compact() || level_host < 12 + ph || level_host % 4 != ph) return 0;
+    if (test_pending) HCHECK(hipEventSynchronize(ev_t));
+    const int nb = M.nb, mt = M.mt;
+    std::vector<int> act(nb);
+    HCHECK(hipMemcpyAsync(act.data(), B.active, nb * sizeof(int), hipMemcpyDeviceToHost, stream));
+    HCHECK(hipStreamSynchronize(stream));
+    int nlive = 0;
+    for (int q = 0; q < nb; ++q) nlive += act[q] ? 1 : 0;
+    if (nlive == 0) {
+      *alive = 0;
+      return 0;
+    }
+    return compact_live(act, nlive, maxdepth - level_host);
+  }
+
+  // the live members (act) to the front when that frees member-tile levels (remaining: levels still to run)
+  int compact_live(const std::vector<int>& act, int nlive, int remaining) {
+    const int nb = M.nb, mt = M.mt;
+    int cur = 0;
+    for (int t = 0; t * mt < nb; ++t) {
+      int any = 0;
+      for (int q = t * mt; q < std::min(nb, (t + 1) * mt); ++q) any |= act[q];
+      cur += any ? 1 : 0;
+    }
+    const int need = (nlive + mt - 1) / mt;
+    // worth a vector gather when it frees member-tile levels: at least one tile for four more levels or two for two
+    if (need >= cur || (cur - need) * remaining < 4) return 0;
+    // stable partition: the running members first, then those that have left; padding slots stay
+    std::vector<int> map(M.nbp);
+    int w = 0;
+    for (int q = 0; q < nb; ++q)
+      if (act[q]) map[w++] = q;
+    for (int q = 0; q < nb; ++q)
+      if (!act[q]) map[w++] = q;
+    for (int q = nb; q < M.nbp; ++q) map[q] = q;
+    if (!perm_d && dalloc(&perm_d, M.nbp)) return 1;
+    HCHECK(hipMemcpyAsync(perm_d, map.data(), M.nbp * sizeof(int), hipMemcpyHostToDevice, stream));
+    // chain vectors (w, t1, t2 are free between levels)
+    if (perm_vector(B.u, B.w, need * mt) || perm_vector(B.up, B.t2, need * mt)) return 1;
+    if (B.tv && perm_vector(B.tv, B.t1, need * mt)) return 1;
+    // member tables of the operator pass
+    const size_t nbp = M.nbp;
+    if (perm_cols(MB.chim, 1) || perm_cols(MB.wts, M.wts.size() / nbp) || perm_cols(MB.fpar, M.fpar.size() / nbp) ||
+        perm_cols(MB.dvec, M.dvec.size() / nbp) || perm_cols(MB.dperm, M.dperm.size() / nbp) ||
+        perm_cols(MB.mtm, M.mtm.size() / nbp) || perm_cols(MB.fsp, M.fsp.size() / nbp) || perm_cols(MB.z, 1))
+      return 1;
+    // the recursion's per-member state
+    if (perm_rows(B.a) || perm_rows(B.b) || (B.c && perm_rows(B.c))) return 1;
+    for (T* p : {B.a_eff, B.b_eff, B.inv_b, sc3[0], sc3[1], sc3[2]})
+      if (p && perm_cols(p, 1)) return 1;
+    for (int* p : {B.depth, B.active, B.mused, B.closed, s0_d, conv_d})
+      if (p && perm_cols(p, 1)) return 1;
+    if constexpr (std::is_same<T, cplx>::value) {
+      if (gprev_d && (perm_cols(gprev_d, 1) || perm_cols(fprev_d, 1) || perm_cols(tnorm_d, 1))) return 1;
+    }
+    // the tiles with running members are the first `need`: later levels launch those alone (member tiles below a wave;
+    // the lane-pair form of tile 64 keeps its full grid)
+    std::vector<int> ta(std::max(ntile, 2), 0);
+    for (int t = 0; t < need; ++t) ta[t] = 1;
+    HCHECK(hipMemcpyAsync(tact, ta.data(), ta.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+    if (M.mt < WAVE) {
+      md.ntl = need;
+      grid = node_grid(op, md);
+    }
+    // host bookkeeping (a window run keeps the chain order of its results; its slots move)
+    auto permute = [&](auto& v) {
+      if (v.size() < (size_t)nb) return;
+      auto old = v;
+      for (int q = 0; q < nb; ++q) v[q] = old[map[q]];
+    };
+    if (!win) permute(order);
+    permute(M.z);
+    permute(gprev);
+    permute(fprev);
+    permute(snorm);
+    permute(tnorm);
+    permute(slot_chain);
+    permute(slot_s0);
+    permute(slot_dirty);
+    HCHECK(hipStreamSynchronize(stream));  // the host sources above and the staging buffer
+    ++ncompact;
+    ++op->ncompact;
+    return 0;
+  }
+
+  // Window runs, after every level: the members that may have left by now (after their own test levels 12, 16, ... of
+  // the adaptive depth, or at their depth) are looked at; those that left hand their results to their chains' rows and
+  // their slots to the next chains of the queue; once the queue is empty the live members are compacted as above.
+  int refill(int* alive) {
+    const int nb = M.nb, mt = M.mt;
+    bool maybe = false;
+    for (int q = 0; q < nb && !maybe; ++q) {
+      const int c = slot_chain[q];
+      if (c < 0) continue;
+      const int rel = level_host - slot_s0[q];
+      // at its depth, or one level after its test levels 12, 16, ... (the test's decision takes effect then)
+      if (rel >= nl_all[c] || (rel >= 13 && rel % 4 == 1)) maybe = true;
+    }
+    if (!maybe) return 0;
+    if (test_pending) HCHECK(hipEventSynchronize(ev_t));
+    std::vector<int> act(nb), cls(nb);
+    HCHECK(hipMemcpyAsync(act.data(), B.active, nb * sizeof(int), hipMemcpyDeviceToHost, stream));
+    HCHECK(hipMemcpyAsync(cls.data(), B.closed, nb * sizeof(int), hipMemcpyDeviceToHost, stream));
+    HCHECK(hipStreamSynchronize(stream));
+    std::vector<int> xs, xc;
+    for (int q = 0; q < nb; ++q)
+      if (slot_chain[q] >= 0 && !act[q]) {
+        xs.push_back(q);
+        xc.push_back(slot_chain[q]);
+        slot_chain[q] = -1;
+        if (cls[q]) slot_dirty[q] = 1;
+      }
+    if (!xs.empty()) {
+      const int n = (int)xs.size();
+      HCHECK(hipMemcpyAsync(wslots, xs.data(), n * sizeof(int), hipMemcpyHostToDevice, stream));
+      HCHECK(hipMemcpyAsync(wchains, xc.data(), n * sizeof(int), hipMemcpyHostToDevice, stream));
+      hipLaunchKernelGGL(k_extract<T>, dim3(n), dim3(64), 0, stream, a_all, b_all, mused_all, (const T*)B.a, (const T*)B.b,
+                         (const int*)B.mused, ld, (const int*)wslots, (const int*)wchains, n);
+      HCHECK(hipGetLastError());
+      HCHECK(hipStreamSynchronize(stream));  // before the slot lists are reused
+    }
+    std::vector<int> ns, nc, nd, nl0;
+    int ndirty = 0;
+    for (int q = 0; q < nb && qnext < nchain; ++q)
+      if (slot_chain[q] < 0) {
+        const int c = qnext++;
+        slot_chain[q] = c;
+        slot_s0[q] = level_host;
+        act[q] = nl_all[c] > 0 ? 1 : 0;
+        ns.push_back(q);
+        nc.push_back(c);
+        nd.push_back(nl_all[c]);
+        nl0.push_back(so_all[c]);
+        ndirty += slot_dirty[q];
+      }
+    // slots with a non-zero vector (the chain before ended early) first, for the full start vector; then the clean ones
+    std::stable_partition(ns.begin(), ns.end(), [&](int q) { return slot_dirty[q] != 0; });
+    {
+      std::vector<int> c2, d2, l2;
+      for (int q : ns) {
+        c2.push_back(slot_chain[q]);
+        d2.push_back(nl_all[slot_chain[q]]);
+        l2.push_back(so_all[slot_chain[q]]);
+      }
+      nc.swap(c2);
+      nd.swap(d2);
+      nl0.swap(l2);
+      for (int q : ns) slot_dirty[q] = 0;
+    }
+    if (!ns.empty()) {
+      const int n = (int)ns.size();
+      HCHECK(hipMemcpyAsync(wslots, ns.data(), n * sizeof(int), hipMemcpyHostToDevice, stream));
+      HCHECK(hipMemcpyAsync(wchains, nc.data(), n * sizeof(int), hipMemcpyHostToDevice, stream));
+      HCHECK(hipMemcpyAsync(wvals, nd.data(), n * sizeof(int), hipMemcpyHostToDevice, stream));
+      HCHECK(hipMemcpyAsync(wl0, nl0.data(), n * sizeof(int), hipMemcpyHostToDevice, stream));
+      const size_t nbp = M.nbp;
+      auto fill = [&](auto* dst, const auto* src, size_t nrows) -> int {
+        using E = std::remove_const_t<std::remove_pointer_t<decltype(src)>>;
+        const size_t tot = nrows * n;
+        hipLaunchKernelGGL(k_fill_cols<E>, dim3((unsigned)std::min<size_t>((tot + 255) / 256, 4096)), dim3(256), 0, stream,
+                           (E*)dst, (const E*)src, nrows, M.nbp, Mall.nbp, (const int*)wslots, (const int*)wchains, n);
+        HCHECK(hipGetLastError());
+        return 0;
+      };
+      if (fill(MB.chim, MBall.chim, 1) || fill(MB.wts, MBall.wts, M.wts.size() / nbp) ||
+          fill(MB.fpar, MBall.fpar, M.fpar.size() / nbp) || fill(MB.dvec, MBall.dvec, M.dvec.size() / nbp) ||
+          fill(MB.dperm, MBall.dperm, M.dperm.size() / nbp) || fill(MB.mtm, MBall.mtm, M.mtm.size() / nbp) ||
+          fill(MB.fsp, MBall.fsp, M.fsp.size() / nbp) || fill(MB.z, MBall.z, 1))
+        return 1;
+      const size_t rows = nv / nbp;
+      if (ndirty > 0) {
+        const size_t tot = rows * ndirty;
+        hipLaunchKernelGGL(k_start_cols<T>, dim3((unsigned)std::min<size_t>((tot + 255) / 256, 16384)), dim3(256), 0, stream,
+                           B.u, B.up, rows, M.nbp, (const int*)wslots, (const int*)wl0, ndirty);
+      }
+      if (n > ndirty)
+        hipLaunchKernelGGL(k_start_unit<T>, dim3((unsigned)((n - ndirty + 63) / 64)), dim3(64), 0, stream, B.u, M.nbp,
+                           (const int*)wslots + ndirty, (const int*)wl0 + ndirty, n - ndirty);
+      cplx *gp = nullptr, *fp = nullptr;
+      if constexpr (std::is_same<T, cplx>::value) {
+        gp = gprev_d;
+        fp = fprev_d;
+      }
+      hipLaunchKernelGGL(k_member_start<T>, dim3(n), dim3(64), 0, stream, cd, (const int*)wslots, (const int*)wvals, n,
+                         level_host, B.a, B.b, B.a_eff, B.b_eff, B.inv_b, sc3[0], sc3[1], sc3[2], gp, fp, s0_d,
+                         B.closed, conv_d);
+      HCHECK(hipGetLastError());
+    }
+    int nlive = 0, remaining = 0;
+    for (int q = 0; q < nb; ++q)
+      if (slot_chain[q] >= 0 && act[q]) {
+        ++nlive;
+        remaining = std::max(remaining, nl_all[slot_chain[q]] - (level_host - slot_s0[q]));
+      }
+    if (nlive == 0 && qnext >= nchain) {
+      HCHECK(hipStreamSynchronize(stream));
+      *alive = 0;
+      return 0;
+    }
+    std::vector<int> live(nb, 0);
+    for (int q = 0; q < nb; ++q) live[q] = (slot_chain[q] >= 0 && act[q]) ? 1 : 0;
+    if (qnext >= nchain && op->compact) {
+      // the queue is empty: compaction as for a batch
+      const int before = ncompact;
+      if (compact_live(live, nlive, remaining)) return 1;
+      if (ncompact != before) return 0;
+    }
+    // the member tiles with running members
+    std::vector<int> ta(std::max(ntile, 2), 0);
+    for (int q = 0; q < nb; ++q)
+      if (live[q]) ta[q / mt] = 1;
+    HCHECK(hipMemcpyAsync(tact, ta.data(), ta.size() * sizeof(int), hipMemcpyHostToDevice, stream));
+    HCHECK(hipStreamSynchronize(stream));  // the host sources above
+    return 0;
+  }
+
+  int level_cap() const { return win ? (nchain / std::max(1, M.nb) + 2) * (maxdepth + 4) : maxdepth; }
+
   int any_active(int* actv) {
     HCHECK(hipMemcpyAsync(actv, B.active, sizeof(int), hipMemcpyDeviceToHost, stream));
     HCHECK(hipStreamSynchronize(stream));
@@ -3713,6 +4273,36 @@ struct ChainRun {
         fail("fused level: the operator is not symmetric on the symmetric subspace; rerun with the separate update pass");
         return RERUN_ASYM;
       }
+    }
+    if (win) {
+      // the members still holding slots hand over their results, then every chain's row in the caller's order
+      std::vector<int> xs, xc;
+      for (int q = 0; q < M.nb; ++q)
+        if (slot_chain[q] >= 0) {
+          xs.push_back(q);
+          xc.push_back(slot_chain[q]);
+        }
+      if (!xs.empty()) {
+        const int n = (int)xs.size();
+        HCHECK(hipMemcpyAsync(wslots, xs.data(), n * sizeof(int), hipMemcpyHostToDevice, stream));
+        HCHECK(hipMemcpyAsync(wchains, xc.data(), n * sizeof(int), hipMemcpyHostToDevice, stream));
+        hipLaunchKernelGGL(k_extract<T>, dim3(n), dim3(64), 0, stream, a_all, b_all, mused_all, (const T*)B.a, (const T*)B.b,
+                           (const int*)B.mused, ld, (const int*)wslots, (const int*)wchains, n);
+        HCHECK(hipGetLastError());
+      }
+      std::vector<T> ah((size_t)nchain * ld), bh((size_t)nchain * ld);
+      std::vector<int> mu(nchain);
+      HCHECK(hipMemcpyAsync(ah.data(), a_all, ah.size() * sizeof(T), hipMemcpyDeviceToHost, stream));
+      HCHECK(hipMemcpyAsync(bh.data(), b_all, bh.size() * sizeof(T), hipMemcpyDeviceToHost, stream));
+      HCHECK(hipMemcpyAsync(mu.data(), mused_all, mu.size() * sizeof(int), hipMemcpyDeviceToHost, stream));
+      HCHECK(hipStreamSynchronize(stream));
+      for (int q = 0; q < nchain; ++q) {
+        const int c = order[q];
+        std::memcpy(a_out + (size_t)c * ld, ah.data() + (size_t)q * ld, sizeof(T) * ld);
+        std::memcpy(b_out + (size_t)c * ld, bh.data() + (size_t)q * ld, sizeof(T) * ld);
+        mused_out[c] = mu[q];
+      }
+      return 0;
     }
     if (proj && cpr_out) {
       // projections in the caller's order: chain x level x target x orbital
@@ -3778,21 +4368,29 @@ int run_chains_once(asr_gpu_op* op, int nchain, const int* start_orb, const doub
   }
   ChainRun<T> R;
   if (R.setup(op, op->stream, nchain, start_orb, params, pab, frozen_sp, conc, z, nlev, project, ldab,
-              mode == 1 ? 1 : op->tile_pref, tiny, real_chain))
+              mode == 1 ? 1 : op->tile_pref, tiny, real_chain, nullptr, nullptr, mode == 0))
     return 1;
   hipEvent_t e0, e1;
   HCHECK(hipEventCreate(&e0));
   HCHECK(hipEventCreate(&e1));
   HCHECK(hipEventRecord(e0, op->stream));
-  for (int k = 0; k < R.maxdepth; ++k) {
+  const int cap = R.level_cap();
+  for (int k = 0; k < cap; ++k) {
     if (R.step()) return 1;
+    if (mode == 0 && (op->compact || R.win)) {
+      int alive = 1;
+      if (R.compact(&alive)) return 1;
+      if (!alive) break;
+    }
     if (mode == 1) {
       // plain form: the host reads the chain's state back after every level and stops when it has left
+      if (R.flush_test()) return 1;
       int actv = 0;
       if (R.any_active(&actv)) return 1;
       if (!actv) break;
     }
   }
+  if (R.flush_test()) return 1;
   HCHECK(hipEventRecord(e1, op->stream));
   HCHECK(hipEventSynchronize(e1));
   float ms = 0.f;
@@ -3891,8 +4489,18 @@ int run_multi_once(int ngroup, asr_gpu_op* const* ops, const int* ngch, const in
     for (int g = 1; g < ngroup; ++g)
       if (hipStreamWaitEvent(st[g], fork, 0) != hipSuccess) return fail("stream wait");
     for (int k = 0; k < maxd; ++k)
-      for (int g = 0; g < ngroup; ++g)
+      for (int g = 0; g < ngroup; ++g) {
         if (ngch[g] > 0 && k < runs[g].maxdepth && runs[g].step()) return 1;
+        // host-issued levels: member compaction after the levels of the depth test (a group whose members have all
+        // left issues no more levels)
+        if (mode == 0 && ngch[g] > 0 && k < runs[g].maxdepth && ops[g]->compact) {
+          int alive = 1;
+          if (runs[g].compact(&alive)) return 1;
+          if (!alive) runs[g].maxdepth = k + 1;
+        }
+      }
+    for (int g = 0; g < ngroup; ++g)
+      if (ngch[g] > 0 && runs[g].flush_test()) return 1;
     for (int g = 1; g < ngroup; ++g) {
       if (hipEventRecord(join[g], st[g]) != hipSuccess) return fail("event record");
       if (hipStreamWaitEvent(s0, join[g], 0) != hipSuccess) return fail("stream wait");
@@ -3981,6 +4589,7 @@ int run_blocks(asr_gpu_op* op, int nmem, const int* so, const double* params, co
   if (mode == 2) {
     HCHECK(hipStreamBeginCapture(op->stream, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < R.maxdepth && !status; ++k) status = R.step();
+    if (!status) status = R.flush_test();
     hipError_t ce = hipStreamEndCapture(op->stream, &graph);
     if (!status && ce != hipSuccess) status = fail(std::string("graph capture: ") + hipGetErrorString(ce));
     if (!status && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) status = fail("graph instantiate");
@@ -3993,12 +4602,19 @@ int run_blocks(asr_gpu_op* op, int nmem, const int* so, const double* params, co
     HCHECK(hipEventRecord(e0, op->stream));
     for (int k = 0; k < R.maxdepth && !status; ++k) {
       status = R.step();
+      if (!status && mode == 0 && op->compact) {
+        int alive = 1;
+        status = R.compact(&alive);
+        if (!status && !alive) break;
+      }
       if (!status && mode == 1) {
+        status = R.flush_test();
         int actv = 0;
-        status = R.any_active(&actv);
+        if (!status) status = R.any_active(&actv);
         if (!status && !actv) break;
       }
     }
+    if (!status) status = R.flush_test();
     HCHECK(hipEventRecord(e1, op->stream));
   }
   if (!status) {
@@ -4517,6 +5133,25 @@ int asr_gpu_chains_edc_proj(asr_gpu_op* op, int nchain, const int* start_orb, co
 }
 
 // Full reorthogonalisation of the real chains (lanczos reorth = .true., src/reference/asr_recursion.f90:17-58).
+int asr_gpu_op_set_compaction(asr_gpu_op* op, int on) {
+  if (!op) return fail("asr_gpu_op_set_compaction: null handle");
+  op->compact = on != 0;
+  return 0;
+}
+
+int asr_gpu_op_set_window(asr_gpu_op* op, int nslot) {
+  if (!op) return fail("asr_gpu_op_set_window: null handle");
+  if (nslot < 0) return fail("asr_gpu_op_set_window: the number of slots must be >= 0");
+  op->window = nslot;
+  return 0;
+}
+
+int asr_gpu_op_get_compactions(asr_gpu_op* op, int* n) {
+  if (!op || !n) return fail("asr_gpu_op_get_compactions: null argument");
+  *n = op->ncompact;
+  return 0;
+}
+
 int asr_gpu_op_set_level_form(asr_gpu_op* op, int form) {
   if (!op) return fail("asr_gpu_op_set_level_form: null operator");
   if (form != 0 && form != 1) return fail("asr_gpu_op_set_level_form: form 0 (fused level) or 1 (separate update pass)");

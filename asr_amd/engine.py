@@ -41,6 +41,9 @@ def lib() -> C.CDLL:
         L.asr_gpu_op_set_depth_tol.argtypes = [P, C.c_double, C.c_int]
         L.asr_gpu_op_set_reorth.argtypes = [P, C.c_int]
         L.asr_gpu_op_set_level_form.argtypes = [P, C.c_int]
+        L.asr_gpu_op_set_compaction.argtypes = [P, C.c_int]
+        L.asr_gpu_op_get_compactions.argtypes = [P, C.POINTER(C.c_int)]
+        L.asr_gpu_op_set_window.argtypes = [P, C.c_int]
         L.asr_gpu_op_set_targets.argtypes = [P, C.c_int, P, P]
         L.asr_gpu_chains_edc_proj.argtypes = [P, C.c_int, P, P, P, P, P, P, P, C.c_int, C.c_int, P, P, P, P, P, P, P, C.c_int, P]
         L.asr_gpu_op_get_tile.argtypes = [P, C.c_int, C.c_int, P]
@@ -188,6 +191,22 @@ class GpuOperator:
         """Form of an edc chain level: 0 the fused level (a_n from the first operator pass, the update folded into the
         second), 1 the separate update pass in the reference's order (edc_green)."""
         _check(lib().asr_gpu_op_set_level_form(self.h, int(form)), "asr_gpu_op_set_level_form")
+
+    def set_compaction(self, on: bool):
+        """Member compaction of batched chains with the adaptive depth (host-issued levels): the running members move to
+        the front after each level of the depth test, so that member tiles whose members have left stop."""
+        _check(lib().asr_gpu_op_set_compaction(self.h, int(bool(on))), "asr_gpu_op_set_compaction")
+
+    def set_window(self, nslot: int):
+        """Window runs of the batched edc chains with the adaptive depth on the device: nslot member slots, the other
+        chains queued for the slots of members that leave (0: off)."""
+        _check(lib().asr_gpu_op_set_window(self.h, int(nslot)), "asr_gpu_op_set_window")
+
+    def compactions(self) -> int:
+        """Member compactions of the last batched run."""
+        n = C.c_int(0)
+        _check(lib().asr_gpu_op_get_compactions(self.h, C.byref(n)), "asr_gpu_op_get_compactions")
+        return int(n.value)
 
     def set_depth_tol(self, depth_tol: float, tail: str = "hull"):
         """Adaptive depth of the edc chains (ed_depth_tol / ed_tail of edc_green); 0 = every chain to its depth."""

@@ -135,7 +135,7 @@ class ChainGreens:
 def chain_greens(zs, classes: list[ChainClass], comps: list[Component], params: list[dict], nlev: int,
                  route: str = "edc", depth_tol: float = 1.0e-5, tail: str = "hull", project: bool = True,
                  rank: int = 0, nrank: int = 1, mem_cap: float = 48.0 * 2 ** 30, max_batch: int = 112, mode: int = 0,
-                 gpu_ops: list | None = None) -> ChainGreens:
+                 gpu_ops: list | None = None, window: bool = True) -> ChainGreens:
     """G(z) of every chain task at the energies zs (chain_greens / chain_worker, src/asr_class_scf.f90:966-1157).
 
     classes: the class operators (one per sublattice); comps: the components (their class and SRO concentrations);
@@ -146,7 +146,9 @@ def chain_greens(zs, classes: list[ChainClass], comps: list[Component], params: 
     112 by default: a node row of the member-fastest vectors stays within 16 KB, and batches of a few hundred members ran
     about 25 % slower per chain-level on the fcc ternary operator);
     mode: 0 batched, 2 one HIP graph per call; gpu_ops: resident GpuOperators of the classes (else made here and closed
-    at the end; resident ones get their cached level vectors released).
+    at the end; resident ones get their cached level vectors released).  Route edc with the adaptive depth tested on the
+    device (tails hull, avg2, none) in mode 0 runs each class's tasks as one window run of max_batch member slots: a task
+    takes the slot of one that has left, as chain_worker's dynamic schedule hands out the next task (window=False: batches).
 
     A class's tasks run energy-major (energy, spin, component, orbital), so that the members of one member tile share an
     energy: the adaptive depth retires them together and the tile leaves the level loop.  The results do not depend on
@@ -187,8 +189,13 @@ def chain_greens(zs, classes: list[ChainClass], comps: list[Component], params: 
             nbmax = max(1, int(mem_cap // _member_bytes(cl.op, ed3)))
             if max_batch > 0:
                 nbmax = min(nbmax, max_batch)
-            for q0 in range(0, len(mine), nbmax):
-                chunk = mine[q0:q0 + nbmax]
+            # route edc with the adaptive depth tested on the device, host-issued levels: one window run of nbmax slots
+            # over the class's tasks (a chain takes the slot of one that has left); otherwise batches of nbmax
+            win = window and (not ed3) and mode == 0 and depth_tol > 0.0 and tail != "bp" and len(mine) > nbmax
+            G.set_window(nbmax if win else 0)
+            step = len(mine) if win else nbmax
+            for q0 in range(0, len(mine), step):
+                chunk = mine[q0:q0 + step]
                 members, cf, oc = [], [], []
                 for (j, o, pp, gg) in chunk:
                     l0 = cl.iorbk[o]
@@ -217,6 +224,7 @@ def chain_greens(zs, classes: list[ChainClass], comps: list[Component], params: 
                     nlevs += int(mu.sum())
                 nchain += len(chunk) * (2 if ed3 else 1)
                 dev_ms += ms
+            G.set_window(0)
     finally:
         for G in gops:
             if G is not None:

@@ -80,9 +80,25 @@ int asr_gpu_op_set_reorth(asr_gpu_op* op, int reorth);
  * level's two operator passes, a_n = sum_i sq_i^2 [u~ (E + X(z)) u~ + u~ h u~ - (h u~) o (h u~)]_i, equal to sum(u w)
  * on the symmetric subspace, and folds the three-term update, the projection and the norm into the second pass; form 1
  * runs the update as a pass of its own after the operator, in the reference's order.  The coefficients of the two forms
- * agree to rounding.  Form 0 applies to chains on the member-tile operator pass (member tile 8 or 16, an operator whose
- * group keeps every orbital, one d for every species); other chains take form 1. */
+ * agree to rounding.  Form 0 applies to chains on the member-tile operator pass (member tile 8 or 16, one d for every
+ * species, the projection on); other chains take form 1. */
 int asr_gpu_op_set_level_form(asr_gpu_op* op, int form);
+/* Member compaction of the batched chains with the adaptive depth (on != 0, the default): edc_green_blk
+ * (src/asr_edc.f90:237-316) lets every member of a batch leave at its own depth; here, between host-issued levels (mode 0),
+ * after each level of the depth test the members still running move to the front of the batch so that whole member
+ * tiles stop, and the batch stops when none runs.  The coefficients are those of the uncompacted run bit for bit (every
+ * member's arithmetic is unchanged); off keeps every member in its slot. */
+int asr_gpu_op_set_compaction(asr_gpu_op* op, int on);
+/* Window runs (nslot > 0; 0, the default, turns them off): route-edc chains with the adaptive depth tested on the device
+ * (tails hull, avg2, none), issued as host levels (mode 0), with more chains than nslot hold nslot member slots; the other
+ * chains wait in a queue (deepest first) and take the slot of a member as soon as it leaves, at its own level.  This is
+ * the dynamic schedule of chain_worker (src/asr_class_scf.f90:1093-1105) over the members of one batched run: the member
+ * tiles stay full while chains remain, and the vectors of nslot members, not of every chain, are allocated.  Each chain's
+ * coefficients equal those of the batched run to rounding (the lagged normalisation rescales at the run's levels, not
+ * the chain's). */
+int asr_gpu_op_set_window(asr_gpu_op* op, int nslot);
+/* The number of member compactions of the operator's last batched run (diagnostics). */
+int asr_gpu_op_get_compactions(asr_gpu_op* op, int* n);
 /* The tile the autotuner picked for nchain members of the given kind (0: none tuned yet). */
 int asr_gpu_op_get_tile(asr_gpu_op* op, int nchain, int is_complex, int* mt);
 /* Release the level-vector blocks the operator keeps cached between batched runs (the batches of a contour pass reuse
