@@ -194,6 +194,10 @@ def chain_greens(zs, classes: list[ChainClass], comps: list[Component], params: 
             win = window and (not ed3) and mode == 0 and depth_tol > 0.0 and tail != "bp" and len(mine) > nbmax
             G.set_window(nbmax if win else 0)
             step = len(mine) if win else nbmax
+            if win:
+                # the queue in order of the broadening |Im z|: the chains closest to the real axis run longest and go
+                # first, those far from it (which leave after a few test levels) last, so that the run's tail is short
+                mine.sort(key=lambda t: (abs(zs[t[0]].imag), t[0], t[3], t[2], t[1]))
             for q0 in range(0, len(mine), step):
                 chunk = mine[q0:q0 + step]
                 members, cf, oc = [], [], []
