@@ -3578,7 +3578,12 @@ struct ChainRun {
         HCHECK(hipMemcpyAsync(p, one.data(), M.nbp * sizeof(T), hipMemcpyHostToDevice, stream));
       }
     }
-    for (T* q : {B.u, B.up, B.w, B.t1, B.t2, B.h1}) HCHECK(hipMemsetAsync(q, 0, nv * sizeof(T), stream));
+    // u (the start vectors) and u_prev (read by the first update with b = 0) start at zero; the level's outputs w, t1, t2,
+    // h1 are written at every node of a running tile before they are read, and only their padding node's row, which the
+    // gathers read for missing links, must be zero
+    for (T* q : {B.u, B.up}) HCHECK(hipMemsetAsync(q, 0, nv * sizeof(T), stream));
+    for (T* q : {B.w, B.t1, B.t2, B.h1})
+      HCHECK(hipMemsetAsync(q + (size_t)op->nn * 9 * M.nbp, 0, (size_t)9 * M.nbp * sizeof(T), stream));
     HCHECK(hipMemsetAsync(B.a, 0, (size_t)M.nbp * ldab * sizeof(T), stream));
     HCHECK(hipMemsetAsync(B.b, 0, (size_t)M.nbp * ldab * sizeof(T), stream));
     for (T* q : {B.a_eff, B.b_eff, B.inv_b}) HCHECK(hipMemsetAsync(q, 0, M.nbp * sizeof(T), stream));
