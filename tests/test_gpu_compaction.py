@@ -47,12 +47,16 @@ def _run_both(G, fn):
     return off, on, n_off, n_on
 
 
-@pytest.mark.parametrize("tail", ["hull", "bp"])
-def test_edc_compaction_is_bitwise_the_uncompacted_run(edc, tail):
+@pytest.mark.parametrize("tail,tile", [("hull", 8), ("bp", 8), ("hull", 16), ("hull", 64), ("hull", 65)])
+def test_edc_compaction_is_bitwise_the_uncompacted_run(edc, tail, tile):
+    """Every member-tile form: the member-tile pass (8, 16; later levels launch the running tiles alone), one node per
+    wave (64) and lane pairs (65, full grid)."""
     eng = _engine()
     wl, members = edc
+    if tile >= 64:
+        members = members * 3  # three tiles of 64, so that a compaction can free one
     G = eng.GpuOperator(wl.op)
-    G.set_tile(8)
+    G.set_tile(tile)
     G.set_depth_tol(1.0e-5, tail)
     off, on, n_off, n_on = _run_both(G, lambda: G.chains_edc(members, 40, mode=0))
     G.set_depth_tol(0.0)
@@ -135,8 +139,8 @@ def rel(a, b):
     return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / max(np.max(np.abs(np.asarray(b))), 1e-300))
 
 
-@pytest.mark.parametrize("nslot", [8, 24])
-def test_window_run_equals_the_batched_run(edc, nslot):
+@pytest.mark.parametrize("nslot,tile", [(8, 8), (24, 8), (32, 16)])
+def test_window_run_equals_the_batched_run(edc, nslot, tile):
     """60 chains through 8 or 24 member slots (the others queued for the slots of members that leave, at their own
     levels): the depths equal those of the batched run and the coefficients agree to rounding (the lagged normalisation
     rescales at the run's levels), and the oracle's chains to the parity bar."""
@@ -144,7 +148,7 @@ def test_window_run_equals_the_batched_run(edc, nslot):
     from oracle.oracle import OracleOp
     wl, members = edc
     G = eng.GpuOperator(wl.op)
-    G.set_tile(8)
+    G.set_tile(tile)
     G.set_depth_tol(1.0e-5, "hull")
     a0, b0, mu0, _ = G.chains_edc(members, 40, mode=0)
     G.set_window(nslot)

@@ -2,9 +2,9 @@
 
 * D4h (class_ops default, src/asr_symmetry.f90:67-92) permutes p_x / p_y and d_xz / d_yz, so pi is not the identity:
   the operator passes take their orbital-permuting forms (k_tile<.., PI = false> at member tiles 8 and 16, with the fused
-  level's projection over the stabiliser in registers; k_node<T, false> at 64, k_pair<false> at 65, with the update's
-  general projector).  Chains and applications against the oracle at tiles 8, 16, 64 and 65; the fused level against the
-  separate update pass.
+  level's projection over the stabiliser in registers; k_fused at 32, k_node<T, false> at 64, k_pair<false> at 65, with
+  the update's general projector).  Chains and applications against the oracle at tiles 8, 16, 32, 64 and 65; the fused
+  level against the separate update pass.
 * On the GPU alone: the D4h- and D2h-reduced chains equal the chains of the unreduced operator (identity group), and the
   144-step chain of the reduced ordered B2 cell equals the dense solve on the unreduced one, 1e-11
   (test/test_augcls_project.f90:100-150).
@@ -44,7 +44,7 @@ def d4h_ternary():
     return wl, members
 
 
-@pytest.mark.parametrize("tile", [8, 16, 64, 65])
+@pytest.mark.parametrize("tile", [8, 16, 32, 64, 65])
 def test_d4h_edc_chains_every_tile(d4h_ternary, tile):
     eng = _engine()
     wl, members = d4h_ternary
