@@ -323,6 +323,15 @@ struct FArgs {
   const void* upv;
   const void *aeff, *beff, *scv, *scpv;
   const int* act;
+  // k_tile, real members, the pass that writes the output y (O2 = 0, no update): the stored output becomes
+  //   y' = lcc_0 lcv_0 + ... + lcc_{lcn-1} lcv_{lcn-1} + y  (own rows, per-member coefficients; lcc_t null: 1), with the
+  //   partial dot <dotu, y'>, in k_lincomb's operation order; and out3 = rib (y - ra u3 - rb up3) (a recomputed level in
+  //   k_recompute's operation order; ra, rb, rib per member)
+  int lcn;
+  const void* lcv[3];
+  const void* lcc[3];
+  void* out3;
+  const void *u3, *up3, *ra, *rb, *rib;
 };
 
 template <class T>
@@ -725,11 +734,23 @@ __global__ void __launch_bounds__(BLOCK, 3) k_tile(OpDev op, MemDev md, FArgs fa
           const cplx* up = (const cplx*)fa.upv;
           cplx ww = zero<cplx>(), uy = zero<cplx>();
           if constexpr (PI) {
+            // the nine u and u_prev values are requested before the first store: the compiler cannot tell the output
+            // from them, and a load behind a store would wait for the store as well (one vector-memory counter).
+            // Measured on the headline level: the second operator pass 1.97 -> 1.88 ms (writing the nine values to the
+            // accumulators first and storing after, 1.93 ms)
+            cplx uq[9], upq[9];
+#pragma unroll
             for (int l = 0; l < 9; ++l) {
               const size_t q = ((size_t)i * 9 + l) * nbp + m;
-              const cplx y = so * acc[l], uq = uo[q];
-              uy = uy + bilin(uq, y);
-              cplx w = off ? zero<cplx>() : y - bilin(a, uq) - bilin(b, NTO ? nt_load(&up[q]) : up[q]);
+              uq[l] = uo[q];
+              upq[l] = NTO ? nt_load(&up[q]) : up[q];
+            }
+#pragma unroll
+            for (int l = 0; l < 9; ++l) {
+              const size_t q = ((size_t)i * 9 + l) * nbp + m;
+              const cplx y = so * acc[l];
+              uy = uy + bilin(uq[l], y);
+              cplx w = off ? zero<cplx>() : y - bilin(a, uq[l]) - bilin(b, upq[l]);
               if (ns > 1) w = (f[l] / (double)ns) * w;
               out_store<NTO>(&out[q], w);
               ww = ww + bilin(w, w);
@@ -776,11 +797,64 @@ __global__ void __launch_bounds__(BLOCK, 3) k_tile(OpDev op, MemDev md, FArgs fa
           done = true;
         }
       }
+      if constexpr (std::is_same<T, double>::value) {
+        if (fa.lcn > 0 || fa.out3) {
+          // the adjoint's combinations on the output in registers (the separate passes k_lincomb and k_recompute)
+          double lc[3];
+          for (int t = 0; t < 3; ++t) lc[t] = t < fa.lcn ? (fa.lcc[t] ? ((const double*)fa.lcc[t])[m] : 1.0) : 0.0;
+          const double ra = fa.out3 ? ((const double*)fa.ra)[m] : 0.0, rb = fa.out3 ? ((const double*)fa.rb)[m] : 0.0,
+                       rib = fa.out3 ? ((const double*)fa.rib)[m] : 0.0;
+          // every row this epilogue reads is requested before the first store (the stores may alias them for the
+          // compiler, which would otherwise wait for each load in turn)
+          const size_t q0 = (size_t)i * 9 * nbp + m;
+          double z[9], p[9];
+          for (int l = 0; l < 9; ++l) {
+            z[l] = 0.0;
+            p[l] = 0.0;
+          }
+          if (fa.lcn > 0) {
+            for (int t = 0; t < fa.lcn; ++t) {
+              const double* __restrict__ v = (const double*)fa.lcv[t] + q0;
+#pragma unroll
+              for (int l = 0; l < 9; ++l) z[l] = fma(lc[t], nt_load(&v[(size_t)l * nbp]), z[l]);
+            }
+          } else if (fa.out3) {
+            const double* __restrict__ u3 = (const double*)fa.u3 + q0;
+            const double* __restrict__ up3 = (const double*)fa.up3 + q0;
+#pragma unroll
+            for (int l = 0; l < 9; ++l) {
+              z[l] = u3[(size_t)l * nbp];
+              p[l] = nt_load(&up3[(size_t)l * nbp]);
+            }
+          }
+          double dv[9];
+#pragma unroll
+          for (int l = 0; l < 9; ++l) dv[l] = du ? du[q0 + (size_t)l * nbp] : 0.0;
+#pragma unroll
+          for (int l = 0; l < 9; ++l) {
+            const size_t q = q0 + (size_t)l * nbp;
+            const double y = so * acc[l];
+            const double o = fa.lcn > 0 ? fma(1.0, y, z[l]) : y;
+            out_store<NTO>(&out[q], o);
+            if (du) dotv = dotv + bilin(dv[l], o);
+            if (fa.out3) {
+              const double xv = y - bilin(ra, z[l]) - bilin(rb, p[l]);
+              out_store<NTO>(&((double*)fa.out3)[q], bilin(rib, xv));
+            }
+          }
+          done = true;
+        }
+      }
       if (!done) {
+        // the dot's rows before the first store (see the update above)
+        T dv[9];
+#pragma unroll
+        for (int l = 0; l < 9; ++l) dv[l] = du ? ld(du, i, l, nbp, m) : zero<T>();
+#pragma unroll
         for (int l = 0; l < 9; ++l) {
           const T y = so * acc[l];
           out_store<NTO>(&out[((size_t)i * 9 + l) * nbp + m], y);
-          if (du) dotv = dotv + bilin(ld(du, i, l, nbp, m), y);
+          if (du) dotv = dotv + bilin(dv[l], y);
         }
       }
     } else {
@@ -1303,32 +1377,40 @@ __global__ void __launch_bounds__(BLOCK) k_lincomb(OpDev op, MemDev md, double* 
 //   dL/dc_sL += sum_i sq_i y_i . (R_sL x~)_i,   dL/dd_sL += sum_i sq_i^2 [(R_sL y~)_i . t2x_i + t2y_i . (R_sL x~)_i]
 // (the normalised operators are symmetric on the symmetric subspace).  L = l shell (s, p, d).  R_s gathers the node's own
 // state and its flip partners with the class-table weights mt(l0, k, s): every partner is read once and its three
-// per-shell products are spread over the species.  The block's partials are added to part (one slot per block and
-// member, planes q = 0..3 NSP-1 for c, 3 NSP..6 NSP-1 for d): the sum over levels stays in place until one final reduction.
-template <int NSP>
+// per-shell products are spread over the species.  A block walks the node blocks blockIdx.x, blockIdx.x + gridDim.x, ...
+// of the nbx node blocks (gridDim.x a multiple of the 8 XCDs, so every block stays in its XCD's node range), sums its
+// nodes in registers, and adds its 6 NSP sums per member to part (one slot per block and member, planes q = 0..3 NSP-1
+// for c, 3 NSP..6 NSP-1 for d) after one reduction for all of them: the sum over levels stays in place until one final
+// reduction.
+template <int NSP, bool BUF>
 __global__ void __launch_bounds__(BLOCK) k_adj_grad(OpDev op, MemDev md, const double* x, const double* r,
                                                     const double* abar, const double* t2x, const double* t2r, const int* spc,
-                                                    double* part) {
-  __shared__ double red[BLOCK];
-  ASR_NODE_MEMBER
-  double gc[NSP][3], gd[NSP][3];
+                                                    double* part, int nbx) {
+  constexpr int NQ = 6 * NSP, CH = 6;  // quantities per member; reduced CH at a time through LDS
+  __shared__ double red[BLOCK / WAVE][CH][WAVE];
+  const int mt = md.mt, npw = WAVE / mt;
+  const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x >> 6;
+  const int m = blockIdx.y * mt + (lane & (mt - 1));
+  double g[NQ];  // gc[s][L] at 3 s + L, gd[s][L] at 3 NSP + 3 s + L
 #pragma unroll
-  for (int s = 0; s < NSP; ++s)
-#pragma unroll
-    for (int L = 0; L < 3; ++L) { gc[s][L] = 0.0; gd[s][L] = 0.0; }
-  if (live) {
-    const int nbp = md.nbp, nsm = op.nsmax;
+  for (int q = 0; q < NQ; ++q) g[q] = 0.0;
+  const int nbp = md.nbp, nsm = op.nsmax;
+  // BUF (vectors below 4 GiB): the own rows as raw buffer loads, one 32-bit offset per lane and the orbital in the scalar
+  // offset, so the 36 loads of a node are in flight together at a small register cost
+  const int nrec = (int)((uint32_t)(op.nn + 1) * 9u * (uint32_t)nbp * 8u);
+  __amdgpu_buffer_rsrc_t rx, rr, rtx, rtr;
+  if constexpr (BUF) {
+    rx = __builtin_amdgcn_make_buffer_rsrc((void*)x, (short)0, nrec, 0x00020000);
+    rr = __builtin_amdgcn_make_buffer_rsrc((void*)r, (short)0, nrec, 0x00020000);
+    rtx = __builtin_amdgcn_make_buffer_rsrc((void*)t2x, (short)0, nrec, 0x00020000);
+    rtr = __builtin_amdgcn_make_buffer_rsrc((void*)t2r, (short)0, nrec, 0x00020000);
+  }
+  for (int vb = blockIdx.x; vb < nbx; vb += gridDim.x) {
+    const int i = (xcd_block(vb, nbx) * (BLOCK / WAVE) + wave) * npw + lane / mt;
+    if (i >= op.nn || m >= md.nb) continue;
     const uint32_t inf = op.info[i];
     const double sqi = op.sq[i], isq = op.isq[i], ab = abar[m];
     const uint64_t chim = md.chimask[m];
-    double yo[9], tx[9], ty[9];
-#pragma unroll
-    for (int l = 0; l < 9; ++l) {
-      const double xv = ld(x, i, l, nbp, m);
-      yo[l] = ld(r, i, l, nbp, m) + ab * xv;
-      tx[l] = ld(t2x, i, l, nbp, m);
-      ty[l] = ld(t2r, i, l, nbp, m) + ab * tx[l];
-    }
     // one pass per gathered term (the node's own state, then every flip partner): px = R-term of x~, py = of y~, and the
     // per-shell products A = y . px, B = t2x . py + t2y . px spread over the species slots with the weights mt(l0, k, s)
     const int fs = frozen_slot(inf, md, m);
@@ -1337,42 +1419,26 @@ __global__ void __launch_bounds__(BLOCK) k_adj_grad(OpDev op, MemDev md, const d
     const int nsc = fz ? 1 : op.nsc[kap];
     const double* mtk = md.mtm + ((size_t)kap * nsm + l0) * nsm * nsm * nbp;  // [k][sl][nbp]
     const uint32_t* fl = op.flip + (size_t)i * nsm;
-    for (int kk = -1; kk < nsc; ++kk) {
-      int rn = i, k = l0;
-      double cs0 = isq;
-      const int* pg = nullptr;
-      const double* sgg = nullptr;
-      if (kk >= 0) {
-        if (fz || kk == l0) continue;
-        const uint32_t e = fl[kk];
-        if (e == 0u) continue;
-        int g;
-        unpack_link(e, rn, g);
-        k = kk;
-        cs0 = (((chim >> g) & 1ull) ? -1.0 : 1.0) * op.isq[rn];
-        pg = op.pi + g * 9;
-        sgg = op.sg + g * 9;
-      }
-      double A[3] = {0.0, 0.0, 0.0}, B[3] = {0.0, 0.0, 0.0};
+    // the sums of one gathered term into the gradient slots: per species q the weight (0 for the other species), so that
+    // every index into g is static and g stays in registers; adding w A with w = 0 leaves a slot unchanged
+    auto spread = [&](int k, double A0, double A1, double A2, double B0, double B1, double B2) {
+      double wq[NSP];
 #pragma unroll
-      for (int l = 0; l < 9; ++l) {
-        const int L = l == 0 ? 0 : (l < 4 ? 1 : 2);
-        const int pl = pg ? pg[l] : l;
-        const double cs = pg ? cs0 * sgg[l] : cs0;
-        const double xv = ld(x, rn, pl, nbp, m);
-        const double pxl = cs * xv;
-        const double pyl = cs * (ld(r, rn, pl, nbp, m) + ab * xv);
-        A[L] = fma(yo[l], pxl, A[L]);
-        B[L] = fma(tx[l], pyl, fma(ty[l], pxl, B[L]));
-      }
+      for (int q = 0; q < NSP; ++q) wq[q] = 0.0;
       if (fz) {
         // a frozen site carries its species: R_s is the identity there for that species alone
         const int s = md.fsp[(size_t)(fs - 1) * nbp + m];
 #pragma unroll
-        for (int q = 0; q < NSP; ++q)
-          if (q == s)
+        for (int q = 0; q < NSP; ++q) wq[q] = q == s ? 1.0 : 0.0;
 #pragma unroll
-            for (int L = 0; L < 3; ++L) { gc[q][L] += sqi * A[L]; gd[q][L] += sqi * sqi * B[L]; }
+        for (int q = 0; q < NSP; ++q) {
+          g[3 * q + 0] += sqi * wq[q] * A0;
+          g[3 * q + 1] += sqi * wq[q] * A1;
+          g[3 * q + 2] += sqi * wq[q] * A2;
+          g[3 * NSP + 3 * q + 0] += sqi * sqi * wq[q] * B0;
+          g[3 * NSP + 3 * q + 1] += sqi * sqi * wq[q] * B1;
+          g[3 * NSP + 3 * q + 2] += sqi * sqi * wq[q] * B2;
+        }
       } else {
 #pragma unroll
         for (int sl = 0; sl < NSP; ++sl) {
@@ -1380,22 +1446,98 @@ __global__ void __launch_bounds__(BLOCK) k_adj_grad(OpDev op, MemDev md, const d
             const double w = mtk[((size_t)k * nsm + sl) * nbp + m];
             const int s = spc[kap * nsm + sl];
 #pragma unroll
-            for (int q = 0; q < NSP; ++q)
-              if (q == s)
-#pragma unroll
-                for (int L = 0; L < 3; ++L) { gc[q][L] += sqi * w * A[L]; gd[q][L] += sqi * sqi * w * B[L]; }
+            for (int q = 0; q < NSP; ++q) {
+              const double ww = q == s ? w : 0.0;
+              g[3 * q + 0] += sqi * ww * A0;
+              g[3 * q + 1] += sqi * ww * A1;
+              g[3 * q + 2] += sqi * ww * A2;
+              g[3 * NSP + 3 * q + 0] += sqi * sqi * ww * B0;
+              g[3 * NSP + 3 * q + 1] += sqi * sqi * ww * B1;
+              g[3 * NSP + 3 * q + 2] += sqi * sqi * ww * B2;
+            }
           }
         }
       }
+    };
+    // the own state: the gathered values are the node's own x and r; three orbitals' rows in flight at a time
+    {
+      const uint32_t voff = ((uint32_t)i * 9u * (uint32_t)nbp + (uint32_t)m) * 8u, ls = (uint32_t)nbp * 8u;
+      double A[3] = {0.0, 0.0, 0.0}, B[3] = {0.0, 0.0, 0.0};
+#pragma unroll 3
+      for (int l = 0; l < 9; ++l) {
+        double xo, ro, txl, trl;
+        if constexpr (BUF) {
+          xo = tile_row<double>(rx, voff, l * ls);
+          ro = tile_row<double>(rr, voff, l * ls);
+          txl = tile_row<double>(rtx, voff, l * ls);
+          trl = tile_row<double>(rtr, voff, l * ls);
+        } else {
+          xo = ld(x, i, l, nbp, m);
+          ro = ld(r, i, l, nbp, m);
+          txl = ld(t2x, i, l, nbp, m);
+          trl = ld(t2r, i, l, nbp, m);
+        }
+        const int L = l == 0 ? 0 : (l < 4 ? 1 : 2);
+        const double yol = ro + ab * xo;
+        const double tyl = trl + ab * txl;
+        const double pxl = isq * xo;
+        const double pyl = isq * (ro + ab * xo);
+        const double a0 = fma(yol, pxl, L == 0 ? A[0] : (L == 1 ? A[1] : A[2]));
+        const double b0 = fma(txl, pyl, fma(tyl, pxl, L == 0 ? B[0] : (L == 1 ? B[1] : B[2])));
+        if (L == 0) { A[0] = a0; B[0] = b0; } else if (L == 1) { A[1] = a0; B[1] = b0; } else { A[2] = a0; B[2] = b0; }
+      }
+      spread(l0, A[0], A[1], A[2], B[0], B[1], B[2]);
+    }
+    // the flip partners (the 0.7 % of nodes that have them): the own rows again (cache hits) and the partner's rows
+    for (int kk = 0; kk < nsc; ++kk) {
+      if (fz || kk == l0) continue;
+      const uint32_t e = fl[kk];
+      if (e == 0u) continue;
+      int rn, gop;
+      unpack_link(e, rn, gop);
+      const double cs0 = (((chim >> gop) & 1ull) ? -1.0 : 1.0) * op.isq[rn];
+      const int* pg = op.pi + gop * 9;
+      const double* sgg = op.sg + gop * 9;
+      double A[3] = {0.0, 0.0, 0.0}, B[3] = {0.0, 0.0, 0.0};
+#pragma unroll 1
+      for (int l = 0; l < 9; ++l) {
+        const int L = l == 0 ? 0 : (l < 4 ? 1 : 2);
+        const double xo = ld(x, i, l, nbp, m), ro = ld(r, i, l, nbp, m);
+        const double yol = ro + ab * xo;
+        const double txl = ld(t2x, i, l, nbp, m);
+        const double tyl = ld(t2r, i, l, nbp, m) + ab * txl;
+        const int pl = pg[l];
+        const double cs = cs0 * sgg[l];
+        const double xv = ld(x, rn, pl, nbp, m);
+        const double pxl = cs * xv;
+        const double pyl = cs * (ld(r, rn, pl, nbp, m) + ab * xv);
+        const double a0 = fma(yol, pxl, L == 0 ? A[0] : (L == 1 ? A[1] : A[2]));
+        const double b0 = fma(txl, pyl, fma(tyl, pxl, L == 0 ? B[0] : (L == 1 ? B[1] : B[2])));
+        if (L == 0) { A[0] = a0; B[0] = b0; } else if (L == 1) { A[1] = a0; B[1] = b0; } else { A[2] = a0; B[2] = b0; }
+      }
+      spread(kk, A[0], A[1], A[2], B[0], B[1], B[2]);
     }
   }
+  // lanes of one member (lane % mt) summed over the wave's nodes, then over the waves in order, CH quantities per round
   const size_t base = ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * mt;
   const size_t plane = (size_t)gridDim.x * gridDim.y * mt;
 #pragma unroll
-  for (int q = 0; q < NSP * 3; ++q) {
-    block_reduce_member_add<double>(gc[q / 3][q % 3], mt, red, part + (size_t)q * plane + base + threadIdx.x);
+  for (int q = 0; q < NQ; ++q)
+    for (int off = mt; off < WAVE; off <<= 1) g[q] += shfl_xor(g[q], off);
+#pragma unroll
+  for (int q0 = 0; q0 < NQ; q0 += CH) {
+    if (lane < mt) {
+#pragma unroll
+      for (int c = 0; c < CH; ++c) red[wave][c][lane] = g[q0 + c];
+    }
     __syncthreads();
-    block_reduce_member_add<double>(gd[q / 3][q % 3], mt, red, part + (size_t)(NSP * 3 + q) * plane + base + threadIdx.x);
+    for (int t = threadIdx.x; t < CH * mt; t += BLOCK) {
+      const int c = t / mt, ml = t % mt;
+      double s = red[0][c][ml];
+      for (int w = 1; w < BLOCK / WAVE; ++w) s += red[w][c][ml];
+      double* p = part + (size_t)(q0 + c) * plane + base + ml;
+      *p = *p + s;
+    }
     __syncthreads();
   }
 }
@@ -2750,7 +2892,8 @@ struct asr_gpu_op {
   int window = 0;                                 // member slots of a window run (0: every chain holds a slot)
   int ncompact = 0;                               // compactions of the last batched run
   int level_form = 0;                             // edc chains: 0 the fused level (a_k from the first pass, the update
-                                                  // in the second), 1 the separate update pass
+                                                  // in the second), 1 the separate update pass; the adjoint's reverse
+                                                  // sweep: 0 its combinations on the operator's last pass, 1 separate
   bool asym = false;                              // the fused level's check found the operator not symmetric: form 1
   bool slot_flippable[2] = {false, false};        // the site of frozen slot 1 / 2 was flippable (no species may be frozen)
   std::vector<std::pair<long long, int>> tuned;   // (members, complex, uniform d, zero o) key -> member tile
@@ -3056,6 +3199,9 @@ int launch_fused(const asr_gpu_op* op, const MemDev& md, const FArgs& fa, hipStr
   dim3 grid = node_grid(op, md);
   size_t lds = fused_lds_bytes(op, sizeof(T));
   if ((fa.upd || fa.apart) && !tile_kernel(op, md, sizeof(T))) return fail("operator pass: the fused level needs the tile kernel");
+  if ((fa.lcn > 0 || fa.out3) && (!std::is_same<T, double>::value || !tile_kernel(op, md, sizeof(T)) || fa.out2 || fa.upd ||
+                                   (fa.lcn > 0 && fa.out3) || fa.lcn > 3))
+    return fail("operator pass: output combinations need real members, the tile kernel, a plain output and one form");
   if (timer_mark(strm, fa.upd ? KIND_SPMM_UPD : fa.xs ? KIND_SPMM : KIND_RAND, true)) return 1;
   const OpDev od = dev_of(op);
   const double* bl = op->blk;
@@ -3105,11 +3251,29 @@ int launch_fused(const asr_gpu_op* op, const MemDev& md, const FArgs& fa, hipStr
   return 0;
 }
 
+// The output combinations of the last pass of H = c + d S d with species-dependent d (real members, o = E_nu = 0):
+// the FArgs fields lcn ... rib (see there).
+struct RealEpi {
+  int lcn = 0;
+  const double* lcv[3] = {nullptr, nullptr, nullptr};
+  const double* lcc[3] = {nullptr, nullptr, nullptr};
+  double* out3 = nullptr;
+  const double *u3 = nullptr, *up3 = nullptr, *ra = nullptr, *rb = nullptr, *rib = nullptr;
+};
+
+// the last pass of H x writes y itself (no h1 pass): species-dependent d, no o, E_nu or X(z) term, the tile kernel
+bool merged_last_pass(const asr_gpu_op* op, const Members& M, const MemDev& md, size_t es, bool lin) {
+  return !M.uniform_d && M.zero_o && M.zero_e && !lin && es == sizeof(double) && tile_kernel(op, md, es);
+}
+
 // y = H x (augcls_apply) for every member, normalised coordinates; w_dot: partial <dotu, y>.  Work vectors: t1, t2, h1.
-// lin: add X(z) x (complex members, energy-dependent diagonal of route edc).
+// lin: add X(z) x (complex members, energy-dependent diagonal of route edc).  epi: output combinations of the last pass
+// (needs merged_last_pass).
 template <class T>
 int apply_chain_op(const asr_gpu_op* op, const Members& M, const MemDev& md, const T* x, T* y, T* t1, T* t2, T* h1, bool lin,
-                   const T* dotu, T* partial, hipStream_t strm) {
+                   const T* dotu, T* partial, hipStream_t strm, const RealEpi* epi = nullptr) {
+  const bool merged = merged_last_pass(op, M, md, sizeof(T), lin);
+  if (epi && !merged) return fail("operator pass: output combinations need the merged last pass");
   // h1 = h x~ = rand_c(x~) + D S D x~; with the tile kernel the same launch writes t1 = rand_o(h1) at the nodes without
   // flip partners (99 % of them), and k_rand_nodes the rest, in place of a full rand_o pass
   bool t1_done = false;
@@ -3141,6 +3305,19 @@ int apply_chain_op(const asr_gpu_op* op, const Members& M, const MemDev& md, con
     f = fargs0();
     f.xr[0] = x; f.qr[0] = QC; f.r_norm[0] = 1; f.r_coef[0] = 1.0;
     f.xr[1] = t2; f.qr[1] = QD; f.r_norm[1] = 0; f.r_coef[1] = 1.0; f.nr = 2;
+    if (merged) {
+      // y = sq (rand_c(x~) + rand_d(t2)) at once: with no o and E_nu term the last pass would only scale h1 by sq (the
+      // values are bit for bit those of the two passes)
+      f.out = y;
+      f.out_sq = 1;
+      f.dotu = dotu; f.partial = partial;
+      if (epi) {
+        f.lcn = epi->lcn;
+        for (int t = 0; t < 3; ++t) { f.lcv[t] = epi->lcv[t]; f.lcc[t] = epi->lcc[t]; }
+        f.out3 = epi->out3; f.u3 = epi->u3; f.up3 = epi->up3; f.ra = epi->ra; f.rb = epi->rb; f.rib = epi->rib;
+      }
+      return launch_fused<T>(op, md, f, strm);
+    }
     f.out = h1;
     if (launch_fused<T>(op, md, f, strm)) return 1;
   }
@@ -4756,7 +4933,7 @@ int run_adjoint(asr_gpu_op* op, int nchain, const int* start_orb, const double* 
       for (double* p : *b) (void)hipFree(p);
     }
   } freer{&bufs};
-  std::vector<double*> CK(2 * nseg), U(ckpt + 2), W(ckpt), T2(ckpt);
+  std::vector<double*> CK(2 * (nseg - 1)), U(ckpt + 2), W(ckpt), T2(ckpt);  // no checkpoint for the last segment
   for (auto& p : CK) if (vec(&p)) return 1;
   for (auto& p : U) if (vec(&p)) return 1;
   for (auto& p : W) if (vec(&p)) return 1;
@@ -4767,12 +4944,17 @@ int run_adjoint(asr_gpu_op* op, int nchain, const int* start_orb, const double* 
   const int nbx = grid.x, mt = R.M.mt, nbp = R.M.nbp;
   const size_t plane = (size_t)grid.x * grid.y * mt;
   const int nq = 2 * nsp * 3;
+  // the gradient pass walks the node blocks with at most 1024 blocks per member tile (a multiple of the 8 XCDs)
+  const int gxg = std::min(((nbx + 7) / 8) * 8, 1024);
+  const dim3 ggrid(gxg, grid.y);
+  const bool gbuf = buffer_ok(op, R.md, sizeof(double));  // own rows of the gradient pass as 32-bit buffer loads
+  const size_t gplane = (size_t)gxg * grid.y * mt;
   double *P1, *P2, *P3, *PG, *coef, *gacc, *dLa, *dLb;
   for (double** p : {&P1, &P2, &P3}) {
     if (dalloc(p, plane)) return 1;
     bufs.push_back(*p);
   }
-  if (dalloc(&PG, plane * nq) || dalloc(&coef, (size_t)12 * nbp) || dalloc(&gacc, (size_t)nbp * nq) ||
+  if (dalloc(&PG, gplane * nq) || dalloc(&coef, (size_t)12 * nbp) || dalloc(&gacc, (size_t)nbp * nq) ||
       dalloc(&dLa, (size_t)nbp * nlev) || dalloc(&dLb, (size_t)nbp * nlev))
     return 1;
   for (double* p : {PG, coef, gacc, dLa, dLb}) bufs.push_back(p);
@@ -4781,7 +4963,7 @@ int run_adjoint(asr_gpu_op* op, int nchain, const int* start_orb, const double* 
          *abar2 = coef + 9 * nbp;
   HCHECK(hipMemsetAsync(coef, 0, (size_t)12 * nbp * 8, st));
   HCHECK(hipMemsetAsync(gacc, 0, (size_t)nbp * nq * 8, st));
-  HCHECK(hipMemsetAsync(PG, 0, plane * nq * 8, st));
+  HCHECK(hipMemsetAsync(PG, 0, gplane * nq * 8, st));
   HCHECK(hipMemsetAsync(P1, 0, plane * 8, st));
   {
     // loss derivatives in the sorted member order
@@ -4809,13 +4991,33 @@ int run_adjoint(asr_gpu_op* op, int nchain, const int* start_orb, const double* 
     HCHECK(hipGetLastError());
     return 0;
   };
+  // the recomputation and the u_bar combination ride on the operator's last pass (merged_last_pass), else separate passes
+  const bool fuse = op->level_form == 0 && merged_last_pass(op, R.M, md0, sizeof(double), false);
   hipEvent_t e0, e1, e2;
   HCHECK(hipEventCreate(&e0));
   HCHECK(hipEventCreate(&e1));
   HCHECK(hipEventCreate(&e2));
-  // ---- forward with checkpoints
+  // ---- forward with checkpoints; the last segment keeps its level vectors (u_k, H u_k, S D u~_k) in the segment buffers,
+  // so the reverse sweep starts without recomputing it
+  const int klast = (nseg - 1) * ckpt;
   HCHECK(hipEventRecord(e0, st));
   for (int k = 0; k < nlev; ++k) {
+    if (k >= klast) {
+      const int j = k - klast;
+      if (j == 0) {
+        HCHECK(hipMemcpyAsync(U[1], R.B.u, nv * 8, hipMemcpyDeviceToDevice, st));
+        HCHECK(hipMemcpyAsync(U[0], R.B.up, nv * 8, hipMemcpyDeviceToDevice, st));
+      }
+      double* t2s = R.B.t2;
+      R.B.t2 = T2[j];
+      if (R.step_apply()) return 1;
+      R.B.t2 = t2s;
+      HCHECK(hipMemcpyAsync(W[j], R.B.w, nv * 8, hipMemcpyDeviceToDevice, st));  // H u_k before the update
+      if (R.step_update()) return 1;
+      // u_nlev is zero in the forward (a member leaves at its last level); the reverse takes it from the recorded b below
+      if (k + 1 < nlev) HCHECK(hipMemcpyAsync(U[2 + j], R.B.u, nv * 8, hipMemcpyDeviceToDevice, st));
+      continue;
+    }
     if (k % ckpt == 0) {
       HCHECK(hipMemcpyAsync(CK[2 * (k / ckpt)], R.B.u, nv * 8, hipMemcpyDeviceToDevice, st));
       HCHECK(hipMemcpyAsync(CK[2 * (k / ckpt) + 1], R.B.up, nv * 8, hipMemcpyDeviceToDevice, st));
@@ -4826,16 +5028,35 @@ int run_adjoint(asr_gpu_op* op, int nchain, const int* start_orb, const double* 
   // ---- reverse: u_bar_nlev = 0, r_bar_nlev = 0, <u_bar_nlev, u_nlev> = 0 (P1 cleared)
   for (int seg = nseg - 1; seg >= 0; --seg) {
     const int k0 = seg * ckpt, k1 = std::min(nlev, k0 + ckpt) - 1;
-    // U[0] = u_{k0 - 1}, U[1 + j] = u_{k0 + j}
-    HCHECK(hipMemcpyAsync(U[1], CK[2 * seg], nv * 8, hipMemcpyDeviceToDevice, st));
-    HCHECK(hipMemcpyAsync(U[0], CK[2 * seg + 1], nv * 8, hipMemcpyDeviceToDevice, st));
-    for (int k = k0; k <= k1; ++k) {
+    // U[0] = u_{k0 - 1}, U[1 + j] = u_{k0 + j} (the last segment's are in place from the forward)
+    if (seg != nseg - 1) {
+      HCHECK(hipMemcpyAsync(U[1], CK[2 * seg], nv * 8, hipMemcpyDeviceToDevice, st));
+      HCHECK(hipMemcpyAsync(U[0], CK[2 * seg + 1], nv * 8, hipMemcpyDeviceToDevice, st));
+    } else {
+      // u_nlev = (H u_k1 - a_k1 u_k1 - b_k1-1 u_k1-1) / b_k1 as a recomputed level would give it
+      const int j = k1 - k0;
+      hipLaunchKernelGGL(k_adj_coef, dim3((nbp + 63) / 64), dim3(64), 0, st, R.M.nb, nlev, k1, R.B.a, R.B.b, a_eff, b_prev,
+                         inv_b, nega, negb);
+      hipLaunchKernelGGL(k_recompute, grid, dim3(BLOCK), 0, st, od, md0, U[2 + j], W[j], U[1 + j], U[j], a_eff, b_prev, inv_b);
+      HCHECK(hipGetLastError());
+    }
+    for (int k = k0; k <= k1 && seg != nseg - 1; ++k) {
       const int j = k - k0;
       hipLaunchKernelGGL(k_adj_coef, dim3((nbp + 63) / 64), dim3(64), 0, st, R.M.nb, nlev, k, R.B.a, R.B.b, a_eff, b_prev,
                          inv_b, nega, negb);
-      if (apply_chain_op<double>(op, R.M, md0, U[1 + j], W[j], R.B.t1, T2[j], R.B.h1, false, nullptr, nullptr, st)) return 1;
-      hipLaunchKernelGGL(k_recompute, grid, dim3(BLOCK), 0, st, od, md0, U[2 + j], W[j], U[1 + j], U[j], a_eff, b_prev, inv_b);
-      HCHECK(hipGetLastError());
+      if (fuse) {
+        // u_{k+1} from the operator's output in the same pass
+        RealEpi e;
+        e.out3 = U[2 + j]; e.u3 = U[1 + j]; e.up3 = U[j]; e.ra = a_eff; e.rb = b_prev; e.rib = inv_b;
+        if (apply_chain_op<double>(op, R.M, md0, U[1 + j], W[j], R.B.t1, T2[j], R.B.h1, false, nullptr, nullptr, st, &e))
+          return 1;
+      } else {
+        if (apply_chain_op<double>(op, R.M, md0, U[1 + j], W[j], R.B.t1, T2[j], R.B.h1, false, nullptr, nullptr, st))
+          return 1;
+        hipLaunchKernelGGL(k_recompute, grid, dim3(BLOCK), 0, st, od, md0, U[2 + j], W[j], U[1 + j], U[j], a_eff, b_prev,
+                           inv_b);
+        HCHECK(hipGetLastError());
+      }
     }
     for (int k = k1; k >= k0; --k) {
       const int j = k - k0;
@@ -4850,21 +5071,38 @@ int run_adjoint(asr_gpu_op* op, int nchain, const int* start_orb, const double* 
       hipLaunchKernelGGL(k_adj_fin2, dim3(R.M.nb), dim3(BLOCK), 0, st, R.M.nb, nlev, k, P2, P3, nbx, mt, dLa, abar, abar2,
                          cbb);
       // w_bar = r_bar + a_bar u_k enters linearly: H w_bar = H r_bar + a_bar w_k, S D w~_bar = S D r~_bar + a_bar S D u~_k
-      if (apply_chain_op<double>(op, R.M, md0, rbar, hr, R.B.t1, t2r, R.B.h1, false, nullptr, nullptr, st)) return 1;
-      switch (nsp) {
-        case 1: hipLaunchKernelGGL(k_adj_grad<1>, grid, dim3(BLOCK), 0, st, od, md0, U[1 + j], rbar, abar, T2[j], t2r, spc_d, PG); break;
-        case 2: hipLaunchKernelGGL(k_adj_grad<2>, grid, dim3(BLOCK), 0, st, od, md0, U[1 + j], rbar, abar, T2[j], t2r, spc_d, PG); break;
-        case 3: hipLaunchKernelGGL(k_adj_grad<3>, grid, dim3(BLOCK), 0, st, od, md0, U[1 + j], rbar, abar, T2[j], t2r, spc_d, PG); break;
-        case 4: hipLaunchKernelGGL(k_adj_grad<4>, grid, dim3(BLOCK), 0, st, od, md0, U[1 + j], rbar, abar, T2[j], t2r, spc_d, PG); break;
-        default: hipLaunchKernelGGL(k_adj_grad<5>, grid, dim3(BLOCK), 0, st, od, md0, U[1 + j], rbar, abar, T2[j], t2r, spc_d, PG); break;
+      // fused: the operator pass on r_bar writes u_bar_k = -b_k r_bar_{k+1} - a_k r_bar_k + 2 a_bar_k w_k + H r_bar_k
+      // with <u_bar_k, u_k> (below) at once
+      if (fuse) {
+        RealEpi e;
+        e.lcn = 3;
+        e.lcv[0] = rprev; e.lcc[0] = negb;
+        e.lcv[1] = rbar; e.lcc[1] = nega;
+        e.lcv[2] = W[j]; e.lcc[2] = abar2;
+        if (apply_chain_op<double>(op, R.M, md0, rbar, ubar, R.B.t1, t2r, R.B.h1, false, U[1 + j], P1, st, &e)) return 1;
+      } else if (apply_chain_op<double>(op, R.M, md0, rbar, hr, R.B.t1, t2r, R.B.h1, false, nullptr, nullptr, st)) {
+        return 1;
       }
+#define ASR_GRAD(N) do { if (gbuf) hipLaunchKernelGGL((k_adj_grad<N, true>), ggrid, dim3(BLOCK), 0, st, od, md0, U[1 + j], rbar, abar, \
+                                                    T2[j], t2r, spc_d, PG, nbx); \
+                            else hipLaunchKernelGGL((k_adj_grad<N, false>), ggrid, dim3(BLOCK), 0, st, od, md0, U[1 + j], rbar, \
+                                                    abar, T2[j], t2r, spc_d, PG, nbx); } while (0)
+      switch (nsp) {
+        case 1: ASR_GRAD(1); break;
+        case 2: ASR_GRAD(2); break;
+        case 3: ASR_GRAD(3); break;
+        case 4: ASR_GRAD(4); break;
+        default: ASR_GRAD(5); break;
+      }
+#undef ASR_GRAD
       // u_bar_k = -b_k r_bar_{k+1} - a_k r_bar_k + 2 a_bar_k w_k + H r_bar_k, with <u_bar_k, u_k> for the next level's b_bar
-      if (lincomb(ubar, 4, rprev, negb, rbar, nega, W[j], abar2, hr, nullptr, 1, U[1 + j], P1, nullptr, nullptr)) return 1;
+      if (!fuse && lincomb(ubar, 4, rprev, negb, rbar, nega, W[j], abar2, hr, nullptr, 1, U[1 + j], P1, nullptr, nullptr))
+        return 1;
       std::swap(unext, ubar);
       std::swap(rprev, rbar);
     }
   }
-  hipLaunchKernelGGL(k_adj_fin_grad, dim3(R.M.nb), dim3(BLOCK), 0, st, R.M.nb, nq, PG, nbx, mt, plane, gacc);
+  hipLaunchKernelGGL(k_adj_fin_grad, dim3(R.M.nb), dim3(BLOCK), 0, st, R.M.nb, nq, PG, gxg, mt, gplane, gacc);
   HCHECK(hipEventRecord(e2, st));
   HCHECK(hipGetLastError());
   HCHECK(hipEventSynchronize(e2));

@@ -81,7 +81,9 @@ int asr_gpu_op_set_reorth(asr_gpu_op* op, int reorth);
  * on the symmetric subspace, and folds the three-term update, the projection and the norm into the second pass; form 1
  * runs the update as a pass of its own after the operator, in the reference's order.  The coefficients of the two forms
  * agree to rounding.  Form 0 applies to chains on the member-tile operator pass (member tile 8 or 16, one d for every
- * species, the projection on); other chains take form 1. */
+ * species, the projection on); other chains take form 1.  For asr_gpu_lanczos_adjoint form 0 (on the member-tile pass)
+ * computes the recomputed level u_k+1 and the combination u_bar_k on the operator's last pass instead of in passes of
+ * their own; the gradients agree to rounding. */
 int asr_gpu_op_set_level_form(asr_gpu_op* op, int form);
 /* Member compaction of the batched chains with the adaptive depth (on != 0, the default): edc_green_blk
  * (src/asr_edc.f90:237-316) lets every member of a batch leave at its own depth; here, between host-issued levels (mode 0),

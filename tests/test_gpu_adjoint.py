@@ -146,3 +146,30 @@ def test_real_chains_beyond_2gb_vectors():
         ar, br, n = O.lanczos(mem[j], mem[j].l0, nlev, tiny=0.0)
         assert np.max(np.abs(a16[j, :n] - ar[:n])) < 1e-9 * np.max(np.abs(ar[:n]))
         assert np.max(np.abs(b16[j, :n] - br[:n])) < 1e-9 * np.max(np.abs(br[:n]))
+
+
+def test_adjoint_fused_and_separate_forms(wl):
+    """The reverse sweep with the recomputed level u_k+1 and the combination u_bar_k written by the operator's last pass
+    (level form 0 on the member-tile pass, tiles 8 and 16) against the separate passes (form 1) and against the
+    one-node-per-wave kernel (tile 64, which keeps the separate passes): the same forward coefficients and gradients to
+    rounding."""
+    eng = _engine()
+    nlev = int(wl.nlev[0])
+    nch = len(wl.members)
+    rng = np.random.default_rng(8)
+    la = rng.standard_normal((nch, nlev))
+    lb = rng.standard_normal((nch, nlev))
+    mem = AD.members_at(wl.members, wl.info["theta"], wl.info["c_offset"])
+    G = eng.GpuOperator(wl.op)
+    res = {}
+    for form, tile in ((0, 16), (1, 16), (0, 8), (1, 8), (0, 64)):
+        G.set_level_form(form)
+        G.set_tile(tile)
+        res[(form, tile)] = G.lanczos_adjoint(mem, nlev, la, lb, ckpt=4)
+    G.set_level_form(0)
+    G.set_tile(0)
+    a0, b0, g0, _ = res[(0, 16)]
+    for key, (a, b, g, _) in res.items():
+        assert np.max(np.abs(a - a0)) < 1e-12 * np.max(np.abs(a0)), key
+        assert np.max(np.abs(b - b0)) < 1e-12 * np.max(np.abs(b0)), key
+        assert np.max(np.abs(g - g0)) < 1e-11 * np.max(np.abs(g0)), key
