@@ -903,10 +903,11 @@ __global__ void __launch_bounds__(BLOCK, 3) k_tile(OpDev op, MemDev md, FArgs fa
 // The species-diagonal term y = coef X_q x (rand_ct without normalisation) at the listed nodes only: the nodes with flip
 // partners, whose rows k_tile's out2 leaves out.  One thread per (listed node, member), members fastest.  Same operation
 // order as rand_ct, so the rows equal those of a full pass.
-template <class T>
-__device__ inline void rand_node_term(const OpDev& op, const MemDev& md, int i, uint32_t inf, int m, const T* __restrict__ x,
-                                      int q, double coef, int xnorm, bool lin, T acc[9]) {
-  // rand_ct for one (node, member) with plain loads (acc += coef X_q x; lin: coef (z X_q + X_q+1) x, complex members)
+template <class T, class F>
+__device__ inline void rand_node_rows(const OpDev& op, const MemDev& md, int i, uint32_t inf, int m, F xrow, int q,
+                                      double coef, int xnorm, bool lin, T acc[9]) {
+  // rand_ct for one (node, member), the rows of x from xrow(node, row) (acc += coef X_q x; lin: coef (z X_q + X_q+1) x,
+  // complex members)
   const int nbp = md.nbp, nsm = op.nsmax;
   const int fs = frozen_slot(inf, md, m);
   const double isi = xnorm ? op.isq[i] : 1.0;
@@ -925,15 +926,14 @@ __device__ inline void rand_node_term(const OpDev& op, const MemDev& md, int i, 
   if (fs) {
     const double* fa_ = frozen_row(md, fs, q) + m;
     const double* fb_ = frozen_row(md, fs, qb) + m;
-    for (int l = 0; l < 9; ++l)
-      add(fa_[(size_t)l * nbp], lin ? fb_[(size_t)l * nbp] : 0.0, isi * x[((size_t)i * 9 + l) * nbp + m], l);
+    for (int l = 0; l < 9; ++l) add(fa_[(size_t)l * nbp], lin ? fb_[(size_t)l * nbp] : 0.0, isi * xrow(i, l), l);
     return;
   }
   const int l0 = inf & 15u, kap = (inf >> 4) & 255u;
   const double* wa_ = md.wts + (((size_t)q * op.ncls + kap) * nsm + l0) * nsm * 9 * nbp + m;
   const double* wb_ = md.wts + (((size_t)qb * op.ncls + kap) * nsm + l0) * nsm * 9 * nbp + m;
   for (int l = 0; l < 9; ++l)
-    add(wa_[((size_t)l0 * 9 + l) * nbp], lin ? wb_[((size_t)l0 * 9 + l) * nbp] : 0.0, isi * x[((size_t)i * 9 + l) * nbp + m], l);
+    add(wa_[((size_t)l0 * 9 + l) * nbp], lin ? wb_[((size_t)l0 * 9 + l) * nbp] : 0.0, isi * xrow(i, l), l);
   const uint64_t chim = md.chimask[m];
   const int nsc = op.nsc[kap];
   const uint32_t* fl = op.flip + (size_t)i * nsm;
@@ -945,8 +945,17 @@ __device__ inline void rand_node_term(const OpDev& op, const MemDev& md, int i, 
     const double chi = ((chim >> g) & 1ull) ? -1.0 : 1.0;
     for (int l = 0; l < 9; ++l)
       add(wa_[((size_t)k * 9 + l) * nbp], lin ? wb_[((size_t)k * 9 + l) * nbp] : 0.0,
-          (isr * (chi * op.sg[g * 9 + l])) * x[((size_t)r * 9 + op.pi[g * 9 + l]) * nbp + m], l);
+          (isr * (chi * op.sg[g * 9 + l])) * xrow(r, op.pi[g * 9 + l]), l);
   }
+}
+
+// rand_ct for one (node, member) with plain loads of x
+template <class T>
+__device__ inline void rand_node_term(const OpDev& op, const MemDev& md, int i, uint32_t inf, int m, const T* __restrict__ x,
+                                      int q, double coef, int xnorm, bool lin, T acc[9]) {
+  const int nbp = md.nbp;
+  rand_node_rows<T>(op, md, i, inf, m, [&](int r, int l) -> T { return x[((size_t)r * 9 + l) * nbp + m]; }, q, coef, xnorm,
+                    lin, acc);
 }
 
 // y = terms at the listed nodes: c2x X_q2x x2 (when x2) + c2 X_q2 x, the form of k_tile's out2 (O2 = 1: x2 null,
@@ -1555,6 +1564,44 @@ __global__ void __launch_bounds__(BLOCK) k_recompute(OpDev op, MemDev md, double
     const double xv = w[q] - bilin(a, u[q]) - bilin(b, up[q]);
     out[q] = bilin(ib, xv);
   }
+}
+
+// The first pass of H r_bar with r_bar itself: r_bar = alpha u_bar_{k+1} + beta u_{k+1} (k_lincomb's operation order) is
+// formed at the rows the pass reads, written once (its own rows), with the partials of <r_bar, u_k> and <r_bar, u_k-1>,
+// and t1 = rand_d(r_bar~) (rand_ct's order): the rows and partials of the two separate passes bit for bit, without the
+// write and re-read of r_bar between them
+__global__ void __launch_bounds__(BLOCK) k_adj_dpass(OpDev op, MemDev md, const double* __restrict__ ub,
+                                                     const double* __restrict__ u2, const double* __restrict__ alpha,
+                                                     const double* __restrict__ beta, const double* __restrict__ u1,
+                                                     const double* __restrict__ u0, double* __restrict__ rb,
+                                                     double* __restrict__ t1, double* part2, double* part3) {
+  __shared__ double red[BLOCK];
+  ASR_NODE_MEMBER
+  double d0 = 0.0, d1 = 0.0;
+  if (live) {
+    const int nbp = md.nbp;
+    const double al = alpha[m], be = beta[m];
+    auto xrow = [&](int r, int l) -> double {
+      const size_t q = ((size_t)r * 9 + l) * nbp + m;
+      return fma(be, u2[q], fma(al, ub[q], 0.0));
+    };
+    double acc[9];
+#pragma unroll
+    for (int l = 0; l < 9; ++l) {
+      const size_t q = ((size_t)i * 9 + l) * nbp + m;
+      const double y = xrow(i, l);
+      rb[q] = y;
+      d0 = fma(u1[q], y, d0);
+      d1 = fma(u0[q], y, d1);
+      acc[l] = 0.0;
+    }
+    rand_node_rows<double>(op, md, i, op.info[i], m, xrow, QD, 1.0, 1, false, acc);
+#pragma unroll
+    for (int l = 0; l < 9; ++l) t1[((size_t)i * 9 + l) * nbp + m] = acc[l];
+  }
+  block_reduce_member<double>(d0, mt, red, part2 + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * mt + threadIdx.x);
+  __syncthreads();
+  block_reduce_member<double>(d1, mt, red, part3 + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * mt + threadIdx.x);
 }
 
 // per member: the forward coefficients of level k (a_k, b_{k-1}, 1 / b_k, -a_k, -b_k)
@@ -3254,6 +3301,7 @@ int launch_fused(const asr_gpu_op* op, const MemDev& md, const FArgs& fa, hipStr
 // The output combinations of the last pass of H = c + d S d with species-dependent d (real members, o = E_nu = 0):
 // the FArgs fields lcn ... rib (see there).
 struct RealEpi {
+  bool t1_ready = false;  // t1 = rand_d(x~) is given (k_adj_dpass): the first pass is skipped
   int lcn = 0;
   const double* lcv[3] = {nullptr, nullptr, nullptr};
   const double* lcc[3] = {nullptr, nullptr, nullptr};
@@ -3298,7 +3346,7 @@ int apply_chain_op(const asr_gpu_op* op, const Members& M, const MemDev& md, con
   } else {
     FArgs f = fargs0();
     f.xr[0] = x; f.qr[0] = QD; f.r_norm[0] = 1; f.r_coef[0] = 1.0; f.nr = 1; f.out = t1;
-    if (launch_fused<T>(op, md, f, strm)) return 1;
+    if (!(epi && epi->t1_ready) && launch_fused<T>(op, md, f, strm)) return 1;
     f = fargs0();
     f.xs = t1; f.s_coef = 1.0; f.out = t2;
     if (launch_fused<T>(op, md, f, strm)) return 1;
@@ -5066,8 +5114,14 @@ int run_adjoint(asr_gpu_op* op, int nchain, const int* start_orb, const double* 
       // <r_bar, u_{k-1}>, then a_bar_k
       hipLaunchKernelGGL(k_adj_fin1, dim3(R.M.nb), dim3(BLOCK), 0, st, R.M.nb, nlev, k, P1, nbx, mt, R.B.b, dLb, cbb, alpha,
                          beta);
-      if (lincomb(rbar, 2, unext, alpha, U[2 + j], beta, nullptr, nullptr, nullptr, nullptr, 2, U[1 + j], P2, U[j], P3))
+      if (fuse) {
+        // r_bar with its dots and the operator's first pass on it (t1) in one sweep
+        hipLaunchKernelGGL(k_adj_dpass, grid, dim3(BLOCK), 0, st, od, md0, unext, U[2 + j], alpha, beta, U[1 + j], U[j], rbar,
+                           R.B.t1, P2, P3);
+        HCHECK(hipGetLastError());
+      } else if (lincomb(rbar, 2, unext, alpha, U[2 + j], beta, nullptr, nullptr, nullptr, nullptr, 2, U[1 + j], P2, U[j], P3)) {
         return 1;
+      }
       hipLaunchKernelGGL(k_adj_fin2, dim3(R.M.nb), dim3(BLOCK), 0, st, R.M.nb, nlev, k, P2, P3, nbx, mt, dLa, abar, abar2,
                          cbb);
       // w_bar = r_bar + a_bar u_k enters linearly: H w_bar = H r_bar + a_bar w_k, S D w~_bar = S D r~_bar + a_bar S D u~_k
@@ -5075,6 +5129,7 @@ int run_adjoint(asr_gpu_op* op, int nchain, const int* start_orb, const double* 
       // with <u_bar_k, u_k> (below) at once
       if (fuse) {
         RealEpi e;
+        e.t1_ready = true;
         e.lcn = 3;
         e.lcv[0] = rprev; e.lcc[0] = negb;
         e.lcv[1] = rbar; e.lcc[1] = nega;
