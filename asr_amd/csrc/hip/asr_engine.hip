@@ -2048,7 +2048,8 @@ __global__ void k_perm_rows(E* __restrict__ dst, const E* __restrict__ src, int 
 template <class T>
 __global__ void k_extract(T* __restrict__ a_all, T* __restrict__ b_all, int* __restrict__ mused_all, const T* __restrict__ a,
                           const T* __restrict__ b, const int* __restrict__ mused, int ld, const int* __restrict__ slots,
-                          const int* __restrict__ chains, int n) {
+                          const int* __restrict__ chains, int n, const int* __restrict__ nd = nullptr) {
+  if (nd) n = *nd;  // the count from the device queue
   const int j = blockIdx.x;
   if (j >= n) return;
   const int sl = slots[j], c = chains[j];
@@ -2060,7 +2061,9 @@ __global__ void k_extract(T* __restrict__ a_all, T* __restrict__ b_all, int* __r
 }
 template <class E>
 __global__ void k_fill_cols(E* __restrict__ dst, const E* __restrict__ src, size_t nrows, int nbpd, int nbps,
-                            const int* __restrict__ slots, const int* __restrict__ chains, int n) {
+                            const int* __restrict__ slots, const int* __restrict__ chains, int n,
+                            const int* __restrict__ nd = nullptr) {
+  if (nd) n = *nd;
   const size_t tot = nrows * (size_t)n;
   for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += (size_t)gridDim.x * blockDim.x) {
     const size_t r = t / (size_t)n;
@@ -2070,7 +2073,8 @@ __global__ void k_fill_cols(E* __restrict__ dst, const E* __restrict__ src, size
 }
 template <class T>
 __global__ void k_start_cols(T* __restrict__ u, T* __restrict__ up, size_t nrows, int nbp, const int* __restrict__ slots,
-                             const int* __restrict__ l0, int n) {
+                             const int* __restrict__ l0, int n, const int* __restrict__ nd = nullptr) {
+  if (nd) n = *nd;
   const size_t tot = nrows * (size_t)n;
   for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += (size_t)gridDim.x * blockDim.x) {
     const size_t r = t / (size_t)n;
@@ -2084,7 +2088,12 @@ __global__ void k_start_cols(T* __restrict__ u, T* __restrict__ up, size_t nrows
 template <class T>
 __global__ void k_member_start(ChainDev cd, const int* __restrict__ slots, const int* __restrict__ depth, int n, int s0,
                                T* a, T* b, T* a_eff, T* b_eff, T* inv_b, T* sc0, T* sc1, T* sc2, cplx* gprev, cplx* fprev,
-                               int* s0v, int* closed, int* conv) {
+                               int* s0v, int* closed, int* conv, const int* __restrict__ nd = nullptr) {
+  if (nd) {
+    // the device queue: the count from the queue, the start level from the level counter
+    n = *nd;
+    s0 = *cd.level;
+  }
   const int j = blockIdx.x;
   if (j >= n) return;
   const int sl = slots[j];
@@ -2114,12 +2123,80 @@ __global__ void k_member_start(ChainDev cd, const int* __restrict__ slots, const
 // the start vectors of new members in slots whose vectors are zero (the member before left by the depth test or at its
 // depth, so its last update wrote zeros): u = 1 on the start orbital's row of the start node
 template <class T>
-__global__ void k_start_unit(T* __restrict__ u, int nbp, const int* __restrict__ slots, const int* __restrict__ l0, int n) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_start_unit(T* __restrict__ u, int nbp, const int* __restrict__ slots, const int* __restrict__ l0, int n,
+                             const int* __restrict__ rng = nullptr) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (rng) {
+    // the device queue: entries rng[0] .. rng[1] - 1 of the start list
+    j += rng[0];
+    n = rng[1];
+  }
   if (j >= n) return;
   T v = zero<T>();
   *(double*)&v = 1.0;
   u[(size_t)l0[j] * nbp + slots[j]] = v;
+}
+
+// The window run's queue on the device (one thread; the slots are at most a few hundred): after a level, the members that
+// have left hand over their slots (extraction list xs -> xc; a slot whose chain ended early, closed, keeps a non-zero
+// vector and is flagged dirty), the free slots take the next chains of the queue in slot order, and the start list is
+// written dirty slots first (they need a full start column) then clean ones, each in slot order: the lists, counts and
+// tile activity of the host's refill, without a host round trip.  wq: [0] next chain, [1] extractions, [2] dirty starts,
+// [3] starts, [4] live members, [5] 1 while anything remains to run.
+__global__ void k_refill(ChainDev cd, int nb, int mt, int nchain, int* wq, int* slot_chain, int* slot_dirty,
+                         const int* nl_all, const int* so_all, int* xs, int* xc, int* ws, int* wc, int* wv, int* wl0, int* tact,
+                         int ntile) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int qn = wq[0], nx = 0;
+  for (int q = 0; q < nb; ++q) {
+    const int c = slot_chain[q];
+    if (c >= 0 && !cd.active[q]) {
+      xs[nx] = q;
+      xc[nx] = c;
+      ++nx;
+      slot_chain[q] = -1;
+      if (cd.closed && cd.closed[q]) slot_dirty[q] = 1;
+    }
+  }
+  // the slots that take a chain now, in slot order: count the dirty ones first so the list can be written partitioned
+  int nfree_take = 0, ndirty = 0;
+  for (int q = 0; q < nb && qn + nfree_take < nchain; ++q)
+    if (slot_chain[q] < 0) {
+      ++nfree_take;
+      ndirty += slot_dirty[q] ? 1 : 0;
+    }
+  int jd = 0, jc = ndirty;
+  for (int q = 0; q < nb && qn < nchain; ++q)
+    if (slot_chain[q] < 0) {
+      const int c = qn++;
+      slot_chain[q] = c;
+      const int j = slot_dirty[q] ? jd++ : jc++;
+      slot_dirty[q] = 0;
+      ws[j] = q;
+      wc[j] = c;
+      wv[j] = nl_all[c];
+      wl0[j] = so_all[c];
+    }
+  wq[0] = qn;
+  wq[1] = nx;
+  wq[2] = ndirty;
+  wq[3] = nfree_take;
+  // running members: those still active, and the new ones with levels to run
+  int nlive = 0;
+  for (int t = 0; t < ntile; ++t) tact[t] = 0;
+  for (int q = 0; q < nb; ++q) {
+    int live = 0;
+    if (slot_chain[q] >= 0) live = cd.active[q] ? 1 : 0;
+    tact[q / mt] |= live;
+    nlive += live;
+  }
+  for (int j = 0; j < nfree_take; ++j)
+    if (wv[j] > 0) {
+      tact[ws[j] / mt] = 1;
+      ++nlive;
+    }
+  wq[4] = nlive;
+  wq[5] = (nlive > 0 || qn < nchain) ? 1 : 0;
 }
 
 __global__ void k_level_end(int* level, const int* active, int nb, int mt, int* tact, int ntile) {
@@ -2937,6 +3014,7 @@ struct asr_gpu_op {
   bool reorth = false;                            // full reorthogonalisation of the real chains (lanczos reorth)
   bool compact = true;                            // member compaction between host-issued levels (adaptive depth)
   int window = 0;                                 // member slots of a window run (0: every chain holds a slot)
+  bool queue_dev = true;                          // the window run's queue on the device (k_refill) while chains wait
   int ncompact = 0;                               // compactions of the last batched run
   int level_form = 0;                             // edc chains: 0 the fused level (a_k from the first pass, the update
                                                   // in the second), 1 the separate update pass; the adjoint's reverse
@@ -3636,6 +3714,12 @@ struct ChainRun {
   int qnext = 0;
   T *a_all = nullptr, *b_all = nullptr;
   int *mused_all = nullptr, *s0_d = nullptr, *wslots = nullptr, *wchains = nullptr, *wvals = nullptr, *wl0 = nullptr;
+  // the queue on the device (op->queue_dev): k_refill after every level while chains wait, the host reads its counters
+  // every 8 levels; once the queue is empty the run continues on the host path (refill, then compaction).  wq_d: [0] next
+  // chain, [1] extractions, [2] dirty starts, [3] starts, [4] live members, [5] 1 while anything remains
+  bool dev_queue = false;
+  int *wq_d = nullptr, *slot_chain_d = nullptr, *slot_dirty_d = nullptr, *nl_all_d = nullptr, *so_all_d = nullptr,
+      *xslots_d = nullptr, *xchains_d = nullptr;
 
   ~ChainRun() {
     // work of an aborted run may still be queued: the cached blocks go back only once the stream is idle
@@ -3649,7 +3733,8 @@ struct ChainRun {
     if (ev_t) (void)hipEventDestroy(ev_t);
     if (tstream) (void)hipStreamDestroy(tstream);
     for (void* p : {(void*)a_all, (void*)b_all, (void*)mused_all, (void*)s0_d, (void*)wslots, (void*)wchains, (void*)wvals,
-                    (void*)wl0})
+                    (void*)wl0, (void*)wq_d, (void*)slot_chain_d, (void*)slot_dirty_d, (void*)nl_all_d, (void*)so_all_d,
+                    (void*)xslots_d, (void*)xchains_d})
       if (p) (void)hipFree(p);
     for (void* p : {(void*)gprev_d, (void*)fprev_d, (void*)tnorm_d})
       if (p) (void)hipFree(p);
@@ -3841,6 +3926,20 @@ struct ChainRun {
     cd.real_chain = real_chain ? 1 : 0; cd.tiny = tiny;
     cd.cvec = B.c; cd.closed = B.closed;
     cd.s0 = win ? s0_d : nullptr;
+    if (win && op->queue_dev) {
+      const int nch = (int)nl_all.size();
+      if (dalloc(&wq_d, 8) || dalloc(&slot_chain_d, M.nbp) || dalloc(&slot_dirty_d, M.nbp) || dalloc(&nl_all_d, nch) ||
+          dalloc(&so_all_d, nch) || dalloc(&xslots_d, M.nbp) || dalloc(&xchains_d, M.nbp))
+        return 1;
+      const int wq0[8] = {qnext, 0, 0, 0, M.nb, 1, 0, 0};
+      HCHECK(hipMemcpyAsync(wq_d, wq0, sizeof(wq0), hipMemcpyHostToDevice, stream));
+      HCHECK(hipMemcpyAsync(slot_chain_d, slot_chain.data(), M.nbp * sizeof(int), hipMemcpyHostToDevice, stream));
+      HCHECK(hipMemsetAsync(slot_dirty_d, 0, M.nbp * sizeof(int), stream));
+      HCHECK(hipMemcpyAsync(nl_all_d, nl_all.data(), nch * sizeof(int), hipMemcpyHostToDevice, stream));
+      HCHECK(hipMemcpyAsync(so_all_d, so_all.data(), nch * sizeof(int), hipMemcpyHostToDevice, stream));
+      HCHECK(hipStreamSynchronize(stream));  // the host sources above
+      dev_queue = true;
+    }
     sgrid = dim3((unsigned)((nv + 255) / 256));
     if (ed3 && block_start(ocs, sides)) return 1;
     if constexpr (std::is_same<T, cplx>::value) {
@@ -3956,8 +4055,9 @@ struct ChainRun {
     } else {
       const int k1 = level_host + 1;
       if (!win && (k1 < 8 || (k1 % 4) != 0)) return 0;
-      if (win) {
-        // window runs: the kernel takes each member's own level count; no launch when no member is at a test level
+      if (win && !dev_queue) {
+        // window runs: the kernel takes each member's own level count; no launch when no member is at a test level (with
+        // the queue on the device the host does not follow the slots, and the kernel runs every level)
         bool any = false;
         for (int q = 0; q < M.nb && !any; ++q) {
           const int c = slot_chain[q];
@@ -4481,6 +4581,70 @@ struct ChainRun {
     return 0;
   }
 
+  // the queue on the device after a level: k_refill, then the same steps as refill with the counts read on the device
+  int refill_dev() {
+    if (join_test()) return 1;  // this level's depth test writes conv, which k_member_start clears for a new member
+    const int nb = M.nb, mt = M.mt, nch = (int)nl_all.size();
+    hipLaunchKernelGGL(k_refill, dim3(1), dim3(64), 0, stream, cd, nb, mt, nch, wq_d, slot_chain_d, slot_dirty_d,
+                       (const int*)nl_all_d, (const int*)so_all_d, xslots_d, xchains_d, wslots, wchains, wvals, wl0, tact,
+                       ntile);
+    hipLaunchKernelGGL(k_extract<T>, dim3(nb), dim3(64), 0, stream, a_all, b_all, mused_all, (const T*)B.a, (const T*)B.b,
+                       (const int*)B.mused, ld, (const int*)xslots_d, (const int*)xchains_d, 0, (const int*)wq_d + 1);
+    const size_t nbp = M.nbp;
+    auto fill = [&](auto* dst, const auto* src, size_t nrows) -> int {
+      using E = std::remove_const_t<std::remove_pointer_t<decltype(src)>>;
+      hipLaunchKernelGGL(k_fill_cols<E>, dim3((unsigned)std::min<size_t>((nrows * nb + 255) / 256, 4096)), dim3(256), 0,
+                         stream, (E*)dst, (const E*)src, nrows, M.nbp, Mall.nbp, (const int*)wslots, (const int*)wchains, 0,
+                         (const int*)wq_d + 3);
+      HCHECK(hipGetLastError());
+      return 0;
+    };
+    if (fill(MB.chim, MBall.chim, 1) || fill(MB.wts, MBall.wts, M.wts.size() / nbp) ||
+        fill(MB.fpar, MBall.fpar, M.fpar.size() / nbp) || fill(MB.dvec, MBall.dvec, M.dvec.size() / nbp) ||
+        fill(MB.dperm, MBall.dperm, M.dperm.size() / nbp) || fill(MB.mtm, MBall.mtm, M.mtm.size() / nbp) ||
+        fill(MB.fsp, MBall.fsp, M.fsp.size() / nbp) || fill(MB.z, MBall.z, 1))
+      return 1;
+    const size_t rows = nv / nbp;
+    hipLaunchKernelGGL(k_start_cols<T>, dim3((unsigned)std::min<size_t>((rows * nb + 255) / 256, 16384)), dim3(256), 0, stream,
+                       B.u, B.up, rows, M.nbp, (const int*)wslots, (const int*)wl0, 0, (const int*)wq_d + 2);
+    hipLaunchKernelGGL(k_start_unit<T>, dim3((unsigned)((nb + 63) / 64)), dim3(64), 0, stream, B.u, M.nbp,
+                       (const int*)wslots, (const int*)wl0, 0, (const int*)wq_d + 2);
+    cplx *gp = nullptr, *fp = nullptr;
+    if constexpr (std::is_same<T, cplx>::value) {
+      gp = gprev_d;
+      fp = fprev_d;
+    }
+    hipLaunchKernelGGL(k_member_start<T>, dim3(nb), dim3(64), 0, stream, cd, (const int*)wslots, (const int*)wvals, 0, 0, B.a,
+                       B.b, B.a_eff, B.b_eff, B.inv_b, sc3[0], sc3[1], sc3[2], gp, fp, s0_d, B.closed, conv_d,
+                       (const int*)wq_d + 3);
+    HCHECK(hipGetLastError());
+    return 0;
+  }
+
+  // every few levels: the queue's counters; once the queue is empty the run goes on with the host path (the slots handed
+  // back), and *alive = 0 when nothing remains to run
+  int dev_check(int* alive) {
+    int wq[8];
+    HCHECK(hipMemcpyAsync(wq, wq_d, sizeof(wq), hipMemcpyDeviceToHost, stream));
+    HCHECK(hipStreamSynchronize(stream));
+    *alive = wq[5];
+    if (wq[0] >= (int)nl_all.size()) return leave_dev_queue();
+    return 0;
+  }
+
+  int leave_dev_queue() {
+    if (!dev_queue) return 0;
+    int wq[8];
+    HCHECK(hipMemcpyAsync(wq, wq_d, sizeof(wq), hipMemcpyDeviceToHost, stream));
+    HCHECK(hipMemcpyAsync(slot_chain.data(), slot_chain_d, M.nbp * sizeof(int), hipMemcpyDeviceToHost, stream));
+    HCHECK(hipMemcpyAsync(slot_dirty.data(), slot_dirty_d, M.nbp * sizeof(int), hipMemcpyDeviceToHost, stream));
+    HCHECK(hipMemcpyAsync(slot_s0.data(), s0_d, M.nbp * sizeof(int), hipMemcpyDeviceToHost, stream));
+    HCHECK(hipStreamSynchronize(stream));
+    qnext = wq[0];
+    dev_queue = false;
+    return 0;
+  }
+
   int level_cap() const { return win ? (nchain / std::max(1, M.nb) + 2) * (maxdepth + 4) : maxdepth; }
 
   int any_active(int* actv) {
@@ -4505,6 +4669,7 @@ struct ChainRun {
       }
     }
     if (win) {
+      if (leave_dev_queue()) return 1;
       // the members still holding slots hand over their results, then every chain's row in the caller's order
       std::vector<int> xs, xc;
       for (int q = 0; q < M.nb; ++q)
@@ -4607,7 +4772,14 @@ int run_chains_once(asr_gpu_op* op, int nchain, const int* start_orb, const doub
   const int cap = R.level_cap();
   for (int k = 0; k < cap; ++k) {
     if (R.step()) return 1;
-    if (mode == 0 && (op->compact || R.win)) {
+    if (mode == 0 && R.dev_queue) {
+      if (R.refill_dev()) return 1;
+      if ((k & 7) == 7) {
+        int alive = 1;
+        if (R.dev_check(&alive)) return 1;
+        if (!alive) break;
+      }
+    } else if (mode == 0 && (op->compact || R.win)) {
       int alive = 1;
       if (R.compact(&alive)) return 1;
       if (!alive) break;
@@ -5441,6 +5613,12 @@ int asr_gpu_op_set_window(asr_gpu_op* op, int nslot) {
   if (!op) return fail("asr_gpu_op_set_window: null handle");
   if (nslot < 0) return fail("asr_gpu_op_set_window: the number of slots must be >= 0");
   op->window = nslot;
+  return 0;
+}
+
+int asr_gpu_op_set_window_queue(asr_gpu_op* op, int on_device) {
+  if (!op) return fail("asr_gpu_op_set_window_queue: null handle");
+  op->queue_dev = on_device != 0;
   return 0;
 }
 

@@ -44,6 +44,7 @@ def lib() -> C.CDLL:
         L.asr_gpu_op_set_compaction.argtypes = [P, C.c_int]
         L.asr_gpu_op_get_compactions.argtypes = [P, C.POINTER(C.c_int)]
         L.asr_gpu_op_set_window.argtypes = [P, C.c_int]
+        L.asr_gpu_op_set_window_queue.argtypes = [P, C.c_int]
         L.asr_gpu_op_set_targets.argtypes = [P, C.c_int, P, P]
         L.asr_gpu_chains_edc_proj.argtypes = [P, C.c_int, P, P, P, P, P, P, P, C.c_int, C.c_int, P, P, P, P, P, P, P, C.c_int, P]
         L.asr_gpu_op_get_tile.argtypes = [P, C.c_int, C.c_int, P]
@@ -202,6 +203,11 @@ class GpuOperator:
         """Window runs of the batched edc chains with the adaptive depth on the device: nslot member slots, the other
         chains queued for the slots of members that leave (0: off)."""
         _check(lib().asr_gpu_op_set_window(self.h, int(nslot)), "asr_gpu_op_set_window")
+
+    def set_window_queue(self, on_device: bool):
+        """Where a window run's queue lives: on the device (the default; a kernel after each level), or followed by the host
+        after every level."""
+        _check(lib().asr_gpu_op_set_window_queue(self.h, int(bool(on_device))), "asr_gpu_op_set_window_queue")
 
     def compactions(self) -> int:
         """Member compactions of the last batched run."""

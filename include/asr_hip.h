@@ -99,6 +99,12 @@ int asr_gpu_op_set_compaction(asr_gpu_op* op, int on);
  * coefficients equal those of the batched run to rounding (the lagged normalisation rescales at the run's levels, not
  * the chain's). */
 int asr_gpu_op_set_window(asr_gpu_op* op, int nslot);
+/* Where a window run's queue lives (on_device != 0, the default: on the device): a kernel after every level hands the
+ * slots of members that have left to the next chains (their results to the chains' rows, their tables and start vectors
+ * to the slots) and the host only reads the queue's counters every 8 levels; once the queue is empty the run continues on
+ * the host path (slot hand-over, then member compaction).  0: the host follows every slot after each level.  The
+ * coefficients are the same either way; no counterpart in the reference (chain_worker's schedule runs on the host). */
+int asr_gpu_op_set_window_queue(asr_gpu_op* op, int on_device);
 /* The number of member compactions of the operator's last batched run (diagnostics). */
 int asr_gpu_op_get_compactions(asr_gpu_op* op, int* n);
 /* The tile the autotuner picked for nchain members of the given kind (0: none tuned yet). */

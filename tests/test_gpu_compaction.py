@@ -183,3 +183,27 @@ def test_window_contour_pass_equals_batches():
     win = SC.chain_greens(cp.zq, classes, comps, params, 24, depth_tol=1e-5, gpu_ops=[G], max_batch=40)
     assert win.nlev == ref.nlev and win.nchain == ref.nchain
     assert rel(win.gz, ref.gz) < 1e-11
+
+
+@pytest.mark.parametrize("nslot", [8, 24])
+def test_window_queue_on_device_equals_host(edc, nslot):
+    """The window run's queue on the device (a kernel hands the slots of members that leave to the next chains after every
+    level) against the queue followed by the host: the same depths and coefficients bit for bit (a member's arithmetic
+    does not depend on who hands it its slot)."""
+    eng = _engine()
+    wl, members = edc
+    G = eng.GpuOperator(wl.op)
+    G.set_tile(8)
+    G.set_depth_tol(1.0e-5, "hull")
+    G.set_window(nslot)
+    res = {}
+    for dev in (True, False):
+        G.set_window_queue(dev)
+        res[dev] = G.chains_edc(members, 40, mode=0)
+    G.set_window_queue(True)
+    G.set_window(0)
+    G.set_depth_tol(0.0)
+    a1, b1, mu1, _ = res[True]
+    a0, b0, mu0, _ = res[False]
+    assert np.array_equal(mu1, mu0)
+    assert np.array_equal(a1, a0) and np.array_equal(b1, b0)
