@@ -2656,10 +2656,21 @@ __global__ void __launch_bounds__(BLOCK) k_kpm_v2(RealDev R, const double* __res
         acc[l] = a;
       }
     }
+    // t_n-1 for the nine orbitals before the first store (a load behind a store would wait for it as well)
+    d2 t0a[9];
+#pragma unroll
+    for (int l = 0; l < 9; ++l) {
+      const nt_d2 t0n = __builtin_nontemporal_load((const nt_d2*)(u0 + ((uint64_t)site * 9 + l) * nvp + v));
+      t0a[l] = d2{t0n.x, t0n.y};
+    }
+    d2 xoa[DBL ? 9 : 1];
+    if constexpr (DBL) {
+#pragma unroll
+      for (int l = 0; l < 9; ++l) xoa[l] = ((const d2*)(u1 + ((uint64_t)site * 9 + l) * nvp + v))[0];
+    }
     for (int l = 0; l < 9; ++l) {
       const uint64_t row = (uint64_t)site * 9 + l;
-      const nt_d2 t0n = __builtin_nontemporal_load((const nt_d2*)(u0 + row * nvp + v));
-      const d2 t0v = {t0n.x, t0n.y};
+      const d2 t0v = t0a[l];
       double t2x = alpha * (dp[l] * acc[l].x) - beta * (t0v.x * id[l]);
       double t2y = alpha * (dp[l] * acc[l].y) - beta * (t0v.y * id[l]);
       if (v + 1 < nvec) {
@@ -2669,7 +2680,7 @@ __global__ void __launch_bounds__(BLOCK) k_kpm_v2(RealDev R, const double* __res
       }
       if constexpr (DBL) {
         // the own row again (an L1/L2 hit) rather than held through the neighbour loop
-        const d2 xo = ((const d2*)(u1 + row * nvp + v))[0];
+        const d2 xo = xoa[l];
         dot.x = fma(t2x, t2x, dot.x);
         dot.y = fma(t2y, t2y, dot.y);
         dot2.x = fma(t2x, xo.x * id[l], dot2.x);
