@@ -981,8 +981,9 @@ __global__ void k_rand_nodes(OpDev op, MemDev md, const int* __restrict__ list, 
 // The listed nodes (those with flip partners) of the fused edc level, one thread per (listed node, member): t1 = X_o h1
 // (when t1 is non-null, as k_rand_nodes) and the node's term of a_k, sq^2 [x~ (E + X(z)) x~ + x~ h1 - h1 t1] with the
 // partner rows (lpart: nlist x nbp, summed by k_fin_a after the block partials of the tile pass).
-__global__ void k_list_terms(OpDev op, MemDev md, const int* __restrict__ list, int nlist, const cplx* __restrict__ h1,
-                             cplx* __restrict__ t1, const cplx* __restrict__ x, int a_e, cplx* __restrict__ lpart) {
+__global__ void __launch_bounds__(256) k_list_terms(OpDev op, MemDev md, const int* __restrict__ list, int nlist,
+                                                     const cplx* __restrict__ h1, cplx* __restrict__ t1,
+                                                     const cplx* __restrict__ x, int a_e, cplx* __restrict__ lpart) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int nbp = md.nbp;
   const int j = (int)(t / nbp), m = (int)(t % nbp);
@@ -992,10 +993,8 @@ __global__ void k_list_terms(OpDev op, MemDev md, const int* __restrict__ list, 
   const uint32_t inf = op.info[i];
   cplx to[9], ex[9];
   for (int l = 0; l < 9; ++l) { to[l] = zero<cplx>(); ex[l] = zero<cplx>(); }
-  if (t1) {
-    rand_node_term<cplx>(op, md, i, inf, m, h1, QO, 1.0, 0, false, to);
-    for (int l = 0; l < 9; ++l) t1[((size_t)i * 9 + l) * nbp + m] = to[l];
-  }
+  // every load before the stores at the end (a load behind a store waits for it as well)
+  if (t1) rand_node_term<cplx>(op, md, i, inf, m, h1, QO, 1.0, 0, false, to);
   if (a_e) rand_node_term<cplx>(op, md, i, inf, m, x, QE, 1.0, 1, false, ex);
   rand_node_term<cplx>(op, md, i, inf, m, x, QPA, 1.0, 1, true, ex);
   const double isi = op.isq[i], sq2 = op.sq[i] * op.sq[i];
@@ -1005,6 +1004,8 @@ __global__ void k_list_terms(OpDev op, MemDev md, const int* __restrict__ list, 
     const cplx xt = isi * x[q], h = h1[q];
     pa = pa + ((bilin(xt, ex[l]) + bilin(xt, h)) - bilin(h, to[l]));
   }
+  if (t1)
+    for (int l = 0; l < 9; ++l) t1[((size_t)i * 9 + l) * nbp + m] = to[l];
   lpart[(size_t)j * nbp + m] = sq2 * pa;
 }
 
@@ -2270,14 +2271,17 @@ __global__ void __launch_bounds__(BLOCK) k_update(OpDev op, MemDev md, T* w, con
         for (int l = 0; l < 9; ++l) x[l] = acc[l] / (double)ns;
       }
     }
+    // the dots (route ed3's tv rows read) before the first store: a load behind a store waits for it as well
+    for (int l = 0; l < 9; ++l) {
+      dv = dv + bilin(x[l], x[l]);
+      if constexpr (std::is_same<T, cplx>::value) {
+        if (tv) dv2 = dv2 + bilin(tv[((size_t)i * 9 + l) * md.nbp + m], x[l]);
+      }
+    }
     for (int l = 0; l < 9; ++l) {
       const size_t q = ((size_t)i * 9 + l) * md.nbp + m;
       if constexpr (NT) nt_store(&w[q], x[l]);  // the next level gathers it: no use in L2 now
       else w[q] = x[l];
-      dv = dv + bilin(x[l], x[l]);
-      if constexpr (std::is_same<T, cplx>::value) {
-        if (tv) dv2 = dv2 + bilin(tv[q], x[l]);
-      }
     }
   }
   block_reduce_member<T>(dv, mt, red, partial + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * mt + threadIdx.x);
