@@ -10,7 +10,8 @@ module asr_gpu
   private
   public :: gpu_check, gpu_last_error, gpu_op_from_tables, gpu_device_count, c_asr_gpu_op_create
   public :: c_asr_gpu_op_destroy, c_asr_gpu_op_trim, c_asr_gpu_op_set_tile, c_asr_gpu_op_set_depth_tol, &
-            c_asr_gpu_op_set_level_form, c_asr_gpu_op_set_window, c_asr_gpu_op_set_compaction, &
+            c_asr_gpu_op_set_level_form, c_asr_gpu_op_set_window, c_asr_gpu_op_set_window_queue, &
+            c_asr_gpu_op_set_compaction, &
             c_asr_gpu_chains_edc, c_asr_gpu_lanczos_real, c_asr_gpu_ed3_blocks, &
             c_asr_gpu_real_create, c_asr_gpu_real_destroy, c_asr_gpu_kpm_moments
 
@@ -83,6 +84,12 @@ module asr_gpu
       import :: c_ptr, c_int
       type(c_ptr), value :: op
       integer(c_int), value :: nslot
+      integer(c_int) :: ierr
+    end function
+    function c_asr_gpu_op_set_window_queue(op, on_device) bind(C, name='asr_gpu_op_set_window_queue') result(ierr)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: op
+      integer(c_int), value :: on_device
       integer(c_int) :: ierr
     end function
     function c_asr_gpu_op_set_compaction(op, on) bind(C, name='asr_gpu_op_set_compaction') result(ierr)
