@@ -90,3 +90,11 @@ def contour_objective(a: torch.Tensor, b: torch.Tensor, z: torch.Tensor, w: torc
     n = -(1.0 / np.pi) * torch.imag((wc * g).sum(dim=1))
     e = -(1.0 / np.pi) * torch.imag((wc * z.to(torch.complex128)[None, :] * g).sum(dim=1))
     return (weight * e).sum() + 0.5 * lam * (weight * (n - q0) ** 2).sum()
+
+
+def chain_dos(a: torch.Tensor, b: torch.Tensor, e, eta: float) -> torch.Tensor:
+    """Density of states of each chain at the real energies e with a broadening eta: -(1/pi) Im G_00(e + i eta) of the
+    terminated continued fraction (cf_green), nchain x ne; differentiable in a and b, so that d(DOS)/d(theta) comes from
+    the engine's adjoint through chain_coefficients."""
+    z = torch.as_tensor(np.asarray(e, dtype=np.float64)).to(torch.complex128) + 1j * float(eta)
+    return -(1.0 / np.pi) * torch.imag(cf_green(a, b, z))

@@ -121,6 +121,38 @@ def test_torch_harness_contour_objective(wl):
     assert np.max(np.abs(grad - fd)) < 1e-6 * scale, np.max(np.abs(grad - fd)) / scale
 
 
+def test_torch_harness_dos_gradient(wl):
+    """d(DOS)/d(theta): a weighted sum of the chains' densities of states at seven energies (broadening 0.05) through
+    torch.autograd and the engine's adjoint against central finite differences of the same sum with the engine's
+    forward."""
+    eng = _engine()
+    nlev = int(wl.nlev[0])
+    nch = len(wl.members)
+    G = eng.GpuOperator(wl.op)
+    e = np.linspace(-1.2, 0.4, 7)
+    rng = np.random.default_rng(11)
+    v = torch.from_numpy(rng.uniform(0.5, 1.5, (nch, e.size)))
+    off = wl.info["c_offset"]
+
+    def L(th):
+        a, b = AD.chain_coefficients(th, G, wl.members, nlev, off)
+        return (v * AD.chain_dos(a, b, e, 0.05)).sum()
+
+    theta = torch.tensor(wl.info["theta"], requires_grad=True)
+    L(theta).backward()
+    grad = theta.grad.numpy()
+    h = 1e-5
+    fd = np.zeros_like(grad)
+    with torch.no_grad():
+        for idx in np.ndindex(*grad.shape):
+            tp = wl.info["theta"].copy(); tp[idx] += h
+            tm = wl.info["theta"].copy(); tm[idx] -= h
+            fd[idx] = (float(L(torch.from_numpy(tp))) - float(L(torch.from_numpy(tm)))) / (2 * h)
+    scale = np.max(np.abs(fd))
+    assert scale > 1e-4
+    assert np.max(np.abs(grad - fd)) < 1e-6 * scale, np.max(np.abs(grad - fd)) / scale
+
+
 def test_real_chains_beyond_2gb_vectors():
     """The adjoint workload at full size: 256 real members on 127,517 nodes, vectors of 2.35 GB (beyond 2^31 bytes).  The
     member-tile kernel (tile 16; unsigned 32-bit buffer offsets) gives the chains of the one-node-per-wave kernel (tile
