@@ -146,17 +146,10 @@ def measured_traffic(workload: str, kernel: str, nodes: int, members: int):
     return None
 
 
-def cpu_cores() -> int:
-    try:
-        return len(os.sched_getaffinity(0))
-    except Exception:
-        return os.cpu_count() or 1
-
-
-def cpu_baseline(wl, nlev_sample: int, cores: int, nchain: int | None = None):
+def cpu_baseline(wl, nlev_sample: int, nchain: int | None = None):
     """The oracle's CPU chains (one chain per thread, dynamic schedule, as chain_worker) on the host cores: nchain chains
-    (default: all of the workload) to nlev_sample levels each."""
-    from oracle.oracle import OracleOp
+    (default: all of the workload) to nlev_sample levels each; cores = the oracle's OpenMP threads."""
+    from oracle.oracle import OracleOp, threads
     os.environ.setdefault("OMP_PROC_BIND", "spread")
     nch = len(wl.members) if nchain is None else min(len(wl.members), nchain)
     members = wl.members[:nch]
@@ -165,8 +158,23 @@ def cpu_baseline(wl, nlev_sample: int, cores: int, nchain: int | None = None):
     t0 = time.perf_counter()
     a, b, mu = O.edc_batch(members, eps, [m.l0 for m in members], [nlev_sample] * nch, project=True)
     dt = time.perf_counter() - t0
-    return dict(value=round(float(np.sum(mu)) / dt, 3), unit="chain-levels/s", cores=cores, kind="port",
+    return dict(value=round(float(np.sum(mu)) / dt, 3), unit="chain-levels/s", cores=threads(), kind="port",
                 sample=f"{nch} chains x {nlev_sample} levels of the same workload ({wl.op.nn} nodes), {dt:.1f} s"), (a, b, mu)
+
+
+def kpm_cpu_baseline(g, species, par, block, block0, cshift, sscale, nvec: int, nmom: int, per_gpu):
+    """The oracle's KPM moments (one vector after another, each step a parallel row loop over the sites as csr_apply) on
+    the host cores: the first nvec Rademacher vectors of the same seed x nmom moments of the same supercell.  per_gpu
+    (nvec x nmom) holds the GPU's per-vector terms <r|t_n>/dim of the same vectors: the largest difference is rounding."""
+    from oracle.oracle import OracleReal, threads
+    os.environ.setdefault("OMP_PROC_BIND", "spread")
+    O = OracleReal(g.nbr, species, par[0], par[1], par[2], par[3], block, block0)
+    t0 = time.perf_counter()
+    _, per = O.kpm_moments(cshift, sscale, nmom, nvec, 5, per_vector=True)
+    dt = time.perf_counter() - t0
+    return dict(value=round(nvec * nmom / dt, 3), unit="vector-moments/s", cores=threads(), kind="port",
+                sample=f"{nvec} vectors x {nmom} moments of the same supercell ({g.n * 9} rows), {dt:.1f} s",
+                max_abs_diff_per_vector_vs_gpu=float(np.max(np.abs(per - per_gpu))))
 
 
 def main():
@@ -514,6 +522,11 @@ def main():
                                   "steps": nmom // 2 - 1, "step_ms": round(rd["step_ms"], 4), "roofline": droof,
                                   "max_abs_diff_vs_direct": float(np.max(np.abs(rd["mu"] - res["mu"])))}
         log(f"kpm doubling: {out['kpm']['doubling']}")
+        if not args.no_cpu:
+            nvc, nmc = (2, 16) if args.quick else (4, 64)
+            per = R.kpm_moments(cshift, sscale, nmc, 64, seed=5, per_vector=True)["mu_per"][:nvc]
+            out["kpm"]["cpu_baseline"] = kpm_cpu_baseline(g, species, par, block, block0, cshift, sscale, nvc, nmc, per)
+            log(f"kpm cpu baseline: {out['kpm']['cpu_baseline']}")
         R.close()
 
     if not args.no_adjoint:
@@ -559,8 +572,7 @@ def main():
         GA.close()
 
     if not args.no_cpu:
-        cores = cpu_cores()
-        cb, _ = cpu_baseline(wl, 2 if args.quick else depth, cores)
+        cb, _ = cpu_baseline(wl, 2 if args.quick else depth)
         out["cpu_baseline"] = cb
         log(f"cpu baseline: {cb}")
     G.close()

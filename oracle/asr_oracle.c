@@ -24,6 +24,7 @@
  * (src/asr_class_scf.f90:1093) does.
  */
 #include <complex.h>
+#include <omp.h>
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -582,11 +583,20 @@ static void real_h_apply(const orc_real *R, const double *x, double *hx, double 
   }
 }
 
-/* y = e x + h x - h o h x (real_apply, order 2); wk: 3 work vectors */
+/* y = e x + h x - h o h x (real_apply, order 2); wk: 3 work vectors.  With o = 0 for every species the second h apply
+ * acts on a zero vector and gives exactly zero, so it is skipped (y = e x + h x, the same rounding). */
 void orc_real_apply(const orc_real *R, const double *x, double *y, double *wk) {
   const size_t n = (size_t)9 * R->nsite;
   double *hx = wk, *w = wk + n, *hw = wk + 2 * n, *t = y;
   real_h_apply(R, x, hx, t);
+  int zero_o = 1;
+  for (int q = 0; q < R->nsp * 9; ++q) zero_o = zero_o && R->o[q] == 0.0;
+  if (zero_o) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < R->nsite; ++i)
+      for (int l = 0; l < 9; ++l) y[9 * i + l] = R->e[R->species[i] * 9 + l] * x[9 * i + l] + hx[9 * i + l];
+    return;
+  }
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < R->nsite; ++i)
     for (int l = 0; l < 9; ++l) w[9 * i + l] = R->o[R->species[i] * 9 + l] * hx[9 * i + l];
@@ -597,6 +607,17 @@ void orc_real_apply(const orc_real *R, const double *x, double *y, double *wk) {
   for (int i = 0; i < R->nsite; ++i)
     for (int l = 0; l < 9; ++l)
       y[9 * i + l] = R->e[R->species[i] * 9 + l] * x[9 * i + l] + hx[9 * i + l] - hw[9 * i + l];
+}
+
+/* The number of OpenMP threads the parallel loops above use (the timed CPU baseline reports it). */
+int orc_threads(void) {
+  int n = 1;
+#pragma omp parallel
+  {
+#pragma omp single
+    n = omp_get_num_threads();
+  }
+  return n;
 }
 
 /* Counter-based Rademacher entry r(seed, vector, row) = +-1 (splitmix64 of a packed counter; the GPU uses the same). */

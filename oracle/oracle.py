@@ -57,6 +57,8 @@ def lib():
         L.orc_real_lanczos.argtypes = [P, C.c_int, C.c_uint64, P, P]
         L.orc_rademacher.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
         L.orc_rademacher.restype = C.c_double
+        L.orc_threads.argtypes = []
+        L.orc_threads.restype = C.c_int
         _LIB = L
     return _LIB
 
@@ -311,6 +313,11 @@ class OracleReal:
         b = np.zeros(nstep)
         lib().orc_real_lanczos(C.byref(self.s), int(nstep), int(seed), _p(a), _p(b))
         return a, b
+
+
+def threads() -> int:
+    """The OpenMP threads of the oracle's parallel loops."""
+    return int(lib().orc_threads())
 
 
 def rademacher(seed: int, v: int, row: int) -> float:

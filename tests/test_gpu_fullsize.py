@@ -5,7 +5,8 @@
 * The L2_1 uneven batch (four sublattice operators with SRO classes, 256 chains of depth 20-60, one HIP graph): the
   deepest and the shallowest chain of every operator against the oracle at full depth, 1e-9.
 * KPM on the 40^3-cell supercell (2,304,000 rows, 64 vectors, 2048 moments): size-independent properties, mu_0 = 1,
-  |mu_n| <= 1, moment doubling = direct moments to 1e-12, the Jackson DOS integrates to 1.
+  |mu_n| <= 1, moment doubling = direct moments to 1e-12, the Jackson DOS integrates to 1; and the per-vector terms
+  <r|t_n>/dim of the first two Rademacher vectors against the oracle's (16 moments), 1e-12.
 * The adjoint (256 real chains x 40 on 127,517 nodes): gradient components of a contour objective against central finite
   differences of the GPU forward (h = 1e-5), 1e-6 relative.
 """
@@ -70,7 +71,13 @@ def test_kpm_full_size_properties():
     mu = R.kpm_moments(c, s, 2048, 64, seed=5)["mu"]
     R.set_doubling(True)
     md = R.kpm_moments(c, s, 2048, 64, seed=5)["mu"]
+    R.set_doubling(False)
+    per = R.kpm_moments(c, s, 16, 64, seed=5, per_vector=True)["mu_per"][:2]
     R.close()
+    from oracle.oracle import OracleReal
+    _, per_o = OracleReal(g.nbr, species, par[0], par[1], par[2], par[3], block, block0).kpm_moments(
+        c, s, 16, 2, 5, per_vector=True)
+    assert np.max(np.abs(per - per_o)) < 1e-12, np.max(np.abs(per - per_o))
     assert abs(mu[0] - 1.0) < 1e-14
     assert np.max(np.abs(mu)) <= 1.0 + 1e-12
     assert np.max(np.abs(md - mu)) < 1e-12

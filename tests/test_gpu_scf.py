@@ -149,3 +149,66 @@ def test_window_run_on_the_ring_queue_on_device_equals_host():
     assert dev.nchain == ref.nchain and dev.nlev == ref.nlev
     assert np.array_equal(dev.gz, host.gz)
     assert np.max(np.abs(dev.gz - ref.gz)) < 1e-5 * np.max(np.abs(ref.gz))
+
+
+
+def test_window_run_with_closed_chains_on_the_ring():
+    """Chains that end early because b_k vanishes (the Krylov space is exhausted): on the ring with orbital 8 decoupled
+    from the bonds (its rows and columns of the bond and on-site blocks zero) and the start site frozen, a chain from
+    orbital 8 closes at its first level (h x = c x there); the others run their 40 levels.  A member that closes keeps a
+    non-zero vector, so its slot is marked dirty and the next chain starts there from a full start column.  40 chains
+    through 8 slots, the queue on the device and the host-followed queue: the depths equal the batched run's and the
+    oracle's, and the coefficients agree with both to rounding.  The two queues do not agree bit for bit here: the host
+    looks at the slots only at the levels where a member may leave by its depth or its test, so it hands a closed
+    member's slot on later, and a chain that runs at other levels of the run rounds differently (lagged
+    normalisation)."""
+    eng = _engine()
+    from exact import build_ring_operator, ring_model
+    from oracle.oracle import OracleOp
+    from test_oracle_exact import ring_ed3_params, rotated
+    model, _ = ring_model()
+    model = dict(model)
+    blk, b0 = model["block"].copy(), model["block0"].copy()
+    for B in (blk, b0):
+        B[..., 8, :] = 0.0
+        B[..., :, 8] = 0.0
+    model["block"], model["block0"] = blk, b0
+    op = build_ring_operator(rotated(model, 1), frozen_start=True)
+    pp0 = ring_ed3_params()
+    zs = [complex(-0.3 + 0.1 * k, 0.05 + 0.03 * k) for k in range(5)]
+    # species 1 and 2 start in the binary class; orbital 8 on every third member
+    members = [edc_member(pp0, 1 + (k // 3) % 2, 8 if k % 3 == 0 else k % 8, zs[k % 5]) for k in range(40)]
+    G = eng.GpuOperator(op)
+    G.set_tile(8)
+    # a tolerance no chain meets: every member runs to its depth or closes
+    G.set_depth_tol(1.0e-300, "none")
+    a0, b0_, mu0, _ = G.chains_edc(members, 40, mode=0)
+    G.set_window(8)
+    res = {}
+    for dev in (True, False):
+        G.set_window_queue(dev)
+        res[dev] = G.chains_edc(members, 40, mode=0)
+    G.set_window_queue(True)
+    G.set_window(0)
+    G.set_depth_tol(0.0)
+    G.close()
+    print(f"ring depths {sorted(set(mu0.tolist()))}", flush=True)
+    assert np.array_equal(mu0, np.where(np.arange(40) % 3 == 0, 1, 40))
+    O = OracleOp(op)
+    for a1, b1, mu1, _ in (res[True], res[False]):
+        assert np.array_equal(mu1, mu0)
+        for j in range(len(members)):
+            k = int(mu0[j])
+            sa = np.max(np.abs(a0[j, :k]))
+            # 40 levels of a complex-symmetric chain: rounding grows to about 1e-11 of the coefficients' size
+            assert np.max(np.abs(a1[j, :k] - a0[j, :k])) <= 1e-10 * sa, j
+            if k > 1:
+                sb = np.max(np.abs(b0_[j, :k - 1]))
+                assert np.max(np.abs(b1[j, :k - 1] - b0_[j, :k - 1])) <= 1e-10 * sb, j
+            if j % 5 == 0:
+                m = members[j]
+                ar, br, n = O.edc_chain(m, m.z * m.pa + m.pb, m.l0, 40)
+                assert n == k, (j, n, k)
+                assert np.max(np.abs(a1[j, :k] - ar[:k])) < 1e-9 * sa, j
+                if k > 1:
+                    assert np.max(np.abs(b1[j, :k - 1] - br[:k - 1])) < 1e-9 * sb, j
