@@ -1725,6 +1725,18 @@ __global__ void k_conv_apply(ChainDev cd, int nb, int* conv, T* b) {
   if (m < nb && conv[m]) apply_conv<T>(cd, conv, b, m);
 }
 
+// Window runs: the members the test of the level just ended found converged leave at once, after that level (the level
+// counter already past it), so that their slots take the next chains at the next level instead of running it for
+// nothing.  Their slots hold the next vector: flagged like a closed chain's, for a full start column.
+template <class T>
+__global__ void k_conv_leave(ChainDev cd, int nb, int* conv, T* b) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m < nb && conv[m]) {
+    apply_conv<T>(cd, conv, b, m);
+    cd.closed[m] = 1;
+  }
+}
+
 template <class T>
 __global__ void __launch_bounds__(BLOCK) k_fin_a(ChainDev cd, int nb, const T* part, int nbx, int mt, T* a, T* a_eff,
                                                   const T* sc, const T* lpart, int nlist, int nbp, int* conv, T* bk) {
@@ -1942,8 +1954,9 @@ __device__ int cf_species_dev(const cplx* a, const cplx* b, const cplx* c, int m
 // thread per member: a member still running below its depth gets G_k1 of its chain (and route ed3's projection f, with
 // the check of ed3's chain, src/asr_ed3.f90:177-184), and leaves when |G - G_prev| <= tol |G| (k1 > 8).  gprev / fprev keep
 // the previous tested level's values.  It runs on a stream of its own after the a_k reduction, beside the level's update;
-// a converged member is flagged (conv) and leaves at the next level's k_fin_a, which takes back what the level recorded
-// for it after a_k: the coefficients and depths are those of the test run before the update.
+// a converged member is flagged (conv) and leaves at the next level's k_fin_a (in a window run at the end of the level,
+// k_conv_leave), which takes back what the level recorded for it after a_k: the coefficients and depths are those of the
+// test run before the update.
 constexpr int DT_TPB = 16;  // threads (members) per block of the device depth test: 2 x 128 x 16 values of LDS
 __global__ void __launch_bounds__(DT_TPB) k_depth_test(ChainDev cd, int nb, const cplx* a, const cplx* b, const cplx* cvec,
                                                        const cplx* z, int k1, double tol, int tail, cplx* gprev, cplx* fprev,
@@ -2044,8 +2057,8 @@ __global__ void k_perm_rows(E* __restrict__ dst, const E* __restrict__ src, int 
 // Window runs: the chains wait in a queue and take the slots of members that have left.  k_extract copies the results of
 // leaving members (their relative levels 0 .. ld - 1) to the rows of their chains; k_fill_cols copies the member tables
 // of the starting chains from the all-chain tables (stride nbps) into their slots (stride nbpd); k_start_cols writes their
-// start vectors (u = the unit vector of the start orbital on the start node, rows 0..8; u_prev = 0); k_member_start
-// resets their recursion state.
+// start vectors (u = the unit vector of the start orbital on the start node, rows 0..8; u_prev keeps what the slot held,
+// finite values that the first level multiplies by b_-1 = 0); k_member_start resets their recursion state.
 template <class T>
 __global__ void k_extract(T* __restrict__ a_all, T* __restrict__ b_all, int* __restrict__ mused_all, const T* __restrict__ a,
                           const T* __restrict__ b, const int* __restrict__ mused, int ld, const int* __restrict__ slots,
@@ -2073,7 +2086,7 @@ __global__ void k_fill_cols(E* __restrict__ dst, const E* __restrict__ src, size
   }
 }
 template <class T>
-__global__ void k_start_cols(T* __restrict__ u, T* __restrict__ up, size_t nrows, int nbp, const int* __restrict__ slots,
+__global__ void k_start_cols(T* __restrict__ u, size_t nrows, int nbp, const int* __restrict__ slots,
                              const int* __restrict__ l0, int n, const int* __restrict__ nd = nullptr) {
   if (nd) n = *nd;
   const size_t tot = nrows * (size_t)n;
@@ -2083,7 +2096,6 @@ __global__ void k_start_cols(T* __restrict__ u, T* __restrict__ up, size_t nrows
     T v = zero<T>();
     if (r == (size_t)l0[j]) *(double*)&v = 1.0;
     u[r * (size_t)nbp + slots[j]] = v;
-    up[r * (size_t)nbp + slots[j]] = zero<T>();
   }
 }
 template <class T>
@@ -3685,7 +3697,8 @@ struct ChainRun {
   bool dev_depth = false;
   cplx *gprev_d = nullptr, *fprev_d = nullptr;
   double* tnorm_d = nullptr;
-  int* conv_d = nullptr;                 // members the last test found converged (they leave at the next k_fin_a)
+  int* conv_d = nullptr;                 // members the last test found converged (they leave at the next k_fin_a, or
+                                         // in a window run at the refill after the test's level)
   hipStream_t tstream = nullptr;         // the test's own stream (default priority)
   hipEvent_t ev_a = nullptr, ev_t = nullptr;
   bool test_pending = false;
@@ -4475,11 +4488,13 @@ struct ChainRun {
       const int c = slot_chain[q];
       if (c < 0) continue;
       const int rel = level_host - slot_s0[q];
-      // at its depth, or one level after its test levels 12, 16, ... (the test's decision takes effect then)
-      if (rel >= nl_all[c] || (rel >= 13 && rel % 4 == 1)) maybe = true;
+      // at its depth, or after its test levels 12, 16, ... (a member the test finds converged leaves at once)
+      if (rel >= nl_all[c] || (rel >= 12 && rel % 4 == 0)) maybe = true;
     }
     if (!maybe) return 0;
-    if (test_pending) HCHECK(hipEventSynchronize(ev_t));
+    if (join_test()) return 1;
+    hipLaunchKernelGGL(k_conv_leave<T>, dim3((nb + 63) / 64), dim3(64), 0, stream, cd, nb, conv_d, B.b);
+    HCHECK(hipGetLastError());
     std::vector<int> act(nb), cls(nb);
     HCHECK(hipMemcpyAsync(act.data(), B.active, nb * sizeof(int), hipMemcpyDeviceToHost, stream));
     HCHECK(hipMemcpyAsync(cls.data(), B.closed, nb * sizeof(int), hipMemcpyDeviceToHost, stream));
@@ -4553,7 +4568,7 @@ struct ChainRun {
       if (ndirty > 0) {
         const size_t tot = rows * ndirty;
         hipLaunchKernelGGL(k_start_cols<T>, dim3((unsigned)std::min<size_t>((tot + 255) / 256, 16384)), dim3(256), 0, stream,
-                           B.u, B.up, rows, M.nbp, (const int*)wslots, (const int*)wl0, ndirty);
+                           B.u, rows, M.nbp, (const int*)wslots, (const int*)wl0, ndirty);
       }
       if (n > ndirty)
         hipLaunchKernelGGL(k_start_unit<T>, dim3((unsigned)((n - ndirty + 63) / 64)), dim3(64), 0, stream, B.u, M.nbp,
@@ -4600,6 +4615,7 @@ struct ChainRun {
   int refill_dev() {
     if (join_test()) return 1;  // this level's depth test writes conv, which k_member_start clears for a new member
     const int nb = M.nb, mt = M.mt, nch = (int)nl_all.size();
+    hipLaunchKernelGGL(k_conv_leave<T>, dim3((nb + 63) / 64), dim3(64), 0, stream, cd, nb, conv_d, B.b);
     hipLaunchKernelGGL(k_refill, dim3(1), dim3(64), 0, stream, cd, nb, mt, nch, wq_d, slot_chain_d, slot_dirty_d,
                        (const int*)nl_all_d, (const int*)so_all_d, xslots_d, xchains_d, wslots, wchains, wvals, wl0, tact,
                        ntile);
@@ -4621,7 +4637,7 @@ struct ChainRun {
       return 1;
     const size_t rows = nv / nbp;
     hipLaunchKernelGGL(k_start_cols<T>, dim3((unsigned)std::min<size_t>((rows * nb + 255) / 256, 16384)), dim3(256), 0, stream,
-                       B.u, B.up, rows, M.nbp, (const int*)wslots, (const int*)wl0, 0, (const int*)wq_d + 2);
+                       B.u, rows, M.nbp, (const int*)wslots, (const int*)wl0, 0, (const int*)wq_d + 2);
     hipLaunchKernelGGL(k_start_unit<T>, dim3((unsigned)((nb + 63) / 64)), dim3(64), 0, stream, B.u, M.nbp,
                        (const int*)wslots, (const int*)wl0, 0, (const int*)wq_d + 2);
     cplx *gp = nullptr, *fp = nullptr;
