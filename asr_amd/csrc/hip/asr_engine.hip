@@ -323,6 +323,10 @@ struct FArgs {
   const void* upv;
   const void *aeff, *beff, *scv, *scpv;
   const int* act;
+  // the chain blocks (route ed3, EPI), O2 = 3 (the update pass of a block level): the partials of <tv, w> of the second
+  // vector tv with the updated and projected w, into partial3 (c_k+1 = <tv, w> / b_k)
+  const void* tvv;
+  void* partial3;
   // k_tile, real members, the pass that writes the output y (O2 = 0, no update): the stored output becomes
   //   y' = lcc_0 lcv_0 + ... + lcc_{lcn-1} lcv_{lcn-1} + y  (own rows, per-member coefficients; lcc_t null: 1), with the
   //   partial dot <dotu, y'>, in k_lincomb's operation order; and out3 = rib (y - ra u3 - rb up3) (a recomputed level in
@@ -524,7 +528,7 @@ __device__ __forceinline__ double nt_load(const double* p) { return __builtin_no
 // class_ops) orbital l2 of a gathered image is row pi(g, l2) of the neighbour, whose nine 4-bit row indices per operation
 // come packed from LDS, and the gather scale carries d of that row.
 template <class T, int MT, int NSLOT, int O2, bool PI>
-__global__ void __launch_bounds__(BLOCK, 3) k_tile(OpDev op, MemDev md, FArgs fa, const double* __restrict__ sblk) {
+__global__ void __launch_bounds__(BLOCK, (O2 == 3 ? 2 : 3)) k_tile(OpDev op, MemDev md, FArgs fa, const double* __restrict__ sblk) {
   extern __shared__ double bls[];
   __shared__ T red[BLOCK];
   __shared__ unsigned long long pik[PI ? 1 : 64];
@@ -569,7 +573,7 @@ __global__ void __launch_bounds__(BLOCK, 3) k_tile(OpDev op, MemDev md, FArgs fa
   const int i = (xcd_block(blockIdx.x, gridDim.x) * (BLOCK / WAVE) + wave) * npw + lane / MT;
   const bool live = (i < op.nn) && (m < md.nb);
   const int nbp = md.nbp;
-  T dotv = zero<T>(), dotv2 = zero<T>();
+  T dotv = zero<T>(), dotv2 = zero<T>(), dotv3 = zero<T>();
   if (live) {
     const uint32_t inf = op.info[i];
     T acc[9];
@@ -687,7 +691,7 @@ __global__ void __launch_bounds__(BLOCK, 3) k_tile(OpDev op, MemDev md, FArgs fa
     };
     if constexpr (std::is_same<T, cplx>::value && O2 < 2) {
       // a_k at a node without flip partners: sq^2 [x~ (E + X(z)) x~ + x~ h1 - h1 t1] with h1 = y (out_sq = 0) and
-      // t1 = c2 X_q2 h1 (O2 = 1) or zero; (E + X(z)) and X_q2 are node-local there
+      // t1 = c2 X_q2 h1 (O2 = 1) or zero; (E + X(z)) and X_q2 are node-local there (route ed3: in the out2 loop below)
       if (fa.apart && !(inf & INFO_PARTNERS)) {
         const cplx z = md.z[m];
         const double* we = fa.a_e ? wrow(QE) : nullptr;
@@ -707,7 +711,7 @@ __global__ void __launch_bounds__(BLOCK, 3) k_tile(OpDev op, MemDev md, FArgs fa
         dotv = sq2 * pa;
       }
     }
-    if constexpr (O2 == 0) {
+    if constexpr (O2 == 0 || O2 == 3) {
       bool done = false;
       if constexpr (std::is_same<T, cplx>::value) {
         if (fa.upd) {
@@ -732,7 +736,14 @@ __global__ void __launch_bounds__(BLOCK, 3) k_tile(OpDev op, MemDev md, FArgs fa
           }
           const cplx* uo = (const cplx*)fa.uo;
           const cplx* up = (const cplx*)fa.upv;
-          cplx ww = zero<cplx>(), uy = zero<cplx>();
+          cplx ww = zero<cplx>(), uy = zero<cplx>(), tw = zero<cplx>();
+          // O2 = 3: the second vector's rows (with orbital-preserving operations requested with the other operands)
+          const cplx* tvr = (const cplx*)fa.tvv;
+          cplx tq[O2 == 3 && PI ? 9 : 1];
+          if constexpr (O2 == 3 && PI) {
+#pragma unroll
+            for (int l = 0; l < 9; ++l) tq[l] = tvr[((size_t)i * 9 + l) * nbp + m];
+          }
           if constexpr (PI) {
             // the nine u and u_prev values are requested before the first store: the compiler cannot tell the output
             // from them, and a load behind a store would wait for the store as well (one vector-memory counter).
@@ -754,6 +765,7 @@ __global__ void __launch_bounds__(BLOCK, 3) k_tile(OpDev op, MemDev md, FArgs fa
               if (ns > 1) w = (f[l] / (double)ns) * w;
               out_store<NTO>(&out[q], w);
               ww = ww + bilin(w, w);
+              if constexpr (O2 == 3) tw = tw + bilin(tq[l], w);
             }
           } else {
             // (P w)_l = sum_{g in S} chi(g) sg(g, l) w_pi(g, l) / |S|: the nine w of the node first (in acc), then each
@@ -783,17 +795,20 @@ __global__ void __launch_bounds__(BLOCK, 3) k_tile(OpDev op, MemDev md, FArgs fa
                 const cplx w = (1.0 / (double)ns) * sacc;
                 out_store<NTO>(&out[((size_t)i * 9 + l) * nbp + m], w);
                 ww = ww + bilin(w, w);
+                if constexpr (O2 == 3) tw = tw + bilin(tvr[((size_t)i * 9 + l) * nbp + m], w);
               }
             } else {
 #pragma unroll
               for (int l = 0; l < 9; ++l) {
                 out_store<NTO>(&out[((size_t)i * 9 + l) * nbp + m], acc[l]);
                 ww = ww + bilin(acc[l], acc[l]);
+                if constexpr (O2 == 3) tw = tw + bilin(tvr[((size_t)i * 9 + l) * nbp + m], acc[l]);
               }
             }
           }
           dotv = ww;
           dotv2 = uy;
+          dotv3 = tw;
           done = true;
         }
       }
@@ -860,15 +875,23 @@ __global__ void __launch_bounds__(BLOCK, 3) k_tile(OpDev op, MemDev md, FArgs fa
     } else {
       // out2: the own-state terms of rand_ct on the output row (and on x2), in rand_ct's operation order, no partner rows
       T* out2 = !(inf & INFO_PARTNERS) ? (T*)fa.out2 : nullptr;
-      const double *wa = nullptr, *wb = nullptr, *xa = nullptr, *xb = nullptr;
+      const double *wa = nullptr, *wb = nullptr, *xa = nullptr, *xb = nullptr, *pa_ = nullptr, *pb_ = nullptr;
       if (out2) {
         wa = wrow(fa.q2);
         if constexpr (O2 == 2) {
           wb = wrow(fa.q2 + 1);
           xa = wrow(fa.q2x);
           xb = wrow(fa.q2x + 1);
+          if (fa.apart) {
+            pa_ = wrow(QPA);
+            pb_ = wrow(QPB);
+          }
         }
       }
+      // route ed3's a_k at a node without flip partners (O2 = 2, apart; z - K' of ed3_kmul, x~ u h1 = h1 u x~ and
+      // x~ h'(t) = h1 t on the symmetric subspace): sq^2 [x~ (z - et) x~ + x~ h1 - h1 (u x~) - h1 t], t = out2 (c2x = 1,
+      // x~ = the normalised x2, the chain vector)
+      cplx pa3 = zero<cplx>();
       for (int l = 0; l < 9; ++l) {
         const T y = so * acc[l];
         out_store<(MT * sizeof(T) >= 128)>(&out[((size_t)i * 9 + l) * nbp + m], y);
@@ -883,9 +906,19 @@ __global__ void __launch_bounds__(BLOCK, 3) k_tile(OpDev op, MemDev md, FArgs fa
             const double xal = xa[(size_t)l * nbp], wal = wa[(size_t)l * nbp];
             const cplx cx = cplx{fa.c2x * fma(z.re, xal, xb[(size_t)l * nbp]), fa.c2x * z.im * xal};
             const cplx cy = cplx{fa.c2 * fma(z.re, wal, wb[(size_t)l * nbp]), fa.c2 * z.im * wal};
-            out_store<(MT * sizeof(T) >= 128)>(&out2[((size_t)i * 9 + l) * nbp + m], (zero<cplx>() + cx * xv) + cy * (1.0 * y));
+            const cplx tv = (zero<cplx>() + cx * xv) + cy * (1.0 * y);
+            out_store<(MT * sizeof(T) >= 128)>(&out2[((size_t)i * 9 + l) * nbp + m], tv);
+            if (pa_) {
+              const double pal = pa_[(size_t)l * nbp];
+              const cplx cw = cplx{fma(z.re, pal, pb_[(size_t)l * nbp]), z.im * pal};
+              const cplx uz = cplx{fma(z.re, xal, xb[(size_t)l * nbp]), z.im * xal};
+              pa3 = pa3 + (((bilin(xv, cw * xv) + bilin(xv, y)) - bilin(y, uz * xv)) - bilin(y, tv));
+            }
           }
         }
+      }
+      if constexpr (O2 == 2 && std::is_same<T, cplx>::value) {
+        if (pa_) dotv = (op.sq[i] * op.sq[i]) * pa3;
       }
     }
   }
@@ -897,6 +930,11 @@ __global__ void __launch_bounds__(BLOCK, 3) k_tile(OpDev op, MemDev md, FArgs fa
     __syncthreads();
     T* part = (T*)fa.partial2;
     block_reduce_member<T>(dotv2, MT, red, part + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * MT + threadIdx.x);
+  }
+  if constexpr (O2 == 3) {
+    __syncthreads();
+    T* part = (T*)fa.partial3;
+    block_reduce_member<T>(dotv3, MT, red, part + ((size_t)blockIdx.y * gridDim.x + blockIdx.x) * MT + threadIdx.x);
   }
 }
 
@@ -980,10 +1018,12 @@ __global__ void k_rand_nodes(OpDev op, MemDev md, const int* __restrict__ list, 
 
 // The listed nodes (those with flip partners) of the fused edc level, one thread per (listed node, member): t1 = X_o h1
 // (when t1 is non-null, as k_rand_nodes) and the node's term of a_k, sq^2 [x~ (E + X(z)) x~ + x~ h1 - h1 t1] with the
-// partner rows (lpart: nlist x nbp, summed by k_fin_a after the block partials of the tile pass).
+// partner rows (lpart: nlist x nbp, summed by k_fin_a after the block partials of the tile pass).  Route ed3 (ed3 != 0):
+// t1 = t = u x~ - Qt h1 (k_rand_nodes' order: the u term first) and sq^2 [x~ (z - et) x~ + x~ h1 - h1 (u x~) - h1 t].
 __global__ void __launch_bounds__(256) k_list_terms(OpDev op, MemDev md, const int* __restrict__ list, int nlist,
                                                      const cplx* __restrict__ h1, cplx* __restrict__ t1,
-                                                     const cplx* __restrict__ x, int a_e, cplx* __restrict__ lpart) {
+                                                     const cplx* __restrict__ x, int a_e, cplx* __restrict__ lpart,
+                                                     int ed3) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int nbp = md.nbp;
   const int j = (int)(t / nbp), m = (int)(t % nbp);
@@ -991,10 +1031,16 @@ __global__ void __launch_bounds__(256) k_list_terms(OpDev op, MemDev md, const i
   if (md.tact && !md.tact[m / md.mt]) return;
   const int i = list[j];
   const uint32_t inf = op.info[i];
-  cplx to[9], ex[9];
-  for (int l = 0; l < 9; ++l) { to[l] = zero<cplx>(); ex[l] = zero<cplx>(); }
+  cplx to[9], ex[9], ux[9];
+  for (int l = 0; l < 9; ++l) { to[l] = zero<cplx>(); ex[l] = zero<cplx>(); ux[l] = zero<cplx>(); }
   // every load before the stores at the end (a load behind a store waits for it as well)
-  if (t1) rand_node_term<cplx>(op, md, i, inf, m, h1, QO, 1.0, 0, false, to);
+  if (ed3) {
+    rand_node_term<cplx>(op, md, i, inf, m, x, QUA, 1.0, 1, true, to);
+    for (int l = 0; l < 9; ++l) ux[l] = to[l];
+    rand_node_term<cplx>(op, md, i, inf, m, h1, QQA, -1.0, 0, true, to);
+  } else if (t1) {
+    rand_node_term<cplx>(op, md, i, inf, m, h1, QO, 1.0, 0, false, to);
+  }
   if (a_e) rand_node_term<cplx>(op, md, i, inf, m, x, QE, 1.0, 1, false, ex);
   rand_node_term<cplx>(op, md, i, inf, m, x, QPA, 1.0, 1, true, ex);
   const double isi = op.isq[i], sq2 = op.sq[i] * op.sq[i];
@@ -1002,7 +1048,9 @@ __global__ void __launch_bounds__(256) k_list_terms(OpDev op, MemDev md, const i
   for (int l = 0; l < 9; ++l) {
     const size_t q = ((size_t)i * 9 + l) * nbp + m;
     const cplx xt = isi * x[q], h = h1[q];
-    pa = pa + ((bilin(xt, ex[l]) + bilin(xt, h)) - bilin(h, to[l]));
+    cplx term = (bilin(xt, ex[l]) + bilin(xt, h)) - bilin(h, to[l]);
+    if (ed3) term = term - bilin(h, ux[l]);
+    pa = pa + term;
   }
   if (t1)
     for (int l = 0; l < 9; ++l) t1[((size_t)i * 9 + l) * nbp + m] = to[l];
@@ -3351,6 +3399,8 @@ int launch_fused(const asr_gpu_op* op, const MemDev& md, const FArgs& fa, hipStr
   dim3 grid = node_grid(op, md);
   size_t lds = fused_lds_bytes(op, sizeof(T));
   if ((fa.upd || fa.apart) && !tile_kernel(op, md, sizeof(T))) return fail("operator pass: the fused level needs the tile kernel");
+  if (fa.tvv && (!fa.upd || fa.out2 || !fa.partial3 || !std::is_same<T, cplx>::value))
+    return fail("operator pass: the block dot needs the fused update of complex members");
   if ((fa.lcn > 0 || fa.out3) && (!std::is_same<T, double>::value || !tile_kernel(op, md, sizeof(T)) || fa.out2 || fa.upd ||
                                    (fa.lcn > 0 && fa.out3) || fa.lcn > 3))
     return fail("operator pass: output combinations need real members, the tile kernel, a plain output and one form");
@@ -3386,7 +3436,8 @@ int launch_fused(const asr_gpu_op* op, const MemDev& md, const FArgs& fa, hipStr
                                                 blds_bytes + (size_t)MT * (2 * op->nop + 1) * 9 * 8, strm, od, md, fa, \
                                                 (const double*)op->blkp + (size_t)op->ntype * (op->nslot + 1) * 90)
 #define ASR_CT3(MT, NS, O2) do { if (op->piid) ASR_CT4(MT, NS, O2, true); else ASR_CT4(MT, NS, O2, false); } while (0)
-#define ASR_CT(MT, NS) do { if (!fa.out2) ASR_CT3(MT, NS, 0); else if (!fa.x2) ASR_CT3(MT, NS, 1); \
+#define ASR_CT(MT, NS) do { if constexpr (std::is_same<T, cplx>::value) { if (fa.tvv) { ASR_CT3(MT, NS, 3); break; } } \
+                            if (!fa.out2) ASR_CT3(MT, NS, 0); else if (!fa.x2) ASR_CT3(MT, NS, 1); \
                             else ASR_CT3(MT, NS, 2); } while (0)
     if (md.mt == 8 && op->nslot == 18) ASR_CT(8, 18);
     else if (md.mt == 8) ASR_CT(8, 14);
@@ -3898,8 +3949,9 @@ struct ChainRun {
       HCHECK(hipMemsetAsync(B.cpr, 0, (size_t)ldab * nt * 9 * M.nbp * sizeof(T), stream));
     }
     lag = std::is_same<T, cplx>::value && !real_chain;
-    // the fused level of the edc chains: the member-tile operator pass with one d for every species
-    fused = lag && lin && project && !ed3 && !reorth && M.uniform_d && op->level_form == 0 && !op->asym &&
+    // the fused level of the edc chains and of the chain blocks (route ed3, EPI): the member-tile operator pass with one
+    // d for every species
+    fused = lag && lin && project && !reorth && M.uniform_d && op->level_form == 0 && !op->asym &&
             tile_kernel(op, md, sizeof(T));
     if (fused) {
       if (op->nplist > 0 && dalloc(&B.lpart, (size_t)op->nplist * M.nbp)) return 1;
@@ -4150,7 +4202,8 @@ struct ChainRun {
   }
 
   // the fused level, first half: h1 = rand_c(u~) + D S D u~, t1 = X_o h1 and the partials of a_k (tile pass at the nodes
-  // without flip partners, k_list_terms at the others)
+  // without flip partners, k_list_terms at the others).  Route ed3: h1 = h' u~ and t1 = t = u u~ - Qt h1 (the first pass
+  // of apply_ed3_op) with the partials of its a_k.
   int fused_apply() {
     if constexpr (!std::is_same<T, cplx>::value) {
       return fail("fused level: complex chains only");
@@ -4159,15 +4212,21 @@ struct ChainRun {
       f.xs = B.u; f.xs_norm = 1; f.s_din = 1; f.s_dout = 1; f.s_coef = 1.0;
       f.xr[0] = B.u; f.qr[0] = QC; f.r_norm[0] = 1; f.r_coef[0] = 1.0; f.nr = 1;
       f.out = B.h1;
-      if (!M.zero_o) { f.out2 = B.t1; f.q2 = QO; f.c2 = 1.0; }
-      f.apart = 1; f.a_e = M.zero_e ? 0 : 1; f.uo = B.u; f.partial = B.part;
+      const bool has_t1 = M.ed3 || !M.zero_o;
+      if (M.ed3) {
+        f.out2 = B.t1; f.q2 = QQA; f.c2 = -1.0;
+        f.x2 = B.u; f.q2x = QUA; f.x2norm = 1; f.c2x = 1.0;
+      } else if (!M.zero_o) {
+        f.out2 = B.t1; f.q2 = QO; f.c2 = 1.0;
+      }
+      f.apart = 1; f.a_e = (M.ed3 || M.zero_e) ? 0 : 1; f.uo = B.u; f.partial = B.part;
       if (launch_fused<T>(op, md, f, stream)) return 1;
       if (op->nplist > 0) {
         if (timer_mark(stream, KIND_FIX, true)) return 1;
         const size_t nt = (size_t)op->nplist * md.nbp;
         hipLaunchKernelGGL(k_list_terms, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, stream, dev_of(op), md,
-                           (const int*)op->plist, op->nplist, (const cplx*)B.h1, M.zero_o ? (cplx*)nullptr : (cplx*)B.t1,
-                           (const cplx*)B.u, M.zero_e ? 0 : 1, (cplx*)B.lpart);
+                           (const int*)op->plist, op->nplist, (const cplx*)B.h1, has_t1 ? (cplx*)B.t1 : (cplx*)nullptr,
+                           (const cplx*)B.u, f.a_e, (cplx*)B.lpart, M.ed3 ? 1 : 0);
         HCHECK(hipGetLastError());
         if (timer_mark(stream, 0, false)) return 1;
       }
@@ -4175,22 +4234,31 @@ struct ChainRun {
     }
   }
 
-  // the fused level, second half: w = sq (E u~ + h1 - h t1 + X(z) u~) - a u - b u_prev, projected, and <w, w>
+  // the fused level, second half: w = sq (E u~ + h1 - h t1 + X(z) u~) - a u - b u_prev, projected, and <w, w>; route ed3
+  // with sq [(z - et) u~ + h1 - u h1 - h'(t)] (the last pass of apply_ed3_op); the chain blocks also <tv, w>
   int fused_update() {
     FArgs f = fargs0();
-    int nr = 0;
-    if (!M.zero_o) {
-      f.xs = B.t1; f.s_din = 1; f.s_dout = 1; f.s_coef = -1.0;
-      f.xr[nr] = B.t1; f.qr[nr] = QC; f.r_norm[nr] = 0; f.r_coef[nr] = -1.0; ++nr;
+    if (M.ed3) {
+      f.xs = B.t1; f.xs_norm = 0; f.s_din = 1; f.s_dout = 1; f.s_coef = -1.0;
+      f.xr[0] = B.t1; f.qr[0] = QC; f.r_norm[0] = 0; f.r_coef[0] = -1.0; f.nr = 1;
+      f.xl[0] = B.u; f.lq[0] = QPA; f.l_norm[0] = 1; f.l_coef[0] = 1.0;
+      f.xl[1] = B.h1; f.lq[1] = QUA; f.l_norm[1] = 0; f.l_coef[1] = -1.0; f.nl = 2;
+    } else {
+      int nr = 0;
+      if (!M.zero_o) {
+        f.xs = B.t1; f.s_din = 1; f.s_dout = 1; f.s_coef = -1.0;
+        f.xr[nr] = B.t1; f.qr[nr] = QC; f.r_norm[nr] = 0; f.r_coef[nr] = -1.0; ++nr;
+      }
+      if (!M.zero_e) { f.xr[nr] = B.u; f.qr[nr] = QE; f.r_norm[nr] = 1; f.r_coef[nr] = 1.0; ++nr; }
+      f.nr = nr;
+      f.xl[0] = B.u; f.lq[0] = QPA; f.l_norm[0] = 1; f.l_coef[0] = 1.0; f.nl = 1;
     }
-    if (!M.zero_e) { f.xr[nr] = B.u; f.qr[nr] = QE; f.r_norm[nr] = 1; f.r_coef[nr] = 1.0; ++nr; }
-    f.nr = nr;
-    f.xl[0] = B.u; f.lq[0] = QPA; f.l_norm[0] = 1; f.l_coef[0] = 1.0; f.nl = 1;
     f.xo[0] = B.h1; f.o_coef[0] = 1.0; f.no = 1;
     f.out_sq = 1;
     f.upd = 1; f.proj = project; f.uo = B.u; f.upv = B.up; f.aeff = B.a_eff; f.beff = B.b_eff; f.scv = s_cur();
     f.scpv = s_prev(); f.act = B.active;
     f.out = B.w; f.partial = B.part; f.partial2 = B.pchk;
+    if (ed3) { f.tvv = B.tv; f.partial3 = B.part2; }
     return launch_fused<T>(op, md, f, stream);
   }
 

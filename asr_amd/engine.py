@@ -189,9 +189,10 @@ class GpuOperator:
         _check(lib().asr_gpu_op_set_reorth(self.h, int(bool(reorth))), "asr_gpu_op_set_reorth")
 
     def set_level_form(self, form: int):
-        """Form of an edc chain level: 0 the fused level (a_n from the first operator pass, the update folded into the
-        second), 1 the separate update pass in the reference's order (edc_green).  For the adjoint: 0 the recomputed level
-        and u_bar on the operator's last pass, 1 in passes of their own."""
+        """Form of an edc chain level (and of the chain blocks of route ed3 and EPI): 0 the fused level (a_n from the first
+        operator pass, the update folded into the second), 1 the separate update pass in the reference's order
+        (edc_green, chain).  For the adjoint: 0 the recomputed level and u_bar on the operator's last pass, 1 in passes of
+        their own."""
         _check(lib().asr_gpu_op_set_level_form(self.h, int(form)), "asr_gpu_op_set_level_form")
 
     def set_compaction(self, on: bool):

@@ -81,7 +81,10 @@ int asr_gpu_op_set_reorth(asr_gpu_op* op, int reorth);
  * on the symmetric subspace, and folds the three-term update, the projection and the norm into the second pass; form 1
  * runs the update as a pass of its own after the operator, in the reference's order.  The coefficients of the two forms
  * agree to rounding.  Form 0 applies to chains on the member-tile operator pass (member tile 8 or 16, one d for every
- * species, the projection on); other chains take form 1.  For asr_gpu_lanczos_adjoint form 0 (on the member-tile pass)
+ * species, the projection on); other chains take form 1.  The chain blocks take the same forms: route ed3 (chain,
+ * src/asr_ed3.f90:169-184) with a_n = sum_i sq_i^2 [u~ (z - et) u~ + u~ h1 - h1 (u u~) - h1 t]_i, h1 = h' u~,
+ * t = u u~ - Qt h1, and the EPI pair blocks; in form 0 the second pass also takes the partial sums of <t, w> for the
+ * projection c_n+1 of the block's second vector.  For asr_gpu_lanczos_adjoint form 0 (on the member-tile pass)
  * computes the recomputed level u_k+1 and the combination u_bar_k on the operator's last pass instead of in passes of
  * their own; the gradients agree to rounding. */
 int asr_gpu_op_set_level_form(asr_gpu_op* op, int form);

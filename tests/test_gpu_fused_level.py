@@ -6,6 +6,11 @@ a_n of the fused level comes from the first operator pass, sum_i sq_i^2 [u~ (E +
 which equals sum(u w) on the symmetric subspace; the coefficients agree with the oracle to the parity bar (1e-9) and with
 the separate pass to rounding, with and without the projection, with members that leave at their own depth, across the
 rescaling of the lagged normalisation (every 16 levels), with o = 0 and with e != 0, at member tiles 8 and 16.
+
+The chain blocks take the same fused level: route ed3 (chain, src/asr_ed3.f90:169-184; a_n from the first pass as
+sum_i sq_i^2 [u~ (z - et) u~ + u~ h1 - h1 (u u~) - h1 t]_i with h1 = h' u~, t = u u~ - Qt h1) and the EPI pair blocks
+(the edc operator), both with the projections c_n = <t, u_n> of the second vector from the update inside the second pass:
+a_n, b_n, c_n and the blocks against the separate pass to rounding and against the oracle to the parity bar.
 """
 import numpy as np
 import pytest
@@ -141,3 +146,71 @@ def test_fused_level_falls_back_on_an_asymmetric_operator():
         assert mu[j] == n and rel(a[j, :n], ar[:n]) < 1e-9 and rel(b[j, :n - 1], br[:n - 1]) < 1e-9
     kinds = list(G.profile(members, 4)[1])
     assert KIND_UPDATE in kinds and KIND_SPMM_UPD not in kinds, kinds
+
+
+def _block_forms(G, run):
+    out = {}
+    for form in (0, 1):
+        G.set_level_form(form)
+        out[form] = run()
+    G.set_level_form(0)
+    return out
+
+
+@pytest.mark.parametrize("tile", [8, 16])
+def test_fused_level_ed3_blocks(tile):
+    eng = _engine()
+    from asr_amd import ed3 as E3
+    from oracle.oracle import OracleOp
+    wl = W.ternary_ed3(L=4, levels=2, depth=20, seed=21)
+    sel = list(range(0, len(wl.members), 2))
+    members = [wl.members[j] for j in sel]
+    nlev = np.array([20 - (k % 5) * 3 for k in range(len(sel))], np.int32)  # 8..20 levels, past the rescale at 16
+    G = eng.GpuOperator(wl.op)
+    G.set_tile(tile)
+    G.set_depth_tol(0.0, "hull")
+    out = _block_forms(G, lambda: G.ed3_blocks(members, wl.info["cf"][sel], wl.info["oc"][sel], nlev, mode=0))
+    (bk0, a0, b0, c0, mu0, _), (bk1, a1, b1, c1, mu1, _) = out[0], out[1]
+    assert np.array_equal(mu0, mu1)
+    # the fused a_n is a different sum: equal to rounding, not bit for bit (so the fused level did run)
+    assert not np.array_equal(a0, a1)
+    # c_n falls to 1e-5 of c_0 over the chain: its rounding, relative to the largest, reaches a few 1e-12
+    assert rel(a0, a1) < 1e-12 and rel(b0, b1) < 1e-12 and rel(c0, c1) < 1e-10 and rel(bk0, bk1) < 1e-10
+    O = OracleOp(wl.op)
+    for q in (0, 3, len(sel) - 1):
+        j = sel[q]
+        m = wl.members[j]
+        et, uu, qt = E3.ed3_coefs(wl.info["ed"][j], m.z)
+        bko, chains, _ = O.ed3_blocks(m, et, uu, qt, wl.info["oc"][j], m.z, m.l0, int(nlev[q]), project=True, tail="hull")
+        for sd in range(2):
+            ao, bo, co, mo, closed = chains[sd]
+            assert mu0[q, sd] == mo
+            assert rel(a0[q, sd, :mo], ao[:mo]) < 1e-9 and rel(b0[q, sd, :mo - 1], bo[:mo - 1]) < 1e-9, (j, sd)
+            assert rel(c0[q, sd, :mo], co[:mo]) < 1e-9, (j, sd)
+        assert rel(bk0[q], bko) < 1e-8, j
+
+
+@pytest.mark.parametrize("tile", [8, 16])
+def test_fused_level_epi_blocks(tile):
+    eng = _engine()
+    from oracle.oracle import OracleOp
+    wl = W.fcc_epi_pair(L=4, levels=2, seed=31)
+    nlev = np.array([20 - (k % 5) * 3 for k in range(len(wl.members))], np.int32)
+    G = eng.GpuOperator(wl.op)
+    G.set_tile(tile)
+    G.set_depth_tol(0.0, "hull")
+    out = _block_forms(G, lambda: G.epi_blocks(wl.members, wl.info["v9"], nlev, mode=0))
+    (x0, a0, b0, c0, mu0, _), (x1, a1, b1, c1, mu1, _) = out[0], out[1]
+    assert np.array_equal(mu0, mu1)
+    assert not np.array_equal(a0, a1)
+    assert rel(a0, a1) < 1e-12 and rel(b0, b1) < 1e-12 and rel(c0, c1) < 1e-10 and rel(x0, x1) < 1e-10
+    O = OracleOp(wl.op)
+    for j in (0, 7, len(wl.members) - 1):
+        m = wl.members[j]
+        xo, chains = O.epi_blocks(m, m.z * m.pa + m.pb, wl.info["v9"][j], m.z, int(nlev[j]), project=True, tail="hull")
+        for sd in range(2):
+            ao, bo, co, mo, closed = chains[sd]
+            assert mu0[j, sd] == mo
+            assert rel(a0[j, sd, :mo], ao[:mo]) < 1e-9 and rel(b0[j, sd, :mo - 1], bo[:mo - 1]) < 1e-9, (j, sd)
+            assert rel(c0[j, sd, :mo], co[:mo]) < 1e-9, (j, sd)
+        assert rel(x0[j], xo) < 1e-8, j
