@@ -5059,7 +5059,10 @@ int run_multi(int ngroup, asr_gpu_op* const* ops, const int* ngch, const int* st
 // falls back to mode 0).
 int run_blocks(asr_gpu_op* op, int nmem, const int* so, const double* params, const double* pab, const int* fz,
                const double* conc, const double* z, const int* nlev, const BlockIn& e3, int project, int ldab, cplx* a, cplx* b,
-               cplx* c, int* mused, int* closed, double* snorm, double* tnorm, int mode, double* elapsed_ms) {
+               cplx* c, int* mused, int* closed, double* snorm, double* tnorm, int mode, double* elapsed_ms);
+int run_blocks_once(asr_gpu_op* op, int nmem, const int* so, const double* params, const double* pab, const int* fz,
+                    const double* conc, const double* z, const int* nlev, const BlockIn& e3, int project, int ldab, cplx* a,
+                    cplx* b, cplx* c, int* mused, int* closed, double* snorm, double* tnorm, int mode, double* elapsed_ms) {
   const int nsp9 = op->nsp * 9, ncn = op->ncls * op->nsmax;
   *elapsed_ms = 0.0;
   if (mode == 2 && op->depth_tol > 0.0 && op->depth_tail == 1) mode = 0;
@@ -5130,6 +5133,18 @@ int run_blocks(asr_gpu_op* op, int nmem, const int* so, const double* params, co
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return status;
+}
+
+// a run whose fused level found the operator not symmetric on the symmetric subspace is repeated with the separate
+// update pass (the operator keeps that form)
+int run_blocks(asr_gpu_op* op, int nmem, const int* so, const double* params, const double* pab, const int* fz,
+               const double* conc, const double* z, const int* nlev, const BlockIn& e3, int project, int ldab, cplx* a, cplx* b,
+               cplx* c, int* mused, int* closed, double* snorm, double* tnorm, int mode, double* elapsed_ms) {
+  int st = RERUN_ASYM;
+  for (int pass = 0; pass < 2 && st == RERUN_ASYM; ++pass)
+    st = run_blocks_once(op, nmem, so, params, pab, fz, conc, z, nlev, e3, project, ldab, a, b, c, mused, closed, snorm,
+                         tnorm, mode, elapsed_ms);
+  return st;
 }
 
 // Projected chains of route edc on one operator (the chains of asr_gpu_chains_edc_proj): mode 0 host-issued levels, 1 the

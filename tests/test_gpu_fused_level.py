@@ -118,22 +118,27 @@ def test_fused_level_graph_and_reproducible(wl):
     assert np.array_equal(res[0][0], a) and np.array_equal(res[0][1], b)
 
 
-def test_fused_level_falls_back_on_an_asymmetric_operator():
-    """Bond blocks chosen by the type of the node's own site only (type 1: x 1.1) make S non-symmetric; the fused level's
-    a_n then differs from the direct <u, H u>, the run is repeated with the separate update pass (the oracle's chains),
-    and the operator keeps that form."""
+def _asymmetric_operator():
+    """Bond blocks chosen by the type of the node's own site only (type 1: x 1.1): S is not symmetric."""
     from asr_amd import augcls as A
     from asr_amd import structure as S
-    eng = _engine()
     g, block, block0 = W.graph_and_blocks("fcc", 4)
     cls = np.zeros(g.n, np.int32)
     perm, mats = S.class_ops(g.coord, 8, cls, "d2h")
     fl = np.ones(g.n, np.uint8)
     fl[0] = 0
     stype = (np.arange(g.n) % 2).astype(np.int32)
-    op = A.reorder_nodes(A.build_operator(g, cls, [3], [[0, 1, 2]], [[0.66, 0.14, 0.2]], perm, S.orbital_rep(mats), 0, 2,
-                                          maxrank=1, flippable=fl, stype=stype, block=np.stack([block, 1.1 * block]),
-                                          block0=np.stack([block0, block0]), nsp=3))
+    return A.reorder_nodes(A.build_operator(g, cls, [3], [[0, 1, 2]], [[0.66, 0.14, 0.2]], perm, S.orbital_rep(mats), 0, 2,
+                                            maxrank=1, flippable=fl, stype=stype, block=np.stack([block, 1.1 * block]),
+                                            block0=np.stack([block0, block0]), nsp=3))
+
+
+def test_fused_level_falls_back_on_an_asymmetric_operator():
+    """Bond blocks chosen by the type of the node's own site only (type 1: x 1.1) make S non-symmetric; the fused level's
+    a_n then differs from the direct <u, H u>, the run is repeated with the separate update pass (the oracle's chains),
+    and the operator keeps that form."""
+    eng = _engine()
+    op = _asymmetric_operator()
     spins = W.edc_params(np.random.default_rng(5), 3, 1)
     members = [W.edc_member(spins[0], cen, l0, complex(-0.2, 0.15)) for cen in range(3) for l0 in (0, 4, 8)]
     G = eng.GpuOperator(op)
@@ -215,3 +220,28 @@ def test_fused_level_epi_blocks(tile):
             assert rel(a0[j, sd, :mo], ao[:mo]) < 1e-9 and rel(b0[j, sd, :mo - 1], bo[:mo - 1]) < 1e-9, (j, sd)
             assert rel(c0[j, sd, :mo], co[:mo]) < 1e-9, (j, sd)
         assert rel(x0[j], xo) < 1e-8, j
+
+
+def test_fused_block_level_falls_back_on_an_asymmetric_operator():
+    """Route ed3's blocks on the same non-symmetric operator: the fused level's a_n differs from the direct one, the run
+    is repeated with the separate update pass, and the chains equal the oracle's."""
+    eng = _engine()
+    from asr_amd import ed3 as E3
+    from oracle.oracle import OracleOp
+    wl = W.ternary_ed3(L=4, levels=2, depth=10, seed=21, op=_asymmetric_operator())
+    sel = [0, 4, 8, 31]
+    members = [wl.members[j] for j in sel]
+    G = eng.GpuOperator(wl.op)
+    G.set_tile(8)
+    G.set_depth_tol(0.0, "hull")
+    bk, a, b, c, mu, _ = G.ed3_blocks(members, wl.info["cf"][sel], wl.info["oc"][sel], 10, mode=0)
+    O = OracleOp(wl.op)
+    for q, j in enumerate(sel):
+        m = wl.members[j]
+        et, uu, qt = E3.ed3_coefs(wl.info["ed"][j], m.z)
+        bko, chains, _ = O.ed3_blocks(m, et, uu, qt, wl.info["oc"][j], m.z, m.l0, 10, project=True, tail="hull")
+        for sd in range(2):
+            ao, bo, co, mo, closed = chains[sd]
+            assert mu[q, sd] == mo
+            assert rel(a[q, sd, :mo], ao[:mo]) < 1e-9 and rel(b[q, sd, :mo - 1], bo[:mo - 1]) < 1e-9, (j, sd)
+            assert rel(c[q, sd, :mo], co[:mo]) < 1e-9, (j, sd)
