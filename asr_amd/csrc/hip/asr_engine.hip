@@ -2198,66 +2198,133 @@ __global__ void k_start_unit(T* __restrict__ u, int nbp, const int* __restrict__
   u[(size_t)l0[j] * nbp + slots[j]] = v;
 }
 
-// The window run's queue on the device (one thread; the slots are at most a few hundred): after a level, the members that
-// have left hand over their slots (extraction list xs -> xc; a slot whose chain ended early, closed, keeps a non-zero
-// vector and is flagged dirty), the free slots take the next chains of the queue in slot order, and the start list is
-// written dirty slots first (they need a full start column) then clean ones, each in slot order: the lists, counts and
-// tile activity of the host's refill, without a host round trip.  wq: [0] next chain, [1] extractions, [2] dirty starts,
-// [3] starts, [4] live members, [5] 1 while anything remains to run.
-__global__ void k_refill(ChainDev cd, int nb, int mt, int nchain, int* wq, int* slot_chain, int* slot_dirty,
-                         const int* nl_all, const int* so_all, int* xs, int* xc, int* ws, int* wc, int* wv, int* wl0, int* tact,
-                         int ntile) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  int qn = wq[0], nx = 0;
-  for (int q = 0; q < nb; ++q) {
-    const int c = slot_chain[q];
-    if (c >= 0 && !cd.active[q]) {
-      xs[nx] = q;
-      xc[nx] = c;
-      ++nx;
+// exclusive prefix sum over the threads of a block (in thread order) and the block's total; every thread calls it
+constexpr int RF_TPB = 256;
+__device__ inline int block_excl_scan(int v, int* wsum, int* total) {
+  const int lane = threadIdx.x & (WAVE - 1), wave = threadIdx.x >> 6;
+  int x = v;
+  for (int off = 1; off < WAVE; off <<= 1) {
+    const int y = __shfl_up(x, off);
+    if (lane >= off) x += y;
+  }
+  if (lane == WAVE - 1) wsum[wave] = x;
+  __syncthreads();
+  int before = 0, tot = 0;
+  for (int w = 0; w < RF_TPB / WAVE; ++w) {
+    if (w < wave) before += wsum[w];
+    tot += wsum[w];
+  }
+  __syncthreads();
+  *total = tot;
+  return before + x - v;
+}
+
+// The window run's queue on the device (one block; the slots in chunks of RF_TPB, slot order kept by prefix sums):
+// after a level, the members that have left hand over their slots (extraction list xs -> xc; a slot whose chain ended
+// early, closed, keeps a non-zero vector and is flagged dirty), the free slots take the next chains of the queue in slot
+// order, and the start list is written dirty slots first (they need a full start column) then clean ones, each in slot
+// order: the lists, counts and tile activity of the host's refill, without a host round trip.  wq: [0] next chain,
+// [1] extractions, [2] dirty starts, [3] starts, [4] live members, [5] 1 while anything remains to run.  Dynamic LDS:
+// nb ints (the slots' activity after the hand-over).
+__global__ void __launch_bounds__(RF_TPB) k_refill(ChainDev cd, int nb, int mt, int nchain, int* wq, int* slot_chain,
+                                                     int* slot_dirty, const int* nl_all, const int* so_all, int* xs, int* xc,
+                                                     int* ws, int* wc, int* wv, int* wl0, int* tact, int ntile) {
+  extern __shared__ int live[];
+  __shared__ int wsum[RF_TPB / WAVE];
+  const int qn = wq[0];
+  // the members that have left, in slot order
+  int nx = 0;
+  for (int b = 0; b < nb; b += RF_TPB) {
+    const int q = b + (int)threadIdx.x;
+    const int c = q < nb ? slot_chain[q] : -1;
+    const int out = (c >= 0 && !cd.active[q]) ? 1 : 0;
+    int tot;
+    const int p = block_excl_scan(out, wsum, &tot);
+    if (out) {
+      xs[nx + p] = q;
+      xc[nx + p] = c;
       slot_chain[q] = -1;
       if (cd.closed && cd.closed[q]) slot_dirty[q] = 1;
     }
+    nx += tot;
   }
-  // the slots that take a chain now, in slot order: count the dirty ones first so the list can be written partitioned
-  int nfree_take = 0, ndirty = 0;
-  for (int q = 0; q < nb && qn + nfree_take < nchain; ++q)
-    if (slot_chain[q] < 0) {
-      ++nfree_take;
-      ndirty += slot_dirty[q] ? 1 : 0;
+  __syncthreads();
+  // the free slots, and the number that take a chain now
+  int nfree = 0;
+  for (int b = 0; b < nb; b += RF_TPB) {
+    const int q = b + (int)threadIdx.x;
+    int tot;
+    (void)block_excl_scan(q < nb && slot_chain[q] < 0 ? 1 : 0, wsum, &tot);
+    nfree += tot;
+  }
+  const int take = min(nfree, max(nchain - qn, 0));
+  // the dirty ones among the takers (their starts come first)
+  int ndirty = 0;
+  {
+    int rf = 0;
+    for (int b = 0; b < nb; b += RF_TPB) {
+      const int q = b + (int)threadIdx.x;
+      const int fr = (q < nb && slot_chain[q] < 0) ? 1 : 0;
+      int tot, totd;
+      const int r = rf + block_excl_scan(fr, wsum, &tot);
+      const int dt = (fr && r < take && slot_dirty[q]) ? 1 : 0;
+      (void)block_excl_scan(dt, wsum, &totd);
+      rf += tot;
+      ndirty += totd;
     }
-  int jd = 0, jc = ndirty;
-  for (int q = 0; q < nb && qn < nchain; ++q)
-    if (slot_chain[q] < 0) {
-      const int c = qn++;
-      slot_chain[q] = c;
-      const int j = slot_dirty[q] ? jd++ : jc++;
-      slot_dirty[q] = 0;
+  }
+  // the takers in slot order: chain qn + (free rank), start list position by dirtiness
+  int rf = 0, rd = 0;
+  for (int b = 0; b < nb; b += RF_TPB) {
+    const int q = b + (int)threadIdx.x;
+    const int fr = (q < nb && slot_chain[q] < 0) ? 1 : 0;
+    int tot, totd;
+    const int r = rf + block_excl_scan(fr, wsum, &tot);
+    const bool taker = fr && r < take;
+    const int dirty = (taker && slot_dirty[q]) ? 1 : 0;
+    const int rdq = rd + block_excl_scan(dirty, wsum, &totd);
+    int lv = 0;
+    if (q < nb) lv = (slot_chain[q] >= 0 && cd.active[q]) ? 1 : 0;
+    if (taker) {
+      const int c = qn + r;
+      const int j = dirty ? rdq : ndirty + (r - rdq);
       ws[j] = q;
       wc[j] = c;
       wv[j] = nl_all[c];
       wl0[j] = so_all[c];
+      lv = nl_all[c] > 0 ? 1 : 0;
     }
-  wq[0] = qn;
-  wq[1] = nx;
-  wq[2] = ndirty;
-  wq[3] = nfree_take;
-  // running members: those still active, and the new ones with levels to run
-  int nlive = 0;
-  for (int t = 0; t < ntile; ++t) tact[t] = 0;
-  for (int q = 0; q < nb; ++q) {
-    int live = 0;
-    if (slot_chain[q] >= 0) live = cd.active[q] ? 1 : 0;
-    tact[q / mt] |= live;
-    nlive += live;
+    __syncthreads();  // every thread has read slot_chain and slot_dirty of this chunk
+    if (taker) {
+      slot_chain[q] = qn + r;
+      slot_dirty[q] = 0;
+    }
+    if (q < nb) live[q] = lv;
+    rf += tot;
+    rd += totd;
   }
-  for (int j = 0; j < nfree_take; ++j)
-    if (wv[j] > 0) {
-      tact[ws[j] / mt] = 1;
-      ++nlive;
-    }
-  wq[4] = nlive;
-  wq[5] = (nlive > 0 || qn < nchain) ? 1 : 0;
+  __syncthreads();
+  // running members per tile and in all
+  int nl = 0;
+  for (int b = 0; b < nb; b += RF_TPB) {
+    const int q = b + (int)threadIdx.x;
+    int tot;
+    (void)block_excl_scan(q < nb ? live[q] : 0, wsum, &tot);
+    nl += tot;
+  }
+  for (int t = (int)threadIdx.x; t < ntile; t += RF_TPB) {
+    int any = 0;
+    for (int q = t * mt; q < min(nb, (t + 1) * mt); ++q) any |= live[q];
+    tact[t] = any;
+  }
+  if (threadIdx.x == 0) {
+    wq[0] = qn + take;
+    wq[1] = nx;
+    wq[2] = ndirty;
+    wq[3] = take;
+    wq[4] = nl;
+    wq[5] = (nl > 0 || qn + take < nchain) ? 1 : 0;
+  }
 }
 
 __global__ void k_level_end(int* level, const int* active, int nb, int mt, int* tact, int ntile) {
@@ -4006,7 +4073,8 @@ struct ChainRun {
     cd.real_chain = real_chain ? 1 : 0; cd.tiny = tiny;
     cd.cvec = B.c; cd.closed = B.closed;
     cd.s0 = win ? s0_d : nullptr;
-    if (win && op->queue_dev) {
+    // (k_refill keeps one int per slot in LDS: windows of up to 8192 slots)
+    if (win && op->queue_dev && M.nb <= 8192) {
       const int nch = (int)nl_all.size();
       if (dalloc(&wq_d, 8) || dalloc(&slot_chain_d, M.nbp) || dalloc(&slot_dirty_d, M.nbp) || dalloc(&nl_all_d, nch) ||
           dalloc(&so_all_d, nch) || dalloc(&xslots_d, M.nbp) || dalloc(&xchains_d, M.nbp))
@@ -4684,7 +4752,8 @@ struct ChainRun {
     if (join_test()) return 1;  // this level's depth test writes conv, which k_member_start clears for a new member
     const int nb = M.nb, mt = M.mt, nch = (int)nl_all.size();
     hipLaunchKernelGGL(k_conv_leave<T>, dim3((nb + 63) / 64), dim3(64), 0, stream, cd, nb, conv_d, B.b);
-    hipLaunchKernelGGL(k_refill, dim3(1), dim3(64), 0, stream, cd, nb, mt, nch, wq_d, slot_chain_d, slot_dirty_d,
+    hipLaunchKernelGGL(k_refill, dim3(1), dim3(RF_TPB), (size_t)nb * sizeof(int), stream, cd, nb, mt, nch, wq_d, slot_chain_d,
+                       slot_dirty_d,
                        (const int*)nl_all_d, (const int*)so_all_d, xslots_d, xchains_d, wslots, wchains, wvals, wl0, tact,
                        ntile);
     hipLaunchKernelGGL(k_extract<T>, dim3(nb), dim3(64), 0, stream, a_all, b_all, mused_all, (const T*)B.a, (const T*)B.b,
