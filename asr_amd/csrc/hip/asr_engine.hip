@@ -3053,13 +3053,81 @@ __global__ void k_rl_shift(double* u, double* up, const double* w, const double*
   }
 }
 
-__global__ void __launch_bounds__(BLOCK) k_kpm_fin(const double* partial, int nbx, int vt, int nvec, double inv_dim,
-                                                   double* mu_per, int nmom, int n) {
-  __shared__ double lds[BLOCK / WAVE];
-  const int v = blockIdx.x;
-  if (v >= nvec) return;
-  double s = block_sum_partials<double>(partial, nbx, vt, v, lds);
-  if (threadIdx.x == 0) mu_per[(size_t)v * nmom + n] = s * inv_dim;
+// The moments' partial sums in two stages (instead of one block per vector, whose strided reads fetched each line once
+// per vector: 73 us per step, 146 with the doubling's two sums, against the 1.65 ms step): k_kpm_psum, block (g, tv) of
+// KPS x ntv with KPW waves, wave w summing the rows b = g + KPS (KPW r + w), r = 0, 1, ... of vector tile tv in order
+// (lanes = the tile's vectors, each row read whole), then the waves in order; k_kpm_pfin, one block per vector tile,
+// sums the KPS values of each vector, KPS / KPF_WAVES in order per wave, then the waves in order.  part2 (moment
+// doubling): the second partials alike, after the first in stage.
+constexpr int KPS = 256, KPW = 4, KPF_WAVES = 16;
+__global__ void __launch_bounds__(WAVE * KPW) k_kpm_psum(const double* __restrict__ part, const double* __restrict__ part2,
+                                                         int nbx, int vt, double* __restrict__ stage) {
+  __shared__ double red[2][KPW][WAVE];
+  const int g = blockIdx.x, tv = blockIdx.y, lane = threadIdx.x & (WAVE - 1), w = threadIdx.x >> 6;
+  double s = 0.0, s2 = 0.0;
+  if (lane < vt) {
+    // the loads of several rows in flight ahead of the (ordered) sum
+    const int b0 = g + KPS * w, bs = KPS * KPW;
+    const int nr = b0 < nbx ? (nbx - 1 - b0) / bs + 1 : 0;
+    const size_t o0 = ((size_t)tv * nbx + b0) * vt + lane, st = (size_t)bs * vt;
+#pragma unroll 8
+    for (int r = 0; r < nr; ++r) s += part[o0 + r * st];
+    if (part2) {
+#pragma unroll 8
+      for (int r = 0; r < nr; ++r) s2 += part2[o0 + r * st];
+    }
+  }
+  red[0][w][lane] = s;
+  red[1][w][lane] = s2;
+  __syncthreads();
+  if (w != 0 || lane >= vt) return;
+  s = red[0][0][lane];
+  s2 = red[1][0][lane];
+  for (int k = 1; k < KPW; ++k) {
+    s += red[0][k][lane];
+    s2 += red[1][k][lane];
+  }
+  const size_t so = ((size_t)tv * KPS + g) * vt + lane;
+  stage[so] = s;
+  if (part2) stage[(size_t)gridDim.y * KPS * vt + so] = s2;
+}
+// m < 0: mu_per[v][n] = sum / dim; m >= 1 (moment doubling, T_2m = 2 T_m^2 - T_0, T_2m-1 = 2 T_m T_m-1 - T_1):
+// mu_2m = 2 <t_m|t_m> / dim - mu_0 and, with the second partials (two), mu_2m-1 = 2 <t_m|t_m-1> / dim - mu_1
+__global__ void __launch_bounds__(WAVE * KPF_WAVES) k_kpm_pfin(const double* __restrict__ stage, int two, int vt, int ntv,
+                                                               int nvec, double inv_dim, double* mu_per, int nmom, int n,
+                                                               int m) {
+  __shared__ double red[2][KPF_WAVES][WAVE];
+  const int tv = blockIdx.x, lane = threadIdx.x & (WAVE - 1), w = threadIdx.x >> 6;
+  double s = 0.0, s2 = 0.0;
+  if (lane < vt) {
+    constexpr int per = KPS / KPF_WAVES;
+    const double* __restrict__ st = stage + ((size_t)tv * KPS + w * per) * vt + lane;
+#pragma unroll
+    for (int g = 0; g < per; ++g) s += st[(size_t)g * vt];
+    if (two) {
+      const double* __restrict__ st2 = st + (size_t)ntv * KPS * vt;
+#pragma unroll
+      for (int g = 0; g < per; ++g) s2 += st2[(size_t)g * vt];
+    }
+  }
+  red[0][w][lane] = s;
+  red[1][w][lane] = s2;
+  __syncthreads();
+  const int v = tv * vt + lane;
+  if (w != 0 || lane >= vt || v >= nvec) return;
+  s = red[0][0][lane];
+  s2 = red[1][0][lane];
+  for (int k = 1; k < KPF_WAVES; ++k) {
+    s += red[0][k][lane];
+    s2 += red[1][k][lane];
+  }
+  double* mv = mu_per + (size_t)v * nmom;
+  if (m < 0) {
+    mv[n] = s * inv_dim;
+  } else {
+    if (2 * m < nmom) mv[2 * m] = 2.0 * s * inv_dim - mv[0];
+    if (two && 2 * m - 1 < nmom) mv[2 * m - 1] = 2.0 * s2 * inv_dim - mv[1];
+  }
 }
 
 // mu[n] = sum_v mu_per[v][n] / nvec, the vectors in order (the host's sum of asr_gpu_kpm_moments, on the device)
@@ -3069,23 +3137,6 @@ __global__ void k_kpm_mean(const double* __restrict__ mu_per, int nvec, int nmom
   double acc = 0.0;
   for (int v = 0; v < nvec; ++v) acc += mu_per[(size_t)v * nmom + n];
   mu[n] = acc / (double)nvec;
-}
-
-// moment doubling (T_2m = 2 T_m^2 - T_0, T_2m-1 = 2 T_m T_m-1 - T_1): from the partials of <t_m|t_m> and <t_m|t_m-1>,
-// mu_2m = 2 <t_m|t_m> / dim - mu_0 and mu_2m-1 = 2 <t_m|t_m-1> / dim - mu_1 (partial2 null: only mu_2m)
-__global__ void __launch_bounds__(BLOCK) k_kpm_fin2(const double* partial, const double* partial2, int nbx, int vt, int nvec,
-                                                    double inv_dim, double* mu_per, int nmom, int m) {
-  __shared__ double lds[BLOCK / WAVE];
-  const int v = blockIdx.x;
-  if (v >= nvec) return;
-  double* mv = mu_per + (size_t)v * nmom;
-  const double s1 = block_sum_partials<double>(partial, nbx, vt, v, lds);
-  __syncthreads();
-  if (threadIdx.x == 0 && 2 * m < nmom) mv[2 * m] = 2.0 * s1 * inv_dim - mv[0];
-  if (partial2) {
-    const double s2 = block_sum_partials<double>(partial2, nbx, vt, v, lds);
-    if (threadIdx.x == 0 && 2 * m - 1 < nmom) mv[2 * m - 1] = 2.0 * s2 * inv_dim - mv[1];
-  }
 }
 
 }  // namespace
@@ -6284,11 +6335,11 @@ int asr_gpu_real_apply(asr_gpu_real* R, int nvec, const double* x, double* y) {
 // The device buffers of one KPM recurrence: three Chebyshev vectors (dim x nvp), the per-block dot partials (and those of
 // moment doubling), the per-vector moments (nvec x nmom).
 struct KpmBufs {
-  double *t0 = nullptr, *t1 = nullptr, *t2 = nullptr, *part = nullptr, *part2 = nullptr, *mper = nullptr;
+  double *t0 = nullptr, *t1 = nullptr, *t2 = nullptr, *part = nullptr, *part2 = nullptr, *mper = nullptr, *stage = nullptr;
 };
 struct KpmSizes {
-  size_t vec, part, part2, mper;
-  size_t total() const { return 3 * vec + part + part2 + mper; }
+  size_t vec, part, part2, mper, stage;
+  size_t total() const { return 3 * vec + part + part2 + mper + stage; }
 };
 static KpmSizes kpm_sizes(const asr_gpu_real* R, int nmom, int nvec) {
   const size_t dim = (size_t)R->nsite * 9;
@@ -6300,6 +6351,7 @@ static KpmSizes kpm_sizes(const asr_gpu_real* R, int nmom, int nvec) {
   z.part = (size_t)grid.x * grid.y * vt;
   z.part2 = R->dbl ? (size_t)((R->nsite + 3) / 4) * grid.y * 64 : 0;
   z.mper = (size_t)nvec * nmom;
+  z.stage = (size_t)2 * grid.y * KPS * vt;
   return z;
 }
 static int kpm_check(int nmom, int nvec, uint64_t seed, const asr_gpu_real* R) {
@@ -6312,7 +6364,7 @@ static int kpm_check(int nmom, int nvec, uint64_t seed, const asr_gpu_real* R) {
 // The Chebyshev recurrence of kpm_moments on the handle's stream into the given buffers: mper[v][n] = <r_v|t_n> / dim.
 // ev (2 + 2 x steps events, or null): the whole recurrence and each timed Chebyshev step.
 static int kpm_run(asr_gpu_real* R, double c, double s, int nmom, int nvec, uint64_t seed, KpmBufs B, hipEvent_t* ev) {
-  double *t0 = B.t0, *t1 = B.t1, *t2 = B.t2, *part = B.part, *part2 = B.part2, *mper = B.mper;
+  double *t0 = B.t0, *t1 = B.t1, *t2 = B.t2, *part = B.part, *part2 = B.part2, *mper = B.mper, *stage = B.stage;
   const size_t dim = (size_t)R->nsite * 9;
   const int vt = 64;
   const int nvp = ((nvec + vt - 1) / vt) * vt;
@@ -6323,17 +6375,24 @@ static int kpm_run(asr_gpu_real* R, double c, double s, int nmom, int nvec, uint
     if (ev) HCHECK(hipEventRecord(ev[q], R->stream));
     return 0;
   };
+  // the moments of one step from its partials (nbx rows per vector tile; m < 0: mu_n, else the doubling pair of t_m)
+  auto fin = [&](const double* p1, const double* p2, int nbx, int n, int m) {
+    hipLaunchKernelGGL(k_kpm_psum, dim3(KPS, grid.y), dim3(WAVE * KPW), 0, R->stream, p1, p2, nbx, vt, stage);
+    hipLaunchKernelGGL(k_kpm_pfin, dim3(grid.y), dim3(WAVE * KPF_WAVES), 0, R->stream, (const double*)stage, p2 ? 1 : 0, vt,
+                       (int)grid.y,
+                       nvec, inv_dim, mper, nmom, n, m);
+  };
   if (real_shift(R, c)) return 1;
   if (mark(0)) return 1;
   // t0 = r (stored as d o r), mu_0 = <r|r>/dim: the step with alpha = 0, beta = -1
   hipLaunchKernelGGL((k_kpm_u<true, 0>), grid, dim3(BLOCK), 0, R->stream, rd, (const double*)R->blk, nvec, nvp,
                      (const double*)nullptr, (const double*)nullptr, t0, 0.0, 0.0, -1.0, seed, part, 1);
-  hipLaunchKernelGGL(k_kpm_fin, dim3(nvec), dim3(BLOCK), 0, R->stream, part, (int)grid.x, vt, nvec, inv_dim, mper, nmom, 0);
+  fin(part, nullptr, (int)grid.x, 0, -1);
   if (nmom > 1) {
     // t1 = (H r - c r) / s
     hipLaunchKernelGGL((k_kpm_u<true, 0>), grid, dim3(BLOCK), 0, R->stream, rd, (const double*)R->blk, nvec, nvp,
                        (const double*)nullptr, (const double*)nullptr, t1, c, 1.0 / s, 0.0, seed, part, 1);
-    hipLaunchKernelGGL(k_kpm_fin, dim3(nvec), dim3(BLOCK), 0, R->stream, part, (int)grid.x, vt, nvec, inv_dim, mper, nmom, 1);
+    fin(part, nullptr, (int)grid.x, 1, -1);
   }
   if (R->dbl) {
     // moment doubling: the step that makes t_m gives mu_2m and mu_2m-1, so t_2 .. t_M with 2 M >= nmom - 1 suffice
@@ -6341,8 +6400,7 @@ static int kpm_run(asr_gpu_real* R, double c, double s, int nmom, int nvec, uint
     const dim3 g2((unsigned)nbx2, grid.y);
     if (nmom > 2) {
       hipLaunchKernelGGL(k_kpm_sq, g2, dim3(BLOCK), 0, R->stream, rd, nvp, (const double*)t1, part);
-      hipLaunchKernelGGL(k_kpm_fin2, dim3(nvec), dim3(BLOCK), 0, R->stream, (const double*)part, (const double*)nullptr,
-                         nbx2, vt, nvec, inv_dim, mper, nmom, 1);
+      fin(part, nullptr, nbx2, 0, 1);
     }
     const int M = nmom / 2;  // the largest m with 2 m - 1 <= nmom - 1
     const bool mfd = R->mf && R->nslot == 18;
@@ -6360,8 +6418,7 @@ static int kpm_run(asr_gpu_real* R, double c, double s, int nmom, int nvec, uint
         hipLaunchKernelGGL((k_kpm_v2<0, true>), g2, dim3(BLOCK), 0, R->stream, rd, (const double*)R->blk, nvec, nvp,
                            (const double*)t1, (const double*)t0, t2, c, 2.0 / s, 1.0, seed, part, part2);
       if (mark(3 + 2 * (m - 2))) return 1;
-      hipLaunchKernelGGL(k_kpm_fin2, dim3(nvec), dim3(BLOCK), 0, R->stream, (const double*)part, (const double*)part2, nbxm,
-                         vt, nvec, inv_dim, mper, nmom, m);
+      fin(part, part2, nbxm, 0, m);
       double* tmp = t0; t0 = t1; t1 = t2; t2 = tmp;
     }
   }
@@ -6393,7 +6450,7 @@ static int kpm_run(asr_gpu_real* R, double c, double s, int nmom, int nvec, uint
     const int nbx = (R->mf && R->nslot == 18) ? (int)((R->nsite + 3) / 4)
                     : (R->s16 && R->nslot == 18) ? (int)((R->nsite + 31) / 32)
                     : R->v2 ? (int)((R->nsite + 7) / 8) : (int)grid.x;
-    hipLaunchKernelGGL(k_kpm_fin, dim3(nvec), dim3(BLOCK), 0, R->stream, part, nbx, vt, nvec, inv_dim, mper, nmom, n);
+    fin(part, nullptr, nbx, n, -1);
     double* tmp = t0; t0 = t1; t1 = t2; t2 = tmp;
   }
   if (mark(1)) return 1;
@@ -6411,7 +6468,7 @@ int asr_gpu_kpm_moments(asr_gpu_real* R, double c, double s, int nmom, int nvec,
   const KpmSizes z = kpm_sizes(R, nmom, nvec);
   KpmBufs B;
   if (dalloc(&B.t0, z.vec) || dalloc(&B.t1, z.vec) || dalloc(&B.t2, z.vec) || dalloc(&B.part, z.part) ||
-      dalloc(&B.mper, z.mper))
+      dalloc(&B.mper, z.mper) || dalloc(&B.stage, z.stage))
     return 1;
   if (R->dbl && dalloc(&B.part2, z.part2)) return 1;
   // Chebyshev steps timed: t_2 .. t_(nmom-1), or t_2 .. t_(nmom/2) with the doubling
@@ -6438,7 +6495,7 @@ int asr_gpu_kpm_moments(asr_gpu_real* R, double c, double s, int nmom, int nvec,
   if (mu_per) std::memcpy(mu_per, per.data(), per.size() * 8);
   if (step_ms) *step_ms = nstep_ev ? ms_steps / nstep_ev : 0.0;
   if (total_ms) *total_ms = ms_all;
-  for (double* p : {B.t0, B.t1, B.t2, B.part, B.mper}) (void)hipFree(p);
+  for (double* p : {B.t0, B.t1, B.t2, B.part, B.mper, B.stage}) (void)hipFree(p);
   if (B.part2) (void)hipFree(B.part2);
   for (hipEvent_t e : ev) (void)hipEventDestroy(e);
   return 0;
@@ -6461,6 +6518,7 @@ int asr_gpu_kpm_moments_noalloc(asr_gpu_real* R, double c, double s, int nmom, i
   B.part = B.t2 + z.vec;
   B.part2 = R->dbl ? B.part + z.part : nullptr;
   B.mper = B.part + z.part + z.part2;
+  B.stage = B.mper + z.mper;
   if (kpm_run(R, c, s, nmom, nvec, seed, B, nullptr)) return 1;
   hipLaunchKernelGGL(k_kpm_mean, dim3((nmom + 255) / 256), dim3(256), 0, R->stream, (const double*)B.mper, nvec, nmom, mu_dev);
   HCHECK(hipGetLastError());
