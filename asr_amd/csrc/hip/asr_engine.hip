@@ -327,6 +327,10 @@ struct FArgs {
   // vector tv with the updated and projected w, into partial3 (c_k+1 = <tv, w> / b_k)
   const void* tvv;
   void* partial3;
+  // fold (the fused edc level): 1 in the first pass, which stores g = h1 + (E + X(z)) x~ in place of h1 at the nodes
+  // without flip partners; 2 in the second pass, which leaves out its terms on uo there (their value comes with the xo
+  // input)
+  int fold;
   // k_tile, real members, the pass that writes the output y (O2 = 0, no update): the stored output becomes
   //   y' = lcc_0 lcv_0 + ... + lcc_{lcn-1} lcv_{lcn-1} + y  (own rows, per-member coefficients; lcc_t null: 1), with the
   //   partial dot <dotu, y'>, in k_lincomb's operation order; and out3 = rib (y - ra u3 - rb up3) (a recomputed level in
@@ -663,12 +667,17 @@ __global__ void __launch_bounds__(BLOCK, (O2 == 3 ? 2 : 3)) k_tile(OpDev op, Mem
       const uint32_t vs16 = (uint32_t)nbp * ES, node16 = 9u * vs16, coff = (uint32_t)m * ES;
       const int nrec = (int)((uint32_t)(op.nn + 1) * node16);
       const double* __restrict__ cs = cst + (size_t)ml * op.nop * 9;
+      // fold = 2 (second pass of the fused edc level): the terms on the chain vector (fa.uo) at nodes without flip
+      // partners are in the xo input already (g of the first pass)
+      const bool skip_u = fa.fold == 2 && !(inf & INFO_PARTNERS);
       for (int t = 0; t < fa.nr; ++t) {
+        if (skip_u && fa.xr[t] == fa.uo) continue;
         const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc((void*)fa.xr[t], (short)0, nrec, 0x00020000);
         rand_ct<T, false>(op, md, cs, i, inf, m, node16, vs16, coff, rr, fa.r_norm[t], fa.qr[t], fa.r_coef[t], acc);
       }
       if constexpr (std::is_same<T, cplx>::value) {
         for (int t = 0; t < fa.nl; ++t) {
+          if (skip_u && fa.xl[t] == fa.uo) continue;
           // X(z) x~ = z X(a) x~ + X(b) x~ in one pass over x
           const __amdgpu_buffer_rsrc_t rl = __builtin_amdgcn_make_buffer_rsrc((void*)fa.xl[t], (short)0, nrec, 0x00020000);
           rand_ct<T, true>(op, md, cs, i, inf, m, node16, vs16, coff, rl, fa.l_norm[t], fa.lq[t], fa.l_coef[t], acc);
@@ -689,6 +698,11 @@ __global__ void __launch_bounds__(BLOCK, (O2 == 3 ? 2 : 3)) k_tile(OpDev op, Mem
       const int l0 = inf & 15u, kap = (inf >> 4) & 255u, nsm = op.nsmax;
       return fs ? frozen_row(md, fs, q) + m : md.wts + ((((size_t)q * op.ncls + kap) * nsm + l0) * nsm + l0) * 9 * nbp + m;
     };
+    // fold (the fused edc level, O2 = 1): at a node without flip partners the stored output is g = h1 + (E + X(z)) x~,
+    // the node-local terms the second pass would otherwise read x~ and the weights for, and t1 = c2 X_q2 h1 is kept in t1r
+    // from the a_k terms below for the out2 store
+    T t1r[O2 == 1 ? 9 : 1];
+    const bool folded = O2 == 1 && fa.fold == 1 && fa.apart && !(inf & INFO_PARTNERS);
     if constexpr (std::is_same<T, cplx>::value && O2 < 2) {
       // a_k at a node without flip partners: sq^2 [x~ (E + X(z)) x~ + x~ h1 - h1 t1] with h1 = y (out_sq = 0) and
       // t1 = c2 X_q2 h1 (O2 = 1) or zero; (E + X(z)) and X_q2 are node-local there (route ed3: in the out2 loop below)
@@ -704,8 +718,16 @@ __global__ void __launch_bounds__(BLOCK, (O2 == 3 ? 2 : 3)) k_tile(OpDev op, Mem
           const double wal = wpa[(size_t)l * nbp];
           const cplx cw = cplx{fma(z.re, wal, wpb[(size_t)l * nbp]) + (we ? we[(size_t)l * nbp] : 0.0), z.im * wal};
           const cplx h = so * acc[l];
-          cplx term = bilin(xt, cw * xt) + bilin(xt, h);
-          if constexpr (O2 == 1) term = term - bilin(h, fma_s(fa.c2 * wo[(size_t)l * nbp], 1.0 * h, zero<T>()));
+          const cplx xw = cw * xt;
+          cplx term = bilin(xt, xw) + bilin(xt, h);
+          if constexpr (O2 == 1) {
+            const cplx t1 = fma_s(fa.c2 * wo[(size_t)l * nbp], 1.0 * h, zero<T>());
+            term = term - bilin(h, t1);
+            if (folded) {
+              t1r[l] = t1;
+              acc[l] = h + xw;  // so = 1 in this pass
+            }
+          }
           pa = pa + term;
         }
         dotv = sq2 * pa;
@@ -898,7 +920,8 @@ __global__ void __launch_bounds__(BLOCK, (O2 == 3 ? 2 : 3)) k_tile(OpDev op, Mem
         if (du) dotv = dotv + bilin(ld(du, i, l, nbp, m), y);
         if (out2) {
           if constexpr (O2 == 1) {
-            out_store<(MT * sizeof(T) >= 128)>(&out2[((size_t)i * 9 + l) * nbp + m], fma_s(fa.c2 * wa[(size_t)l * nbp], 1.0 * y, zero<T>()));
+            out_store<(MT * sizeof(T) >= 128)>(&out2[((size_t)i * 9 + l) * nbp + m],
+                                               folded ? t1r[l] : fma_s(fa.c2 * wa[(size_t)l * nbp], 1.0 * y, zero<T>()));
           } else if constexpr (std::is_same<T, cplx>::value) {
             const cplx z = md.z[m];
             const double sx = fa.x2norm ? op.isq[i] : 1.0;
@@ -4337,6 +4360,7 @@ struct ChainRun {
         f.x2 = B.u; f.q2x = QUA; f.x2norm = 1; f.c2x = 1.0;
       } else if (!M.zero_o) {
         f.out2 = B.t1; f.q2 = QO; f.c2 = 1.0;
+        f.fold = 1;
       }
       f.apart = 1; f.a_e = (M.ed3 || M.zero_e) ? 0 : 1; f.uo = B.u; f.partial = B.part;
       if (launch_fused<T>(op, md, f, stream)) return 1;
@@ -4377,6 +4401,7 @@ struct ChainRun {
     f.upd = 1; f.proj = project; f.uo = B.u; f.upv = B.up; f.aeff = B.a_eff; f.beff = B.b_eff; f.scv = s_cur();
     f.scpv = s_prev(); f.act = B.active;
     f.out = B.w; f.partial = B.part; f.partial2 = B.pchk;
+    f.fold = (!M.ed3 && !M.zero_o) ? 2 : 0;  // h1 holds g at the nodes without flip partners (the first pass, O2 = 1)
     if (ed3) { f.tvv = B.tv; f.partial3 = B.part2; }
     return launch_fused<T>(op, md, f, stream);
   }
