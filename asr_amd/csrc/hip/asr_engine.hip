@@ -327,9 +327,9 @@ struct FArgs {
   // vector tv with the updated and projected w, into partial3 (c_k+1 = <tv, w> / b_k)
   const void* tvv;
   void* partial3;
-  // fold (the fused edc level): 1 in the first pass, which stores g = h1 + (E + X(z)) x~ in place of h1 at the nodes
-  // without flip partners; 2 in the second pass, which leaves out its terms on uo there (their value comes with the xo
-  // input)
+  // fold (the fused level): 1 in the first pass, which stores g = h1 + (E + X(z)) x~ (route ed3: h1 + (z - et) x~ - u h1)
+  // in place of h1 at the nodes without flip partners; 2 in the second pass, which leaves out its terms on uo and its
+  // energy-linear terms there (their value comes with the xo input)
   int fold;
   // k_tile, real members, the pass that writes the output y (O2 = 0, no update): the stored output becomes
   //   y' = lcc_0 lcv_0 + ... + lcc_{lcn-1} lcv_{lcn-1} + y  (own rows, per-member coefficients; lcc_t null: 1), with the
@@ -667,8 +667,8 @@ __global__ void __launch_bounds__(BLOCK, (O2 == 3 ? 2 : 3)) k_tile(OpDev op, Mem
       const uint32_t vs16 = (uint32_t)nbp * ES, node16 = 9u * vs16, coff = (uint32_t)m * ES;
       const int nrec = (int)((uint32_t)(op.nn + 1) * node16);
       const double* __restrict__ cs = cst + (size_t)ml * op.nop * 9;
-      // fold = 2 (second pass of the fused edc level): the terms on the chain vector (fa.uo) at nodes without flip
-      // partners are in the xo input already (g of the first pass)
+      // fold = 2 (second pass of the fused level): the species-diagonal terms on the chain vector (fa.uo) and the
+      // energy-linear ones at nodes without flip partners are in the xo input already (g of the first pass)
       const bool skip_u = fa.fold == 2 && !(inf & INFO_PARTNERS);
       for (int t = 0; t < fa.nr; ++t) {
         if (skip_u && fa.xr[t] == fa.uo) continue;
@@ -677,7 +677,7 @@ __global__ void __launch_bounds__(BLOCK, (O2 == 3 ? 2 : 3)) k_tile(OpDev op, Mem
       }
       if constexpr (std::is_same<T, cplx>::value) {
         for (int t = 0; t < fa.nl; ++t) {
-          if (skip_u && fa.xl[t] == fa.uo) continue;
+          if (skip_u) continue;  // the energy-linear terms (on u, and route ed3's on h1) are in g
           // X(z) x~ = z X(a) x~ + X(b) x~ in one pass over x
           const __amdgpu_buffer_rsrc_t rl = __builtin_amdgcn_make_buffer_rsrc((void*)fa.xl[t], (short)0, nrec, 0x00020000);
           rand_ct<T, true>(op, md, cs, i, inf, m, node16, vs16, coff, rl, fa.l_norm[t], fa.lq[t], fa.l_coef[t], acc);
@@ -913,10 +913,12 @@ __global__ void __launch_bounds__(BLOCK, (O2 == 3 ? 2 : 3)) k_tile(OpDev op, Mem
       // route ed3's a_k at a node without flip partners (O2 = 2, apart; z - K' of ed3_kmul, x~ u h1 = h1 u x~ and
       // x~ h'(t) = h1 t on the symmetric subspace): sq^2 [x~ (z - et) x~ + x~ h1 - h1 (u x~) - h1 t], t = out2 (c2x = 1,
       // x~ = the normalised x2, the chain vector)
+      // fold (O2 = 2, the fused level of route ed3): at a node without flip partners the stored output is
+      // g = h1 + (z - et) x~ - u h1, the node-local energy-linear terms of the second pass
       cplx pa3 = zero<cplx>();
       for (int l = 0; l < 9; ++l) {
         const T y = so * acc[l];
-        out_store<(MT * sizeof(T) >= 128)>(&out[((size_t)i * 9 + l) * nbp + m], y);
+        if constexpr (O2 != 2) out_store<(MT * sizeof(T) >= 128)>(&out[((size_t)i * 9 + l) * nbp + m], y);
         if (du) dotv = dotv + bilin(ld(du, i, l, nbp, m), y);
         if (out2) {
           if constexpr (O2 == 1) {
@@ -931,13 +933,20 @@ __global__ void __launch_bounds__(BLOCK, (O2 == 3 ? 2 : 3)) k_tile(OpDev op, Mem
             const cplx cy = cplx{fa.c2 * fma(z.re, wal, wb[(size_t)l * nbp]), fa.c2 * z.im * wal};
             const cplx tv = (zero<cplx>() + cx * xv) + cy * (1.0 * y);
             out_store<(MT * sizeof(T) >= 128)>(&out2[((size_t)i * 9 + l) * nbp + m], tv);
+            cplx yo = y;
             if (pa_) {
               const double pal = pa_[(size_t)l * nbp];
               const cplx cw = cplx{fma(z.re, pal, pb_[(size_t)l * nbp]), z.im * pal};
               const cplx uz = cplx{fma(z.re, xal, xb[(size_t)l * nbp]), z.im * xal};
               pa3 = pa3 + (((bilin(xv, cw * xv) + bilin(xv, y)) - bilin(y, uz * xv)) - bilin(y, tv));
+              if (fa.fold == 1) yo = (y + cw * xv) - uz * y;
             }
+            out_store<(MT * sizeof(T) >= 128)>(&out[((size_t)i * 9 + l) * nbp + m], yo);
+          } else {
+            out_store<(MT * sizeof(T) >= 128)>(&out[((size_t)i * 9 + l) * nbp + m], y);
           }
+        } else if constexpr (O2 == 2) {
+          out_store<(MT * sizeof(T) >= 128)>(&out[((size_t)i * 9 + l) * nbp + m], y);
         }
       }
       if constexpr (O2 == 2 && std::is_same<T, cplx>::value) {
@@ -4358,6 +4367,7 @@ struct ChainRun {
       if (M.ed3) {
         f.out2 = B.t1; f.q2 = QQA; f.c2 = -1.0;
         f.x2 = B.u; f.q2x = QUA; f.x2norm = 1; f.c2x = 1.0;
+        f.fold = 1;
       } else if (!M.zero_o) {
         f.out2 = B.t1; f.q2 = QO; f.c2 = 1.0;
         f.fold = 1;
@@ -4401,7 +4411,7 @@ struct ChainRun {
     f.upd = 1; f.proj = project; f.uo = B.u; f.upv = B.up; f.aeff = B.a_eff; f.beff = B.b_eff; f.scv = s_cur();
     f.scpv = s_prev(); f.act = B.active;
     f.out = B.w; f.partial = B.part; f.partial2 = B.pchk;
-    f.fold = (!M.ed3 && !M.zero_o) ? 2 : 0;  // h1 holds g at the nodes without flip partners (the first pass, O2 = 1)
+    f.fold = (M.ed3 || !M.zero_o) ? 2 : 0;  // h1 holds g at the nodes without flip partners (the first pass folded)
     if (ed3) { f.tvv = B.tv; f.partial3 = B.part2; }
     return launch_fused<T>(op, md, f, stream);
   }
