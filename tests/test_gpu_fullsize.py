@@ -2,6 +2,8 @@
 
 * Headline (fcc ternary, 127,517 nodes, levels 4, D2h): all 54 chains x 30 levels of the GPU run against the oracle's
   chains, 1e-9 relative.
+* The same chains on the reference's own group (D4h, orbital-permuting, 64,567 nodes): the 30 chains with a
+  one-dimensional start orbital x 30 levels against the oracle, 1e-9 relative.
 * The L2_1 uneven batch (four sublattice operators with SRO classes, 256 chains of depth 20-60, one HIP graph): the
   deepest and the shallowest chain of every operator against the oracle at full depth, 1e-9.
 * KPM on the 40^3-cell supercell (2,304,000 rows, 64 vectors, 2048 moments): size-independent properties, mu_0 = 1,
@@ -36,6 +38,26 @@ def test_headline_all_chains_against_the_oracle():
     ar, br, mr = OracleOp(wl.op).edc_batch(wl.members, eps, [m.l0 for m in wl.members], list(wl.nlev), project=True)
     assert np.array_equal(mu, mr)
     for j in range(len(wl.members)):
+        n = int(mu[j])
+        assert np.max(np.abs(a[j, :n] - ar[j, :n])) < 1e-9 * np.max(np.abs(ar[j, :n])), f"chain {j} a"
+        assert np.max(np.abs(b[j, :n - 1] - br[j, :n - 1])) < 1e-9 * np.max(np.abs(br[j, :n - 1])), f"chain {j} b"
+
+
+def test_d4h_headline_all_chains_against_the_oracle():
+    eng = _engine()
+    from oracle.oracle import OracleOp
+    wl = W.ternary_edc(L=8, levels=4, depth=30, group="d4h")
+    assert not np.all(wl.op.pi == np.arange(9)[None, :]), "D4h must permute orbitals"
+    members = [m for m in wl.members if m.l0 in (0, 2, 4, 6, 8)]  # s, p_z, d_z2, d_xy, d_x2-y2
+    nlev = np.full(len(members), 30, np.int32)
+    print(f"d4h headline: {wl.op.nn} nodes, {len(members)} chains x 30", flush=True)
+    G = eng.GpuOperator(wl.op)
+    a, b, mu, _ = G.chains_edc(members, nlev)
+    G.close()
+    eps = np.stack([m.z * m.pa + m.pb for m in members])
+    ar, br, mr = OracleOp(wl.op).edc_batch(members, eps, [m.l0 for m in members], list(nlev), project=True)
+    assert np.array_equal(mu, mr)
+    for j in range(len(members)):
         n = int(mu[j])
         assert np.max(np.abs(a[j, :n] - ar[j, :n])) < 1e-9 * np.max(np.abs(ar[j, :n])), f"chain {j} a"
         assert np.max(np.abs(b[j, :n - 1] - br[j, :n - 1])) < 1e-9 * np.max(np.abs(br[j, :n - 1])), f"chain {j} b"
