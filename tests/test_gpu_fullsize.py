@@ -4,6 +4,8 @@
   chains, 1e-9 relative.
 * The same chains on the reference's own group (D4h, orbital-permuting, 64,567 nodes): the 30 chains with a
   one-dimensional start orbital x 30 levels against the oracle, 1e-9 relative.
+* Route ed3 on the headline operator: all 54 blocks x 2 chains x 30 levels in one batched run, three blocks (first,
+  middle, last) against the oracle's ed3_blocks: a, b, c to 1e-9, the blocks to 1e-8.
 * The L2_1 uneven batch (four sublattice operators with SRO classes, 256 chains of depth 20-60, one HIP graph): the
   deepest and the shallowest chain of every operator against the oracle at full depth, 1e-9.
 * KPM on the 40^3-cell supercell (2,304,000 rows, 64 vectors, 2048 moments): size-independent properties, mu_0 = 1,
@@ -61,6 +63,30 @@ def test_d4h_headline_all_chains_against_the_oracle():
         n = int(mu[j])
         assert np.max(np.abs(a[j, :n] - ar[j, :n])) < 1e-9 * np.max(np.abs(ar[j, :n])), f"chain {j} a"
         assert np.max(np.abs(b[j, :n - 1] - br[j, :n - 1])) < 1e-9 * np.max(np.abs(br[j, :n - 1])), f"chain {j} b"
+
+
+def test_ed3_blocks_full_size_sample():
+    eng = _engine()
+    from oracle.oracle import OracleOp
+    from asr_amd import ed3 as E3
+    we = W.ternary_ed3(L=8, levels=4, depth=30)
+    print(f"ed3: {we.op.nn} nodes, {len(we.members)} blocks x 2 chains x 30", flush=True)
+    G = eng.GpuOperator(we.op)
+    G.set_depth_tol(0.0, "hull")
+    bk, a, b, c, mu, _ = G.ed3_blocks(we.members, we.info["cf"], we.info["oc"], 30)
+    G.close()
+    O = OracleOp(we.op)
+    for j in (0, len(we.members) // 2, len(we.members) - 1):
+        m = we.members[j]
+        et, uu, qt = E3.ed3_coefs(we.info["ed"][j], m.z)
+        bko, chains, _ = O.ed3_blocks(m, et, uu, qt, we.info["oc"][j], m.z, m.l0, 30, project=True, tail="hull")
+        for sd in range(2):
+            ao, bo, co, mo, closed = chains[sd]
+            assert mu[j, sd] == mo == 30 and not closed
+            assert np.max(np.abs(a[j, sd, :mo] - ao[:mo])) < 1e-9 * np.max(np.abs(ao[:mo])), (j, sd, "a")
+            assert np.max(np.abs(b[j, sd, :mo - 1] - bo[:mo - 1])) < 1e-9 * np.max(np.abs(bo[:mo - 1])), (j, sd, "b")
+            assert np.max(np.abs(c[j, sd, :mo] - co[:mo])) < 1e-9 * np.max(np.abs(co[:mo])), (j, sd, "c")
+        assert np.max(np.abs(bk[j] - bko)) < 1e-8 * np.max(np.abs(bko)), (j, "blocks")
 
 
 def test_l21_uneven_batch_full_depth_sample():
