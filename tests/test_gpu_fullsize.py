@@ -6,6 +6,8 @@
   one-dimensional start orbital x 30 levels against the oracle, 1e-9 relative.
 * Route ed3 on the headline operator: all 54 blocks x 2 chains x 30 levels in one batched run, three blocks (first,
   middle, last) against the oracle's ed3_blocks: a, b, c to 1e-9, the blocks to 1e-8.
+* EPI pair blocks (fcc ternary first-shell pair, levels 3, 113,661 nodes): all 18 blocks x 2 chains x 30 in one batched
+  run, three blocks against the oracle's epi_blocks: a, b, c to 1e-9, x to 1e-8.
 * The L2_1 uneven batch (four sublattice operators with SRO classes, 256 chains of depth 20-60, one HIP graph): the
   deepest and the shallowest chain of every operator against the oracle at full depth, 1e-9.
 * KPM on the 40^3-cell supercell (2,304,000 rows, 64 vectors, 2048 moments): size-independent properties, mu_0 = 1,
@@ -87,6 +89,28 @@ def test_ed3_blocks_full_size_sample():
             assert np.max(np.abs(b[j, sd, :mo - 1] - bo[:mo - 1])) < 1e-9 * np.max(np.abs(bo[:mo - 1])), (j, sd, "b")
             assert np.max(np.abs(c[j, sd, :mo] - co[:mo])) < 1e-9 * np.max(np.abs(co[:mo])), (j, sd, "c")
         assert np.max(np.abs(bk[j] - bko)) < 1e-8 * np.max(np.abs(bko)), (j, "blocks")
+
+
+def test_epi_blocks_full_size_sample():
+    eng = _engine()
+    from oracle.oracle import OracleOp
+    wp = W.fcc_epi_pair(L=8, levels=3)
+    print(f"epi: {wp.op.nn} nodes, {len(wp.members)} blocks x 2 chains x 30", flush=True)
+    G = eng.GpuOperator(wp.op)
+    G.set_depth_tol(0.0, "hull")
+    x, a, b, c, mu, _ = G.epi_blocks(wp.members, wp.info["v9"], 30)
+    G.close()
+    O = OracleOp(wp.op)
+    for j in (0, len(wp.members) // 2, len(wp.members) - 1):
+        m = wp.members[j]
+        xo, chains = O.epi_blocks(m, m.z * m.pa + m.pb, wp.info["v9"][j], m.z, 30, project=True, tail="hull")
+        for sd in range(2):
+            ao, bo, co, mo, closed = chains[sd]
+            assert mu[j, sd] == mo == 30 and not closed
+            assert np.max(np.abs(a[j, sd, :mo] - ao[:mo])) < 1e-9 * np.max(np.abs(ao[:mo])), (j, sd, "a")
+            assert np.max(np.abs(b[j, sd, :mo - 1] - bo[:mo - 1])) < 1e-9 * np.max(np.abs(bo[:mo - 1])), (j, sd, "b")
+            assert np.max(np.abs(c[j, sd, :mo] - co[:mo])) < 1e-9 * np.max(np.abs(co[:mo])), (j, sd, "c")
+        assert np.max(np.abs(x[j] - xo)) < 1e-8 * np.max(np.abs(xo)), (j, "x")
 
 
 def test_l21_uneven_batch_full_depth_sample():
