@@ -8,6 +8,8 @@
   middle, last) against the oracle's ed3_blocks: a, b, c to 1e-9, the blocks to 1e-8.
 * EPI pair blocks (fcc ternary first-shell pair, levels 3, 113,661 nodes): all 18 blocks x 2 chains x 30 in one batched
   run, three blocks against the oracle's epi_blocks: a, b, c to 1e-9, x to 1e-8.
+* Projected chains on the headline operator (54 chains x 30, projections on the 19 sites of the start site's first two
+  shells): three chains against the oracle's edc_green_proj: a, G, the projections to 1e-9.
 * The L2_1 uneven batch (four sublattice operators with SRO classes, 256 chains of depth 20-60, one HIP graph): the
   deepest and the shallowest chain of every operator against the oracle at full depth, 1e-9.
 * KPM on the 40^3-cell supercell (2,304,000 rows, 64 vectors, 2048 moments): size-independent properties, mu_0 = 1,
@@ -111,6 +113,29 @@ def test_epi_blocks_full_size_sample():
             assert np.max(np.abs(b[j, sd, :mo - 1] - bo[:mo - 1])) < 1e-9 * np.max(np.abs(bo[:mo - 1])), (j, sd, "b")
             assert np.max(np.abs(c[j, sd, :mo] - co[:mo])) < 1e-9 * np.max(np.abs(co[:mo])), (j, sd, "c")
         assert np.max(np.abs(x[j] - xo)) < 1e-8 * np.max(np.abs(xo)), (j, "x")
+
+
+def test_projected_chains_full_size_sample():
+    eng = _engine()
+    from oracle.oracle import OracleOp
+    ws = W.ternary_edc(L=8, levels=4, depth=30, targets=True)
+    tn, tg, tw = ws.op.meta["tnode"], ws.op.meta["tgop"], ws.info["tw"]
+    print(f"projected: {ws.op.nn} nodes, {len(ws.members)} chains x 30, {len(ws.info['tsite'])} targets", flush=True)
+    G = eng.GpuOperator(ws.op)
+    G.set_targets(tn, tg)
+    G.set_depth_tol(0.0, "hull")
+    g, gp, gp4, a, b, mu, _ = G.chains_edc_proj(ws.members, ws.nlev, tw)
+    G.close()
+    O = OracleOp(ws.op)
+    for j in (0, len(ws.members) // 2, len(ws.members) - 1):
+        m = ws.members[j]
+        go, gpo, gp4o, (ao, bo, mo, _) = O.edc_green_proj(m, m.z * m.pa + m.pb, m.l0, int(ws.nlev[j]), tn, tg, tw[j], m.z,
+                                                           tail="hull")
+        assert mu[j] == mo
+        assert np.max(np.abs(a[j, :mo] - ao[:mo])) < 1e-9 * np.max(np.abs(ao[:mo])), (j, "a")
+        assert abs(g[j] - go) < 1e-9 * abs(go), (j, "G")
+        assert np.max(np.abs(gp[j] - gpo)) < 1e-9 * np.max(np.abs(gpo)), (j, "projections")
+        assert np.max(np.abs(gp4[j] - gp4o)) < 1e-9 * np.max(np.abs(gp4o)), (j, "projections, 4 levels back")
 
 
 def test_l21_uneven_batch_full_depth_sample():
